@@ -1,0 +1,110 @@
+// Shared host/device definitions of the MI355X HSC solver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hsc {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kNodeTile = 256;     // nodes per elementwise tile (1 thread / node)
+constexpr int kLrTile = 64;        // low-rank rows (W rows) per tile
+constexpr int kMaxSys = 128;       // systems per persistent launch
+constexpr int kMaxRankLds = 256;   // rank limit of the low-rank apply (4 x 64 lanes)
+
+// Per-system descriptor for the persistent PCG kernel and the operator
+// kernels.  All pointers are device pointers.  VT buffers (p, z, q) hold
+// double (FP64 mode) or float (mixed mode).
+struct SysDesc {
+  int nb, n, a, k, nnzb;
+  // global tile ranges
+  int rt_begin, rt_end;   // row tiles (rows_per_tile block rows each)
+  int et_begin, et_end;   // node tiles (kNodeTile nodes each)
+  int lt_begin, lt_end;   // low-rank tiles (kLrTile W rows each)
+  // chunk ranges (inclusive) of each chunking that cover this system
+  int cA0, cA1;           // row tiles over the "row" workgroups (stage A, T1)
+  int cF0, cF1;           // row tiles over all workgroups (Neumann stages)
+  int cE0, cE1;           // node tiles over all workgroups (B, T2, init)
+  int cL0, cL1;           // low-rank tiles over the low-rank workgroups
+  int t_off;              // offset of this system's t vector in LDS
+  int lrp_off;            // offset of this system's low-rank partials
+  // matrix
+  const int* rowptr;
+  const int* colidx;
+  const double* val64;    // SoA: component c (col-major 3x3 index) at c*nnzb + p
+  const float* val32;
+  const double* invD64;   // [nb*9] row-major inverse diagonal blocks
+  const float* invD32;
+  const int* act_node;    // [nb] first W row of the node, or -1
+  const int* act_dof;     // [a] FEM DOF of W row
+  const double* W64;      // [a*k] row-major, V C V^T = W W^T
+  const float* W32;
+  // vectors
+  const double* rhs;
+  double* x;
+  double* r;
+  void* p[2];
+  void* z[2];
+  void* q;
+  double* q64;            // FP64 D x for the true residual (== q in FP64 mode)
+  double tol;
+  int max_iter;
+  int pad_;
+};
+
+// Per-system outputs of the persistent PCG kernel.
+struct SysOut {
+  int iterations;
+  int status;             // hsc_status
+  int checks;             // true-residual evaluations
+  int restarts;
+  double rel;             // final relative residual (true when converged)
+  double normb;
+};
+
+struct PcgParams {
+  const SysDesc* sys;
+  SysOut* out;
+  int nsys;
+  int order;
+  double omega;
+  int mode_true_every;    // core.py schedule: true residual each iteration
+  int replace_every;      // mixed: replace r by the FP64 true residual
+  int nwg;                // grid size
+  int n_lr_wg;            // workgroups [0, n_lr_wg) do the low-rank reductions
+  int nA, nF, nL;         // chunk counts (nF == nwg)
+  int rt_total, et_total, lt_total;
+  int rows_per_tile;
+  const int* chunkA;      // [nwg] chunk of row-workgroup b in the A chunking (-1 for LR wgs)
+  const int* chunkF;      // [nwg]
+  const int* chunkL;      // [nwg] (-1 for row wgs)
+  double* partA;          // [nsys * nA]
+  double* partE;          // [2 * nsys * nF]  (rr, rz from node tiles)
+  double* partF;          // [nsys * nF]      (rz from Neumann stages)
+  double* partL;          // [sum over systems (cL1-cL0+1)*k]
+  unsigned* barrier;      // grid barrier counter (zeroed before launch)
+  int* error;             // device error flag
+  long long timeout_ticks;// bounded wait (s_memrealtime ticks, 100 MHz)
+  int t_total;            // LDS doubles for t vectors
+};
+
+// Setup-time kernels' scratch layout ------------------------------------
+struct AcaParams {
+  const double* E;        // qm x a row-major (input, untouched)
+  double* R0;             // working copies (double buffer)
+  double* R1;
+  int qm, a, limit;
+  double stop;            // rtol (0 = none)
+  double* U;              // qm x limit, column-major
+  double* V;              // a x limit, column-major
+  double* hist;           // limit + 1
+  int* pivots;            // limit x 2
+  double* part;           // [nwg * 4]
+  int* rank_out;
+  double* scale_out;
+  unsigned* barrier;
+  int* error;
+  long long timeout_ticks;
+  int nwg;
+};
+
+}  // namespace hsc

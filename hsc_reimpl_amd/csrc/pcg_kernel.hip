@@ -1,0 +1,746 @@
+// Persistent Schur-complement PCG for the HSC solve on gfx950 (MI355X).
+//
+// One cooperative launch runs the whole PCG of every system in a batch
+// (linear_hsc.cu:359-393 of the reference, one system; core.py:95-138 for the
+// true-residual schedule).  Per iteration and system:
+//   A  p <- z + beta p (on the fly), q = D p, <p,q>, t = W^T p       [SpMV + low-rank]
+//   B  r -= alpha (q - W t), x += alpha p, z0 = w Dinv r, <r,r>     [elementwise]
+//   C1..C(r-1)  z <- z + w Dinv (r - D z) (Horner form of the Neumann
+//        series sum_k (I - w Dinv D)^k w Dinv r), last one with <r,z>
+// S = D - V C V^T is applied as D - W W^T with W = V L, C = L L^T.
+// Scalars live in LDS; every workgroup reduces the per-chunk partials in the
+// same fixed order, so all of them take the same decisions and the result is
+// bitwise reproducible.  Grid barriers have bounded waits.
+#include "common.hpp"
+#include "device_util.hpp"
+
+namespace hsc {
+
+enum { ST_RUN = 0, ST_DONE = 1 };
+enum { CHK_NONE = 0, CHK_CONVERGE = 1, CHK_REPLACE = 2, CHK_NORM = 3 };
+constexpr int kBreakdown = 7, kNotConverged = 8;
+
+__device__ __forceinline__ double combine(double z, double p, double beta) { return fma(beta, p, z); }
+__device__ __forceinline__ float combine(float z, float p, float beta) { return fmaf(beta, p, z); }
+
+template <typename MT> __device__ __forceinline__ const MT* mat_vals(const SysDesc& S);
+template <> __device__ __forceinline__ const double* mat_vals<double>(const SysDesc& S) { return S.val64; }
+template <> __device__ __forceinline__ const float* mat_vals<float>(const SysDesc& S) { return S.val32; }
+template <typename MT> __device__ __forceinline__ const MT* inv_vals(const SysDesc& S);
+template <> __device__ __forceinline__ const double* inv_vals<double>(const SysDesc& S) { return S.invD64; }
+template <> __device__ __forceinline__ const float* inv_vals<float>(const SysDesc& S) { return S.invD32; }
+template <typename MT> __device__ __forceinline__ const MT* w_vals(const SysDesc& S);
+template <> __device__ __forceinline__ const double* w_vals<double>(const SysDesc& S) { return S.W64; }
+template <> __device__ __forceinline__ const float* w_vals<float>(const SysDesc& S) { return S.W32; }
+
+// y_row = sum_j D_ij src(j) over one block row with G lanes (lane strides the
+// row's 3x3 blocks); every lane of the group gets the full 3-vector.
+template <int G, typename MT, typename Src>
+__device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict__ val, int row,
+                                         int lane, Src src, MT& a0, MT& a1, MT& a2) {
+  a0 = a1 = a2 = MT(0);
+  if (row < S.nb) {
+    const int p0 = S.rowptr[row], p1 = S.rowptr[row + 1];
+    const int nnzb = S.nnzb;
+    for (int p = p0 + lane; p < p1; p += G) {
+      const int c = S.colidx[p];
+      MT x0, x1, x2;
+      src(c, x0, x1, x2);
+      const MT* v = val + p;
+      const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
+      const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
+      const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
+      a0 += v0 * x0 + v3 * x1 + v6 * x2;
+      a1 += v1 * x0 + v4 * x1 + v7 * x2;
+      a2 += v2 * x0 + v5 * x1 + v8 * x2;
+    }
+  }
+  a0 = group_sum<G>(a0);
+  a1 = group_sum<G>(a1);
+  a2 = group_sum<G>(a2);
+}
+
+template <typename T>
+__device__ __forceinline__ T pick3(int d, T a0, T a1, T a2) { return d == 0 ? a0 : (d == 1 ? a1 : a2); }
+
+// ---------------------------------------------------------------- stages
+// Stage A on one row tile: p_new = z + beta p_old, q = D p_new, dot += p_new.q
+template <int G, typename VT>
+__device__ __forceinline__ void tile_A(const SysDesc& S, int tl, const VT* __restrict__ z,
+                                       const VT* __restrict__ pold, VT* __restrict__ pnew,
+                                       VT* __restrict__ q, VT beta, double& dot) {
+  const int g = threadIdx.x / G, lane = threadIdx.x % G;
+  const int row = tl * (kBlock / G) + g;
+  VT a0, a1, a2;
+  row_spmv<G, VT>(S, mat_vals<VT>(S), row, lane,
+                  [&](int c, VT& x0, VT& x1, VT& x2) {
+                    x0 = combine(z[3 * c], pold[3 * c], beta);
+                    x1 = combine(z[3 * c + 1], pold[3 * c + 1], beta);
+                    x2 = combine(z[3 * c + 2], pold[3 * c + 2], beta);
+                  },
+                  a0, a1, a2);
+  if (row < S.nb && lane < 3) {
+    const int idx = 3 * row + lane;
+    const VT pn = combine(z[idx], pold[idx], beta);
+    const VT qv = pick3(lane, a0, a1, a2);
+    pnew[idx] = pn;
+    q[idx] = qv;
+    dot += (double)pn * (double)qv;
+  }
+}
+
+// Neumann (Horner) step on one row tile: zout = zin + w Dinv (r - D zin)
+template <int G, typename VT>
+__device__ __forceinline__ void tile_N(const SysDesc& S, int tl, const VT* __restrict__ zin,
+                                       VT* __restrict__ zout, VT omega, bool last, double& rz) {
+  const int g = threadIdx.x / G, lane = threadIdx.x % G;
+  const int row = tl * (kBlock / G) + g;
+  VT a0, a1, a2;
+  row_spmv<G, VT>(S, mat_vals<VT>(S), row, lane,
+                  [&](int c, VT& x0, VT& x1, VT& x2) {
+                    x0 = zin[3 * c];
+                    x1 = zin[3 * c + 1];
+                    x2 = zin[3 * c + 2];
+                  },
+                  a0, a1, a2);
+  if (row < S.nb && lane < 3) {
+    const int i3 = 3 * row;
+    const double r0 = S.r[i3], r1 = S.r[i3 + 1], r2 = S.r[i3 + 2];
+    const VT e0 = (VT)r0 - a0, e1 = (VT)r1 - a1, e2 = (VT)r2 - a2;
+    const VT* inv = inv_vals<VT>(S) + 9 * row + 3 * lane;
+    const VT out = zin[i3 + lane] + omega * (inv[0] * e0 + inv[1] * e1 + inv[2] * e2);
+    zout[i3 + lane] = out;
+    if (last) rz += pick3(lane, r0, r1, r2) * (double)out;
+  }
+}
+
+// D x in FP64 on one row tile (true residual / operator tests)
+template <int G>
+__device__ __forceinline__ void tile_Dx64(const SysDesc& S, int tl, const double* __restrict__ x,
+                                          double* __restrict__ y) {
+  const int g = threadIdx.x / G, lane = threadIdx.x % G;
+  const int row = tl * (kBlock / G) + g;
+  double a0, a1, a2;
+  row_spmv<G, double>(S, S.val64, row, lane,
+                      [&](int c, double& x0, double& x1, double& x2) {
+                        x0 = x[3 * c];
+                        x1 = x[3 * c + 1];
+                        x2 = x[3 * c + 2];
+                      },
+                      a0, a1, a2);
+  if (row < S.nb && lane < 3) y[3 * row + lane] = pick3(lane, a0, a1, a2);
+}
+
+// low-rank partial t[l] += sum_{j in tile} W[j,l] src(act_dof[j]); acc[c]
+// holds column l = lane + 64c.  Rows of the tile are split over the 4 waves.
+template <typename WT, typename Src>
+__device__ __forceinline__ void tile_Wt(const SysDesc& S, int tl, const WT* __restrict__ W, Src src,
+                                        double acc[4]) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j0 = tl * kLrTile;
+  const int j1 = min(j0 + kLrTile, S.a);
+  const int k = S.k;
+  for (int j = j0 + wave; j < j1; j += kBlock / 64) {
+    const double s = src(S.act_dof[j]);
+    const WT* w = W + (size_t)j * k;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int l = lane + 64 * c;
+      if (l < k) acc[c] += (double)w[l] * s;
+    }
+  }
+}
+
+// sum over the four waves' acc, written to dst[l] for l < k (thread-ordered)
+__device__ __forceinline__ void flush_Wt(double acc[4], int k, double* dst, double (*lr)[kBlock]) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = 0; c < 4; ++c) {
+    if (64 * c >= k) break;
+    lr[wave][lane] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int l = lane + 64 * c;
+      double s = lr[0][lane];
+      for (int w = 1; w < kBlock / 64; ++w) s += lr[w][lane];
+      if (l < k) dst[l] = s;
+    }
+    __syncthreads();
+    acc[c] = 0.0;
+  }
+  for (int c = 0; c < 4; ++c) acc[c] = 0.0;
+}
+
+// ---------------------------------------------------------------- helpers
+struct TileRange {
+  int t0, t1;
+};
+
+__device__ __forceinline__ int sys_of_tile_begin(const SysDesc* sys, int nsys, int t, int which) {
+  int s = 0;
+  for (; s < nsys; ++s) {
+    const int e = which == 0 ? sys[s].rt_end : (which == 1 ? sys[s].et_end : sys[s].lt_end);
+    if (t < e) break;
+  }
+  return s;
+}
+
+__device__ __forceinline__ int tiles_begin(const SysDesc& S, int which) {
+  return which == 0 ? S.rt_begin : (which == 1 ? S.et_begin : S.lt_begin);
+}
+__device__ __forceinline__ int tiles_end(const SysDesc& S, int which) {
+  return which == 0 ? S.rt_end : (which == 1 ? S.et_end : S.lt_end);
+}
+
+// Iterate the tiles [t0,t1) of a tiling grouped by system: body(s, local_tile)
+// for each tile, flush(s) after the last tile of each system.  Uniform over
+// the workgroup.
+template <typename Body, typename Flush>
+__device__ __forceinline__ void for_tiles(const SysDesc* sys, int nsys, TileRange tr, int which,
+                                          Body body, Flush flush) {
+  if (tr.t0 >= tr.t1) return;
+  int s = sys_of_tile_begin(sys, nsys, tr.t0, which);
+  int t = tr.t0;
+  while (t < tr.t1 && s < nsys) {
+    const int b = tiles_begin(sys[s], which);
+    const int e = min(tr.t1, tiles_end(sys[s], which));
+    for (; t < e; ++t) body(s, t - b);
+    flush(s);
+    ++s;
+  }
+}
+
+// sum_{c=c0}^{c1} part[s*nC + c] by one wave (lanes strided, butterfly)
+__device__ __forceinline__ double wave_reduce_chunks(const double* part, int s, int nC, int c0, int c1) {
+  const int lane = threadIdx.x & 63;
+  double v = 0.0;
+  for (int c = c0 + lane; c <= c1; c += 64) v += part[(size_t)s * nC + c];
+  return wave_sum(v);
+}
+
+// ---------------------------------------------------------------- kernel
+template <int G, typename VT>
+__global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
+  extern __shared__ double s_t[];  // t vectors (sum of ranks)
+  __shared__ double s_rz[kMaxSys], s_alpha[kMaxSys], s_beta[kMaxSys], s_normb[kMaxSys];
+  __shared__ double s_rel[kMaxSys], s_v0[kMaxSys], s_v1[kMaxSys], s_tt[kMaxSys];
+  __shared__ int s_st[kMaxSys], s_it[kMaxSys], s_chk[kMaxSys], s_rs[kMaxSys];
+  __shared__ int s_nchk[kMaxSys], s_nrs[kMaxSys];
+  __shared__ double s_red[kBlock / 64];
+  __shared__ double s_lr[kBlock / 64][kBlock];
+  __shared__ int s_any;
+
+  const SysDesc* sys = P.sys;
+  const int nsys = P.nsys;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const bool is_lr = b < P.n_lr_wg;
+  unsigned epoch = 0;
+  const VT omega = (VT)P.omega;
+
+  TileRange trA{0, 0}, trF{0, 0}, trE{0, 0}, trL{0, 0};
+  const int cA = P.chunkA[b], cF = P.chunkF[b], cL = P.chunkL[b];
+  if (!is_lr) trA = {tile_of_chunk(cA, P.nA, P.rt_total), tile_of_chunk(cA + 1, P.nA, P.rt_total)};
+  trF = {tile_of_chunk(cF, P.nF, P.rt_total), tile_of_chunk(cF + 1, P.nF, P.rt_total)};
+  trE = {tile_of_chunk(cF, P.nF, P.et_total), tile_of_chunk(cF + 1, P.nF, P.et_total)};
+  if (is_lr) trL = {tile_of_chunk(cL, P.nL, P.lt_total), tile_of_chunk(cL + 1, P.nL, P.lt_total)};
+
+  for (int s = tid; s < nsys; s += kBlock) {
+    s_st[s] = ST_RUN;
+    s_it[s] = 0;
+    s_chk[s] = CHK_NONE;
+    s_rs[s] = 0;
+    s_nchk[s] = 0;
+    s_nrs[s] = 0;
+    s_rel[s] = 1.0;
+    s_beta[s] = 0.0;
+    s_rz[s] = 0.0;
+  }
+  __syncthreads();
+
+  int zc = 0, pc = 0;
+  const int order = P.order;
+  double* partE_rr = P.partE;
+  double* partE_rz = P.partE + (size_t)nsys * P.nF;
+
+  // ---- node-tile stage: z0 = w Dinv r (and residual norms) ------------
+  // kind 0: init (r == rhs), 1: stage B, 2: T2
+  auto node_stage = [&](int kind) {
+    double rr = 0.0, rz = 0.0;
+    for_tiles(sys, nsys, trE, 1,
+              [&](int s, int tl) {
+                const SysDesc& S = sys[s];
+                const int st = s_st[s];
+                if (st != ST_RUN) return;
+                if (kind == 2 && s_chk[s] == CHK_NONE) return;
+                const int i = tl * kNodeTile + tid;
+                if (i >= S.nb) return;
+                const int i3 = 3 * i;
+                double rv[3];
+                bool write_r = false, write_z = true;
+                if (kind == 0) {
+                  rv[0] = S.r[i3];
+                  rv[1] = S.r[i3 + 1];
+                  rv[2] = S.r[i3 + 2];
+                } else {
+                  const int an = S.act_node[i];
+                  double g[3] = {0.0, 0.0, 0.0};
+                  const double* t = s_t + S.t_off;
+                  if (an >= 0) {
+                    const int k = S.k;
+                    for (int d = 0; d < 3; ++d) {
+                      double acc = 0.0;
+                      if (kind == 1) {
+                        const VT* w = w_vals<VT>(S) + (size_t)(an + d) * k;
+                        for (int l = 0; l < k; ++l) acc += (double)w[l] * t[l];
+                      } else {
+                        const double* w = S.W64 + (size_t)(an + d) * k;
+                        for (int l = 0; l < k; ++l) acc += w[l] * t[l];
+                      }
+                      g[d] = acc;
+                    }
+                  }
+                  if (kind == 1) {
+                    const VT* q = (const VT*)S.q;
+                    const VT* pn = (const VT*)S.p[pc];
+                    const double al = s_alpha[s];
+                    for (int d = 0; d < 3; ++d) {
+                      const double qd = (double)q[i3 + d] - g[d];
+                      rv[d] = S.r[i3 + d] - al * qd;
+                      S.x[i3 + d] += al * (double)pn[i3 + d];
+                    }
+                    write_r = true;
+                  } else {
+                    for (int d = 0; d < 3; ++d) rv[d] = S.rhs[i3 + d] - S.q64[i3 + d] + g[d];
+                    const int ck = s_chk[s];
+                    write_r = (ck == CHK_CONVERGE || ck == CHK_REPLACE);
+                    write_z = write_r;
+                  }
+                }
+                if (write_r)
+                  for (int d = 0; d < 3; ++d) S.r[i3 + d] = rv[d];
+                rr += rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+                if (write_z) {
+                  const VT* inv = inv_vals<VT>(S) + 9 * i;
+                  VT* z = (VT*)S.z[zc];
+                  const VT r0 = (VT)rv[0], r1 = (VT)rv[1], r2 = (VT)rv[2];
+                  for (int d = 0; d < 3; ++d) {
+                    const VT zd = omega * (inv[3 * d] * r0 + inv[3 * d + 1] * r1 + inv[3 * d + 2] * r2);
+                    z[i3 + d] = zd;
+                    if (order == 1) rz += rv[d] * (double)zd;
+                  }
+                }
+              },
+              [&](int s) {
+                const bool active = s_st[s] == ST_RUN && (kind != 2 || s_chk[s] != CHK_NONE);
+                // T2 in norm-only mode keeps stage B's z and <r,z> partials
+                const bool new_z = active && (kind != 2 || s_chk[s] == CHK_CONVERGE || s_chk[s] == CHK_REPLACE);
+                const double a = block_sum_t0(rr, s_red);
+                const double c = block_sum_t0(rz, s_red);
+                if (tid == 0) {
+                  if (active) partE_rr[(size_t)s * P.nF + cF] = a;
+                  if (new_z) partE_rz[(size_t)s * P.nF + cF] = c;
+                }
+                rr = 0.0;
+                rz = 0.0;
+              });
+  };
+
+  // ---- Neumann steps C1..C(order-1) -----------------------------------
+  auto neumann_steps = [&]() -> bool {
+    for (int j = 1; j < order; ++j) {
+      const bool last = (j == order - 1);
+      double rz = 0.0;
+      for_tiles(sys, nsys, trF, 0,
+                [&](int s, int tl) {
+                  if (s_st[s] != ST_RUN) return;
+                  const SysDesc& S = sys[s];
+                  tile_N<G, VT>(S, tl, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], omega, last, rz);
+                },
+                [&](int s) {
+                  if (last) {
+                    const double c = block_sum_t0(rz, s_red);
+                    if (s_st[s] == ST_RUN && tid == 0) P.partF[(size_t)s * P.nF + cF] = c;
+                    rz = 0.0;
+                  }
+                });
+      zc ^= 1;
+      if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) return false;
+    }
+    return true;
+  };
+
+  // reduce rz (after the last preconditioner stage) into s_v0
+  auto reduce_rz = [&]() {
+    for (int s = wave; s < nsys; s += kBlock / 64) {
+      const SysDesc& S = sys[s];
+      double v = order > 1 ? wave_reduce_chunks(P.partF, s, P.nF, S.cF0, S.cF1)
+                           : wave_reduce_chunks(partE_rz, s, P.nF, S.cE0, S.cE1);
+      if ((tid & 63) == 0) s_v0[s] = v;
+    }
+    __syncthreads();
+  };
+
+  // reduce low-rank partials into s_t for systems selected by `want`
+  auto reduce_t = [&](auto want) {
+    for (int s = 0; s < nsys; ++s) {
+      const SysDesc& S = sys[s];
+      if (!want(s) || S.k == 0) continue;
+      for (int l = tid; l < S.k; l += kBlock) {
+        double v = 0.0;
+        for (int c = S.cL0; c <= S.cL1; ++c) v += P.partL[S.lrp_off + (size_t)(c - S.cL0) * S.k + l];
+        s_t[S.t_off + l] = v;
+      }
+    }
+    __syncthreads();
+    for (int s = tid; s < nsys; s += kBlock) {
+      const SysDesc& S = sys[s];
+      double tt = 0.0;
+      if (want(s))
+        for (int l = 0; l < S.k; ++l) tt += s_t[S.t_off + l] * s_t[S.t_off + l];
+      s_tt[s] = tt;
+    }
+    __syncthreads();
+  };
+
+  // low-rank stage for LR workgroups; mode 0: src = z + beta p_old, 1: src = x
+  auto lr_stage = [&](int mode) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for_tiles(sys, nsys, trL, 2,
+              [&](int s, int tl) {
+                const SysDesc& S = sys[s];
+                if (s_st[s] != ST_RUN) return;
+                if (mode == 0) {
+                  const VT* z = (const VT*)S.z[zc];
+                  const VT* po = (const VT*)S.p[pc];
+                  const VT be = (VT)s_beta[s];
+                  tile_Wt(S, tl, w_vals<VT>(S),
+                          [&](int dof) { return (double)combine(z[dof], po[dof], be); }, acc);
+                } else {
+                  if (s_chk[s] == CHK_NONE) return;
+                  const double* x = S.x;
+                  tile_Wt(S, tl, S.W64, [&](int dof) { return x[dof]; }, acc);
+                }
+              },
+              [&](int s) {
+                const SysDesc& S = sys[s];
+                const bool active = s_st[s] == ST_RUN && (mode == 0 || s_chk[s] != CHK_NONE);
+                double* dst = P.partL + S.lrp_off + (size_t)(cL - S.cL0) * S.k;
+                if (active && S.k > 0) {
+                  flush_Wt(acc, S.k, dst, s_lr);
+                } else {
+                  for (int c = 0; c < 4; ++c) acc[c] = 0.0;
+                }
+              });
+  };
+
+  // ================================================================ init
+  node_stage(0);
+  if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+  for (int s = wave; s < nsys; s += kBlock / 64) {
+    const SysDesc& S = sys[s];
+    const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1);
+    if ((tid & 63) == 0) {
+      const double nb = sqrt(v);
+      s_normb[s] = nb;
+      if (!(nb > 0.0)) {
+        s_st[s] = ST_DONE;
+        s_rel[s] = 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (!neumann_steps()) goto out;
+  reduce_rz();
+  for (int s = tid; s < nsys; s += kBlock) {
+    s_rz[s] = s_v0[s];
+    s_beta[s] = 0.0;
+  }
+  __syncthreads();
+
+  // ================================================================ loop
+  for (;;) {
+    if (tid == 0) {
+      int any = 0;
+      for (int s = 0; s < nsys; ++s) any |= (s_st[s] == ST_RUN);
+      s_any = any;
+    }
+    __syncthreads();
+    if (!s_any) break;
+
+    // ---------------- stage A
+    if (is_lr) {
+      lr_stage(0);
+    } else {
+      double dot = 0.0;
+      for_tiles(sys, nsys, trA, 0,
+                [&](int s, int tl) {
+                  if (s_st[s] != ST_RUN) return;
+                  const SysDesc& S = sys[s];
+                  tile_A<G, VT>(S, tl, (const VT*)S.z[zc], (const VT*)S.p[pc], (VT*)S.p[pc ^ 1],
+                                (VT*)S.q, (VT)s_beta[s], dot);
+                },
+                [&](int s) {
+                  const double d = block_sum_t0(dot, s_red);
+                  if (s_st[s] == ST_RUN && tid == 0) P.partA[(size_t)s * P.nA + cA] = d;
+                  dot = 0.0;
+                });
+    }
+    pc ^= 1;
+    if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+    for (int s = wave; s < nsys; s += kBlock / 64) {
+      const SysDesc& S = sys[s];
+      const double v = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1);
+      if ((tid & 63) == 0) s_v1[s] = v;
+    }
+    reduce_t([&](int s) { return s_st[s] == ST_RUN; });
+    for (int s = tid; s < nsys; s += kBlock) {
+      if (s_st[s] != ST_RUN) continue;
+      const double pAp = s_v1[s] - s_tt[s];
+      const double rz = s_rz[s];
+      if (!(isfinite(pAp) && pAp > 0.0 && isfinite(rz) && rz > 0.0)) {
+        s_st[s] = kBreakdown;
+      } else {
+        s_alpha[s] = rz / pAp;
+      }
+    }
+    __syncthreads();
+
+    // ---------------- stage B
+    node_stage(1);
+    if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+    for (int s = wave; s < nsys; s += kBlock / 64) {
+      const SysDesc& S = sys[s];
+      const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1);
+      if ((tid & 63) == 0) s_v1[s] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int any = 0;
+      for (int s = 0; s < nsys; ++s) {
+        s_chk[s] = CHK_NONE;
+        if (s_st[s] != ST_RUN) continue;
+        const SysDesc& S = sys[s];
+        s_it[s] += 1;
+        const double rel = sqrt(s_v1[s]) / s_normb[s];
+        s_rel[s] = rel;
+        int ck = CHK_NONE;
+        if (rel <= S.tol) ck = CHK_CONVERGE;
+        else if (P.mode_true_every) ck = CHK_NORM;
+        else if (P.replace_every > 0 && (s_it[s] % P.replace_every) == 0) ck = CHK_REPLACE;
+        s_chk[s] = ck;
+        any |= (ck != CHK_NONE);
+      }
+      s_any = any;
+    }
+    __syncthreads();
+
+    // ---------------- true residual (T1, T2) when any system asks for it
+    if (s_any) {
+      if (is_lr) {
+        lr_stage(1);
+      } else {
+        for_tiles(sys, nsys, trA, 0,
+                  [&](int s, int tl) {
+                    if (s_st[s] != ST_RUN || s_chk[s] == CHK_NONE) return;
+                    const SysDesc& S = sys[s];
+                    tile_Dx64<G>(S, tl, S.x, S.q64);
+                  },
+                  [&](int) {});
+      }
+      if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+      reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; });
+      node_stage(2);
+      if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+      for (int s = wave; s < nsys; s += kBlock / 64) {
+        if (s_st[s] != ST_RUN || s_chk[s] == CHK_NONE) continue;
+        const SysDesc& S = sys[s];
+        const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1);
+        if ((tid & 63) == 0) s_v1[s] = v;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        for (int s = 0; s < nsys; ++s) {
+          if (s_st[s] != ST_RUN || s_chk[s] == CHK_NONE) continue;
+          const SysDesc& S = sys[s];
+          const double rel = sqrt(s_v1[s]) / s_normb[s];
+          s_nchk[s] += 1;
+          s_rel[s] = rel;
+          if (rel <= S.tol) {
+            s_st[s] = ST_DONE;
+          } else if (s_chk[s] == CHK_CONVERGE) {
+            s_rs[s] = 1;
+            s_nrs[s] += 1;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      for (int s = 0; s < nsys; ++s)
+        if (s_st[s] == ST_RUN && s_it[s] >= sys[s].max_iter) s_st[s] = kNotConverged;
+    }
+    __syncthreads();
+
+    // ---------------- preconditioner stages
+    if (!neumann_steps()) goto out;
+    reduce_rz();
+    for (int s = tid; s < nsys; s += kBlock) {
+      if (s_st[s] != ST_RUN) continue;
+      const double rzn = s_v0[s];
+      if (s_rs[s]) {
+        s_beta[s] = 0.0;
+        s_rs[s] = 0;
+      } else {
+        s_beta[s] = rzn / s_rz[s];
+      }
+      s_rz[s] = rzn;
+    }
+    __syncthreads();
+  }
+
+out:
+  __syncthreads();
+  if (b == 0) {
+    for (int s = tid; s < nsys; s += kBlock) {
+      SysOut o;
+      o.iterations = s_it[s];
+      o.status = s_st[s] == ST_DONE ? 0 : (s_st[s] == ST_RUN ? HSC_ERR_DEVICE_TIMEOUT_CODE : s_st[s]);
+      o.checks = s_nchk[s];
+      o.restarts = s_nrs[s];
+      o.rel = s_rel[s];
+      o.normb = s_normb[s];
+      P.out[s] = o;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- operator kernels
+// (plain launches: one tile per workgroup; used by the parity entry points)
+template <int G>
+__global__ __launch_bounds__(kBlock) void op_Dx64(SysDesc S, const double* x, double* y) {
+  tile_Dx64<G>(S, blockIdx.x, x, y);
+}
+
+template <int G, typename VT>
+__global__ __launch_bounds__(kBlock) void op_neumann_step(SysDesc S, const VT* zin, VT* zout, double omega) {
+  double rz = 0.0;
+  tile_N<G, VT>(S, blockIdx.x, zin, zout, (VT)omega, false, rz);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void op_jacobi(SysDesc S, const double* r, VT* z, double omega) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.nb) return;
+  const VT* inv = inv_vals<VT>(S) + 9 * i;
+  const VT r0 = (VT)r[3 * i], r1 = (VT)r[3 * i + 1], r2 = (VT)r[3 * i + 2];
+  for (int d = 0; d < 3; ++d)
+    z[3 * i + d] = (VT)omega * (inv[3 * d] * r0 + inv[3 * d + 1] * r1 + inv[3 * d + 2] * r2);
+}
+
+// t = W^T x (single workgroup, fixed order) then y[act] -= W t
+__global__ __launch_bounds__(kBlock) void op_lowrank64(SysDesc S, const double* x, double* y, double* t) {
+  __shared__ double s_tv[kMaxRankLds];
+  __shared__ double lr[kBlock / 64][kBlock];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int ntile = (S.a + kLrTile - 1) / kLrTile;
+  for (int tl = 0; tl < ntile; ++tl) tile_Wt(S, tl, S.W64, [&](int dof) { return x[dof]; }, acc);
+  flush_Wt(acc, S.k, s_tv, lr);
+  __syncthreads();
+  for (int l = threadIdx.x; l < S.k; l += kBlock) t[l] = s_tv[l];
+  for (int j = threadIdx.x; j < S.a; j += kBlock) {
+    double g = 0.0;
+    for (int l = 0; l < S.k; ++l) g += S.W64[(size_t)j * S.k + l] * s_tv[l];
+    y[S.act_dof[j]] -= g;
+  }
+}
+
+// explicit instantiations via host launchers
+template <int G, typename VT>
+static hipError_t launch_pcg_t(const PcgParams& P, size_t shmem, hipStream_t st) {
+  void* args[] = {(void*)&P};
+  return hipLaunchCooperativeKernel((const void*)pcg_persistent<G, VT>, dim3(P.nwg), dim3(kBlock), args,
+                                    shmem, st);
+}
+
+template <int G, typename VT>
+static hipError_t occ_t(int* blocks, size_t shmem) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pcg_persistent<G, VT>, kBlock, shmem);
+}
+
+hipError_t launch_pcg(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st) {
+  if (mixed) {
+    switch (G) {
+      case 4: return launch_pcg_t<4, float>(P, shmem, st);
+      case 8: return launch_pcg_t<8, float>(P, shmem, st);
+      case 16: return launch_pcg_t<16, float>(P, shmem, st);
+    }
+  } else {
+    switch (G) {
+      case 4: return launch_pcg_t<4, double>(P, shmem, st);
+      case 8: return launch_pcg_t<8, double>(P, shmem, st);
+      case 16: return launch_pcg_t<16, double>(P, shmem, st);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t pcg_occupancy(int G, bool mixed, size_t shmem, int* blocks) {
+  if (mixed) {
+    switch (G) {
+      case 4: return occ_t<4, float>(blocks, shmem);
+      case 8: return occ_t<8, float>(blocks, shmem);
+      case 16: return occ_t<16, float>(blocks, shmem);
+    }
+  } else {
+    switch (G) {
+      case 4: return occ_t<4, double>(blocks, shmem);
+      case 8: return occ_t<8, double>(blocks, shmem);
+      case 16: return occ_t<16, double>(blocks, shmem);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st) {
+  const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
+  if (tiles == 0) return hipSuccess;
+  switch (G) {
+    case 4: hipLaunchKernelGGL(op_Dx64<4>, dim3(tiles), dim3(kBlock), 0, st, S, x, y); break;
+    case 8: hipLaunchKernelGGL(op_Dx64<8>, dim3(tiles), dim3(kBlock), 0, st, S, x, y); break;
+    case 16: hipLaunchKernelGGL(op_Dx64<16>, dim3(tiles), dim3(kBlock), 0, st, S, x, y); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_op_precond64(int G, const SysDesc& S, int order, double omega, const double* r,
+                               double* z, double* tmp, hipStream_t st) {
+  const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
+  const int ntile = (S.nb + kBlock - 1) / kBlock;
+  if (S.nb == 0) return hipSuccess;
+  // z0 into (order odd ? z : tmp) so that the last step lands in z
+  double* bufs[2] = {z, tmp};
+  int cur = ((order - 1) % 2 == 0) ? 0 : 1;
+  hipLaunchKernelGGL(op_jacobi<double>, dim3(ntile), dim3(kBlock), 0, st, S, r, bufs[cur], omega);
+  SysDesc S2 = S;
+  S2.r = const_cast<double*>(r);
+  for (int j = 1; j < order; ++j) {
+    switch (G) {
+      case 4: hipLaunchKernelGGL((op_neumann_step<4, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], omega); break;
+      case 8: hipLaunchKernelGGL((op_neumann_step<8, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], omega); break;
+      case 16: hipLaunchKernelGGL((op_neumann_step<16, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], omega); break;
+      default: return hipErrorInvalidValue;
+    }
+    cur ^= 1;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_op_lowrank64(const SysDesc& S, const double* x, double* y, double* t, hipStream_t st) {
+  if (S.k == 0 || S.a == 0) return hipSuccess;
+  hipLaunchKernelGGL(op_lowrank64, dim3(1), dim3(kBlock), 0, st, S, x, y, t);
+  return hipGetLastError();
+}
+
+}  // namespace hsc

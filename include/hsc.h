@@ -1,0 +1,193 @@
+/*
+ * hsc.h -- C ABI of the MI355X HSC (Heterogeneous Subspace Correction) linear
+ * solver.  Plain C: device pointers, sizes and status codes; no C++ or torch
+ * types cross this boundary, no exceptions escape it.
+ *
+ * It replaces the reference's libuipc linear-solver plugin
+ *   class LinearHSC final : public IterativeSolver
+ *   (reproduction/hsc_backend/linear_hsc.cu:153-438), whose contract is
+ *   IterativeSolver::do_build / do_solve
+ *   (reproduction/hsc_backend/upstream/src/backends/cuda/linear_system/iterative_solver.h:19-21).
+ *
+ * Threading: one handle = one host thread = one HIP stream at a time.  Calls
+ * on one handle are not thread-safe.  All device buffers are caller-owned
+ * unless stated; hsc_setup allocates the handle's workspaces (re-used, grown
+ * only when a bigger system arrives), hsc_solve allocates nothing.
+ */
+#ifndef HSC_MI355X_H
+#define HSC_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSC_ABI_VERSION 1
+
+/* Status codes: one per failure mode of the reference plugin. */
+enum hsc_status {
+  HSC_OK = 0,
+  HSC_ERR_INVALID_ARGUMENT = 1,   /* bad config / DOF layout: linear_hsc.cu:188-189,199-200 */
+  HSC_ERR_FEM_DIAGONAL = 2,       /* non-positive FEM 3x3 diagonal block: linear_hsc.cu:73-78,208 */
+  HSC_ERR_UNEXPECTED_COUPLING = 3,/* coupling block not ABD<->FEM: linear_hsc.cu:230 */
+  HSC_ERR_ABD_NOT_SPD = 4,        /* non-SPD 12x12 ABD block: linear_hsc.cu:268-269 */
+  HSC_ERR_ABD_INVERSE = 5,        /* ABD Neumann doubling diverged / tol missed: linear_hsc.cu:292-297 */
+  HSC_ERR_NONFINITE_FACTORS = 6,  /* non-finite P or C (or C not SPD): linear_hsc.cu:300 */
+  HSC_ERR_BREAKDOWN = 7,          /* PCG pSp<=0 or rz<=0 or non-finite: linear_hsc.cu:370-371 */
+  HSC_ERR_NOT_CONVERGED = 8,      /* max_iter reached: linear_hsc.cu:394 */
+  HSC_ERR_NOT_DESCENT = 9,        /* b^T x <= 0: linear_hsc.cu:400-402 */
+  HSC_ERR_FULL_RESIDUAL = 10,     /* full residual > full_residual_limit: linear_hsc.cu:433-435 */
+  HSC_ERR_HIP = 11,               /* HIP runtime error: linear_hsc.cu:34-37 */
+  HSC_ERR_NOT_SETUP = 12,         /* solve before setup (no reference counterpart) */
+  HSC_ERR_DEVICE_TIMEOUT = 13,    /* a persistent kernel's bounded wait expired (no counterpart) */
+  HSC_ERR_CAPACITY = 14           /* a system exceeds a compiled-in limit (no counterpart) */
+};
+
+/* Arithmetic of the bulk (matrix/vector) traffic. */
+enum hsc_precision {
+  HSC_PRECISION_FP64 = 0,         /* everything FP64 (the reference) */
+  HSC_PRECISION_MIXED = 1         /* FP32 SpMV / low-rank / Neumann terms; FP64 dots, alpha/beta,
+                                     x, r and a true FP64 residual every residual_replace_every */
+};
+
+/* How A^-1 (the ABD block inverse) is formed. */
+enum hsc_abd_inverse {
+  HSC_ABD_EXACT = 0,              /* FP64 Cholesky of A (core.py:149-150 exact_affine=True) */
+  HSC_ABD_NEUMANN = 1,            /* fixed-order block Neumann polynomial (core.py:74-92;
+                                     linear_hsc.cu:264-275 with adaptive_abd_inverse = 0) */
+  HSC_ABD_NEUMANN_ADAPTIVE = 2    /* polynomial + doubling to abd_inverse_tol (linear_hsc.cu:276-297) */
+};
+
+/* Preconditioner. */
+enum hsc_preconditioner {
+  HSC_PRECOND_NEUMANN = 0,        /* block-Jacobi Neumann series of D (linear_hsc.cu:330-343) */
+  HSC_PRECOND_LEARNED = 1         /* Neumann + graph-MLP correction, flexible PCG (no counterpart) */
+};
+
+/* Solver configuration.  The first eight fields are the reference's scene
+ * config keys and defaults (integration.patch:34-55; linear_system/tol_rate
+ * default 1e-3 in scene_default_config.cpp:146-151); the rest are additions. */
+typedef struct hsc_config {
+  int32_t max_rank;              /* linear_system/hsc/max_rank            (128)  */
+  int32_t neumann_order;         /* linear_system/hsc/neumann_order       (5)    */
+  int32_t max_iter;              /* linear_system/hsc/max_iter            (2000) */
+  int32_t adaptive_abd_inverse;  /* linear_system/hsc/adaptive_abd_inverse(1)    */
+  double abd_inverse_tol;        /* linear_system/hsc/abd_inverse_tol     (1e-5) */
+  double full_residual_limit;    /* linear_system/hsc/full_residual_limit (0.1)  */
+  double omega;                  /* linear_system/hsc/omega               (0.7)  */
+  double tol_rate;               /* linear_system/tol_rate                (1e-3) */
+  /* --- additions --- */
+  int32_t abd_inverse;           /* enum hsc_abd_inverse (HSC_ABD_EXACT)          */
+  int32_t precision;             /* enum hsc_precision (HSC_PRECISION_FP64)       */
+  int32_t residual_replace_every;/* mixed: FP64 true-residual period (25); 0 = off */
+  int32_t true_residual_every_iteration; /* 1 = stop test on the true residual each
+                                     iteration (core.py:120-132 schedule); 0 = recursive
+                                     residual + one true check (linear_hsc.cu:374-387) */
+  double aca_rtol;               /* ACA stop on ||E-UV^T||_F/||E||_F (core.py:22-50); 0 = none */
+  int32_t lanes_per_row;         /* SpMV lanes per block row: 0 = autotuned, else 4/8/16 */
+  int32_t wg_per_cu;             /* persistent workgroups per CU: 0 = autotuned, else 1..4 */
+  int32_t preconditioner;        /* enum hsc_preconditioner (HSC_PRECOND_NEUMANN)  */
+  int32_t check_acceptance;      /* 1 = fail on non-descent / full residual (reference); 0 = report only */
+} hsc_config;
+
+/* Device view of the assembled Hessian: the stored triangle of 3x3 blocks
+ * (libuipc CBCOOMatrixView<double,3> after ge2sym/convert: one block per
+ * unordered block pair, diagonal blocks full).  Mirrors SolvingInfo::A()
+ * (integration.patch:7,17-20). */
+typedef struct hsc_bcoo3_view {
+  const int32_t* row_indices;    /* [triplet_count] block row (device) */
+  const int32_t* col_indices;    /* [triplet_count] block col (device) */
+  const double* values;          /* [triplet_count*9] 3x3 column-major (device) */
+  int64_t triplet_count;
+  int64_t block_dim;             /* number of block rows = N/3 */
+} hsc_bcoo3_view;
+
+/* Per-solve statistics: the fields of the reference's hsc_stats.jsonl row
+ * (linear_hsc.cu:421-431) plus additions. */
+typedef struct hsc_stats {
+  int32_t fem_dofs, abd_dofs, active_fem_dofs, rank;
+  double aca_relative_error;
+  int32_t pcg_iterations;
+  double schur_relative_residual, full_relative_residual;
+  double setup_ms, total_ms;
+  int32_t adaptive_abd_inverse, neumann_doublings;
+  double abd_inverse_residual, full_residual_limit;
+  int32_t accepted, fallback;    /* fallback: learned preconditioner fell back to Neumann */
+  int32_t status;                /* enum hsc_status of this solve */
+  /* --- additions --- */
+  int32_t precision, preconditioner, true_residual_checks, restarts;
+  double pcg_ms;                 /* device time of the persistent PCG launch */
+  double descent;                /* b^T x */
+} hsc_stats;
+
+typedef struct hsc_handle hsc_handle;
+typedef void* hsc_stream;        /* hipStream_t (NULL = default stream) */
+
+int32_t hsc_abi_version(void);
+void hsc_config_default(hsc_config* cfg);
+
+/* Create / destroy.  Mirrors do_build (linear_hsc.cu:172-193): validates the
+ * config (HSC_ERR_INVALID_ARGUMENT like its UIPC_ASSERT at :188-189). */
+int32_t hsc_create(const hsc_config* cfg, int32_t device, hsc_handle** out);
+void hsc_destroy(hsc_handle* h);
+const char* hsc_last_error(const hsc_handle* h);
+int32_t hsc_set_config(hsc_handle* h, const hsc_config* cfg);
+
+/* Setup for one Newton system (linear_hsc.cu:195-304): split A into D | E |
+ * A_abd, invert the FEM 3x3 diagonal blocks, ACA(E) = U V^T, A^-1, C, all on
+ * the device.  qo/qm: ABD DOF offset/count (qm % 12 == 0); fo/fn: FEM DOF
+ * offset/count (fn % 3 == 0); qm + fn = 3*A->block_dim. */
+int32_t hsc_setup(hsc_handle* h, const hsc_bcoo3_view* A, int32_t qo, int32_t qm,
+                  int32_t fo, int32_t fn, hsc_stream stream);
+
+/* Solve H x = b after hsc_setup (linear_hsc.cu:349-435): Schur PCG on the FEM
+ * DOFs, back-substitution of the ABD DOFs, descent and full-residual checks.
+ * b, x: device FP64 vectors of N = qm + fn; x is fully overwritten.
+ * Synchronous (returns when x and *stats are final). */
+int32_t hsc_solve(hsc_handle* h, const double* b, double* x, hsc_stats* stats,
+                  hsc_stream stream);
+
+/* do_solve in one call: hsc_setup + hsc_solve (setup_ms covers the setup). */
+int32_t hsc_do_solve(hsc_handle* h, const hsc_bcoo3_view* A, int32_t qo, int32_t qm,
+                     int32_t fo, int32_t fn, const double* b, double* x,
+                     hsc_stats* stats, hsc_stream stream);
+
+/* Append one JSONL row in the reference's hsc_stats.jsonl schema
+ * (linear_hsc.cu:420-432) to `path`; frame/newton label the row. */
+int32_t hsc_append_jsonl(const hsc_stats* s, const char* path, int32_t frame, int32_t newton);
+
+/* ---- batched solves (no reference counterpart: many independent systems
+ * of uneven size in ONE persistent launch) ----
+ * handles[i] must each have been hsc_setup; all share one config precision
+ * and preconditioner. b[i], x[i]: device vectors of system i. */
+int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count,
+                          const double* const* b, double* const* x,
+                          hsc_stats* stats, hsc_stream stream);
+
+/* ---- operator access for parity tests (device vectors of fn FEM DOFs) ---- */
+int32_t hsc_apply_D(hsc_handle* h, const double* x, double* y, hsc_stream stream);
+int32_t hsc_apply_schur(hsc_handle* h, const double* x, double* y, hsc_stream stream);
+int32_t hsc_apply_precond(hsc_handle* h, const double* r, double* z, hsc_stream stream);
+
+/* Setup summary and host copies of the small factors (any pointer may be NULL).
+ * U: qm x rank column-major; V: active x rank column-major; C: rank x rank;
+ * P: qm x qm column-major (A^-1 approximation); active: active FEM DOFs
+ * (relative to fo); aca_history: rank+1 entries. */
+typedef struct hsc_setup_info {
+  int32_t fem_dofs, abd_dofs, fem_nodes, nnzb, active_fem_dofs, rank;
+  int32_t lanes_per_row, workgroups, neumann_doublings, abd_block_diagonal;
+  double aca_relative_error, abd_inverse_residual;
+} hsc_setup_info;
+int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
+int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* P,
+                        int32_t* active, double* aca_history);
+
+/* ---- learned preconditioner weights (BF16 graph MLP, see hsc_learned.h) ---- */
+int32_t hsc_set_learned_weights(hsc_handle* h, const void* weights_dev, int64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSC_MI355X_H */
