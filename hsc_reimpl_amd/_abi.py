@@ -1,0 +1,121 @@
+"""ctypes binding of the C ABI in include/hsc.h (libhsc_mi355x.so).
+
+There is no fallback: if the HIP library is missing, ``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libhsc_mi355x.so"
+
+HSC_ABI_VERSION = 1
+
+STATUS = {
+    0: "HSC_OK", 1: "HSC_ERR_INVALID_ARGUMENT", 2: "HSC_ERR_FEM_DIAGONAL",
+    3: "HSC_ERR_UNEXPECTED_COUPLING", 4: "HSC_ERR_ABD_NOT_SPD", 5: "HSC_ERR_ABD_INVERSE",
+    6: "HSC_ERR_NONFINITE_FACTORS", 7: "HSC_ERR_BREAKDOWN", 8: "HSC_ERR_NOT_CONVERGED",
+    9: "HSC_ERR_NOT_DESCENT", 10: "HSC_ERR_FULL_RESIDUAL", 11: "HSC_ERR_HIP",
+    12: "HSC_ERR_NOT_SETUP", 13: "HSC_ERR_DEVICE_TIMEOUT", 14: "HSC_ERR_CAPACITY",
+}
+globals().update({v: k for k, v in STATUS.items()})
+
+PRECISION = {"fp64": 0, "mixed": 1}
+ABD_INVERSE = {"exact": 0, "neumann": 1, "neumann_adaptive": 2}
+PRECONDITIONER = {"neumann": 0, "learned": 1}
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("max_rank", C.c_int32), ("neumann_order", C.c_int32), ("max_iter", C.c_int32),
+        ("adaptive_abd_inverse", C.c_int32), ("abd_inverse_tol", C.c_double),
+        ("full_residual_limit", C.c_double), ("omega", C.c_double), ("tol_rate", C.c_double),
+        ("abd_inverse", C.c_int32), ("precision", C.c_int32), ("residual_replace_every", C.c_int32),
+        ("true_residual_every_iteration", C.c_int32), ("aca_rtol", C.c_double),
+        ("lanes_per_row", C.c_int32), ("wg_per_cu", C.c_int32), ("preconditioner", C.c_int32),
+        ("check_acceptance", C.c_int32),
+    ]
+
+
+class BCOO3View(C.Structure):
+    _fields_ = [("row_indices", C.c_void_p), ("col_indices", C.c_void_p), ("values", C.c_void_p),
+                ("triplet_count", C.c_int64), ("block_dim", C.c_int64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("fem_dofs", C.c_int32), ("abd_dofs", C.c_int32), ("active_fem_dofs", C.c_int32), ("rank", C.c_int32),
+        ("aca_relative_error", C.c_double), ("pcg_iterations", C.c_int32),
+        ("schur_relative_residual", C.c_double), ("full_relative_residual", C.c_double),
+        ("setup_ms", C.c_double), ("total_ms", C.c_double),
+        ("adaptive_abd_inverse", C.c_int32), ("neumann_doublings", C.c_int32),
+        ("abd_inverse_residual", C.c_double), ("full_residual_limit", C.c_double),
+        ("accepted", C.c_int32), ("fallback", C.c_int32), ("status", C.c_int32),
+        ("precision", C.c_int32), ("preconditioner", C.c_int32), ("true_residual_checks", C.c_int32),
+        ("restarts", C.c_int32), ("pcg_ms", C.c_double), ("descent", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class SetupInfo(C.Structure):
+    _fields_ = [
+        ("fem_dofs", C.c_int32), ("abd_dofs", C.c_int32), ("fem_nodes", C.c_int32), ("nnzb", C.c_int32),
+        ("active_fem_dofs", C.c_int32), ("rank", C.c_int32), ("lanes_per_row", C.c_int32),
+        ("workgroups", C.c_int32), ("neumann_doublings", C.c_int32), ("abd_block_diagonal", C.c_int32),
+        ("aca_relative_error", C.c_double), ("abd_inverse_residual", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# exported symbol -> (restype, argtypes)
+SIGNATURES = {
+    "hsc_abi_version": (C.c_int32, []),
+    "hsc_config_default": (None, [C.POINTER(Config)]),
+    "hsc_create": (C.c_int32, [C.POINTER(Config), C.c_int32, C.POINTER(C.c_void_p)]),
+    "hsc_destroy": (None, [C.c_void_p]),
+    "hsc_last_error": (C.c_char_p, [C.c_void_p]),
+    "hsc_set_config": (C.c_int32, [C.c_void_p, C.POINTER(Config)]),
+    "hsc_setup": (C.c_int32, [C.c_void_p, C.POINTER(BCOO3View), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                              C.c_void_p]),
+    "hsc_solve": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Stats), C.c_void_p]),
+    "hsc_do_solve": (C.c_int32, [C.c_void_p, C.POINTER(BCOO3View), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                 C.c_void_p, C.c_void_p, C.POINTER(Stats), C.c_void_p]),
+    "hsc_append_jsonl": (C.c_int32, [C.POINTER(Stats), C.c_char_p, C.c_int32, C.c_int32]),
+    "hsc_solve_batched": (C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p), C.POINTER(Stats), C.c_void_p]),
+    "hsc_apply_D": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hsc_apply_schur": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hsc_apply_precond": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hsc_get_setup_info": (C.c_int32, [C.c_void_p, C.POINTER(SetupInfo)]),
+    "hsc_get_factors": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
+    "hsc_set_learned_weights": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
+}
+
+_lib = None
+
+
+def load(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load the HIP library (raises when it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"HSC HIP library not built: {p} is missing "
+                           "(run `python -m hsc_reimpl_amd.build`); there is no CPU fallback")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.hsc_abi_version() != HSC_ABI_VERSION:
+        raise RuntimeError("HSC ABI version mismatch")
+    if path is None:
+        _lib = lib
+    return lib

@@ -1,0 +1,94 @@
+// Kernel parameter blocks and host launchers shared by the HIP sources and
+// the host orchestration.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.hpp"
+
+namespace hsc {
+
+struct SplitParams {
+  const int* rows;
+  const int* cols;
+  const double* vals;  // col-major 3x3
+  long long count;
+  int qo, qm, fo, fn;
+  int* row_count;      // [nb]
+  int* active_flag;    // [nb]
+  double* invD64;      // [nb*9] row-major
+  float* invD32;
+  double* A;           // [qm*qm] row-major
+  int* err;            // [8]
+};
+
+struct PlaceParams {
+  const int* rows;
+  const int* cols;
+  const double* vals;
+  long long count;
+  int qo, qm, fo, fn;
+  const int* rowptr;
+  int* cursor;
+  int* key_col;
+  int* key_src;        // 2*triplet + transposed
+  const int* act_node;
+  int a;
+  double* E;           // [qm*a] row-major
+};
+
+struct AbdParams {
+  const double* A;
+  const double* Binv;
+  double* P;        // out, qm x qm
+  double* s1;       // scratch qm*qm each
+  double* s2;
+  double* s3;
+  int qm;
+  int mode;         // 0 exact, 1 neumann, 2 adaptive
+  int blockdiag;    // A has no inter-body blocks
+  int order;
+  double omega;
+  double tol;
+  double* out;      // [0] residual, [1] doublings, [2] status (0 ok, 4 not SPD, 5 inverse, 6 nonfinite)
+};
+
+struct FactorParams {
+  const double* P;   // qm x qm row-major
+  const double* U;   // qm x ldk col-major (ld = qm)
+  const double* V;   // a x ldk col-major (ld = a)
+  int qm, a, k;
+  double* W1;        // qm x k scratch
+  double* C;         // k x k (out: C; then L in lower triangle of Lm)
+  double* Lm;        // k x k lower factor
+  double* out;       // [0] status (0 ok, 6 nonfinite / not SPD)
+};
+
+// setup_kernels.hip
+hipError_t launch_split(const SplitParams& S, hipStream_t st);
+hipError_t launch_scan(const int* in, int* out, int n, hipStream_t st);
+hipError_t launch_active(const int* flag, const int* prefix, int nb, int* act_node, int* act_dof, hipStream_t st);
+hipError_t launch_place(const PlaceParams& S, hipStream_t st);
+hipError_t launch_sort_gather(const int* rowptr, int nb, int* key_col, int* key_src, const double* vals, int nnzb,
+                              int* colidx, double* v64, float* v32, hipStream_t st);
+hipError_t aca_occupancy(int* blocks);
+hipError_t launch_aca(const AcaParams& P, hipStream_t st);
+hipError_t launch_body_inverse(const double* A, int qm, double* Binv, int* err, hipStream_t st);
+hipError_t launch_abd_inverse(const AbdParams& S, hipStream_t st);
+hipError_t launch_coupling_core(const FactorParams& S, hipStream_t st);
+hipError_t launch_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32, hipStream_t st);
+hipError_t launch_rhs(const double* P, const double* U, const double* Lm, const double* b, int qo, int qm, int fo,
+                      int n, int k, const int* act_node, const double* W64, double* tmp, double* y, double* rhs,
+                      double* r, double* x, hipStream_t st);
+hipError_t launch_backsub(const double* P, const double* U, const double* V, const int* act_dof, const double* b,
+                          double* x, int qo, int qm, int fo, int a, int k, double* tmp, hipStream_t st);
+hipError_t launch_full_residual(const double* yf, const double* E, const double* A, const int* act_node,
+                                const int* act_dof, int a, int qm, const double* b, const double* x, int fo, int qo,
+                                int n, double* part, double* out, hipStream_t st);
+// pcg_kernel.hip
+hipError_t launch_pcg(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st);
+hipError_t pcg_occupancy(int G, bool mixed, size_t shmem, int* blocks);
+hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st);
+hipError_t launch_op_precond64(int G, const SysDesc& S, int order, double omega, const double* r, double* z,
+                               double* tmp, hipStream_t st);
+hipError_t launch_op_lowrank64(const SysDesc& S, const double* x, double* y, double* t, hipStream_t st);
+
+}  // namespace hsc

@@ -13,6 +13,7 @@
 // bitwise reproducible.  Grid barriers have bounded waits.
 #include "common.hpp"
 #include "device_util.hpp"
+#include "kernels.hpp"
 
 namespace hsc {
 
@@ -209,11 +210,17 @@ __device__ __forceinline__ void for_tiles(const SysDesc* sys, int nsys, TileRang
   }
 }
 
-// sum_{c=c0}^{c1} part[s*nC + c] by one wave (lanes strided, butterfly)
-__device__ __forceinline__ double wave_reduce_chunks(const double* part, int s, int nC, int c0, int c1) {
+// sum_{c=c0}^{c1} part[s*nC + c] by one wave (lanes strided, butterfly).
+// Chunks that own no tile of the tiling (T tiles over nC chunks) never write
+// a partial and are skipped.
+__device__ __forceinline__ bool chunk_empty(int c, int nC, int T) {
+  return tile_of_chunk(c, nC, T) == tile_of_chunk(c + 1, nC, T);
+}
+__device__ __forceinline__ double wave_reduce_chunks(const double* part, int s, int nC, int c0, int c1, int T) {
   const int lane = threadIdx.x & 63;
   double v = 0.0;
-  for (int c = c0 + lane; c <= c1; c += 64) v += part[(size_t)s * nC + c];
+  for (int c = c0 + lane; c <= c1; c += 64)
+    if (!chunk_empty(c, nC, T)) v += part[(size_t)s * nC + c];
   return wave_sum(v);
 }
 
@@ -374,8 +381,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   auto reduce_rz = [&]() {
     for (int s = wave; s < nsys; s += kBlock / 64) {
       const SysDesc& S = sys[s];
-      double v = order > 1 ? wave_reduce_chunks(P.partF, s, P.nF, S.cF0, S.cF1)
-                           : wave_reduce_chunks(partE_rz, s, P.nF, S.cE0, S.cE1);
+      double v = order > 1 ? wave_reduce_chunks(P.partF, s, P.nF, S.cF0, S.cF1, P.rt_total)
+                           : wave_reduce_chunks(partE_rz, s, P.nF, S.cE0, S.cE1, P.et_total);
       if ((tid & 63) == 0) s_v0[s] = v;
     }
     __syncthreads();
@@ -388,7 +395,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       if (!want(s) || S.k == 0) continue;
       for (int l = tid; l < S.k; l += kBlock) {
         double v = 0.0;
-        for (int c = S.cL0; c <= S.cL1; ++c) v += P.partL[S.lrp_off + (size_t)(c - S.cL0) * S.k + l];
+        for (int c = S.cL0; c <= S.cL1; ++c)
+          if (!chunk_empty(c, P.nL, P.lt_total)) v += P.partL[S.lrp_off + (size_t)(c - S.cL0) * S.k + l];
         s_t[S.t_off + l] = v;
       }
     }
@@ -439,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
   for (int s = wave; s < nsys; s += kBlock / 64) {
     const SysDesc& S = sys[s];
-    const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1);
+    const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1, P.et_total);
     if ((tid & 63) == 0) {
       const double nb = sqrt(v);
       s_normb[s] = nb;
@@ -490,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
     for (int s = wave; s < nsys; s += kBlock / 64) {
       const SysDesc& S = sys[s];
-      const double v = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1);
+      const double v = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1, P.rt_total);
       if ((tid & 63) == 0) s_v1[s] = v;
     }
     reduce_t([&](int s) { return s_st[s] == ST_RUN; });
@@ -511,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
     for (int s = wave; s < nsys; s += kBlock / 64) {
       const SysDesc& S = sys[s];
-      const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1);
+      const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1, P.et_total);
       if ((tid & 63) == 0) s_v1[s] = v;
     }
     __syncthreads();
@@ -555,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       for (int s = wave; s < nsys; s += kBlock / 64) {
         if (s_st[s] != ST_RUN || s_chk[s] == CHK_NONE) continue;
         const SysDesc& S = sys[s];
-        const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1);
+        const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1, P.et_total);
         if ((tid & 63) == 0) s_v1[s] = v;
       }
       __syncthreads();

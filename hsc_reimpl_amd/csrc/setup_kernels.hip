@@ -1,0 +1,806 @@
+// Setup kernels of the HSC solve on gfx950: split of the BCOO Hessian into
+// D (full-storage BSR 3x3, SoA values), the dense ABD block A and the dense
+// coupling E over the active FEM DOFs; FEM 3x3 diagonal inverses; ACA of E;
+// A^-1; C = U^T A^-1 U = L L^T and W = V L; reduced right-hand side,
+// back-substitution and acceptance checks.
+// Reference semantics: linear_hsc.cu:61-91,195-304,349-435 and
+// core.py:22-50,53-92,141-168 (reproduction/ of the reference).
+#include "common.hpp"
+#include "device_util.hpp"
+#include "kernels.hpp"
+#include <cfloat>
+
+namespace hsc {
+
+enum SetupErr { E_FEM_DIAG = 0, E_UNCOVERED = 1, E_INTERBODY = 2, E_COUPLING_DOF = 3, E_NONFINITE = 4 };
+
+
+__device__ __forceinline__ bool in_range(int v, int o, int n) { return v >= o && v < o + n; }
+
+// Classify each stored block (linear_hsc.cu:61-81, :218-233): count the
+// full-storage D entries per block row, invert FEM diagonal blocks, scatter
+// ABD-ABD blocks into dense A (mirrored), flag active FEM nodes.
+__global__ void k_classify(SplitParams S) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.count) return;
+  const int R = S.rows[t] * 3, C = S.cols[t] * 3;
+  const double* v = S.vals + 9 * t;
+  const bool iq = in_range(R, S.qo, S.qm), jq = in_range(C, S.qo, S.qm);
+  const bool iff = in_range(R, S.fo, S.fn), jf = in_range(C, S.fo, S.fn);
+  if (!(iq || iff) || !(jq || jf)) {
+    atomicAdd(&S.err[E_UNCOVERED], 1);
+    return;
+  }
+  double sq = 0.0, B[3][3];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) {
+      B[r][c] = v[r + 3 * c];
+      sq += B[r][c] * B[r][c];
+    }
+  if (!isfinite(sq)) atomicAdd(&S.err[E_NONFINITE], 1);
+  if (iff && jf) {
+    const int i = (R - S.fo) / 3, j = (C - S.fo) / 3;
+    atomicAdd(&S.row_count[i], 1);
+    if (i != j) {
+      atomicAdd(&S.row_count[j], 1);
+    } else {
+      // closed-form 3x3 inverse (adjugate / determinant)
+      const double c00 = B[1][1] * B[2][2] - B[1][2] * B[2][1];
+      const double c01 = B[1][2] * B[2][0] - B[1][0] * B[2][2];
+      const double c02 = B[1][0] * B[2][1] - B[1][1] * B[2][0];
+      const double det = B[0][0] * c00 + B[0][1] * c01 + B[0][2] * c02;
+      double inv[9];
+      if (!(det > 0.0) || !isfinite(det)) {
+        atomicAdd(&S.err[E_FEM_DIAG], 1);
+        for (int e = 0; e < 9; ++e) inv[e] = 0.0;
+      } else {
+        const double id = 1.0 / det;
+        inv[0] = c00 * id;
+        inv[1] = (B[0][2] * B[2][1] - B[0][1] * B[2][2]) * id;
+        inv[2] = (B[0][1] * B[1][2] - B[0][2] * B[1][1]) * id;
+        inv[3] = c01 * id;
+        inv[4] = (B[0][0] * B[2][2] - B[0][2] * B[2][0]) * id;
+        inv[5] = (B[0][2] * B[1][0] - B[0][0] * B[1][2]) * id;
+        inv[6] = c02 * id;
+        inv[7] = (B[0][1] * B[2][0] - B[0][0] * B[2][1]) * id;
+        inv[8] = (B[0][0] * B[1][1] - B[0][1] * B[1][0]) * id;
+      }
+      for (int e = 0; e < 9; ++e) {
+        S.invD64[9 * i + e] = inv[e];
+        S.invD32[9 * i + e] = (float)inv[e];
+      }
+    }
+  } else if (iq && jq) {
+    const int a = R - S.qo, b = C - S.qo;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        atomicAdd(&S.A[(size_t)(a + r) * S.qm + b + c], B[r][c]);
+        if (a != b) atomicAdd(&S.A[(size_t)(b + c) * S.qm + a + r], B[r][c]);
+      }
+    if (a / 12 != b / 12) atomicOr(&S.err[E_INTERBODY], 1);
+  } else if (sq != 0.0) {
+    const int f = iq ? C : R;
+    if (!in_range(f, S.fo, S.fn)) {
+      atomicAdd(&S.err[E_COUPLING_DOF], 1);
+      return;
+    }
+    S.active_flag[(f - S.fo) / 3] = 1;
+  }
+}
+
+// Single-workgroup exclusive scan: out[i] = sum_{j<i} in[i], out[n] = total.
+// If mul != 1 the output is scaled (used to turn a node count into W rows).
+__global__ __launch_bounds__(1024) void k_scan(const int* in, int* out, int n) {
+  __shared__ int s[1024];
+  const int tid = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
+  int sum = 0;
+  for (int i = b0; i < b1; ++i) sum += in[i];
+  s[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = tid >= off ? s[tid - off] : 0;
+    __syncthreads();
+    s[tid] += v;
+    __syncthreads();
+  }
+  int run = tid ? s[tid - 1] : 0;
+  for (int i = b0; i < b1; ++i) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (tid == 1023) out[n] = s[1023];
+}
+
+// act_node[i] = 3*prefix (first W row of the node) or -1; act_dof[W row] = FEM DOF
+__global__ void k_active(const int* flag, const int* prefix, int nb, int* act_node, int* act_dof) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  if (flag[i]) {
+    const int w = 3 * prefix[i];
+    act_node[i] = w;
+    act_dof[w] = 3 * i;
+    act_dof[w + 1] = 3 * i + 1;
+    act_dof[w + 2] = 3 * i + 2;
+  } else {
+    act_node[i] = -1;
+  }
+}
+
+
+// Place FEM-FEM blocks into their BSR rows (slot order fixed later by the
+// per-row sort) and scatter coupling blocks into dense E (linear_hsc.cu:237-247).
+__global__ void k_place(PlaceParams S) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.count) return;
+  const int R = S.rows[t] * 3, C = S.cols[t] * 3;
+  const bool iq = in_range(R, S.qo, S.qm), jq = in_range(C, S.qo, S.qm);
+  const bool iff = in_range(R, S.fo, S.fn), jf = in_range(C, S.fo, S.fn);
+  if (iff && jf) {
+    const int i = (R - S.fo) / 3, j = (C - S.fo) / 3;
+    int pos = S.rowptr[i] + atomicAdd(&S.cursor[i], 1);
+    S.key_col[pos] = j;
+    S.key_src[pos] = (int)(2 * t);
+    if (i != j) {
+      pos = S.rowptr[j] + atomicAdd(&S.cursor[j], 1);
+      S.key_col[pos] = i;
+      S.key_src[pos] = (int)(2 * t + 1);
+    }
+  } else if (iq != jq && (iq ? jf : iff)) {
+    const double* v = S.vals + 9 * t;
+    const int f = iq ? C : R, q = (iq ? R : C) - S.qo;
+    const int col = S.act_node[(f - S.fo) / 3];
+    if (col < 0) return;  // zero block
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        // iq: block = E[q+r, col+c]; else block = E^T: E[q+c, col+r]
+        const double bv = v[r + 3 * c];
+        if (iq) atomicAdd(&S.E[(size_t)(q + r) * S.a + col + c], bv);
+        else atomicAdd(&S.E[(size_t)(q + c) * S.a + col + r], bv);
+      }
+  }
+}
+
+// Sort each BSR row by (column, source) -- deterministic slot order.
+__global__ void k_sort_rows(const int* rowptr, int nb, int* key_col, int* key_src) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int p0 = rowptr[i], p1 = rowptr[i + 1];
+  for (int p = p0 + 1; p < p1; ++p) {
+    const int kc = key_col[p], ks = key_src[p];
+    int q = p - 1;
+    while (q >= p0 && (key_col[q] > kc || (key_col[q] == kc && key_src[q] > ks))) {
+      key_col[q + 1] = key_col[q];
+      key_src[q + 1] = key_src[q];
+      --q;
+    }
+    key_col[q + 1] = kc;
+    key_src[q + 1] = ks;
+  }
+}
+
+// Gather block values into SoA (component c = r + 3*col of the 3x3) FP64/FP32.
+__global__ void k_gather(const double* vals, const int* key_col, const int* key_src, int nnzb,
+                         int* colidx, double* v64, float* v32) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnzb) return;
+  const int src = key_src[p];
+  const int t = src >> 1;
+  const bool tr = src & 1;
+  const double* v = vals + 9 * (size_t)t;
+  colidx[p] = key_col[p];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) {
+      const double b = tr ? v[c + 3 * r] : v[r + 3 * c];
+      v64[(size_t)(r + 3 * c) * nnzb + p] = b;
+      v32[(size_t)(r + 3 * c) * nnzb + p] = (float)b;
+    }
+}
+
+// ---------------------------------------------------------------- ACA
+// Full-pivot ACA (core.py:22-50; linear_hsc.cu:248-262) as one cooperative
+// launch.  R is double-buffered so u = R[:,j], v = R[i,:]/piv are read from
+// the previous iterate while the next is written; products and differences
+// are rounded separately (no FMA) exactly like NumPy's R -= outer(u, v), so
+// pivots agree with the NumPy method.  Pivot ties go to the lowest row-major
+// index (np.argmax).
+__device__ __forceinline__ void argmax_combine(double& m, long long& i, double m2, long long i2) {
+  if (m2 > m || (m2 == m && i2 < i)) {
+    m = m2;
+    i = i2;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
+  __shared__ double s_m[kBlock / 64], s_q[kBlock / 64];
+  __shared__ long long s_i[kBlock / 64];
+  __shared__ double s_scale, s_res, s_piv;
+  __shared__ long long s_idx;
+  __shared__ int s_go;
+  const int tid = threadIdx.x, b = blockIdx.x;
+  const long long total = (long long)P.qm * P.a;
+  const long long gthreads = (long long)P.nwg * kBlock;
+  const long long g = (long long)b * kBlock + tid;
+  unsigned epoch = 0;
+
+  auto wg_reduce = [&](double m, long long idx, double q) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      double m2 = __shfl_xor(m, off, 64);
+      long long i2 = __shfl_xor(idx, off, 64);
+      argmax_combine(m, idx, m2, i2);
+      q += __shfl_xor(q, off, 64);
+    }
+    const int w = tid >> 6;
+    if ((tid & 63) == 0) {
+      s_m[w] = m;
+      s_i[w] = idx;
+      s_q[w] = q;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double M = s_m[0], Q = s_q[0];
+      long long I = s_i[0];
+      for (int w2 = 1; w2 < kBlock / 64; ++w2) {
+        argmax_combine(M, I, s_m[w2], s_i[w2]);
+        Q += s_q[w2];
+      }
+      P.part[4 * b + 0] = M;
+      P.part[4 * b + 1] = (double)I;
+      P.part[4 * b + 2] = Q;
+    }
+  };
+  auto all_reduce = [&]() {
+    if (tid == 0) {
+      double M = -1.0, Q = 0.0;
+      long long I = 0x7fffffffffffffffLL;
+      for (int w = 0; w < P.nwg; ++w) {
+        argmax_combine(M, I, P.part[4 * w + 0], (long long)P.part[4 * w + 1]);
+        Q += P.part[4 * w + 2];
+      }
+      s_idx = I;
+      s_q[0] = Q;
+    }
+    __syncthreads();
+  };
+
+  double m = -1.0, q = 0.0;
+  long long mi = 0x7fffffffffffffffLL;
+  for (long long e = g; e < total; e += gthreads) {
+    const double v = P.E[e];
+    P.R0[e] = v;
+    argmax_combine(m, mi, fabs(v), e);
+    q += v * v;
+  }
+  wg_reduce(m, mi, q);
+  if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) return;
+  all_reduce();
+  if (tid == 0) {
+    s_scale = sqrt(s_q[0]);
+    s_res = s_scale > 0.0 ? 1.0 : 0.0;
+    if (b == 0) {
+      P.hist[0] = s_res;
+      *P.scale_out = s_scale;
+    }
+  }
+  __syncthreads();
+  const double scale = s_scale;
+  const double guard = DBL_EPSILON * fmax(scale, DBL_MIN);
+  double* cur = P.R0;
+  double* nxt = P.R1;
+  int kk = 0;
+  for (;;) {
+    if (tid == 0) {
+      s_go = (kk < P.limit && s_res > P.stop && isfinite(scale)) ? 1 : 0;
+      if (s_go) {
+        s_piv = cur[s_idx];
+        if (!(fabs(s_piv) > guard)) s_go = 0;
+      }
+    }
+    __syncthreads();
+    if (!s_go) break;
+    const long long pi = s_idx / P.a, pj = s_idx % P.a;
+    const double piv = s_piv;
+    // factor columns
+    for (long long r = g; r < P.qm; r += gthreads) P.U[r + (long long)P.qm * kk] = cur[r * P.a + pj];
+    for (long long c = g; c < P.a; c += gthreads) P.V[c + (long long)P.a * kk] = cur[pi * P.a + c] / piv;
+    if (b == 0 && tid == 0) {
+      P.pivots[2 * kk] = (int)pi;
+      P.pivots[2 * kk + 1] = (int)pj;
+    }
+    m = -1.0;
+    q = 0.0;
+    mi = 0x7fffffffffffffffLL;
+    for (long long e = g; e < total; e += gthreads) {
+      const long long r = e / P.a, c = e - r * P.a;
+      const double u = cur[r * P.a + pj];
+      const double v = cur[pi * P.a + c] / piv;
+      const double val = __dsub_rn(cur[e], __dmul_rn(u, v));
+      nxt[e] = val;
+      argmax_combine(m, mi, fabs(val), e);
+      q += val * val;
+    }
+    wg_reduce(m, mi, q);
+    if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) return;
+    all_reduce();
+    ++kk;
+    if (tid == 0) {
+      s_res = sqrt(s_q[0]) / scale;
+      if (b == 0) P.hist[kk] = s_res;
+    }
+    __syncthreads();
+    double* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  if (b == 0 && tid == 0) *P.rank_out = kk;
+}
+
+// ---------------------------------------------------------------- small dense LA
+// Per-body 12x12 Cholesky + inverse of the diagonal ABD blocks
+// (core.py:53-71 BlockInverse; linear_hsc.cu:264-271).  One wave per body.
+__global__ void k_body_inverse(const double* A, int qm, double* Binv, int* err) {
+  __shared__ double L[12][13];
+  __shared__ int ok;
+  const int body = blockIdx.x, tid = threadIdx.x, o = 12 * body;
+  if (tid < 144) L[tid / 12][tid % 12] = A[(size_t)(o + tid / 12) * qm + o + tid % 12];
+  __syncthreads();
+  if (tid == 0) {
+    ok = 1;
+    for (int j = 0; j < 12 && ok; ++j) {
+      double d = L[j][j];
+      for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+      if (!(d > 0.0) || !isfinite(d)) {
+        ok = 0;
+        break;
+      }
+      d = sqrt(d);
+      L[j][j] = d;
+      for (int i = j + 1; i < 12; ++i) {
+        double s = L[i][j];
+        for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+        L[i][j] = s / d;
+      }
+    }
+    if (!ok) atomicAdd(err, 1);
+  }
+  __syncthreads();
+  if (!ok) return;
+  if (tid < 12) {  // column tid of (L L^T)^-1
+    double y[12];
+    for (int i = 0; i < 12; ++i) {
+      double s = (i == tid) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+      y[i] = s / L[i][i];
+    }
+    for (int i = 11; i >= 0; --i) {
+      double s = y[i];
+      for (int k = i + 1; k < 12; ++k) s -= L[k][i] * y[k];
+      y[i] = s / L[i][i];
+    }
+    for (int i = 0; i < 12; ++i) Binv[(size_t)body * 144 + i * 12 + tid] = y[i];
+  }
+}
+
+// In-place Cholesky of an n x n row-major SPD matrix by one workgroup
+// (lower triangle = L).  Returns false (uniformly) when not SPD / not finite.
+__device__ bool wg_cholesky(double* M, int n) {
+  __shared__ int s_ok;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) s_ok = 1;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (tid == 0) {
+      const double d = M[(size_t)j * n + j];
+      if (!(d > 0.0) || !isfinite(d)) s_ok = 0;
+      else M[(size_t)j * n + j] = sqrt(d);
+    }
+    __syncthreads();
+    if (!s_ok) return false;
+    const double djj = M[(size_t)j * n + j];
+    for (int i = j + 1 + tid; i < n; i += nt) M[(size_t)i * n + j] /= djj;
+    __syncthreads();
+    const int m = n - j - 1;
+    for (int e = tid; e < m * m; e += nt) {
+      const int i = j + 1 + e / m, c = j + 1 + e % m;
+      if (c <= i) M[(size_t)i * n + c] -= M[(size_t)i * n + j] * M[(size_t)c * n + j];
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
+// Inv = (L L^T)^-1 from the lower factor in M (n x n); column per thread.
+__device__ void wg_chol_inverse(const double* M, int n, double* Inv, double* scratch) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int col = tid; col < n; col += nt) {
+    double* y = scratch + (size_t)col * n;
+    for (int i = 0; i < n; ++i) {
+      double s = (i == col) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) s -= M[(size_t)i * n + k] * y[k];
+      y[i] = s / M[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = y[i];
+      for (int k = i + 1; k < n; ++k) s -= M[(size_t)k * n + i] * y[k];
+      y[i] = s / M[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; ++i) Inv[(size_t)i * n + col] = y[i];
+  }
+  __syncthreads();
+}
+
+// out = X * Y (n x n row-major), one workgroup
+__device__ void wg_gemm(const double* X, const double* Y, double* out, int n) {
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e % n;
+    double s = 0.0;
+    for (int m = 0; m < n; ++m) s += X[(size_t)i * n + m] * Y[(size_t)m * n + j];
+    out[e] = s;
+  }
+  __syncthreads();
+}
+
+// block-diagonal B^-1 (from Binv) times X, n x n
+__device__ void wg_binv_mul(const double* Binv, const double* X, double* out, int n, double scale) {
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e % n, body = i / 12, o = 12 * body;
+    double s = 0.0;
+    for (int m = 0; m < 12; ++m) s += Binv[(size_t)body * 144 + (i - o) * 12 + m] * X[(size_t)(o + m) * n + j];
+    out[e] = scale * s;
+  }
+  __syncthreads();
+}
+
+// residual ||I - A P||_F / sqrt(n); Rm = I - A P
+__device__ double wg_inverse_residual(const double* A, const double* P, double* Rm, int n, double* red) {
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e % n;
+    double s = (i == j) ? 1.0 : 0.0;
+    for (int m = 0; m < n; ++m) s -= A[(size_t)i * n + m] * P[(size_t)m * n + j];
+    Rm[e] = s;
+    acc += s * s;
+  }
+  __syncthreads();
+  const double tot = block_sum_t0(acc, red);
+  __shared__ double s_tot;
+  if (threadIdx.x == 0) s_tot = tot;
+  __syncthreads();
+  return sqrt(s_tot) / sqrt((double)n);
+}
+
+__device__ void wg_symmetrize(double* P, int n) {
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e % n;
+    if (j > i) {
+      const double s = 0.5 * (P[(size_t)i * n + j] + P[(size_t)j * n + i]);
+      P[(size_t)i * n + j] = s;
+      P[(size_t)j * n + i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+
+// A^-1 approximation P (one workgroup): exact Cholesky inverse, or the block
+// Neumann polynomial P_r = sum_{k<r} (I - w B^-1 A)^k w B^-1 symmetrised, then
+// optional doubling P <- P + P (I - A P) until ||I-AP||_F/sqrt(qm) <= tol
+// (linear_hsc.cu:263-297; core.py:74-92,149-150).
+__global__ __launch_bounds__(1024) void k_abd_inverse(AbdParams S) {
+  __shared__ double red[1024 / 64];
+  const int n = S.qm, nn = n * n;
+  int status = 0, doublings = 0;
+  if (S.mode == 0) {
+    if (S.blockdiag) {
+      for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        const int i = e / n, j = e % n;
+        S.P[e] = (i / 12 == j / 12) ? S.Binv[(size_t)(i / 12) * 144 + (i % 12) * 12 + (j % 12)] : 0.0;
+      }
+      __syncthreads();
+    } else {
+      for (int e = threadIdx.x; e < nn; e += blockDim.x) S.s1[e] = S.A[e];
+      __syncthreads();
+      if (!wg_cholesky(S.s1, n)) status = 4;
+      else wg_chol_inverse(S.s1, n, S.P, S.s2);
+    }
+  } else {
+    // term = w B^-1 ; P = term
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+      const int i = e / n, j = e % n;
+      const double v = (i / 12 == j / 12) ? S.omega * S.Binv[(size_t)(i / 12) * 144 + (i % 12) * 12 + (j % 12)] : 0.0;
+      S.s1[e] = v;
+      S.P[e] = v;
+    }
+    __syncthreads();
+    for (int k = 1; k < S.order; ++k) {
+      wg_gemm(S.A, S.s1, S.s2, n);                  // A term
+      wg_binv_mul(S.Binv, S.s2, S.s3, n, S.omega);  // w B^-1 A term
+      for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        S.s1[e] -= S.s3[e];
+        S.P[e] += S.s1[e];
+      }
+      __syncthreads();
+    }
+    wg_symmetrize(S.P, n);
+  }
+  double res = 0.0;
+  if (status == 0) {
+    res = wg_inverse_residual(S.A, S.P, S.s3, n, red);
+    if (S.mode == 2) {
+      while (res > S.tol && doublings < 48) {
+        wg_gemm(S.P, S.s3, S.s2, n);  // P (I - A P)
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) S.P[e] += S.s2[e];
+        __syncthreads();
+        wg_symmetrize(S.P, n);
+        res = wg_inverse_residual(S.A, S.P, S.s3, n, red);
+        ++doublings;
+        if (!isfinite(res)) break;
+      }
+      if (!isfinite(res) || res > S.tol) status = 5;
+    }
+    if (!isfinite(res)) status = status ? status : 6;
+  }
+  if (threadIdx.x == 0) {
+    S.out[0] = res;
+    S.out[1] = (double)doublings;
+    S.out[2] = (double)status;
+  }
+}
+
+
+// C = U^T P U symmetrised (linear_hsc.cu:298-300) and its Cholesky C = L L^T.
+__global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
+  const int n = S.qm, k = S.k;
+  for (int e = threadIdx.x; e < n * k; e += blockDim.x) {
+    const int i = e / k, l = e % k;
+    double s = 0.0;
+    for (int m = 0; m < n; ++m) s += S.P[(size_t)i * n + m] * S.U[m + (size_t)n * l];
+    S.W1[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    const int l1 = e / k, l2 = e % k;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += S.U[i + (size_t)n * l1] * S.W1[(size_t)i * k + l2];
+    S.C[e] = s;
+  }
+  __syncthreads();
+  wg_symmetrize(S.C, k);
+  int bad = 0;
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    S.Lm[e] = S.C[e];
+    if (!isfinite(S.C[e])) bad = 1;
+  }
+  __shared__ int s_bad;
+  if (threadIdx.x == 0) s_bad = 0;
+  __syncthreads();
+  if (bad) atomicOr(&s_bad, 1);
+  __syncthreads();
+  int status = s_bad ? 6 : 0;
+  if (!status && !wg_cholesky(S.Lm, k)) status = 6;
+  if (threadIdx.x == 0) S.out[0] = (double)status;
+}
+
+// W = V L (a x k row-major, FP64 and FP32): V C V^T = W W^T
+__global__ void k_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)a * k) return;
+  const int j = (int)(e / k), l = (int)(e % k);
+  double s = 0.0;
+  for (int m = l; m < k; ++m) s += V[j + (size_t)a * m] * Lm[(size_t)m * k + l];
+  W64[e] = s;
+  W32[e] = (float)s;
+}
+
+// y = L^-1 (U^T P b_q)  (reduced right-hand side, core.py:163-164;
+// linear_hsc.cu:351-357 with V c = W L^-1 c)
+__global__ __launch_bounds__(256) void k_rhs_small(const double* P, const double* U, const double* Lm,
+                                                   const double* b, int qo, int qm, int k, double* tmp, double* y) {
+  // tmp[0:qm] = P b_q
+  for (int i = threadIdx.x; i < qm; i += blockDim.x) {
+    double s = 0.0;
+    for (int m = 0; m < qm; ++m) s += P[(size_t)i * qm + m] * b[qo + m];
+    tmp[i] = s;
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < k; l += blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < qm; ++i) s += U[i + (size_t)qm * l] * tmp[i];
+    tmp[qm + l] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < k; ++i) {
+      double s = tmp[qm + i];
+      for (int m = 0; m < i; ++m) s -= Lm[(size_t)i * k + m] * y[m];
+      y[i] = s / Lm[(size_t)i * k + i];
+    }
+  }
+}
+
+// rhs = b_x - W y ; r = rhs ; x_fem = 0
+__global__ void k_rhs_vec(const double* b, int fo, int n, const int* act_node, const double* W64, int k,
+                          const double* y, double* rhs, double* r, double* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = b[fo + i];
+  const int an = act_node[i / 3];
+  if (an >= 0 && k > 0) {
+    const double* w = W64 + (size_t)(an + i % 3) * k;
+    double g = 0.0;
+    for (int l = 0; l < k; ++l) g += w[l] * y[l];
+    v -= g;
+  }
+  rhs[i] = v;
+  r[i] = v;
+  x[fo + i] = 0.0;
+}
+
+// dq = P (b_q - U V^T dx) written to x[qo:qo+qm]  (core.py:167; linear_hsc.cu:395-399)
+__global__ __launch_bounds__(256) void k_backsub(const double* P, const double* U, const double* V,
+                                                 const int* act_dof, const double* b, double* x, int qo,
+                                                 int qm, int fo, int a, int k, double* tmp) {
+  __shared__ double red[256 / 64];
+  for (int l = 0; l < k; ++l) {
+    double s = 0.0;
+    for (int j = threadIdx.x; j < a; j += blockDim.x) s += V[j + (size_t)a * l] * x[fo + act_dof[j]];
+    const double tot = block_sum_t0(s, red);
+    if (threadIdx.x == 0) tmp[l] = tot;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < qm; i += blockDim.x) {
+    double s = b[qo + i];
+    for (int l = 0; l < k; ++l) s -= U[i + (size_t)qm * l] * tmp[l];
+    tmp[k + i] = s;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < qm; i += blockDim.x) {
+    double s = 0.0;
+    for (int m = 0; m < qm; ++m) s += P[(size_t)i * qm + m] * tmp[k + m];
+    x[qo + i] = s;
+  }
+}
+
+// full H x from the split parts: y_fem = D x_f (precomputed in yf) + E^T x_q,
+// y_abd = E x_f[act] + A x_q; then partial sums of (b - Hx)^2, b^2, b.x
+__global__ void k_full_residual_fem(const double* yf, const double* E, const int* act_node, int a, int qm,
+                                    const double* b, const double* x, int fo, int qo, int n, double* part) {
+  __shared__ double red[kBlock / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, bb = 0.0, bx = 0.0;
+  if (i < n) {
+    double hx = yf[i];
+    const int an = act_node[i / 3];
+    if (an >= 0) {
+      const int col = an + i % 3;
+      for (int q = 0; q < qm; ++q) hx += E[(size_t)q * a + col] * x[qo + q];
+    }
+    const double bi = b[fo + i];
+    rr = (bi - hx) * (bi - hx);
+    bb = bi * bi;
+    bx = bi * x[fo + i];
+  }
+  const double s0 = block_sum_t0(rr, red);
+  const double s1 = block_sum_t0(bb, red);
+  const double s2 = block_sum_t0(bx, red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = s0;
+    part[3 * blockIdx.x + 1] = s1;
+    part[3 * blockIdx.x + 2] = s2;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_full_residual_final(const double* E, const double* A, const int* act_dof,
+                                                             int a, int qm, const double* b, const double* x,
+                                                             int fo, int qo, const double* part, int nparts,
+                                                             double* out) {
+  __shared__ double red[256 / 64];
+  double rr = 0.0, bb = 0.0, bx = 0.0;
+  for (int i = threadIdx.x; i < qm; i += blockDim.x) {
+    double hx = 0.0;
+    for (int j = 0; j < a; ++j) hx += E[(size_t)i * a + j] * x[fo + act_dof[j]];
+    for (int m = 0; m < qm; ++m) hx += A[(size_t)i * qm + m] * x[qo + m];
+    const double bi = b[qo + i];
+    rr += (bi - hx) * (bi - hx);
+    bb += bi * bi;
+    bx += bi * x[qo + i];
+  }
+  const double s0 = block_sum_t0(rr, red);
+  const double s1 = block_sum_t0(bb, red);
+  const double s2 = block_sum_t0(bx, red);
+  if (threadIdx.x == 0) {
+    double R = s0, B = s1, X = s2;
+    for (int p = 0; p < nparts; ++p) {
+      R += part[3 * p];
+      B += part[3 * p + 1];
+      X += part[3 * p + 2];
+    }
+    out[0] = R;
+    out[1] = B;
+    out[2] = X;
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+hipError_t launch_split(const SplitParams& S, hipStream_t st) {
+  if (S.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_classify, dim3(cdiv(S.count, 256)), dim3(256), 0, st, S);
+  return hipGetLastError();
+}
+hipError_t launch_scan(const int* in, int* out, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, in, out, n);
+  return hipGetLastError();
+}
+hipError_t launch_active(const int* flag, const int* prefix, int nb, int* act_node, int* act_dof, hipStream_t st) {
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_active, dim3(cdiv(nb, 256)), dim3(256), 0, st, flag, prefix, nb, act_node, act_dof);
+  return hipGetLastError();
+}
+hipError_t launch_place(const PlaceParams& S, hipStream_t st) {
+  if (S.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_place, dim3(cdiv(S.count, 256)), dim3(256), 0, st, S);
+  return hipGetLastError();
+}
+hipError_t launch_sort_gather(const int* rowptr, int nb, int* key_col, int* key_src, const double* vals, int nnzb,
+                              int* colidx, double* v64, float* v32, hipStream_t st) {
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_rows, dim3(cdiv(nb, 128)), dim3(128), 0, st, rowptr, nb, key_col, key_src);
+  if (nnzb > 0)
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(nnzb, 256)), dim3(256), 0, st, vals, key_col, key_src, nnzb, colidx, v64, v32);
+  return hipGetLastError();
+}
+hipError_t aca_occupancy(int* blocks) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_aca, kBlock, 0);
+}
+hipError_t launch_aca(const AcaParams& P, hipStream_t st) {
+  void* args[] = {(void*)&P};
+  return hipLaunchCooperativeKernel((const void*)k_aca, dim3(P.nwg), dim3(kBlock), args, 0, st);
+}
+hipError_t launch_body_inverse(const double* A, int qm, double* Binv, int* err, hipStream_t st) {
+  if (qm == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_body_inverse, dim3(qm / 12), dim3(64), 0, st, A, qm, Binv, err);
+  return hipGetLastError();
+}
+hipError_t launch_abd_inverse(const AbdParams& S, hipStream_t st) {
+  hipLaunchKernelGGL(k_abd_inverse, dim3(1), dim3(1024), 0, st, S);
+  return hipGetLastError();
+}
+hipError_t launch_coupling_core(const FactorParams& S, hipStream_t st) {
+  hipLaunchKernelGGL(k_coupling_core, dim3(1), dim3(1024), 0, st, S);
+  return hipGetLastError();
+}
+hipError_t launch_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32, hipStream_t st) {
+  if ((long long)a * k == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_make_W, dim3(cdiv((long long)a * k, 256)), dim3(256), 0, st, V, Lm, a, k, W64, W32);
+  return hipGetLastError();
+}
+hipError_t launch_rhs(const double* P, const double* U, const double* Lm, const double* b, int qo, int qm, int fo,
+                      int n, int k, const int* act_node, const double* W64, double* tmp, double* y, double* rhs,
+                      double* r, double* x, hipStream_t st) {
+  if (k > 0) hipLaunchKernelGGL(k_rhs_small, dim3(1), dim3(256), 0, st, P, U, Lm, b, qo, qm, k, tmp, y);
+  if (n > 0)
+    hipLaunchKernelGGL(k_rhs_vec, dim3(cdiv(n, 256)), dim3(256), 0, st, b, fo, n, act_node, W64, k, y, rhs, r, x);
+  return hipGetLastError();
+}
+hipError_t launch_backsub(const double* P, const double* U, const double* V, const int* act_dof, const double* b,
+                          double* x, int qo, int qm, int fo, int a, int k, double* tmp, hipStream_t st) {
+  if (qm == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(256), 0, st, P, U, V, act_dof, b, x, qo, qm, fo, a, k, tmp);
+  return hipGetLastError();
+}
+hipError_t launch_full_residual(const double* yf, const double* E, const double* A, const int* act_node,
+                                const int* act_dof, int a, int qm, const double* b, const double* x, int fo, int qo,
+                                int n, double* part, double* out, hipStream_t st) {
+  const int nparts = cdiv(n, kBlock);
+  if (nparts > 0)
+    hipLaunchKernelGGL(k_full_residual_fem, dim3(nparts), dim3(kBlock), 0, st, yf, E, act_node, a, qm, b, x, fo,
+                       qo, n, part);
+  hipLaunchKernelGGL(k_full_residual_final, dim3(1), dim3(256), 0, st, E, A, act_dof, a, qm, b, x, fo, qo, part,
+                     nparts, out);
+  return hipGetLastError();
+}
+
+}  // namespace hsc

@@ -1,0 +1,983 @@
+// Host side of the MI355X HSC solver: workspaces, setup pipeline, persistent
+// PCG launch (single and batched), autotuner, acceptance checks and the C ABI
+// declared in include/hsc.h.  Mirrors LinearHSC::do_build / setup / do_solve
+// of the reference (reproduction/hsc_backend/linear_hsc.cu:172-436).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/hsc.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes == 0) bytes = 64;
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct HscError {
+  int code;
+  std::string msg;
+};
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      throw HscError{HSC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)};     \
+  } while (0)
+
+constexpr long long kTimeoutTicks = 100LL * 1000 * 1000 * 20;  // 20 s at 100 MHz
+
+struct DeviceInfo {
+  int cu = 0;
+};
+
+DeviceInfo device_info(int dev) {
+  static std::mutex mu;
+  static std::map<int, DeviceInfo> cache;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  hipDeviceProp_t prop;
+  DeviceInfo d;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) d.cu = prop.multiProcessorCount;
+  if (d.cu <= 0) d.cu = 1;
+  cache[dev] = d;
+  return d;
+}
+
+// autotune table: (nb bucket, nnzb per row bucket, rank, precision) -> (G, wg per CU)
+struct TuneKey {
+  int nb_log2, deg, kbucket, mixed, batch;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(nb_log2, deg, kbucket, mixed, batch) < std::tie(o.nb_log2, o.deg, o.kbucket, o.mixed, o.batch);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tune;
+
+}  // namespace
+
+struct hsc_handle {
+  hsc_config cfg{};
+  int device = 0;
+  std::string err;
+  // system layout
+  int N = 0, qo = 0, qm = 0, fo = 0, fn = 0, nb = 0, nnzb = 0, a = 0, k = 0, limit = 0;
+  long long count = 0;
+  bool blockdiag = true, setup_ok = false;
+  double aca_err = 0.0, abd_res = 0.0, setup_ms = 0.0;
+  int doublings = 0;
+  int G = 8, wgpcu = 1;
+  // buffers
+  Buf row_count, rowptr, cursor, active_flag, act_prefix, act_node, act_dof;
+  Buf key_col, key_src, colidx, val64, val32, invD64, invD32;
+  Buf A, E, R0, R1, U, V, hist, pivots, acapart, Binv, P, s1, s2, s3, W1, C, Lm, W64, W32;
+  Buf rhs, r, p0, p1, z0, z1, q, q64, yfull, small, ybuf, misc, ints;
+  // persistent-launch scratch
+  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, respart;
+  // learned preconditioner weights (device, BF16 blob)
+  Buf learned;
+  long long learned_bytes = 0;
+};
+
+namespace {
+
+void set_err(hsc_handle* h, int code, const std::string& m) {
+  if (h) h->err = m;
+  (void)code;
+}
+
+int validate_config(const hsc_config& c, std::string& why) {
+  // linear_hsc.cu:188-189
+  if (!(c.max_rank > 0 && c.neumann_order > 0 && c.max_iter > 0 && c.omega > 0 && c.omega <= 1)) {
+    why = "Invalid HSC parameters (max_rank, neumann_order, max_iter > 0 and 0 < omega <= 1 required)";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  if (!(c.tol_rate > 0) || !(c.abd_inverse_tol > 0) || !(c.full_residual_limit > 0) || c.aca_rtol < 0) {
+    why = "Invalid HSC tolerances";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  if (c.abd_inverse < 0 || c.abd_inverse > 2 || c.precision < 0 || c.precision > 1 || c.preconditioner < 0 ||
+      c.preconditioner > 1 || c.residual_replace_every < 0) {
+    why = "Invalid HSC mode selection";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  if (!(c.lanes_per_row == 0 || c.lanes_per_row == 4 || c.lanes_per_row == 8 || c.lanes_per_row == 16)) {
+    why = "lanes_per_row must be 0, 4, 8 or 16";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  if (c.wg_per_cu < 0 || c.wg_per_cu > 4) {
+    why = "wg_per_cu must be in 0..4";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  return HSC_OK;
+}
+
+hsc::SysDesc make_desc(hsc_handle* h) {
+  hsc::SysDesc S{};
+  S.nb = h->nb;
+  S.n = h->fn;
+  S.a = h->k > 0 ? h->a : 0;
+  S.k = h->k;
+  S.nnzb = h->nnzb;
+  S.rowptr = h->rowptr.as<int>();
+  S.colidx = h->colidx.as<int>();
+  S.val64 = h->val64.as<double>();
+  S.val32 = h->val32.as<float>();
+  S.invD64 = h->invD64.as<double>();
+  S.invD32 = h->invD32.as<float>();
+  S.act_node = h->act_node.as<int>();
+  S.act_dof = h->act_dof.as<int>();
+  S.W64 = h->W64.as<double>();
+  S.W32 = h->W32.as<float>();
+  S.rhs = h->rhs.as<double>();
+  S.r = h->r.as<double>();
+  S.p[0] = h->p0.p;
+  S.p[1] = h->p1.p;
+  S.z[0] = h->z0.p;
+  S.z[1] = h->z1.p;
+  S.q = h->q.p;
+  S.q64 = h->cfg.precision == HSC_PRECISION_MIXED ? h->q64.as<double>() : h->q.as<double>();
+  S.tol = h->cfg.tol_rate;
+  S.max_iter = h->cfg.max_iter;
+  return S;
+}
+
+// XCD-aware chunk order: workgroup b runs on XCD b % 8; chunks are numbered
+// so that each XCD owns a contiguous run of chunks (neighbouring rows share
+// that XCD's L2).
+void xcd_ranks(int b0, int n, std::vector<int>& out) {
+  int cnt[8] = {0};
+  for (int b = b0; b < b0 + n; ++b) cnt[b % 8]++;
+  int base[8];
+  int acc = 0;
+  for (int x = 0; x < 8; ++x) {
+    base[x] = acc;
+    acc += cnt[x];
+  }
+  int seen[8] = {0};
+  for (int b = b0; b < b0 + n; ++b) {
+    const int x = b % 8;
+    out[b] = base[x] + seen[x]++;
+  }
+}
+
+int tile_of_chunk(int c, int nC, int T) { return (int)(((long long)c * T) / nC); }
+int chunk_of(int t, int nC, int T) {
+  int c = (int)(((long long)t * nC) / T);
+  c = std::min(std::max(c, 0), nC - 1);
+  while (c + 1 < nC && tile_of_chunk(c + 1, nC, T) <= t) ++c;
+  while (c > 0 && tile_of_chunk(c, nC, T) > t) --c;
+  return c;
+}
+
+struct LaunchCfg {
+  int G, wgpcu;
+};
+
+// Persistent PCG over a batch of set-up handles.  Returns per-system outputs.
+std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_override,
+                                 hipStream_t st, float* pcg_ms) {
+  hsc_handle* h0 = hs[0];
+  const bool mixed = h0->cfg.precision == HSC_PRECISION_MIXED;
+  const int G = lc.G;
+  const int R = hsc::kBlock / G;
+  std::vector<hsc::SysDesc> S(nsys);
+  int rt = 0, et = 0, lt = 0, t_total = 0;
+  double lr_bytes = 0, row_bytes = 0;
+  const double eltM = mixed ? 4.0 : 8.0;
+  for (int s = 0; s < nsys; ++s) {
+    S[s] = make_desc(hs[s]);
+    if (max_iter_override > 0) S[s].max_iter = max_iter_override;
+    S[s].rt_begin = rt;
+    rt += (S[s].nb + R - 1) / R;
+    S[s].rt_end = rt;
+    S[s].et_begin = et;
+    et += (S[s].nb + hsc::kNodeTile - 1) / hsc::kNodeTile;
+    S[s].et_end = et;
+    S[s].lt_begin = lt;
+    if (S[s].k > 0) lt += (S[s].a + hsc::kLrTile - 1) / hsc::kLrTile;
+    S[s].lt_end = lt;
+    S[s].t_off = t_total;
+    t_total += S[s].k;
+    lr_bytes += (double)S[s].a * S[s].k * eltM;
+    row_bytes += (double)S[s].nnzb * (9 * eltM + 4);
+    if (S[s].k > hsc::kMaxRankLds) throw HscError{HSC_ERR_CAPACITY, "rank exceeds 256"};
+  }
+  const DeviceInfo di = device_info(h0->device);
+  const size_t shmem = (size_t)std::max(t_total, 1) * sizeof(double);
+  int occ = 0;
+  HIPCHK(hsc::pcg_occupancy(G, mixed, shmem, &occ));
+  if (occ <= 0) throw HscError{HSC_ERR_CAPACITY, "persistent PCG kernel does not fit on a CU"};
+  const int cap = occ * di.cu;
+  int nwg = std::min(std::min(lc.wgpcu, occ) * di.cu, std::max(rt, 1));
+  int nL = 0;
+  if (lt > 0) {
+    nwg = std::max(nwg, 2);
+    nL = (int)std::lround(nwg * lr_bytes / std::max(lr_bytes + row_bytes, 1.0));
+    nL = std::max(1, std::min(nL, std::min(lt, nwg - 1)));
+  }
+  if (nwg > cap) throw HscError{HSC_ERR_CAPACITY, "grid exceeds co-resident capacity"};
+  const int nA = nwg - nL, nF = nwg;
+  std::vector<int> chA(nwg, -1), chF(nwg, -1), chL(nwg, -1), tmp(nwg, -1);
+  xcd_ranks(0, nwg, chF);
+  if (nL > 0) {
+    xcd_ranks(0, nL, tmp);
+    for (int b = 0; b < nL; ++b) chL[b] = tmp[b];
+  }
+  xcd_ranks(nL, nA, tmp);
+  for (int b = nL; b < nwg; ++b) chA[b] = tmp[b];
+  int lrp = 0;
+  for (int s = 0; s < nsys; ++s) {
+    S[s].cA0 = chunk_of(S[s].rt_begin, nA, rt);
+    S[s].cA1 = chunk_of(S[s].rt_end - 1, nA, rt);
+    S[s].cF0 = chunk_of(S[s].rt_begin, nF, rt);
+    S[s].cF1 = chunk_of(S[s].rt_end - 1, nF, rt);
+    S[s].cE0 = chunk_of(S[s].et_begin, nF, et);
+    S[s].cE1 = chunk_of(S[s].et_end - 1, nF, et);
+    if (S[s].k > 0) {
+      S[s].cL0 = chunk_of(S[s].lt_begin, nL, lt);
+      S[s].cL1 = chunk_of(S[s].lt_end - 1, nL, lt);
+      S[s].lrp_off = lrp;
+      lrp += (S[s].cL1 - S[s].cL0 + 1) * S[s].k;
+    } else {
+      S[s].cL0 = 0;
+      S[s].cL1 = -1;
+      S[s].lrp_off = lrp;
+    }
+  }
+  HIPCHK(h0->sysdesc.ensure(sizeof(hsc::SysDesc) * nsys));
+  HIPCHK(h0->sysout.ensure(sizeof(hsc::SysOut) * nsys));
+  HIPCHK(h0->chunks.ensure(sizeof(int) * 3 * nwg));
+  HIPCHK(h0->partA.ensure(sizeof(double) * (size_t)nsys * nA));
+  HIPCHK(h0->partE.ensure(sizeof(double) * 2 * (size_t)nsys * nF));
+  HIPCHK(h0->partF.ensure(sizeof(double) * (size_t)nsys * nF));
+  HIPCHK(h0->partL.ensure(sizeof(double) * (size_t)std::max(lrp, 1)));
+  HIPCHK(h0->barrier.ensure(64));
+  HIPCHK(h0->errflag.ensure(64));
+  std::vector<int> ch(3 * nwg);
+  std::copy(chA.begin(), chA.end(), ch.begin());
+  std::copy(chF.begin(), chF.end(), ch.begin() + nwg);
+  std::copy(chL.begin(), chL.end(), ch.begin() + 2 * nwg);
+  HIPCHK(hipMemcpyAsync(h0->sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(h0->chunks.p, ch.data(), sizeof(int) * 3 * nwg, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(h0->barrier.p, 0, 64, st));
+  HIPCHK(hipMemsetAsync(h0->errflag.p, 0, 64, st));
+  // p_old of the first iteration is multiplied by beta = 0: it must be finite
+  for (int s = 0; s < nsys; ++s) {
+    HIPCHK(hipMemsetAsync(hs[s]->p0.p, 0, sizeof(double) * std::max(hs[s]->fn, 1), st));
+  }
+  hsc::PcgParams P{};
+  P.sys = h0->sysdesc.as<hsc::SysDesc>();
+  P.out = h0->sysout.as<hsc::SysOut>();
+  P.nsys = nsys;
+  P.order = h0->cfg.neumann_order;
+  P.omega = h0->cfg.omega;
+  P.mode_true_every = h0->cfg.true_residual_every_iteration;
+  P.replace_every = mixed ? h0->cfg.residual_replace_every : 0;
+  P.nwg = nwg;
+  P.n_lr_wg = nL;
+  P.nA = nA;
+  P.nF = nF;
+  P.nL = std::max(nL, 1);
+  P.rt_total = rt;
+  P.et_total = et;
+  P.lt_total = lt;
+  P.rows_per_tile = R;
+  P.chunkA = h0->chunks.as<int>();
+  P.chunkF = h0->chunks.as<int>() + nwg;
+  P.chunkL = h0->chunks.as<int>() + 2 * nwg;
+  P.partA = h0->partA.as<double>();
+  P.partE = h0->partE.as<double>();
+  P.partF = h0->partF.as<double>();
+  P.partL = h0->partL.as<double>();
+  P.barrier = h0->barrier.as<unsigned>();
+  P.error = h0->errflag.as<int>();
+  P.timeout_ticks = kTimeoutTicks;
+  P.t_total = t_total;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  hipError_t le = hsc::launch_pcg(G, mixed, P, shmem, st);
+  HIPCHK(hipEventRecord(e1, st));
+  std::vector<hsc::SysOut> out(nsys);
+  int derr = 0;
+  hipError_t ce = hipSuccess;
+  if (le == hipSuccess) {
+    ce = hipMemcpyAsync(out.data(), h0->sysout.p, sizeof(hsc::SysOut) * nsys, hipMemcpyDeviceToHost, st);
+    if (ce == hipSuccess) ce = hipMemcpyAsync(&derr, h0->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (ce == hipSuccess) ce = hipStreamSynchronize(st);
+  }
+  float ms = 0.f;
+  if (le == hipSuccess && ce == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (le != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("PCG launch: ") + hipGetErrorString(le)};
+  if (ce != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("PCG: ") + hipGetErrorString(ce)};
+  if (derr) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "persistent PCG: bounded grid wait expired"};
+  if (pcg_ms) *pcg_ms = ms;
+  return out;
+}
+
+LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
+  hsc_handle* h0 = hs[0];
+  const bool mixed = h0->cfg.precision == HSC_PRECISION_MIXED;
+  if (h0->cfg.lanes_per_row && h0->cfg.wg_per_cu) return {h0->cfg.lanes_per_row, h0->cfg.wg_per_cu};
+  long long nbt = 0, nnz = 0;
+  int kmax = 0;
+  for (int s = 0; s < nsys; ++s) {
+    nbt += hs[s]->nb;
+    nnz += hs[s]->nnzb;
+    kmax = std::max(kmax, hs[s]->k);
+  }
+  TuneKey key{(int)std::log2((double)std::max(nbt, 1LL)), (int)(nnz / std::max(nbt, 1LL)), kmax / 16, mixed ? 1 : 0,
+              nsys > 1 ? 1 : 0};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      return {h0->cfg.lanes_per_row ? h0->cfg.lanes_per_row : it->second.first,
+              h0->cfg.wg_per_cu ? h0->cfg.wg_per_cu : it->second.second};
+    }
+  }
+  // time a few PCG iterations per candidate; the real solve re-initialises x, r
+  const int Gs[3] = {4, 8, 16};
+  double best = 1e300;
+  LaunchCfg bestc{8, 1};
+  for (int gi = 0; gi < 3; ++gi) {
+    if (h0->cfg.lanes_per_row && Gs[gi] != h0->cfg.lanes_per_row) continue;
+    for (int w = 1; w <= 2; ++w) {
+      if (h0->cfg.wg_per_cu && w != h0->cfg.wg_per_cu) continue;
+      LaunchCfg c{Gs[gi], w};
+      try {
+        float ms = 0.f;
+        std::vector<hsc::SysOut> o;
+        o = run_pcg(hs, nsys, c, 12, st, &ms);  // warm
+        o = run_pcg(hs, nsys, c, 12, st, &ms);
+        int it = 1;
+        for (auto& x : o) it = std::max(it, x.iterations);
+        const double per = ms / it;
+        if (per < best) {
+          best = per;
+          bestc = c;
+        }
+      } catch (const HscError&) {
+        continue;
+      }
+    }
+  }
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[key] = {bestc.G, bestc.wgpcu};
+  return bestc;
+}
+
+// ------------------------------------------------------------------ setup
+void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, int fn, hipStream_t st) {
+  auto t0 = Clock::now();
+  h->setup_ok = false;
+  if (!Av) throw HscError{HSC_ERR_INVALID_ARGUMENT, "null matrix view"};
+  const long long N = 3 * Av->block_dim;
+  // linear_hsc.cu:197-200
+  if (!(qm > 0 && fn > 0 && qm % 12 == 0 && fn % 3 == 0 && (long long)qm + fn == N && qo % 3 == 0 && fo % 3 == 0 &&
+        qo >= 0 && fo >= 0 && qo + qm <= N && fo + fn <= N && (qo + qm <= fo || fo + fn <= qo)))
+    throw HscError{HSC_ERR_INVALID_ARGUMENT, "HSC requires only ABD (12/body) and FEM (3/node) DOFs"};
+  if (Av->triplet_count < 0 || (Av->triplet_count > 0 && (!Av->row_indices || !Av->col_indices || !Av->values)))
+    throw HscError{HSC_ERR_INVALID_ARGUMENT, "invalid BCOO view"};
+  if (Av->triplet_count >= (1LL << 30)) throw HscError{HSC_ERR_CAPACITY, "too many triplets"};
+  h->N = (int)N;
+  h->qo = qo;
+  h->qm = qm;
+  h->fo = fo;
+  h->fn = fn;
+  h->nb = fn / 3;
+  h->count = Av->triplet_count;
+  const int nb = h->nb;
+  HIPCHK(h->row_count.ensure(sizeof(int) * nb));
+  HIPCHK(h->rowptr.ensure(sizeof(int) * (nb + 1)));
+  HIPCHK(h->cursor.ensure(sizeof(int) * nb));
+  HIPCHK(h->active_flag.ensure(sizeof(int) * nb));
+  HIPCHK(h->act_prefix.ensure(sizeof(int) * (nb + 1)));
+  HIPCHK(h->act_node.ensure(sizeof(int) * nb));
+  HIPCHK(h->invD64.ensure(sizeof(double) * 9 * nb));
+  HIPCHK(h->invD32.ensure(sizeof(float) * 9 * nb));
+  HIPCHK(h->A.ensure(sizeof(double) * qm * qm));
+  HIPCHK(h->ints.ensure(sizeof(int) * 32));
+  HIPCHK(hipMemsetAsync(h->row_count.p, 0, sizeof(int) * nb, st));
+  HIPCHK(hipMemsetAsync(h->cursor.p, 0, sizeof(int) * nb, st));
+  HIPCHK(hipMemsetAsync(h->active_flag.p, 0, sizeof(int) * nb, st));
+  HIPCHK(hipMemsetAsync(h->invD64.p, 0, sizeof(double) * 9 * nb, st));
+  HIPCHK(hipMemsetAsync(h->invD32.p, 0, sizeof(float) * 9 * nb, st));
+  HIPCHK(hipMemsetAsync(h->A.p, 0, sizeof(double) * qm * qm, st));
+  HIPCHK(hipMemsetAsync(h->ints.p, 0, sizeof(int) * 32, st));
+  int* err = h->ints.as<int>();
+  hsc::SplitParams sp{Av->row_indices, Av->col_indices, Av->values, Av->triplet_count, qo, qm, fo, fn,
+                      h->row_count.as<int>(), h->active_flag.as<int>(), h->invD64.as<double>(),
+                      h->invD32.as<float>(), h->A.as<double>(), err};
+  HIPCHK(hsc::launch_split(sp, st));
+  HIPCHK(hsc::launch_scan(h->row_count.as<int>(), h->rowptr.as<int>(), nb, st));
+  HIPCHK(hsc::launch_scan(h->active_flag.as<int>(), h->act_prefix.as<int>(), nb, st));
+  int herr[8], nnzb = 0, nact = 0;
+  HIPCHK(hipMemcpyAsync(herr, err, sizeof(herr), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&nnzb, h->rowptr.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&nact, h->act_prefix.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (herr[1]) throw HscError{HSC_ERR_INVALID_ARGUMENT, "HSC requires only ABD (12/body) and FEM (3/node) DOFs"};
+  if (herr[3]) throw HscError{HSC_ERR_UNEXPECTED_COUPLING, "Unexpected HSC subsystem coupling"};
+  if (herr[4]) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: non-finite Hessian entries"};
+  if (herr[0])
+    throw HscError{HSC_ERR_FEM_DIAGONAL, "HSC: " + std::to_string(herr[0]) + " non-positive FEM diagonal blocks"};
+  h->blockdiag = herr[2] == 0;
+  h->nnzb = nnzb;
+  h->a = 3 * nact;
+  const int a = h->a;
+  const int limit = std::min(std::min(h->cfg.max_rank, qm), a);
+  h->limit = limit;
+  const int lim1 = std::max(limit, 1);
+  HIPCHK(h->act_dof.ensure(sizeof(int) * std::max(a, 1)));
+  HIPCHK(h->key_col.ensure(sizeof(int) * nnzb));
+  HIPCHK(h->key_src.ensure(sizeof(int) * nnzb));
+  HIPCHK(h->colidx.ensure(sizeof(int) * nnzb));
+  HIPCHK(h->val64.ensure(sizeof(double) * 9 * (size_t)nnzb));
+  HIPCHK(h->val32.ensure(sizeof(float) * 9 * (size_t)nnzb));
+  HIPCHK(h->E.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
+  HIPCHK(h->R0.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
+  HIPCHK(h->R1.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
+  HIPCHK(h->U.ensure(sizeof(double) * (size_t)qm * lim1));
+  HIPCHK(h->V.ensure(sizeof(double) * (size_t)std::max(a, 1) * lim1));
+  HIPCHK(h->hist.ensure(sizeof(double) * (lim1 + 1)));
+  HIPCHK(h->pivots.ensure(sizeof(int) * 2 * lim1));
+  HIPCHK(h->Binv.ensure(sizeof(double) * 144 * (qm / 12)));
+  HIPCHK(h->P.ensure(sizeof(double) * qm * qm));
+  HIPCHK(h->s1.ensure(sizeof(double) * qm * qm));
+  HIPCHK(h->s2.ensure(sizeof(double) * qm * qm));
+  HIPCHK(h->s3.ensure(sizeof(double) * qm * qm));
+  HIPCHK(h->W1.ensure(sizeof(double) * (size_t)qm * lim1));
+  HIPCHK(h->C.ensure(sizeof(double) * (size_t)lim1 * lim1));
+  HIPCHK(h->Lm.ensure(sizeof(double) * (size_t)lim1 * lim1));
+  HIPCHK(h->W64.ensure(sizeof(double) * (size_t)std::max(a, 1) * lim1));
+  HIPCHK(h->W32.ensure(sizeof(float) * (size_t)std::max(a, 1) * lim1));
+  for (Buf* b : {&h->rhs, &h->r, &h->p0, &h->p1, &h->z0, &h->z1, &h->q, &h->q64, &h->yfull})
+    HIPCHK(b->ensure(sizeof(double) * fn));
+  HIPCHK(h->small.ensure(sizeof(double) * (qm + 2 * lim1 + 64)));
+  HIPCHK(h->ybuf.ensure(sizeof(double) * lim1));
+  HIPCHK(h->misc.ensure(sizeof(double) * 64));
+  HIPCHK(h->respart.ensure(sizeof(double) * 3 * ((fn + hsc::kBlock - 1) / hsc::kBlock + 1)));
+  if (a > 0) HIPCHK(hipMemsetAsync(h->E.p, 0, sizeof(double) * (size_t)qm * a, st));
+  HIPCHK(hsc::launch_active(h->active_flag.as<int>(), h->act_prefix.as<int>(), nb, h->act_node.as<int>(),
+                            h->act_dof.as<int>(), st));
+  hsc::PlaceParams pp{Av->row_indices, Av->col_indices, Av->values, Av->triplet_count, qo, qm, fo, fn,
+                      h->rowptr.as<int>(), h->cursor.as<int>(), h->key_col.as<int>(), h->key_src.as<int>(),
+                      h->act_node.as<int>(), a, h->E.as<double>()};
+  HIPCHK(hsc::launch_place(pp, st));
+  HIPCHK(hsc::launch_sort_gather(h->rowptr.as<int>(), nb, h->key_col.as<int>(), h->key_src.as<int>(), Av->values,
+                                 nnzb, h->colidx.as<int>(), h->val64.as<double>(), h->val32.as<float>(), st));
+  // ---- ACA
+  int* rank_dev = err + 8;
+  double* dmisc = h->misc.as<double>();
+  HIPCHK(hipMemsetAsync(h->hist.p, 0, sizeof(double) * (lim1 + 1), st));
+  if (limit > 0) {
+    int occ = 0;
+    HIPCHK(hsc::aca_occupancy(&occ));
+    const DeviceInfo di = device_info(h->device);
+    const long long total = (long long)qm * a;
+    int nwg = (int)std::min<long long>((long long)std::max(occ, 1) * di.cu, std::max<long long>(1, (total + 1023) / 1024));
+    nwg = std::max(1, std::min(nwg, std::max(occ, 1) * di.cu));
+    HIPCHK(h->acapart.ensure(sizeof(double) * 4 * nwg));
+    HIPCHK(h->barrier.ensure(64));
+    HIPCHK(h->errflag.ensure(64));
+    HIPCHK(hipMemsetAsync(h->barrier.p, 0, 64, st));
+    HIPCHK(hipMemsetAsync(h->errflag.p, 0, 64, st));
+    hsc::AcaParams ap{};
+    ap.E = h->E.as<double>();
+    ap.R0 = h->R0.as<double>();
+    ap.R1 = h->R1.as<double>();
+    ap.qm = qm;
+    ap.a = a;
+    ap.limit = limit;
+    ap.stop = h->cfg.aca_rtol;
+    ap.U = h->U.as<double>();
+    ap.V = h->V.as<double>();
+    ap.hist = h->hist.as<double>();
+    ap.pivots = h->pivots.as<int>();
+    ap.part = h->acapart.as<double>();
+    ap.rank_out = rank_dev;
+    ap.scale_out = dmisc + 10;
+    ap.barrier = h->barrier.as<unsigned>();
+    ap.error = h->errflag.as<int>();
+    ap.timeout_ticks = kTimeoutTicks;
+    ap.nwg = nwg;
+    HIPCHK(hsc::launch_aca(ap, st));
+  }
+  // ---- ABD inverse and P
+  HIPCHK(hsc::launch_body_inverse(h->A.as<double>(), qm, h->Binv.as<double>(), err + 9, st));
+  hsc::AbdParams abp{h->A.as<double>(), h->Binv.as<double>(), h->P.as<double>(), h->s1.as<double>(),
+                     h->s2.as<double>(), h->s3.as<double>(), qm, h->cfg.abd_inverse, h->blockdiag ? 1 : 0,
+                     h->cfg.neumann_order, h->cfg.omega, h->cfg.abd_inverse_tol, dmisc};
+  if (h->cfg.abd_inverse == HSC_ABD_NEUMANN && h->cfg.adaptive_abd_inverse) abp.mode = 2;
+  HIPCHK(hsc::launch_abd_inverse(abp, st));
+  int ints2[2] = {0, 0};
+  double dm[4] = {0, 0, 0, 0};
+  int aca_err = 0;
+  HIPCHK(hipMemcpyAsync(ints2, err + 8, sizeof(ints2), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(dm, dmisc, sizeof(dm), hipMemcpyDeviceToHost, st));
+  if (limit > 0) HIPCHK(hipMemcpyAsync(&aca_err, h->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (aca_err) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "ACA: bounded grid wait expired"};
+  const int rank = limit > 0 ? ints2[0] : 0;
+  if (ints2[1]) throw HscError{HSC_ERR_ABD_NOT_SPD, "HSC: non-SPD ABD block"};
+  const int abd_status = (int)dm[2];
+  h->abd_res = dm[0];
+  h->doublings = (int)dm[1];
+  if (abd_status == 4) throw HscError{HSC_ERR_ABD_NOT_SPD, "HSC: non-SPD ABD block"};
+  if (abd_status == 5)
+    throw HscError{HSC_ERR_ABD_INVERSE, "HSC ABD inverse residual exceeds tolerance after " +
+                                            std::to_string(h->doublings) + " doublings"};
+  if (abd_status == 6) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite Neumann/Schur factors"};
+  double hv = 0.0;
+  HIPCHK(hipMemcpy(&hv, h->hist.as<double>() + rank, sizeof(double), hipMemcpyDeviceToHost));
+  h->aca_err = (limit > 0) ? hv : 0.0;
+  h->k = rank;
+  if (rank > 0) {
+    hsc::FactorParams fp{h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), qm, a, rank,
+                         h->W1.as<double>(), h->C.as<double>(), h->Lm.as<double>(), dmisc + 4};
+    HIPCHK(hsc::launch_coupling_core(fp, st));
+    HIPCHK(hsc::launch_make_W(h->V.as<double>(), h->Lm.as<double>(), a, rank, h->W64.as<double>(),
+                              h->W32.as<float>(), st));
+    double fst = 0.0;
+    HIPCHK(hipMemcpyAsync(&fst, dmisc + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
+  }
+  h->setup_ok = true;
+  h->setup_ms = ms_since(t0);
+}
+
+void fill_common_stats(hsc_handle* h, hsc_stats* s) {
+  std::memset(s, 0, sizeof(*s));
+  s->fem_dofs = h->fn;
+  s->abd_dofs = h->qm;
+  s->active_fem_dofs = h->a;
+  s->rank = h->k;
+  s->aca_relative_error = h->aca_err;
+  s->adaptive_abd_inverse = h->cfg.adaptive_abd_inverse;
+  s->neumann_doublings = h->doublings;
+  s->abd_inverse_residual = h->abd_res;
+  s->full_residual_limit = h->cfg.full_residual_limit;
+  s->precision = h->cfg.precision;
+  s->preconditioner = h->cfg.preconditioner;
+  s->setup_ms = h->setup_ms;
+}
+
+// back-substitution + descent + full residual (linear_hsc.cu:395-435)
+int finish_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStream_t st) {
+  HIPCHK(hsc::launch_backsub(h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), h->act_dof.as<int>(), b, x,
+                             h->qo, h->qm, h->fo, h->a, h->k, h->small.as<double>(), st));
+  hsc::SysDesc S = make_desc(h);
+  HIPCHK(hsc::launch_op_Dx64(8, S, x + h->fo, h->yfull.as<double>(), st));
+  HIPCHK(hsc::launch_full_residual(h->yfull.as<double>(), h->E.as<double>(), h->A.as<double>(),
+                                   h->act_node.as<int>(), h->act_dof.as<int>(), h->a, h->qm, b, x, h->fo, h->qo,
+                                   h->fn, h->respart.as<double>(), h->misc.as<double>() + 20, st));
+  double out[3];
+  HIPCHK(hipMemcpyAsync(out, h->misc.as<double>() + 20, sizeof(out), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const double bn = std::sqrt(out[1]);
+  s->descent = out[2];
+  s->full_relative_residual = bn > 0 ? std::sqrt(out[0]) / bn : 0.0;
+  s->accepted = std::isfinite(s->full_relative_residual) && s->full_relative_residual <= h->cfg.full_residual_limit;
+  if (h->cfg.check_acceptance) {
+    if (!(std::isfinite(out[2]) && (out[2] > 0 || bn == 0))) {
+      h->err = "HSC returned a non-descent direction";
+      return HSC_ERR_NOT_DESCENT;
+    }
+    if (!s->accepted) {
+      char m[256];
+      std::snprintf(m, sizeof m, "HSC full-system residual %g exceeds %g despite Schur residual %g. No fallback.",
+                    s->full_relative_residual, h->cfg.full_residual_limit, s->schur_relative_residual);
+      h->err = m;
+      return HSC_ERR_FULL_RESIDUAL;
+    }
+  }
+  return HSC_OK;
+}
+
+int status_from_out(hsc_handle* h, const hsc::SysOut& o) {
+  char m[256];
+  if (o.status == HSC_ERR_BREAKDOWN) {
+    std::snprintf(m, sizeof m, "HSC Schur PCG breakdown: iteration=%d. No fallback.", o.iterations);
+    h->err = m;
+  } else if (o.status == HSC_ERR_NOT_CONVERGED) {
+    std::snprintf(m, sizeof m, "HSC Schur PCG did not converge: iterations=%d, residual=%g", o.iterations, o.rel);
+    h->err = m;
+  } else if (o.status != 0) {
+    h->err = "persistent PCG: device wait expired";
+  }
+  return o.status;
+}
+
+int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStream_t st, Clock::time_point t0) {
+  if (!h->setup_ok) {
+    h->err = "hsc_solve before a successful hsc_setup";
+    return HSC_ERR_NOT_SETUP;
+  }
+  if (!b || !x) throw HscError{HSC_ERR_INVALID_ARGUMENT, "null b or x"};
+  hsc_stats tmp;
+  if (!s) s = &tmp;
+  fill_common_stats(h, s);
+  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
+                         h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
+                         h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
+  hsc_handle* hs[1] = {h};
+  LaunchCfg lc = tuned_config(hs, 1, st);
+  h->G = lc.G;
+  h->wgpcu = lc.wgpcu;
+  // tuning runs overwrite r: re-derive it
+  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
+                         h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
+                         h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
+  float pms = 0.f;
+  auto out = run_pcg(hs, 1, lc, 0, st, &pms);
+  s->pcg_ms = pms;
+  s->pcg_iterations = out[0].iterations;
+  s->schur_relative_residual = out[0].rel;
+  s->true_residual_checks = out[0].checks;
+  s->restarts = out[0].restarts;
+  int code = status_from_out(h, out[0]);
+  int c2 = finish_solve(h, b, x, s, st);
+  if (code == HSC_OK) code = c2;
+  s->status = code;
+  s->total_ms = ms_since(t0) + (s->setup_ms > 0 ? 0.0 : 0.0);
+  return code;
+}
+
+template <class F>
+int guarded(hsc_handle* h, F&& f) {
+  try {
+    return f();
+  } catch (const HscError& e) {
+    if (h) h->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    if (h) h->err = e.what();
+    return HSC_ERR_HIP;
+  }
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int32_t hsc_abi_version(void) { return HSC_ABI_VERSION; }
+
+void hsc_config_default(hsc_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->max_rank = 128;
+  c->neumann_order = 5;
+  c->max_iter = 2000;
+  c->adaptive_abd_inverse = 1;
+  c->abd_inverse_tol = 1e-5;
+  c->full_residual_limit = 0.1;
+  c->omega = 0.7;
+  c->tol_rate = 1e-3;
+  c->abd_inverse = HSC_ABD_EXACT;
+  c->precision = HSC_PRECISION_FP64;
+  c->residual_replace_every = 25;
+  c->true_residual_every_iteration = 0;
+  c->aca_rtol = 0.0;
+  c->lanes_per_row = 0;
+  c->wg_per_cu = 0;
+  c->preconditioner = HSC_PRECOND_NEUMANN;
+  c->check_acceptance = 1;
+}
+
+int32_t hsc_create(const hsc_config* cfg, int32_t device, hsc_handle** out) {
+  if (!out) return HSC_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  hsc_config c;
+  if (cfg) c = *cfg;
+  else hsc_config_default(&c);
+  std::string why;
+  int v = validate_config(c, why);
+  if (v != HSC_OK) return v;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return HSC_ERR_HIP;
+  hsc_handle* h = new hsc_handle();
+  h->cfg = c;
+  h->device = device;
+  *out = h;
+  return HSC_OK;
+}
+
+void hsc_destroy(hsc_handle* h) {
+  if (!h) return;
+  DeviceGuard g(h->device);
+  delete h;
+}
+
+const char* hsc_last_error(const hsc_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int32_t hsc_set_config(hsc_handle* h, const hsc_config* cfg) {
+  if (!h || !cfg) return HSC_ERR_INVALID_ARGUMENT;
+  std::string why;
+  int v = validate_config(*cfg, why);
+  if (v != HSC_OK) {
+    h->err = why;
+    return v;
+  }
+  const bool refactor = cfg->max_rank != h->cfg.max_rank || cfg->abd_inverse != h->cfg.abd_inverse ||
+                        cfg->neumann_order != h->cfg.neumann_order || cfg->omega != h->cfg.omega ||
+                        cfg->aca_rtol != h->cfg.aca_rtol || cfg->adaptive_abd_inverse != h->cfg.adaptive_abd_inverse ||
+                        cfg->abd_inverse_tol != h->cfg.abd_inverse_tol;
+  h->cfg = *cfg;
+  if (refactor) h->setup_ok = false;
+  return HSC_OK;
+}
+
+int32_t hsc_setup(hsc_handle* h, const hsc_bcoo3_view* A, int32_t qo, int32_t qm, int32_t fo, int32_t fn,
+                  hsc_stream stream) {
+  if (!h) return HSC_ERR_INVALID_ARGUMENT;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    do_setup(h, A, qo, qm, fo, fn, (hipStream_t)stream);
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_solve(hsc_handle* h, const double* b, double* x, hsc_stats* stats, hsc_stream stream) {
+  if (!h) return HSC_ERR_INVALID_ARGUMENT;
+  DeviceGuard g(h->device);
+  auto t0 = Clock::now();
+  int code = guarded(h, [&] { return solve_impl(h, b, x, stats, (hipStream_t)stream, t0); });
+  if (stats) {
+    stats->status = code;
+    stats->setup_ms = 0.0;
+    stats->total_ms = ms_since(t0);
+  }
+  return code;
+}
+
+int32_t hsc_do_solve(hsc_handle* h, const hsc_bcoo3_view* A, int32_t qo, int32_t qm, int32_t fo, int32_t fn,
+                     const double* b, double* x, hsc_stats* stats, hsc_stream stream) {
+  if (!h) return HSC_ERR_INVALID_ARGUMENT;
+  DeviceGuard g(h->device);
+  auto t0 = Clock::now();
+  int code = guarded(h, [&] {
+    do_setup(h, A, qo, qm, fo, fn, (hipStream_t)stream);
+    return HSC_OK;
+  });
+  if (code == HSC_OK) code = guarded(h, [&] { return solve_impl(h, b, x, stats, (hipStream_t)stream, t0); });
+  if (stats) {
+    if (code != HSC_OK && !h->setup_ok) fill_common_stats(h, stats);
+    stats->status = code;
+    stats->setup_ms = h->setup_ok ? h->setup_ms : ms_since(t0);
+    stats->total_ms = ms_since(t0);
+  }
+  return code;
+}
+
+int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count, const double* const* b, double* const* x,
+                          hsc_stats* stats, hsc_stream stream) {
+  if (!handles || count <= 0 || !b || !x) return HSC_ERR_INVALID_ARGUMENT;
+  if (count > hsc::kMaxSys) return HSC_ERR_CAPACITY;
+  hsc_handle* h0 = handles[0];
+  if (!h0) return HSC_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < count; ++i) {
+    if (!handles[i] || !handles[i]->setup_ok) return HSC_ERR_NOT_SETUP;
+    if (handles[i]->device != h0->device || handles[i]->cfg.precision != h0->cfg.precision ||
+        handles[i]->cfg.neumann_order != h0->cfg.neumann_order || handles[i]->cfg.omega != h0->cfg.omega)
+      return HSC_ERR_INVALID_ARGUMENT;
+  }
+  DeviceGuard g(h0->device);
+  hipStream_t st = (hipStream_t)stream;
+  auto t0 = Clock::now();
+  return guarded(h0, [&] {
+    for (int i = 0; i < count; ++i) {
+      hsc_handle* h = handles[i];
+      HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b[i], h->qo, h->qm, h->fo,
+                             h->fn, h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
+                             h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x[i], st));
+    }
+    LaunchCfg lc = tuned_config(handles, count, st);
+    for (int i = 0; i < count; ++i) {
+      hsc_handle* h = handles[i];
+      HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b[i], h->qo, h->qm, h->fo,
+                             h->fn, h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
+                             h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x[i], st));
+    }
+    float pms = 0.f;
+    auto out = run_pcg(handles, count, lc, 0, st, &pms);
+    int first = HSC_OK;
+    for (int i = 0; i < count; ++i) {
+      hsc_handle* h = handles[i];
+      hsc_stats tmp;
+      hsc_stats* s = stats ? &stats[i] : &tmp;
+      fill_common_stats(h, s);
+      s->setup_ms = 0.0;
+      s->pcg_ms = pms;
+      s->pcg_iterations = out[i].iterations;
+      s->schur_relative_residual = out[i].rel;
+      s->true_residual_checks = out[i].checks;
+      s->restarts = out[i].restarts;
+      int code = status_from_out(h, out[i]);
+      int c2 = finish_solve(h, b[i], x[i], s, st);
+      if (code == HSC_OK) code = c2;
+      s->status = code;
+      s->total_ms = ms_since(t0);
+      if (first == HSC_OK) first = code;
+    }
+    return first;
+  });
+}
+
+int32_t hsc_append_jsonl(const hsc_stats* s, const char* path, int32_t frame, int32_t newton) {
+  if (!s || !path) return HSC_ERR_INVALID_ARGUMENT;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return HSC_ERR_INVALID_ARGUMENT;
+  // field order of linear_hsc.cu:421-431; the contact counters are not known
+  // at this boundary and are written as 0
+  std::fprintf(f,
+               "{\"frame\":%d,\"newton\":%d,\"fem_dofs\":%d,\"abd_dofs\":%d,\"active_fem_dofs\":%d,\"rank\":%d,"
+               "\"aca_relative_error\":%.17g,\"pcg_iterations\":%d,\"schur_relative_residual\":%.17g,"
+               "\"full_relative_residual\":%.17g,\"setup_ms\":%.6g,\"total_ms\":%.6g,\"active_mesh_contacts\":0,"
+               "\"active_ground_contacts\":0,\"adaptive_abd_inverse\":%s,\"neumann_doublings\":%d,"
+               "\"abd_inverse_residual\":%.17g,\"full_residual_limit\":%.17g,\"accepted\":%s,"
+               "\"solver\":\"linear_hsc\",\"fallback\":false,\"precision\":\"%s\",\"preconditioner\":\"%s\","
+               "\"learned_fallback\":%s,\"status\":%d,\"pcg_ms\":%.6g}\n",
+               frame, newton, s->fem_dofs, s->abd_dofs, s->active_fem_dofs, s->rank, s->aca_relative_error,
+               s->pcg_iterations, s->schur_relative_residual, s->full_relative_residual, s->setup_ms, s->total_ms,
+               s->adaptive_abd_inverse ? "true" : "false", s->neumann_doublings, s->abd_inverse_residual,
+               s->full_residual_limit, s->accepted ? "true" : "false", s->precision ? "mixed" : "fp64",
+               s->preconditioner ? "learned" : "neumann", s->fallback ? "true" : "false", s->status, s->pcg_ms);
+  std::fclose(f);
+  return HSC_OK;
+}
+
+int32_t hsc_apply_D(hsc_handle* h, const double* x, double* y, hsc_stream stream) {
+  if (!h || !x || !y) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    hsc::SysDesc S = make_desc(h);
+    HIPCHK(hsc::launch_op_Dx64(h->G, S, x, y, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_apply_schur(hsc_handle* h, const double* x, double* y, hsc_stream stream) {
+  if (!h || !x || !y) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    hsc::SysDesc S = make_desc(h);
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(hsc::launch_op_Dx64(h->G, S, x, y, st));
+    HIPCHK(hsc::launch_op_lowrank64(S, x, y, h->ybuf.as<double>(), st));
+    HIPCHK(hipStreamSynchronize(st));
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_apply_precond(hsc_handle* h, const double* r, double* z, hsc_stream stream) {
+  if (!h || !r || !z) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    hsc::SysDesc S = make_desc(h);
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(hsc::launch_op_precond64(h->G, S, h->cfg.neumann_order, h->cfg.omega, r, z, h->z1.as<double>(), st));
+    HIPCHK(hipStreamSynchronize(st));
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
+  if (!h || !o) return HSC_ERR_INVALID_ARGUMENT;
+  std::memset(o, 0, sizeof(*o));
+  o->fem_dofs = h->fn;
+  o->abd_dofs = h->qm;
+  o->fem_nodes = h->nb;
+  o->nnzb = h->nnzb;
+  o->active_fem_dofs = h->a;
+  o->rank = h->k;
+  o->lanes_per_row = h->G;
+  o->workgroups = h->wgpcu;
+  o->neumann_doublings = h->doublings;
+  o->abd_block_diagonal = h->blockdiag ? 1 : 0;
+  o->aca_relative_error = h->aca_err;
+  o->abd_inverse_residual = h->abd_res;
+  return h->setup_ok ? HSC_OK : HSC_ERR_NOT_SETUP;
+}
+
+int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* P, int32_t* active,
+                        double* aca_history) {
+  if (!h) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    if (U && h->k) HIPCHK(hipMemcpy(U, h->U.p, sizeof(double) * h->qm * h->k, hipMemcpyDeviceToHost));
+    if (V && h->k) HIPCHK(hipMemcpy(V, h->V.p, sizeof(double) * (size_t)h->a * h->k, hipMemcpyDeviceToHost));
+    if (C && h->k) HIPCHK(hipMemcpy(C, h->C.p, sizeof(double) * h->k * h->k, hipMemcpyDeviceToHost));
+    if (P) HIPCHK(hipMemcpy(P, h->P.p, sizeof(double) * h->qm * h->qm, hipMemcpyDeviceToHost));
+    if (active && h->a) HIPCHK(hipMemcpy(active, h->act_dof.p, sizeof(int) * h->a, hipMemcpyDeviceToHost));
+    if (aca_history) HIPCHK(hipMemcpy(aca_history, h->hist.p, sizeof(double) * (h->k + 1), hipMemcpyDeviceToHost));
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_set_learned_weights(hsc_handle* h, const void* w, int64_t bytes) {
+  if (!h || (!w && bytes)) return HSC_ERR_INVALID_ARGUMENT;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    HIPCHK(h->learned.ensure((size_t)bytes));
+    if (bytes) HIPCHK(hipMemcpy(h->learned.p, w, (size_t)bytes, hipMemcpyDeviceToDevice));
+    h->learned_bytes = bytes;
+    return HSC_OK;
+  });
+}
+
+}  // extern "C"

@@ -107,7 +107,7 @@ class SparseHSC:
         self.factors = hsc_core.aca(E, rtol=aca_rtol, max_rank=aca_max_rank)
         self.U, self.Vc = self.factors.U, self.factors.V
         self.cho = sla.cho_factor(A)
-        self.precond = SparseNeumann(D, order, omega)
+        self.precond_op = SparseNeumann(D, order, omega)
         C = self.U.T @ self.Ainv(self.U)
         self.C = 0.5 * (C + C.T)
 
@@ -116,6 +116,9 @@ class SparseHSC:
         D, A, E, active = split_bcoo(sysm.rows, sysm.cols, sysm.vals, sysm.qo, sysm.qm,
                                      sysm.fo, sysm.fn)
         return cls(D, A, E, active, **kw)
+
+    def precond(self, r):
+        return self.precond_op(r)
 
     def Ainv(self, v):
         return sla.cho_solve(self.cho, v)
@@ -140,10 +143,10 @@ class SparseHSC:
     def solve(self, bx, bq, rtol=1e-8, maxiter=20000, dropin=True):
         rhs = self.reduced_rhs(bx, bq)
         if dropin:
-            dx, it, rel = hsc_core.pcg_dropin(self.matvec, rhs, self.precond, rtol, maxiter)
+            dx, it, rel = hsc_core.pcg_dropin(self.matvec, rhs, self.precond_op, rtol, maxiter)
             stats = {"iterations": it, "relative_residual": rel, "converged": rel <= rtol}
         else:
-            dx, stats = hsc_core.pcg(self.matvec, rhs, self.precond, rtol, maxiter)
+            dx, stats = hsc_core.pcg(self.matvec, rhs, self.precond_op, rtol, maxiter)
         return dx, self.back_substitute(bq, dx), stats
 
 

@@ -1,0 +1,76 @@
+"""C ABI: the HIP library loads, exports every function include/hsc.h
+declares, keeps the reference's config defaults and writes the reference's
+hsc_stats.jsonl schema (host-only calls; no GPU needed)."""
+import ctypes as C
+import json
+import re
+
+import pytest
+
+from hsc_reimpl_amd import _abi
+from tests.conftest import ROOT
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not _abi.LIB_PATH.exists():
+        from hsc_reimpl_amd import build
+        build.build()
+    return _abi.load()
+
+
+def declared_functions():
+    text = (ROOT / "include" / "hsc.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hsc_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _abi.SIGNATURES, n
+
+
+def test_abi_version_and_defaults(lib):
+    assert lib.hsc_abi_version() == 1
+    c = _abi.Config()
+    lib.hsc_config_default(C.byref(c))
+    # integration.patch:34-55 and scene_default_config.cpp:146-151 of the reference
+    assert (c.max_rank, c.neumann_order, c.max_iter, c.adaptive_abd_inverse) == (128, 5, 2000, 1)
+    assert (c.abd_inverse_tol, c.full_residual_limit, c.omega, c.tol_rate) == (1e-5, 0.1, 0.7, 1e-3)
+    assert c.residual_replace_every == 25 and c.check_acceptance == 1
+
+
+def test_invalid_config_rejected(lib):
+    c = _abi.Config()
+    lib.hsc_config_default(C.byref(c))
+    c.omega = 1.5
+    h = C.c_void_p()
+    assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
+    lib.hsc_config_default(C.byref(c))
+    c.neumann_order = 0
+    assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
+    assert lib.hsc_solve(None, None, None, None, None) == _abi.HSC_ERR_INVALID_ARGUMENT
+
+
+def test_jsonl_row_matches_reference_schema(lib, tmp_path):
+    s = _abi.Stats()
+    s.fem_dofs, s.abd_dofs, s.active_fem_dofs, s.rank = 74508, 144, 819, 84
+    s.pcg_iterations, s.schur_relative_residual, s.full_relative_residual = 499, 9.8e-5, 1.4e-7
+    s.accepted, s.adaptive_abd_inverse, s.full_residual_limit = 1, 1, 0.1
+    p = tmp_path / "hsc_stats.jsonl"
+    assert lib.hsc_append_jsonl(C.byref(s), str(p).encode(), 1, 0) == 0
+    assert lib.hsc_append_jsonl(C.byref(s), str(p).encode(), 1, 1) == 0
+    rows = [json.loads(line) for line in p.read_text().splitlines()]
+    # field list of linear_hsc.cu:421-431 and the consumer storage.py:48-61 of the reference
+    ref_keys = ["frame", "newton", "fem_dofs", "abd_dofs", "active_fem_dofs", "rank", "aca_relative_error",
+                "pcg_iterations", "schur_relative_residual", "full_relative_residual", "setup_ms", "total_ms",
+                "active_mesh_contacts", "active_ground_contacts", "adaptive_abd_inverse", "neumann_doublings",
+                "abd_inverse_residual", "full_residual_limit", "accepted", "solver", "fallback"]
+    assert list(rows[0].keys())[:len(ref_keys)] == ref_keys
+    assert rows[1]["newton"] == 1 and rows[0]["solver"] == "linear_hsc" and rows[0]["fallback"] is False
+    assert rows[0]["accepted"] is True and rows[0]["adaptive_abd_inverse"] is True
+    # the consumer's acceptance rule
+    assert not (rows[0]["solver"] != "linear_hsc" or rows[0]["fallback"] or not rows[0]["accepted"])
