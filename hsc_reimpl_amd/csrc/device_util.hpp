@@ -57,12 +57,15 @@ __device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned& epoch, unsign
   epoch += 1;
   if (threadIdx.x == 0) {
     const unsigned target = epoch * nwg;
+    // release: one agent-scope fence before arriving; the spin itself uses
+    // relaxed loads (an acquire load per spin would invalidate the L2 each
+    // time) and a single agent-scope fence after the wait is the acquire.
     __threadfence();
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const long long t0 = rt_ticks();
     int ok = 1;
     for (;;) {
-      unsigned v = __hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      unsigned v = __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if ((int)(v - target) >= 0) break;
       if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
       if (rt_ticks() - t0 > timeout) {

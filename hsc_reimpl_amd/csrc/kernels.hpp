@@ -91,4 +91,12 @@ hipError_t launch_op_precond64(int G, const SysDesc& S, int order, double omega,
                                double* tmp, hipStream_t st);
 hipError_t launch_op_lowrank64(const SysDesc& S, const double* x, double* y, double* t, hipStream_t st);
 
+hipError_t launch_dot(const double* a, const double* b, int n, double* part, double* out, hipStream_t st);
+hipError_t launch_update_xr(double* x, double* r, const double* p, const double* Ap, double alpha, int n,
+                            hipStream_t st);
+hipError_t launch_axpby(double* y, const double* x, double a, double b, int n, hipStream_t st);
+hipError_t launch_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr, hipStream_t st);
+hipError_t launch_node_features(const int* rowptr, const int* colidx, const double* val64, int nnzb, int nb,
+                                float* Dfeat, float* sdiag, hipStream_t st);
+
 }  // namespace hsc

@@ -1,0 +1,292 @@
+// BF16 graph-MLP kernels of the learned preconditioner (gfx950 MFMA).
+//
+// Activations and weights are bfloat16, every product sum is FP32
+// (v_mfma_f32_32x32x8_bf16).  Rows are nodes of a batch of samples
+// (row = sample*nb + node).  Network, loss and gradients follow the NumPy
+// statement in oracle/learned.py:
+//   h0 = relu(x W0 + b0); h1 = relu(h0 Ws1 + agg(h0) Wn1 + b1);
+//   h2 = relu(h1 Ws2 + agg(h1) Wn2 + b2); o = h2 Wo + bo
+// agg = mean over graph neighbours (self excluded).
+#include "common.hpp"
+#include "device_util.hpp"
+#include "kernels.hpp"
+#include "learned.hpp"
+
+namespace hsc {
+
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ float bf2f(bf16_t v) { return __uint_as_float(((unsigned)v) << 16); }
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+
+// Y[M x ldy] = act( sum_s A_s[M x K_s] * B_s + bias ), B_s given as Bt_s[N x K_s]
+// (row n holds column n of B).  One wave computes 32 rows x N (N/32 MFMA
+// tiles); lane l supplies A[row0 + l%32][k0 + 4*(l/32) + 0..3] and
+// B[k0 + 4*(l/32) + 0..3][n0 + l%32]; accumulator register r of lane l is
+// D[8*(r/4) + 4*(l/32) + r%4][n0 + l%32].
+template <int NT>
+__global__ __launch_bounds__(256) void k_mlp_gemm(MlpGemm g) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int row0 = (blockIdx.x * 4 + wave) * 32;
+  if (row0 >= g.M) return;
+  f32x16 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+    for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+  const int arow = row0 + (l & 31);
+  const int kof = 4 * (l >> 5);
+  for (int s = 0; s < g.nsrc; ++s) {
+    const int K = g.K[s];
+    const bf16_t* A = g.A[s];
+    const bf16_t* Bt = g.Bt[s];
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      bf16x4 a;
+      if (arow < g.M) {
+        a = *reinterpret_cast<const bf16x4*>(A + (size_t)arow * K + k0 + kof);
+      } else {
+        a[0] = a[1] = a[2] = a[3] = 0;
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int col = t * 32 + (l & 31);
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(Bt + (size_t)col * K + k0 + kof);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int col = t * 32 + (l & 31);
+    if (col >= g.ldy) continue;
+    const float bias = g.bias ? g.bias[col] : 0.f;
+    for (int r = 0; r < 16; ++r) {
+      const int row = row0 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+      if (row >= g.M) continue;
+      float v = acc[t][r] + bias;
+      if (g.relu) v = fmaxf(v, 0.f);
+      if (g.Y) g.Y[(size_t)row * g.ldy + col] = f2bf(v);
+      if (g.Yf) g.Yf[(size_t)row * g.ldy + col] = v;
+    }
+  }
+}
+
+// Partial weight gradient G[K x N] = sum_{m in chunk} A[m][k] dY[m][n] (one
+// 32x32 tile per wave, 4 waves = 64x64 tiles), partial[chunk][K*N].
+__global__ __launch_bounds__(256) void k_mlp_wgrad(MlpWgrad g) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int tk = wave >> 1, tn = wave & 1;
+  const int k0 = tk * 32, n0 = tn * 32;
+  const int m_begin = blockIdx.x * g.rows_per_chunk;
+  const int m_end = min(g.M, m_begin + g.rows_per_chunk);
+  f32x16 acc;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  const bool kin = (k0 < g.K), nin = (n0 < g.N);
+  if (kin && nin) {
+    const int kk = k0 + (l & 31), nn = n0 + (l & 31);
+    for (int m0 = m_begin; m0 < m_end; m0 += 8) {
+      bf16x4 a, b;
+      for (int j = 0; j < 4; ++j) {
+        const int m = m0 + 4 * (l >> 5) + j;
+        const bool ok = m < m_end;
+        a[j] = (ok && kk < g.K) ? (short)g.A[(size_t)m * g.lda + kk] : (short)0;
+        b[j] = (ok && nn < g.N) ? (short)g.dY[(size_t)m * g.ldd + nn] : (short)0;
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+  }
+  float* P = g.partial + (size_t)blockIdx.x * g.K * g.N;
+  if (kin && nin) {
+    const int col = n0 + (l & 31);
+    for (int r = 0; r < 16; ++r) {
+      const int row = k0 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+      if (row < g.K && col < g.N) P[(size_t)row * g.N + col] = acc[r];
+    }
+  }
+}
+
+// column sums of dY (bias gradient): partial[chunk][N]
+__global__ __launch_bounds__(256) void k_mlp_colsum(const bf16_t* dY, int M, int N, int ldd, int rows_per_chunk,
+                                                    float* partial) {
+  const int n = threadIdx.x;
+  if (n >= N) return;
+  const int m_begin = blockIdx.x * rows_per_chunk, m_end = min(M, m_begin + rows_per_chunk);
+  float s = 0.f;
+  for (int m = m_begin; m < m_end; ++m) s += bf2f(dY[(size_t)m * ldd + n]);
+  partial[(size_t)blockIdx.x * N + n] = s;
+}
+
+// out[e] = sum_c partial[c][e] (fixed order), e < count
+__global__ void k_mlp_reduce(const float* partial, int nchunks, int count, float* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunks; ++c) s += partial[(size_t)c * count + e];
+  out[e] = s;
+}
+
+// Y[b,i,:] = mean_{j in N(i)} H[b,j,:] (64 columns, bf16)
+__global__ void k_mlp_aggregate(const bf16_t* H, const int* ptr, const int* nbr, int nb, int M, bf16_t* Y) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(t >> 6), c = (int)(t & 63);
+  if (row >= M) return;
+  const int b = row / nb, i = row - b * nb;
+  const int p0 = ptr[i], p1 = ptr[i + 1];
+  float s = 0.f;
+  for (int p = p0; p < p1; ++p) s += bf2f(H[((size_t)b * nb + nbr[p]) * 64 + c]);
+  Y[(size_t)row * 64 + c] = f2bf(p1 > p0 ? s / (float)(p1 - p0) : 0.f);
+}
+
+// dA_prev = (G1 + agg^T(G2)) * (Hprev > 0), agg^T(g)_j = sum_{i in N(j)} g_i / deg_i
+__global__ void k_mlp_aggT_mask(const float* G1, const float* G2, const bf16_t* Hprev, const int* ptr,
+                                const int* nbr, int nb, int M, bf16_t* dA) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(t >> 6), c = (int)(t & 63);
+  if (row >= M) return;
+  const int b = row / nb, j = row - b * nb;
+  float s = G1[(size_t)row * 64 + c];
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) {
+    const int i = nbr[p];
+    const float inv = 1.f / (float)(ptr[i + 1] - ptr[i]);
+    s += G2[((size_t)b * nb + i) * 64 + c] * inv;
+  }
+  dA[(size_t)row * 64 + c] = f2bf(bf2f(Hprev[(size_t)row * 64 + c]) > 0.f ? s : 0.f);
+}
+
+// loss and dO: O fp32 [M x 3]; T fp32 [M x 3]; den[b]; dO bf16 [M x 32] (cols 3.. zero);
+// dH2 = (dO Wo^T) masked by H2>0 -> dA2 bf16 [M x 64]; loss partial per block
+__global__ __launch_bounds__(256) void k_mlp_loss(const float* O, const float* T, const float* den, int nb, int M,
+                                                  int B, const bf16_t* Wo /* bf16 64x32 */, const bf16_t* H2,
+                                                  bf16_t* dO, bf16_t* dA2, float* loss_part) {
+  __shared__ double red[4];
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  double lsum = 0.0;
+  if (row < M) {
+    const int b = row / nb;
+    const float inv = 1.f / den[b];
+    float d[3];
+    for (int c = 0; c < 3; ++c) {
+      const float diff = O[(size_t)row * 3 + c] - T[(size_t)row * 3 + c];
+      lsum += (double)diff * diff * inv;
+      d[c] = 2.f * diff * inv / (float)B;
+    }
+    for (int c = 0; c < 32; ++c) dO[(size_t)row * 32 + c] = c < 3 ? f2bf(d[c]) : (bf16_t)0;
+    for (int h = 0; h < 64; ++h) {
+      const float g = d[0] * bf2f(Wo[h * 32 + 0]) + d[1] * bf2f(Wo[h * 32 + 1]) + d[2] * bf2f(Wo[h * 32 + 2]);
+      dA2[(size_t)row * 64 + h] = f2bf(bf2f(H2[(size_t)row * 64 + h]) > 0.f ? g : 0.f);
+    }
+  }
+  const double s = block_sum_t0(lsum / (double)B, red);
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = (float)s;
+}
+
+// Adam on the flat FP32 master parameters, then refresh bf16 copies
+__global__ void k_mlp_adam(float* theta, const float* grad, float* m, float* v, int count, float lr, float b1,
+                           float b2, float eps, int step, const unsigned char* frozen) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  if (frozen && frozen[e]) return;
+  const float g = grad[e];
+  const float mm = b1 * m[e] + (1.f - b1) * g;
+  const float vv = b2 * v[e] + (1.f - b2) * g * g;
+  m[e] = mm;
+  v[e] = vv;
+  const float mh = mm / (1.f - powf(b1, (float)step));
+  const float vh = vv / (1.f - powf(b2, (float)step));
+  theta[e] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// bf16 copies: W (K x N) and Wt (N x K) of one matrix from fp32 theta
+__global__ void k_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= K * N) return;
+  const int k = e / N, n = e % N;
+  const bf16_t v = f2bf(W[e]);
+  Wb[e] = v;
+  Wt[(size_t)n * K + k] = v;
+}
+
+// learned correction inside the preconditioner:
+// features x_i = [Dfeat_i (9), r_i/rho (3), 0 x 4] in bf16
+__global__ void k_mlp_features(const float* Dfeat /* nb x 9 */, const double* r, double rho, int nb, bf16_t* X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  for (int c = 0; c < 9; ++c) X[(size_t)i * 16 + c] = f2bf(Dfeat[i * 9 + c]);
+  for (int c = 0; c < 3; ++c) X[(size_t)i * 16 + 9 + c] = f2bf((float)(r[3 * i + c] / rho));
+  for (int c = 12; c < 16; ++c) X[(size_t)i * 16 + c] = 0;
+}
+
+// z_i += (rho / s_i) o_i
+__global__ void k_mlp_apply(const float* O, const float* sdiag, double rho, int nb, double* z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const double sc = rho / (double)sdiag[i];
+  for (int c = 0; c < 3; ++c) z[3 * i + c] += sc * (double)O[(size_t)i * 3 + c];
+}
+
+static inline int cdivl(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+hipError_t launch_mlp_gemm(const MlpGemm& g, hipStream_t st) {
+  if (g.M == 0) return hipSuccess;
+  const int grid = cdivl(g.M, 128);
+  if (g.N == 64) hipLaunchKernelGGL(k_mlp_gemm<2>, dim3(grid), dim3(256), 0, st, g);
+  else if (g.N == 32) hipLaunchKernelGGL(k_mlp_gemm<1>, dim3(grid), dim3(256), 0, st, g);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t launch_mlp_wgrad(const MlpWgrad& g, float* out, hipStream_t st) {
+  const int chunks = cdivl(g.M, g.rows_per_chunk);
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(chunks), dim3(256), 0, st, g);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(cdivl((long long)g.K * g.N, 256)), dim3(256), 0, st, g.partial, chunks,
+                     g.K * g.N, out);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_colsum(const bf16_t* dY, int M, int N, int ldd, int rows_per_chunk, float* partial, float* out,
+                             hipStream_t st) {
+  const int chunks = cdivl(M, rows_per_chunk);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(chunks), dim3(256), 0, st, dY, M, N, ldd, rows_per_chunk, partial);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(1), dim3(256), 0, st, partial, chunks, N, out);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_aggregate(const bf16_t* H, const int* ptr, const int* nbr, int nb, int M, bf16_t* Y,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_aggregate, dim3(cdivl((long long)M * 64, 256)), dim3(256), 0, st, H, ptr, nbr, nb, M, Y);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_aggT_mask(const float* G1, const float* G2, const bf16_t* Hprev, const int* ptr, const int* nbr,
+                                int nb, int M, bf16_t* dA, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_aggT_mask, dim3(cdivl((long long)M * 64, 256)), dim3(256), 0, st, G1, G2, Hprev, ptr, nbr,
+                     nb, M, dA);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_loss(const float* O, const float* T, const float* den, int nb, int M, int B, const bf16_t* Wo,
+                           const bf16_t* H2, bf16_t* dO, bf16_t* dA2, float* loss_part, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_loss, dim3(cdivl(M, 256)), dim3(256), 0, st, O, T, den, nb, M, B, Wo, H2, dO, dA2,
+                     loss_part);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_adam(float* theta, const float* grad, float* m, float* v, int count, float lr, int step,
+                           const unsigned char* frozen, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_adam, dim3(cdivl(count, 256)), dim3(256), 0, st, theta, grad, m, v, count, lr, 0.9f,
+                     0.999f, 1e-8f, step, frozen);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_cast, dim3(cdivl(K * N, 256)), dim3(256), 0, st, W, K, N, Wb, Wt);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_features(const float* Dfeat, const double* r, double rho, int nb, bf16_t* X, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_features, dim3(cdivl(nb, 256)), dim3(256), 0, st, Dfeat, r, rho, nb, X);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_apply(const float* O, const float* sdiag, double rho, int nb, double* z, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_apply, dim3(cdivl(nb, 256)), dim3(256), 0, st, O, sdiag, rho, nb, z);
+  return hipGetLastError();
+}
+
+}  // namespace hsc

@@ -663,6 +663,91 @@ __global__ __launch_bounds__(kBlock) void op_lowrank64(SysDesc S, const double* 
   }
 }
 
+// ---------------------------------------------------------------- vector kernels
+// (host-driven flexible PCG of the learned preconditioner)
+__global__ __launch_bounds__(kBlock) void k_dot_partial(const double* a, const double* b, int n, double* part) {
+  __shared__ double red[kBlock / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += a[i] * b[i];
+  const double t = block_sum_t0(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+__global__ void k_dot_final(const double* part, int np, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < np; ++i) s += part[i];
+    *out = s;
+  }
+}
+// x += alpha p ; r -= alpha Ap
+__global__ void k_update_xr(double* x, double* r, const double* p, const double* Ap, double alpha, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] += alpha * p[i];
+    r[i] -= alpha * Ap[i];
+  }
+}
+// y = a x + b y
+__global__ void k_axpby(double* y, const double* x, double a, double b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// neighbour CSR of the BSR pattern without the diagonal block (one per row)
+__global__ void k_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nb) return;
+  nbr_ptr[i] = rowptr[i] - i;
+  if (i == nb) return;
+  int o = rowptr[i] - i;
+  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p)
+    if (colidx[p] != i) nbr[o++] = colidx[p];
+}
+// Dfeat_i = D_ii / s_i (row-major 3x3), s_i = tr(D_ii)/3
+__global__ void k_node_features(const int* rowptr, const int* colidx, const double* val64, int nnzb, int nb,
+                                float* Dfeat, float* sdiag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  int pd = -1;
+  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p)
+    if (colidx[p] == i) pd = p;
+  double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (pd >= 0)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) B[3 * r + c] = val64[(size_t)(r + 3 * c) * nnzb + pd];
+  double s = (B[0] + B[4] + B[8]) / 3.0;
+  if (!(s > 0.0)) s = 1.0;
+  for (int e = 0; e < 9; ++e) Dfeat[9 * i + e] = (float)(B[e] / s);
+  sdiag[i] = (float)s;
+}
+
+hipError_t launch_dot(const double* a, const double* b, int n, double* part, double* out, hipStream_t st) {
+  const int np = min(256, max(1, (n + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(kBlock), 0, st, a, b, n, part);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, st, part, np, out);
+  return hipGetLastError();
+}
+hipError_t launch_update_xr(double* x, double* r, const double* p, const double* Ap, double alpha, int n,
+                            hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_update_xr, dim3((n + 255) / 256), dim3(256), 0, st, x, r, p, Ap, alpha, n);
+  return hipGetLastError();
+}
+hipError_t launch_axpby(double* y, const double* x, double a, double b, int n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, st, y, x, a, b, n);
+  return hipGetLastError();
+}
+hipError_t launch_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr, hipStream_t st) {
+  hipLaunchKernelGGL(k_graph, dim3((nb + 1 + 255) / 256), dim3(256), 0, st, rowptr, colidx, nb, nbr_ptr, nbr);
+  return hipGetLastError();
+}
+hipError_t launch_node_features(const int* rowptr, const int* colidx, const double* val64, int nnzb, int nb,
+                                float* Dfeat, float* sdiag, hipStream_t st) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_node_features, dim3((nb + 255) / 256), dim3(256), 0, st, rowptr, colidx, val64, nnzb, nb,
+                       Dfeat, sdiag);
+  return hipGetLastError();
+}
+
 // explicit instantiations via host launchers
 template <int G, typename VT>
 static hipError_t launch_pcg_t(const PcgParams& P, size_t shmem, hipStream_t st) {

@@ -223,8 +223,14 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
   const long long gthreads = (long long)P.nwg * kBlock;
   const long long g = (long long)b * kBlock + tid;
   unsigned epoch = 0;
+  // partials alternate between two halves of P.part: a workgroup rewrites a
+  // half only two grid barriers after it was written, so every workgroup has
+  // finished reading it (a single buffer lets a fast workgroup overwrite a
+  // partial that a slower one is still reducing)
+  int phase = 0;
 
   auto wg_reduce = [&](double m, long long idx, double q) {
+    double* part = P.part + (size_t)(phase & 1) * 4 * P.nwg;
     for (int off = 32; off >= 1; off >>= 1) {
       double m2 = __shfl_xor(m, off, 64);
       long long i2 = __shfl_xor(idx, off, 64);
@@ -245,18 +251,20 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
         argmax_combine(M, I, s_m[w2], s_i[w2]);
         Q += s_q[w2];
       }
-      P.part[4 * b + 0] = M;
-      P.part[4 * b + 1] = (double)I;
-      P.part[4 * b + 2] = Q;
+      part[4 * b + 0] = M;
+      part[4 * b + 1] = (double)I;
+      part[4 * b + 2] = Q;
     }
   };
   auto all_reduce = [&]() {
+    const double* part = P.part + (size_t)(phase & 1) * 4 * P.nwg;
+    ++phase;
     if (tid == 0) {
       double M = -1.0, Q = 0.0;
       long long I = 0x7fffffffffffffffLL;
       for (int w = 0; w < P.nwg; ++w) {
-        argmax_combine(M, I, P.part[4 * w + 0], (long long)P.part[4 * w + 1]);
-        Q += P.part[4 * w + 2];
+        argmax_combine(M, I, part[4 * w + 0], (long long)part[4 * w + 1]);
+        Q += part[4 * w + 2];
       }
       s_idx = I;
       s_q[0] = Q;
@@ -343,7 +351,7 @@ __global__ void k_body_inverse(const double* A, int qm, double* Binv, int* err) 
   __shared__ double L[12][13];
   __shared__ int ok;
   const int body = blockIdx.x, tid = threadIdx.x, o = 12 * body;
-  if (tid < 144) L[tid / 12][tid % 12] = A[(size_t)(o + tid / 12) * qm + o + tid % 12];
+  for (int e = tid; e < 144; e += blockDim.x) L[e / 12][e % 12] = A[(size_t)(o + e / 12) * qm + o + e % 12];
   __syncthreads();
   if (tid == 0) {
     ok = 1;

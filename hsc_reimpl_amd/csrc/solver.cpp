@@ -18,6 +18,10 @@
 #include "../../include/hsc.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "learned.hpp"
+#include "mlp.hpp"
+
+#include <memory>
 
 namespace {
 
@@ -100,6 +104,7 @@ struct hsc_handle {
   double aca_err = 0.0, abd_res = 0.0, setup_ms = 0.0;
   int doublings = 0;
   int G = 8, wgpcu = 1;
+  double* xsol = nullptr;  // FEM part of the caller's x during a solve
   // buffers
   Buf row_count, rowptr, cursor, active_flag, act_prefix, act_node, act_dof;
   Buf key_col, key_src, colidx, val64, val32, invD64, invD32;
@@ -107,9 +112,11 @@ struct hsc_handle {
   Buf rhs, r, p0, p1, z0, z1, q, q64, yfull, small, ybuf, misc, ints;
   // persistent-launch scratch
   Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, respart;
-  // learned preconditioner weights (device, BF16 blob)
-  Buf learned;
+  // learned preconditioner: graph MLP over D's node graph
+  Buf learned, nbr_ptr, nbr, dfeat, sdiag, xfeat, omlp, ptmp, dotbuf;
   long long learned_bytes = 0;
+  std::unique_ptr<hsc::MlpNet> mlp;
+  int mlp_nb = -1;
 };
 
 namespace {
@@ -163,6 +170,7 @@ hsc::SysDesc make_desc(hsc_handle* h) {
   S.W64 = h->W64.as<double>();
   S.W32 = h->W32.as<float>();
   S.rhs = h->rhs.as<double>();
+  S.x = h->xsol;
   S.r = h->r.as<double>();
   S.p[0] = h->p0.p;
   S.p[1] = h->p1.p;
@@ -220,6 +228,7 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   const double eltM = mixed ? 4.0 : 8.0;
   for (int s = 0; s < nsys; ++s) {
     S[s] = make_desc(hs[s]);
+    if (!S[s].x) throw HscError{HSC_ERR_INVALID_ARGUMENT, "solution vector not bound"};
     if (max_iter_override > 0) S[s].max_iter = max_iter_override;
     S[s].rt_begin = rt;
     rt += (S[s].nb + R - 1) / R;
@@ -394,8 +403,9 @@ LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
           best = per;
           bestc = c;
         }
-      } catch (const HscError&) {
-        continue;
+      } catch (const HscError& e) {
+        if (e.code == HSC_ERR_CAPACITY) continue;  // candidate does not fit: try the next
+        throw;                                     // a HIP or device error ends the solve
       }
     }
   }
@@ -515,7 +525,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     const long long total = (long long)qm * a;
     int nwg = (int)std::min<long long>((long long)std::max(occ, 1) * di.cu, std::max<long long>(1, (total + 1023) / 1024));
     nwg = std::max(1, std::min(nwg, std::max(occ, 1) * di.cu));
-    HIPCHK(h->acapart.ensure(sizeof(double) * 4 * nwg));
+    HIPCHK(h->acapart.ensure(sizeof(double) * 8 * nwg));  // two halves, see k_aca
     HIPCHK(h->barrier.ensure(64));
     HIPCHK(h->errflag.ensure(64));
     HIPCHK(hipMemsetAsync(h->barrier.p, 0, 64, st));
@@ -583,6 +593,153 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   }
   h->setup_ok = true;
   h->setup_ms = ms_since(t0);
+}
+
+// ------------------------------------------------------------------ learned / traced PCG
+// Host-driven (flexible) PCG over the same operators (D, W W^T, Neumann(D)):
+// used for the learned preconditioner (Polak-Ribiere beta, per-iteration
+// fallback to the plain Neumann z when the corrected z is not a descent
+// direction) and for harvesting (r_k, x_k) snapshots of the plain solver.
+struct FpcgOpts {
+  bool learned = false;
+  double* trace_r = nullptr;
+  double* trace_x = nullptr;
+  int every = 0, max_slots = 0;
+  int* nslots = nullptr;
+};
+
+void ensure_graph(hsc_handle* h, hipStream_t st) {
+  const int nb = h->nb, ne = h->nnzb - h->nb;
+  HIPCHK(h->nbr_ptr.ensure(sizeof(int) * (nb + 1)));
+  HIPCHK(h->nbr.ensure(sizeof(int) * std::max(ne, 1)));
+  HIPCHK(h->dfeat.ensure(sizeof(float) * 9 * nb));
+  HIPCHK(h->sdiag.ensure(sizeof(float) * nb));
+  HIPCHK(hsc::launch_graph(h->rowptr.as<int>(), h->colidx.as<int>(), nb, h->nbr_ptr.as<int>(), h->nbr.as<int>(), st));
+  HIPCHK(hsc::launch_node_features(h->rowptr.as<int>(), h->colidx.as<int>(), h->val64.as<double>(), h->nnzb, nb,
+                                   h->dfeat.as<float>(), h->sdiag.as<float>(), st));
+}
+
+void ensure_mlp(hsc_handle* h, hipStream_t st) {
+  if (h->learned_bytes != (long long)sizeof(float) * hsc::MlpLayout::count)
+    throw HscError{HSC_ERR_INVALID_ARGUMENT, "learned preconditioner selected but no weights set"};
+  ensure_graph(h, st);
+  if (!h->mlp || h->mlp_nb != h->nb) {
+    h->mlp.reset(new hsc::MlpNet());
+    HIPCHK(h->mlp->init(h->nb, h->nbr_ptr.as<int>(), h->nbr.as<int>(), h->nnzb - h->nb, 1, st));
+    h->mlp_nb = h->nb;
+  }
+  HIPCHK(h->mlp->set_params(h->learned.as<float>(), true, st));
+  HIPCHK(h->xfeat.ensure(sizeof(hsc::bf16_t) * 16 * h->nb));
+  HIPCHK(h->omlp.ensure(sizeof(float) * 3 * h->nb));
+}
+
+int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStream_t st, const FpcgOpts& o) {
+  auto t0 = Clock::now();
+  const int n = h->fn;
+  HIPCHK(h->ptmp.ensure(sizeof(double) * n));
+  HIPCHK(h->dotbuf.ensure(sizeof(double) * 300));
+  hsc::SysDesc S = make_desc(h);
+  const int G = h->G;
+  double* r = h->r.as<double>();
+  double* rhs = h->rhs.as<double>();
+  double* p = h->p0.as<double>();
+  double* Ap = h->q.as<double>();
+  double* z = h->z0.as<double>();
+  double* zold = h->z1.as<double>();
+  double* tmp = h->ptmp.as<double>();
+  double* xf = x + h->fo;
+  double* part = h->dotbuf.as<double>();
+  double* dout = part + 290;
+  auto dot = [&](const double* a, const double* c) {
+    HIPCHK(hsc::launch_dot(a, c, n, part, dout, st));
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, dout, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return v;
+  };
+  auto schur = [&](const double* in, double* out) {
+    HIPCHK(hsc::launch_op_Dx64(G, S, in, out, st));
+    HIPCHK(hsc::launch_op_lowrank64(S, in, out, h->ybuf.as<double>(), st));
+  };
+  int fallbacks = 0;
+  // z = M(r) (+ learned correction); returns r.z
+  auto precondition = [&](double rr) {
+    HIPCHK(hsc::launch_op_precond64(G, S, h->cfg.neumann_order, h->cfg.omega, r, z, tmp, st));
+    if (!o.learned) return dot(r, z);
+    HIPCHK(hipMemcpyAsync(tmp, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    const double rho = std::sqrt(rr / n) > 0 ? std::sqrt(rr / n) : 1.0;
+    HIPCHK(hsc::launch_mlp_features(h->dfeat.as<float>(), r, rho, h->nb, h->xfeat.as<hsc::bf16_t>(), st));
+    HIPCHK(h->mlp->forward(h->xfeat.as<hsc::bf16_t>(), 1, h->omlp.as<float>(), st));
+    HIPCHK(hsc::launch_mlp_apply(h->omlp.as<float>(), h->sdiag.as<float>(), rho, h->nb, z, st));
+    const double rz = dot(r, z);
+    if (!(std::isfinite(rz) && rz > 0)) {  // fall back to the plain Neumann z
+      ++fallbacks;
+      HIPCHK(hipMemcpyAsync(z, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      return dot(r, z);
+    }
+    return rz;
+  };
+  if (o.learned) ensure_mlp(h, st);
+  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, n, h->k,
+                         h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(), h->ybuf.as<double>(),
+                         rhs, r, x, st));
+  const double nb2 = dot(rhs, rhs);
+  const double normb = std::sqrt(nb2);
+  double rel = normb > 0 ? 1.0 : 0.0;
+  int it = 0, restarts = 0, checks = 0, slots = 0, status = HSC_OK;
+  const double tol = h->cfg.tol_rate;
+  if (normb > 0) {
+    double rz = precondition(nb2);
+    HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    while (it < h->cfg.max_iter && rel > tol) {
+      schur(p, Ap);
+      const double pAp = dot(p, Ap);
+      if (!(std::isfinite(pAp) && pAp > 0 && std::isfinite(rz) && rz > 0)) {
+        status = HSC_ERR_BREAKDOWN;
+        break;
+      }
+      const double alpha = rz / pAp;
+      HIPCHK(hsc::launch_update_xr(xf, r, p, Ap, alpha, n, st));
+      ++it;
+      if (o.every > 0 && it % o.every == 0 && slots < o.max_slots) {
+        HIPCHK(hipMemcpyAsync(o.trace_r + (size_t)slots * n, r, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(o.trace_x + (size_t)slots * n, xf, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        ++slots;
+      }
+      double rr = dot(r, r);
+      rel = std::sqrt(rr) / normb;
+      bool restart = false;
+      if (rel <= tol) {  // one true residual check (linear_hsc.cu:375-387)
+        ++checks;
+        schur(xf, Ap);
+        HIPCHK(hipMemcpyAsync(r, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hsc::launch_axpby(r, Ap, -1.0, 1.0, n, st));
+        rr = dot(r, r);
+        rel = std::sqrt(rr) / normb;
+        if (rel <= tol) break;
+        restart = true;
+        ++restarts;
+      }
+      HIPCHK(hipMemcpyAsync(zold, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      const double rzn = precondition(rr);
+      double beta = 0.0;
+      if (!restart) {
+        if (o.learned) beta = (rzn - dot(r, zold)) / rz;  // Polak-Ribiere (flexible)
+        else beta = rzn / rz;
+      }
+      HIPCHK(hsc::launch_axpby(p, z, 1.0, beta, n, st));
+      rz = rzn;
+    }
+    if (status == HSC_OK && rel > tol) status = HSC_ERR_NOT_CONVERGED;
+  }
+  if (o.nslots) *o.nslots = slots;
+  s->pcg_iterations = it;
+  s->schur_relative_residual = rel;
+  s->restarts = restarts;
+  s->true_residual_checks = checks;
+  s->fallback = fallbacks > 0;
+  s->pcg_ms = ms_since(t0);
+  return status;
 }
 
 void fill_common_stats(hsc_handle* h, hsc_stats* s) {
@@ -656,10 +813,23 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   hsc_stats tmp;
   if (!s) s = &tmp;
   fill_common_stats(h, s);
+  if (h->cfg.preconditioner == HSC_PRECOND_LEARNED) {
+    FpcgOpts o;
+    o.learned = true;
+    int code = fpcg_solve(h, b, x, s, st, o);
+    if (code == HSC_ERR_BREAKDOWN) h->err = "HSC flexible PCG breakdown. No fallback.";
+    if (code == HSC_ERR_NOT_CONVERGED) h->err = "HSC flexible PCG did not converge";
+    int c2 = finish_solve(h, b, x, s, st);
+    if (code == HSC_OK) code = c2;
+    s->status = code;
+    s->total_ms = ms_since(t0);
+    return code;
+  }
   HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
                          h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
                          h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
   hsc_handle* hs[1] = {h};
+  h->xsol = x + h->fo;
   LaunchCfg lc = tuned_config(hs, 1, st);
   h->G = lc.G;
   h->wgpcu = lc.wgpcu;
@@ -678,7 +848,7 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   int c2 = finish_solve(h, b, x, s, st);
   if (code == HSC_OK) code = c2;
   s->status = code;
-  s->total_ms = ms_since(t0) + (s->setup_ms > 0 ? 0.0 : 0.0);
+  s->total_ms = ms_since(t0);
   return code;
 }
 
@@ -841,6 +1011,7 @@ int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count, const doubl
                              h->fn, h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
                              h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x[i], st));
     }
+    for (int i = 0; i < count; ++i) handles[i]->xsol = x[i] + handles[i]->fo;
     LaunchCfg lc = tuned_config(handles, count, st);
     for (int i = 0; i < count; ++i) {
       hsc_handle* h = handles[i];
@@ -971,11 +1142,67 @@ int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* 
 
 int32_t hsc_set_learned_weights(hsc_handle* h, const void* w, int64_t bytes) {
   if (!h || (!w && bytes)) return HSC_ERR_INVALID_ARGUMENT;
+  if (bytes != 0 && bytes != (int64_t)sizeof(float) * hsc::MlpLayout::count) {
+    h->err = "learned weights: expected hsc_mlp_param_count() float32 values";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
   DeviceGuard g(h->device);
   return guarded(h, [&] {
-    HIPCHK(h->learned.ensure((size_t)bytes));
-    if (bytes) HIPCHK(hipMemcpy(h->learned.p, w, (size_t)bytes, hipMemcpyDeviceToDevice));
+    HIPCHK(h->learned.ensure((size_t)std::max<int64_t>(bytes, 4)));
+    if (bytes) HIPCHK(hipMemcpy(h->learned.p, w, (size_t)bytes, hipMemcpyDefault));
     h->learned_bytes = bytes;
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_solve_traced(hsc_handle* h, const double* b, double* x, double* trace_r, double* trace_x, int32_t every,
+                         int32_t max_slots, int32_t* nslots, hsc_stats* stats, hsc_stream stream) {
+  if (!h || !b || !x || (every > 0 && (!trace_r || !trace_x || max_slots < 0))) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  auto t0 = Clock::now();
+  hsc_stats tmp;
+  hsc_stats* s = stats ? stats : &tmp;
+  return guarded(h, [&] {
+    fill_common_stats(h, s);
+    FpcgOpts o;
+    o.trace_r = trace_r;
+    o.trace_x = trace_x;
+    o.every = every;
+    o.max_slots = max_slots;
+    o.nslots = nslots;
+    int code = fpcg_solve(h, b, x, s, (hipStream_t)stream, o);
+    int c2 = finish_solve(h, b, x, s, (hipStream_t)stream);
+    if (code == HSC_OK) code = c2;
+    s->status = code;
+    s->total_ms = ms_since(t0);
+    return code;
+  });
+}
+
+int32_t hsc_get_node_graph(hsc_handle* h, int32_t* nbr_ptr, int32_t* nbr, int64_t* n_edges) {
+  if (!h || !n_edges) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    *n_edges = (int64_t)h->nnzb - h->nb;
+    if (nbr_ptr || nbr) {
+      ensure_graph(h, nullptr);
+      if (nbr_ptr) HIPCHK(hipMemcpy(nbr_ptr, h->nbr_ptr.p, sizeof(int) * (h->nb + 1), hipMemcpyDefault));
+      if (nbr && *n_edges) HIPCHK(hipMemcpy(nbr, h->nbr.p, sizeof(int) * (size_t)*n_edges, hipMemcpyDefault));
+    }
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_get_node_features(hsc_handle* h, float* dfeat, float* sdiag) {
+  if (!h || !dfeat || !sdiag) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    ensure_graph(h, nullptr);
+    HIPCHK(hipMemcpy(dfeat, h->dfeat.p, sizeof(float) * 9 * h->nb, hipMemcpyDefault));
+    HIPCHK(hipMemcpy(sdiag, h->sdiag.p, sizeof(float) * h->nb, hipMemcpyDefault));
     return HSC_OK;
   });
 }

@@ -184,8 +184,42 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* P,
                         int32_t* active, double* aca_history);
 
-/* ---- learned preconditioner weights (BF16 graph MLP, see hsc_learned.h) ---- */
-int32_t hsc_set_learned_weights(hsc_handle* h, const void* weights_dev, int64_t bytes);
+/* ---- learned preconditioner (no reference counterpart) ----
+ * A per-node graph MLP over D's node graph (2 message-passing layers, hidden
+ * 64, BF16 weights/activations, FP32 accumulation) corrects the Neumann z
+ * inside flexible PCG (Polak-Ribiere beta); when the corrected z is not a
+ * descent direction the plain Neumann z is used for that iteration and
+ * hsc_stats.fallback is set.  Selected by hsc_config.preconditioner. */
+int32_t hsc_set_learned_weights(hsc_handle* h, const void* theta_f32, int64_t bytes);
+
+/* Host-driven PCG of the plain solver (same operators) that stores r_k and
+ * the FEM part of x_k every `every` iterations into trace_r/trace_x (device,
+ * max_slots x fem_dofs doubles each): residual/error pairs for training. */
+int32_t hsc_solve_traced(hsc_handle* h, const double* b, double* x, double* trace_r, double* trace_x,
+                         int32_t every, int32_t max_slots, int32_t* nslots, hsc_stats* stats, hsc_stream stream);
+
+/* Node graph of D (neighbour CSR without self; device or host destination)
+ * and per-node features D_ii / s_i (row-major 3x3) and s_i = tr(D_ii)/3. */
+int32_t hsc_get_node_graph(hsc_handle* h, int32_t* nbr_ptr, int32_t* nbr, int64_t* n_edges);
+int32_t hsc_get_node_features(hsc_handle* h, float* dfeat, float* sdiag);
+
+/* The network object used for training on the GPU (hand-written BF16
+ * forward/backward kernels + Adam).  X: [B x nb x 16] bf16 features
+ * (D_ii/s_i, r_i/rms(r), 0 x 4); T: [B x nb x 3] fp32 targets; den: [B]
+ * per-sample normalisers; O: [B x nb x 3] fp32 outputs.  Parameters are a
+ * flat float32 vector of hsc_mlp_param_count() values. */
+typedef struct hsc_mlp hsc_mlp;
+int32_t hsc_mlp_param_count(void);
+int32_t hsc_mlp_create(int32_t device, int32_t nb, const int32_t* nbr_ptr, const int32_t* nbr, int64_t n_edges,
+                       int32_t max_batch, hsc_mlp** out);
+void hsc_mlp_destroy(hsc_mlp* m);
+int32_t hsc_mlp_set_params(hsc_mlp* m, const float* theta, int32_t device_ptr);
+int32_t hsc_mlp_get_params(hsc_mlp* m, float* theta_host);
+int32_t hsc_mlp_forward(hsc_mlp* m, const uint16_t* X, int32_t B, float* O, hsc_stream stream);
+int32_t hsc_mlp_gradients(hsc_mlp* m, const uint16_t* X, const float* T, const float* den, int32_t B,
+                          float* grad_host, float* loss, hsc_stream stream);
+int32_t hsc_mlp_train_step(hsc_mlp* m, const uint16_t* X, const float* T, const float* den, int32_t B, float lr,
+                           float* loss, hsc_stream stream);
 
 #ifdef __cplusplus
 }
