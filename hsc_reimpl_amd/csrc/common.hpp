@@ -5,11 +5,12 @@
 
 namespace hsc {
 
-constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
-constexpr int kNodeTile = 256;     // nodes per elementwise tile (1 thread / node)
+constexpr int kBlock = 512;        // threads per workgroup (8 wave64)
+constexpr int kNodeTile = 128;     // nodes per elementwise tile (threads < kNodeTile own one node each)
 constexpr int kLrTile = 64;        // low-rank rows (W rows) per tile
 constexpr int kMaxSys = 128;       // systems per persistent launch
-constexpr int kMaxRankLds = 256;   // rank limit of the low-rank apply (4 x 64 lanes)
+constexpr int kMaxRankLds = 256;   // rank limit of the low-rank apply (lanes hold columns l, l+64, l+128, l+192)
+constexpr int kBarrierWords = 1024;  // grid-barrier counters (see device_util.hpp)
 
 // Per-system descriptor for the persistent PCG kernel and the operator
 // kernels.  All pointers are device pointers.  VT buffers (p, z, q) hold
@@ -85,6 +86,8 @@ struct PcgParams {
   int* error;             // device error flag
   long long timeout_ticks;// bounded wait (s_memrealtime ticks, 100 MHz)
   int t_total;            // LDS doubles for t vectors
+  long long* prof;        // optional per-workgroup stage timers [nwg * 32] (null = off):
+                          // site si: [2si] ticks computing before barrier si, [2si+1] ticks waiting in it
 };
 
 // Setup-time kernels' scratch layout ------------------------------------

@@ -133,27 +133,95 @@ __device__ __forceinline__ void tile_Dx64(const SysDesc& S, int tl, const double
 }
 
 // low-rank partial t[l] += sum_{j in tile} W[j,l] src(act_dof[j]); acc[c]
-// holds column l = lane + 64c.  Rows of the tile are split over the 4 waves.
+// holds column l = lane + 64c.  Each wave owns 16 consecutive rows of the
+// tile: lanes 0..15 fetch the 16 source values at once, then every lane
+// issues its 16 W loads before the multiply-adds (one latency, not 16).
 template <typename WT, typename Src>
 __device__ __forceinline__ void tile_Wt(const SysDesc& S, int tl, const WT* __restrict__ W, Src src,
                                         double acc[4]) {
+  constexpr int RW = kLrTile / (kBlock / 64);  // rows per wave
+  constexpr int RB = 8;                        // rows whose W loads are in flight together
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int j0 = tl * kLrTile;
-  const int j1 = min(j0 + kLrTile, S.a);
+  const int jw = tl * kLrTile + wave * RW;
+  const int j1 = min(tl * kLrTile + kLrTile, S.a);
   const int k = S.k;
-  for (int j = j0 + wave; j < j1; j += kBlock / 64) {
-    const double s = src(S.act_dof[j]);
-    const WT* w = W + (size_t)j * k;
-#pragma unroll
+  double my_src = 0.0;
+  if (lane < RW && jw + lane < j1) my_src = src(S.act_dof[jw + lane]);
+#pragma unroll 1
+  for (int jb = 0; jb < RW; jb += RB) {
+#pragma unroll 1
     for (int c = 0; c < 4; ++c) {
       const int l = lane + 64 * c;
-      if (l < k) acc[c] += (double)w[l] * s;
+      if (64 * c >= k) break;
+      double w[RB];
+#pragma unroll
+      for (int jj = 0; jj < RB; ++jj) {
+        const int j = jw + jb + jj;
+        w[jj] = (l < k && j < j1) ? (double)W[(size_t)j * k + l] : 0.0;
+      }
+#pragma unroll
+      for (int jj = 0; jj < RB; ++jj) acc[c] += w[jj] * __shfl(my_src, jb + jj, 64);
     }
   }
 }
 
+// g_d = sum_l W[an+d, l] t[l] for the active nodes of one node tile, computed
+// cooperatively: active nodes are listed in LDS and each wave reduces two of
+// them at a time over its lanes (l = lane + 64c).  Uniform over the workgroup.
+template <typename WT>
+__device__ __forceinline__ void tile_lowrank_g(const SysDesc& S, int an, const WT* __restrict__ W,
+                                               const double* __restrict__ t, double g[3], int* s_list,
+                                               int* s_slot, double (*s_g)[3], int* s_cnt) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int k = S.k;
+  if (tid == 0) *s_cnt = 0;
+  __syncthreads();
+  if (an >= 0) {
+    const int slot = atomicAdd(s_cnt, 1);
+    s_list[slot] = an;
+    s_slot[tid] = slot;
+  }
+  __syncthreads();
+  const int m = *s_cnt;
+  constexpr int NW = kBlock / 64;
+  for (int it = wave; it < m; it += 2 * NW) {
+    const int it2 = it + NW;
+    const int a1 = s_list[it];
+    const int a2 = it2 < m ? s_list[it2] : -1;
+    double p1[3] = {0.0, 0.0, 0.0}, p2[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 4; ++c) {
+      if (64 * c >= k) break;
+      const int l = lane + 64 * c;
+      if (l < k) {
+        const double tl_ = t[l];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          p1[d] += (double)W[(size_t)(a1 + d) * k + l] * tl_;
+          if (a2 >= 0) p2[d] += (double)W[(size_t)(a2 + d) * k + l] * tl_;
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double v1 = wave_sum(p1[d]);
+      const double v2 = wave_sum(p2[d]);
+      if (lane == 0) {
+        s_g[it][d] = v1;
+        if (a2 >= 0) s_g[it2][d] = v2;
+      }
+    }
+  }
+  __syncthreads();
+  if (an >= 0) {
+    const int slot = s_slot[tid];
+    g[0] = s_g[slot][0];
+    g[1] = s_g[slot][1];
+    g[2] = s_g[slot][2];
+  }
+}
+
 // sum over the four waves' acc, written to dst[l] for l < k (thread-ordered)
-__device__ __forceinline__ void flush_Wt(double acc[4], int k, double* dst, double (*lr)[kBlock]) {
+__device__ __forceinline__ void flush_Wt(double acc[4], int k, double* dst, double (*lr)[64]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int c = 0; c < 4; ++c) {
     if (64 * c >= k) break;
@@ -233,8 +301,11 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   __shared__ int s_st[kMaxSys], s_it[kMaxSys], s_chk[kMaxSys], s_rs[kMaxSys];
   __shared__ int s_nchk[kMaxSys], s_nrs[kMaxSys];
   __shared__ double s_red[kBlock / 64];
-  __shared__ double s_lr[kBlock / 64][kBlock];
+  __shared__ double s_lr[kBlock / 64][64];
   __shared__ int s_any;
+  __shared__ int s_list[kNodeTile], s_slot[kNodeTile], s_cnt;
+  __shared__ double s_g[kNodeTile][3];
+  __shared__ double s_tred[kBlock];
 
   const SysDesc* sys = P.sys;
   const int nsys = P.nsys;
@@ -242,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const bool is_lr = b < P.n_lr_wg;
-  unsigned epoch = 0;
+  GridBarrier gbar;
   const VT omega = (VT)P.omega;
 
   TileRange trA{0, 0}, trF{0, 0}, trE{0, 0}, trL{0, 0};
@@ -265,6 +336,26 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   }
   __syncthreads();
 
+  // optional stage timers (workgroup-private slots, thread 0 only)
+  long long t_last = 0;
+  auto sync_at = [&](int si) -> bool {
+    long long t_before = 0;
+    if (P.prof && tid == 0) t_before = rt_ticks();
+    const bool ok = gb_sync(gbar);
+    if (P.prof && tid == 0) {
+      const long long t_after = rt_ticks();
+      long long* slot = P.prof + (size_t)b * 32;
+      slot[2 * si] += t_before - t_last;
+      slot[2 * si + 1] += t_after - t_before;
+      slot[20 + si] += 1;
+      t_last = t_after;
+    }
+    return ok;
+  };
+
+  const bool started = gb_init(gbar, P.barrier, P.error, P.timeout_ticks, (unsigned)P.nwg);
+  if (P.prof && tid == 0) t_last = rt_ticks();
+  if (started) {
   int zc = 0, pc = 0;
   const int order = P.order;
   double* partE_rr = P.partE;
@@ -281,7 +372,16 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                 if (st != ST_RUN) return;
                 if (kind == 2 && s_chk[s] == CHK_NONE) return;
                 const int i = tl * kNodeTile + tid;
-                if (i >= S.nb) return;
+                const bool valid = tid < kNodeTile && i < S.nb;
+                double g[3] = {0.0, 0.0, 0.0};
+                if (kind != 0 && S.k > 0) {
+                  const int an = valid ? S.act_node[i] : -1;
+                  if (kind == 1)
+                    tile_lowrank_g(S, an, w_vals<VT>(S), s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
+                  else
+                    tile_lowrank_g(S, an, S.W64, s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
+                }
+                if (!valid) return;
                 const int i3 = 3 * i;
                 double rv[3];
                 bool write_r = false, write_z = true;
@@ -290,23 +390,6 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   rv[1] = S.r[i3 + 1];
                   rv[2] = S.r[i3 + 2];
                 } else {
-                  const int an = S.act_node[i];
-                  double g[3] = {0.0, 0.0, 0.0};
-                  const double* t = s_t + S.t_off;
-                  if (an >= 0) {
-                    const int k = S.k;
-                    for (int d = 0; d < 3; ++d) {
-                      double acc = 0.0;
-                      if (kind == 1) {
-                        const VT* w = w_vals<VT>(S) + (size_t)(an + d) * k;
-                        for (int l = 0; l < k; ++l) acc += (double)w[l] * t[l];
-                      } else {
-                        const double* w = S.W64 + (size_t)(an + d) * k;
-                        for (int l = 0; l < k; ++l) acc += w[l] * t[l];
-                      }
-                      g[d] = acc;
-                    }
-                  }
                   if (kind == 1) {
                     const VT* q = (const VT*)S.q;
                     const VT* pn = (const VT*)S.p[pc];
@@ -372,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   }
                 });
       zc ^= 1;
-      if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) return false;
+      if (!sync_at(1)) return false;
     }
     return true;
   };
@@ -393,11 +476,23 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     for (int s = 0; s < nsys; ++s) {
       const SysDesc& S = sys[s];
       if (!want(s) || S.k == 0) continue;
-      for (int l = tid; l < S.k; l += kBlock) {
+      // thread (grp, l) sums chunks cL0+grp, +ng, ...; groups are then added in order
+      const int k = S.k;
+      const int ng = max(1, kBlock / k);
+      const int l = tid % k, grp = tid / k;
+      if (k <= kBlock) {
         double v = 0.0;
-        for (int c = S.cL0; c <= S.cL1; ++c)
-          if (!chunk_empty(c, P.nL, P.lt_total)) v += P.partL[S.lrp_off + (size_t)(c - S.cL0) * S.k + l];
-        s_t[S.t_off + l] = v;
+        if (grp < ng)
+          for (int c = S.cL0 + grp; c <= S.cL1; c += ng)
+            if (!chunk_empty(c, P.nL, P.lt_total)) v += P.partL[S.lrp_off + (size_t)(c - S.cL0) * k + l];
+        if (grp < ng) s_tred[grp * k + l] = v;
+        __syncthreads();
+        if (tid < k) {
+          double acc = 0.0;
+          for (int q = 0; q < ng; ++q) acc += s_tred[q * k + tid];
+          s_t[S.t_off + tid] = acc;
+        }
+        __syncthreads();
       }
     }
     __syncthreads();
@@ -444,7 +539,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
 
   // ================================================================ init
   node_stage(0);
-  if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+  if (!sync_at(0)) goto out;
   for (int s = wave; s < nsys; s += kBlock / 64) {
     const SysDesc& S = sys[s];
     const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1, P.et_total);
@@ -495,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                 });
     }
     pc ^= 1;
-    if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+    if (!sync_at(2)) goto out;
     for (int s = wave; s < nsys; s += kBlock / 64) {
       const SysDesc& S = sys[s];
       const double v = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1, P.rt_total);
@@ -516,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
 
     // ---------------- stage B
     node_stage(1);
-    if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+    if (!sync_at(3)) goto out;
     for (int s = wave; s < nsys; s += kBlock / 64) {
       const SysDesc& S = sys[s];
       const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1, P.et_total);
@@ -556,10 +651,10 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   },
                   [&](int) {});
       }
-      if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+      if (!sync_at(4)) goto out;
       reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; });
       node_stage(2);
-      if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) goto out;
+      if (!sync_at(5)) goto out;
       for (int s = wave; s < nsys; s += kBlock / 64) {
         if (s_st[s] != ST_RUN || s_chk[s] == CHK_NONE) continue;
         const SysDesc& S = sys[s];
@@ -607,6 +702,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     __syncthreads();
   }
 
+  }  // started
 out:
   __syncthreads();
   if (b == 0) {
@@ -649,7 +745,7 @@ __global__ __launch_bounds__(kBlock) void op_jacobi(SysDesc S, const double* r, 
 // t = W^T x (single workgroup, fixed order) then y[act] -= W t
 __global__ __launch_bounds__(kBlock) void op_lowrank64(SysDesc S, const double* x, double* y, double* t) {
   __shared__ double s_tv[kMaxRankLds];
-  __shared__ double lr[kBlock / 64][kBlock];
+  __shared__ double lr[kBlock / 64][64];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int ntile = (S.a + kLrTile - 1) / kLrTile;
   for (int tl = 0; tl < ntile; ++tl) tile_Wt(S, tl, S.W64, [&](int dof) { return x[dof]; }, acc);

@@ -222,7 +222,8 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
   const long long total = (long long)P.qm * P.a;
   const long long gthreads = (long long)P.nwg * kBlock;
   const long long g = (long long)b * kBlock + tid;
-  unsigned epoch = 0;
+  GridBarrier gbar;
+  if (!gb_init(gbar, P.barrier, P.error, P.timeout_ticks, (unsigned)P.nwg)) return;
   // partials alternate between two halves of P.part: a workgroup rewrites a
   // half only two grid barriers after it was written, so every workgroup has
   // finished reading it (a single buffer lets a fast workgroup overwrite a
@@ -256,18 +257,28 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
       part[4 * b + 2] = Q;
     }
   };
+  // every workgroup combines all partials the same way (wave 0: lanes strided
+  // over workgroups, then a butterfly), so all take the same pivot
   auto all_reduce = [&]() {
     const double* part = P.part + (size_t)(phase & 1) * 4 * P.nwg;
     ++phase;
-    if (tid == 0) {
+    if (tid < 64) {
       double M = -1.0, Q = 0.0;
       long long I = 0x7fffffffffffffffLL;
-      for (int w = 0; w < P.nwg; ++w) {
+      for (int w = tid; w < P.nwg; w += 64) {
         argmax_combine(M, I, part[4 * w + 0], (long long)part[4 * w + 1]);
         Q += part[4 * w + 2];
       }
-      s_idx = I;
-      s_q[0] = Q;
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double m2 = __shfl_xor(M, off, 64);
+        const long long i2 = __shfl_xor(I, off, 64);
+        argmax_combine(M, I, m2, i2);
+        Q += __shfl_xor(Q, off, 64);
+      }
+      if (tid == 0) {
+        s_idx = I;
+        s_q[0] = Q;
+      }
     }
     __syncthreads();
   };
@@ -281,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
     q += v * v;
   }
   wg_reduce(m, mi, q);
-  if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) return;
+  if (!gb_sync(gbar)) return;
   all_reduce();
   if (tid == 0) {
     s_scale = sqrt(s_q[0]);
@@ -329,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
       q += val * val;
     }
     wg_reduce(m, mi, q);
-    if (!grid_sync(P.barrier, epoch, P.nwg, P.error, P.timeout_ticks)) return;
+    if (!gb_sync(gbar)) return;
     all_reduce();
     ++kk;
     if (tid == 0) {
@@ -533,7 +544,22 @@ __global__ __launch_bounds__(1024) void k_abd_inverse(AbdParams S) {
     wg_symmetrize(S.P, n);
   }
   double res = 0.0;
-  if (status == 0) {
+  if (status == 0 && S.mode == 0 && S.blockdiag) {
+    // both A and P are block diagonal: only the 12x12 diagonal blocks of I - A P are nonzero
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < n * 12; e += blockDim.x) {
+      const int i = e / 12, jj = e % 12, o = 12 * (i / 12), j = o + jj;
+      double v = (i == j) ? 1.0 : 0.0;
+      for (int m = 0; m < 12; ++m) v -= S.A[(size_t)i * n + o + m] * S.P[(size_t)(o + m) * n + j];
+      acc += v * v;
+    }
+    const double tot = block_sum_t0(acc, red);
+    __shared__ double s_tot2;
+    if (threadIdx.x == 0) s_tot2 = tot;
+    __syncthreads();
+    res = sqrt(s_tot2) / sqrt((double)n);
+    if (!isfinite(res)) status = 6;
+  } else if (status == 0) {
     res = wg_inverse_residual(S.A, S.P, S.s3, n, red);
     if (S.mode == 2) {
       while (res > S.tol && doublings < 48) {
@@ -557,22 +583,35 @@ __global__ __launch_bounds__(1024) void k_abd_inverse(AbdParams S) {
 }
 
 
+// C[M x N] = op(A) B for small dense FP64 blocks: 16x16 output tiles per
+// workgroup staged through LDS.  op(A) = A (row-major, lda) or A^T when
+// transA (A stored column-major with leading dim lda, i.e. A^T row-major).
+// B is addressed as B[k*ldb_k + n*ldb_n] so both layouts are covered.
+__global__ __launch_bounds__(256) void k_gemm_small(const double* A, int lda, int transA, const double* B,
+                                                    int ldb_k, int ldb_n, double* C, int ldc, int M, int N, int K) {
+  __shared__ double As[16][17], Bs[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
+  double acc = 0.0;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int ka = k0 + tx, kb = k0 + ty;
+    const int ar = blockIdx.y * 16 + ty;
+    As[ty][tx] = (ar < M && ka < K) ? (transA ? A[(size_t)ka * lda + ar] : A[(size_t)ar * lda + ka]) : 0.0;
+    Bs[ty][tx] = (kb < K && col < N) ? B[(size_t)kb * ldb_k + (size_t)col * ldb_n] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) acc += As[ty][kk] * Bs[kk][tx];
+    __syncthreads();
+  }
+  if (row < M && col < N) C[(size_t)row * ldc + col] = acc;
+}
+
+constexpr int kLdsChol = 80;  // C of rank <= 80 is factored in LDS (50 KB)
+
 // C = U^T P U symmetrised (linear_hsc.cu:298-300) and its Cholesky C = L L^T.
+// (the products W1 = P U and C = U^T W1 are done by k_gemm_small before this)
 __global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
-  const int n = S.qm, k = S.k;
-  for (int e = threadIdx.x; e < n * k; e += blockDim.x) {
-    const int i = e / k, l = e % k;
-    double s = 0.0;
-    for (int m = 0; m < n; ++m) s += S.P[(size_t)i * n + m] * S.U[m + (size_t)n * l];
-    S.W1[e] = s;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
-    const int l1 = e / k, l2 = e % k;
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += S.U[i + (size_t)n * l1] * S.W1[(size_t)i * k + l2];
-    S.C[e] = s;
-  }
+  const int k = S.k;
   __syncthreads();
   wg_symmetrize(S.C, k);
   int bad = 0;
@@ -586,7 +625,18 @@ __global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
   if (bad) atomicOr(&s_bad, 1);
   __syncthreads();
   int status = s_bad ? 6 : 0;
-  if (!status && !wg_cholesky(S.Lm, k)) status = 6;
+  if (!status) {
+    if (k <= kLdsChol) {
+      __shared__ double Ls[kLdsChol * kLdsChol];
+      for (int e = threadIdx.x; e < k * k; e += blockDim.x) Ls[e] = S.Lm[e];
+      __syncthreads();
+      if (!wg_cholesky(Ls, k)) status = 6;
+      for (int e = threadIdx.x; e < k * k; e += blockDim.x) S.Lm[e] = Ls[e];
+      __syncthreads();
+    } else if (!wg_cholesky(S.Lm, k)) {
+      status = 6;
+    }
+  }
   if (threadIdx.x == 0) S.out[0] = (double)status;
 }
 
@@ -602,29 +652,40 @@ __global__ void k_make_W(const double* V, const double* Lm, int a, int k, double
 }
 
 // y = L^-1 (U^T P b_q)  (reduced right-hand side, core.py:163-164;
-// linear_hsc.cu:351-357 with V c = W L^-1 c)
-__global__ __launch_bounds__(256) void k_rhs_small(const double* P, const double* U, const double* Lm,
+// linear_hsc.cu:351-357 with V c = W L^-1 c).  Dot products are spread over
+// the waves (lanes over the inner index, butterfly sums).
+__global__ __launch_bounds__(512) void k_rhs_small(const double* P, const double* U, const double* Lm,
                                                    const double* b, int qo, int qm, int k, double* tmp, double* y) {
-  // tmp[0:qm] = P b_q
-  for (int i = threadIdx.x; i < qm; i += blockDim.x) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int i = wave; i < qm; i += nw) {  // tmp[0:qm] = P b_q
     double s = 0.0;
-    for (int m = 0; m < qm; ++m) s += P[(size_t)i * qm + m] * b[qo + m];
-    tmp[i] = s;
+    for (int m = lane; m < qm; m += 64) s += P[(size_t)i * qm + m] * b[qo + m];
+    s = wave_sum(s);
+    if (lane == 0) tmp[i] = s;
   }
   __syncthreads();
-  for (int l = threadIdx.x; l < k; l += blockDim.x) {
+  for (int l = wave; l < k; l += nw) {  // c = U^T tmp
     double s = 0.0;
-    for (int i = 0; i < qm; ++i) s += U[i + (size_t)qm * l] * tmp[i];
-    tmp[qm + l] = s;
+    for (int i = lane; i < qm; i += 64) s += U[i + (size_t)qm * l] * tmp[i];
+    s = wave_sum(s);
+    if (lane == 0) tmp[qm + l] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  // forward substitution L y = c by wave 0 (lanes over the inner index, y in LDS)
+  __shared__ double ys[kMaxRankLds];
+  if (wave == 0) {
     for (int i = 0; i < k; ++i) {
-      double s = tmp[qm + i];
-      for (int m = 0; m < i; ++m) s -= Lm[(size_t)i * k + m] * y[m];
-      y[i] = s / Lm[(size_t)i * k + i];
+      double part = 0.0;
+      for (int m = lane; m < i; m += 64) part += Lm[(size_t)i * k + m] * ys[m];
+      part = wave_sum(part);
+      const double yi = (tmp[qm + i] - part) / Lm[(size_t)i * k + i];
+      if (lane == 0) ys[i] = yi;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     }
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) y[i] = ys[i];
 }
 
 // rhs = b_x - W y ; r = rhs ; x_fem = 0
@@ -646,27 +707,30 @@ __global__ void k_rhs_vec(const double* b, int fo, int n, const int* act_node, c
 }
 
 // dq = P (b_q - U V^T dx) written to x[qo:qo+qm]  (core.py:167; linear_hsc.cu:395-399)
-__global__ __launch_bounds__(256) void k_backsub(const double* P, const double* U, const double* V,
-                                                 const int* act_dof, const double* b, double* x, int qo,
-                                                 int qm, int fo, int a, int k, double* tmp) {
+// k_backsub_vt: one workgroup per rank column, vt[l] = V[:,l] . dx[active]
+__global__ __launch_bounds__(256) void k_backsub_vt(const double* V, const int* act_dof, const double* x, int fo,
+                                                    int a, double* vt) {
   __shared__ double red[256 / 64];
-  for (int l = 0; l < k; ++l) {
-    double s = 0.0;
-    for (int j = threadIdx.x; j < a; j += blockDim.x) s += V[j + (size_t)a * l] * x[fo + act_dof[j]];
-    const double tot = block_sum_t0(s, red);
-    if (threadIdx.x == 0) tmp[l] = tot;
-  }
-  __syncthreads();
+  const int l = blockIdx.x;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < a; j += blockDim.x) s += V[j + (size_t)a * l] * x[fo + act_dof[j]];
+  const double tot = block_sum_t0(s, red);
+  if (threadIdx.x == 0) vt[l] = tot;
+}
+__global__ __launch_bounds__(512) void k_backsub(const double* P, const double* U, const double* b, double* x,
+                                                 int qo, int qm, int k, double* tmp) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   for (int i = threadIdx.x; i < qm; i += blockDim.x) {
     double s = b[qo + i];
     for (int l = 0; l < k; ++l) s -= U[i + (size_t)qm * l] * tmp[l];
     tmp[k + i] = s;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < qm; i += blockDim.x) {
+  for (int i = wave; i < qm; i += nw) {
     double s = 0.0;
-    for (int m = 0; m < qm; ++m) s += P[(size_t)i * qm + m] * tmp[k + m];
-    x[qo + i] = s;
+    for (int m = lane; m < qm; m += 64) s += P[(size_t)i * qm + m] * tmp[k + m];
+    s = wave_sum(s);
+    if (lane == 0) x[qo + i] = s;
   }
 }
 
@@ -699,26 +763,26 @@ __global__ void k_full_residual_fem(const double* yf, const double* E, const int
   }
 }
 
-__global__ __launch_bounds__(256) void k_full_residual_final(const double* E, const double* A, const int* act_dof,
-                                                             int a, int qm, const double* b, const double* x,
-                                                             int fo, int qo, const double* part, int nparts,
-                                                             double* out) {
+// ABD rows of b - H x, one workgroup per row: partial (r^2, b^2, b.x)
+__global__ __launch_bounds__(256) void k_full_residual_abd(const double* E, const double* A, const int* act_dof,
+                                                           int a, int qm, const double* b, const double* x, int fo,
+                                                           int qo, double* part) {
   __shared__ double red[256 / 64];
-  double rr = 0.0, bb = 0.0, bx = 0.0;
-  for (int i = threadIdx.x; i < qm; i += blockDim.x) {
-    double hx = 0.0;
-    for (int j = 0; j < a; ++j) hx += E[(size_t)i * a + j] * x[fo + act_dof[j]];
-    for (int m = 0; m < qm; ++m) hx += A[(size_t)i * qm + m] * x[qo + m];
-    const double bi = b[qo + i];
-    rr += (bi - hx) * (bi - hx);
-    bb += bi * bi;
-    bx += bi * x[qo + i];
-  }
-  const double s0 = block_sum_t0(rr, red);
-  const double s1 = block_sum_t0(bb, red);
-  const double s2 = block_sum_t0(bx, red);
+  const int i = blockIdx.x;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < a; j += blockDim.x) s += E[(size_t)i * a + j] * x[fo + act_dof[j]];
+  for (int m = threadIdx.x; m < qm; m += blockDim.x) s += A[(size_t)i * qm + m] * x[qo + m];
+  const double hx = block_sum_t0(s, red);
   if (threadIdx.x == 0) {
-    double R = s0, B = s1, X = s2;
+    const double bi = b[qo + i];
+    part[3 * i] = (bi - hx) * (bi - hx);
+    part[3 * i + 1] = bi * bi;
+    part[3 * i + 2] = bi * x[qo + i];
+  }
+}
+__global__ void k_full_residual_final(const double* part, int nparts, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double R = 0.0, B = 0.0, X = 0.0;
     for (int p = 0; p < nparts; ++p) {
       R += part[3 * p];
       B += part[3 * p + 1];
@@ -777,6 +841,13 @@ hipError_t launch_abd_inverse(const AbdParams& S, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_coupling_core(const FactorParams& S, hipStream_t st) {
+  const int n = S.qm, k = S.k;
+  // W1 = P U  (P row-major n x n; U column-major n x k: U[m + n*l])
+  hipLaunchKernelGGL(k_gemm_small, dim3((k + 15) / 16, (n + 15) / 16), dim3(256), 0, st, S.P, n, 0, S.U, 1, n,
+                     S.W1, k, n, k, n);
+  // C = U^T W1 (column-major U read as row-major U^T with lda = n; W1 row-major n x k)
+  hipLaunchKernelGGL(k_gemm_small, dim3((k + 15) / 16, (k + 15) / 16), dim3(256), 0, st, S.U, n, 0, S.W1, k, 1,
+                     S.C, k, k, k, n);
   hipLaunchKernelGGL(k_coupling_core, dim3(1), dim3(1024), 0, st, S);
   return hipGetLastError();
 }
@@ -788,7 +859,7 @@ hipError_t launch_make_W(const double* V, const double* Lm, int a, int k, double
 hipError_t launch_rhs(const double* P, const double* U, const double* Lm, const double* b, int qo, int qm, int fo,
                       int n, int k, const int* act_node, const double* W64, double* tmp, double* y, double* rhs,
                       double* r, double* x, hipStream_t st) {
-  if (k > 0) hipLaunchKernelGGL(k_rhs_small, dim3(1), dim3(256), 0, st, P, U, Lm, b, qo, qm, k, tmp, y);
+  if (k > 0) hipLaunchKernelGGL(k_rhs_small, dim3(1), dim3(512), 0, st, P, U, Lm, b, qo, qm, k, tmp, y);
   if (n > 0)
     hipLaunchKernelGGL(k_rhs_vec, dim3(cdiv(n, 256)), dim3(256), 0, st, b, fo, n, act_node, W64, k, y, rhs, r, x);
   return hipGetLastError();
@@ -796,7 +867,8 @@ hipError_t launch_rhs(const double* P, const double* U, const double* Lm, const 
 hipError_t launch_backsub(const double* P, const double* U, const double* V, const int* act_dof, const double* b,
                           double* x, int qo, int qm, int fo, int a, int k, double* tmp, hipStream_t st) {
   if (qm == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(256), 0, st, P, U, V, act_dof, b, x, qo, qm, fo, a, k, tmp);
+  if (k > 0) hipLaunchKernelGGL(k_backsub_vt, dim3(k), dim3(256), 0, st, V, act_dof, x, fo, a, tmp);
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(512), 0, st, P, U, b, x, qo, qm, k, tmp);
   return hipGetLastError();
 }
 hipError_t launch_full_residual(const double* yf, const double* E, const double* A, const int* act_node,
@@ -806,8 +878,10 @@ hipError_t launch_full_residual(const double* yf, const double* E, const double*
   if (nparts > 0)
     hipLaunchKernelGGL(k_full_residual_fem, dim3(nparts), dim3(kBlock), 0, st, yf, E, act_node, a, qm, b, x, fo,
                        qo, n, part);
-  hipLaunchKernelGGL(k_full_residual_final, dim3(1), dim3(256), 0, st, E, A, act_dof, a, qm, b, x, fo, qo, part,
-                     nparts, out);
+  if (qm > 0)
+    hipLaunchKernelGGL(k_full_residual_abd, dim3(qm), dim3(256), 0, st, E, A, act_dof, a, qm, b, x, fo, qo,
+                       part + 3 * nparts);
+  hipLaunchKernelGGL(k_full_residual_final, dim3(1), dim3(64), 0, st, part, nparts + qm, out);
   return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -111,7 +112,7 @@ struct hsc_handle {
   Buf A, E, R0, R1, U, V, hist, pivots, acapart, Binv, P, s1, s2, s3, W1, C, Lm, W64, W32;
   Buf rhs, r, p0, p1, z0, z1, q, q64, yfull, small, ybuf, misc, ints;
   // persistent-launch scratch
-  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, respart;
+  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, respart, profbuf;
   // learned preconditioner: graph MLP over D's node graph
   Buf learned, nbr_ptr, nbr, dfeat, sdiag, xfeat, omlp, ptmp, dotbuf;
   long long learned_bytes = 0;
@@ -294,7 +295,7 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   HIPCHK(h0->partE.ensure(sizeof(double) * 2 * (size_t)nsys * nF));
   HIPCHK(h0->partF.ensure(sizeof(double) * (size_t)nsys * nF));
   HIPCHK(h0->partL.ensure(sizeof(double) * (size_t)std::max(lrp, 1)));
-  HIPCHK(h0->barrier.ensure(64));
+  HIPCHK(h0->barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
   HIPCHK(h0->errflag.ensure(64));
   std::vector<int> ch(3 * nwg);
   std::copy(chA.begin(), chA.end(), ch.begin());
@@ -302,7 +303,7 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   std::copy(chL.begin(), chL.end(), ch.begin() + 2 * nwg);
   HIPCHK(hipMemcpyAsync(h0->sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(h0->chunks.p, ch.data(), sizeof(int) * 3 * nwg, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(h0->barrier.p, 0, 64, st));
+  HIPCHK(hipMemsetAsync(h0->barrier.p, 0, sizeof(unsigned) * hsc::kBarrierWords, st));
   HIPCHK(hipMemsetAsync(h0->errflag.p, 0, 64, st));
   // p_old of the first iteration is multiplied by beta = 0: it must be finite
   for (int s = 0; s < nsys; ++s) {
@@ -336,6 +337,13 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   P.error = h0->errflag.as<int>();
   P.timeout_ticks = kTimeoutTicks;
   P.t_total = t_total;
+  P.prof = nullptr;
+  static const bool stage_timing = std::getenv("HSC_STAGE_TIMING") != nullptr;
+  if (stage_timing) {
+    HIPCHK(h0->profbuf.ensure(sizeof(long long) * 32 * nwg));
+    HIPCHK(hipMemsetAsync(h0->profbuf.p, 0, sizeof(long long) * 32 * nwg, st));
+    P.prof = h0->profbuf.as<long long>();
+  }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -358,13 +366,25 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   if (ce != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("PCG: ") + hipGetErrorString(ce)};
   if (derr) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "persistent PCG: bounded grid wait expired"};
   if (pcg_ms) *pcg_ms = ms;
+  if (P.prof && max_iter_override <= 0) {
+    std::vector<long long> pr((size_t)32 * nwg);
+    HIPCHK(hipMemcpy(pr.data(), P.prof, sizeof(long long) * pr.size(), hipMemcpyDeviceToHost));
+    const char* names[6] = {"init", "neumann", "A", "B", "T1", "T2"};
+    std::fprintf(stderr, "[hsc stage timing] G=%d nwg=%d nL=%d iters=%d pcg_ms=%.3f (us per barrier site: compute / wait; wg0 | wg%d)\n",
+                 G, nwg, nL, out[0].iterations, ms, nwg - 1);
+    for (int si = 0; si < 6; ++si) {
+      const long long* a0 = pr.data();
+      const long long* a1 = pr.data() + (size_t)32 * (nwg - 1);
+      if (!a0[20 + si]) continue;
+      std::fprintf(stderr, "  %-8s n=%6lld  %8.2f / %8.2f | %8.2f / %8.2f\n", names[si], a0[20 + si],
+                   a0[2 * si] / 100.0 / a0[20 + si], a0[2 * si + 1] / 100.0 / a0[20 + si],
+                   a1[2 * si] / 100.0 / a1[20 + si], a1[2 * si + 1] / 100.0 / a1[20 + si]);
+    }
+  }
   return out;
 }
 
-LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
-  hsc_handle* h0 = hs[0];
-  const bool mixed = h0->cfg.precision == HSC_PRECISION_MIXED;
-  if (h0->cfg.lanes_per_row && h0->cfg.wg_per_cu) return {h0->cfg.lanes_per_row, h0->cfg.wg_per_cu};
+TuneKey tune_key(hsc_handle* const* hs, int nsys) {
   long long nbt = 0, nnz = 0;
   int kmax = 0;
   for (int s = 0; s < nsys; ++s) {
@@ -372,8 +392,22 @@ LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
     nnz += hs[s]->nnzb;
     kmax = std::max(kmax, hs[s]->k);
   }
-  TuneKey key{(int)std::log2((double)std::max(nbt, 1LL)), (int)(nnz / std::max(nbt, 1LL)), kmax / 16, mixed ? 1 : 0,
-              nsys > 1 ? 1 : 0};
+  const bool mixed = hs[0]->cfg.precision == HSC_PRECISION_MIXED;
+  return TuneKey{(int)std::log2((double)std::max(nbt, 1LL)), (int)(nnz / std::max(nbt, 1LL)), kmax / 16,
+                 mixed ? 1 : 0, nsys > 1 ? 1 : 0};
+}
+
+bool tune_cached(hsc_handle* const* hs, int nsys) {
+  if (hs[0]->cfg.lanes_per_row && hs[0]->cfg.wg_per_cu) return true;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  return g_tune.count(tune_key(hs, nsys)) > 0;
+}
+
+LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
+  hsc_handle* h0 = hs[0];
+  const bool mixed = h0->cfg.precision == HSC_PRECISION_MIXED;
+  if (h0->cfg.lanes_per_row && h0->cfg.wg_per_cu) return {h0->cfg.lanes_per_row, h0->cfg.wg_per_cu};
+  const TuneKey key = tune_key(hs, nsys);
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(key);
@@ -504,7 +538,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->small.ensure(sizeof(double) * (qm + 2 * lim1 + 64)));
   HIPCHK(h->ybuf.ensure(sizeof(double) * lim1));
   HIPCHK(h->misc.ensure(sizeof(double) * 64));
-  HIPCHK(h->respart.ensure(sizeof(double) * 3 * ((fn + hsc::kBlock - 1) / hsc::kBlock + 1)));
+  HIPCHK(h->respart.ensure(sizeof(double) * 3 * ((fn + hsc::kBlock - 1) / hsc::kBlock + qm + 1)));
   if (a > 0) HIPCHK(hipMemsetAsync(h->E.p, 0, sizeof(double) * (size_t)qm * a, st));
   HIPCHK(hsc::launch_active(h->active_flag.as<int>(), h->act_prefix.as<int>(), nb, h->act_node.as<int>(),
                             h->act_dof.as<int>(), st));
@@ -523,12 +557,14 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     HIPCHK(hsc::aca_occupancy(&occ));
     const DeviceInfo di = device_info(h->device);
     const long long total = (long long)qm * a;
-    int nwg = (int)std::min<long long>((long long)std::max(occ, 1) * di.cu, std::max<long long>(1, (total + 1023) / 1024));
+    // one workgroup per CU at most: each ACA step is one grid barrier plus a
+    // pass over E, so fewer, fuller workgroups are cheaper
+    int nwg = (int)std::min<long long>(di.cu, std::max<long long>(1, (total + 2047) / 2048));
     nwg = std::max(1, std::min(nwg, std::max(occ, 1) * di.cu));
     HIPCHK(h->acapart.ensure(sizeof(double) * 8 * nwg));  // two halves, see k_aca
-    HIPCHK(h->barrier.ensure(64));
+    HIPCHK(h->barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
     HIPCHK(h->errflag.ensure(64));
-    HIPCHK(hipMemsetAsync(h->barrier.p, 0, 64, st));
+    HIPCHK(hipMemsetAsync(h->barrier.p, 0, sizeof(unsigned) * hsc::kBarrierWords, st));
     HIPCHK(hipMemsetAsync(h->errflag.p, 0, 64, st));
     hsc::AcaParams ap{};
     ap.E = h->E.as<double>();
@@ -825,15 +861,17 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
     s->total_ms = ms_since(t0);
     return code;
   }
-  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
-                         h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
-                         h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
   hsc_handle* hs[1] = {h};
   h->xsol = x + h->fo;
+  if (!tune_cached(hs, 1)) {
+    // the tuning runs start from the real right-hand side
+    HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
+                           h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
+                           h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
+  }
   LaunchCfg lc = tuned_config(hs, 1, st);
   h->G = lc.G;
   h->wgpcu = lc.wgpcu;
-  // tuning runs overwrite r: re-derive it
   HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
                          h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
                          h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
