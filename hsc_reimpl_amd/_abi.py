@@ -95,6 +95,21 @@ SIGNATURES = {
     "hsc_get_factors": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),
     "hsc_set_learned_weights": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "hsc_solve_traced": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                     C.c_int32, C.POINTER(C.c_int32), C.POINTER(Stats), C.c_void_p]),
+    "hsc_get_node_graph": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
+    "hsc_get_node_features": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hsc_mlp_param_count": (C.c_int32, []),
+    "hsc_mlp_create": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                   C.POINTER(C.c_void_p)]),
+    "hsc_mlp_destroy": (None, [C.c_void_p]),
+    "hsc_mlp_set_params": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "hsc_mlp_get_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "hsc_mlp_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "hsc_mlp_gradients": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                      C.POINTER(C.c_float), C.c_void_p]),
+    "hsc_mlp_train_step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float,
+                                       C.POINTER(C.c_float), C.c_void_p]),
 }
 
 _lib = None
