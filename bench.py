@@ -154,7 +154,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "pcg_persistent<G, float> (whole Schur PCG, one cooperative launch)",
-                     "bytes_per_iteration": bpi},
+                     "bytes_per_iteration": bpi, "iterations": new["pcg_iterations"],
+                     "kernel_ms": round(new["pcg_ms"], 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -698,9 +698,11 @@ int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
     HIPCHK(hsc::launch_op_lowrank64(S, in, out, h->ybuf.as<double>(), st));
   };
   int fallbacks = 0;
+  bool corrected = false;  // the current z carries a learned correction
   // z = M(r) (+ learned correction); returns r.z
   auto precondition = [&](double rr) {
     HIPCHK(hsc::launch_op_precond64(G, S, h->cfg.neumann_order, h->cfg.omega, r, z, tmp, st));
+    corrected = false;
     if (!o.learned) return dot(r, z);
     HIPCHK(hipMemcpyAsync(tmp, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     const double rho = std::sqrt(rr / n) > 0 ? std::sqrt(rr / n) : 1.0;
@@ -713,6 +715,7 @@ int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
       HIPCHK(hipMemcpyAsync(z, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
       return dot(r, z);
     }
+    corrected = true;
     return rz;
   };
   if (o.learned) ensure_mlp(h, st);
@@ -724,8 +727,10 @@ int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   double rel = normb > 0 ? 1.0 : 0.0;
   int it = 0, restarts = 0, checks = 0, slots = 0, status = HSC_OK;
   const double tol = h->cfg.tol_rate;
+  bool prev_corrected = false;
   if (normb > 0) {
     double rz = precondition(nb2);
+    prev_corrected = corrected;
     HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     while (it < h->cfg.max_iter && rel > tol) {
       schur(p, Ap);
@@ -760,9 +765,12 @@ int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
       const double rzn = precondition(rr);
       double beta = 0.0;
       if (!restart) {
-        if (o.learned) beta = (rzn - dot(r, zold)) / rz;  // Polak-Ribiere (flexible)
+        // Polak-Ribiere (flexible) whenever the preconditioner varied; with
+        // every correction rejected this is exactly the plain PCG recurrence
+        if (corrected || prev_corrected) beta = (rzn - dot(r, zold)) / rz;
         else beta = rzn / rz;
       }
+      prev_corrected = corrected;
       HIPCHK(hsc::launch_axpby(p, z, 1.0, beta, n, st));
       rz = rzn;
     }
