@@ -99,6 +99,8 @@ SIGNATURES = {
                                      C.c_int32, C.POINTER(C.c_int32), C.POINTER(Stats), C.c_void_p]),
     "hsc_get_node_graph": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
     "hsc_get_node_features": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hsc_energy_targets": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]),
     "hsc_mlp_param_count": (C.c_int32, []),
     "hsc_mlp_create": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
                                    C.POINTER(C.c_void_p)]),

@@ -41,6 +41,10 @@ hipError_t launch_mlp_adam(float* theta, const float* grad, float* m, float* v, 
                            const unsigned char* frozen, hipStream_t st);
 hipError_t launch_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt, hipStream_t st);
 hipError_t launch_mlp_features(const float* Dfeat, const double* r, double rho, int nb, bf16_t* X, hipStream_t st);
+hipError_t launch_energy_w(const float* O, const float* sdiag, double rho, const double* U, int nb, double* w,
+                           hipStream_t st);
+hipError_t launch_energy_T(const float* O, const float* sdiag, double rho, const double* Sw, double inv_den, int nb,
+                           float* T, hipStream_t st);
 hipError_t launch_mlp_apply(const float* O, const float* sdiag, double rho, int nb, double* z, hipStream_t st);
 
 }  // namespace hsc

@@ -229,7 +229,35 @@ __global__ void k_mlp_apply(const float* O, const float* sdiag, double rho, int 
   for (int c = 0; c < 3; ++c) z[3 * i + c] += sc * (double)O[(size_t)i * 3 + c];
 }
 
+// energy-norm training signal: w = U - (rho / s_i) o_i
+__global__ void k_energy_w(const float* O, const float* sdiag, double rho, const double* U, int nb, double* w) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * nb) return;
+  const double o = O ? (double)O[e] : 0.0;
+  w[e] = U[e] - rho / (double)sdiag[e / 3] * o;
+}
+
+// T = o + (rho / s_i) (S w)_i / den
+__global__ void k_energy_T(const float* O, const float* sdiag, double rho, const double* Sw, double inv_den, int nb,
+                           float* T) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * nb) return;
+  const double o = O ? (double)O[e] : 0.0;
+  T[e] = (float)(o + rho / (double)sdiag[e / 3] * Sw[e] * inv_den);
+}
+
 static inline int cdivl(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+hipError_t launch_energy_w(const float* O, const float* sdiag, double rho, const double* U, int nb, double* w,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_energy_w, dim3(cdivl(3LL * nb, 256)), dim3(256), 0, st, O, sdiag, rho, U, nb, w);
+  return hipGetLastError();
+}
+hipError_t launch_energy_T(const float* O, const float* sdiag, double rho, const double* Sw, double inv_den, int nb,
+                           float* T, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy_T, dim3(cdivl(3LL * nb, 256)), dim3(256), 0, st, O, sdiag, rho, Sw, inv_den, nb, T);
+  return hipGetLastError();
+}
 
 hipError_t launch_mlp_gemm(const MlpGemm& g, hipStream_t st) {
   if (g.M == 0) return hipSuccess;

@@ -114,7 +114,7 @@ struct hsc_handle {
   // persistent-launch scratch
   Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, respart, profbuf;
   // learned preconditioner: graph MLP over D's node graph
-  Buf learned, nbr_ptr, nbr, dfeat, sdiag, xfeat, omlp, ptmp, dotbuf;
+  Buf learned, nbr_ptr, nbr, dfeat, sdiag, xfeat, omlp, ptmp, dotbuf, ebuf;
   long long learned_bytes = 0;
   std::unique_ptr<hsc::MlpNet> mlp;
   int mlp_nb = -1;
@@ -1249,6 +1249,36 @@ int32_t hsc_get_node_features(hsc_handle* h, float* dfeat, float* sdiag) {
     ensure_graph(h, nullptr);
     HIPCHK(hipMemcpy(dfeat, h->dfeat.p, sizeof(float) * 9 * h->nb, hipMemcpyDefault));
     HIPCHK(hipMemcpy(sdiag, h->sdiag.p, sizeof(float) * h->nb, hipMemcpyDefault));
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_energy_targets(hsc_handle* h, const float* O, const double* U, const double* rho, const double* den,
+                           int32_t B, float* T, double* energy, hsc_stream stream) {
+  if (!h || !U || !rho || !energy || B <= 0 || (T && !den)) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    hipStream_t st = (hipStream_t)stream;
+    const int n = h->fn, nb = h->nb;
+    ensure_graph(h, st);
+    HIPCHK(h->ebuf.ensure(sizeof(double) * (2 * (size_t)n + B)));
+    HIPCHK(h->dotbuf.ensure(sizeof(double) * 300));
+    double* w = h->ebuf.as<double>();
+    double* Sw = w + n;
+    double* en = Sw + n;
+    hsc::SysDesc S = make_desc(h);
+    for (int b = 0; b < B; ++b) {
+      if (T && !(den[b] > 0)) throw HscError{HSC_ERR_INVALID_ARGUMENT, "energy targets: den must be positive"};
+      const float* Ob = O ? O + (size_t)b * n : nullptr;
+      HIPCHK(hsc::launch_energy_w(Ob, h->sdiag.as<float>(), rho[b], U + (size_t)b * n, nb, w, st));
+      HIPCHK(hsc::launch_op_Dx64(h->G, S, w, Sw, st));
+      HIPCHK(hsc::launch_op_lowrank64(S, w, Sw, h->ybuf.as<double>(), st));
+      HIPCHK(hsc::launch_dot(w, Sw, n, h->dotbuf.as<double>(), en + b, st));
+      if (T) HIPCHK(hsc::launch_energy_T(Ob, h->sdiag.as<float>(), rho[b], Sw, 1.0 / den[b], nb, T + (size_t)b * n, st));
+    }
+    HIPCHK(hipMemcpyAsync(energy, en, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return HSC_OK;
   });
 }

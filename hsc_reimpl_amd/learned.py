@@ -10,9 +10,13 @@ direction the plain Neumann z is used for that iteration (the fallback).
 Everything numeric runs in the HIP library through the C ABI:
   * harvesting: hsc_solve_traced stores (r_k, x_k) of the plain solver's own
     PCG iterations; e_k = x* - x_k;
+  * training signal: hsc_energy_targets (energy norm |e - z|_S^2, S applied
+    by the solver's own Schur operator);
   * training: hsc_mlp_train_step (hand-written BF16 forward/backward + Adam);
   * inference: hsc_set_learned_weights + preconditioner = "learned".
 torch tensors are only the device buffers.
+
+Train the committed weights with ``python -m hsc_reimpl_amd.learned``.
 """
 from __future__ import annotations
 
@@ -56,17 +60,24 @@ def init_params(seed=0) -> np.ndarray:
     return theta
 
 
+# flat parameter layout of the library (row-major matrices, Wo/bo padded to 32 outputs)
+MLP_SHAPES = [("W0", (16, 64)), ("b0", (64,)), ("Ws1", (64, 64)), ("Wn1", (64, 64)), ("b1", (64,)),
+              ("Ws2", (64, 64)), ("Wn2", (64, 64)), ("b2", (64,)), ("Wo", (64, 32)), ("bo", (32,))]
+MLP_OFFSETS = {}
+_o = 0
+for _name, _shp in MLP_SHAPES:
+    MLP_OFFSETS[_name] = _o
+    _o += int(np.prod(_shp))
+MLP_COUNT = _o
+
+
 def to_oracle_params(theta: np.ndarray) -> dict:
     """Unpack the flat layout into the oracle's named parameters (Wo/bo
     trimmed to 3 outputs)."""
-    HID, IN, OUTP = 64, 16, 32
-    o = 0
     out = {}
-    for name, shp in [("W0", (IN, HID)), ("b0", (HID,)), ("Ws1", (HID, HID)), ("Wn1", (HID, HID)), ("b1", (HID,)),
-                      ("Ws2", (HID, HID)), ("Wn2", (HID, HID)), ("b2", (HID,)), ("Wo", (HID, OUTP)), ("bo", (OUTP,))]:
-        n = int(np.prod(shp))
-        out[name] = np.asarray(theta[o:o + n], dtype=np.float64).reshape(shp)
-        o += n
+    for name, shp in MLP_SHAPES:
+        o = MLP_OFFSETS[name]
+        out[name] = np.asarray(theta[o:o + int(np.prod(shp))], dtype=np.float64).reshape(shp)
     out["Wo"] = out["Wo"][:, :3]
     out["bo"] = out["bo"][:3]
     return out
@@ -164,11 +175,19 @@ def features(dfeat: torch.Tensor, r: torch.Tensor):
 
 
 @dataclass
-class Pairs:
-    X: torch.Tensor      # [P, nb, 16] bf16
-    T: torch.Tensor      # [P, nb, 3] fp32 targets
-    den: torch.Tensor    # [P] fp32 per-sample normalisers
-    seeds: np.ndarray    # [P] seed of each pair
+class SystemPairs:
+    """Training pairs of one system of the family (kept with its handle,
+    whose Schur operator defines the energy norm)."""
+    seed: int
+    handle: plugin.Handle
+    A: plugin.BCOO3
+    X: torch.Tensor      # [P, nb, 16] bf16 features of r
+    U: torch.Tensor      # [P, n] fp64: e - Neumann(r), the part of the error the network should add
+    rho: np.ndarray      # [P] rms(r)
+    den: np.ndarray      # [P] U^T S U (energy of the Neumann error)
+
+    def __len__(self):
+        return self.X.shape[0]
 
 
 def family_system(seed: int):
@@ -177,74 +196,102 @@ def family_system(seed: int):
     return systems.cube_system(seed=seed, E=E)
 
 
-def harvest(seeds, per_seed=200, every=1, device="cuda", log=None) -> Pairs:
-    """Residual/error pairs from the plain solver's own traced solves."""
-    Xs, Ts, dens, sd = [], [], [], []
+def energy_targets(sp: SystemPairs, idx, O=None, T=None) -> np.ndarray:
+    """Per-sample energies |U - (rho/s) o|_S^2 (and T for hsc_mlp_train_step)."""
+    idx = np.asarray(idx)
+    U = sp.U[torch.as_tensor(idx, device=sp.U.device)].contiguous()
+    rho = np.ascontiguousarray(sp.rho[idx], dtype=np.float64)
+    den = np.ascontiguousarray(sp.den[idx], dtype=np.float64) if T is not None else None
+    en = np.zeros(len(idx))
+    h = sp.handle
+    h._check(h.lib.hsc_energy_targets(
+        h.h, C.c_void_p(O.data_ptr()) if O is not None else None, C.c_void_p(U.data_ptr()),
+        rho.ctypes.data_as(C.c_void_p), den.ctypes.data_as(C.c_void_p) if den is not None else None, len(idx),
+        C.c_void_p(T.data_ptr()) if T is not None else None, en.ctypes.data_as(C.c_void_p), plugin._stream(None)))
+    return en
+
+
+def harvest(seeds, per_seed=200, device="cuda", log=None, max_rhs=4) -> list:
+    """Residual/error pairs (r_k, e_k = x* - x_k) from the plain solver's own
+    traced PCG iterations, on the system's b and further N(0,1) right-hand
+    sides until `per_seed` pairs are collected.  Pairs whose residual fell
+    below 1e-6 of the initial one are dropped (their error is dominated by
+    the solve's own tolerance)."""
     lib = _abi.load()
+    out = []
     for seed in seeds:
         s = family_system(seed)
         h = plugin.Handle(plugin.make_config(C1_SCENE, **C1_EXT))
         A = plugin.BCOO3.from_system(s, device)
         h.setup(A, s.qo, s.qm, s.fo, s.fn)
         n = s.fn
-        b = torch.as_tensor(s.b, device=device)
-        x = torch.zeros_like(b)
-        slots = 2 * per_seed
-        tr = torch.zeros((slots, n), dtype=torch.float64, device=device)
-        tx = torch.zeros((slots, n), dtype=torch.float64, device=device)
-        ns = C.c_int32()
-        st = _abi.Stats()
-        h._check(lib.hsc_solve_traced(h.h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
-                                      C.c_void_p(tr.data_ptr()), C.c_void_p(tx.data_ptr()), every, slots,
-                                      C.byref(ns), C.byref(st), plugin._stream(None)))
-        k = ns.value
-        xs = x[s.fo:s.fo + n]
-        R = tr[:k]
-        Ev = xs[None, :] - tx[:k]
-        # keep pairs whose error is well above the solve's own accuracy
-        keep = torch.linalg.vector_norm(R, dim=1) >= 1e-6 * torch.linalg.vector_norm(R[0])
-        R, Ev = R[keep], Ev[keep]
-        if R.shape[0] > per_seed:
-            idx = torch.linspace(0, R.shape[0] - 1, per_seed, device=device).round().long()
-            R, Ev = R[idx], Ev[idx]
-        dfeat, sdiag = node_features(h, device)
+        rng = np.random.default_rng(20_000 + seed)
+        Rs, Es = [], []
+        got = 0
+        for j in range(max_rhs):
+            b = torch.as_tensor(s.b if j == 0 else rng.normal(size=s.N), device=device)
+            x = torch.zeros_like(b)
+            slots = 4 * per_seed
+            tr = torch.zeros((slots, n), dtype=torch.float64, device=device)
+            tx = torch.zeros((slots, n), dtype=torch.float64, device=device)
+            ns = C.c_int32()
+            st = _abi.Stats()
+            h._check(lib.hsc_solve_traced(h.h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
+                                          C.c_void_p(tr.data_ptr()), C.c_void_p(tx.data_ptr()), 1, slots,
+                                          C.byref(ns), C.byref(st), plugin._stream(None)))
+            k = ns.value
+            R = tr[:k]
+            Ev = x[s.fo:s.fo + n][None, :] - tx[:k]
+            keep = torch.linalg.vector_norm(R, dim=1) >= 1e-6 * torch.linalg.vector_norm(R[0])
+            Rs.append(R[keep])
+            Es.append(Ev[keep])
+            got += int(keep.sum())
+            if got >= per_seed:
+                break
+        R = torch.cat(Rs)[:per_seed]
+        Ev = torch.cat(Es)[:per_seed]
+        dfeat, _ = node_features(h, device)
         zN = torch.zeros_like(R)
         for i in range(R.shape[0]):
             h.apply_precond(R[i].contiguous(), zN[i])
         X, rho = features(dfeat, R)
-        T = ((Ev - zN).view(R.shape[0], -1, 3) * (sdiag[None, :, None].double() / rho[:, None, None])).float()
-        den = torch.sum(T * T, dim=(1, 2)).clamp_min(1e-30)
-        Xs.append(X)
-        Ts.append(T)
-        dens.append(den)
-        sd.append(np.full(R.shape[0], seed))
+        sp = SystemPairs(seed, h, A, X, (Ev - zN).contiguous(), rho.cpu().numpy(), np.zeros(R.shape[0]))
+        sp.den = energy_targets(sp, np.arange(len(sp)))
+        out.append(sp)
         if log:
-            log(f"harvest seed {seed}: {R.shape[0]} pairs from {st.pcg_iterations} iterations")
-    return Pairs(torch.cat(Xs), torch.cat(Ts), torch.cat(dens), np.concatenate(sd))
+            log(f"harvest seed {seed}: {len(sp)} pairs")
+    return out
 
 
-def train(pairs: Pairs, graph, epochs=20, batch=64, lr=2e-3, seed=0, log=None, theta0=None):
-    """Train on the GPU with the library's BF16 kernels; returns (theta, history)."""
+def train(data: list, graph, epochs=20, batch=32, lr=2e-3, seed=0, log=None, theta0=None):
+    """Train on the GPU with the library's BF16 kernels, minimising the mean
+    of |e - z|_S^2 / |e - Neumann(r)|_S^2 over pairs (z = Neumann(r) plus the
+    network's correction), with Adam and a cosine learning-rate schedule.
+    Batches never mix systems.  Returns (theta, per-epoch mean loss)."""
     ptr, nbr = graph
     net = GraphMLP(ptr, nbr, batch)
     net.set_params(init_params(seed) if theta0 is None else theta0)
-    P = pairs.X.shape[0]
-    g = torch.Generator(device="cpu").manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    batches = [(j, np.arange(i, min(i + batch, len(sp)))) for j, sp in enumerate(data)
+               for i in range(0, len(sp), batch)]
+    ones = torch.ones(batch, dtype=torch.float32, device=data[0].X.device)
     hist = []
     for ep in range(epochs):
-        perm = torch.randperm(P, generator=g)
         lr_ep = lr * 0.5 * (1 + np.cos(np.pi * ep / epochs))
-        tot, nbat = 0.0, 0
+        tot = 0.0
         t0 = time.perf_counter()
-        for s in range(0, P - batch + 1, batch):
-            idx = perm[s:s + batch].to(pairs.X.device)
-            loss = net.train_step(pairs.X[idx].contiguous(), pairs.T[idx].contiguous(),
-                                  pairs.den[idx].contiguous(), lr_ep)
-            tot += loss
-            nbat += 1
-        hist.append(tot / max(nbat, 1))
+        for bi in rng.permutation(len(batches)):
+            j, idx = batches[bi]
+            sp = data[j]
+            X = sp.X[torch.as_tensor(idx, device=sp.X.device)].contiguous()
+            O = net.forward(X)
+            T = torch.empty_like(O)
+            en = energy_targets(sp, idx, O, T)
+            tot += float(np.mean(en / sp.den[idx]))
+            net.train_step(X, T, ones[:len(idx)], lr_ep)
+        hist.append(tot / len(batches))
         if log:
-            log(f"epoch {ep}: loss {hist[-1]:.4f} ({time.perf_counter() - t0:.2f} s)")
+            log(f"epoch {ep}: energy loss {hist[-1]:.4f} ({time.perf_counter() - t0:.2f} s)")
     return net.get_params(), hist
 
 
@@ -290,3 +337,40 @@ def save_weights(theta, path=WEIGHTS, meta=None):
 
 def load_weights(path=WEIGHTS) -> np.ndarray:
     return np.load(path)["theta"]
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="harvest, train and evaluate the learned preconditioner")
+    ap.add_argument("--seeds", type=int, nargs=2, default=[100, 200], help="training seed range [a, b)")
+    ap.add_argument("--per-seed", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--eval-seeds", type=int, nargs=2, default=[200, 204])
+    ap.add_argument("--out", default=str(WEIGHTS))
+    a = ap.parse_args(argv)
+
+    def log(m):
+        print(m, flush=True)
+
+    t0 = time.perf_counter()
+    data = harvest(range(*a.seeds), per_seed=a.per_seed, log=log)
+    npairs = sum(len(sp) for sp in data)
+    log(f"harvested {npairs} pairs in {time.perf_counter() - t0:.1f} s")
+    graph = node_graph(data[0].handle)
+    t0 = time.perf_counter()
+    theta, hist = train(data, graph, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    log(f"trained in {time.perf_counter() - t0:.1f} s")
+    rows = evaluate(theta, list(range(*a.eval_seeds)) + ["c1"])
+    for r in rows:
+        log(json.dumps(r))
+    save_weights(theta, Path(a.out), meta={"seeds": np.array(a.seeds), "pairs": np.array(npairs),
+                                           "epochs": np.array(a.epochs), "loss": np.array(hist),
+                                           "evaluation": np.array(json.dumps(rows))})
+    log(f"saved {a.out}")
+
+
+if __name__ == "__main__":
+    main()

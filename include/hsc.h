@@ -203,6 +203,15 @@ int32_t hsc_solve_traced(hsc_handle* h, const double* b, double* x, double* trac
 int32_t hsc_get_node_graph(hsc_handle* h, int32_t* nbr_ptr, int32_t* nbr, int64_t* n_edges);
 int32_t hsc_get_node_features(hsc_handle* h, float* dfeat, float* sdiag);
 
+/* Energy-norm training signal of the learned correction.  For sample b
+ * (U: [B x fem_dofs] = e - Neumann(r) of training pair b; O: [B x fem_dofs]
+ * fp32 network outputs or null for zero; rho, den: host arrays of B):
+ *   w_b = U_b - (rho_b / s_i) o_b,  energy_b = w_b^T S w_b  (host out),
+ *   T_b = o_b + (rho_b / s_i) (S w_b)_i / den_b  (device out, if T != null),
+ * so that hsc_mlp_train_step(X, T, den = 1) descends mean_b energy_b / den_b. */
+int32_t hsc_energy_targets(hsc_handle* h, const float* O, const double* U, const double* rho, const double* den,
+                           int32_t B, float* T, double* energy, hsc_stream stream);
+
 /* The network object used for training on the GPU (hand-written BF16
  * forward/backward kernels + Adam).  X: [B x nb x 16] bf16 features
  * (D_ii/s_i, r_i/rms(r), 0 x 4); T: [B x nb x 3] fp32 targets; den: [B]
