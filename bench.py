@@ -56,19 +56,115 @@ def pcg_bytes_per_iteration(nb, nnzb, a, k, order, mixed):
     return neu + stage_a + lowrank + stage_b
 
 
-def cpu_baseline(sysm, order, omega, rtol):
-    """Oracle (sparse NumPy/SciPy restatement of the reference algorithm) on
-    one full solve of the same system, single thread."""
+def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
+    """Oracle (sparse NumPy/SciPy restatement of the reference algorithm):
+    full solves (setup + PCG + back-substitution) of the same system, single
+    thread, repeated until about `budget_s` of CPU work; median rate."""
     from oracle import sparse as osp
-    t0 = time.perf_counter()
-    ref = osp.SparseHSC.from_system(sysm, order=order, omega=omega)
+    import numpy as np
     n = sysm.fn
-    _, _, st = ref.solve(sysm.b[:n], sysm.b[n:], rtol=rtol, maxiter=20000)
-    el = time.perf_counter() - t0
-    return {"value": 1.0 / el, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"one full solve (setup + {st['iterations']} PCG iterations) of the same 78k-tet system, "
-                      f"{el:.2f} s, SciPy sparse matvecs single-threaded",
-            "iterations": st["iterations"], "cpu": platform.processor() or platform.machine()}
+    times, iters = [], 0
+    while len(times) < max_solves and sum(times) < budget_s:
+        t0 = time.perf_counter()
+        ref = osp.SparseHSC.from_system(sysm, order=order, omega=omega)
+        _, _, st = ref.solve(sysm.b[:n], sysm.b[n:], rtol=rtol, maxiter=20000)
+        times.append(time.perf_counter() - t0)
+        iters = st["iterations"]
+    med = float(np.median(times))
+    return {"value": 1.0 / med, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"{len(times)} full solves (setup + {iters} PCG iterations each) of the same 78k-tet system, "
+                      f"median {med:.2f} s, {sum(times):.1f} s total; SciPy sparse matvecs, BLAS pinned to 1 thread",
+            "iterations": iters, "cpu": platform.processor() or platform.machine()}
+
+
+def run_batch64(args, dev, order, omega, rtol):
+    """64 uneven systems (seed 2: 1-16 ABD bodies, 1-8 tori of 2k-15k tets,
+    ranks 4-16), FP32 bulk / FP64 reductions, rtol 1e-8 each: one captured
+    hipGraph replay (right-hand sides, one persistent PCG launch over all 64,
+    back-substitution and acceptance) vs the same systems solved one at a
+    time.  A step = one solve of all 64 systems (setup excluded: the systems
+    are set up once, as a batch of Newton systems would be between solves)."""
+    import numpy as np
+    import torch
+    from hsc_reimpl_amd import plugin, systems
+
+    t0 = time.perf_counter()
+    syss = systems.batch_systems(seed=2, progress=lambda i, s: (i % 8 == 7) and print(
+        f"[batch64] generated {i + 1} systems ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True))
+    scene = {"linear_system/tol_rate": rtol, "linear_system/hsc/neumann_order": order,
+             "linear_system/hsc/omega": omega, "linear_system/hsc/max_iter": 20000}
+    cfg = dict(precision="mixed", residual_replace_every=25)
+    hs, As, bs = [], [], []
+    t0 = time.perf_counter()
+    for s in syss:
+        h = plugin.Handle(plugin.make_config(scene, **cfg))
+        A = plugin.BCOO3.from_system(s, dev)
+        h.setup(A, s.qo, s.qm, s.fo, s.fn)
+        hs.append(h)
+        As.append(A)
+        bs.append(torch.as_tensor(s.b, device=dev))
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    print(f"[batch64] set up 64 systems in {setup_s:.1f} s", file=sys.stderr, flush=True)
+    # one at a time
+    xs = [torch.zeros_like(b) for b in bs]
+    for h, b, x in zip(hs, bs, xs):
+        h.solve(b, x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ind = [h.solve(b, x) for h, b, x in zip(hs, bs, xs)]
+    torch.cuda.synchronize()
+    t_ind = (time.perf_counter() - t0) / args.steps
+    print(f"[batch64] one at a time: {1e3 * t_ind:.1f} ms per pass", file=sys.stderr, flush=True)
+    # captured batch
+    xg = [torch.zeros_like(b) for b in bs]
+    batch = plugin.Batch(hs, bs, xg)
+    for _ in range(args.warmup):
+        batch.launch()
+        batch.stats()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    replay_ms = []
+    for _ in range(args.steps):
+        e0.record()
+        batch.launch()
+        e1.record()
+        st = batch.stats()
+        replay_ms.append(e0.elapsed_time(e1))
+    torch.cuda.synchronize()
+    t_b = (time.perf_counter() - t0) / args.steps
+    ok = all(d["status"] == 0 for d in st) and all(d["status"] == 0 for d in ind)
+    diff = max(float((torch.linalg.vector_norm(a - c) / torch.linalg.vector_norm(c)).item()) for a, c in zip(xg, xs))
+    byts = 0
+    for h, d in zip(hs, st):
+        i = h.setup_info()
+        byts += pcg_bytes_per_iteration(i["fem_nodes"], i["nnzb"], i["active_fem_dofs"], i["rank"], order,
+                                        mixed=True) * d["pcg_iterations"]
+    iters = [d["pcg_iterations"] for d in st]
+    med_ms = float(np.median(replay_ms))
+    achieved = byts / (med_ms * 1e-3) / 1e9
+    line = {
+        "metric": "batched HSC: systems/s @ rtol 1e-8, 64 uneven systems (seed 2), captured batch vs one at a time",
+        "value": round(64 / t_b, 2), "unit": "systems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_b, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (seeded generator; no external data)",
+        "config": {"workload": "64 uneven HSC systems, one captured persistent launch",
+                   "tets": int(sum(s.info["tets"] for s in syss)), "dofs": int(sum(s.N for s in syss)),
+                   "nnzb": int(sum(h.setup_info()["nnzb"] for h in hs)), "neumann_order": order, "omega": omega,
+                   "rtol": rtol, **batch.info()},
+        "one_at_a_time": {"systems_per_s": round(64 / t_ind, 2), "ms_per_pass": round(1e3 * t_ind, 3)},
+        "speedup_vs_one_at_a_time": round(t_ind / t_b, 3),
+        "all_converged": ok, "max_rel_diff_vs_one_at_a_time": diff,
+        "pcg_iterations": {"min": int(min(iters)), "median": int(np.median(iters)), "max": int(max(iters))},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "captured batch replay (rhs + pcg_persistent<G, float> over 64 systems + finish)",
+                     "bytes_per_replay": int(byts), "replay_ms": round(med_ms, 3)},
+        "setup_s_all_64": round(setup_s, 2),
+    }
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -77,12 +173,22 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one mixed solve with the launch shape pinned (no tuner launches), then a 256 MiB "
+                         "device copy: the program profiled by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--workload", choices=["78k-tet", "batch64"], default="78k-tet",
+                    help="78k-tet: the headline single-system solve; batch64: 64 uneven systems (seed 2) "
+                         "in one captured persistent launch vs one at a time")
+    ap.add_argument("--lanes-per-row", type=int, default=4)
+    ap.add_argument("--wg-per-cu", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the HSC solve of one system runs on one MI355X and no "
               "multi-GPU figure is produced (see the sharding note in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    # the CPU baseline is a single-thread measurement: pin BLAS before NumPy loads
+    for var in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ[var] = "1"
     import numpy as np
     import torch
     from hsc_reimpl_amd import plugin, systems
@@ -90,11 +196,34 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     order, omega, rtol = 5, 0.7, 1e-8
+    if args.workload == "batch64":
+        return run_batch64(args, dev, order, omega, rtol)
     sysm = systems.torus_system(seed=1)
     A = plugin.BCOO3.from_system(sysm, dev)
     b = torch.as_tensor(sysm.b, device=dev)
     scene = {"linear_system/tol_rate": rtol, "linear_system/hsc/neumann_order": order,
              "linear_system/hsc/omega": omega, "linear_system/hsc/max_iter": 20000}
+
+    if args.traffic_probe:
+        h = plugin.Handle(plugin.make_config(scene, precision="mixed", residual_replace_every=25,
+                                             lanes_per_row=args.lanes_per_row, wg_per_cu=args.wg_per_cu))
+        h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
+        x = torch.zeros_like(b)
+        st = h.solve(b, x)
+        torch.cuda.synchronize()
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        info = h.setup_info()
+        print(json.dumps({"probe": "traffic", "pcg_iterations": st["pcg_iterations"], "pcg_ms": st["pcg_ms"],
+                          "bytes_per_iteration": pcg_bytes_per_iteration(info["fem_nodes"], info["nnzb"],
+                                                                         info["active_fem_dofs"], info["rank"],
+                                                                         order, mixed=True),
+                          "copy_bytes": nbytes}), flush=True)
+        return
 
     def run(variant):
         ext = {"precision": "mixed", "residual_replace_every": 25} if variant == "mixed" else {}

@@ -88,6 +88,13 @@ SIGNATURES = {
     "hsc_append_jsonl": (C.c_int32, [C.POINTER(Stats), C.c_char_p, C.c_int32, C.c_int32]),
     "hsc_solve_batched": (C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_void_p), C.POINTER(Stats), C.c_void_p]),
+    "hsc_batch_create": (C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "hsc_batch_launch": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "hsc_batch_stats": (C.c_int32, [C.c_void_p, C.POINTER(Stats), C.c_void_p]),
+    "hsc_batch_info": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32)]),
+    "hsc_batch_destroy": (None, [C.c_void_p]),
     "hsc_apply_D": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hsc_apply_schur": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hsc_apply_precond": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -86,6 +86,9 @@ hipError_t launch_full_residual(const double* yf, const double* E, const double*
 // pcg_kernel.hip
 hipError_t launch_pcg(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st);
 hipError_t pcg_occupancy(int G, bool mixed, size_t shmem, int* blocks);
+hipError_t launch_pcg_plain(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st);
+const void* pcg_kernel_fn(int G, bool mixed);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);  // bytes % 4 == 0; a kernel, safe in graphs
 hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st);
 hipError_t launch_op_precond64(int G, const SysDesc& S, int order, double omega, const double* r, double* z,
                                double* tmp, hipStream_t st);

@@ -11,6 +11,8 @@
 // Scalars live in LDS; every workgroup reduces the per-chunk partials in the
 // same fixed order, so all of them take the same decisions and the result is
 // bitwise reproducible.  Grid barriers have bounded waits.
+#include <algorithm>
+
 #include "common.hpp"
 #include "device_util.hpp"
 #include "kernels.hpp"
@@ -850,6 +852,64 @@ static hipError_t launch_pcg_t(const PcgParams& P, size_t shmem, hipStream_t st)
   void* args[] = {(void*)&P};
   return hipLaunchCooperativeKernel((const void*)pcg_persistent<G, VT>, dim3(P.nwg), dim3(kBlock), args,
                                     shmem, st);
+}
+
+// the same kernel through an ordinary launch (used inside a captured graph,
+// whose kernel node is then marked cooperative); the grid never exceeds the
+// co-resident capacity computed from the occupancy, and every grid wait is
+// bounded
+template <int G, typename VT>
+static hipError_t launch_pcg_plain_t(const PcgParams& P, size_t shmem, hipStream_t st) {
+  hipLaunchKernelGGL((pcg_persistent<G, VT>), dim3(P.nwg), dim3(kBlock), shmem, st, P);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_u32(unsigned* p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
+  if (bytes % 4 != 0) return hipErrorInvalidValue;
+  const long long n = (long long)(bytes / 4);
+  if (n == 0) return hipSuccess;
+  const int grid = (int)std::min<long long>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_zero_u32, dim3(grid), dim3(256), 0, st, (unsigned*)p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pcg_plain(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st) {
+  if (mixed) {
+    switch (G) {
+      case 4: return launch_pcg_plain_t<4, float>(P, shmem, st);
+      case 8: return launch_pcg_plain_t<8, float>(P, shmem, st);
+      case 16: return launch_pcg_plain_t<16, float>(P, shmem, st);
+    }
+  } else {
+    switch (G) {
+      case 4: return launch_pcg_plain_t<4, double>(P, shmem, st);
+      case 8: return launch_pcg_plain_t<8, double>(P, shmem, st);
+      case 16: return launch_pcg_plain_t<16, double>(P, shmem, st);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+const void* pcg_kernel_fn(int G, bool mixed) {
+  if (mixed) {
+    switch (G) {
+      case 4: return (const void*)pcg_persistent<4, float>;
+      case 8: return (const void*)pcg_persistent<8, float>;
+      case 16: return (const void*)pcg_persistent<16, float>;
+    }
+  } else {
+    switch (G) {
+      case 4: return (const void*)pcg_persistent<4, double>;
+      case 8: return (const void*)pcg_persistent<8, double>;
+      case 16: return (const void*)pcg_persistent<16, double>;
+    }
+  }
+  return nullptr;
 }
 
 template <int G, typename VT>

@@ -50,6 +50,12 @@ struct Buf {
   }
 };
 
+// Scratch of one persistent PCG launch: system descriptors and outputs,
+// chunk tables, reduction partials, grid-barrier counters, error flag.
+struct PcgScratch {
+  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag;
+};
+
 struct HscError {
   int code;
   std::string msg;
@@ -112,7 +118,8 @@ struct hsc_handle {
   Buf A, E, R0, R1, U, V, hist, pivots, acapart, Binv, P, s1, s2, s3, W1, C, Lm, W64, W32;
   Buf rhs, r, p0, p1, z0, z1, q, q64, yfull, small, ybuf, misc, ints;
   // persistent-launch scratch
-  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, respart, profbuf;
+  Buf barrier, errflag, respart, profbuf;  // barrier/errflag: setup kernels (ACA)
+  PcgScratch pcg;                           // the persistent PCG launch of hsc_solve / hsc_solve_batched
   // learned preconditioner: graph MLP over D's node graph
   Buf learned, nbr_ptr, nbr, dfeat, sdiag, xfeat, omlp, ptmp, dotbuf, ebuf;
   long long learned_bytes = 0;
@@ -216,14 +223,29 @@ struct LaunchCfg {
   int G, wgpcu;
 };
 
-// Persistent PCG over a batch of set-up handles.  Returns per-system outputs.
-std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_override,
-                                 hipStream_t st, float* pcg_ms) {
+// Launch plan of the persistent PCG over a batch of set-up handles: tile and
+// chunk tables (uploaded to h0's scratch on `st`), kernel parameters.  The
+// host copies of the tables live in the plan until the uploads complete.
+struct PcgPlan {
+  hsc::PcgParams P{};
+  size_t shmem = 0;
+  int G = 8;
+  bool mixed = false;
+  std::vector<hsc::SysDesc> S;
+  std::vector<int> ch;
+};
+
+PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_override, hipStream_t st,
+                 PcgScratch& sc) {
   hsc_handle* h0 = hs[0];
+  PcgPlan plan;
   const bool mixed = h0->cfg.precision == HSC_PRECISION_MIXED;
   const int G = lc.G;
   const int R = hsc::kBlock / G;
-  std::vector<hsc::SysDesc> S(nsys);
+  plan.G = G;
+  plan.mixed = mixed;
+  std::vector<hsc::SysDesc>& S = plan.S;
+  S.resize(nsys);
   int rt = 0, et = 0, lt = 0, t_total = 0;
   double lr_bytes = 0, row_bytes = 0;
   const double eltM = mixed ? 4.0 : 8.0;
@@ -288,30 +310,25 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
       S[s].lrp_off = lrp;
     }
   }
-  HIPCHK(h0->sysdesc.ensure(sizeof(hsc::SysDesc) * nsys));
-  HIPCHK(h0->sysout.ensure(sizeof(hsc::SysOut) * nsys));
-  HIPCHK(h0->chunks.ensure(sizeof(int) * 3 * nwg));
-  HIPCHK(h0->partA.ensure(sizeof(double) * (size_t)nsys * nA));
-  HIPCHK(h0->partE.ensure(sizeof(double) * 2 * (size_t)nsys * nF));
-  HIPCHK(h0->partF.ensure(sizeof(double) * (size_t)nsys * nF));
-  HIPCHK(h0->partL.ensure(sizeof(double) * (size_t)std::max(lrp, 1)));
-  HIPCHK(h0->barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
-  HIPCHK(h0->errflag.ensure(64));
-  std::vector<int> ch(3 * nwg);
+  HIPCHK(sc.sysdesc.ensure(sizeof(hsc::SysDesc) * nsys));
+  HIPCHK(sc.sysout.ensure(sizeof(hsc::SysOut) * nsys));
+  HIPCHK(sc.chunks.ensure(sizeof(int) * 3 * nwg));
+  HIPCHK(sc.partA.ensure(sizeof(double) * (size_t)nsys * nA));
+  HIPCHK(sc.partE.ensure(sizeof(double) * 2 * (size_t)nsys * nF));
+  HIPCHK(sc.partF.ensure(sizeof(double) * (size_t)nsys * nF));
+  HIPCHK(sc.partL.ensure(sizeof(double) * (size_t)std::max(lrp, 1)));
+  HIPCHK(sc.barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
+  HIPCHK(sc.errflag.ensure(64));
+  std::vector<int>& ch = plan.ch;
+  ch.resize(3 * nwg);
   std::copy(chA.begin(), chA.end(), ch.begin());
   std::copy(chF.begin(), chF.end(), ch.begin() + nwg);
   std::copy(chL.begin(), chL.end(), ch.begin() + 2 * nwg);
-  HIPCHK(hipMemcpyAsync(h0->sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(h0->chunks.p, ch.data(), sizeof(int) * 3 * nwg, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(h0->barrier.p, 0, sizeof(unsigned) * hsc::kBarrierWords, st));
-  HIPCHK(hipMemsetAsync(h0->errflag.p, 0, 64, st));
-  // p_old of the first iteration is multiplied by beta = 0: it must be finite
-  for (int s = 0; s < nsys; ++s) {
-    HIPCHK(hipMemsetAsync(hs[s]->p0.p, 0, sizeof(double) * std::max(hs[s]->fn, 1), st));
-  }
-  hsc::PcgParams P{};
-  P.sys = h0->sysdesc.as<hsc::SysDesc>();
-  P.out = h0->sysout.as<hsc::SysOut>();
+  HIPCHK(hipMemcpyAsync(sc.sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(sc.chunks.p, ch.data(), sizeof(int) * 3 * nwg, hipMemcpyHostToDevice, st));
+  hsc::PcgParams& P = plan.P;
+  P.sys = sc.sysdesc.as<hsc::SysDesc>();
+  P.out = sc.sysout.as<hsc::SysOut>();
   P.nsys = nsys;
   P.order = h0->cfg.neumann_order;
   P.omega = h0->cfg.omega;
@@ -326,52 +343,89 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   P.et_total = et;
   P.lt_total = lt;
   P.rows_per_tile = R;
-  P.chunkA = h0->chunks.as<int>();
-  P.chunkF = h0->chunks.as<int>() + nwg;
-  P.chunkL = h0->chunks.as<int>() + 2 * nwg;
-  P.partA = h0->partA.as<double>();
-  P.partE = h0->partE.as<double>();
-  P.partF = h0->partF.as<double>();
-  P.partL = h0->partL.as<double>();
-  P.barrier = h0->barrier.as<unsigned>();
-  P.error = h0->errflag.as<int>();
+  P.chunkA = sc.chunks.as<int>();
+  P.chunkF = sc.chunks.as<int>() + nwg;
+  P.chunkL = sc.chunks.as<int>() + 2 * nwg;
+  P.partA = sc.partA.as<double>();
+  P.partE = sc.partE.as<double>();
+  P.partF = sc.partF.as<double>();
+  P.partL = sc.partL.as<double>();
+  P.barrier = sc.barrier.as<unsigned>();
+  P.error = sc.errflag.as<int>();
   P.timeout_ticks = kTimeoutTicks;
   P.t_total = t_total;
   P.prof = nullptr;
+  plan.shmem = shmem;
+  return plan;
+}
+
+// Zero the grid-barrier counters, the error flag and every system's p_old
+// (multiplied by beta = 0 in the first iteration, so it must be finite).
+// Fill kernels, not memsets, so that the sequence can be captured in a graph.
+void reset_pcg(hsc_handle* const* hs, int nsys, PcgScratch& sc, hipStream_t st) {
+  HIPCHK(hsc::launch_zero(sc.barrier.p, sizeof(unsigned) * hsc::kBarrierWords, st));
+  HIPCHK(hsc::launch_zero(sc.errflag.p, 64, st));
+  for (int s = 0; s < nsys; ++s)
+    HIPCHK(hsc::launch_zero(hs[s]->p0.p, sizeof(double) * std::max(hs[s]->fn, 1), st));
+}
+
+// Copy the per-system outputs back and wait; raises on a device timeout.
+std::vector<hsc::SysOut> collect_pcg(PcgScratch& sc, int nsys, hipStream_t st) {
+  std::vector<hsc::SysOut> out(nsys);
+  int derr = 0;
+  hipError_t ce = hipMemcpyAsync(out.data(), sc.sysout.p, sizeof(hsc::SysOut) * nsys, hipMemcpyDeviceToHost, st);
+  if (ce == hipSuccess) ce = hipMemcpyAsync(&derr, sc.errflag.p, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(st);
+  if (ce != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("PCG: ") + hipGetErrorString(ce)};
+  if (derr) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "persistent PCG: bounded grid wait expired"};
+  return out;
+}
+
+// Persistent PCG over a batch of set-up handles (one cooperative launch).
+std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_override,
+                                 hipStream_t st, float* pcg_ms) {
+  hsc_handle* h0 = hs[0];
+  PcgPlan plan = plan_pcg(hs, nsys, lc, max_iter_override, st, h0->pcg);
+  reset_pcg(hs, nsys, h0->pcg, st);
+  hsc::PcgParams& P = plan.P;
+  const int nwg = P.nwg;
   static const bool stage_timing = std::getenv("HSC_STAGE_TIMING") != nullptr;
   if (stage_timing) {
     HIPCHK(h0->profbuf.ensure(sizeof(long long) * 32 * nwg));
-    HIPCHK(hipMemsetAsync(h0->profbuf.p, 0, sizeof(long long) * 32 * nwg, st));
+    HIPCHK(hsc::launch_zero(h0->profbuf.p, sizeof(long long) * 32 * nwg, st));
     P.prof = h0->profbuf.as<long long>();
   }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
-  hipError_t le = hsc::launch_pcg(G, mixed, P, shmem, st);
+  hipError_t le = hsc::launch_pcg(plan.G, plan.mixed, P, plan.shmem, st);
   HIPCHK(hipEventRecord(e1, st));
-  std::vector<hsc::SysOut> out(nsys);
-  int derr = 0;
-  hipError_t ce = hipSuccess;
-  if (le == hipSuccess) {
-    ce = hipMemcpyAsync(out.data(), h0->sysout.p, sizeof(hsc::SysOut) * nsys, hipMemcpyDeviceToHost, st);
-    if (ce == hipSuccess) ce = hipMemcpyAsync(&derr, h0->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (ce == hipSuccess) ce = hipStreamSynchronize(st);
+  if (le != hipSuccess) {
+    (void)hipStreamSynchronize(st);  // the table uploads read plan's host copies
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw HscError{HSC_ERR_HIP, std::string("PCG launch: ") + hipGetErrorString(le)};
+  }
+  std::vector<hsc::SysOut> out;
+  try {
+    out = collect_pcg(h0->pcg, nsys, st);
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
   }
   float ms = 0.f;
-  if (le == hipSuccess && ce == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (le != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("PCG launch: ") + hipGetErrorString(le)};
-  if (ce != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("PCG: ") + hipGetErrorString(ce)};
-  if (derr) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "persistent PCG: bounded grid wait expired"};
   if (pcg_ms) *pcg_ms = ms;
   if (P.prof && max_iter_override <= 0) {
     std::vector<long long> pr((size_t)32 * nwg);
     HIPCHK(hipMemcpy(pr.data(), P.prof, sizeof(long long) * pr.size(), hipMemcpyDeviceToHost));
     const char* names[6] = {"init", "neumann", "A", "B", "T1", "T2"};
     std::fprintf(stderr, "[hsc stage timing] G=%d nwg=%d nL=%d iters=%d pcg_ms=%.3f (us per barrier site: compute / wait; wg0 | wg%d)\n",
-                 G, nwg, nL, out[0].iterations, ms, nwg - 1);
+                 plan.G, nwg, P.n_lr_wg, out[0].iterations, ms, nwg - 1);
     for (int si = 0; si < 6; ++si) {
       const long long* a0 = pr.data();
       const long long* a1 = pr.data() + (size_t)32 * (nwg - 1);
@@ -803,7 +857,9 @@ void fill_common_stats(hsc_handle* h, hsc_stats* s) {
 }
 
 // back-substitution + descent + full residual (linear_hsc.cu:395-435)
-int finish_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStream_t st) {
+// Back-substitution of the ABD part and the acceptance reductions (full
+// residual, descent) into h->misc[20..22]; kernels only (capturable).
+void enqueue_finish(hsc_handle* h, const double* b, double* x, hipStream_t st) {
   HIPCHK(hsc::launch_backsub(h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), h->act_dof.as<int>(), b, x,
                              h->qo, h->qm, h->fo, h->a, h->k, h->small.as<double>(), st));
   hsc::SysDesc S = make_desc(h);
@@ -811,9 +867,10 @@ int finish_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStr
   HIPCHK(hsc::launch_full_residual(h->yfull.as<double>(), h->E.as<double>(), h->A.as<double>(),
                                    h->act_node.as<int>(), h->act_dof.as<int>(), h->a, h->qm, b, x, h->fo, h->qo,
                                    h->fn, h->respart.as<double>(), h->misc.as<double>() + 20, st));
-  double out[3];
-  HIPCHK(hipMemcpyAsync(out, h->misc.as<double>() + 20, sizeof(out), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+}
+
+// Acceptance (linear_hsc.cu:400-415) from out = {|b - Hx|^2, |b|^2, b.x}.
+int judge_finish(hsc_handle* h, hsc_stats* s, const double out[3]) {
   const double bn = std::sqrt(out[1]);
   s->descent = out[2];
   s->full_relative_residual = bn > 0 ? std::sqrt(out[0]) / bn : 0.0;
@@ -832,6 +889,14 @@ int finish_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStr
     }
   }
   return HSC_OK;
+}
+
+int finish_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStream_t st) {
+  enqueue_finish(h, b, x, st);
+  double out[3];
+  HIPCHK(hipMemcpyAsync(out, h->misc.as<double>() + 20, sizeof(out), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return judge_finish(h, s, out);
 }
 
 int status_from_out(hsc_handle* h, const hsc::SysOut& o) {
@@ -1089,6 +1154,191 @@ int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count, const doubl
     return first;
   });
 }
+
+}  // extern "C"
+
+// A batched solve captured once into a hipGraph (right-hand sides, state
+// reset, the persistent PCG launch, back-substitution and acceptance
+// reductions of every system) and replayed for fixed b/x buffers.
+struct hsc_batch {
+  std::vector<hsc_handle*> hs;
+  std::vector<const double*> b;
+  std::vector<double*> x;
+  PcgScratch sc;  // own launch scratch: eager solves on the handles do not touch the captured tables
+  PcgPlan plan;
+  LaunchCfg lc{8, 1};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int device = 0;
+  int cooperative_node = 0;
+  ~hsc_batch() {
+    DeviceGuard g(device);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+int validate_batch(hsc_handle* const* handles, int32_t count, const double* const* b, double* const* x) {
+  if (!handles || count <= 0 || !b || !x) return HSC_ERR_INVALID_ARGUMENT;
+  if (count > hsc::kMaxSys) return HSC_ERR_CAPACITY;
+  hsc_handle* h0 = handles[0];
+  if (!h0) return HSC_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < count; ++i) {
+    if (!handles[i] || !handles[i]->setup_ok) return HSC_ERR_NOT_SETUP;
+    if (!b[i] || !x[i]) return HSC_ERR_INVALID_ARGUMENT;
+    if (handles[i]->device != h0->device || handles[i]->cfg.precision != h0->cfg.precision ||
+        handles[i]->cfg.neumann_order != h0->cfg.neumann_order || handles[i]->cfg.omega != h0->cfg.omega ||
+        handles[i]->cfg.preconditioner != HSC_PRECOND_NEUMANN)
+      return HSC_ERR_INVALID_ARGUMENT;
+    for (int j = 0; j < i; ++j)
+      if (handles[j] == handles[i]) return HSC_ERR_INVALID_ARGUMENT;  // one system per handle
+  }
+  return HSC_OK;
+}
+
+void enqueue_rhs(hsc_handle* h, const double* b, double* x, hipStream_t st) {
+  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
+                         h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
+                         h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double* const* b, double* const* x,
+                         hsc_stream stream, hsc_batch** out) {
+  if (!out) return HSC_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  const int v = validate_batch(handles, count, b, x);
+  if (v != HSC_OK) return v;
+  hsc_handle* h0 = handles[0];
+  DeviceGuard g(h0->device);
+  hipStream_t st = (hipStream_t)stream;
+  return guarded(h0, [&] {
+    std::unique_ptr<hsc_batch> bt(new hsc_batch());
+    bt->device = h0->device;
+    bt->hs.assign(handles, handles + count);
+    bt->b.assign(b, b + count);
+    bt->x.assign(x, x + count);
+    for (int i = 0; i < count; ++i) handles[i]->xsol = x[i] + handles[i]->fo;
+    if (!tune_cached(handles, count))
+      for (int i = 0; i < count; ++i) enqueue_rhs(handles[i], b[i], x[i], st);
+    bt->lc = tuned_config(handles, count, st);
+    bt->plan = plan_pcg(handles, count, bt->lc, 0, st, bt->sc);
+    HIPCHK(hipStreamSynchronize(st));
+    hipStream_t cap = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipError_t ce = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    std::string what;
+    if (ce == hipSuccess) {
+      try {
+        for (int i = 0; i < count; ++i) enqueue_rhs(handles[i], b[i], x[i], cap);
+        reset_pcg(handles, count, bt->sc, cap);
+        HIPCHK(hsc::launch_pcg_plain(bt->plan.G, bt->plan.mixed, bt->plan.P, bt->plan.shmem, cap));
+        for (int i = 0; i < count; ++i) enqueue_finish(handles[i], b[i], x[i], cap);
+      } catch (const HscError& e) {
+        what = e.msg;
+      }
+      hipGraph_t graph = nullptr;
+      const hipError_t ee = hipStreamEndCapture(cap, &graph);
+      bt->graph = graph;
+      if (what.empty() && ee != hipSuccess) what = std::string("graph capture: ") + hipGetErrorString(ee);
+    } else {
+      what = std::string("graph capture: ") + hipGetErrorString(ce);
+    }
+    (void)hipStreamDestroy(cap);
+    if (!what.empty()) throw HscError{HSC_ERR_HIP, what};
+    // mark the persistent PCG node cooperative (co-residency of its grid)
+    const void* fn = hsc::pcg_kernel_fn(bt->plan.G, bt->plan.mixed);
+    size_t nn = 0;
+    HIPCHK(hipGraphGetNodes(bt->graph, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn);
+    HIPCHK(hipGraphGetNodes(bt->graph, nodes.data(), &nn));
+    hipGraphNode_t pcg_node = nullptr;
+    for (auto nd : nodes) {
+      hipGraphNodeType t;
+      if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+      hipKernelNodeParams kp{};
+      if (hipGraphKernelNodeGetParams(nd, &kp) == hipSuccess && kp.func == fn) pcg_node = nd;
+    }
+    if (!pcg_node) throw HscError{HSC_ERR_HIP, "graph capture: persistent PCG node not found"};
+    hipKernelNodeAttrValue av{};
+    av.cooperative = 1;
+    bt->cooperative_node = hipGraphKernelNodeSetAttribute(pcg_node, hipKernelNodeAttributeCooperative, &av) ==
+                           hipSuccess;
+    (void)hipGetLastError();
+    hipError_t ie = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess && bt->cooperative_node) {
+      (void)hipGetLastError();
+      av.cooperative = 0;
+      (void)hipGraphKernelNodeSetAttribute(pcg_node, hipKernelNodeAttributeCooperative, &av);
+      bt->cooperative_node = 0;
+      ie = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
+    }
+    if (ie != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie)};
+    *out = bt.release();
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_batch_launch(hsc_batch* bt, hsc_stream stream) {
+  if (!bt || !bt->exec) return HSC_ERR_INVALID_ARGUMENT;
+  DeviceGuard g(bt->device);
+  for (auto* h : bt->hs)
+    if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  const hipError_t e = hipGraphLaunch(bt->exec, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    bt->hs[0]->err = std::string("graph launch: ") + hipGetErrorString(e);
+    return HSC_ERR_HIP;
+  }
+  return HSC_OK;
+}
+
+int32_t hsc_batch_stats(hsc_batch* bt, hsc_stats* stats, hsc_stream stream) {
+  if (!bt || !stats) return HSC_ERR_INVALID_ARGUMENT;
+  DeviceGuard g(bt->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int n = (int)bt->hs.size();
+  hsc_handle* h0 = bt->hs[0];
+  return guarded(h0, [&] {
+    std::vector<double> res(3 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+      HIPCHK(hipMemcpyAsync(res.data() + 3 * i, bt->hs[i]->misc.as<double>() + 20, 3 * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    std::vector<hsc::SysOut> out = collect_pcg(bt->sc, n, st);
+    int first = HSC_OK;
+    for (int i = 0; i < n; ++i) {
+      hsc_handle* h = bt->hs[i];
+      hsc_stats* s = &stats[i];
+      fill_common_stats(h, s);
+      s->pcg_iterations = out[i].iterations;
+      s->schur_relative_residual = out[i].rel;
+      s->true_residual_checks = out[i].checks;
+      s->restarts = out[i].restarts;
+      int code = status_from_out(h, out[i]);
+      const int c2 = judge_finish(h, s, res.data() + 3 * i);
+      if (code == HSC_OK) code = c2;
+      s->status = code;
+      if (first == HSC_OK) first = code;
+    }
+    return first;
+  });
+}
+
+int32_t hsc_batch_info(const hsc_batch* bt, int32_t* lanes_per_row, int32_t* wg_per_cu, int32_t* workgroups,
+                       int32_t* cooperative_node) {
+  if (!bt) return HSC_ERR_INVALID_ARGUMENT;
+  if (lanes_per_row) *lanes_per_row = bt->lc.G;
+  if (wg_per_cu) *wg_per_cu = bt->lc.wgpcu;
+  if (workgroups) *workgroups = bt->plan.P.nwg;
+  if (cooperative_node) *cooperative_node = bt->cooperative_node;
+  return HSC_OK;
+}
+
+void hsc_batch_destroy(hsc_batch* bt) { delete bt; }
 
 int32_t hsc_append_jsonl(const hsc_stats* s, const char* path, int32_t frame, int32_t newton) {
   if (!s || !path) return HSC_ERR_INVALID_ARGUMENT;

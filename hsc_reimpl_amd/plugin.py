@@ -233,6 +233,61 @@ def solve_batched(handles: list[Handle], bs: list[torch.Tensor], xs: list[torch.
     return out
 
 
+class Batch:
+    """A batched solve captured once into a hipGraph (hsc_batch_*), replayed
+    for the same b/x buffers.  Keeps its handles and buffers alive."""
+
+    def __init__(self, handles: list[Handle], bs: list[torch.Tensor], xs: list[torch.Tensor], stream=None):
+        self.lib = _abi.load()
+        self.handles, self.bs, self.xs = list(handles), list(bs), list(xs)
+        n = len(handles)
+        hs = (C.c_void_p * n)(*[h.h.value for h in handles])
+        bp = (C.c_void_p * n)(*[_ptr(b) for b in bs])
+        xp = (C.c_void_p * n)(*[_ptr(x) for x in xs])
+        out = C.c_void_p()
+        code = self.lib.hsc_batch_create(hs, n, bp, xp, _stream(stream), C.byref(out))
+        if code != 0:
+            msg = self.lib.hsc_last_error(handles[0].h)
+            raise HSCError(code, msg.decode() if msg else "hsc_batch_create")
+        self.b = out
+
+    def launch(self, stream=None):
+        code = self.lib.hsc_batch_launch(self.b, _stream(stream))
+        if code != 0:
+            msg = self.lib.hsc_last_error(self.handles[0].h)
+            raise HSCError(code, msg.decode() if msg else "hsc_batch_launch")
+
+    def stats(self, stream=None, check=True) -> list[dict]:
+        n = len(self.handles)
+        st = (_abi.Stats * n)()
+        code = self.lib.hsc_batch_stats(self.b, st, _stream(stream))
+        out = []
+        for i in range(n):
+            d = st[i].as_dict()
+            d["status_name"] = _abi.STATUS.get(d["status"], str(d["status"]))
+            out.append(d)
+        if check and code != 0:
+            msg = self.lib.hsc_last_error(self.handles[0].h)
+            raise HSCError(code, msg.decode() if msg else "hsc_batch_stats")
+        return out
+
+    def info(self) -> dict:
+        v = [C.c_int32() for _ in range(4)]
+        self.lib.hsc_batch_info(self.b, *[C.byref(x) for x in v])
+        return dict(zip(("lanes_per_row", "wg_per_cu", "workgroups", "cooperative_node"), [x.value for x in v]))
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.hsc_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class LinearHSC:
     """Plugin-shaped adapter: do_build(scene config) / do_solve(SolvingInfo)."""
 

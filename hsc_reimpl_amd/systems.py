@@ -342,7 +342,7 @@ def torus_system(seed=1, n_tori=7, n_bodies=12, youngs=(1e10,) * 4 + (1e8,) * 3,
 
 
 def batch_systems(seed=2, count=64, bodies=(1, 16), tori=(1, 8), tets=(2000, 15000),
-                  ranks=(4, 16), youngs=(1e8, 1e10)):
+                  ranks=(4, 16), youngs=(1e8, 1e10), progress=None):
     """Uneven batch: per system U{1..16} bodies, U{1..8} tori of
     log-U[2k, 15k] tets, coupling rank U{4..16} (capped at 12 per hub
     block, the most a 12-row block can carry)."""
@@ -379,6 +379,8 @@ def batch_systems(seed=2, count=64, bodies=(1, 16), tori=(1, 8), tets=(2000, 150
         info = dict(kind="batch", index=s, tets=int(nt), tori=ntor, bodies=nbod, rank=rank)
         out.append(_finish(base, np.concatenate(Is), np.concatenate(Js), np.concatenate(Bs),
                            bods, couplings, sub, info))
+        if progress:
+            progress(s, out[-1])
     return out
 
 

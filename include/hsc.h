@@ -184,6 +184,25 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* P,
                         int32_t* active, double* aca_history);
 
+/* ---- captured batch (no reference counterpart) ----
+ * The whole batched solve of `count` set-up handles for fixed b/x buffers
+ * (right-hand-side reductions, one persistent PCG launch over all systems,
+ * back-substitution and acceptance reductions) captured once into a
+ * hipGraph.  hsc_batch_launch enqueues one replay on `stream` (no host
+ * synchronisation); hsc_batch_stats waits for it and fills one hsc_stats per
+ * system (pcg_ms is not measured inside a replay and reads 0).  The batch is
+ * valid until one of its handles is set up again, reconfigured or
+ * destroyed.  Handles must share device, precision, neumann_order and omega
+ * and use the Neumann preconditioner. */
+typedef struct hsc_batch hsc_batch;
+int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double* const* b, double* const* x,
+                         hsc_stream stream, hsc_batch** out);
+int32_t hsc_batch_launch(hsc_batch* batch, hsc_stream stream);
+int32_t hsc_batch_stats(hsc_batch* batch, hsc_stats* stats, hsc_stream stream);
+int32_t hsc_batch_info(const hsc_batch* batch, int32_t* lanes_per_row, int32_t* wg_per_cu, int32_t* workgroups,
+                       int32_t* cooperative_node);
+void hsc_batch_destroy(hsc_batch* batch);
+
 /* ---- learned preconditioner (no reference counterpart) ----
  * A per-node graph MLP over D's node graph (2 message-passing layers, hidden
  * 64, BF16 weights/activations, FP32 accumulation) corrects the Neumann z
