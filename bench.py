@@ -181,6 +181,7 @@ def main():
                          "in one captured persistent launch vs one at a time")
     ap.add_argument("--lanes-per-row", type=int, default=4)
     ap.add_argument("--wg-per-cu", type=int, default=1)
+    ap.add_argument("--schedule", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the HSC solve of one system runs on one MI355X and no "
@@ -206,7 +207,8 @@ def main():
 
     if args.traffic_probe:
         h = plugin.Handle(plugin.make_config(scene, precision="mixed", residual_replace_every=25,
-                                             lanes_per_row=args.lanes_per_row, wg_per_cu=args.wg_per_cu))
+                                             lanes_per_row=args.lanes_per_row, wg_per_cu=args.wg_per_cu,
+                                             schedule=args.schedule))
         h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
         x = torch.zeros_like(b)
         st = h.solve(b, x)
@@ -273,7 +275,8 @@ def main():
         "config": {"workload": "78k-tet vehicle-scale HSC system (7 tori x 11,200 Tet4, 12 ABD bodies)",
                    "fem_nodes": nb, "fem_dofs": 3 * nb, "abd_dofs": info["abd_dofs"], "nnzb": nnzb,
                    "active_fem_dofs": a, "rank": k, "neumann_order": order, "omega": omega, "rtol": rtol,
-                   "lanes_per_row": info["lanes_per_row"], "wg_per_cu": info["workgroups"]},
+                   "lanes_per_row": info["lanes_per_row"], "wg_per_cu": info["workgroups"],
+                   "schedule": info["schedule"]},
         "pcg_iterations": {"new": new["pcg_iterations"], "plain_fp64": plain["pcg_iterations"]},
         "plain_fp64": {"solves_per_s": round(plain["solves_per_s"], 3), "ms_per_solve": round(plain["ms_per_solve"], 3),
                        "pcg_ms": round(plain["pcg_ms"], 3), "pcg_gbs": round(achieved64, 1)},

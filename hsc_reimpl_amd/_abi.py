@@ -34,7 +34,7 @@ class Config(C.Structure):
         ("abd_inverse", C.c_int32), ("precision", C.c_int32), ("residual_replace_every", C.c_int32),
         ("true_residual_every_iteration", C.c_int32), ("aca_rtol", C.c_double),
         ("lanes_per_row", C.c_int32), ("wg_per_cu", C.c_int32), ("preconditioner", C.c_int32),
-        ("check_acceptance", C.c_int32),
+        ("check_acceptance", C.c_int32), ("schedule", C.c_int32),
     ]
 
 
@@ -66,6 +66,7 @@ class SetupInfo(C.Structure):
         ("active_fem_dofs", C.c_int32), ("rank", C.c_int32), ("lanes_per_row", C.c_int32),
         ("workgroups", C.c_int32), ("neumann_doublings", C.c_int32), ("abd_block_diagonal", C.c_int32),
         ("aca_relative_error", C.c_double), ("abd_inverse_residual", C.c_double),
+        ("schedule", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -93,7 +94,7 @@ SIGNATURES = {
     "hsc_batch_launch": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "hsc_batch_stats": (C.c_int32, [C.c_void_p, C.POINTER(Stats), C.c_void_p]),
     "hsc_batch_info": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                   C.POINTER(C.c_int32)]),
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "hsc_batch_destroy": (None, [C.c_void_p]),
     "hsc_apply_D": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hsc_apply_schur": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

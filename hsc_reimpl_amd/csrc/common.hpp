@@ -6,7 +6,7 @@
 namespace hsc {
 
 constexpr int kBlock = 512;        // threads per workgroup (8 wave64)
-constexpr int kNodeTile = 128;     // nodes per elementwise tile (threads < kNodeTile own one node each)
+constexpr int kNodeTileMax = 512;  // nodes per elementwise tile (one node per thread; runtime 128 or 512)
 constexpr int kLrTile = 64;        // low-rank rows (W rows) per tile
 constexpr int kMaxSys = 128;       // systems per persistent launch
 constexpr int kMaxRankLds = 256;   // rank limit of the low-rank apply (lanes hold columns l, l+64, l+128, l+192)
@@ -19,7 +19,7 @@ struct SysDesc {
   int nb, n, a, k, nnzb;
   // global tile ranges
   int rt_begin, rt_end;   // row tiles (rows_per_tile block rows each)
-  int et_begin, et_end;   // node tiles (kNodeTile nodes each)
+  int et_begin, et_end;   // node tiles (PcgParams::node_tile nodes each)
   int lt_begin, lt_end;   // low-rank tiles (kLrTile W rows each)
   // chunk ranges (inclusive) of each chunking that cover this system
   int cA0, cA1;           // row tiles over the "row" workgroups (stage A, T1)
@@ -75,6 +75,7 @@ struct PcgParams {
   int nA, nF, nL;         // chunk counts (nF == nwg)
   int rt_total, et_total, lt_total;
   int rows_per_tile;
+  int node_tile;          // nodes per elementwise tile (<= kNodeTileMax)
   const int* chunkA;      // [nwg] chunk of row-workgroup b in the A chunking (-1 for LR wgs)
   const int* chunkF;      // [nwg]
   const int* chunkL;      // [nwg] (-1 for row wgs)

@@ -167,14 +167,19 @@ __device__ __forceinline__ void tile_Wt(const SysDesc& S, int tl, const WT* __re
   }
 }
 
-// g_d = sum_l W[an+d, l] t[l] for the active nodes of one node tile, computed
-// cooperatively: active nodes are listed in LDS and each wave reduces two of
-// them at a time over its lanes (l = lane + 64c).  Uniform over the workgroup.
+// g_d = sum_l W[an+d, l] t[l] for the coupled nodes of one node tile (an =
+// the node's first W row, -1 if uncoupled).  The coupled nodes are listed in
+// LDS; groups of 4 lanes take one node each (128 nodes per pass of the
+// workgroup), every lane strides the rank by 4 and has its W loads
+// for the three rows in flight before the group sum.  Uniform over the
+// workgroup.
 template <typename WT>
 __device__ __forceinline__ void tile_lowrank_g(const SysDesc& S, int an, const WT* __restrict__ W,
                                                const double* __restrict__ t, double g[3], int* s_list,
                                                int* s_slot, double (*s_g)[3], int* s_cnt) {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  constexpr int GL = 4;               // lanes per node
+  constexpr int NPR = kBlock / GL;    // nodes per pass
+  const int tid = threadIdx.x;
   const int k = S.k;
   if (tid == 0) *s_cnt = 0;
   __syncthreads();
@@ -185,32 +190,29 @@ __device__ __forceinline__ void tile_lowrank_g(const SysDesc& S, int an, const W
   }
   __syncthreads();
   const int m = *s_cnt;
-  constexpr int NW = kBlock / 64;
-  for (int it = wave; it < m; it += 2 * NW) {
-    const int it2 = it + NW;
-    const int a1 = s_list[it];
-    const int a2 = it2 < m ? s_list[it2] : -1;
-    double p1[3] = {0.0, 0.0, 0.0}, p2[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < 4; ++c) {
-      if (64 * c >= k) break;
-      const int l = lane + 64 * c;
-      if (l < k) {
+  const int sub = tid % GL, grp = tid / GL;
+  for (int base = 0; base < m; base += NPR) {
+    const int it = base + grp;
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    if (it < m) {
+      const WT* w0 = W + (size_t)s_list[it] * k;
+      const WT* w1 = w0 + k;
+      const WT* w2 = w1 + k;
+#pragma unroll 8
+      for (int l = sub; l < k; l += GL) {
         const double tl_ = t[l];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          p1[d] += (double)W[(size_t)(a1 + d) * k + l] * tl_;
-          if (a2 >= 0) p2[d] += (double)W[(size_t)(a2 + d) * k + l] * tl_;
-        }
+        p0 += (double)w0[l] * tl_;
+        p1 += (double)w1[l] * tl_;
+        p2 += (double)w2[l] * tl_;
       }
     }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double v1 = wave_sum(p1[d]);
-      const double v2 = wave_sum(p2[d]);
-      if (lane == 0) {
-        s_g[it][d] = v1;
-        if (a2 >= 0) s_g[it2][d] = v2;
-      }
+    p0 = group_sum<GL>(p0);
+    p1 = group_sum<GL>(p1);
+    p2 = group_sum<GL>(p2);
+    if (it < m && sub == 0) {
+      s_g[it][0] = p0;
+      s_g[it][1] = p1;
+      s_g[it][2] = p2;
     }
   }
   __syncthreads();
@@ -289,6 +291,7 @@ __device__ __forceinline__ bool chunk_empty(int c, int nC, int T) {
 __device__ __forceinline__ double wave_reduce_chunks(const double* part, int s, int nC, int c0, int c1, int T) {
   const int lane = threadIdx.x & 63;
   double v = 0.0;
+#pragma unroll 4
   for (int c = c0 + lane; c <= c1; c += 64)
     if (!chunk_empty(c, nC, T)) v += part[(size_t)s * nC + c];
   return wave_sum(v);
@@ -305,8 +308,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   __shared__ double s_red[kBlock / 64];
   __shared__ double s_lr[kBlock / 64][64];
   __shared__ int s_any;
-  __shared__ int s_list[kNodeTile], s_slot[kNodeTile], s_cnt;
-  __shared__ double s_g[kNodeTile][3];
+  __shared__ int s_list[kNodeTileMax], s_slot[kNodeTileMax], s_cnt;
+  __shared__ double s_g[kNodeTileMax][3];
   __shared__ double s_tred[kBlock];
 
   const SysDesc* sys = P.sys;
@@ -373,8 +376,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                 const int st = s_st[s];
                 if (st != ST_RUN) return;
                 if (kind == 2 && s_chk[s] == CHK_NONE) return;
-                const int i = tl * kNodeTile + tid;
-                const bool valid = tid < kNodeTile && i < S.nb;
+                const int i = tl * P.node_tile + tid;
+                const bool valid = tid < P.node_tile && i < S.nb;
                 double g[3] = {0.0, 0.0, 0.0};
                 if (kind != 0 && S.k > 0) {
                   const int an = valid ? S.act_node[i] : -1;
@@ -484,9 +487,11 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       const int l = tid % k, grp = tid / k;
       if (k <= kBlock) {
         double v = 0.0;
-        if (grp < ng)
+        if (grp < ng) {
+#pragma unroll 8
           for (int c = S.cL0 + grp; c <= S.cL1; c += ng)
             if (!chunk_empty(c, P.nL, P.lt_total)) v += P.partL[S.lrp_off + (size_t)(c - S.cL0) * k + l];
+        }
         if (grp < ng) s_tred[grp * k + l] = v;
         __syncthreads();
         if (tid < k) {

@@ -96,7 +96,10 @@ struct TuneKey {
   }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<int, int>> g_tune;
+struct TuneVal {
+  int G, wgpcu, sched;
+};
+std::map<TuneKey, TuneVal> g_tune;
 
 }  // namespace
 
@@ -110,7 +113,7 @@ struct hsc_handle {
   bool blockdiag = true, setup_ok = false;
   double aca_err = 0.0, abd_res = 0.0, setup_ms = 0.0;
   int doublings = 0;
-  int G = 8, wgpcu = 1;
+  int G = 8, wgpcu = 1, sched = 1;
   double* xsol = nullptr;  // FEM part of the caller's x during a solve
   // buffers
   Buf row_count, rowptr, cursor, active_flag, act_prefix, act_node, act_dof;
@@ -155,6 +158,10 @@ int validate_config(const hsc_config& c, std::string& why) {
   }
   if (c.wg_per_cu < 0 || c.wg_per_cu > 4) {
     why = "wg_per_cu must be in 0..4";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  if (c.schedule < 0 || c.schedule > 2) {
+    why = "schedule must be 0, 1 or 2";
     return HSC_ERR_INVALID_ARGUMENT;
   }
   return HSC_OK;
@@ -219,8 +226,13 @@ int chunk_of(int t, int nC, int T) {
   return c;
 }
 
+// Launch shape of the persistent PCG: lanes per block row, workgroups per CU,
+// and the work split (1 = latency: the grid is sized by the SpMV row tiles,
+// the low-rank workgroups are split off by byte share, 128-node elementwise
+// tiles; 2 = throughput: the grid covers row and low-rank tiles, split by
+// tile count, 512-node elementwise tiles).
 struct LaunchCfg {
-  int G, wgpcu;
+  int G, wgpcu, sched;
 };
 
 // Launch plan of the persistent PCG over a batch of set-up handles: tile and
@@ -242,6 +254,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   const bool mixed = h0->cfg.precision == HSC_PRECISION_MIXED;
   const int G = lc.G;
   const int R = hsc::kBlock / G;
+  const int node_tile = lc.sched == 2 ? hsc::kNodeTileMax : 128;
   plan.G = G;
   plan.mixed = mixed;
   std::vector<hsc::SysDesc>& S = plan.S;
@@ -257,7 +270,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
     rt += (S[s].nb + R - 1) / R;
     S[s].rt_end = rt;
     S[s].et_begin = et;
-    et += (S[s].nb + hsc::kNodeTile - 1) / hsc::kNodeTile;
+    et += (S[s].nb + node_tile - 1) / node_tile;
     S[s].et_end = et;
     S[s].lt_begin = lt;
     if (S[s].k > 0) lt += (S[s].a + hsc::kLrTile - 1) / hsc::kLrTile;
@@ -274,11 +287,19 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   HIPCHK(hsc::pcg_occupancy(G, mixed, shmem, &occ));
   if (occ <= 0) throw HscError{HSC_ERR_CAPACITY, "persistent PCG kernel does not fit on a CU"};
   const int cap = occ * di.cu;
-  int nwg = std::min(std::min(lc.wgpcu, occ) * di.cu, std::max(rt, 1));
+  // Stage A runs row tiles (SpMV) on nA workgroups and low-rank tiles (W^T p)
+  // on nL others.  Latency split: the grid is sized by the row tiles and the
+  // low-rank share follows the bytes.  Throughput split: both kinds of tile
+  // cost about one dependent-load chain (measured on MI355X), so the share
+  // follows the tile counts and the grid covers both kinds when the
+  // co-resident capacity allows.
+  const bool thr = lc.sched == 2;
+  int nwg = std::min(std::min(lc.wgpcu, occ) * di.cu, std::max(thr ? rt + lt : rt, 1));
   int nL = 0;
   if (lt > 0) {
     nwg = std::max(nwg, 2);
-    nL = (int)std::lround(nwg * lr_bytes / std::max(lr_bytes + row_bytes, 1.0));
+    nL = thr ? (int)std::lround((double)nwg * lt / (double)(rt + lt))
+             : (int)std::lround(nwg * lr_bytes / std::max(lr_bytes + row_bytes, 1.0));
     nL = std::max(1, std::min(nL, std::min(lt, nwg - 1)));
   }
   if (nwg > cap) throw HscError{HSC_ERR_CAPACITY, "grid exceeds co-resident capacity"};
@@ -343,6 +364,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   P.et_total = et;
   P.lt_total = lt;
   P.rows_per_tile = R;
+  P.node_tile = node_tile;
   P.chunkA = sc.chunks.as<int>();
   P.chunkF = sc.chunks.as<int>() + nwg;
   P.chunkL = sc.chunks.as<int>() + 2 * nwg;
@@ -424,15 +446,24 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
     std::vector<long long> pr((size_t)32 * nwg);
     HIPCHK(hipMemcpy(pr.data(), P.prof, sizeof(long long) * pr.size(), hipMemcpyDeviceToHost));
     const char* names[6] = {"init", "neumann", "A", "B", "T1", "T2"};
-    std::fprintf(stderr, "[hsc stage timing] G=%d nwg=%d nL=%d iters=%d pcg_ms=%.3f (us per barrier site: compute / wait; wg0 | wg%d)\n",
-                 plan.G, nwg, P.n_lr_wg, out[0].iterations, ms, nwg - 1);
+    std::fprintf(stderr,
+                 "[hsc stage timing] G=%d nwg=%d nL=%d iters=%d pcg_ms=%.3f (us per visit of each barrier site: "
+                 "compute min/median/max over workgroups [argmax, LR?] | wait median)\n",
+                 plan.G, nwg, P.n_lr_wg, out[0].iterations, ms);
     for (int si = 0; si < 6; ++si) {
-      const long long* a0 = pr.data();
-      const long long* a1 = pr.data() + (size_t)32 * (nwg - 1);
-      if (!a0[20 + si]) continue;
-      std::fprintf(stderr, "  %-8s n=%6lld  %8.2f / %8.2f | %8.2f / %8.2f\n", names[si], a0[20 + si],
-                   a0[2 * si] / 100.0 / a0[20 + si], a0[2 * si + 1] / 100.0 / a0[20 + si],
-                   a1[2 * si] / 100.0 / a1[20 + si], a1[2 * si + 1] / 100.0 / a1[20 + si]);
+      const long long n = pr[20 + si];
+      if (!n) continue;
+      std::vector<std::pair<double, int>> c(nwg);
+      std::vector<double> w(nwg);
+      for (int b = 0; b < nwg; ++b) {
+        c[b] = {pr[(size_t)32 * b + 2 * si] / 100.0 / n, b};
+        w[b] = pr[(size_t)32 * b + 2 * si + 1] / 100.0 / n;
+      }
+      std::sort(c.begin(), c.end());
+      std::sort(w.begin(), w.end());
+      const int am = c.back().second;
+      std::fprintf(stderr, "  %-8s n=%6lld  %8.2f %8.2f %8.2f [wg%d, %s] | %8.2f\n", names[si], n, c.front().first,
+                   c[nwg / 2].first, c.back().first, am, am < P.n_lr_wg ? "LR" : "row", w[nwg / 2]);
     }
   }
   return out;
@@ -451,54 +482,59 @@ TuneKey tune_key(hsc_handle* const* hs, int nsys) {
                  mixed ? 1 : 0, nsys > 1 ? 1 : 0};
 }
 
+bool pinned(const hsc_config& c) { return c.lanes_per_row && c.wg_per_cu && c.schedule; }
+
 bool tune_cached(hsc_handle* const* hs, int nsys) {
-  if (hs[0]->cfg.lanes_per_row && hs[0]->cfg.wg_per_cu) return true;
+  if (pinned(hs[0]->cfg)) return true;
   std::lock_guard<std::mutex> g(g_tune_mu);
   return g_tune.count(tune_key(hs, nsys)) > 0;
 }
 
 LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
   hsc_handle* h0 = hs[0];
-  const bool mixed = h0->cfg.precision == HSC_PRECISION_MIXED;
-  if (h0->cfg.lanes_per_row && h0->cfg.wg_per_cu) return {h0->cfg.lanes_per_row, h0->cfg.wg_per_cu};
+  const hsc_config& c = h0->cfg;
+  if (pinned(c)) return {c.lanes_per_row, c.wg_per_cu, c.schedule};
   const TuneKey key = tune_key(hs, nsys);
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
-      return {h0->cfg.lanes_per_row ? h0->cfg.lanes_per_row : it->second.first,
-              h0->cfg.wg_per_cu ? h0->cfg.wg_per_cu : it->second.second};
+      return {c.lanes_per_row ? c.lanes_per_row : it->second.G, c.wg_per_cu ? c.wg_per_cu : it->second.wgpcu,
+              c.schedule ? c.schedule : it->second.sched};
     }
   }
   // time a few PCG iterations per candidate; the real solve re-initialises x, r
   const int Gs[3] = {4, 8, 16};
   double best = 1e300;
-  LaunchCfg bestc{8, 1};
+  LaunchCfg bestc{8, 1, 1};
   for (int gi = 0; gi < 3; ++gi) {
-    if (h0->cfg.lanes_per_row && Gs[gi] != h0->cfg.lanes_per_row) continue;
+    if (c.lanes_per_row && Gs[gi] != c.lanes_per_row) continue;
     for (int w = 1; w <= 2; ++w) {
-      if (h0->cfg.wg_per_cu && w != h0->cfg.wg_per_cu) continue;
-      LaunchCfg c{Gs[gi], w};
-      try {
-        float ms = 0.f;
-        std::vector<hsc::SysOut> o;
-        o = run_pcg(hs, nsys, c, 12, st, &ms);  // warm
-        o = run_pcg(hs, nsys, c, 12, st, &ms);
-        int it = 1;
-        for (auto& x : o) it = std::max(it, x.iterations);
-        const double per = ms / it;
-        if (per < best) {
-          best = per;
-          bestc = c;
+      if (c.wg_per_cu && w != c.wg_per_cu) continue;
+      for (int sc = 1; sc <= 2; ++sc) {
+        if (c.schedule && sc != c.schedule) continue;
+        LaunchCfg cand{Gs[gi], w, sc};
+        try {
+          float ms = 0.f;
+          std::vector<hsc::SysOut> o;
+          o = run_pcg(hs, nsys, cand, 12, st, &ms);  // warm
+          o = run_pcg(hs, nsys, cand, 12, st, &ms);
+          int it = 1;
+          for (auto& x : o) it = std::max(it, x.iterations);
+          const double per = ms / it;
+          if (per < best) {
+            best = per;
+            bestc = cand;
+          }
+        } catch (const HscError& e) {
+          if (e.code == HSC_ERR_CAPACITY) continue;  // candidate does not fit: try the next
+          throw;                                     // a HIP or device error ends the solve
         }
-      } catch (const HscError& e) {
-        if (e.code == HSC_ERR_CAPACITY) continue;  // candidate does not fit: try the next
-        throw;                                     // a HIP or device error ends the solve
       }
     }
   }
   std::lock_guard<std::mutex> g(g_tune_mu);
-  g_tune[key] = {bestc.G, bestc.wgpcu};
+  g_tune[key] = {bestc.G, bestc.wgpcu, bestc.sched};
   return bestc;
 }
 
@@ -945,6 +981,7 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   LaunchCfg lc = tuned_config(hs, 1, st);
   h->G = lc.G;
   h->wgpcu = lc.wgpcu;
+  h->sched = lc.sched;
   HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
                          h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
                          h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
@@ -1011,6 +1048,7 @@ void hsc_config_default(hsc_config* c) {
   c->aca_rtol = 0.0;
   c->lanes_per_row = 0;
   c->wg_per_cu = 0;
+  c->schedule = 0;
   c->preconditioner = HSC_PRECOND_NEUMANN;
   c->check_acceptance = 1;
 }
@@ -1166,7 +1204,7 @@ struct hsc_batch {
   std::vector<double*> x;
   PcgScratch sc;  // own launch scratch: eager solves on the handles do not touch the captured tables
   PcgPlan plan;
-  LaunchCfg lc{8, 1};
+  LaunchCfg lc{8, 1, 1};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int device = 0;
@@ -1329,12 +1367,13 @@ int32_t hsc_batch_stats(hsc_batch* bt, hsc_stats* stats, hsc_stream stream) {
 }
 
 int32_t hsc_batch_info(const hsc_batch* bt, int32_t* lanes_per_row, int32_t* wg_per_cu, int32_t* workgroups,
-                       int32_t* cooperative_node) {
+                       int32_t* cooperative_node, int32_t* schedule) {
   if (!bt) return HSC_ERR_INVALID_ARGUMENT;
   if (lanes_per_row) *lanes_per_row = bt->lc.G;
   if (wg_per_cu) *wg_per_cu = bt->lc.wgpcu;
   if (workgroups) *workgroups = bt->plan.P.nwg;
   if (cooperative_node) *cooperative_node = bt->cooperative_node;
+  if (schedule) *schedule = bt->lc.sched;
   return HSC_OK;
 }
 
@@ -1413,6 +1452,7 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   o->rank = h->k;
   o->lanes_per_row = h->G;
   o->workgroups = h->wgpcu;
+  o->schedule = h->sched;
   o->neumann_doublings = h->doublings;
   o->abd_block_diagonal = h->blockdiag ? 1 : 0;
   o->aca_relative_error = h->aca_err;

@@ -54,7 +54,7 @@ class HSCError(RuntimeError):
 def make_config(scene: dict | None = None, **extra) -> _abi.Config:
     """Build an hsc_config from libuipc-style keys plus extension fields
     (abd_inverse, precision, residual_replace_every,
-    true_residual_every_iteration, aca_rtol, lanes_per_row, wg_per_cu,
+    true_residual_every_iteration, aca_rtol, lanes_per_row, wg_per_cu, schedule,
     preconditioner, check_acceptance)."""
     lib = _abi.load()
     cfg = _abi.Config()
@@ -272,9 +272,10 @@ class Batch:
         return out
 
     def info(self) -> dict:
-        v = [C.c_int32() for _ in range(4)]
+        v = [C.c_int32() for _ in range(5)]
         self.lib.hsc_batch_info(self.b, *[C.byref(x) for x in v])
-        return dict(zip(("lanes_per_row", "wg_per_cu", "workgroups", "cooperative_node"), [x.value for x in v]))
+        return dict(zip(("lanes_per_row", "wg_per_cu", "workgroups", "cooperative_node", "schedule"),
+                        [x.value for x in v]))
 
     def close(self):
         if getattr(self, "b", None):

@@ -90,6 +90,9 @@ typedef struct hsc_config {
   int32_t wg_per_cu;             /* persistent workgroups per CU: 0 = autotuned, else 1..4 */
   int32_t preconditioner;        /* enum hsc_preconditioner (HSC_PRECOND_NEUMANN)  */
   int32_t check_acceptance;      /* 1 = fail on non-descent / full residual (reference); 0 = report only */
+  int32_t schedule;              /* persistent-PCG work split: 0 = autotuned; 1 = latency (grid sized by the
+                                    SpMV row tiles, 128-node elementwise tiles); 2 = throughput (grid covers
+                                    row and low-rank tiles, split by tile count, 512-node tiles) */
 } hsc_config;
 
 /* Device view of the assembled Hessian: the stored triangle of 3x3 blocks
@@ -179,6 +182,7 @@ typedef struct hsc_setup_info {
   int32_t fem_dofs, abd_dofs, fem_nodes, nnzb, active_fem_dofs, rank;
   int32_t lanes_per_row, workgroups, neumann_doublings, abd_block_diagonal;
   double aca_relative_error, abd_inverse_residual;
+  int32_t schedule, reserved;
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* P,
@@ -200,7 +204,7 @@ int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double
 int32_t hsc_batch_launch(hsc_batch* batch, hsc_stream stream);
 int32_t hsc_batch_stats(hsc_batch* batch, hsc_stats* stats, hsc_stream stream);
 int32_t hsc_batch_info(const hsc_batch* batch, int32_t* lanes_per_row, int32_t* wg_per_cu, int32_t* workgroups,
-                       int32_t* cooperative_node);
+                       int32_t* cooperative_node, int32_t* schedule);
 void hsc_batch_destroy(hsc_batch* batch);
 
 /* ---- learned preconditioner (no reference counterpart) ----
