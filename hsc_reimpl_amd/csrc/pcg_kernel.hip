@@ -117,6 +117,162 @@ __device__ __forceinline__ void tile_N(const SysDesc& S, int tl, const VT* __res
   }
 }
 
+// ---------------------------------------------------------------- resident rows
+// When a workgroup owns exactly one row tile in the Neumann chunking (the
+// latency schedule), the first U blocks of each lane's share of its row --
+// column indices and values -- and the row's inverse diagonal block are
+// loaded once per launch and kept in registers; every Neumann pass then
+// costs one gather latency instead of the row-pointer -> index -> gather
+// chain.  Blocks beyond the first U per lane are streamed as before.
+template <typename MT>
+struct ResidentU {
+  static constexpr int value = 4;
+};
+template <>
+struct ResidentU<double> {
+  static constexpr int value = 2;
+};
+
+template <int G, typename MT>
+struct RowRes {
+  static constexpr int U = ResidentU<MT>::value * 4 / G > 0 ? ResidentU<MT>::value * 4 / G : 1;
+  int p0, p1;          // block range of the row (p1 == p0: no row)
+  int c[U];
+  MT v[U][9];          // zero beyond the row's end (the column then is the row itself)
+  MT d0, d1, d2;       // row `lane` (< 3) of the inverse diagonal block
+};
+
+template <int G, typename MT>
+__device__ __forceinline__ void row_preload(const SysDesc& S, const MT* __restrict__ val, const MT* __restrict__ inv,
+                                            int row, int lane, RowRes<G, MT>& rr) {
+  constexpr int U = RowRes<G, MT>::U;
+  rr.p0 = rr.p1 = 0;
+  rr.d0 = rr.d1 = rr.d2 = MT(0);
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    rr.c[u] = 0;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) rr.v[u][e] = MT(0);
+  }
+  if (row >= S.nb) return;
+  rr.p0 = S.rowptr[row];
+  rr.p1 = S.rowptr[row + 1];
+  const int nnzb = S.nnzb;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int p = rr.p0 + lane + u * G;
+    rr.c[u] = row;
+    if (p < rr.p1) {
+      rr.c[u] = S.colidx[p];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) rr.v[u][e] = val[p + (size_t)e * nnzb];
+    }
+  }
+  if (lane < 3) {
+    rr.d0 = inv[9 * row + 3 * lane];
+    rr.d1 = inv[9 * row + 3 * lane + 1];
+    rr.d2 = inv[9 * row + 3 * lane + 2];
+  }
+}
+
+// row_spmv with the first U blocks of each lane from registers
+template <int G, typename MT, typename Src>
+__device__ __forceinline__ void row_spmv_res(const SysDesc& S, const MT* __restrict__ val, const RowRes<G, MT>& rr,
+                                             int lane, Src src, MT& a0, MT& a1, MT& a2) {
+  constexpr int U = RowRes<G, MT>::U;
+  a0 = a1 = a2 = MT(0);
+  if (rr.p1 > rr.p0) {
+    MT x[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) src(rr.c[u], x[u][0], x[u][1], x[u][2]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (rr.p0 + lane + u * G < rr.p1) {
+        a0 += rr.v[u][0] * x[u][0] + rr.v[u][3] * x[u][1] + rr.v[u][6] * x[u][2];
+        a1 += rr.v[u][1] * x[u][0] + rr.v[u][4] * x[u][1] + rr.v[u][7] * x[u][2];
+        a2 += rr.v[u][2] * x[u][0] + rr.v[u][5] * x[u][1] + rr.v[u][8] * x[u][2];
+      }
+    }
+    const int nnzb = S.nnzb;
+    for (int p = rr.p0 + lane + U * G; p < rr.p1; p += G) {
+      const int c = S.colidx[p];
+      MT x0, x1, x2;
+      src(c, x0, x1, x2);
+      const MT* v = val + p;
+      const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
+      const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
+      const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
+      a0 += v0 * x0 + v3 * x1 + v6 * x2;
+      a1 += v1 * x0 + v4 * x1 + v7 * x2;
+      a2 += v2 * x0 + v5 * x1 + v8 * x2;
+    }
+  }
+  a0 = group_sum<G>(a0);
+  a1 = group_sum<G>(a1);
+  a2 = group_sum<G>(a2);
+}
+
+// Neumann step on the resident row tile (same arithmetic as tile_N)
+template <int G, typename VT>
+__device__ __forceinline__ void tile_N_res(const SysDesc& S, int row, const RowRes<G, VT>& rr,
+                                           const VT* __restrict__ zin, VT* __restrict__ zout, VT omega, bool last,
+                                           double& rz) {
+  const int lane = threadIdx.x % G;
+  const bool own = row < S.nb && lane < 3;
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  VT zo = VT(0);
+  if (own) {
+    r0 = S.r[3 * row];
+    r1 = S.r[3 * row + 1];
+    r2 = S.r[3 * row + 2];
+    zo = zin[3 * row + lane];
+  }
+  VT a0, a1, a2;
+  row_spmv_res<G, VT>(S, mat_vals<VT>(S), rr, lane,
+                      [&](int c, VT& x0, VT& x1, VT& x2) {
+                        x0 = zin[3 * c];
+                        x1 = zin[3 * c + 1];
+                        x2 = zin[3 * c + 2];
+                      },
+                      a0, a1, a2);
+  if (own) {
+    const VT e0 = (VT)r0 - a0, e1 = (VT)r1 - a1, e2 = (VT)r2 - a2;
+    const VT out = zo + omega * (rr.d0 * e0 + rr.d1 * e1 + rr.d2 * e2);
+    zout[3 * row + lane] = out;
+    if (last) rz += pick3(lane, r0, r1, r2) * (double)out;
+  }
+}
+
+// stage A on the resident row tile (same arithmetic as tile_A)
+template <int G, typename VT>
+__device__ __forceinline__ void tile_A_res(const SysDesc& S, int row, const RowRes<G, VT>& rr,
+                                           const VT* __restrict__ z, const VT* __restrict__ pold,
+                                           VT* __restrict__ pnew, VT* __restrict__ q, VT beta, double& dot) {
+  const int lane = threadIdx.x % G;
+  const bool own = row < S.nb && lane < 3;
+  VT zo = VT(0), po = VT(0);
+  if (own) {
+    zo = z[3 * row + lane];
+    po = pold[3 * row + lane];
+  }
+  VT a0, a1, a2;
+  row_spmv_res<G, VT>(S, mat_vals<VT>(S), rr, lane,
+                      [&](int c, VT& x0, VT& x1, VT& x2) {
+                        x0 = combine(z[3 * c], pold[3 * c], beta);
+                        x1 = combine(z[3 * c + 1], pold[3 * c + 1], beta);
+                        x2 = combine(z[3 * c + 2], pold[3 * c + 2], beta);
+                      },
+                      a0, a1, a2);
+  if (own) {
+    const int idx = 3 * row + lane;
+    const VT pn = combine(zo, po, beta);
+    const VT qv = pick3(lane, a0, a1, a2);
+    pnew[idx] = pn;
+    q[idx] = qv;
+    dot += (double)pn * (double)qv;
+  }
+}
+
 // D x in FP64 on one row tile (true residual / operator tests)
 template <int G>
 __device__ __forceinline__ void tile_Dx64(const SysDesc& S, int tl, const double* __restrict__ x,
@@ -324,8 +480,10 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   TileRange trA{0, 0}, trF{0, 0}, trE{0, 0}, trL{0, 0};
   const int cA = P.chunkA[b], cF = P.chunkF[b], cL = P.chunkL[b];
   if (!is_lr) trA = {tile_of_chunk(cA, P.nA, P.rt_total), tile_of_chunk(cA + 1, P.nA, P.rt_total)};
-  trF = {tile_of_chunk(cF, P.nF, P.rt_total), tile_of_chunk(cF + 1, P.nF, P.rt_total)};
-  trE = {tile_of_chunk(cF, P.nF, P.et_total), tile_of_chunk(cF + 1, P.nF, P.et_total)};
+  if (cF >= 0) {  // (no row or node tiles for the low-rank workgroups of the aligned split)
+    trF = {tile_of_chunk(cF, P.nF, P.rt_total), tile_of_chunk(cF + 1, P.nF, P.rt_total)};
+    trE = {tile_of_chunk(cF, P.nF, P.et_total), tile_of_chunk(cF + 1, P.nF, P.et_total)};
+  }
   if (is_lr) trL = {tile_of_chunk(cL, P.nL, P.lt_total), tile_of_chunk(cL + 1, P.nL, P.lt_total)};
 
   for (int s = tid; s < nsys; s += kBlock) {
@@ -357,6 +515,18 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     }
     return ok;
   };
+
+  // resident Neumann row tile (exactly one row tile in the Neumann chunking)
+  const bool resF = trF.t1 - trF.t0 == 1;
+  int sF = 0, rowF = 0;
+  RowRes<G, VT> rrF;
+  if (resF) {
+    sF = sys_of_tile_begin(sys, nsys, trF.t0, 0);
+    rowF = (trF.t0 - sys[sF].rt_begin) * (kBlock / G) + tid / G;
+    row_preload<G, VT>(sys[sF], mat_vals<VT>(sys[sF]), inv_vals<VT>(sys[sF]), rowF, tid % G, rrF);
+  }
+  // stage A on the same resident tile (aligned split)
+  const bool resA = resF && !is_lr && P.nA == P.nF && cA == cF;
 
   const bool started = gb_init(gbar, P.barrier, P.error, P.timeout_ticks, (unsigned)P.nwg);
   if (P.prof && tid == 0) t_last = rt_ticks();
@@ -446,19 +616,26 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     for (int j = 1; j < order; ++j) {
       const bool last = (j == order - 1);
       double rz = 0.0;
-      for_tiles(sys, nsys, trF, 0,
-                [&](int s, int tl) {
-                  if (s_st[s] != ST_RUN) return;
-                  const SysDesc& S = sys[s];
-                  tile_N<G, VT>(S, tl, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], omega, last, rz);
-                },
-                [&](int s) {
-                  if (last) {
-                    const double c = block_sum_t0(rz, s_red);
-                    if (s_st[s] == ST_RUN && tid == 0) P.partF[(size_t)s * P.nF + cF] = c;
-                    rz = 0.0;
-                  }
-                });
+      auto flushF = [&](int s) {
+        if (last) {
+          const double c = block_sum_t0(rz, s_red);
+          if (s_st[s] == ST_RUN && tid == 0) P.partF[(size_t)s * P.nF + cF] = c;
+          rz = 0.0;
+        }
+      };
+      if (resF) {
+        const SysDesc& S = sys[sF];
+        if (s_st[sF] == ST_RUN) tile_N_res<G, VT>(S, rowF, rrF, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], omega, last, rz);
+        flushF(sF);
+      } else {
+        for_tiles(sys, nsys, trF, 0,
+                  [&](int s, int tl) {
+                    if (s_st[s] != ST_RUN) return;
+                    const SysDesc& S = sys[s];
+                    tile_N<G, VT>(S, tl, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], omega, last, rz);
+                  },
+                  flushF);
+      }
       zc ^= 1;
       if (!sync_at(1)) return false;
     }
@@ -581,6 +758,14 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     // ---------------- stage A
     if (is_lr) {
       lr_stage(0);
+    } else if (resA) {
+      double dot = 0.0;
+      const SysDesc& S = sys[sF];
+      if (s_st[sF] == ST_RUN)
+        tile_A_res<G, VT>(S, rowF, rrF, (const VT*)S.z[zc], (const VT*)S.p[pc], (VT*)S.p[pc ^ 1], (VT*)S.q,
+                          (VT)s_beta[sF], dot);
+      const double d = block_sum_t0(dot, s_red);
+      if (s_st[sF] == ST_RUN && tid == 0) P.partA[(size_t)sF * P.nA + cA] = d;
     } else {
       double dot = 0.0;
       for_tiles(sys, nsys, trA, 0,

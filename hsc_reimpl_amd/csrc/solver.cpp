@@ -293,25 +293,39 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   // cost about one dependent-load chain (measured on MI355X), so the share
   // follows the tile counts and the grid covers both kinds when the
   // co-resident capacity allows.
+  // Aligned latency split: when every row tile gets its own workgroup and
+  // room is left, the row workgroups own the same row tile in stage A and in
+  // the Neumann passes (so its matrix rows stay in registers), and the
+  // remaining workgroups take the low-rank tiles.
   const bool thr = lc.sched == 2;
-  int nwg = std::min(std::min(lc.wgpcu, occ) * di.cu, std::max(thr ? rt + lt : rt, 1));
+  const int capw = std::min(lc.wgpcu, occ) * di.cu;
+  bool aligned = false;
+  int nwg = std::min(capw, std::max(thr ? rt + lt : rt, 1));
   int nL = 0;
-  if (lt > 0) {
+  if (!thr && rt + (lt > 0 ? 1 : 0) <= capw) {
+    aligned = true;
+    nL = lt > 0 ? std::min(lt, capw - rt) : 0;
+    nwg = rt + nL;
+  } else if (lt > 0) {
     nwg = std::max(nwg, 2);
     nL = thr ? (int)std::lround((double)nwg * lt / (double)(rt + lt))
              : (int)std::lround(nwg * lr_bytes / std::max(lr_bytes + row_bytes, 1.0));
     nL = std::max(1, std::min(nL, std::min(lt, nwg - 1)));
   }
   if (nwg > cap) throw HscError{HSC_ERR_CAPACITY, "grid exceeds co-resident capacity"};
-  const int nA = nwg - nL, nF = nwg;
+  const int nA = nwg - nL, nF = aligned ? nA : nwg;
   std::vector<int> chA(nwg, -1), chF(nwg, -1), chL(nwg, -1), tmp(nwg, -1);
-  xcd_ranks(0, nwg, chF);
   if (nL > 0) {
     xcd_ranks(0, nL, tmp);
     for (int b = 0; b < nL; ++b) chL[b] = tmp[b];
   }
   xcd_ranks(nL, nA, tmp);
   for (int b = nL; b < nwg; ++b) chA[b] = tmp[b];
+  if (aligned) {
+    for (int b = nL; b < nwg; ++b) chF[b] = chA[b];  // low-rank workgroups: no row or node tiles
+  } else {
+    xcd_ranks(0, nwg, chF);
+  }
   int lrp = 0;
   for (int s = 0; s < nsys; ++s) {
     S[s].cA0 = chunk_of(S[s].rt_begin, nA, rt);
