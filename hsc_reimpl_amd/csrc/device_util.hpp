@@ -162,7 +162,9 @@ __device__ bool gb_sync(GridBarrier& B) {
 }
 
 __device__ __forceinline__ int tile_of_chunk(int c, int nC, int T) {
-  return (int)(((long long)c * T) / nC);
+  const long long prod = (long long)c * T;
+  if (prod < (1LL << 31)) return (int)((unsigned)prod / (unsigned)nC);  // 32-bit division when it fits
+  return (int)(prod / nC);
 }
 
 }  // namespace hsc

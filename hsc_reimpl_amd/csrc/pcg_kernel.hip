@@ -323,20 +323,11 @@ __device__ __forceinline__ void tile_Wt(const SysDesc& S, int tl, const WT* __re
   }
 }
 
-// g_d = sum_l W[an+d, l] t[l] for the coupled nodes of one node tile (an =
-// the node's first W row, -1 if uncoupled).  The coupled nodes are listed in
-// LDS; groups of 4 lanes take one node each (128 nodes per pass of the
-// workgroup), every lane strides the rank by 4 and has its W loads
-// for the three rows in flight before the group sum.  Uniform over the
-// workgroup.
-template <typename WT>
-__device__ __forceinline__ void tile_lowrank_g(const SysDesc& S, int an, const WT* __restrict__ W,
-                                               const double* __restrict__ t, double g[3], int* s_list,
-                                               int* s_slot, double (*s_g)[3], int* s_cnt) {
-  constexpr int GL = 4;               // lanes per node
-  constexpr int NPR = kBlock / GL;    // nodes per pass
+// List the coupled nodes of one node tile in LDS (an = the node's first W
+// row, -1 if uncoupled): s_list[slot] = an, s_slot[tid] = slot.  Uniform
+// over the workgroup.
+__device__ __forceinline__ void lowrank_list(int an, int* s_list, int* s_slot, int* s_cnt) {
   const int tid = threadIdx.x;
-  const int k = S.k;
   if (tid == 0) *s_cnt = 0;
   __syncthreads();
   if (an >= 0) {
@@ -345,6 +336,19 @@ __device__ __forceinline__ void tile_lowrank_g(const SysDesc& S, int an, const W
     s_slot[tid] = slot;
   }
   __syncthreads();
+}
+
+// g_d = sum_l W[an+d, l] t[l] for the listed coupled nodes: groups of 4
+// lanes take one node each (128 nodes per pass of the workgroup), every lane
+// strides the rank by 4 and has its W loads for the three rows in flight
+// before the group sum.  Uniform over the workgroup.
+template <typename WT>
+__device__ __forceinline__ void lowrank_apply(int k, int an, const WT* __restrict__ W, const double* __restrict__ t,
+                                              double g[3], const int* s_list, const int* s_slot,
+                                              double (*s_g)[3], const int* s_cnt) {
+  constexpr int GL = 4;               // lanes per node
+  constexpr int NPR = kBlock / GL;    // nodes per pass
+  const int tid = threadIdx.x;
   const int m = *s_cnt;
   const int sub = tid % GL, grp = tid / GL;
   for (int base = 0; base < m; base += NPR) {
@@ -378,6 +382,14 @@ __device__ __forceinline__ void tile_lowrank_g(const SysDesc& S, int an, const W
     g[1] = s_g[slot][1];
     g[2] = s_g[slot][2];
   }
+}
+
+template <typename WT>
+__device__ __forceinline__ void tile_lowrank_g(const SysDesc& S, int an, const WT* __restrict__ W,
+                                               const double* __restrict__ t, double g[3], int* s_list,
+                                               int* s_slot, double (*s_g)[3], int* s_cnt) {
+  lowrank_list(an, s_list, s_slot, s_cnt);
+  lowrank_apply(S.k, an, W, t, g, s_list, s_slot, s_g, s_cnt);
 }
 
 // sum over the four waves' acc, written to dst[l] for l < k (thread-ordered)
@@ -442,14 +454,19 @@ __device__ __forceinline__ void for_tiles(const SysDesc* sys, int nsys, TileRang
 // Chunks that own no tile of the tiling (T tiles over nC chunks) never write
 // a partial and are skipped.
 __device__ __forceinline__ bool chunk_empty(int c, int nC, int T) {
+  if (nC <= T) return false;  // every chunk owns at least one tile (no division needed)
   return tile_of_chunk(c, nC, T) == tile_of_chunk(c + 1, nC, T);
 }
 __device__ __forceinline__ double wave_reduce_chunks(const double* part, int s, int nC, int c0, int c1, int T) {
   const int lane = threadIdx.x & 63;
   double v = 0.0;
+  // loads are unconditional (every chunk slot exists) so they issue together;
+  // slots of empty chunks hold stale data and are masked out by a select
 #pragma unroll 4
-  for (int c = c0 + lane; c <= c1; c += 64)
-    if (!chunk_empty(c, nC, T)) v += part[(size_t)s * nC + c];
+  for (int c = c0 + lane; c <= c1; c += 64) {
+    const double x = part[(size_t)s * nC + c];
+    v += chunk_empty(c, nC, T) ? 0.0 : x;
+  }
   return wave_sum(v);
 }
 
@@ -527,10 +544,21 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   }
   // stage A on the same resident tile (aligned split)
   const bool resA = resF && !is_lr && P.nA == P.nF && cA == cF;
+  // resident node tile (exactly one node tile): its node and coupled-node
+  // list are fixed for the launch, and stage B's vector loads are issued
+  // before the stage-A reductions
+  const bool resE = trE.t1 - trE.t0 == 1;
+  int sE = 0, iE = 0, anE = -1;
+  if (resE) {
+    sE = sys_of_tile_begin(sys, nsys, trE.t0, 1);
+    iE = (trE.t0 - sys[sE].et_begin) * P.node_tile + tid;
+    if (tid < P.node_tile && iE < sys[sE].nb && sys[sE].k > 0) anE = sys[sE].act_node[iE];
+  }
 
   const bool started = gb_init(gbar, P.barrier, P.error, P.timeout_ticks, (unsigned)P.nwg);
   if (P.prof && tid == 0) t_last = rt_ticks();
   if (started) {
+  if (resE) lowrank_list(anE, s_list, s_slot, &s_cnt);  // kept for the whole launch
   int zc = 0, pc = 0;
   const int order = P.order;
   double* partE_rr = P.partE;
@@ -538,6 +566,27 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
 
   // ---- node-tile stage: z0 = w Dinv r (and residual norms) ------------
   // kind 0: init (r == rhs), 1: stage B, 2: T2
+  // stage-B vectors of the resident node tile, loaded before the reductions
+  VT preq[3] = {VT(0), VT(0), VT(0)}, prep[3] = {VT(0), VT(0), VT(0)};
+  double prer[3] = {0.0, 0.0, 0.0}, prex[3] = {0.0, 0.0, 0.0};
+  bool pre_ok = false;
+  auto prefetch_B = [&]() {
+    pre_ok = false;
+    if (!resE || s_st[sE] != ST_RUN) return;
+    const SysDesc& S = sys[sE];
+    if (tid >= P.node_tile || iE >= S.nb) return;
+    const int i3 = 3 * iE;
+    const VT* q = (const VT*)S.q;
+    const VT* pn = (const VT*)S.p[pc];
+    for (int d = 0; d < 3; ++d) {
+      preq[d] = q[i3 + d];
+      prep[d] = pn[i3 + d];
+      prer[d] = S.r[i3 + d];
+      prex[d] = S.x[i3 + d];
+    }
+    pre_ok = true;
+  };
+
   auto node_stage = [&](int kind) {
     double rr = 0.0, rz = 0.0;
     for_tiles(sys, nsys, trE, 1,
@@ -550,11 +599,12 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                 const bool valid = tid < P.node_tile && i < S.nb;
                 double g[3] = {0.0, 0.0, 0.0};
                 if (kind != 0 && S.k > 0) {
-                  const int an = valid ? S.act_node[i] : -1;
+                  const int an = resE ? anE : (valid ? S.act_node[i] : -1);
+                  if (!resE) lowrank_list(an, s_list, s_slot, &s_cnt);
                   if (kind == 1)
-                    tile_lowrank_g(S, an, w_vals<VT>(S), s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
+                    lowrank_apply(S.k, an, w_vals<VT>(S), s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
                   else
-                    tile_lowrank_g(S, an, S.W64, s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
+                    lowrank_apply(S.k, an, S.W64, s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
                 }
                 if (!valid) return;
                 const int i3 = 3 * i;
@@ -566,13 +616,21 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   rv[2] = S.r[i3 + 2];
                 } else {
                   if (kind == 1) {
-                    const VT* q = (const VT*)S.q;
-                    const VT* pn = (const VT*)S.p[pc];
                     const double al = s_alpha[s];
-                    for (int d = 0; d < 3; ++d) {
-                      const double qd = (double)q[i3 + d] - g[d];
-                      rv[d] = S.r[i3 + d] - al * qd;
-                      S.x[i3 + d] += al * (double)pn[i3 + d];
+                    if (pre_ok) {
+                      for (int d = 0; d < 3; ++d) {
+                        const double qd = (double)preq[d] - g[d];
+                        rv[d] = prer[d] - al * qd;
+                        S.x[i3 + d] = prex[d] + al * (double)prep[d];
+                      }
+                    } else {
+                      const VT* q = (const VT*)S.q;
+                      const VT* pn = (const VT*)S.p[pc];
+                      for (int d = 0; d < 3; ++d) {
+                        const double qd = (double)q[i3 + d] - g[d];
+                        rv[d] = S.r[i3 + d] - al * qd;
+                        S.x[i3 + d] += al * (double)pn[i3 + d];
+                      }
                     }
                     write_r = true;
                   } else {
@@ -653,39 +711,97 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     __syncthreads();
   };
 
-  // reduce low-rank partials into s_t for systems selected by `want`
-  auto reduce_t = [&](auto want) {
-    for (int s = 0; s < nsys; ++s) {
-      const SysDesc& S = sys[s];
-      if (!want(s) || S.k == 0) continue;
-      // thread (grp, l) sums chunks cL0+grp, +ng, ...; groups are then added in order
-      const int k = S.k;
-      const int ng = max(1, kBlock / k);
-      const int l = tid % k, grp = tid / k;
-      if (k <= kBlock) {
-        double v = 0.0;
-        if (grp < ng) {
+  // Reduce the low-rank partials into s_t (and |t|^2 into s_tt) for the
+  // systems selected by `want`; with with_pA also the stage-A partials
+  // <p, D p> into s_v1 (every system).  When the ranks of all systems fit in
+  // one workgroup (sum k <= kBlock), thread (g, q) sums every ng-th chunk of
+  // entry q of the concatenated t vectors and the groups are added in order;
+  // the stage-A partials are reduced by the waves in between, so all loads
+  // are in flight before the first wait.  Larger rank totals go system by
+  // system.
+  auto reduce_t = [&](auto want, bool with_pA) {
+    const int T = P.t_total;
+    if (T <= kBlock) {
+      const int ng = max(1, kBlock / max(T, 1));
+      const int q = tid % max(T, 1), grp = tid / max(T, 1);
+      double v = 0.0;
+      if (T > 0 && grp < ng) {
+        int s = 0;
+        while (s + 1 < nsys && sys[s + 1].t_off <= q) ++s;
+        const SysDesc& S = sys[s];
+        if (want(s) && S.k > 0) {
+          const int l = q - S.t_off;
 #pragma unroll 8
-          for (int c = S.cL0 + grp; c <= S.cL1; c += ng)
-            if (!chunk_empty(c, P.nL, P.lt_total)) v += P.partL[S.lrp_off + (size_t)(c - S.cL0) * k + l];
+          for (int c = S.cL0 + grp; c <= S.cL1; c += ng) {
+            const double x = P.partL[S.lrp_off + (size_t)(c - S.cL0) * S.k + l];
+            v += chunk_empty(c, P.nL, P.lt_total) ? 0.0 : x;
+          }
         }
-        if (grp < ng) s_tred[grp * k + l] = v;
-        __syncthreads();
-        if (tid < k) {
-          double acc = 0.0;
-          for (int q = 0; q < ng; ++q) acc += s_tred[q * k + tid];
-          s_t[S.t_off + tid] = acc;
+      }
+      if (with_pA) {
+        for (int s = wave; s < nsys; s += kBlock / 64) {
+          const SysDesc& S = sys[s];
+          const double va = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1, P.rt_total);
+          if ((tid & 63) == 0) s_v1[s] = va;
         }
-        __syncthreads();
+      }
+      if (T > 0 && grp < ng) s_tred[grp * T + q] = v;
+      __syncthreads();
+      if (tid < T) {
+        double acc = 0.0;
+        for (int g = 0; g < ng; ++g) acc += s_tred[g * T + tid];
+        s_t[tid] = acc;
+      }
+      __syncthreads();
+      for (int s = wave; s < nsys; s += kBlock / 64) {
+        const SysDesc& S = sys[s];
+        double a = 0.0;
+        if (want(s))
+          for (int j = tid & 63; j < S.k; j += 64) a += s_t[S.t_off + j] * s_t[S.t_off + j];
+        a = wave_sum(a);
+        if ((tid & 63) == 0) s_tt[s] = a;
+      }
+      __syncthreads();
+      return;
+    }
+    if (with_pA) {
+      for (int s = wave; s < nsys; s += kBlock / 64) {
+        const SysDesc& S = sys[s];
+        const double va = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1, P.rt_total);
+        if ((tid & 63) == 0) s_v1[s] = va;
       }
     }
-    __syncthreads();
-    for (int s = tid; s < nsys; s += kBlock) {
+    for (int s = 0; s < nsys; ++s) {
       const SysDesc& S = sys[s];
-      double tt = 0.0;
-      if (want(s))
-        for (int l = 0; l < S.k; ++l) tt += s_t[S.t_off + l] * s_t[S.t_off + l];
-      s_tt[s] = tt;
+      const int k = S.k;
+      if (!want(s) || k == 0) {
+        if (tid == 0) s_tt[s] = 0.0;
+        continue;
+      }
+      const int ng = max(1, kBlock / k);
+      const int l = tid % k, grp = tid / k;
+      double v = 0.0;
+      if (grp < ng) {
+#pragma unroll 8
+        for (int c = S.cL0 + grp; c <= S.cL1; c += ng) {
+          const double x = P.partL[S.lrp_off + (size_t)(c - S.cL0) * k + l];
+          v += chunk_empty(c, P.nL, P.lt_total) ? 0.0 : x;
+        }
+        s_tred[grp * k + l] = v;
+      }
+      __syncthreads();
+      if (tid < k) {
+        double acc = 0.0;
+        for (int g = 0; g < ng; ++g) acc += s_tred[g * k + tid];
+        s_t[S.t_off + tid] = acc;
+      }
+      __syncthreads();
+      if (wave == 0) {
+        double a = 0.0;
+        for (int j = tid; j < k; j += 64) a += s_t[S.t_off + j] * s_t[S.t_off + j];
+        a = wave_sum(a);
+        if (tid == 0) s_tt[s] = a;
+      }
     }
     __syncthreads();
   };
@@ -783,12 +899,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     }
     pc ^= 1;
     if (!sync_at(2)) goto out;
-    for (int s = wave; s < nsys; s += kBlock / 64) {
-      const SysDesc& S = sys[s];
-      const double v = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1, P.rt_total);
-      if ((tid & 63) == 0) s_v1[s] = v;
-    }
-    reduce_t([&](int s) { return s_st[s] == ST_RUN; });
+    prefetch_B();
+    reduce_t([&](int s) { return s_st[s] == ST_RUN; }, true);
     for (int s = tid; s < nsys; s += kBlock) {
       if (s_st[s] != ST_RUN) continue;
       const double pAp = s_v1[s] - s_tt[s];
@@ -803,6 +915,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
 
     // ---------------- stage B
     node_stage(1);
+    pre_ok = false;
     if (!sync_at(3)) goto out;
     for (int s = wave; s < nsys; s += kBlock / 64) {
       const SysDesc& S = sys[s];
@@ -844,7 +957,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   [&](int) {});
       }
       if (!sync_at(4)) goto out;
-      reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; });
+      reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, false);
       node_stage(2);
       if (!sync_at(5)) goto out;
       for (int s = wave; s < nsys; s += kBlock / 64) {
