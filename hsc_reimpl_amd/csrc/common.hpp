@@ -7,7 +7,7 @@ namespace hsc {
 
 constexpr int kBlock = 512;        // threads per workgroup (8 wave64)
 constexpr int kNodeTileMax = 512;  // nodes per elementwise tile (one node per thread; runtime 128 or 512)
-constexpr int kLrTile = 64;        // low-rank rows (W rows) per tile
+constexpr int kLrTile = 128;       // low-rank rows (W rows) per tile
 constexpr int kMaxSys = 128;       // systems per persistent launch
 constexpr int kMaxRankLds = 256;   // rank limit of the low-rank apply (lanes hold columns l, l+64, l+128, l+192)
 constexpr int kBarrierWords = 1024;  // grid-barrier counters (see device_util.hpp)

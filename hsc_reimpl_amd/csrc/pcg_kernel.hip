@@ -291,34 +291,48 @@ __device__ __forceinline__ void tile_Dx64(const SysDesc& S, int tl, const double
 }
 
 // low-rank partial t[l] += sum_{j in tile} W[j,l] src(act_dof[j]); acc[c]
-// holds column l = lane + 64c.  Each wave owns 16 consecutive rows of the
-// tile: lanes 0..15 fetch the 16 source values at once, then every lane
-// issues its 16 W loads before the multiply-adds (one latency, not 16).
+// holds column l = lane + 64c.  Each wave owns RW consecutive rows of the
+// tile: lanes 0..RW-1 fetch the source values at once, then every lane
+// issues its W loads for RB rows at a time before the multiply-adds; with
+// FP32 W both batches of a wave's rows are in flight together.  Loads are
+// unconditional (indices clamped into the tile) and masked by a select.
 template <typename WT, typename Src>
 __device__ __forceinline__ void tile_Wt(const SysDesc& S, int tl, const WT* __restrict__ W, Src src,
                                         double acc[4]) {
   constexpr int RW = kLrTile / (kBlock / 64);  // rows per wave
-  constexpr int RB = 8;                        // rows whose W loads are in flight together
+  constexpr int RB = 8;                        // rows per batch of W loads
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int jw = tl * kLrTile + wave * RW;
-  const int j1 = min(tl * kLrTile + kLrTile, S.a);
+  const int j0 = tl * kLrTile;
+  const int jw = j0 + wave * RW;
+  const int j1 = min(j0 + kLrTile, S.a);
   const int k = S.k;
   double my_src = 0.0;
   if (lane < RW && jw + lane < j1) my_src = src(S.act_dof[jw + lane]);
-#pragma unroll 1
-  for (int jb = 0; jb < RW; jb += RB) {
-#pragma unroll 1
-    for (int c = 0; c < 4; ++c) {
-      const int l = lane + 64 * c;
-      if (64 * c >= k) break;
-      double w[RB];
+  auto batch = [&](int jb, int c) {
+    const int l = lane + 64 * c;
+    const int lc = min(l, k - 1);
+    WT w[RB];
 #pragma unroll
-      for (int jj = 0; jj < RB; ++jj) {
-        const int j = jw + jb + jj;
-        w[jj] = (l < k && j < j1) ? (double)W[(size_t)j * k + l] : 0.0;
-      }
+    for (int jj = 0; jj < RB; ++jj) {
+      const int j = min(jw + jb + jj, j1 - 1);
+      w[jj] = W[(size_t)j * k + lc];
+    }
 #pragma unroll
-      for (int jj = 0; jj < RB; ++jj) acc[c] += w[jj] * __shfl(my_src, jb + jj, 64);
+    for (int jj = 0; jj < RB; ++jj) {
+      const bool ok = l < k && jw + jb + jj < j1;
+      acc[c] += (ok ? (double)w[jj] : 0.0) * __shfl(my_src, jb + jj, 64);
+    }
+  };
+  if (jw >= j1) return;
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {
+    if (64 * c >= k) break;
+    if constexpr (sizeof(WT) == 4) {
+#pragma unroll
+      for (int jb = 0; jb < RW; jb += RB) batch(jb, c);
+    } else {
+#pragma unroll 1
+      for (int jb = 0; jb < RW; jb += RB) batch(jb, c);
     }
   }
 }
