@@ -264,6 +264,15 @@ def main():
     prof = ROOT / "profiles" / "pcg_persistent_traffic.json"
     if prof.exists():
         traffic = json.loads(prof.read_text()).get("traffic_bytes_per_launch_mixed")
+    learned_c1 = None
+    from hsc_reimpl_amd import learned
+    if learned.WEIGHTS.exists():
+        r = learned.evaluate(learned.load_weights(), ["c1"], device=dev)[0]
+        learned_c1 = {"system": "2,178-DOF system (Neumann order 2, omega 0.6, rtol 1e-8)",
+                      "loop": "host-driven flexible PCG, same operators, with and without the graph MLP",
+                      "iterations": {"plain": r["plain"]["iterations"], "learned": r["learned"]["iterations"]},
+                      "ms": {"plain": round(r["plain"]["ms"], 2), "learned": round(r["learned"]["ms"], 2)},
+                      "converged": r["plain"]["status"] == 0 and r["learned"]["status"] == 0}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sysm, order, omega, rtol)
@@ -289,6 +298,7 @@ def main():
                      "bytes_per_iteration": bpi, "iterations": new["pcg_iterations"],
                      "kernel_ms": round(new["pcg_ms"], 3)},
         "cpu_baseline": cpu,
+        "learned_preconditioner": learned_c1,
     }
     print(json.dumps(line), flush=True)
 

@@ -166,6 +166,25 @@ def test_vehicle_scale_system():
     assert np.linalg.norm(x3 - 1e-6 * x) / np.linalg.norm(1e-6 * x) <= 1e-8
 
 
+@pytest.mark.parametrize("G", [4, 8, 16])
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_launch_shapes_agree(c1, c1_oracle, G, schedule):
+    """Every launch shape the tuner can pick (lanes per block row x work
+    split) solves the 2,178-DOF system to the same answer: iterations within
+    1 of the oracle, solution within 1e-7 of it (the shapes differ only in
+    the fixed order of the partial sums)."""
+    h, _ = _setup(c1, C1_SCENE, aca_rtol=1e-6, lanes_per_row=G, wg_per_cu=1, schedule=schedule)
+    x, st = _solve(h, c1)
+    assert st["status"] == 0
+    info = h.setup_info()
+    assert info["schedule"] == schedule and info["lanes_per_row"] == G
+    n = c1.fn
+    dx, dq, ost = c1_oracle.solve(c1.b[:n], c1.b[n:], rtol=1e-8, maxiter=5000)
+    xo = np.concatenate([dx, dq])
+    assert abs(st["pcg_iterations"] - ost["iterations"]) <= 1
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-7
+
+
 def test_mixed_precision_reaches_fp64_tolerance():
     s = systems.torus_system(seed=1)
     h64, _ = _setup(s, {"linear_system/tol_rate": 1e-8, "linear_system/hsc/max_iter": 20000})
