@@ -76,6 +76,8 @@ struct PcgParams {
   int rt_total, et_total, lt_total;
   int rows_per_tile;
   int node_tile;          // nodes per elementwise tile (<= kNodeTileMax)
+  int wc_rows;            // coupled nodes per resident node tile whose W rows are cached in LDS (0 = off)
+  int wc_off;             // byte offset of that cache in dynamic LDS (after the t vectors)
   const int* chunkA;      // [nwg] chunk of row-workgroup b in the A chunking (-1 for LR wgs)
   const int* chunkF;      // [nwg]
   const int* chunkL;      // [nwg] (-1 for row wgs)

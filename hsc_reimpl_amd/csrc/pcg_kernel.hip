@@ -359,7 +359,7 @@ __device__ __forceinline__ void lowrank_list(int an, int* s_list, int* s_slot, i
 template <typename WT>
 __device__ __forceinline__ void lowrank_apply(int k, int an, const WT* __restrict__ W, const double* __restrict__ t,
                                               double g[3], const int* s_list, const int* s_slot,
-                                              double (*s_g)[3], const int* s_cnt) {
+                                              double (*s_g)[3], const int* s_cnt, const WT* wc = nullptr) {
   constexpr int GL = 4;               // lanes per node
   constexpr int NPR = kBlock / GL;    // nodes per pass
   const int tid = threadIdx.x;
@@ -369,7 +369,8 @@ __device__ __forceinline__ void lowrank_apply(int k, int an, const WT* __restric
     const int it = base + grp;
     double p0 = 0.0, p1 = 0.0, p2 = 0.0;
     if (it < m) {
-      const WT* w0 = W + (size_t)s_list[it] * k;
+      // W rows of the listed node: from the LDS cache (list order) or global
+      const WT* w0 = wc ? wc + (size_t)it * 3 * k : W + (size_t)s_list[it] * k;
       const WT* w1 = w0 + k;
       const WT* w2 = w1 + k;
 #pragma unroll 8
@@ -573,6 +574,19 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   if (P.prof && tid == 0) t_last = rt_ticks();
   if (started) {
   if (resE) lowrank_list(anE, s_list, s_slot, &s_cnt);  // kept for the whole launch
+  // W rows of the resident tile's coupled nodes, cached in LDS for stage B
+  VT* s_wc = nullptr;
+  if (resE && P.wc_rows > 0 && sys[sE].k > 0 && s_cnt <= P.wc_rows) {
+    s_wc = reinterpret_cast<VT*>(reinterpret_cast<char*>(s_t) + P.wc_off);
+    const SysDesc& S = sys[sE];
+    const int k = S.k, m = s_cnt, n3k = 3 * k;
+    const VT* W = w_vals<VT>(S);
+    for (int e = tid; e < m * n3k; e += kBlock) {
+      const int slot = e / n3k, rem = e - slot * n3k;
+      s_wc[e] = W[(size_t)s_list[slot] * k + rem];
+    }
+    __syncthreads();
+  }
   int zc = 0, pc = 0;
   const int order = P.order;
   double* partE_rr = P.partE;
@@ -583,6 +597,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // stage-B vectors of the resident node tile, loaded before the reductions
   VT preq[3] = {VT(0), VT(0), VT(0)}, prep[3] = {VT(0), VT(0), VT(0)};
   double prer[3] = {0.0, 0.0, 0.0}, prex[3] = {0.0, 0.0, 0.0};
+  VT pi0 = VT(0), pi1 = VT(0), pi2 = VT(0), pi3 = VT(0), pi4 = VT(0), pi5 = VT(0), pi6 = VT(0), pi7 = VT(0),
+     pi8 = VT(0);
   bool pre_ok = false;
   auto prefetch_B = [&]() {
     pre_ok = false;
@@ -592,12 +608,22 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     const int i3 = 3 * iE;
     const VT* q = (const VT*)S.q;
     const VT* pn = (const VT*)S.p[pc];
+    const VT* inv = inv_vals<VT>(S) + 9 * iE;
     for (int d = 0; d < 3; ++d) {
       preq[d] = q[i3 + d];
       prep[d] = pn[i3 + d];
       prer[d] = S.r[i3 + d];
       prex[d] = S.x[i3 + d];
     }
+    pi0 = inv[0];
+    pi1 = inv[1];
+    pi2 = inv[2];
+    pi3 = inv[3];
+    pi4 = inv[4];
+    pi5 = inv[5];
+    pi6 = inv[6];
+    pi7 = inv[7];
+    pi8 = inv[8];
     pre_ok = true;
   };
 
@@ -616,7 +642,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   const int an = resE ? anE : (valid ? S.act_node[i] : -1);
                   if (!resE) lowrank_list(an, s_list, s_slot, &s_cnt);
                   if (kind == 1)
-                    lowrank_apply(S.k, an, w_vals<VT>(S), s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
+                    lowrank_apply(S.k, an, w_vals<VT>(S), s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt,
+                                  resE ? s_wc : (const VT*)nullptr);
                   else
                     lowrank_apply(S.k, an, S.W64, s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
                 }
@@ -658,13 +685,31 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   for (int d = 0; d < 3; ++d) S.r[i3 + d] = rv[d];
                 rr += rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
                 if (write_z) {
-                  const VT* inv = inv_vals<VT>(S) + 9 * i;
+                  VT i0 = pi0, i1 = pi1, i2 = pi2, i3v = pi3, i4 = pi4, i5 = pi5, i6 = pi6, i7 = pi7, i8 = pi8;
+                  if (!pre_ok) {
+                    const VT* ig = inv_vals<VT>(S) + 9 * i;
+                    i0 = ig[0];
+                    i1 = ig[1];
+                    i2 = ig[2];
+                    i3v = ig[3];
+                    i4 = ig[4];
+                    i5 = ig[5];
+                    i6 = ig[6];
+                    i7 = ig[7];
+                    i8 = ig[8];
+                  }
                   VT* z = (VT*)S.z[zc];
                   const VT r0 = (VT)rv[0], r1 = (VT)rv[1], r2 = (VT)rv[2];
-                  for (int d = 0; d < 3; ++d) {
-                    const VT zd = omega * (inv[3 * d] * r0 + inv[3 * d + 1] * r1 + inv[3 * d + 2] * r2);
-                    z[i3 + d] = zd;
-                    if (order == 1) rz += rv[d] * (double)zd;
+                  const VT z0 = omega * (i0 * r0 + i1 * r1 + i2 * r2);
+                  const VT z1 = omega * (i3v * r0 + i4 * r1 + i5 * r2);
+                  const VT z2 = omega * (i6 * r0 + i7 * r1 + i8 * r2);
+                  z[i3] = z0;
+                  z[i3 + 1] = z1;
+                  z[i3 + 2] = z2;
+                  if (order == 1) {
+                    rz += rv[0] * (double)z0;
+                    rz += rv[1] * (double)z1;
+                    rz += rv[2] * (double)z2;
                   }
                 }
               },
@@ -672,11 +717,13 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                 const bool active = s_st[s] == ST_RUN && (kind != 2 || s_chk[s] != CHK_NONE);
                 // T2 in norm-only mode keeps stage B's z and <r,z> partials
                 const bool new_z = active && (kind != 2 || s_chk[s] == CHK_CONVERGE || s_chk[s] == CHK_REPLACE);
+                // <r, z0> is the preconditioned product only for order 1
+                // (higher orders take it from the last Neumann pass)
                 const double a = block_sum_t0(rr, s_red);
-                const double c = block_sum_t0(rz, s_red);
+                const double c = order == 1 ? block_sum_t0(rz, s_red) : 0.0;
                 if (tid == 0) {
                   if (active) partE_rr[(size_t)s * P.nF + cF] = a;
-                  if (new_z) partE_rz[(size_t)s * P.nF + cF] = c;
+                  if (new_z && order == 1) partE_rz[(size_t)s * P.nF + cF] = c;
                 }
                 rr = 0.0;
                 rz = 0.0;
@@ -928,6 +975,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     __syncthreads();
 
     // ---------------- stage B
+    if (P.prof && tid == 0) P.prof[(size_t)b * 32 + 12] += rt_ticks() - t_last;  // B: reductions part
     node_stage(1);
     pre_ok = false;
     if (!sync_at(3)) goto out;

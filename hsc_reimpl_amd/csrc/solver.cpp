@@ -114,6 +114,7 @@ struct hsc_handle {
   double aca_err = 0.0, abd_res = 0.0, setup_ms = 0.0;
   int doublings = 0;
   int G = 8, wgpcu = 1, sched = 1;
+  int max_coupled128 = 0;  // most coupled nodes in one 128-node tile
   double* xsol = nullptr;  // FEM part of the caller's x during a solve
   // buffers
   Buf row_count, rowptr, cursor, active_flag, act_prefix, act_node, act_dof;
@@ -282,7 +283,24 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
     if (S[s].k > hsc::kMaxRankLds) throw HscError{HSC_ERR_CAPACITY, "rank exceeds 256"};
   }
   const DeviceInfo di = device_info(h0->device);
-  const size_t shmem = (size_t)std::max(t_total, 1) * sizeof(double);
+  // LDS cache of the coupled W rows of a resident node tile (latency
+  // schedule, 128-node tiles): sized by the most coupled nodes in any tile
+  int wc_rows = 0;
+  size_t wc_bytes = 0;
+  const size_t t_bytes = ((size_t)std::max(t_total, 1) * sizeof(double) + 15) / 16 * 16;
+  if (lc.sched != 2) {
+    int kmax = 0;
+    for (int s = 0; s < nsys; ++s) {
+      wc_rows = std::max(wc_rows, hs[s]->max_coupled128);
+      kmax = std::max(kmax, hs[s]->k);
+    }
+    wc_bytes = (size_t)wc_rows * 3 * kmax * (mixed ? sizeof(float) : sizeof(double));
+    if (wc_rows == 0 || kmax == 0 || wc_bytes > 64 * 1024) {
+      wc_rows = 0;
+      wc_bytes = 0;
+    }
+  }
+  const size_t shmem = t_bytes + wc_bytes;
   int occ = 0;
   HIPCHK(hsc::pcg_occupancy(G, mixed, shmem, &occ));
   if (occ <= 0) throw HscError{HSC_ERR_CAPACITY, "persistent PCG kernel does not fit on a CU"};
@@ -379,6 +397,8 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   P.lt_total = lt;
   P.rows_per_tile = R;
   P.node_tile = node_tile;
+  P.wc_rows = wc_rows;
+  P.wc_off = (int)t_bytes;
   P.chunkA = sc.chunks.as<int>();
   P.chunkF = sc.chunks.as<int>() + nwg;
   P.chunkL = sc.chunks.as<int>() + 2 * nwg;
@@ -478,6 +498,14 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
       const int am = c.back().second;
       std::fprintf(stderr, "  %-8s n=%6lld  %8.2f %8.2f %8.2f [wg%d, %s] | %8.2f\n", names[si], n, c.front().first,
                    c[nwg / 2].first, c.back().first, am, am < P.n_lr_wg ? "LR" : "row", w[nwg / 2]);
+    }
+    {
+      // stage B split: reductions (slot 12) vs node tiles (rest of site 3)
+      const long long n = pr[23];
+      std::vector<double> red(nwg);
+      for (int b = 0; b < nwg; ++b) red[b] = n ? pr[(size_t)32 * b + 12] / 100.0 / n : 0.0;
+      std::sort(red.begin(), red.end());
+      if (n) std::fprintf(stderr, "  B reductions part: min/median/max %8.2f %8.2f %8.2f\n", red.front(), red[nwg / 2], red.back());
     }
   }
   return out;
@@ -646,6 +674,20 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   if (a > 0) HIPCHK(hipMemsetAsync(h->E.p, 0, sizeof(double) * (size_t)qm * a, st));
   HIPCHK(hsc::launch_active(h->active_flag.as<int>(), h->act_prefix.as<int>(), nb, h->act_node.as<int>(),
                             h->act_dof.as<int>(), st));
+  {
+    // coupled nodes per 128-node tile: sizes the LDS cache of W rows used by
+    // the persistent PCG's resident node tiles
+    std::vector<int> an(nb);
+    HIPCHK(hipMemcpyAsync(an.data(), h->act_node.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int best = 0;
+    for (int t0 = 0; t0 < nb; t0 += 128) {
+      int c = 0;
+      for (int i = t0; i < std::min(nb, t0 + 128); ++i) c += an[i] >= 0;
+      best = std::max(best, c);
+    }
+    h->max_coupled128 = best;
+  }
   hsc::PlaceParams pp{Av->row_indices, Av->col_indices, Av->values, Av->triplet_count, qo, qm, fo, fn,
                       h->rowptr.as<int>(), h->cursor.as<int>(), h->key_col.as<int>(), h->key_src.as<int>(),
                       h->act_node.as<int>(), a, h->E.as<double>()};
