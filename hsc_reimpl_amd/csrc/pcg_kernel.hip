@@ -499,8 +499,18 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   __shared__ int s_list[kNodeTileMax], s_slot[kNodeTileMax], s_cnt;
   __shared__ double s_g[kNodeTileMax][3];
   __shared__ double s_tred[kBlock];
+  __shared__ SysDesc s_sys[kMaxSys];
 
-  const SysDesc* sys = P.sys;
+  // the descriptors live in LDS for the launch: after every grid barrier's
+  // acquire they are read without a trip to memory
+  {
+    const int words = (int)(P.nsys * sizeof(SysDesc) / sizeof(int));
+    const int* src = reinterpret_cast<const int*>(P.sys);
+    int* dst = reinterpret_cast<int*>(s_sys);
+    for (int e = threadIdx.x; e < words; e += kBlock) dst[e] = src[e];
+    __syncthreads();
+  }
+  const SysDesc* sys = s_sys;
   const int nsys = P.nsys;
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
