@@ -465,24 +465,55 @@ __device__ __forceinline__ void for_tiles(const SysDesc* sys, int nsys, TileRang
   }
 }
 
-// sum_{c=c0}^{c1} part[s*nC + c] by one wave (lanes strided, butterfly).
-// Chunks that own no tile of the tiling (T tiles over nC chunks) never write
-// a partial and are skipped.
+// Chunks that own no tile of a tiling (T tiles over nC chunks) never write a
+// partial; their slots are skipped by the reductions.
 __device__ __forceinline__ bool chunk_empty(int c, int nC, int T) {
   if (nC <= T) return false;  // every chunk owns at least one tile (no division needed)
   return tile_of_chunk(c, nC, T) == tile_of_chunk(c + 1, nC, T);
 }
-__device__ __forceinline__ double wave_reduce_chunks(const double* part, int s, int nC, int c0, int c1, int T) {
-  const int lane = threadIdx.x & 63;
+
+// Per-system sums of chunk partials for every system at once: groups of L
+// lanes (L = 64 for up to 8 systems, halved until all systems fit in the
+// workgroup) take one system each, every lane strides the system's chunks
+// by L with all loads issued before the sums, and the group adds its lanes
+// by butterfly (for up to 8 systems: wave w sums system w over its 64
+// lanes).  out[s] is valid after the next __syncthreads.
+template <typename Rng, typename Want>
+__device__ __forceinline__ void reduce_chunks_all(const double* __restrict__ part, int nsys, int nC, int T, Rng rng,
+                                                  Want want, double* out) {
+  int L = 64;
+  while (L > 1 && L * nsys > kBlock) L >>= 1;
+  const int tid = threadIdx.x;
+  const int s = tid / L, l = tid % L;
   double v = 0.0;
-  // loads are unconditional (every chunk slot exists) so they issue together;
-  // slots of empty chunks hold stale data and are masked out by a select
+  const bool act = s < nsys && want(s);
+  if (act) {
+    int c0, c1;
+    rng(s, c0, c1);
 #pragma unroll 4
-  for (int c = c0 + lane; c <= c1; c += 64) {
-    const double x = part[(size_t)s * nC + c];
-    v += chunk_empty(c, nC, T) ? 0.0 : x;
+    for (int c = c0 + l; c <= c1; c += L) {
+      const double x = part[(size_t)s * nC + c];
+      v += chunk_empty(c, nC, T) ? 0.0 : x;
+    }
   }
-  return wave_sum(v);
+  for (int m = L >> 1; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  if (act && l == 0) out[s] = v;
+}
+
+// |t_s|^2 for every system at once (same grouping as reduce_chunks_all)
+template <typename Want>
+__device__ __forceinline__ void sum_tt_all(const SysDesc* sys, int nsys, const double* t, Want want, double* out) {
+  int L = 64;
+  while (L > 1 && L * nsys > kBlock) L >>= 1;
+  const int tid = threadIdx.x;
+  const int s = tid / L, l = tid % L;
+  double v = 0.0;
+  const bool in = s < nsys;
+  const bool act = in && want(s);
+  if (act)
+    for (int j = l; j < sys[s].k; j += L) v += t[sys[s].t_off + j] * t[sys[s].t_off + j];
+  for (int m = L >> 1; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  if (in && l == 0) out[s] = act ? v : 0.0;
 }
 
 // ---------------------------------------------------------------- kernel
@@ -772,108 +803,60 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   };
 
   // reduce rz (after the last preconditioner stage) into s_v0
+  auto all_sys = [](int) { return true; };
+  auto rngF = [&](int s, int& c0, int& c1) { c0 = sys[s].cF0; c1 = sys[s].cF1; };
+  auto rngE = [&](int s, int& c0, int& c1) { c0 = sys[s].cE0; c1 = sys[s].cE1; };
+  auto rngA = [&](int s, int& c0, int& c1) { c0 = sys[s].cA0; c1 = sys[s].cA1; };
   auto reduce_rz = [&]() {
-    for (int s = wave; s < nsys; s += kBlock / 64) {
-      const SysDesc& S = sys[s];
-      double v = order > 1 ? wave_reduce_chunks(P.partF, s, P.nF, S.cF0, S.cF1, P.rt_total)
-                           : wave_reduce_chunks(partE_rz, s, P.nF, S.cE0, S.cE1, P.et_total);
-      if ((tid & 63) == 0) s_v0[s] = v;
-    }
+    if (order > 1)
+      reduce_chunks_all(P.partF, nsys, P.nF, P.rt_total, rngF, all_sys, s_v0);
+    else
+      reduce_chunks_all(partE_rz, nsys, P.nF, P.et_total, rngE, all_sys, s_v0);
     __syncthreads();
   };
 
   // Reduce the low-rank partials into s_t (and |t|^2 into s_tt) for the
   // systems selected by `want`; with with_pA also the stage-A partials
-  // <p, D p> into s_v1 (every system).  When the ranks of all systems fit in
-  // one workgroup (sum k <= kBlock), thread (g, q) sums every ng-th chunk of
-  // entry q of the concatenated t vectors and the groups are added in order;
-  // the stage-A partials are reduced by the waves in between, so all loads
-  // are in flight before the first wait.  Larger rank totals go system by
-  // system.
+  // <p, D p> into s_v1 (every system).  Entry q of the concatenated t vectors
+  // is summed over its system's low-rank chunks: when all ranks fit in one
+  // workgroup (sum k <= kBlock) by ng groups of threads (every ng-th chunk,
+  // groups added in order), else one thread per entry.  The stage-A loads are
+  // issued before the first wait, so all latencies overlap.
   auto reduce_t = [&](auto want, bool with_pA) {
     const int T = P.t_total;
-    if (T <= kBlock) {
-      const int ng = max(1, kBlock / max(T, 1));
-      const int q = tid % max(T, 1), grp = tid / max(T, 1);
+    const int ng = T > 0 && T <= kBlock ? kBlock / T : 1;
+    auto sum_entry = [&](int q, int g0, int gs) {
+      int s = 0;
+      while (s + 1 < nsys && sys[s + 1].t_off <= q) ++s;
+      const SysDesc& S = sys[s];
       double v = 0.0;
-      if (T > 0 && grp < ng) {
-        int s = 0;
-        while (s + 1 < nsys && sys[s + 1].t_off <= q) ++s;
-        const SysDesc& S = sys[s];
-        if (want(s) && S.k > 0) {
-          const int l = q - S.t_off;
+      if (want(s) && S.k > 0) {
+        const int l = q - S.t_off;
 #pragma unroll 8
-          for (int c = S.cL0 + grp; c <= S.cL1; c += ng) {
-            const double x = P.partL[S.lrp_off + (size_t)(c - S.cL0) * S.k + l];
-            v += chunk_empty(c, P.nL, P.lt_total) ? 0.0 : x;
-          }
+        for (int c = S.cL0 + g0; c <= S.cL1; c += gs) {
+          const double x = P.partL[S.lrp_off + (size_t)(c - S.cL0) * S.k + l];
+          v += chunk_empty(c, P.nL, P.lt_total) ? 0.0 : x;
         }
       }
-      if (with_pA) {
-        for (int s = wave; s < nsys; s += kBlock / 64) {
-          const SysDesc& S = sys[s];
-          const double va = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1, P.rt_total);
-          if ((tid & 63) == 0) s_v1[s] = va;
-        }
-      }
-      if (T > 0 && grp < ng) s_tred[grp * T + q] = v;
-      __syncthreads();
+      return v;
+    };
+    if (T > 0 && T <= kBlock) {
+      const int q = tid % T, grp = tid / T;
+      if (grp < ng) s_tred[grp * T + q] = sum_entry(q, grp, ng);
+    } else {
+      for (int q = tid; q < T; q += kBlock) s_t[q] = sum_entry(q, 0, 1);
+    }
+    if (with_pA) reduce_chunks_all(P.partA, nsys, P.nA, P.rt_total, rngA, all_sys, s_v1);
+    __syncthreads();
+    if (T > 0 && T <= kBlock) {
       if (tid < T) {
         double acc = 0.0;
         for (int g = 0; g < ng; ++g) acc += s_tred[g * T + tid];
         s_t[tid] = acc;
       }
       __syncthreads();
-      for (int s = wave; s < nsys; s += kBlock / 64) {
-        const SysDesc& S = sys[s];
-        double a = 0.0;
-        if (want(s))
-          for (int j = tid & 63; j < S.k; j += 64) a += s_t[S.t_off + j] * s_t[S.t_off + j];
-        a = wave_sum(a);
-        if ((tid & 63) == 0) s_tt[s] = a;
-      }
-      __syncthreads();
-      return;
     }
-    if (with_pA) {
-      for (int s = wave; s < nsys; s += kBlock / 64) {
-        const SysDesc& S = sys[s];
-        const double va = wave_reduce_chunks(P.partA, s, P.nA, S.cA0, S.cA1, P.rt_total);
-        if ((tid & 63) == 0) s_v1[s] = va;
-      }
-    }
-    for (int s = 0; s < nsys; ++s) {
-      const SysDesc& S = sys[s];
-      const int k = S.k;
-      if (!want(s) || k == 0) {
-        if (tid == 0) s_tt[s] = 0.0;
-        continue;
-      }
-      const int ng = max(1, kBlock / k);
-      const int l = tid % k, grp = tid / k;
-      double v = 0.0;
-      if (grp < ng) {
-#pragma unroll 8
-        for (int c = S.cL0 + grp; c <= S.cL1; c += ng) {
-          const double x = P.partL[S.lrp_off + (size_t)(c - S.cL0) * k + l];
-          v += chunk_empty(c, P.nL, P.lt_total) ? 0.0 : x;
-        }
-        s_tred[grp * k + l] = v;
-      }
-      __syncthreads();
-      if (tid < k) {
-        double acc = 0.0;
-        for (int g = 0; g < ng; ++g) acc += s_tred[g * k + tid];
-        s_t[S.t_off + tid] = acc;
-      }
-      __syncthreads();
-      if (wave == 0) {
-        double a = 0.0;
-        for (int j = tid; j < k; j += 64) a += s_t[S.t_off + j] * s_t[S.t_off + j];
-        a = wave_sum(a);
-        if (tid == 0) s_tt[s] = a;
-      }
-    }
+    sum_tt_all(sys, nsys, s_t, want, s_tt);
     __syncthreads();
   };
 
@@ -911,16 +894,14 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // ================================================================ init
   node_stage(0);
   if (!sync_at(0)) goto out;
-  for (int s = wave; s < nsys; s += kBlock / 64) {
-    const SysDesc& S = sys[s];
-    const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1, P.et_total);
-    if ((tid & 63) == 0) {
-      const double nb = sqrt(v);
-      s_normb[s] = nb;
-      if (!(nb > 0.0)) {
-        s_st[s] = ST_DONE;
-        s_rel[s] = 0.0;
-      }
+  reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE, all_sys, s_v1);
+  __syncthreads();
+  for (int s = tid; s < nsys; s += kBlock) {
+    const double nb = sqrt(s_v1[s]);
+    s_normb[s] = nb;
+    if (!(nb > 0.0)) {
+      s_st[s] = ST_DONE;
+      s_rel[s] = 0.0;
     }
   }
   __syncthreads();
@@ -989,11 +970,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     node_stage(1);
     pre_ok = false;
     if (!sync_at(3)) goto out;
-    for (int s = wave; s < nsys; s += kBlock / 64) {
-      const SysDesc& S = sys[s];
-      const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1, P.et_total);
-      if ((tid & 63) == 0) s_v1[s] = v;
-    }
+    reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE, all_sys, s_v1);
     __syncthreads();
     if (tid == 0) {
       int any = 0;
@@ -1032,12 +1009,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, false);
       node_stage(2);
       if (!sync_at(5)) goto out;
-      for (int s = wave; s < nsys; s += kBlock / 64) {
-        if (s_st[s] != ST_RUN || s_chk[s] == CHK_NONE) continue;
-        const SysDesc& S = sys[s];
-        const double v = wave_reduce_chunks(partE_rr, s, P.nF, S.cE0, S.cE1, P.et_total);
-        if ((tid & 63) == 0) s_v1[s] = v;
-      }
+      reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE,
+                        [&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, s_v1);
       __syncthreads();
       if (tid == 0) {
         for (int s = 0; s < nsys; ++s) {
