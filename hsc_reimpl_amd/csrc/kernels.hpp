@@ -84,10 +84,13 @@ hipError_t launch_full_residual(const double* yf, const double* E, const double*
                                 const int* act_dof, int a, int qm, const double* b, const double* x, int fo, int qo,
                                 int n, double* part, double* out, hipStream_t st);
 // pcg_kernel.hip
-hipError_t launch_pcg(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st);
-hipError_t pcg_occupancy(int G, bool mixed, size_t shmem, int* blocks);
-hipError_t launch_pcg_plain(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st);
-const void* pcg_kernel_fn(int G, bool mixed);
+// res: the latency-schedule kernel (register-resident rows and node tile);
+// otherwise the throughput-schedule kernel (streaming SpMV with several
+// blocks per lane in flight)
+hipError_t launch_pcg(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st);
+hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks);
+hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st);
+const void* pcg_kernel_fn(int G, bool mixed, bool res);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);  // bytes % 4 == 0; a kernel, safe in graphs
 hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st);
 hipError_t launch_op_precond64(int G, const SysDesc& S, int order, double omega, const double* r, double* z,

@@ -37,25 +37,55 @@ template <> __device__ __forceinline__ const double* w_vals<double>(const SysDes
 template <> __device__ __forceinline__ const float* w_vals<float>(const SysDesc& S) { return S.W32; }
 
 // y_row = sum_j D_ij src(j) over one block row with G lanes (lane strides the
-// row's 3x3 blocks); every lane of the group gets the full 3-vector.
-template <int G, typename MT, typename Src>
+// row's 3x3 blocks); every lane of the group gets the full 3-vector.  With
+// U > 1 each round puts U blocks per lane in flight at once (column indices
+// first, then the gathers and value streams, then the FMAs), so a row costs
+// two dependent latencies per round instead of two per block; every lane
+// still adds its blocks p0+lane, p0+lane+G, ... in increasing order.
+template <int G, typename MT, int U = 1, typename Src>
 __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict__ val, int row,
                                          int lane, Src src, MT& a0, MT& a1, MT& a2) {
   a0 = a1 = a2 = MT(0);
   if (row < S.nb) {
     const int p0 = S.rowptr[row], p1 = S.rowptr[row + 1];
     const int nnzb = S.nnzb;
-    for (int p = p0 + lane; p < p1; p += G) {
-      const int c = S.colidx[p];
-      MT x0, x1, x2;
-      src(c, x0, x1, x2);
-      const MT* v = val + p;
-      const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
-      const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
-      const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
-      a0 += v0 * x0 + v3 * x1 + v6 * x2;
-      a1 += v1 * x0 + v4 * x1 + v7 * x2;
-      a2 += v2 * x0 + v5 * x1 + v8 * x2;
+    if constexpr (U == 1) {
+      for (int p = p0 + lane; p < p1; p += G) {
+        const int c = S.colidx[p];
+        MT x0, x1, x2;
+        src(c, x0, x1, x2);
+        const MT* v = val + p;
+        const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
+        const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
+        const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
+        a0 += v0 * x0 + v3 * x1 + v6 * x2;
+        a1 += v1 * x0 + v4 * x1 + v7 * x2;
+        a2 += v2 * x0 + v5 * x1 + v8 * x2;
+      }
+    } else {
+      for (int pb = p0 + lane; pb < p1; pb += G * U) {
+        int pp[U], cc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          pp[u] = min(pb + u * G, p1 - 1);  // rows are never empty (diagonal block)
+          cc[u] = S.colidx[pp[u]];
+        }
+        MT x[U][3], v[U][9];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          src(cc[u], x[u][0], x[u][1], x[u][2]);
+#pragma unroll
+          for (int e = 0; e < 9; ++e) v[u][e] = val[pp[u] + (size_t)e * nnzb];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (pb + u * G < p1) {
+            a0 += v[u][0] * x[u][0] + v[u][3] * x[u][1] + v[u][6] * x[u][2];
+            a1 += v[u][1] * x[u][0] + v[u][4] * x[u][1] + v[u][7] * x[u][2];
+            a2 += v[u][2] * x[u][0] + v[u][5] * x[u][1] + v[u][8] * x[u][2];
+          }
+        }
+      }
     }
   }
   a0 = group_sum<G>(a0);
@@ -63,19 +93,27 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
   a2 = group_sum<G>(a2);
 }
 
+// blocks per lane in flight per round of the streaming SpMV: 1 in the
+// latency schedule (its rows are register-resident), more in the throughput
+// schedule, where each workgroup streams many row tiles
+template <int G, bool RES>
+struct StreamU {
+  static constexpr int value = RES ? 1 : (G >= 16 ? 1 : (G == 8 ? 2 : 3));
+};
+
 template <typename T>
 __device__ __forceinline__ T pick3(int d, T a0, T a1, T a2) { return d == 0 ? a0 : (d == 1 ? a1 : a2); }
 
 // ---------------------------------------------------------------- stages
 // Stage A on one row tile: p_new = z + beta p_old, q = D p_new, dot += p_new.q
-template <int G, typename VT>
+template <int G, typename VT, int U = 1>
 __device__ __forceinline__ void tile_A(const SysDesc& S, int tl, const VT* __restrict__ z,
                                        const VT* __restrict__ pold, VT* __restrict__ pnew,
                                        VT* __restrict__ q, VT beta, double& dot) {
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const int row = tl * (kBlock / G) + g;
   VT a0, a1, a2;
-  row_spmv<G, VT>(S, mat_vals<VT>(S), row, lane,
+  row_spmv<G, VT, U>(S, mat_vals<VT>(S), row, lane,
                   [&](int c, VT& x0, VT& x1, VT& x2) {
                     x0 = combine(z[3 * c], pold[3 * c], beta);
                     x1 = combine(z[3 * c + 1], pold[3 * c + 1], beta);
@@ -93,13 +131,13 @@ __device__ __forceinline__ void tile_A(const SysDesc& S, int tl, const VT* __res
 }
 
 // Neumann (Horner) step on one row tile: zout = zin + w Dinv (r - D zin)
-template <int G, typename VT>
+template <int G, typename VT, int U = 1>
 __device__ __forceinline__ void tile_N(const SysDesc& S, int tl, const VT* __restrict__ zin,
                                        VT* __restrict__ zout, VT omega, bool last, double& rz) {
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const int row = tl * (kBlock / G) + g;
   VT a0, a1, a2;
-  row_spmv<G, VT>(S, mat_vals<VT>(S), row, lane,
+  row_spmv<G, VT, U>(S, mat_vals<VT>(S), row, lane,
                   [&](int c, VT& x0, VT& x1, VT& x2) {
                     x0 = zin[3 * c];
                     x1 = zin[3 * c + 1];
@@ -274,13 +312,13 @@ __device__ __forceinline__ void tile_A_res(const SysDesc& S, int row, const RowR
 }
 
 // D x in FP64 on one row tile (true residual / operator tests)
-template <int G>
+template <int G, int U = 1>
 __device__ __forceinline__ void tile_Dx64(const SysDesc& S, int tl, const double* __restrict__ x,
                                           double* __restrict__ y) {
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const int row = tl * (kBlock / G) + g;
   double a0, a1, a2;
-  row_spmv<G, double>(S, S.val64, row, lane,
+  row_spmv<G, double, U>(S, S.val64, row, lane,
                       [&](int c, double& x0, double& x1, double& x2) {
                         x0 = x[3 * c];
                         x1 = x[3 * c + 1];
@@ -517,8 +555,9 @@ __device__ __forceinline__ void sum_tt_all(const SysDesc* sys, int nsys, const d
 }
 
 // ---------------------------------------------------------------- kernel
-template <int G, typename VT>
+template <int G, typename VT, bool RES>
 __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
+  constexpr int SU = StreamU<G, RES>::value;
   extern __shared__ double s_t[];  // t vectors (sum of ranks)
   __shared__ double s_rz[kMaxSys], s_alpha[kMaxSys], s_beta[kMaxSys], s_normb[kMaxSys];
   __shared__ double s_rel[kMaxSys], s_v0[kMaxSys], s_v1[kMaxSys], s_tt[kMaxSys];
@@ -590,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   };
 
   // resident Neumann row tile (exactly one row tile in the Neumann chunking)
-  const bool resF = trF.t1 - trF.t0 == 1;
+  const bool resF = RES && trF.t1 - trF.t0 == 1;
   int sF = 0, rowF = 0;
   RowRes<G, VT> rrF;
   if (resF) {
@@ -603,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // resident node tile (exactly one node tile): its node and coupled-node
   // list are fixed for the launch, and stage B's vector loads are issued
   // before the stage-A reductions
-  const bool resE = trE.t1 - trE.t0 == 1;
+  const bool resE = RES && trE.t1 - trE.t0 == 1;
   int sE = 0, iE = 0, anE = -1;
   if (resE) {
     sE = sys_of_tile_begin(sys, nsys, trE.t0, 1);
@@ -792,7 +831,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   [&](int s, int tl) {
                     if (s_st[s] != ST_RUN) return;
                     const SysDesc& S = sys[s];
-                    tile_N<G, VT>(S, tl, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], omega, last, rz);
+                    tile_N<G, VT, SU>(S, tl, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], omega, last, rz);
                   },
                   flushF);
       }
@@ -826,8 +865,12 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     const int T = P.t_total;
     const int ng = T > 0 && T <= kBlock ? kBlock / T : 1;
     auto sum_entry = [&](int q, int g0, int gs) {
-      int s = 0;
-      while (s + 1 < nsys && sys[s + 1].t_off <= q) ++s;
+      int s = 0, hi = nsys - 1;  // last system with t_off <= q (binary search)
+      while (s < hi) {
+        const int mid = (s + hi + 1) >> 1;
+        if (sys[mid].t_off <= q) s = mid;
+        else hi = mid - 1;
+      }
       const SysDesc& S = sys[s];
       double v = 0.0;
       if (want(s) && S.k > 0) {
@@ -940,7 +983,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                 [&](int s, int tl) {
                   if (s_st[s] != ST_RUN) return;
                   const SysDesc& S = sys[s];
-                  tile_A<G, VT>(S, tl, (const VT*)S.z[zc], (const VT*)S.p[pc], (VT*)S.p[pc ^ 1],
+                  tile_A<G, VT, SU>(S, tl, (const VT*)S.z[zc], (const VT*)S.p[pc], (VT*)S.p[pc ^ 1],
                                 (VT*)S.q, (VT)s_beta[s], dot);
                 },
                 [&](int s) {
@@ -1001,7 +1044,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   [&](int s, int tl) {
                     if (s_st[s] != ST_RUN || s_chk[s] == CHK_NONE) return;
                     const SysDesc& S = sys[s];
-                    tile_Dx64<G>(S, tl, S.x, S.q64);
+                    tile_Dx64<G, SU>(S, tl, S.x, S.q64);
                   },
                   [&](int) {});
       }
@@ -1195,10 +1238,10 @@ hipError_t launch_node_features(const int* rowptr, const int* colidx, const doub
 }
 
 // explicit instantiations via host launchers
-template <int G, typename VT>
+template <int G, typename VT, bool RES>
 static hipError_t launch_pcg_t(const PcgParams& P, size_t shmem, hipStream_t st) {
   void* args[] = {(void*)&P};
-  return hipLaunchCooperativeKernel((const void*)pcg_persistent<G, VT>, dim3(P.nwg), dim3(kBlock), args,
+  return hipLaunchCooperativeKernel((const void*)pcg_persistent<G, VT, RES>, dim3(P.nwg), dim3(kBlock), args,
                                     shmem, st);
 }
 
@@ -1206,10 +1249,15 @@ static hipError_t launch_pcg_t(const PcgParams& P, size_t shmem, hipStream_t st)
 // whose kernel node is then marked cooperative); the grid never exceeds the
 // co-resident capacity computed from the occupancy, and every grid wait is
 // bounded
-template <int G, typename VT>
+template <int G, typename VT, bool RES>
 static hipError_t launch_pcg_plain_t(const PcgParams& P, size_t shmem, hipStream_t st) {
-  hipLaunchKernelGGL((pcg_persistent<G, VT>), dim3(P.nwg), dim3(kBlock), shmem, st, P);
+  hipLaunchKernelGGL((pcg_persistent<G, VT, RES>), dim3(P.nwg), dim3(kBlock), shmem, st, P);
   return hipGetLastError();
+}
+
+template <int G, typename VT, bool RES>
+static hipError_t occ_t(int* blocks, size_t shmem) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pcg_persistent<G, VT, RES>, kBlock, shmem);
 }
 
 __global__ void k_zero_u32(unsigned* p, long long n) {
@@ -1226,77 +1274,55 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_pcg_plain(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st) {
-  if (mixed) {
-    switch (G) {
-      case 4: return launch_pcg_plain_t<4, float>(P, shmem, st);
-      case 8: return launch_pcg_plain_t<8, float>(P, shmem, st);
-      case 16: return launch_pcg_plain_t<16, float>(P, shmem, st);
-    }
-  } else {
-    switch (G) {
-      case 4: return launch_pcg_plain_t<4, double>(P, shmem, st);
-      case 8: return launch_pcg_plain_t<8, double>(P, shmem, st);
-      case 16: return launch_pcg_plain_t<16, double>(P, shmem, st);
-    }
-  }
+// dispatch over lanes per row (4/8/16), precision and schedule
+#define HSC_PCG_DISPATCH(FN, ...)                                                       \
+  do {                                                                                  \
+    if (mixed) {                                                                        \
+      if (res) {                                                                        \
+        if (G == 4) return FN<4, float, true>(__VA_ARGS__);                             \
+        if (G == 8) return FN<8, float, true>(__VA_ARGS__);                             \
+        if (G == 16) return FN<16, float, true>(__VA_ARGS__);                           \
+      } else {                                                                          \
+        if (G == 4) return FN<4, float, false>(__VA_ARGS__);                            \
+        if (G == 8) return FN<8, float, false>(__VA_ARGS__);                            \
+        if (G == 16) return FN<16, float, false>(__VA_ARGS__);                          \
+      }                                                                                 \
+    } else {                                                                            \
+      if (res) {                                                                        \
+        if (G == 4) return FN<4, double, true>(__VA_ARGS__);                            \
+        if (G == 8) return FN<8, double, true>(__VA_ARGS__);                            \
+        if (G == 16) return FN<16, double, true>(__VA_ARGS__);                          \
+      } else {                                                                          \
+        if (G == 4) return FN<4, double, false>(__VA_ARGS__);                           \
+        if (G == 8) return FN<8, double, false>(__VA_ARGS__);                           \
+        if (G == 16) return FN<16, double, false>(__VA_ARGS__);                         \
+      }                                                                                 \
+    }                                                                                   \
+  } while (0)
+
+hipError_t launch_pcg(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st) {
+  HSC_PCG_DISPATCH(launch_pcg_t, P, shmem, st);
   return hipErrorInvalidValue;
 }
 
-const void* pcg_kernel_fn(int G, bool mixed) {
-  if (mixed) {
-    switch (G) {
-      case 4: return (const void*)pcg_persistent<4, float>;
-      case 8: return (const void*)pcg_persistent<8, float>;
-      case 16: return (const void*)pcg_persistent<16, float>;
-    }
-  } else {
-    switch (G) {
-      case 4: return (const void*)pcg_persistent<4, double>;
-      case 8: return (const void*)pcg_persistent<8, double>;
-      case 16: return (const void*)pcg_persistent<16, double>;
-    }
-  }
+hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st) {
+  HSC_PCG_DISPATCH(launch_pcg_plain_t, P, shmem, st);
+  return hipErrorInvalidValue;
+}
+
+hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks) {
+  HSC_PCG_DISPATCH(occ_t, blocks, shmem);
+  return hipErrorInvalidValue;
+}
+
+template <int G, typename VT, bool RES>
+static const void* kernel_fn_t() {
+  return (const void*)pcg_persistent<G, VT, RES>;
+}
+
+const void* pcg_kernel_fn(int G, bool mixed, bool res) {
+  HSC_PCG_DISPATCH(kernel_fn_t);
   return nullptr;
-}
-
-template <int G, typename VT>
-static hipError_t occ_t(int* blocks, size_t shmem) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pcg_persistent<G, VT>, kBlock, shmem);
-}
-
-hipError_t launch_pcg(int G, bool mixed, const PcgParams& P, size_t shmem, hipStream_t st) {
-  if (mixed) {
-    switch (G) {
-      case 4: return launch_pcg_t<4, float>(P, shmem, st);
-      case 8: return launch_pcg_t<8, float>(P, shmem, st);
-      case 16: return launch_pcg_t<16, float>(P, shmem, st);
-    }
-  } else {
-    switch (G) {
-      case 4: return launch_pcg_t<4, double>(P, shmem, st);
-      case 8: return launch_pcg_t<8, double>(P, shmem, st);
-      case 16: return launch_pcg_t<16, double>(P, shmem, st);
-    }
-  }
-  return hipErrorInvalidValue;
-}
-
-hipError_t pcg_occupancy(int G, bool mixed, size_t shmem, int* blocks) {
-  if (mixed) {
-    switch (G) {
-      case 4: return occ_t<4, float>(blocks, shmem);
-      case 8: return occ_t<8, float>(blocks, shmem);
-      case 16: return occ_t<16, float>(blocks, shmem);
-    }
-  } else {
-    switch (G) {
-      case 4: return occ_t<4, double>(blocks, shmem);
-      case 8: return occ_t<8, double>(blocks, shmem);
-      case 16: return occ_t<16, double>(blocks, shmem);
-    }
-  }
-  return hipErrorInvalidValue;
 }
 
 hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st) {

@@ -244,6 +244,7 @@ struct PcgPlan {
   size_t shmem = 0;
   int G = 8;
   bool mixed = false;
+  bool res = true;  // latency-schedule kernel
   std::vector<hsc::SysDesc> S;
   std::vector<int> ch;
 };
@@ -258,6 +259,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   const int node_tile = lc.sched == 2 ? hsc::kNodeTileMax : 128;
   plan.G = G;
   plan.mixed = mixed;
+  plan.res = lc.sched != 2;
   std::vector<hsc::SysDesc>& S = plan.S;
   S.resize(nsys);
   int rt = 0, et = 0, lt = 0, t_total = 0;
@@ -302,7 +304,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   }
   const size_t shmem = t_bytes + wc_bytes;
   int occ = 0;
-  HIPCHK(hsc::pcg_occupancy(G, mixed, shmem, &occ));
+  HIPCHK(hsc::pcg_occupancy(G, mixed, lc.sched != 2, shmem, &occ));
   if (occ <= 0) throw HscError{HSC_ERR_CAPACITY, "persistent PCG kernel does not fit on a CU"};
   const int cap = occ * di.cu;
   // Stage A runs row tiles (SpMV) on nA workgroups and low-rank tiles (W^T p)
@@ -455,7 +457,7 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
-  hipError_t le = hsc::launch_pcg(plan.G, plan.mixed, P, plan.shmem, st);
+  hipError_t le = hsc::launch_pcg(plan.G, plan.mixed, plan.res, P, plan.shmem, st);
   HIPCHK(hipEventRecord(e1, st));
   if (le != hipSuccess) {
     (void)hipStreamSynchronize(st);  // the table uploads read plan's host copies
@@ -1331,7 +1333,7 @@ int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double
       try {
         for (int i = 0; i < count; ++i) enqueue_rhs(handles[i], b[i], x[i], cap);
         reset_pcg(handles, count, bt->sc, cap);
-        HIPCHK(hsc::launch_pcg_plain(bt->plan.G, bt->plan.mixed, bt->plan.P, bt->plan.shmem, cap));
+        HIPCHK(hsc::launch_pcg_plain(bt->plan.G, bt->plan.mixed, bt->plan.res, bt->plan.P, bt->plan.shmem, cap));
         for (int i = 0; i < count; ++i) enqueue_finish(handles[i], b[i], x[i], cap);
       } catch (const HscError& e) {
         what = e.msg;
@@ -1346,7 +1348,7 @@ int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double
     (void)hipStreamDestroy(cap);
     if (!what.empty()) throw HscError{HSC_ERR_HIP, what};
     // mark the persistent PCG node cooperative (co-residency of its grid)
-    const void* fn = hsc::pcg_kernel_fn(bt->plan.G, bt->plan.mixed);
+    const void* fn = hsc::pcg_kernel_fn(bt->plan.G, bt->plan.mixed, bt->plan.res);
     size_t nn = 0;
     HIPCHK(hipGraphGetNodes(bt->graph, nullptr, &nn));
     std::vector<hipGraphNode_t> nodes(nn);
