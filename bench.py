@@ -264,6 +264,31 @@ def main():
     prof = ROOT / "profiles" / "pcg_persistent_traffic.json"
     if prof.exists():
         traffic = json.loads(prof.read_text()).get("traffic_bytes_per_launch_mixed")
+    # mixed vs FP64 on the 2,178-DOF system across FEM stiffness (the mixed-
+    # precision configuration's sweep); error of the mixed solution vs the FP64 one
+    sweep = []
+    c1_scene = {"linear_system/tol_rate": rtol, "linear_system/hsc/neumann_order": 2,
+                "linear_system/hsc/omega": 0.6, "linear_system/hsc/max_iter": 20000}
+    for E in (1e6, 1e8, 1e10):
+        sc1 = systems.cube_system(seed=0, E=E)
+        A1 = plugin.BCOO3.from_system(sc1, dev)
+        b1 = torch.as_tensor(sc1.b, device=dev)
+        res = {}
+        for prec in ("fp64", "mixed"):
+            ext = {"aca_rtol": 1e-6}
+            if prec == "mixed":
+                ext.update(precision="mixed", residual_replace_every=25)
+            h1 = plugin.Handle(plugin.make_config(c1_scene, **ext))
+            h1.setup(A1, sc1.qo, sc1.qm, sc1.fo, sc1.fn)
+            x1 = torch.zeros_like(b1)
+            st1 = h1.solve(b1, x1)
+            res[prec] = (st1, x1)
+        diff = float((torch.linalg.vector_norm(res["mixed"][1] - res["fp64"][1]) /
+                      torch.linalg.vector_norm(res["fp64"][1])).item())
+        sweep.append({"E": E, "iterations": {"fp64": res["fp64"][0]["pcg_iterations"],
+                                             "mixed": res["mixed"][0]["pcg_iterations"]},
+                      "mixed_rel_diff_vs_fp64": diff,
+                      "schur_relative_residual_mixed": res["mixed"][0]["schur_relative_residual"]})
     learned_c1 = None
     from hsc_reimpl_amd import learned
     if learned.WEIGHTS.exists():
@@ -299,6 +324,7 @@ def main():
                      "kernel_ms": round(new["pcg_ms"], 3)},
         "cpu_baseline": cpu,
         "learned_preconditioner": learned_c1,
+        "mixed_stiffness_sweep_2178_dof": sweep,
     }
     print(json.dumps(line), flush=True)
 

@@ -199,6 +199,27 @@ def test_mixed_precision_reaches_fp64_tolerance():
     assert st32["pcg_iterations"] <= 2 * st64["pcg_iterations"]
 
 
+@pytest.mark.parametrize("E", [1e6, 1e8, 1e10])
+def test_mixed_precision_stiffness_sweep(E):
+    """Mixed precision on the 2,178-DOF system with FEM E in {1e6, 1e8, 1e10}
+    (Neumann order 2, omega 0.6, rtol 1e-8): converges, stays within 1e-7 of
+    the FP64 direct solve, and takes at most 10% more iterations than FP64."""
+    s = systems.cube_system(seed=0, E=E)
+    h64, _ = _setup(s, C1_SCENE, aca_rtol=1e-6)
+    x64, st64 = _solve(h64, s)
+    h32, _ = _setup(s, C1_SCENE, aca_rtol=1e-6, precision="mixed", residual_replace_every=25)
+    x32, st32 = _solve(h32, s)
+    assert st64["status"] == 0 and st32["status"] == 0
+    assert st32["schur_relative_residual"] <= 1e-8
+    xd = osp.direct_solve(s)
+    err32 = np.linalg.norm(x32 - xd) / np.linalg.norm(xd)
+    err64 = np.linalg.norm(x64 - xd) / np.linalg.norm(xd)
+    print(f"E={E:g}: iterations fp64 {st64['pcg_iterations']} mixed {st32['pcg_iterations']}, "
+          f"error vs direct fp64 {err64:.2e} mixed {err32:.2e}")
+    assert err32 <= 1e-7 and err64 <= 1e-7
+    assert st32["pcg_iterations"] <= 1.1 * st64["pcg_iterations"] + 2
+
+
 def test_batched_matches_individual(c1):
     systems_ = [c1] + systems.batch_systems(seed=2, count=3, tets=(2000, 4000), tori=(1, 2))
     scene = {"linear_system/tol_rate": 1e-8, "linear_system/hsc/neumann_order": 3,
