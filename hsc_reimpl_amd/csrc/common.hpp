@@ -11,6 +11,7 @@ constexpr int kLrTile = 128;       // low-rank rows (W rows) per tile
 constexpr int kMaxSys = 128;       // systems per persistent launch
 constexpr int kMaxRankLds = 256;   // rank limit of the low-rank apply (lanes hold columns l, l+64, l+128, l+192)
 constexpr int kBarrierWords = 1024;  // grid-barrier counters (see device_util.hpp)
+constexpr int kMaxPolySteps = 32;    // per-step weights of the polynomial preconditioner
 
 // Per-system descriptor for the persistent PCG kernel and the operator
 // kernels.  All pointers are device pointers.  VT buffers (p, z, q) hold
@@ -67,7 +68,7 @@ struct PcgParams {
   SysOut* out;
   int nsys;
   int order;
-  double omega;
+  double omegas[kMaxPolySteps];  // step j weight: z0 = w0 Dinv r; z_j = z_(j-1) + w_j Dinv (r - D z_(j-1))
   int mode_true_every;    // core.py schedule: true residual each iteration
   int replace_every;      // mixed: replace r by the FP64 true residual
   int nwg;                // grid size

@@ -93,8 +93,9 @@ hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, siz
 const void* pcg_kernel_fn(int G, bool mixed, bool res);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);  // bytes % 4 == 0; a kernel, safe in graphs
 hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st);
-hipError_t launch_op_precond64(int G, const SysDesc& S, int order, double omega, const double* r, double* z,
+hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double* omegas, const double* r, double* z,
                                double* tmp, hipStream_t st);
+hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st);
 hipError_t launch_op_lowrank64(const SysDesc& S, const double* x, double* y, double* t, hipStream_t st);
 
 hipError_t launch_dot(const double* a, const double* b, int n, double* part, double* out, hipStream_t st);

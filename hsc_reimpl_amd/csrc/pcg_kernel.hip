@@ -587,7 +587,9 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   const int wave = tid >> 6;
   const bool is_lr = b < P.n_lr_wg;
   GridBarrier gbar;
-  const VT omega = (VT)P.omega;
+  // step weights (all equal to omega for the Neumann series)
+  auto step_w = [&](int j) { return (VT)P.omegas[j < kMaxPolySteps ? j : kMaxPolySteps - 1]; };
+  const VT omega = step_w(0);
 
   TileRange trA{0, 0}, trF{0, 0}, trE{0, 0}, trL{0, 0};
   const int cA = P.chunkA[b], cF = P.chunkF[b], cL = P.chunkL[b];
@@ -824,14 +826,15 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       };
       if (resF) {
         const SysDesc& S = sys[sF];
-        if (s_st[sF] == ST_RUN) tile_N_res<G, VT>(S, rowF, rrF, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], omega, last, rz);
+        if (s_st[sF] == ST_RUN)
+          tile_N_res<G, VT>(S, rowF, rrF, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], step_w(j), last, rz);
         flushF(sF);
       } else {
         for_tiles(sys, nsys, trF, 0,
                   [&](int s, int tl) {
                     if (s_st[s] != ST_RUN) return;
                     const SysDesc& S = sys[s];
-                    tile_N<G, VT, SU>(S, tl, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], omega, last, rz);
+                    tile_N<G, VT, SU>(S, tl, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], step_w(j), last, rz);
                   },
                   flushF);
       }
@@ -1337,26 +1340,45 @@ hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, h
   return hipGetLastError();
 }
 
-hipError_t launch_op_precond64(int G, const SysDesc& S, int order, double omega, const double* r,
+hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double* omegas, const double* r,
                                double* z, double* tmp, hipStream_t st) {
   const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
   const int ntile = (S.nb + kBlock - 1) / kBlock;
   if (S.nb == 0) return hipSuccess;
+  auto w = [&](int j) { return omegas[j < kMaxPolySteps ? j : kMaxPolySteps - 1]; };
   // z0 into (order odd ? z : tmp) so that the last step lands in z
   double* bufs[2] = {z, tmp};
   int cur = ((order - 1) % 2 == 0) ? 0 : 1;
-  hipLaunchKernelGGL(op_jacobi<double>, dim3(ntile), dim3(kBlock), 0, st, S, r, bufs[cur], omega);
+  hipLaunchKernelGGL(op_jacobi<double>, dim3(ntile), dim3(kBlock), 0, st, S, r, bufs[cur], w(0));
   SysDesc S2 = S;
   S2.r = const_cast<double*>(r);
   for (int j = 1; j < order; ++j) {
     switch (G) {
-      case 4: hipLaunchKernelGGL((op_neumann_step<4, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], omega); break;
-      case 8: hipLaunchKernelGGL((op_neumann_step<8, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], omega); break;
-      case 16: hipLaunchKernelGGL((op_neumann_step<16, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], omega); break;
+      case 4: hipLaunchKernelGGL((op_neumann_step<4, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], w(j)); break;
+      case 8: hipLaunchKernelGGL((op_neumann_step<8, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], w(j)); break;
+      case 16: hipLaunchKernelGGL((op_neumann_step<16, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], w(j)); break;
       default: return hipErrorInvalidValue;
     }
     cur ^= 1;
   }
+  return hipGetLastError();
+}
+
+// deterministic pseudo-random start vector in [-1, 1) (power iteration)
+__global__ void k_hash_fill(double* x, int n, unsigned seed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned h = (unsigned)i * 2654435761u ^ seed;
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  x[i] = (double)(h >> 8) * (2.0 / 16777216.0) - 1.0;
+}
+
+hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_hash_fill, dim3((n + 255) / 256), dim3(256), 0, st, x, n, seed);
   return hipGetLastError();
 }
 

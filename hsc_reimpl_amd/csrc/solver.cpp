@@ -115,6 +115,8 @@ struct hsc_handle {
   int doublings = 0;
   int G = 8, wgpcu = 1, sched = 1;
   int max_coupled128 = 0;  // most coupled nodes in one 128-node tile
+  double lambda_max = 0.0;  // power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights)
+  double omegas[hsc::kMaxPolySteps] = {};  // per-pass preconditioner weights
   double* xsol = nullptr;  // FEM part of the caller's x during a solve
   // buffers
   Buf row_count, rowptr, cursor, active_flag, act_prefix, act_node, act_dof;
@@ -149,7 +151,7 @@ int validate_config(const hsc_config& c, std::string& why) {
     return HSC_ERR_INVALID_ARGUMENT;
   }
   if (c.abd_inverse < 0 || c.abd_inverse > 2 || c.precision < 0 || c.precision > 1 || c.preconditioner < 0 ||
-      c.preconditioner > 1 || c.residual_replace_every < 0) {
+      c.preconditioner > 2 || c.residual_replace_every < 0) {
     why = "Invalid HSC mode selection";
     return HSC_ERR_INVALID_ARGUMENT;
   }
@@ -159,6 +161,11 @@ int validate_config(const hsc_config& c, std::string& why) {
   }
   if (c.wg_per_cu < 0 || c.wg_per_cu > 4) {
     why = "wg_per_cu must be in 0..4";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  if (c.preconditioner == HSC_PRECOND_CHEBYSHEV &&
+      !(c.neumann_order <= hsc::kMaxPolySteps && c.poly_interval_ratio > 1.0)) {
+    why = "Chebyshev preconditioner: neumann_order <= 32 and poly_interval_ratio > 1 required";
     return HSC_ERR_INVALID_ARGUMENT;
   }
   if (c.schedule < 0 || c.schedule > 2) {
@@ -386,7 +393,14 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   P.out = sc.sysout.as<hsc::SysOut>();
   P.nsys = nsys;
   P.order = h0->cfg.neumann_order;
-  P.omega = h0->cfg.omega;
+  {
+    // per-pass weights; a Chebyshev batch uses the interval of its largest
+    // spectrum bound (an upper bound for every system)
+    const hsc_handle* hw = h0;
+    for (int s = 1; s < nsys; ++s)
+      if (hs[s]->lambda_max > hw->lambda_max) hw = hs[s];
+    for (int j = 0; j < hsc::kMaxPolySteps; ++j) P.omegas[j] = hw->omegas[j];
+  }
   P.mode_true_every = h0->cfg.true_residual_every_iteration;
   P.replace_every = mixed ? h0->cfg.residual_replace_every : 0;
   P.nwg = nwg;
@@ -583,6 +597,58 @@ LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
 }
 
 // ------------------------------------------------------------------ setup
+// Per-pass preconditioner weights.  Neumann: omega for every pass.
+// Chebyshev: the passes z0 = w0 Dinv r, z_j = z_(j-1) + w_j Dinv (r - D z_(j-1))
+// have residual polynomial prod_j (1 - w_j mu) over the eigenvalues mu of
+// Dinv D; w_j = 1 / mu_j with mu_j the roots of the degree-m Chebyshev
+// polynomial shifted to [a, b] makes it the Chebyshev polynomial of that
+// interval (the min-max choice).  b = 1.2 x a 12-step power-iteration
+// estimate of lambda_max (from below), a = b / poly_interval_ratio.  The
+// preconditioner stays SPD for every mu in (0, b]: there each factor
+// 1 - mu / mu_j of the residual polynomial keeps the product below 1.
+void set_poly_weights(hsc_handle* h, hipStream_t st) {
+  const int m = h->cfg.neumann_order;
+  for (int j = 0; j < hsc::kMaxPolySteps; ++j) h->omegas[j] = h->cfg.omega;
+  h->lambda_max = 0.0;
+  if (h->cfg.preconditioner != HSC_PRECOND_CHEBYSHEV) return;
+  const int n = h->fn;
+  HIPCHK(h->ptmp.ensure(sizeof(double) * 3 * (size_t)n));
+  HIPCHK(h->dotbuf.ensure(sizeof(double) * 300));
+  double* x = h->ptmp.as<double>();
+  double* y = x + n;
+  double* z = y + n;
+  double* part = h->dotbuf.as<double>();
+  double* dout = part + 290;
+  hsc::SysDesc S = make_desc(h);
+  const double one[1] = {1.0};
+  auto norm = [&](const double* v) {
+    HIPCHK(hsc::launch_dot(v, v, n, part, dout, st));
+    double r = 0.0;
+    HIPCHK(hipMemcpyAsync(&r, dout, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return std::sqrt(r);
+  };
+  HIPCHK(hsc::launch_hash_fill(x, n, 0x9e3779b9u, st));
+  double nx = norm(x), lam = 0.0;
+  for (int it = 0; it < 12 && nx > 0; ++it) {
+    HIPCHK(hsc::launch_op_Dx64(8, S, x, y, st));
+    HIPCHK(hsc::launch_op_precond64(8, S, 1, one, y, z, x, st));  // z = Dinv D x
+    const double nz = norm(z);
+    lam = nz / nx;
+    if (!(nz > 0) || !std::isfinite(nz)) break;
+    HIPCHK(hsc::launch_axpby(x, z, 1.0 / nz, 0.0, n, st));  // x = z / |z|
+    nx = 1.0;
+  }
+  if (!(lam > 0) || !std::isfinite(lam)) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: spectrum estimate failed"};
+  h->lambda_max = lam;
+  const double b = 1.2 * lam, a = b / h->cfg.poly_interval_ratio;
+  std::vector<double> mu(m);
+  for (int j = 0; j < m; ++j) mu[j] = 0.5 * (b + a) + 0.5 * (b - a) * std::cos((2.0 * j + 1.0) * M_PI / (2.0 * m));
+  std::sort(mu.begin(), mu.end());
+  for (int j = 0; j < m; ++j) h->omegas[j] = 1.0 / mu[j];
+  for (int j = m; j < hsc::kMaxPolySteps; ++j) h->omegas[j] = h->omegas[m - 1];
+}
+
 void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, int fn, hipStream_t st) {
   auto t0 = Clock::now();
   h->setup_ok = false;
@@ -775,6 +841,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     HIPCHK(hipStreamSynchronize(st));
     if (fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
   }
+  set_poly_weights(h, st);
   h->setup_ok = true;
   h->setup_ms = ms_since(t0);
 }
@@ -849,7 +916,7 @@ int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   bool corrected = false;  // the current z carries a learned correction
   // z = M(r) (+ learned correction); returns r.z
   auto precondition = [&](double rr) {
-    HIPCHK(hsc::launch_op_precond64(G, S, h->cfg.neumann_order, h->cfg.omega, r, z, tmp, st));
+    HIPCHK(hsc::launch_op_precond64(G, S, h->cfg.neumann_order, h->omegas, r, z, tmp, st));
     corrected = false;
     if (!o.learned) return dot(r, z);
     HIPCHK(hipMemcpyAsync(tmp, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
@@ -1109,6 +1176,7 @@ void hsc_config_default(hsc_config* c) {
   c->schedule = 0;
   c->preconditioner = HSC_PRECOND_NEUMANN;
   c->check_acceptance = 1;
+  c->poly_interval_ratio = 100.0;
 }
 
 int32_t hsc_create(const hsc_config* cfg, int32_t device, hsc_handle** out) {
@@ -1148,7 +1216,9 @@ int32_t hsc_set_config(hsc_handle* h, const hsc_config* cfg) {
   const bool refactor = cfg->max_rank != h->cfg.max_rank || cfg->abd_inverse != h->cfg.abd_inverse ||
                         cfg->neumann_order != h->cfg.neumann_order || cfg->omega != h->cfg.omega ||
                         cfg->aca_rtol != h->cfg.aca_rtol || cfg->adaptive_abd_inverse != h->cfg.adaptive_abd_inverse ||
-                        cfg->abd_inverse_tol != h->cfg.abd_inverse_tol;
+                        cfg->abd_inverse_tol != h->cfg.abd_inverse_tol ||
+                        cfg->preconditioner != h->cfg.preconditioner ||
+                        cfg->poly_interval_ratio != h->cfg.poly_interval_ratio;
   h->cfg = *cfg;
   if (refactor) h->setup_ok = false;
   return HSC_OK;
@@ -1286,7 +1356,8 @@ int validate_batch(hsc_handle* const* handles, int32_t count, const double* cons
     if (!b[i] || !x[i]) return HSC_ERR_INVALID_ARGUMENT;
     if (handles[i]->device != h0->device || handles[i]->cfg.precision != h0->cfg.precision ||
         handles[i]->cfg.neumann_order != h0->cfg.neumann_order || handles[i]->cfg.omega != h0->cfg.omega ||
-        handles[i]->cfg.preconditioner != HSC_PRECOND_NEUMANN)
+        handles[i]->cfg.preconditioner == HSC_PRECOND_LEARNED ||
+        handles[i]->cfg.preconditioner != h0->cfg.preconditioner)
       return HSC_ERR_INVALID_ARGUMENT;
     for (int j = 0; j < i; ++j)
       if (handles[j] == handles[i]) return HSC_ERR_INVALID_ARGUMENT;  // one system per handle
@@ -1493,7 +1564,7 @@ int32_t hsc_apply_precond(hsc_handle* h, const double* r, double* z, hsc_stream 
   return guarded(h, [&] {
     hsc::SysDesc S = make_desc(h);
     hipStream_t st = (hipStream_t)stream;
-    HIPCHK(hsc::launch_op_precond64(h->G, S, h->cfg.neumann_order, h->cfg.omega, r, z, h->z1.as<double>(), st));
+    HIPCHK(hsc::launch_op_precond64(h->G, S, h->cfg.neumann_order, h->omegas, r, z, h->z1.as<double>(), st));
     HIPCHK(hipStreamSynchronize(st));
     return HSC_OK;
   });
@@ -1511,6 +1582,7 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   o->lanes_per_row = h->G;
   o->workgroups = h->wgpcu;
   o->schedule = h->sched;
+  o->lambda_max = h->lambda_max;
   o->neumann_doublings = h->doublings;
   o->abd_block_diagonal = h->blockdiag ? 1 : 0;
   o->aca_relative_error = h->aca_err;

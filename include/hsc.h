@@ -63,7 +63,10 @@ enum hsc_abd_inverse {
 /* Preconditioner. */
 enum hsc_preconditioner {
   HSC_PRECOND_NEUMANN = 0,        /* block-Jacobi Neumann series of D (linear_hsc.cu:330-343) */
-  HSC_PRECOND_LEARNED = 1         /* Neumann + graph-MLP correction, flexible PCG (no counterpart) */
+  HSC_PRECOND_LEARNED = 1,        /* Neumann + graph-MLP correction, flexible PCG (no counterpart) */
+  HSC_PRECOND_CHEBYSHEV = 2       /* same passes, per-pass weights from the Chebyshev roots on
+                                     [b/poly_interval_ratio, b], b = 1.2 x a power-iteration estimate
+                                     of lambda_max(Dinv D) (no counterpart) */
 };
 
 /* Solver configuration.  The first eight fields are the reference's scene
@@ -93,6 +96,7 @@ typedef struct hsc_config {
   int32_t schedule;              /* persistent-PCG work split: 0 = autotuned; 1 = latency (grid sized by the
                                     SpMV row tiles, 128-node elementwise tiles); 2 = throughput (grid covers
                                     row and low-rank tiles, split by tile count, 512-node tiles) */
+  double poly_interval_ratio;    /* HSC_PRECOND_CHEBYSHEV: b / a of the fitted interval (100) */
 } hsc_config;
 
 /* Device view of the assembled Hessian: the stored triangle of 3x3 blocks
@@ -183,6 +187,7 @@ typedef struct hsc_setup_info {
   int32_t lanes_per_row, workgroups, neumann_doublings, abd_block_diagonal;
   double aca_relative_error, abd_inverse_residual;
   int32_t schedule, reserved;
+  double lambda_max;             /* power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights; else 0) */
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* P,

@@ -97,6 +97,32 @@ class SparseNeumann:
         return z
 
 
+class SparsePolyJacobi:
+    """Block-Jacobi polynomial preconditioner with per-pass weights (the
+    build's Chebyshev option; no reference counterpart):
+    z0 = w0 B^-1 r, z_j = z_(j-1) + w_j B^-1 (r - D z_(j-1)).  With every w_j
+    equal to omega this is SparseNeumann's series in Horner form."""
+
+    def __init__(self, D, omegas):
+        self.D, self.omegas = D, list(omegas)
+        self.J = BlockJacobi3(D, 1.0)
+
+    def __call__(self, r):
+        z = self.omegas[0] * self.J(r)
+        for w in self.omegas[1:]:
+            z = z + w * self.J(r - self.D @ z)
+        return z
+
+
+def chebyshev_weights(lam_est, m, ratio, safety=1.2):
+    """Per-pass weights 1/mu_j, mu_j the roots of the degree-m Chebyshev
+    polynomial on [b/ratio, b], b = safety * lam_est, ascending mu."""
+    b = safety * lam_est
+    a = b / ratio
+    mu = sorted(0.5 * (b + a) + 0.5 * (b - a) * np.cos((2 * j + 1) * np.pi / (2 * m)) for j in range(m))
+    return [1.0 / x for x in mu], a, b
+
+
 class SparseHSC:
     """HSC with exact FP64 A^-1 (Cholesky) and sparse D (core.py:141-168 with
     exact_affine=True).  E is the compacted qm x active coupling."""

@@ -41,3 +41,36 @@ def test_cube_system_direct_solve():
     assert st["converged"]
     x = np.concatenate([dx, dq])
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7
+
+
+def test_poly_jacobi_with_equal_weights_is_neumann():
+    """The per-pass-weight preconditioner with every weight = omega is the
+    Neumann series (whose oracle is pinned by the reference's histories)."""
+    sysm = systems.cube_system(seed=0)
+    D, _, _, _ = osp.split_bcoo(sysm.rows, sysm.cols, sysm.vals, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
+    r = np.random.default_rng(3).normal(size=sysm.fn)
+    for order, omega in ((2, 0.6), (5, 0.7)):
+        a = osp.SparseNeumann(D, order, omega)(r)
+        b = osp.SparsePolyJacobi(D, [omega] * order)(r)
+        np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-14 * np.abs(a).max())
+
+
+def test_chebyshev_weights_give_the_chebyshev_residual_polynomial():
+    """For D = diag(mu) with B = I the residual polynomial of the passes,
+    1 - mu q(mu) = prod_j (1 - w_j mu), equals T_m((b+a-2mu)/(b-a)) /
+    T_m((b+a)/(b-a)) (numpy's Chebyshev series), whose max on [a, b] is
+    1 / T_m((b+a)/(b-a)); and it stays in (-1, 1) on (0, b]."""
+    from numpy.polynomial import chebyshev as Ch
+    for m, ratio in ((5, 100.0), (2, 10.0), (8, 30.0)):
+        w, a, b = osp.chebyshev_weights(2.5, m, ratio)
+        mu = np.linspace(1e-6, b, 4001)
+        res = np.ones_like(mu)
+        for wj in w:
+            res *= 1.0 - wj * mu
+        x0 = (b + a) / (b - a)
+        Tm = lambda x: Ch.chebval(x, [0] * m + [1])
+        ref = Tm((b + a - 2 * mu) / (b - a)) / Tm(x0)
+        inside = mu >= a
+        np.testing.assert_allclose(res[inside], ref[inside], rtol=1e-9, atol=1e-12)
+        assert np.abs(res[inside]).max() <= 1.0 / Tm(x0) * (1 + 1e-9)
+        assert np.all(res < 1.0) and np.all(res > -1.0)
