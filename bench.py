@@ -7,9 +7,12 @@ A step = one full HSC solve (device setup: split, ACA, ABD inverse, coupling
 factors; persistent Schur PCG; back-substitution; acceptance checks) with the
 Hessian and b already resident in HBM.
 
-Prints ONE JSON line.  `value` = solves/s of the new solver (FP32 bulk /
-FP64 reductions with an FP64 true residual every 25 iterations); the plain
-FP64 solver is timed in the same run and reported beside it.
+Prints ONE JSON line.  `value` = solves/s of the new solver: FP32 bulk /
+FP64 reductions with an FP64 true residual every 25 iterations, and the
+Neumann preconditioner's 5 passes weighted by the Chebyshev roots of a
+power-iteration spectrum bound.  The plain FP64 solver (the reference
+algorithm: Neumann series, FP64) is timed in the same run and reported
+beside it, with the mixed-precision Neumann and FP64 Chebyshev variants.
 """
 from __future__ import annotations
 
@@ -79,7 +82,8 @@ def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
 
 def run_batch64(args, dev, order, omega, rtol):
     """64 uneven systems (seed 2: 1-16 ABD bodies, 1-8 tori of 2k-15k tets,
-    ranks 4-16), FP32 bulk / FP64 reductions, rtol 1e-8 each: one captured
+    ranks 4-16), the new solver (FP32 bulk / FP64 reductions, Chebyshev-
+    weighted passes; one interval for the batch), rtol 1e-8 each: one captured
     hipGraph replay (right-hand sides, one persistent PCG launch over all 64,
     back-substitution and acceptance) vs the same systems solved one at a
     time.  A step = one solve of all 64 systems (setup excluded: the systems
@@ -93,7 +97,7 @@ def run_batch64(args, dev, order, omega, rtol):
         f"[batch64] generated {i + 1} systems ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True))
     scene = {"linear_system/tol_rate": rtol, "linear_system/hsc/neumann_order": order,
              "linear_system/hsc/omega": omega, "linear_system/hsc/max_iter": 20000}
-    cfg = dict(precision="mixed", residual_replace_every=25)
+    cfg = dict(precision="mixed", residual_replace_every=25, preconditioner="chebyshev")
     hs, As, bs = [], [], []
     t0 = time.perf_counter()
     for s in syss:
@@ -182,6 +186,7 @@ def main():
     ap.add_argument("--lanes-per-row", type=int, default=4)
     ap.add_argument("--wg-per-cu", type=int, default=1)
     ap.add_argument("--schedule", type=int, default=1)
+    ap.add_argument("--preconditioner", default="chebyshev", choices=["neumann", "chebyshev"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the HSC solve of one system runs on one MI355X and no "
@@ -208,7 +213,7 @@ def main():
     if args.traffic_probe:
         h = plugin.Handle(plugin.make_config(scene, precision="mixed", residual_replace_every=25,
                                              lanes_per_row=args.lanes_per_row, wg_per_cu=args.wg_per_cu,
-                                             schedule=args.schedule))
+                                             schedule=args.schedule, preconditioner=args.preconditioner))
         h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
         x = torch.zeros_like(b)
         st = h.solve(b, x)
@@ -227,8 +232,9 @@ def main():
                           "copy_bytes": nbytes}), flush=True)
         return
 
-    def run(variant):
-        ext = {"precision": "mixed", "residual_replace_every": 25} if variant == "mixed" else {}
+    def run(precision, preconditioner="neumann"):
+        ext = {"precision": "mixed", "residual_replace_every": 25} if precision == "mixed" else {}
+        ext["preconditioner"] = preconditioner
         h = plugin.Handle(plugin.make_config(scene, **ext))
         x = torch.zeros_like(b)
         for _ in range(args.warmup):
@@ -251,8 +257,10 @@ def main():
                 "schur_relative_residual": st["schur_relative_residual"],
                 "full_relative_residual": st["full_relative_residual"], "info": info, "x": xs}
 
-    plain = run("fp64")
-    new = run("mixed")
+    plain = run("fp64")                      # the reference algorithm: Neumann series, FP64
+    mixed = run("mixed")                     # + FP32 bulk / FP64 reductions
+    new = run("mixed", "chebyshev")          # + Chebyshev-weighted passes (same pass count)
+    cheb64 = run("fp64", "chebyshev")
     info = plain["info"]
     nb, nnzb, a, k = info["fem_nodes"], info["nnzb"], info["active_fem_dofs"], info["rank"]
     err_vs_fp64 = float(np.linalg.norm(new["x"] - plain["x"]) / np.linalg.norm(plain["x"]))
@@ -263,7 +271,7 @@ def main():
     traffic = None
     prof = ROOT / "profiles" / "pcg_persistent_traffic.json"
     if prof.exists():
-        traffic = json.loads(prof.read_text()).get("traffic_bytes_per_launch_mixed")
+        traffic = json.loads(prof.read_text()).get("traffic_bytes_per_launch_new")
     # mixed vs FP64 on the 2,178-DOF system across FEM stiffness (the mixed-
     # precision configuration's sweep); error of the mixed solution vs the FP64 one
     sweep = []
@@ -311,15 +319,22 @@ def main():
                    "active_fem_dofs": a, "rank": k, "neumann_order": order, "omega": omega, "rtol": rtol,
                    "lanes_per_row": info["lanes_per_row"], "wg_per_cu": info["workgroups"],
                    "schedule": info["schedule"]},
-        "pcg_iterations": {"new": new["pcg_iterations"], "plain_fp64": plain["pcg_iterations"]},
+        "new": "mixed precision (FP32 bulk, FP64 reductions and true residual every 25) + Chebyshev-weighted "
+               "Neumann passes (5 passes, weights from a power-iteration spectrum bound)",
+        "pcg_iterations": {"new": new["pcg_iterations"], "plain_fp64": plain["pcg_iterations"],
+                           "mixed_neumann": mixed["pcg_iterations"], "fp64_chebyshev": cheb64["pcg_iterations"]},
         "plain_fp64": {"solves_per_s": round(plain["solves_per_s"], 3), "ms_per_solve": round(plain["ms_per_solve"], 3),
                        "pcg_ms": round(plain["pcg_ms"], 3), "pcg_gbs": round(achieved64, 1)},
+        "mixed_neumann": {"solves_per_s": round(mixed["solves_per_s"], 3),
+                          "ms_per_solve": round(mixed["ms_per_solve"], 3), "pcg_ms": round(mixed["pcg_ms"], 3)},
+        "fp64_chebyshev": {"solves_per_s": round(cheb64["solves_per_s"], 3),
+                           "ms_per_solve": round(cheb64["ms_per_solve"], 3), "pcg_ms": round(cheb64["pcg_ms"], 3)},
         "speedup_vs_plain_fp64": round(new["solves_per_s"] / plain["solves_per_s"], 3),
         "fp64_solution_rel_diff": err_vs_fp64,
         "schur_relative_residual": new["schur_relative_residual"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "pcg_persistent<G, float> (whole Schur PCG, one cooperative launch)",
+                     "kernel": "pcg_persistent<G, float, true> (whole Schur PCG of the new solver, one cooperative launch)",
                      "bytes_per_iteration": bpi, "iterations": new["pcg_iterations"],
                      "kernel_ms": round(new["pcg_ms"], 3)},
         "cpu_baseline": cpu,

@@ -1377,6 +1377,22 @@ __global__ void k_hash_fill(double* x, int n, unsigned seed) {
   x[i] = (double)(h >> 8) * (2.0 / 16777216.0) - 1.0;
 }
 
+// power-iteration step on the device: out[it] = |z| (from nrm2 = |z|^2),
+// x = z / |z|; a zero or non-finite norm leaves x untouched
+__global__ void k_normalize(double* x, const double* z, int n, const double* nrm2, double* out, int it) {
+  const double nz = sqrt(*nrm2);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) out[it] = nz;
+  if (i >= n || !(nz > 0.0) || !isfinite(nz)) return;
+  x[i] = z[i] / nz;
+}
+
+hipError_t launch_normalize(double* x, const double* z, int n, const double* nrm2, double* out, int it,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_normalize, dim3((n + 255) / 256), dim3(256), 0, st, x, z, n, nrm2, out, it);
+  return hipGetLastError();
+}
+
 hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_hash_fill, dim3((n + 255) / 256), dim3(256), 0, st, x, n, seed);
   return hipGetLastError();

@@ -621,24 +621,22 @@ void set_poly_weights(hsc_handle* h, hipStream_t st) {
   double* dout = part + 290;
   hsc::SysDesc S = make_desc(h);
   const double one[1] = {1.0};
-  auto norm = [&](const double* v) {
-    HIPCHK(hsc::launch_dot(v, v, n, part, dout, st));
-    double r = 0.0;
-    HIPCHK(hipMemcpyAsync(&r, dout, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return std::sqrt(r);
-  };
-  HIPCHK(hsc::launch_hash_fill(x, n, 0x9e3779b9u, st));
-  double nx = norm(x), lam = 0.0;
-  for (int it = 0; it < 12 && nx > 0; ++it) {
+  // 12 steps from a fixed pseudo-random start, normalised on the device (one
+  // host synchronisation at the end): lambda estimate = |Dinv D x| for |x| = 1
+  constexpr int kPowerSteps = 12;
+  double* lams = part + 260;  // kPowerSteps slots of the dot scratch
+  HIPCHK(hsc::launch_hash_fill(z, n, 0x9e3779b9u, st));
+  HIPCHK(hsc::launch_dot(z, z, n, part, dout, st));
+  HIPCHK(hsc::launch_normalize(x, z, n, dout, lams, 0, st));
+  for (int it = 0; it < kPowerSteps; ++it) {
     HIPCHK(hsc::launch_op_Dx64(8, S, x, y, st));
-    HIPCHK(hsc::launch_op_precond64(8, S, 1, one, y, z, x, st));  // z = Dinv D x
-    const double nz = norm(z);
-    lam = nz / nx;
-    if (!(nz > 0) || !std::isfinite(nz)) break;
-    HIPCHK(hsc::launch_axpby(x, z, 1.0 / nz, 0.0, n, st));  // x = z / |z|
-    nx = 1.0;
+    HIPCHK(hsc::launch_op_precond64(8, S, 1, one, y, z, x, st));  // z = Dinv D x (order 1 leaves x alone)
+    HIPCHK(hsc::launch_dot(z, z, n, part, dout, st));
+    HIPCHK(hsc::launch_normalize(x, z, n, dout, lams, it, st));
   }
+  double lam = 0.0;
+  HIPCHK(hipMemcpyAsync(&lam, lams + kPowerSteps - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   if (!(lam > 0) || !std::isfinite(lam)) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: spectrum estimate failed"};
   h->lambda_max = lam;
   const double b = 1.2 * lam, a = b / h->cfg.poly_interval_ratio;
