@@ -96,6 +96,8 @@ hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, h
 hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double* omegas, const double* r, double* z,
                                double* tmp, hipStream_t st);
 hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st);
+hipError_t launch_power_step(int G, const SysDesc& S, const double* x, const double* nrm2_in, double* z,
+                             double* part, double* nrm2_out, hipStream_t st);
 hipError_t launch_normalize(double* x, const double* z, int n, const double* nrm2, double* out, int it,
                             hipStream_t st);
 hipError_t launch_op_lowrank64(const SysDesc& S, const double* x, double* y, double* t, hipStream_t st);

@@ -1122,6 +1122,34 @@ __global__ __launch_bounds__(kBlock) void op_Dx64(SysDesc S, const double* x, do
   tile_Dx64<G>(S, blockIdx.x, x, y);
 }
 
+// One power-iteration step for lambda_max(Dinv D), fused: z = Dinv D (x s)
+// with s = 1 / sqrt(*nrm2_in) (1 when null), and the tile's partial |z|^2.
+template <int G>
+__global__ __launch_bounds__(kBlock) void op_power_step(SysDesc S, const double* x, const double* nrm2_in,
+                                                        double* z, double* part) {
+  __shared__ double red[kBlock / 64];
+  const double sc = nrm2_in ? 1.0 / sqrt(*nrm2_in) : 1.0;
+  const int g = threadIdx.x / G, lane = threadIdx.x % G;
+  const int row = blockIdx.x * (kBlock / G) + g;
+  double a0, a1, a2;
+  row_spmv<G, double>(S, S.val64, row, lane,
+                      [&](int c, double& x0, double& x1, double& x2) {
+                        x0 = x[3 * c] * sc;
+                        x1 = x[3 * c + 1] * sc;
+                        x2 = x[3 * c + 2] * sc;
+                      },
+                      a0, a1, a2);
+  double zz = 0.0;
+  if (row < S.nb && lane < 3) {
+    const double* inv = S.invD64 + 9 * row + 3 * lane;
+    const double zv = inv[0] * a0 + inv[1] * a1 + inv[2] * a2;
+    z[3 * row + lane] = zv;
+    zz = zv * zv;
+  }
+  const double t = block_sum_t0(zz, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 template <int G, typename VT>
 __global__ __launch_bounds__(kBlock) void op_neumann_step(SysDesc S, const VT* zin, VT* zout, double omega) {
   double rz = 0.0;
@@ -1361,6 +1389,21 @@ hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double*
     }
     cur ^= 1;
   }
+  return hipGetLastError();
+}
+
+// power iteration: the step and the fixed-order sum of its partials into nrm2_out
+hipError_t launch_power_step(int G, const SysDesc& S, const double* x, const double* nrm2_in, double* z,
+                             double* part, double* nrm2_out, hipStream_t st) {
+  const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
+  if (tiles == 0) return hipSuccess;
+  switch (G) {
+    case 4: hipLaunchKernelGGL(op_power_step<4>, dim3(tiles), dim3(kBlock), 0, st, S, x, nrm2_in, z, part); break;
+    case 8: hipLaunchKernelGGL(op_power_step<8>, dim3(tiles), dim3(kBlock), 0, st, S, x, nrm2_in, z, part); break;
+    case 16: hipLaunchKernelGGL(op_power_step<16>, dim3(tiles), dim3(kBlock), 0, st, S, x, nrm2_in, z, part); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, st, part, tiles, nrm2_out);
   return hipGetLastError();
 }
 

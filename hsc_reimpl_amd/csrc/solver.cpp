@@ -612,31 +612,32 @@ void set_poly_weights(hsc_handle* h, hipStream_t st) {
   h->lambda_max = 0.0;
   if (h->cfg.preconditioner != HSC_PRECOND_CHEBYSHEV) return;
   const int n = h->fn;
-  HIPCHK(h->ptmp.ensure(sizeof(double) * 3 * (size_t)n));
+  HIPCHK(h->ptmp.ensure(sizeof(double) * 2 * (size_t)n));
   HIPCHK(h->dotbuf.ensure(sizeof(double) * 300));
   double* x = h->ptmp.as<double>();
-  double* y = x + n;
-  double* z = y + n;
+  double* z = x + n;
   double* part = h->dotbuf.as<double>();
-  double* dout = part + 290;
   hsc::SysDesc S = make_desc(h);
-  const double one[1] = {1.0};
-  // 12 steps from a fixed pseudo-random start, normalised on the device (one
-  // host synchronisation at the end): lambda estimate = |Dinv D x| for |x| = 1
+  // 12 steps from a fixed pseudo-random start, each one fused kernel plus a
+  // fixed-order sum (one host synchronisation at the end): with x_k the
+  // unnormalised iterate, lambda estimate = |Dinv D x_k| / |x_k|
   constexpr int kPowerSteps = 12;
-  double* lams = part + 260;  // kPowerSteps slots of the dot scratch
-  HIPCHK(hsc::launch_hash_fill(z, n, 0x9e3779b9u, st));
-  HIPCHK(hsc::launch_dot(z, z, n, part, dout, st));
-  HIPCHK(hsc::launch_normalize(x, z, n, dout, lams, 0, st));
+  const int tiles = (h->nb + hsc::kBlock / 8 - 1) / (hsc::kBlock / 8);
+  HIPCHK(h->ebuf.ensure(sizeof(double) * ((size_t)tiles + kPowerSteps + 1)));
+  double* ppart = h->ebuf.as<double>();
+  double* nrm2 = ppart + tiles;  // nrm2[k] = |x_(k+1)|^2 with x_0 normalised to 1 below
+  HIPCHK(hsc::launch_hash_fill(x, n, 0x9e3779b9u, st));
+  HIPCHK(hsc::launch_dot(x, x, n, part, nrm2 + kPowerSteps, st));  // |x_0|^2
+  double* bufs[2] = {x, z};
+  const double* prev = nrm2 + kPowerSteps;
   for (int it = 0; it < kPowerSteps; ++it) {
-    HIPCHK(hsc::launch_op_Dx64(8, S, x, y, st));
-    HIPCHK(hsc::launch_op_precond64(8, S, 1, one, y, z, x, st));  // z = Dinv D x (order 1 leaves x alone)
-    HIPCHK(hsc::launch_dot(z, z, n, part, dout, st));
-    HIPCHK(hsc::launch_normalize(x, z, n, dout, lams, it, st));
+    HIPCHK(hsc::launch_power_step(8, S, bufs[it & 1], prev, bufs[(it + 1) & 1], ppart, nrm2 + it, st));
+    prev = nrm2 + it;
   }
-  double lam = 0.0;
-  HIPCHK(hipMemcpyAsync(&lam, lams + kPowerSteps - 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  double last = 0.0;
+  HIPCHK(hipMemcpyAsync(&last, nrm2 + kPowerSteps - 1, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  const double lam = std::sqrt(last);  // x_(k-1) enters each step scaled to unit norm
   if (!(lam > 0) || !std::isfinite(lam)) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: spectrum estimate failed"};
   h->lambda_max = lam;
   const double b = 1.2 * lam, a = b / h->cfg.poly_interval_ratio;
