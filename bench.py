@@ -2,17 +2,18 @@
 
 Workload: the 78k-tet vehicle-scale system (7 hollow tori, 78,400 Tet4,
 23,520 FEM nodes = 70,560 FEM DOFs, 12 ABD bodies = 144 ABD DOFs, coupling
-rank 56), Neumann order 5, omega 0.7, rtol 1e-8, synthetic data (seed 1).
+rank 56), rtol 1e-8, synthetic data (seed 1).
 A step = one full HSC solve (device setup: split, ACA, ABD inverse, coupling
 factors; persistent Schur PCG; back-substitution; acceptance checks) with the
 Hessian and b already resident in HBM.
 
 Prints ONE JSON line.  `value` = solves/s of the new solver: FP32 bulk /
-FP64 reductions with an FP64 true residual every 25 iterations, and the
-Neumann preconditioner's 5 passes weighted by the Chebyshev roots of a
-power-iteration spectrum bound.  The plain FP64 solver (the reference
-algorithm: Neumann series, FP64) is timed in the same run and reported
-beside it, with the mixed-precision Neumann and FP64 Chebyshev variants.
+FP64 reductions with an FP64 true residual every 25 iterations, and 10
+block-Jacobi passes weighted by the Chebyshev roots of [b/300, b] with b
+from a power-iteration spectrum bound.  The plain FP64 solver (the
+reference algorithm: Neumann series of order 5, omega 0.7, FP64) is timed
+in the same run and reported beside it, with the mixed-precision Neumann
+and FP64 Chebyshev variants.
 """
 from __future__ import annotations
 
@@ -28,6 +29,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+NEW_PASSES, NEW_RATIO = 10, 300.0  # the new solver's preconditioner: passes, Chebyshev interval b/a
 METRIC = "HSC solves/s & PCG iters @ rtol 1e-8, 78k-tet system: new vs plain FP64"
 
 
@@ -82,8 +84,8 @@ def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
 
 def run_batch64(args, dev, order, omega, rtol):
     """64 uneven systems (seed 2: 1-16 ABD bodies, 1-8 tori of 2k-15k tets,
-    ranks 4-16), the new solver (FP32 bulk / FP64 reductions, Chebyshev-
-    weighted passes; one interval for the batch), rtol 1e-8 each: one captured
+    ranks 4-16), the new solver (FP32 bulk / FP64 reductions, 10 Chebyshev-
+    weighted passes, ratio 300; one interval for the batch), rtol 1e-8 each: one captured
     hipGraph replay (right-hand sides, one persistent PCG launch over all 64,
     back-substitution and acceptance) vs the same systems solved one at a
     time.  A step = one solve of all 64 systems (setup excluded: the systems
@@ -97,7 +99,8 @@ def run_batch64(args, dev, order, omega, rtol):
         f"[batch64] generated {i + 1} systems ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True))
     scene = {"linear_system/tol_rate": rtol, "linear_system/hsc/neumann_order": order,
              "linear_system/hsc/omega": omega, "linear_system/hsc/max_iter": 20000}
-    cfg = dict(precision="mixed", residual_replace_every=25, preconditioner="chebyshev")
+    cfg = dict(precision="mixed", residual_replace_every=25, preconditioner="chebyshev",
+               poly_interval_ratio=NEW_RATIO)
     hs, As, bs = [], [], []
     t0 = time.perf_counter()
     for s in syss:
@@ -190,7 +193,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the HSC solve of one system runs on one MI355X and no "
-              "multi-GPU figure is produced (see the sharding note in DESIGN.md)", file=sys.stderr)
+              "multi-GPU figure is produced", file=sys.stderr)
         sys.exit(2)
     # the CPU baseline is a single-thread measurement: pin BLAS before NumPy loads
     for var in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
@@ -203,17 +206,21 @@ def main():
     torch.cuda.set_device(dev)
     order, omega, rtol = 5, 0.7, 1e-8
     if args.workload == "batch64":
-        return run_batch64(args, dev, order, omega, rtol)
+        return run_batch64(args, dev, NEW_PASSES, omega, rtol)
     sysm = systems.torus_system(seed=1)
     A = plugin.BCOO3.from_system(sysm, dev)
     b = torch.as_tensor(sysm.b, device=dev)
     scene = {"linear_system/tol_rate": rtol, "linear_system/hsc/neumann_order": order,
              "linear_system/hsc/omega": omega, "linear_system/hsc/max_iter": 20000}
 
+    new_scene = dict(scene, **{"linear_system/hsc/neumann_order": NEW_PASSES})
     if args.traffic_probe:
-        h = plugin.Handle(plugin.make_config(scene, precision="mixed", residual_replace_every=25,
-                                             lanes_per_row=args.lanes_per_row, wg_per_cu=args.wg_per_cu,
-                                             schedule=args.schedule, preconditioner=args.preconditioner))
+        cheb = args.preconditioner == "chebyshev"
+        ext = {"poly_interval_ratio": NEW_RATIO} if cheb else {}
+        h = plugin.Handle(plugin.make_config(new_scene if cheb else scene, precision="mixed",
+                                             residual_replace_every=25, lanes_per_row=args.lanes_per_row,
+                                             wg_per_cu=args.wg_per_cu, schedule=args.schedule,
+                                             preconditioner=args.preconditioner, **ext))
         h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
         x = torch.zeros_like(b)
         st = h.solve(b, x)
@@ -228,14 +235,17 @@ def main():
         print(json.dumps({"probe": "traffic", "pcg_iterations": st["pcg_iterations"], "pcg_ms": st["pcg_ms"],
                           "bytes_per_iteration": pcg_bytes_per_iteration(info["fem_nodes"], info["nnzb"],
                                                                          info["active_fem_dofs"], info["rank"],
-                                                                         order, mixed=True),
+                                                                         NEW_PASSES if cheb else order,
+                                                                         mixed=True),
                           "copy_bytes": nbytes}), flush=True)
         return
 
     def run(precision, preconditioner="neumann"):
         ext = {"precision": "mixed", "residual_replace_every": 25} if precision == "mixed" else {}
         ext["preconditioner"] = preconditioner
-        h = plugin.Handle(plugin.make_config(scene, **ext))
+        if preconditioner == "chebyshev":
+            ext["poly_interval_ratio"] = NEW_RATIO
+        h = plugin.Handle(plugin.make_config(new_scene if preconditioner == "chebyshev" else scene, **ext))
         x = torch.zeros_like(b)
         for _ in range(args.warmup):
             h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
@@ -259,12 +269,13 @@ def main():
 
     plain = run("fp64")                      # the reference algorithm: Neumann series, FP64
     mixed = run("mixed")                     # + FP32 bulk / FP64 reductions
-    new = run("mixed", "chebyshev")          # + Chebyshev-weighted passes (same pass count)
+    new = run("mixed", "chebyshev")          # + 10 Chebyshev-weighted passes
     cheb64 = run("fp64", "chebyshev")
     info = plain["info"]
+    info_new = new["info"]
     nb, nnzb, a, k = info["fem_nodes"], info["nnzb"], info["active_fem_dofs"], info["rank"]
     err_vs_fp64 = float(np.linalg.norm(new["x"] - plain["x"]) / np.linalg.norm(plain["x"]))
-    bpi = pcg_bytes_per_iteration(nb, nnzb, a, k, order, mixed=True)
+    bpi = pcg_bytes_per_iteration(nb, nnzb, a, k, NEW_PASSES, mixed=True)
     achieved = bpi * new["pcg_iterations"] / (new["pcg_ms"] * 1e-3) / 1e9
     bpi64 = pcg_bytes_per_iteration(nb, nnzb, a, k, order, mixed=False)
     achieved64 = bpi64 * plain["pcg_iterations"] / (plain["pcg_ms"] * 1e-3) / 1e9
@@ -317,10 +328,12 @@ def main():
         "config": {"workload": "78k-tet vehicle-scale HSC system (7 tori x 11,200 Tet4, 12 ABD bodies)",
                    "fem_nodes": nb, "fem_dofs": 3 * nb, "abd_dofs": info["abd_dofs"], "nnzb": nnzb,
                    "active_fem_dofs": a, "rank": k, "neumann_order": order, "omega": omega, "rtol": rtol,
-                   "lanes_per_row": info["lanes_per_row"], "wg_per_cu": info["workgroups"],
-                   "schedule": info["schedule"]},
-        "new": "mixed precision (FP32 bulk, FP64 reductions and true residual every 25) + Chebyshev-weighted "
-               "Neumann passes (5 passes, weights from a power-iteration spectrum bound)",
+                   "new_passes": NEW_PASSES, "new_interval_ratio": NEW_RATIO, "pass_sync": info_new["pass_sync"],
+                   "lanes_per_row": info_new["lanes_per_row"], "wg_per_cu": info_new["workgroups"],
+                   "schedule": info_new["schedule"]},
+        "new": "mixed precision (FP32 bulk, FP64 reductions and true residual every 25) + 10 block-Jacobi "
+               "passes weighted by the Chebyshev roots of [b/300, b], b = 1.2 x a power-iteration bound; "
+               "neighbour-tile flags between passes",
         "pcg_iterations": {"new": new["pcg_iterations"], "plain_fp64": plain["pcg_iterations"],
                            "mixed_neumann": mixed["pcg_iterations"], "fp64_chebyshev": cheb64["pcg_iterations"]},
         "plain_fp64": {"solves_per_s": round(plain["solves_per_s"], 3), "ms_per_solve": round(plain["ms_per_solve"], 3),

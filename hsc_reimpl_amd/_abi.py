@@ -35,6 +35,7 @@ class Config(C.Structure):
         ("true_residual_every_iteration", C.c_int32), ("aca_rtol", C.c_double),
         ("lanes_per_row", C.c_int32), ("wg_per_cu", C.c_int32), ("preconditioner", C.c_int32),
         ("check_acceptance", C.c_int32), ("schedule", C.c_int32), ("poly_interval_ratio", C.c_double),
+        ("pass_sync", C.c_int32), ("reserved0", C.c_int32),
     ]
 
 
@@ -66,7 +67,7 @@ class SetupInfo(C.Structure):
         ("active_fem_dofs", C.c_int32), ("rank", C.c_int32), ("lanes_per_row", C.c_int32),
         ("workgroups", C.c_int32), ("neumann_doublings", C.c_int32), ("abd_block_diagonal", C.c_int32),
         ("aca_relative_error", C.c_double), ("abd_inverse_residual", C.c_double),
-        ("schedule", C.c_int32), ("reserved", C.c_int32), ("lambda_max", C.c_double),
+        ("schedule", C.c_int32), ("pass_sync", C.c_int32), ("lambda_max", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -12,6 +12,8 @@ constexpr int kMaxSys = 128;       // systems per persistent launch
 constexpr int kMaxRankLds = 256;   // rank limit of the low-rank apply (lanes hold columns l, l+64, l+128, l+192)
 constexpr int kBarrierWords = 1024;  // grid-barrier counters (see device_util.hpp)
 constexpr int kMaxPolySteps = 32;    // per-step weights of the polynomial preconditioner
+constexpr int kNbrStride = 64;       // row-tile neighbour list: [count, up to 63 tiles]; count -1 = every tile
+constexpr int kNbrMaxTiles = 1024;   // row tiles per system for which neighbour lists are built
 
 // Per-system descriptor for the persistent PCG kernel and the operator
 // kernels.  All pointers are device pointers.  VT buffers (p, z, q) hold
@@ -48,6 +50,7 @@ struct SysDesc {
   void* z[2];
   void* q;
   double* q64;            // FP64 D x for the true residual (== q in FP64 mode)
+  const int* nbr;         // [row tiles * kNbrStride] neighbour lists (local tile ids), or null
   double tol;
   int max_iter;
   int pad_;
@@ -90,6 +93,8 @@ struct PcgParams {
   int* error;             // device error flag
   long long timeout_ticks;// bounded wait (s_memrealtime ticks, 100 MHz)
   int t_total;            // LDS doubles for t vectors
+  int pass_sync;          // 1: passes before the last publish per-tile flags and wait on neighbour tiles
+  unsigned* pass_flags;   // [rt_total] last completed pass of each row tile (zeroed before launch)
   long long* prof;        // optional per-workgroup stage timers [nwg * 32] (null = off):
                           // site si: [2si] ticks computing before barrier si, [2si+1] ticks waiting in it
 };

@@ -96,6 +96,10 @@ hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, h
 hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double* omegas, const double* r, double* z,
                                double* tmp, hipStream_t st);
 hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st);
+// row-tile neighbour lists for R rows per tile (bits: ceil(rtl/32)*rtl words,
+// list: rtl*kNbrStride ints, rtl = ceil(nb/R) <= kNbrMaxTiles)
+hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, int R, unsigned* bits, int* list,
+                                 hipStream_t st);
 hipError_t launch_power_step(int G, const SysDesc& S, const double* x, const double* nrm2_in, double* z,
                              double* part, double* nrm2_out, hipStream_t st);
 hipError_t launch_normalize(double* x, const double* z, int n, const double* nrm2, double* out, int it,

@@ -250,8 +250,16 @@ __device__ __forceinline__ void row_spmv_res(const SysDesc& S, const MT* __restr
   a2 = group_sum<G>(a2);
 }
 
+// z store of a preconditioner pass: plain, or (WT) a relaxed agent-scope
+// atomic store, written through this XCD's L2 to the coherence point
+template <bool WT, typename VT>
+__device__ __forceinline__ void zst(VT* p, VT v) {
+  if constexpr (WT) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
 // Neumann step on the resident row tile (same arithmetic as tile_N)
-template <int G, typename VT>
+template <int G, typename VT, bool WT = false>
 __device__ __forceinline__ void tile_N_res(const SysDesc& S, int row, const RowRes<G, VT>& rr,
                                            const VT* __restrict__ zin, VT* __restrict__ zout, VT omega, bool last,
                                            double& rz) {
@@ -276,7 +284,7 @@ __device__ __forceinline__ void tile_N_res(const SysDesc& S, int row, const RowR
   if (own) {
     const VT e0 = (VT)r0 - a0, e1 = (VT)r1 - a1, e2 = (VT)r2 - a2;
     const VT out = zo + omega * (rr.d0 * e0 + rr.d1 * e1 + rr.d2 * e2);
-    zout[3 * row + lane] = out;
+    zst<WT>(zout + 3 * row + lane, out);
     if (last) rz += pick3(lane, r0, r1, r2) * (double)out;
   }
 }
@@ -565,7 +573,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   __shared__ int s_nchk[kMaxSys], s_nrs[kMaxSys];
   __shared__ double s_red[kBlock / 64];
   __shared__ double s_lr[kBlock / 64][64];
-  __shared__ int s_any;
+  __shared__ int s_any, s_pfail;
   __shared__ int s_list[kNodeTileMax], s_slot[kNodeTileMax], s_cnt;
   __shared__ double s_g[kNodeTileMax][3];
   __shared__ double s_tred[kBlock];
@@ -812,6 +820,63 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
               });
   };
 
+  // ---- point-to-point sync between preconditioner passes --------------
+  // A pass reads z only at the columns of its tile's rows, so pass j+1 of a
+  // row tile may start once the tiles its rows reference (and, for a
+  // symmetric pattern, the tiles that reference it) have finished pass j.
+  // With two z buffers this also covers the reuse of a buffer: a neighbour
+  // that has finished pass j has finished reading the buffer pass j+1
+  // overwrites.  A pass followed by such a wait stores z with agent-scope
+  // atomic stores (written through to the coherence point), waits for them
+  // to complete and then publishes the tile's pass count: for stores that
+  // are themselves agent-scope atomics that wait is the release, and no
+  // writeback of this XCD's L2 is needed (measured: a writeback per
+  // workgroup and pass costs more than the grid barrier it replaces).  The
+  // waiter polls its neighbours' counts and acquires (invalidating stale
+  // lines in its L1 and L2), then reads z with plain loads, so the
+  // arithmetic is the grid-barrier path's, bit for bit.  Only the resident
+  // row workgroups of the aligned split take part; the low-rank workgroups
+  // idle until the next grid barrier.
+  unsigned pseq = 0;
+  auto pass_wait = [&]() -> bool {
+    pseq += 1;
+    if (!resF) return true;
+    long long t_before = 0;
+    if (P.prof && tid == 0) t_before = rt_ticks();
+    wait_vm_zero();
+    __syncthreads();
+    const SysDesc& S = sys[sF];
+    const int tg = trF.t0, tl = tg - S.rt_begin;
+    if (tid == 0) {
+      __hip_atomic_store(P.pass_flags + tg, pseq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_pfail = 0;
+    }
+    __syncthreads();
+    if (s_st[sF] == ST_RUN) {
+      const int* L = S.nbr + (size_t)tl * kNbrStride;
+      const int cnt = L[0];
+      const int n = cnt >= 0 ? cnt : S.rt_end - S.rt_begin;
+      const long long t0 = rt_ticks();
+      for (int i = tid; i < n; i += kBlock) {
+        const int u = S.rt_begin + (cnt >= 0 ? L[1 + i] : i);
+        if (!gb_spin_until(P.pass_flags + u, pseq, P.error, t0, P.timeout_ticks)) s_pfail = 1;
+      }
+    }
+    __syncthreads();
+    const bool ok = s_pfail == 0;
+    if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+    if (P.prof && tid == 0) {
+      const long long t_after = rt_ticks();
+      long long* slot = P.prof + (size_t)b * 32;
+      slot[14] += t_before - t_last;
+      slot[15] += t_after - t_before;
+      slot[27] += 1;
+      t_last = t_after;
+    }
+    return ok;
+  };
+
   // ---- Neumann steps C1..C(order-1) -----------------------------------
   auto neumann_steps = [&]() -> bool {
     for (int j = 1; j < order; ++j) {
@@ -826,8 +891,16 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       };
       if (resF) {
         const SysDesc& S = sys[sF];
-        if (s_st[sF] == ST_RUN)
-          tile_N_res<G, VT>(S, rowF, rrF, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], step_w(j), last, rz);
+        if (s_st[sF] == ST_RUN) {
+          // write-through stores before a point-to-point wait
+          const VT* zi = (const VT*)S.z[zc];
+          VT* zn = (VT*)S.z[zc ^ 1];
+          const VT w = step_w(j);
+          if (P.pass_sync && !last)
+            tile_N_res<G, VT, true>(S, rowF, rrF, zi, zn, w, last, rz);
+          else
+            tile_N_res<G, VT>(S, rowF, rrF, zi, zn, w, last, rz);
+        }
         flushF(sF);
       } else {
         for_tiles(sys, nsys, trF, 0,
@@ -839,7 +912,11 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   flushF);
       }
       zc ^= 1;
-      if (!sync_at(1)) return false;
+      if (P.pass_sync && !last) {
+        if (!pass_wait()) return false;
+      } else {
+        if (!sync_at(1)) return false;
+      }
     }
     return true;
   };
@@ -1389,6 +1466,67 @@ hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double*
     }
     cur ^= 1;
   }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- row-tile neighbours
+// bits[t][u] = 1 when a row of tile t (R rows per tile) has a block in a row
+// of tile u != t: one workgroup per tile, the tile's blocks are contiguous
+__global__ __launch_bounds__(kBlock) void k_tile_nbr_bits(const int* __restrict__ rowptr,
+                                                          const int* __restrict__ colidx, int nb, int R, int words,
+                                                          unsigned* __restrict__ bits) {
+  __shared__ unsigned s_bits[kNbrMaxTiles / 32];
+  const int t = blockIdx.x;
+  for (int w = threadIdx.x; w < words; w += kBlock) s_bits[w] = 0u;
+  __syncthreads();
+  const int r0 = t * R, r1 = min(nb, r0 + R);
+  const int p0 = rowptr[r0], p1 = rowptr[r1];
+  for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+    const int u = colidx[p] / R;
+    if (u != t) atomicOr(&s_bits[u >> 5], 1u << (u & 31));
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < words; w += kBlock) bits[(size_t)t * words + w] = s_bits[w];
+}
+
+// list[t] = {count, u...} over u with bits[t][u] | bits[u][t], ascending;
+// count -1 when more than kNbrStride - 1 (the tile then waits on every tile)
+__global__ __launch_bounds__(kBlock) void k_tile_nbr_list(const unsigned* __restrict__ bits, int rtl, int words,
+                                                          int* __restrict__ list) {
+  __shared__ int s_wc[kBlock / 64];
+  const int t = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int base = 0;
+  for (int u0 = 0; u0 < rtl; u0 += kBlock) {
+    const int u = u0 + threadIdx.x;
+    bool f = false;
+    if (u < rtl && u != t)
+      f = (((bits[(size_t)t * words + (u >> 5)] >> (u & 31)) | (bits[(size_t)u * words + (t >> 5)] >> (t & 31))) &
+           1u) != 0u;
+    const unsigned long long m = __ballot(f);
+    if (lane == 0) s_wc[w] = __popcll(m);
+    __syncthreads();
+    int off = base, tot = base;
+    for (int i = 0; i < kBlock / 64; ++i) {
+      if (i < w) off += s_wc[i];
+      tot += s_wc[i];
+    }
+    if (f) {
+      const int pos = off + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < kNbrStride - 1) list[(size_t)t * kNbrStride + 1 + pos] = u;
+    }
+    __syncthreads();
+    base = tot;
+  }
+  if (threadIdx.x == 0) list[(size_t)t * kNbrStride] = base <= kNbrStride - 1 ? base : -1;
+}
+
+hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, int R, unsigned* bits, int* list,
+                                 hipStream_t st) {
+  const int rtl = (nb + R - 1) / R;
+  if (rtl <= 0 || rtl > kNbrMaxTiles) return hipErrorInvalidValue;
+  const int words = (rtl + 31) / 32;
+  hipLaunchKernelGGL(k_tile_nbr_bits, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, nb, R, words, bits);
+  hipLaunchKernelGGL(k_tile_nbr_list, dim3(rtl), dim3(kBlock), 0, st, bits, rtl, words, list);
   return hipGetLastError();
 }
 

@@ -53,7 +53,8 @@ struct Buf {
 // Scratch of one persistent PCG launch: system descriptors and outputs,
 // chunk tables, reduction partials, grid-barrier counters, error flag.
 struct PcgScratch {
-  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag;
+  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, flags;
+  int nflags = 0;  // row-tile pass flags zeroed before each launch (0: point-to-point pass sync off)
 };
 
 struct HscError {
@@ -114,7 +115,11 @@ struct hsc_handle {
   double aca_err = 0.0, abd_res = 0.0, setup_ms = 0.0;
   int doublings = 0;
   int G = 8, wgpcu = 1, sched = 1;
+  bool pass_sync_used = false;  // the last PCG launch synchronised its passes per tile
   int max_coupled128 = 0;  // most coupled nodes in one 128-node tile
+  // row-tile neighbour lists for 128, 64 and 32 rows per tile (lanes per row 4, 8, 16)
+  Buf tnbr[3], tnbr_bits;
+  bool tnbr_ok[3] = {false, false, false};
   double lambda_max = 0.0;  // power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights)
   double omegas[hsc::kMaxPolySteps] = {};  // per-pass preconditioner weights
   double* xsol = nullptr;  // FEM part of the caller's x during a solve
@@ -166,6 +171,10 @@ int validate_config(const hsc_config& c, std::string& why) {
   if (c.preconditioner == HSC_PRECOND_CHEBYSHEV &&
       !(c.neumann_order <= hsc::kMaxPolySteps && c.poly_interval_ratio > 1.0)) {
     why = "Chebyshev preconditioner: neumann_order <= 32 and poly_interval_ratio > 1 required";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  if (c.pass_sync < 0 || c.pass_sync > 1) {
+    why = "pass_sync must be 0 or 1";
     return HSC_ERR_INVALID_ARGUMENT;
   }
   if (c.schedule < 0 || c.schedule > 2) {
@@ -372,6 +381,28 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
       S[s].lrp_off = lrp;
     }
   }
+  // point-to-point sync between preconditioner passes (aligned latency
+  // split only: every row workgroup owns one resident row tile)
+  bool psync = h0->cfg.pass_sync && aligned && !thr && h0->cfg.neumann_order > 2;
+  for (int s = 0; s < nsys && psync; ++s) psync = (hs[s]->nb + R - 1) / R <= hsc::kNbrMaxTiles;
+  if (psync) {
+    const int li = G == 4 ? 0 : (G == 8 ? 1 : 2);
+    for (int s = 0; s < nsys; ++s) {
+      hsc_handle* h = hs[s];
+      if (!h->tnbr_ok[li]) {
+        const int rtl = (h->nb + R - 1) / R, words = (rtl + 31) / 32;
+        HIPCHK(h->tnbr[li].ensure(sizeof(int) * (size_t)rtl * hsc::kNbrStride));
+        HIPCHK(h->tnbr_bits.ensure(sizeof(unsigned) * (size_t)rtl * words));
+        HIPCHK(hsc::launch_tile_neighbors(h->rowptr.as<int>(), h->colidx.as<int>(), h->nb, R,
+                                          h->tnbr_bits.as<unsigned>(), h->tnbr[li].as<int>(), st));
+        h->tnbr_ok[li] = true;
+      }
+      S[s].nbr = h->tnbr[li].as<int>();
+    }
+    HIPCHK(sc.flags.ensure(sizeof(unsigned) * (size_t)rt));
+  }
+  sc.nflags = psync ? rt : 0;
+  for (int s = 0; s < nsys; ++s) hs[s]->pass_sync_used = psync;
   HIPCHK(sc.sysdesc.ensure(sizeof(hsc::SysDesc) * nsys));
   HIPCHK(sc.sysout.ensure(sizeof(hsc::SysOut) * nsys));
   HIPCHK(sc.chunks.ensure(sizeof(int) * 3 * nwg));
@@ -389,6 +420,8 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   HIPCHK(hipMemcpyAsync(sc.sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(sc.chunks.p, ch.data(), sizeof(int) * 3 * nwg, hipMemcpyHostToDevice, st));
   hsc::PcgParams& P = plan.P;
+  P.pass_sync = psync ? 1 : 0;
+  P.pass_flags = psync ? sc.flags.as<unsigned>() : nullptr;
   P.sys = sc.sysdesc.as<hsc::SysDesc>();
   P.out = sc.sysout.as<hsc::SysOut>();
   P.nsys = nsys;
@@ -439,6 +472,7 @@ void reset_pcg(hsc_handle* const* hs, int nsys, PcgScratch& sc, hipStream_t st) 
   HIPCHK(hsc::launch_zero(sc.errflag.p, 64, st));
   for (int s = 0; s < nsys; ++s)
     HIPCHK(hsc::launch_zero(hs[s]->p0.p, sizeof(double) * std::max(hs[s]->fn, 1), st));
+  if (sc.nflags > 0) HIPCHK(hsc::launch_zero(sc.flags.p, sizeof(unsigned) * sc.nflags, st));
 }
 
 // Copy the per-system outputs back and wait; raises on a device timeout.
@@ -495,25 +529,31 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   if (P.prof && max_iter_override <= 0) {
     std::vector<long long> pr((size_t)32 * nwg);
     HIPCHK(hipMemcpy(pr.data(), P.prof, sizeof(long long) * pr.size(), hipMemcpyDeviceToHost));
-    const char* names[6] = {"init", "neumann", "A", "B", "T1", "T2"};
+    const char* names[8] = {"init", "neumann", "A", "B", "T1", "T2", "", "nbr"};
     std::fprintf(stderr,
                  "[hsc stage timing] G=%d nwg=%d nL=%d iters=%d pcg_ms=%.3f (us per visit of each barrier site: "
                  "compute min/median/max over workgroups [argmax, LR?] | wait median)\n",
                  plan.G, nwg, P.n_lr_wg, out[0].iterations, ms);
-    for (int si = 0; si < 6; ++si) {
-      const long long n = pr[20 + si];
-      if (!n) continue;
-      std::vector<std::pair<double, int>> c(nwg);
-      std::vector<double> w(nwg);
+    for (int si = 0; si < 8; ++si) {
+      if (si == 6) continue;  // (slot 12 holds the stage-B reduction ticks)
+      // workgroups that visited the site (the point-to-point pass sync
+      // is visited by the row workgroups only)
+      long long n = 0;
+      std::vector<std::pair<double, int>> c;
+      std::vector<double> w;
       for (int b = 0; b < nwg; ++b) {
-        c[b] = {pr[(size_t)32 * b + 2 * si] / 100.0 / n, b};
-        w[b] = pr[(size_t)32 * b + 2 * si + 1] / 100.0 / n;
+        const long long nb = pr[(size_t)32 * b + 20 + si];
+        if (!nb) continue;
+        n = std::max(n, nb);
+        c.push_back({pr[(size_t)32 * b + 2 * si] / 100.0 / nb, b});
+        w.push_back(pr[(size_t)32 * b + 2 * si + 1] / 100.0 / nb);
       }
+      if (!n) continue;
       std::sort(c.begin(), c.end());
       std::sort(w.begin(), w.end());
-      const int am = c.back().second;
+      const int am = c.back().second, mid = (int)c.size() / 2;
       std::fprintf(stderr, "  %-8s n=%6lld  %8.2f %8.2f %8.2f [wg%d, %s] | %8.2f\n", names[si], n, c.front().first,
-                   c[nwg / 2].first, c.back().first, am, am < P.n_lr_wg ? "LR" : "row", w[nwg / 2]);
+                   c[mid].first, c.back().first, am, am < P.n_lr_wg ? "LR" : "row", w[mid]);
     }
     {
       // stage B split: reductions (slot 12) vs node tiles (rest of site 3)
@@ -651,6 +691,7 @@ void set_poly_weights(hsc_handle* h, hipStream_t st) {
 void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, int fn, hipStream_t st) {
   auto t0 = Clock::now();
   h->setup_ok = false;
+  for (bool& ok : h->tnbr_ok) ok = false;
   if (!Av) throw HscError{HSC_ERR_INVALID_ARGUMENT, "null matrix view"};
   const long long N = 3 * Av->block_dim;
   // linear_hsc.cu:197-200
@@ -1176,6 +1217,8 @@ void hsc_config_default(hsc_config* c) {
   c->preconditioner = HSC_PRECOND_NEUMANN;
   c->check_acceptance = 1;
   c->poly_interval_ratio = 100.0;
+  c->pass_sync = 1;
+  c->reserved0 = 0;
 }
 
 int32_t hsc_create(const hsc_config* cfg, int32_t device, hsc_handle** out) {
@@ -1581,6 +1624,7 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   o->lanes_per_row = h->G;
   o->workgroups = h->wgpcu;
   o->schedule = h->sched;
+  o->pass_sync = h->pass_sync_used ? 1 : 0;
   o->lambda_max = h->lambda_max;
   o->neumann_doublings = h->doublings;
   o->abd_block_diagonal = h->blockdiag ? 1 : 0;

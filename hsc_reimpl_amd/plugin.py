@@ -55,7 +55,7 @@ def make_config(scene: dict | None = None, **extra) -> _abi.Config:
     """Build an hsc_config from libuipc-style keys plus extension fields
     (abd_inverse, precision, residual_replace_every,
     true_residual_every_iteration, aca_rtol, lanes_per_row, wg_per_cu, schedule,
-    poly_interval_ratio,
+    poly_interval_ratio, pass_sync,
     preconditioner, check_acceptance)."""
     lib = _abi.load()
     cfg = _abi.Config()

@@ -97,6 +97,10 @@ typedef struct hsc_config {
                                     SpMV row tiles, 128-node elementwise tiles); 2 = throughput (grid covers
                                     row and low-rank tiles, split by tile count, 512-node tiles) */
   double poly_interval_ratio;    /* HSC_PRECOND_CHEBYSHEV: b / a of the fitted interval (100) */
+  int32_t pass_sync;             /* latency schedule, order > 2: 1 (default) = each row tile waits only for
+                                    its neighbour tiles between preconditioner passes; 0 = grid barriers.
+                                    Same arithmetic either way (bitwise-equal results) */
+  int32_t reserved0;
 } hsc_config;
 
 /* Device view of the assembled Hessian: the stored triangle of 3x3 blocks
@@ -186,7 +190,8 @@ typedef struct hsc_setup_info {
   int32_t fem_dofs, abd_dofs, fem_nodes, nnzb, active_fem_dofs, rank;
   int32_t lanes_per_row, workgroups, neumann_doublings, abd_block_diagonal;
   double aca_relative_error, abd_inverse_residual;
-  int32_t schedule, reserved;
+  int32_t schedule;              /* lanes_per_row, workgroups (per CU), schedule: the last solve's launch */
+  int32_t pass_sync;             /* 1 when the last solve's preconditioner passes synchronised per tile */
   double lambda_max;             /* power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights; else 0) */
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);

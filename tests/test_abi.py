@@ -41,6 +41,8 @@ def test_abi_version_and_defaults(lib):
     assert (c.max_rank, c.neumann_order, c.max_iter, c.adaptive_abd_inverse) == (128, 5, 2000, 1)
     assert (c.abd_inverse_tol, c.full_residual_limit, c.omega, c.tol_rate) == (1e-5, 0.1, 0.7, 1e-3)
     assert c.residual_replace_every == 25 and c.check_acceptance == 1
+    # extensions: autotuned schedule, Chebyshev interval ratio, neighbour-tile pass sync
+    assert (c.schedule, c.poly_interval_ratio, c.pass_sync) == (0, 100.0, 1)
 
 
 def test_invalid_config_rejected(lib):
@@ -51,6 +53,14 @@ def test_invalid_config_rejected(lib):
     assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
     lib.hsc_config_default(C.byref(c))
     c.neumann_order = 0
+    assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
+    for field, bad in (("schedule", 3), ("pass_sync", 2), ("pass_sync", -1), ("lanes_per_row", 6)):
+        lib.hsc_config_default(C.byref(c))
+        setattr(c, field, bad)
+        assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT, field
+    lib.hsc_config_default(C.byref(c))
+    c.preconditioner = _abi.PRECONDITIONER["chebyshev"]
+    c.poly_interval_ratio = 1.0
     assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
     assert lib.hsc_solve(None, None, None, None, None) == _abi.HSC_ERR_INVALID_ARGUMENT
 

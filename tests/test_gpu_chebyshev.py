@@ -8,6 +8,9 @@ through the C ABI.
 * on the 78k-tet system it converges to rtol 1e-8 in far fewer iterations than
   the Neumann series at the same number of passes, to the FP64 direct
   solution (FP64 and mixed precision).
+* the point-to-point sync between preconditioner passes (pass_sync = 1)
+  returns bit for bit what grid barriers return (pass_sync = 0), for one
+  system and for a captured batch.
 Tolerances are stated per test.
 """
 import numpy as np
@@ -123,3 +126,51 @@ def test_chebyshev_batch_matches_individual():
         err = (torch.linalg.vector_norm(xg[i] - xi[i]) / torch.linalg.vector_norm(xi[i])).item()
         assert err <= 1e-5, (i, err)
     batch.close()
+
+
+# latency schedule pinned (4 lanes per block row, 2 workgroups per CU): the
+# aligned split in which the row workgroups own one resident row tile each
+LATENCY = {"lanes_per_row": 4, "wg_per_cu": 2, "schedule": 1}
+
+
+def test_pass_sync_bitwise_equal_to_grid_barriers():
+    """78k-tet system, mixed precision, 10 Chebyshev-weighted passes, ratio
+    300: x, iteration count and residuals bitwise equal with neighbour-tile
+    flags and with grid barriers between the passes (same arithmetic in the
+    same order; only the waiting differs)."""
+    s = systems.torus_system(seed=1)
+    scene = dict(C2_SCENE, **{"linear_system/hsc/neumann_order": 10})
+    out = {}
+    for ps in (0, 1):
+        h = _setup(s, scene, preconditioner="chebyshev", precision="mixed", residual_replace_every=25,
+                   poly_interval_ratio=300.0, pass_sync=ps, **LATENCY)
+        x, st = _solve(h, s)
+        assert st["status"] == 0 and st["schur_relative_residual"] <= 1e-8
+        assert h.setup_info()["pass_sync"] == ps
+        out[ps] = (x, st)
+    assert np.array_equal(out[0][0], out[1][0])
+    for key in ("pcg_iterations", "schur_relative_residual", "full_relative_residual"):
+        assert out[0][1][key] == out[1][1][key], key
+
+
+def test_pass_sync_batch_bitwise():
+    """Captured batch of 3 uneven systems (seed 2), Chebyshev, mixed, latency
+    schedule: the same x and statistics bit for bit with and without the
+    neighbour-tile pass sync, every system converged to rtol 1e-8."""
+    syss = systems.batch_systems(seed=2, count=3)
+    res = {}
+    for ps in (0, 1):
+        hs = [_setup(s, C2_SCENE, preconditioner="chebyshev", precision="mixed", pass_sync=ps, **LATENCY)
+              for s in syss]
+        bs = [torch.as_tensor(s.b, device=DEV) for s in syss]
+        xg = [torch.zeros_like(b) for b in bs]
+        batch = plugin.Batch(hs, bs, xg)
+        batch.launch()
+        st = batch.stats()
+        assert all(d["status"] == 0 and d["schur_relative_residual"] <= 1e-8 for d in st)
+        assert all(h.setup_info()["pass_sync"] == ps for h in hs)
+        res[ps] = ([x.cpu().numpy() for x in xg], [(d["pcg_iterations"], d["schur_relative_residual"]) for d in st])
+        batch.close()
+    for a, b in zip(res[0][0], res[1][0]):
+        assert np.array_equal(a, b)
+    assert res[0][1] == res[1][1]
