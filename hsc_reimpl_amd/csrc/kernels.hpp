@@ -66,6 +66,8 @@ struct FactorParams {
 hipError_t launch_split(const SplitParams& S, hipStream_t st);
 hipError_t launch_scan(const int* in, int* out, int n, hipStream_t st);
 hipError_t launch_active(const int* flag, const int* prefix, int nb, int* act_node, int* act_dof, hipStream_t st);
+// *out = max over tiles of `tile` consecutive nodes of the active-node count (*out zeroed before)
+hipError_t launch_max_tile_active(const int* act_node, int nb, int tile, int* out, hipStream_t st);
 hipError_t launch_place(const PlaceParams& S, hipStream_t st);
 hipError_t launch_sort_gather(const int* rowptr, int nb, int* key_col, int* key_src, const double* vals, int nnzb,
                               int* colidx, double* v64, float* v32, hipStream_t st);
@@ -100,13 +102,20 @@ hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st);
 // list: rtl*kNbrStride ints, rtl = ceil(nb/R) <= kNbrMaxTiles)
 hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, int R, unsigned* bits, int* list,
                                  hipStream_t st);
-hipError_t launch_power_step(int G, const SysDesc& S, const double* x, const double* nrm2_in, double* z,
-                             double* part, double* nrm2_out, hipStream_t st);
+// power iteration: z = Dinv D x / |x| with |x|^2 the sum of part_in[0:np_in];
+// part gets power_step_tiles(G, nb) partials of |z|^2
+int power_step_tiles(int G, int nb);
+hipError_t launch_power_step(int G, const SysDesc& S, const double* x, const double* part_in, int np_in, double* z,
+                             double* part, hipStream_t st);
+// fixed-order sum of np partials into *out
+hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st);
 hipError_t launch_normalize(double* x, const double* z, int n, const double* nrm2, double* out, int it,
                             hipStream_t st);
 hipError_t launch_op_lowrank64(const SysDesc& S, const double* x, double* y, double* t, hipStream_t st);
 
 hipError_t launch_dot(const double* a, const double* b, int n, double* part, double* out, hipStream_t st);
+// partials of a.b only (*np_out <= 256 of them, into part)
+hipError_t launch_dot_partial(const double* a, const double* b, int n, double* part, int* np_out, hipStream_t st);
 hipError_t launch_update_xr(double* x, double* r, const double* p, const double* Ap, double alpha, int n,
                             hipStream_t st);
 hipError_t launch_axpby(double* y, const double* x, double a, double b, int n, hipStream_t st);

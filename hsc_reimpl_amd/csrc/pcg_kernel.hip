@@ -1200,12 +1200,22 @@ __global__ __launch_bounds__(kBlock) void op_Dx64(SysDesc S, const double* x, do
 }
 
 // One power-iteration step for lambda_max(Dinv D), fused: z = Dinv D (x s)
-// with s = 1 / sqrt(*nrm2_in) (1 when null), and the tile's partial |z|^2.
+// with s = 1 / |x| (|x|^2 the fixed-order sum of the previous step's np_in
+// partials, which every workgroup forms the same way), and the tile's
+// partial |z|^2.
 template <int G>
-__global__ __launch_bounds__(kBlock) void op_power_step(SysDesc S, const double* x, const double* nrm2_in,
-                                                        double* z, double* part) {
+__global__ __launch_bounds__(kBlock) void op_power_step(SysDesc S, const double* x, const double* part_in,
+                                                        int np_in, double* z, double* part) {
   __shared__ double red[kBlock / 64];
-  const double sc = nrm2_in ? 1.0 / sqrt(*nrm2_in) : 1.0;
+  __shared__ double s_sc;
+  {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np_in; i += kBlock) v += part_in[i];
+    const double t = block_sum_t0(v, red);
+    if (threadIdx.x == 0) s_sc = 1.0 / sqrt(t);
+    __syncthreads();
+  }
+  const double sc = s_sc;
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const int row = blockIdx.x * (kBlock / G) + g;
   double a0, a1, a2;
@@ -1269,12 +1279,13 @@ __global__ __launch_bounds__(kBlock) void k_dot_partial(const double* a, const d
   const double t = block_sum_t0(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
+// fixed-order sum of np partials by one wave: lane l adds l, l+64, ... in
+// order, then a butterfly (launch with 64 threads)
 __global__ void k_dot_final(const double* part, int np, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < np; ++i) s += part[i];
-    *out = s;
-  }
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += 64) s += part[i];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) *out = s;
 }
 // x += alpha p ; r -= alpha Ap
 __global__ void k_update_xr(double* x, double* r, const double* p, const double* Ap, double alpha, int n) {
@@ -1318,6 +1329,12 @@ __global__ void k_node_features(const int* rowptr, const int* colidx, const doub
   sdiag[i] = (float)s;
 }
 
+hipError_t launch_dot_partial(const double* a, const double* b, int n, double* part, int* np_out, hipStream_t st) {
+  const int np = min(256, max(1, (n + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(kBlock), 0, st, a, b, n, part);
+  *np_out = np;
+  return hipGetLastError();
+}
 hipError_t launch_dot(const double* a, const double* b, int n, double* part, double* out, hipStream_t st) {
   const int np = min(256, max(1, (n + kBlock - 1) / kBlock));
   hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(kBlock), 0, st, a, b, n, part);
@@ -1530,18 +1547,22 @@ hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, i
   return hipGetLastError();
 }
 
-// power iteration: the step and the fixed-order sum of its partials into nrm2_out
-hipError_t launch_power_step(int G, const SysDesc& S, const double* x, const double* nrm2_in, double* z,
-                             double* part, double* nrm2_out, hipStream_t st) {
-  const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
+// power iteration step (partials of |z|^2 into part, one per tile)
+int power_step_tiles(int G, int nb) { return (nb + (kBlock / G) - 1) / (kBlock / G); }
+hipError_t launch_power_step(int G, const SysDesc& S, const double* x, const double* part_in, int np_in, double* z,
+                             double* part, hipStream_t st) {
+  const int tiles = power_step_tiles(G, S.nb);
   if (tiles == 0) return hipSuccess;
   switch (G) {
-    case 4: hipLaunchKernelGGL(op_power_step<4>, dim3(tiles), dim3(kBlock), 0, st, S, x, nrm2_in, z, part); break;
-    case 8: hipLaunchKernelGGL(op_power_step<8>, dim3(tiles), dim3(kBlock), 0, st, S, x, nrm2_in, z, part); break;
-    case 16: hipLaunchKernelGGL(op_power_step<16>, dim3(tiles), dim3(kBlock), 0, st, S, x, nrm2_in, z, part); break;
+    case 4: hipLaunchKernelGGL(op_power_step<4>, dim3(tiles), dim3(kBlock), 0, st, S, x, part_in, np_in, z, part); break;
+    case 8: hipLaunchKernelGGL(op_power_step<8>, dim3(tiles), dim3(kBlock), 0, st, S, x, part_in, np_in, z, part); break;
+    case 16: hipLaunchKernelGGL(op_power_step<16>, dim3(tiles), dim3(kBlock), 0, st, S, x, part_in, np_in, z, part); break;
     default: return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, st, part, tiles, nrm2_out);
+  return hipGetLastError();
+}
+hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, st, part, np, out);
   return hipGetLastError();
 }
 

@@ -652,40 +652,57 @@ __global__ void k_make_W(const double* V, const double* Lm, int a, int k, double
 }
 
 // y = L^-1 (U^T P b_q)  (reduced right-hand side, core.py:163-164;
-// linear_hsc.cu:351-357 with V c = W L^-1 c).  Dot products are spread over
-// the waves (lanes over the inner index, butterfly sums).
+// linear_hsc.cu:351-357 with V c = W L^-1 c).  One workgroup: b_q, P b_q and
+// U^T P b_q in LDS, one thread per output entry (inner index ascending); the
+// forward substitution is column-oriented in one wave (lane j owns rows j,
+// j+64, ...; after y_i is final every later row subtracts L[j][i] y_i), with
+// L's lower triangle in LDS when it fits (k <= 64).
 __global__ __launch_bounds__(512) void k_rhs_small(const double* P, const double* U, const double* Lm,
                                                    const double* b, int qo, int qm, int k, double* tmp, double* y) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int i = wave; i < qm; i += nw) {  // tmp[0:qm] = P b_q
+  constexpr int kMaxQm = 1536;  // ABD DOFs staged in LDS (128 bodies)
+  __shared__ double s_b[kMaxQm];
+  __shared__ double s_pb[kMaxQm];
+  __shared__ double s_c[kMaxRankLds];
+  __shared__ double s_L[64 * 64];
+  const bool lds_b = qm <= kMaxQm;
+  const double* bq = lds_b ? s_b : b + qo;
+  double* pb = lds_b ? s_pb : tmp;
+  if (lds_b)
+    for (int i = threadIdx.x; i < qm; i += blockDim.x) s_b[i] = b[qo + i];
+  const bool lds_L = k <= 64;
+  if (lds_L)
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) s_L[e] = Lm[e];
+  __syncthreads();
+  for (int i = threadIdx.x; i < qm; i += blockDim.x) {  // P b_q
+    const double* row = P + (size_t)i * qm;
     double s = 0.0;
-    for (int m = lane; m < qm; m += 64) s += P[(size_t)i * qm + m] * b[qo + m];
-    s = wave_sum(s);
-    if (lane == 0) tmp[i] = s;
+    for (int m = 0; m < qm; ++m) s += row[m] * bq[m];
+    pb[i] = s;
   }
   __syncthreads();
-  for (int l = wave; l < k; l += nw) {  // c = U^T tmp
+  for (int l = threadIdx.x; l < k; l += blockDim.x) {  // c = U^T P b_q
+    const double* col = U + (size_t)qm * l;
     double s = 0.0;
-    for (int i = lane; i < qm; i += 64) s += U[i + (size_t)qm * l] * tmp[i];
-    s = wave_sum(s);
-    if (lane == 0) tmp[qm + l] = s;
+    for (int i = 0; i < qm; ++i) s += col[i] * pb[i];
+    s_c[l] = s;
   }
   __syncthreads();
-  // forward substitution L y = c by wave 0 (lanes over the inner index, y in LDS)
-  __shared__ double ys[kMaxRankLds];
-  if (wave == 0) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const double* L = lds_L ? s_L : Lm;
     for (int i = 0; i < k; ++i) {
-      double part = 0.0;
-      for (int m = lane; m < i; m += 64) part += Lm[(size_t)i * k + m] * ys[m];
-      part = wave_sum(part);
-      const double yi = (tmp[qm + i] - part) / Lm[(size_t)i * k + i];
-      if (lane == 0) ys[i] = yi;
+      double yi = 0.0;
+      if (lane == (i & 63)) {
+        yi = s_c[i] / L[(size_t)i * k + i];
+        s_c[i] = yi;
+      }
+      yi = __shfl(yi, i & 63, 64);
+      for (int j = i + 1 + ((lane - (i + 1)) & 63); j < k; j += 64) s_c[j] -= L[(size_t)j * k + i] * yi;
       __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < k; i += blockDim.x) y[i] = ys[i];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) y[i] = s_c[i];
 }
 
 // rhs = b_x - W y ; r = rhs ; x_fem = 0
@@ -719,18 +736,17 @@ __global__ __launch_bounds__(256) void k_backsub_vt(const double* V, const int* 
 }
 __global__ __launch_bounds__(512) void k_backsub(const double* P, const double* U, const double* b, double* x,
                                                  int qo, int qm, int k, double* tmp) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   for (int i = threadIdx.x; i < qm; i += blockDim.x) {
     double s = b[qo + i];
     for (int l = 0; l < k; ++l) s -= U[i + (size_t)qm * l] * tmp[l];
     tmp[k + i] = s;
   }
   __syncthreads();
-  for (int i = wave; i < qm; i += nw) {
+  for (int i = threadIdx.x; i < qm; i += blockDim.x) {  // one thread per row, inner index ascending
+    const double* row = P + (size_t)i * qm;
     double s = 0.0;
-    for (int m = lane; m < qm; m += 64) s += P[(size_t)i * qm + m] * tmp[k + m];
-    s = wave_sum(s);
-    if (lane == 0) x[qo + i] = s;
+    for (int m = 0; m < qm; ++m) s += row[m] * tmp[k + m];
+    x[qo + i] = s;
   }
 }
 
@@ -780,18 +796,31 @@ __global__ __launch_bounds__(256) void k_full_residual_abd(const double* E, cons
     part[3 * i + 2] = bi * x[qo + i];
   }
 }
+// fixed-order sums of the partial triples by one wave (lane l adds
+// triples l, l+64, ... in order, then a butterfly)
 __global__ void k_full_residual_final(const double* part, int nparts, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double R = 0.0, B = 0.0, X = 0.0;
-    for (int p = 0; p < nparts; ++p) {
-      R += part[3 * p];
-      B += part[3 * p + 1];
-      X += part[3 * p + 2];
-    }
+  double R = 0.0, B = 0.0, X = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 64) {
+    R += part[3 * p];
+    B += part[3 * p + 1];
+    X += part[3 * p + 2];
+  }
+  R = wave_sum(R);
+  B = wave_sum(B);
+  X = wave_sum(X);
+  if (threadIdx.x == 0) {
     out[0] = R;
     out[1] = B;
     out[2] = X;
   }
+}
+
+// most active (coupled) nodes in any tile of `tile` consecutive nodes
+// (tile <= blockDim.x); *out must start at zero
+__global__ void k_max_tile_active(const int* act_node, int nb, int tile, int* out) {
+  const int i = blockIdx.x * tile + threadIdx.x;
+  const int c = __syncthreads_count(threadIdx.x < tile && i < nb && act_node[i] >= 0);
+  if (threadIdx.x == 0) atomicMax(out, c);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -809,6 +838,13 @@ hipError_t launch_scan(const int* in, int* out, int n, hipStream_t st) {
 hipError_t launch_active(const int* flag, const int* prefix, int nb, int* act_node, int* act_dof, hipStream_t st) {
   if (nb == 0) return hipSuccess;
   hipLaunchKernelGGL(k_active, dim3(cdiv(nb, 256)), dim3(256), 0, st, flag, prefix, nb, act_node, act_dof);
+  return hipGetLastError();
+}
+hipError_t launch_max_tile_active(const int* act_node, int nb, int tile, int* out, hipStream_t st) {
+  if (nb == 0) return hipSuccess;
+  if (tile <= 0 || tile > 1024) return hipErrorInvalidValue;
+  const int threads = tile <= 64 ? 64 : (tile + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_max_tile_active, dim3(cdiv(nb, tile)), dim3(threads), 0, st, act_node, nb, tile, out);
   return hipGetLastError();
 }
 hipError_t launch_place(const PlaceParams& S, hipStream_t st) {

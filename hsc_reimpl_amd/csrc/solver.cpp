@@ -22,6 +22,8 @@
 #include "learned.hpp"
 #include "mlp.hpp"
 
+#include <cstring>
+#include <functional>
 #include <memory>
 
 namespace {
@@ -50,11 +52,41 @@ struct Buf {
   }
 };
 
+// Pinned host staging: device-to-host copies into it are asynchronous, so
+// several results are fetched with one stream synchronisation.
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 // Scratch of one persistent PCG launch: system descriptors and outputs,
 // chunk tables, reduction partials, grid-barrier counters, error flag.
 struct PcgScratch {
   Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, flags;
   int nflags = 0;  // row-tile pass flags zeroed before each launch (0: point-to-point pass sync off)
+  HostBuf host;    // pinned copies of the outputs and the error flag
+  // last uploaded descriptor and chunk tables (an identical plan is not uploaded again)
+  std::vector<unsigned char> up_sys, up_ch;
+  void* up_sys_ptr = nullptr;
+  void* up_ch_ptr = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ~PcgScratch() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
 };
 
 struct HscError {
@@ -133,9 +165,20 @@ struct hsc_handle {
   PcgScratch pcg;                           // the persistent PCG launch of hsc_solve / hsc_solve_batched
   // learned preconditioner: graph MLP over D's node graph
   Buf learned, nbr_ptr, nbr, dfeat, sdiag, xfeat, omlp, ptmp, dotbuf, ebuf;
+  Buf pwr;  // power iteration (Chebyshev weights)
   long long learned_bytes = 0;
   std::unique_ptr<hsc::MlpNet> mlp;
   int mlp_nb = -1;
+  // setup: a second stream runs the D-side work (row sort, value gather,
+  // power iteration) beside ACA; pinned staging for the setup's small results
+  hipStream_t aux = nullptr;
+  hipEvent_t aux_ev = nullptr;
+  HostBuf stage;
+  ~hsc_handle() {
+    if (aux) (void)hipStreamSynchronize(aux);
+    if (aux_ev) (void)hipEventDestroy(aux_ev);
+    if (aux) (void)hipStreamDestroy(aux);
+  }
 };
 
 namespace {
@@ -147,6 +190,10 @@ void set_err(hsc_handle* h, int code, const std::string& m) {
 
 int validate_config(const hsc_config& c, std::string& why) {
   // linear_hsc.cu:188-189
+  if (c.max_rank > hsc::kMaxRankLds) {
+    why = "max_rank must be at most 256 (coupling rank held in LDS by the Schur PCG)";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
   if (!(c.max_rank > 0 && c.neumann_order > 0 && c.max_iter > 0 && c.omega > 0 && c.omega <= 1)) {
     why = "Invalid HSC parameters (max_rank, neumann_order, max_iter > 0 and 0 < omega <= 1 required)";
     return HSC_ERR_INVALID_ARGUMENT;
@@ -417,8 +464,17 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   std::copy(chA.begin(), chA.end(), ch.begin());
   std::copy(chF.begin(), chF.end(), ch.begin() + nwg);
   std::copy(chL.begin(), chL.end(), ch.begin() + 2 * nwg);
-  HIPCHK(hipMemcpyAsync(sc.sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(sc.chunks.p, ch.data(), sizeof(int) * 3 * nwg, hipMemcpyHostToDevice, st));
+  // upload the tables unless the same ones are already in place (a solve
+  // of the same handles with the same launch shape reuses them)
+  auto upload = [&](void* dst, const void* src, size_t bytes, std::vector<unsigned char>& last, void*& last_ptr) {
+    const unsigned char* b = static_cast<const unsigned char*>(src);
+    if (last_ptr == dst && last.size() == bytes && std::memcmp(last.data(), b, bytes) == 0) return;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    last.assign(b, b + bytes);
+    last_ptr = dst;
+  };
+  upload(sc.sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, sc.up_sys, sc.up_sys_ptr);
+  upload(sc.chunks.p, ch.data(), sizeof(int) * 3 * nwg, sc.up_ch, sc.up_ch_ptr);
   hsc::PcgParams& P = plan.P;
   P.pass_sync = psync ? 1 : 0;
   P.pass_flags = psync ? sc.flags.as<unsigned>() : nullptr;
@@ -475,21 +531,40 @@ void reset_pcg(hsc_handle* const* hs, int nsys, PcgScratch& sc, hipStream_t st) 
   if (sc.nflags > 0) HIPCHK(hsc::launch_zero(sc.flags.p, sizeof(unsigned) * sc.nflags, st));
 }
 
-// Copy the per-system outputs back and wait; raises on a device timeout.
-std::vector<hsc::SysOut> collect_pcg(PcgScratch& sc, int nsys, hipStream_t st) {
-  std::vector<hsc::SysOut> out(nsys);
-  int derr = 0;
-  hipError_t ce = hipMemcpyAsync(out.data(), sc.sysout.p, sizeof(hsc::SysOut) * nsys, hipMemcpyDeviceToHost, st);
-  if (ce == hipSuccess) ce = hipMemcpyAsync(&derr, sc.errflag.p, sizeof(int), hipMemcpyDeviceToHost, st);
+// Copy the per-system outputs back (pinned) and wait; raises on a device
+// timeout.  With fin_hs, also each handle's acceptance sums (|b - Hx|^2, |b|^2,
+// b.x) into *fin, fetched in the same synchronisation.
+std::vector<hsc::SysOut> collect_pcg(PcgScratch& sc, int nsys, hipStream_t st, hsc_handle* const* fin_hs = nullptr,
+                                     std::vector<double>* fin = nullptr) {
+  const size_t out_bytes = sizeof(hsc::SysOut) * nsys;
+  const size_t fin_off = (out_bytes + 64 + 15) / 16 * 16;
+  HIPCHK(sc.host.ensure(fin_off + sizeof(double) * 3 * (size_t)nsys));
+  unsigned char* hb = sc.host.as<unsigned char>();
+  int* derr = reinterpret_cast<int*>(hb + out_bytes);
+  double* hfin = reinterpret_cast<double*>(hb + fin_off);
+  *derr = 0;
+  hipError_t ce = hipMemcpyAsync(hb, sc.sysout.p, out_bytes, hipMemcpyDeviceToHost, st);
+  if (ce == hipSuccess) ce = hipMemcpyAsync(derr, sc.errflag.p, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (fin_hs)
+    for (int i = 0; i < nsys && ce == hipSuccess; ++i)
+      ce = hipMemcpyAsync(hfin + 3 * i, fin_hs[i]->misc.as<double>() + 20, 3 * sizeof(double),
+                          hipMemcpyDeviceToHost, st);
   if (ce == hipSuccess) ce = hipStreamSynchronize(st);
   if (ce != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("PCG: ") + hipGetErrorString(ce)};
-  if (derr) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "persistent PCG: bounded grid wait expired"};
+  if (*derr) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "persistent PCG: bounded grid wait expired"};
+  std::vector<hsc::SysOut> out(nsys);
+  std::memcpy(out.data(), hb, out_bytes);
+  if (fin_hs && fin) fin->assign(hfin, hfin + 3 * (size_t)nsys);
   return out;
 }
 
 // Persistent PCG over a batch of set-up handles (one cooperative launch).
+// `after` enqueues further work behind the launch before the one host
+// synchronisation (the back-substitution and acceptance sums of a solve,
+// whose results come back in *fin).
 std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_override,
-                                 hipStream_t st, float* pcg_ms) {
+                                 hipStream_t st, float* pcg_ms, const std::function<void()>& after = {},
+                                 std::vector<double>* fin = nullptr) {
   hsc_handle* h0 = hs[0];
   PcgPlan plan = plan_pcg(hs, nsys, lc, max_iter_override, st, h0->pcg);
   reset_pcg(hs, nsys, h0->pcg, st);
@@ -501,30 +576,21 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
     HIPCHK(hsc::launch_zero(h0->profbuf.p, sizeof(long long) * 32 * nwg, st));
     P.prof = h0->profbuf.as<long long>();
   }
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  PcgScratch& sc = h0->pcg;
+  if (!sc.ev0) HIPCHK(hipEventCreate(&sc.ev0));
+  if (!sc.ev1) HIPCHK(hipEventCreate(&sc.ev1));
+  hipEvent_t e0 = sc.ev0, e1 = sc.ev1;
   HIPCHK(hipEventRecord(e0, st));
   hipError_t le = hsc::launch_pcg(plan.G, plan.mixed, plan.res, P, plan.shmem, st);
   HIPCHK(hipEventRecord(e1, st));
   if (le != hipSuccess) {
-    (void)hipStreamSynchronize(st);  // the table uploads read plan's host copies
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
     throw HscError{HSC_ERR_HIP, std::string("PCG launch: ") + hipGetErrorString(le)};
   }
-  std::vector<hsc::SysOut> out;
-  try {
-    out = collect_pcg(h0->pcg, nsys, st);
-  } catch (...) {
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    throw;
-  }
+  if (after) after();
+  std::vector<hsc::SysOut> out = collect_pcg(sc, nsys, st, fin ? hs : nullptr, fin);
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (pcg_ms) *pcg_ms = ms;
   if (P.prof && max_iter_override <= 0) {
     std::vector<long long> pr((size_t)32 * nwg);
@@ -646,38 +712,52 @@ LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
 // estimate of lambda_max (from below), a = b / poly_interval_ratio.  The
 // preconditioner stays SPD for every mu in (0, b]: there each factor
 // 1 - mu / mu_j of the residual polynomial keeps the product below 1.
-void set_poly_weights(hsc_handle* h, hipStream_t st) {
+constexpr int kPowerSteps = 12;
+
+// Device buffer of the power iteration: x, z (n each), two partial arrays.
+size_t power_buffer_doubles(int n, int nb) {
+  const size_t cap = (size_t)std::max(hsc::power_step_tiles(8, nb), 256);
+  return 2 * (size_t)n + 2 * cap + 1;
+}
+
+// Enqueue the power iteration for lambda_max(Dinv D) on st (D's rows and
+// inverse diagonal blocks ready there): 12 fused steps from a fixed
+// pseudo-random start, each normalising its input by the fixed-order norm of
+// the previous step's partials; *lam2_host (pinned) receives |Dinv D x_11|^2
+// with |x_11| = 1 once st has drained.
+void enqueue_power_iteration(hsc_handle* h, hipStream_t st, double* lam2_host) {
+  const int n = h->fn, nb = h->nb;
+  const size_t cap = (size_t)std::max(hsc::power_step_tiles(8, nb), 256);
+  double* x = h->pwr.as<double>();
+  double* z = x + n;
+  double* pa = z + n;
+  double* pb = pa + cap;
+  double* lam2 = pb + cap;
+  hsc::SysDesc S = make_desc(h);
+  HIPCHK(hsc::launch_hash_fill(x, n, 0x9e3779b9u, st));
+  int np_in = 0;
+  HIPCHK(hsc::launch_dot_partial(x, x, n, pa, &np_in, st));  // |x_0|^2 partials
+  double* bufs[2] = {x, z};
+  double* parts[2] = {pa, pb};
+  const int tiles = hsc::power_step_tiles(8, nb);
+  for (int it = 0; it < kPowerSteps; ++it) {
+    HIPCHK(hsc::launch_power_step(8, S, bufs[it & 1], parts[it & 1], np_in, bufs[(it + 1) & 1],
+                                  parts[(it + 1) & 1], st));
+    np_in = tiles;
+  }
+  HIPCHK(hsc::launch_sum_partials(parts[kPowerSteps & 1], tiles, lam2, st));
+  HIPCHK(hipMemcpyAsync(lam2_host, lam2, sizeof(double), hipMemcpyDeviceToHost, st));
+}
+
+// Per-pass weights: omega for the Neumann series; for the Chebyshev
+// preconditioner the reciprocal Chebyshev roots of [b / ratio, b] with
+// b = 1.2 x sqrt(lam2) (see the comment above).
+void set_poly_weights(hsc_handle* h, double lam2) {
   const int m = h->cfg.neumann_order;
   for (int j = 0; j < hsc::kMaxPolySteps; ++j) h->omegas[j] = h->cfg.omega;
   h->lambda_max = 0.0;
   if (h->cfg.preconditioner != HSC_PRECOND_CHEBYSHEV) return;
-  const int n = h->fn;
-  HIPCHK(h->ptmp.ensure(sizeof(double) * 2 * (size_t)n));
-  HIPCHK(h->dotbuf.ensure(sizeof(double) * 300));
-  double* x = h->ptmp.as<double>();
-  double* z = x + n;
-  double* part = h->dotbuf.as<double>();
-  hsc::SysDesc S = make_desc(h);
-  // 12 steps from a fixed pseudo-random start, each one fused kernel plus a
-  // fixed-order sum (one host synchronisation at the end): with x_k the
-  // unnormalised iterate, lambda estimate = |Dinv D x_k| / |x_k|
-  constexpr int kPowerSteps = 12;
-  const int tiles = (h->nb + hsc::kBlock / 8 - 1) / (hsc::kBlock / 8);
-  HIPCHK(h->ebuf.ensure(sizeof(double) * ((size_t)tiles + kPowerSteps + 1)));
-  double* ppart = h->ebuf.as<double>();
-  double* nrm2 = ppart + tiles;  // nrm2[k] = |x_(k+1)|^2 with x_0 normalised to 1 below
-  HIPCHK(hsc::launch_hash_fill(x, n, 0x9e3779b9u, st));
-  HIPCHK(hsc::launch_dot(x, x, n, part, nrm2 + kPowerSteps, st));  // |x_0|^2
-  double* bufs[2] = {x, z};
-  const double* prev = nrm2 + kPowerSteps;
-  for (int it = 0; it < kPowerSteps; ++it) {
-    HIPCHK(hsc::launch_power_step(8, S, bufs[it & 1], prev, bufs[(it + 1) & 1], ppart, nrm2 + it, st));
-    prev = nrm2 + it;
-  }
-  double last = 0.0;
-  HIPCHK(hipMemcpyAsync(&last, nrm2 + kPowerSteps - 1, sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  const double lam = std::sqrt(last);  // x_(k-1) enters each step scaled to unit norm
+  const double lam = std::sqrt(lam2);
   if (!(lam > 0) || !std::isfinite(lam)) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: spectrum estimate failed"};
   h->lambda_max = lam;
   const double b = 1.2 * lam, a = b / h->cfg.poly_interval_ratio;
@@ -687,6 +767,26 @@ void set_poly_weights(hsc_handle* h, hipStream_t st) {
   for (int j = 0; j < m; ++j) h->omegas[j] = 1.0 / mu[j];
   for (int j = m; j < hsc::kMaxPolySteps; ++j) h->omegas[j] = h->omegas[m - 1];
 }
+
+// Small setup results fetched from the device (pinned host memory).
+struct SetupStage {
+  int herr[8];
+  int nnzb, nact;
+  int ints2[2];
+  int aca_err, max_coupled;
+  double dm[4];
+  double lam2, fst;
+  double hist[hsc::kMaxRankLds + 1];
+};
+
+// Synchronises the setup's second stream when the setup leaves (normally or
+// by an exception), so no D-side kernel outlives the call.
+struct AuxJoin {
+  hipStream_t s;
+  ~AuxJoin() {
+    if (s) (void)hipStreamSynchronize(s);
+  }
+};
 
 void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, int fn, hipStream_t st) {
   auto t0 = Clock::now();
@@ -719,6 +819,8 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->invD32.ensure(sizeof(float) * 9 * nb));
   HIPCHK(h->A.ensure(sizeof(double) * qm * qm));
   HIPCHK(h->ints.ensure(sizeof(int) * 32));
+  HIPCHK(h->stage.ensure(sizeof(SetupStage)));
+  SetupStage* ss = h->stage.as<SetupStage>();
   HIPCHK(hipMemsetAsync(h->row_count.p, 0, sizeof(int) * nb, st));
   HIPCHK(hipMemsetAsync(h->cursor.p, 0, sizeof(int) * nb, st));
   HIPCHK(hipMemsetAsync(h->active_flag.p, 0, sizeof(int) * nb, st));
@@ -733,11 +835,12 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(hsc::launch_split(sp, st));
   HIPCHK(hsc::launch_scan(h->row_count.as<int>(), h->rowptr.as<int>(), nb, st));
   HIPCHK(hsc::launch_scan(h->active_flag.as<int>(), h->act_prefix.as<int>(), nb, st));
-  int herr[8], nnzb = 0, nact = 0;
-  HIPCHK(hipMemcpyAsync(herr, err, sizeof(herr), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&nnzb, h->rowptr.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&nact, h->act_prefix.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(ss->herr, err, sizeof(ss->herr), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&ss->nnzb, h->rowptr.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&ss->nact, h->act_prefix.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  const int* herr = ss->herr;
+  const int nnzb = ss->nnzb, nact = ss->nact;
   if (herr[1]) throw HscError{HSC_ERR_INVALID_ARGUMENT, "HSC requires only ABD (12/body) and FEM (3/node) DOFs"};
   if (herr[3]) throw HscError{HSC_ERR_UNEXPECTED_COUPLING, "Unexpected HSC subsystem coupling"};
   if (herr[4]) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: non-finite Hessian entries"};
@@ -779,29 +882,31 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->ybuf.ensure(sizeof(double) * lim1));
   HIPCHK(h->misc.ensure(sizeof(double) * 64));
   HIPCHK(h->respart.ensure(sizeof(double) * 3 * ((fn + hsc::kBlock - 1) / hsc::kBlock + qm + 1)));
+  const bool cheb = h->cfg.preconditioner == HSC_PRECOND_CHEBYSHEV;
+  if (cheb) HIPCHK(h->pwr.ensure(sizeof(double) * power_buffer_doubles(fn, nb)));
+  if (!h->aux) {
+    HIPCHK(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->aux_ev, hipEventDisableTiming));
+  }
   if (a > 0) HIPCHK(hipMemsetAsync(h->E.p, 0, sizeof(double) * (size_t)qm * a, st));
   HIPCHK(hsc::launch_active(h->active_flag.as<int>(), h->act_prefix.as<int>(), nb, h->act_node.as<int>(),
                             h->act_dof.as<int>(), st));
-  {
-    // coupled nodes per 128-node tile: sizes the LDS cache of W rows used by
-    // the persistent PCG's resident node tiles
-    std::vector<int> an(nb);
-    HIPCHK(hipMemcpyAsync(an.data(), h->act_node.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    int best = 0;
-    for (int t0 = 0; t0 < nb; t0 += 128) {
-      int c = 0;
-      for (int i = t0; i < std::min(nb, t0 + 128); ++i) c += an[i] >= 0;
-      best = std::max(best, c);
-    }
-    h->max_coupled128 = best;
-  }
+  // coupled nodes per 128-node tile (err[12]): sizes the LDS cache of W rows
+  // used by the persistent PCG's resident node tiles
+  HIPCHK(hsc::launch_max_tile_active(h->act_node.as<int>(), nb, 128, err + 12, st));
   hsc::PlaceParams pp{Av->row_indices, Av->col_indices, Av->values, Av->triplet_count, qo, qm, fo, fn,
                       h->rowptr.as<int>(), h->cursor.as<int>(), h->key_col.as<int>(), h->key_src.as<int>(),
                       h->act_node.as<int>(), a, h->E.as<double>()};
   HIPCHK(hsc::launch_place(pp, st));
+  // D side on the second stream, beside ACA and the ABD inverse: the row
+  // sort, the value gather and (Chebyshev) the power iteration
+  HIPCHK(hipEventRecord(h->aux_ev, st));
+  HIPCHK(hipStreamWaitEvent(h->aux, h->aux_ev, 0));
+  AuxJoin join{h->aux};
   HIPCHK(hsc::launch_sort_gather(h->rowptr.as<int>(), nb, h->key_col.as<int>(), h->key_src.as<int>(), Av->values,
-                                 nnzb, h->colidx.as<int>(), h->val64.as<double>(), h->val32.as<float>(), st));
+                                 nnzb, h->colidx.as<int>(), h->val64.as<double>(), h->val32.as<float>(), h->aux));
+  ss->lam2 = 0.0;
+  if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
   // ---- ACA
   int* rank_dev = err + 8;
   double* dmisc = h->misc.as<double>();
@@ -848,13 +953,17 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                      h->cfg.neumann_order, h->cfg.omega, h->cfg.abd_inverse_tol, dmisc};
   if (h->cfg.abd_inverse == HSC_ABD_NEUMANN && h->cfg.adaptive_abd_inverse) abp.mode = 2;
   HIPCHK(hsc::launch_abd_inverse(abp, st));
-  int ints2[2] = {0, 0};
-  double dm[4] = {0, 0, 0, 0};
-  int aca_err = 0;
-  HIPCHK(hipMemcpyAsync(ints2, err + 8, sizeof(ints2), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(dm, dmisc, sizeof(dm), hipMemcpyDeviceToHost, st));
-  if (limit > 0) HIPCHK(hipMemcpyAsync(&aca_err, h->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  ss->aca_err = 0;
+  HIPCHK(hipMemcpyAsync(ss->ints2, err + 8, sizeof(ss->ints2), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&ss->max_coupled, err + 12, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(ss->dm, dmisc, sizeof(ss->dm), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(ss->hist, h->hist.as<double>(), sizeof(double) * (lim1 + 1), hipMemcpyDeviceToHost, st));
+  if (limit > 0) HIPCHK(hipMemcpyAsync(&ss->aca_err, h->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  const int* ints2 = ss->ints2;
+  const double* dm = ss->dm;
+  const int aca_err = ss->aca_err;
+  h->max_coupled128 = ss->max_coupled;
   if (aca_err) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "ACA: bounded grid wait expired"};
   const int rank = limit > 0 ? ints2[0] : 0;
   if (ints2[1]) throw HscError{HSC_ERR_ABD_NOT_SPD, "HSC: non-SPD ABD block"};
@@ -866,9 +975,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     throw HscError{HSC_ERR_ABD_INVERSE, "HSC ABD inverse residual exceeds tolerance after " +
                                             std::to_string(h->doublings) + " doublings"};
   if (abd_status == 6) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite Neumann/Schur factors"};
-  double hv = 0.0;
-  HIPCHK(hipMemcpy(&hv, h->hist.as<double>() + rank, sizeof(double), hipMemcpyDeviceToHost));
-  h->aca_err = (limit > 0) ? hv : 0.0;
+  h->aca_err = (limit > 0) ? ss->hist[rank] : 0.0;
   h->k = rank;
   if (rank > 0) {
     hsc::FactorParams fp{h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), qm, a, rank,
@@ -876,12 +983,13 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     HIPCHK(hsc::launch_coupling_core(fp, st));
     HIPCHK(hsc::launch_make_W(h->V.as<double>(), h->Lm.as<double>(), a, rank, h->W64.as<double>(),
                               h->W32.as<float>(), st));
-    double fst = 0.0;
-    HIPCHK(hipMemcpyAsync(&fst, dmisc + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+    ss->fst = 0.0;
+    HIPCHK(hipMemcpyAsync(&ss->fst, dmisc + 4, sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    if (fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
+    if (ss->fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
   }
-  set_poly_weights(h, st);
+  HIPCHK(hipStreamSynchronize(h->aux));
+  set_poly_weights(h, ss->lam2);
   h->setup_ok = true;
   h->setup_ms = ms_since(t0);
 }
@@ -1151,14 +1259,16 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
                          h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
                          h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
   float pms = 0.f;
-  auto out = run_pcg(hs, 1, lc, 0, st, &pms);
+  std::vector<double> fin;
+  auto out = run_pcg(hs, 1, lc, 0, st, &pms, [&] { enqueue_finish(h, b, x, st); }, &fin);
   s->pcg_ms = pms;
   s->pcg_iterations = out[0].iterations;
   s->schur_relative_residual = out[0].rel;
   s->true_residual_checks = out[0].checks;
   s->restarts = out[0].restarts;
+  // the PCG's own failure message takes precedence over the acceptance one
+  int c2 = judge_finish(h, s, fin.data());
   int code = status_from_out(h, out[0]);
-  int c2 = finish_solve(h, b, x, s, st);
   if (code == HSC_OK) code = c2;
   s->status = code;
   s->total_ms = ms_since(t0);
@@ -1339,7 +1449,13 @@ int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count, const doubl
                              h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x[i], st));
     }
     float pms = 0.f;
-    auto out = run_pcg(handles, count, lc, 0, st, &pms);
+    std::vector<double> fin;
+    auto out = run_pcg(
+        handles, count, lc, 0, st, &pms,
+        [&] {
+          for (int i = 0; i < count; ++i) enqueue_finish(handles[i], b[i], x[i], st);
+        },
+        &fin);
     int first = HSC_OK;
     for (int i = 0; i < count; ++i) {
       hsc_handle* h = handles[i];
@@ -1352,8 +1468,8 @@ int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count, const doubl
       s->schur_relative_residual = out[i].rel;
       s->true_residual_checks = out[i].checks;
       s->restarts = out[i].restarts;
+      int c2 = judge_finish(h, s, fin.data() + 3 * i);
       int code = status_from_out(h, out[i]);
-      int c2 = finish_solve(h, b[i], x[i], s, st);
       if (code == HSC_OK) code = c2;
       s->status = code;
       s->total_ms = ms_since(t0);
@@ -1513,11 +1629,8 @@ int32_t hsc_batch_stats(hsc_batch* bt, hsc_stats* stats, hsc_stream stream) {
   const int n = (int)bt->hs.size();
   hsc_handle* h0 = bt->hs[0];
   return guarded(h0, [&] {
-    std::vector<double> res(3 * (size_t)n);
-    for (int i = 0; i < n; ++i)
-      HIPCHK(hipMemcpyAsync(res.data() + 3 * i, bt->hs[i]->misc.as<double>() + 20, 3 * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
-    std::vector<hsc::SysOut> out = collect_pcg(bt->sc, n, st);
+    std::vector<double> res;
+    std::vector<hsc::SysOut> out = collect_pcg(bt->sc, n, st, bt->hs.data(), &res);
     int first = HSC_OK;
     for (int i = 0; i < n; ++i) {
       hsc_handle* h = bt->hs[i];
@@ -1527,8 +1640,8 @@ int32_t hsc_batch_stats(hsc_batch* bt, hsc_stats* stats, hsc_stream stream) {
       s->schur_relative_residual = out[i].rel;
       s->true_residual_checks = out[i].checks;
       s->restarts = out[i].restarts;
-      int code = status_from_out(h, out[i]);
       const int c2 = judge_finish(h, s, res.data() + 3 * i);
+      int code = status_from_out(h, out[i]);
       if (code == HSC_OK) code = c2;
       s->status = code;
       if (first == HSC_OK) first = code;
