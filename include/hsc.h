@@ -206,8 +206,9 @@ int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* 
  * synchronisation); hsc_batch_stats waits for it and fills one hsc_stats per
  * system (pcg_ms is not measured inside a replay and reads 0).  The batch is
  * valid until one of its handles is set up again, reconfigured or
- * destroyed.  Handles must share device, precision, neumann_order and omega
- * and use the Neumann preconditioner. */
+ * destroyed.  Handles must share device, precision, neumann_order, omega and
+ * preconditioner (Neumann or Chebyshev; with Chebyshev the batch uses the
+ * weights of the handle with the largest spectrum bound, valid for all). */
 typedef struct hsc_batch hsc_batch;
 int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double* const* b, double* const* x,
                          hsc_stream stream, hsc_batch** out);

@@ -55,6 +55,28 @@ def test_poly_jacobi_with_equal_weights_is_neumann():
         np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-14 * np.abs(a).max())
 
 
+def test_poly_jacobi_residual_is_the_product_of_its_factors():
+    """Unequal weights: the passes' residual r - D M(r) equals
+    prod_j (I - w_j D B^-1) r, applied factor by factor (B = D's 3x3 diagonal
+    blocks), i.e. the recursion uses each weight exactly once."""
+    sysm = systems.cube_system(seed=0)
+    D, _, _, _ = osp.split_bcoo(sysm.rows, sysm.cols, sysm.vals, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
+    Dd = D.tocsr()
+    n = Dd.shape[0]
+    Binv = np.zeros((n // 3, 3, 3))
+    for i in range(n // 3):
+        Binv[i] = np.linalg.inv(Dd[3 * i:3 * i + 3, 3 * i:3 * i + 3].toarray())
+    apply_binv = lambda v: np.einsum("nij,nj->ni", Binv, v.reshape(-1, 3)).ravel()
+    r = np.random.default_rng(5).normal(size=n)
+    w, _, _ = osp.chebyshev_weights(2.9, 6, 50.0)
+    z = osp.SparsePolyJacobi(D, w)(r)
+    e = r.copy()
+    for wj in w:
+        e = e - wj * (Dd @ apply_binv(e))
+    res = r - Dd @ z
+    np.testing.assert_allclose(res, e, rtol=1e-9, atol=1e-11 * np.abs(r).max())
+
+
 def test_chebyshev_weights_give_the_chebyshev_residual_polynomial():
     """For D = diag(mu) with B = I the residual polynomial of the passes,
     1 - mu q(mu) = prod_j (1 - w_j mu), equals T_m((b+a-2mu)/(b-a)) /
