@@ -8,9 +8,9 @@ factors; persistent Schur PCG; back-substitution; acceptance checks) with the
 Hessian and b already resident in HBM.
 
 Prints ONE JSON line.  `value` = solves/s of the new solver: FP32 bulk /
-FP64 reductions with an FP64 true residual every 25 iterations, and 10
-block-Jacobi passes weighted by the Chebyshev roots of [b/300, b] with b
-from a power-iteration spectrum bound.  The plain FP64 solver (the
+FP64 reductions with an FP64 true residual every 25 iterations, and 16
+block-Jacobi passes weighted by the Chebyshev roots of [b/1000, b] (Leja
+order) with b from a power-iteration spectrum bound.  The plain FP64 solver (the
 reference algorithm: Neumann series of order 5, omega 0.7, FP64) is timed
 in the same run and reported beside it, with the mixed-precision Neumann
 and FP64 Chebyshev variants.
@@ -29,7 +29,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-NEW_PASSES, NEW_RATIO = 10, 300.0  # the new solver's preconditioner: passes, Chebyshev interval b/a
+NEW_PASSES, NEW_RATIO = 16, 1000.0  # the new solver's preconditioner: passes, Chebyshev interval b/a
 METRIC = "HSC solves/s & PCG iters @ rtol 1e-8, 78k-tet system: new vs plain FP64"
 
 
@@ -84,8 +84,8 @@ def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
 
 def run_batch64(args, dev, order, omega, rtol):
     """64 uneven systems (seed 2: 1-16 ABD bodies, 1-8 tori of 2k-15k tets,
-    ranks 4-16), the new solver (FP32 bulk / FP64 reductions, 10 Chebyshev-
-    weighted passes, ratio 300; one interval for the batch), rtol 1e-8 each: one captured
+    ranks 4-16), the new solver (FP32 bulk / FP64 reductions, 16 Chebyshev-
+    weighted passes, ratio 1000; one interval for the batch), rtol 1e-8 each: one captured
     hipGraph replay (right-hand sides, one persistent PCG launch over all 64,
     back-substitution and acceptance) vs the same systems solved one at a
     time.  A step = one solve of all 64 systems (setup excluded: the systems
@@ -269,7 +269,7 @@ def main():
 
     plain = run("fp64")                      # the reference algorithm: Neumann series, FP64
     mixed = run("mixed")                     # + FP32 bulk / FP64 reductions
-    new = run("mixed", "chebyshev")          # + 10 Chebyshev-weighted passes
+    new = run("mixed", "chebyshev")          # + 16 Chebyshev-weighted passes
     cheb64 = run("fp64", "chebyshev")
     info = plain["info"]
     info_new = new["info"]
@@ -331,9 +331,9 @@ def main():
                    "new_passes": NEW_PASSES, "new_interval_ratio": NEW_RATIO, "pass_sync": info_new["pass_sync"],
                    "lanes_per_row": info_new["lanes_per_row"], "wg_per_cu": info_new["workgroups"],
                    "schedule": info_new["schedule"]},
-        "new": "mixed precision (FP32 bulk, FP64 reductions and true residual every 25) + 10 block-Jacobi "
-               "passes weighted by the Chebyshev roots of [b/300, b], b = 1.2 x a power-iteration bound; "
-               "neighbour-tile flags between passes",
+        "new": "mixed precision (FP32 bulk, FP64 reductions and true residual every 25) + 16 block-Jacobi "
+               "passes weighted by the Chebyshev roots of [b/1000, b] in Leja order, b = 1.2 x a power-iteration "
+               "bound; neighbour-tile flags between passes",
         "pcg_iterations": {"new": new["pcg_iterations"], "plain_fp64": plain["pcg_iterations"],
                            "mixed_neumann": mixed["pcg_iterations"], "fp64_chebyshev": cheb64["pcg_iterations"]},
         "plain_fp64": {"solves_per_s": round(plain["solves_per_s"], 3), "ms_per_solve": round(plain["ms_per_solve"], 3),

@@ -764,7 +764,29 @@ void set_poly_weights(hsc_handle* h, double lam2) {
   std::vector<double> mu(m);
   for (int j = 0; j < m; ++j) mu[j] = 0.5 * (b + a) + 0.5 * (b - a) * std::cos((2.0 * j + 1.0) * M_PI / (2.0 * m));
   std::sort(mu.begin(), mu.end());
-  for (int j = 0; j < m; ++j) h->omegas[j] = 1.0 / mu[j];
+  // Leja order: the largest root first, then repeatedly the root maximising
+  // the product of distances to those taken.  Every partial product of the
+  // passes' factors (1 - mu / mu_j), and every remaining one, stays bounded
+  // (about 1e2 at degree 16), where ascending roots grow it to 1e7 and FP32
+  // passes lose all precision beyond degree ~16.
+  std::vector<double> ord;
+  ord.push_back(mu.back());
+  mu.pop_back();
+  while (!mu.empty()) {
+    size_t best = 0;
+    double best_v = -INFINITY;
+    for (size_t i = 0; i < mu.size(); ++i) {
+      double v = 0.0;
+      for (double o : ord) v += std::log(std::fabs(mu[i] - o));
+      if (v > best_v) {
+        best = i;
+        best_v = v;
+      }
+    }
+    ord.push_back(mu[best]);
+    mu.erase(mu.begin() + (long)best);
+  }
+  for (int j = 0; j < m; ++j) h->omegas[j] = 1.0 / ord[j];
   for (int j = m; j < hsc::kMaxPolySteps; ++j) h->omegas[j] = h->omegas[m - 1];
 }
 

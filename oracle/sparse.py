@@ -114,12 +114,33 @@ class SparsePolyJacobi:
         return z
 
 
+def leja_order(points):
+    """Leja order of `points`: the largest first, then repeatedly the point
+    maximising the product of its distances to those already taken (sum of
+    logs, candidates scanned in ascending order, first maximum kept)."""
+    import math
+    rest = sorted(points)
+    out = [rest.pop()]
+    while rest:
+        best, best_v = 0, -math.inf
+        for i, x in enumerate(rest):
+            v = 0.0
+            for o in out:
+                v += math.log(abs(x - o))
+            if v > best_v:
+                best, best_v = i, v
+        out.append(rest.pop(best))
+    return out
+
+
 def chebyshev_weights(lam_est, m, ratio, safety=1.2):
     """Per-pass weights 1/mu_j, mu_j the roots of the degree-m Chebyshev
-    polynomial on [b/ratio, b], b = safety * lam_est, ascending mu."""
+    polynomial on [b/ratio, b], b = safety * lam_est, in Leja order (keeps
+    every partial product of the passes' factors, and every remaining one,
+    bounded, so that rounding in FP32 passes is not amplified)."""
     b = safety * lam_est
     a = b / ratio
-    mu = sorted(0.5 * (b + a) + 0.5 * (b - a) * np.cos((2 * j + 1) * np.pi / (2 * m)) for j in range(m))
+    mu = leja_order([0.5 * (b + a) + 0.5 * (b - a) * np.cos((2 * j + 1) * np.pi / (2 * m)) for j in range(m)])
     return [1.0 / x for x in mu], a, b
 
 

@@ -77,6 +77,30 @@ def test_poly_jacobi_residual_is_the_product_of_its_factors():
     np.testing.assert_allclose(res, e, rtol=1e-9, atol=1e-11 * np.abs(r).max())
 
 
+def test_chebyshev_weights_leja_order_bounds_partial_products():
+    """The weights are the reciprocal Chebyshev roots (a permutation of the
+    ascending set) whose partial products prod_{j<k} (1 - w_j mu) and
+    remainders prod_{j>=k} stay below 1e3 on [0, b] up to degree 32, where
+    ascending roots reach 1e11 (the amplification FP32 passes suffer)."""
+    for m, ratio in ((10, 300.0), (24, 1000.0), (32, 3000.0)):
+        w, a, b = osp.chebyshev_weights(2.9, m, ratio)
+        mu = sorted(0.5 * (b + a) + 0.5 * (b - a) * np.cos((2 * j + 1) * np.pi / (2 * m)) for j in range(m))
+        np.testing.assert_allclose(sorted(1.0 / np.array(w)), mu, rtol=1e-14)
+        assert np.isclose(1.0 / w[0], max(mu), rtol=1e-14)
+        x = np.linspace(0.0, b, 4001)
+
+        def worst(order):
+            pre = suf = 1.0
+            for k in range(m + 1):
+                pre = max(pre, np.abs(np.prod([1 - x / v for v in order[:k]], axis=0)).max())
+                suf = max(suf, np.abs(np.prod([1 - x / v for v in order[k:]], axis=0)).max())
+            return max(pre, suf)
+
+        assert worst([1.0 / v for v in w]) < 1e3
+        if m >= 24:
+            assert worst(mu) > 1e10
+
+
 def test_chebyshev_weights_give_the_chebyshev_residual_polynomial():
     """For D = diag(mu) with B = I the residual polynomial of the passes,
     1 - mu q(mu) = prod_j (1 - w_j mu), equals T_m((b+a-2mu)/(b-a)) /
