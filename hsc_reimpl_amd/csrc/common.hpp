@@ -111,6 +111,7 @@ struct AcaParams {
   double* hist;           // limit + 1
   int* pivots;            // limit x 2
   double* part;           // [nwg * 4]
+  double* cand;           // [2 * nwg * qm] published candidate pivot columns (register-resident ACA)
   int* rank_out;
   double* scale_out;
   unsigned* barrier;

@@ -73,6 +73,10 @@ hipError_t launch_sort_gather(const int* rowptr, int nb, int* key_col, int* key_
                               int* colidx, double* v64, float* v32, hipStream_t st);
 hipError_t aca_occupancy(int* blocks);
 hipError_t launch_aca(const AcaParams& P, hipStream_t st);
+// register-resident ACA: grid size for (qm, a) within max_wg workgroups, 0 when it does not fit
+int aca_reg_grid(int qm, int a, int max_wg);
+hipError_t aca_reg_occupancy(int* blocks);
+hipError_t launch_aca_reg(const AcaParams& P, hipStream_t st);
 hipError_t launch_body_inverse(const double* A, int qm, double* Binv, int* err, hipStream_t st);
 hipError_t launch_abd_inverse(const AbdParams& S, hipStream_t st);
 hipError_t launch_coupling_core(const FactorParams& S, hipStream_t st);

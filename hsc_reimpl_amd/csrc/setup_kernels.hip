@@ -8,6 +8,7 @@
 #include "common.hpp"
 #include "device_util.hpp"
 #include "kernels.hpp"
+#include <algorithm>
 #include <cfloat>
 
 namespace hsc {
@@ -351,6 +352,202 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
     double* tmp = cur;
     cur = nxt;
     nxt = tmp;
+  }
+  if (b == 0 && tid == 0) *P.rank_out = kk;
+}
+
+// ---------------------------------------------------------------- ACA, register-resident residual
+// The same steps and arithmetic as k_aca with the residual held in
+// registers: workgroup b owns the columns [b*cpw, (b+1)*cpw) of R (all qm
+// rows), thread t the entries t, t + kBlock, ... of that block in (row,
+// column) order.  With its argmax partial each workgroup publishes the column
+// of its best entry, so after the one grid barrier of a step every workgroup
+// reads the pivot column from the winner's slot and holds its own share of
+// the pivot row: R is never re-read from memory.  Published columns and
+// partials alternate between two slots, as in k_aca.
+constexpr int kAcaRegPer = 16;     // residual entries per thread
+constexpr int kAcaRegQm = 512;     // ABD DOFs (pivot column staged in LDS)
+constexpr int kAcaRegCols = 1024;  // columns per workgroup (pivot-row share staged in LDS)
+
+int aca_reg_grid(int qm, int a, int max_wg) {
+  if (qm <= 0 || a <= 0 || qm > kAcaRegQm || max_wg <= 0) return 0;
+  const long long total = (long long)qm * a;
+  // about 12 entries per thread: few workgroups keep the barrier and the
+  // reduction over partials short, and the per-step update stays small
+  int nwg = (int)std::min<long long>(max_wg, std::max<long long>(1, (total + 12LL * kBlock - 1) / (12LL * kBlock)));
+  for (; nwg <= max_wg; ++nwg) {
+    const int cpw = (a + nwg - 1) / nwg;
+    if ((long long)qm * cpw <= (long long)kAcaRegPer * kBlock && cpw <= kAcaRegCols) return nwg;
+  }
+  return 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_aca_reg(AcaParams P) {
+  __shared__ double s_m[kBlock / 64], s_q[kBlock / 64];
+  __shared__ long long s_i[kBlock / 64];
+  __shared__ double s_u[kAcaRegQm];
+  __shared__ double s_v[kAcaRegCols];
+  __shared__ double s_scale, s_res, s_piv;
+  __shared__ long long s_idx, s_best;
+  __shared__ int s_go;
+  const int tid = threadIdx.x, b = blockIdx.x;
+  const int qm = P.qm, a = P.a, nwg = P.nwg;
+  const int cpw = (a + nwg - 1) / nwg;
+  const int c0 = min(a, b * cpw), nc = min(a, c0 + cpw) - c0;
+  const int cnt = qm * nc;
+  GridBarrier gbar;
+  if (!gb_init(gbar, P.barrier, P.error, P.timeout_ticks, (unsigned)nwg)) return;
+  int phase = 0;
+  double R[kAcaRegPer];
+  int rr[kAcaRegPer], cc[kAcaRegPer];  // row and local column of each entry (row -1: none)
+#pragma unroll
+  for (int j = 0; j < kAcaRegPer; ++j) {
+    const int le = tid + j * kBlock;
+    rr[j] = -1;
+    cc[j] = 0;
+    R[j] = 0.0;
+    if (le < cnt) {
+      rr[j] = le / nc;
+      cc[j] = le - rr[j] * nc;
+      R[j] = P.E[(size_t)rr[j] * a + c0 + cc[j]];
+    }
+  }
+
+  // argmax |R| (lowest row-major index on ties) and sum of squares of the
+  // owned entries; the partial into slot `phase`, then the column of the
+  // workgroup's best entry into the same slot of the published columns
+  auto reduce_publish = [&]() {
+    double m = -1.0, q = 0.0;
+    long long mi = 0x7fffffffffffffffLL;
+#pragma unroll
+    for (int j = 0; j < kAcaRegPer; ++j) {
+      if (rr[j] >= 0) {
+        argmax_combine(m, mi, fabs(R[j]), (long long)rr[j] * a + c0 + cc[j]);
+        q += R[j] * R[j];
+      }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double m2 = __shfl_xor(m, off, 64);
+      const long long i2 = __shfl_xor(mi, off, 64);
+      argmax_combine(m, mi, m2, i2);
+      q += __shfl_xor(q, off, 64);
+    }
+    const int w = tid >> 6;
+    if ((tid & 63) == 0) {
+      s_m[w] = m;
+      s_i[w] = mi;
+      s_q[w] = q;
+    }
+    __syncthreads();
+    double* part = P.part + (size_t)(phase & 1) * 4 * nwg;
+    if (tid == 0) {
+      double M = s_m[0], Q = s_q[0];
+      long long I = s_i[0];
+      for (int w2 = 1; w2 < kBlock / 64; ++w2) {
+        argmax_combine(M, I, s_m[w2], s_i[w2]);
+        Q += s_q[w2];
+      }
+      part[4 * b + 0] = M;
+      part[4 * b + 1] = (double)I;
+      part[4 * b + 2] = Q;
+      s_best = I;
+    }
+    __syncthreads();
+    const long long I = s_best;
+    if (nc > 0 && I != 0x7fffffffffffffffLL) {
+      const int bc = (int)(I % a) - c0;
+      double* cand = P.cand + ((size_t)(phase & 1) * nwg + b) * qm;
+#pragma unroll
+      for (int j = 0; j < kAcaRegPer; ++j)
+        if (rr[j] >= 0 && cc[j] == bc) cand[rr[j]] = R[j];
+    }
+  };
+  // every workgroup combines all partials the same way (as in k_aca)
+  auto all_reduce = [&]() {
+    const double* part = P.part + (size_t)(phase & 1) * 4 * nwg;
+    ++phase;
+    if (tid < 64) {
+      double M = -1.0, Q = 0.0;
+      long long I = 0x7fffffffffffffffLL;
+      for (int w = tid; w < nwg; w += 64) {
+        argmax_combine(M, I, part[4 * w + 0], (long long)part[4 * w + 1]);
+        Q += part[4 * w + 2];
+      }
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double m2 = __shfl_xor(M, off, 64);
+        const long long i2 = __shfl_xor(I, off, 64);
+        argmax_combine(M, I, m2, i2);
+        Q += __shfl_xor(Q, off, 64);
+      }
+      if (tid == 0) {
+        s_idx = I;
+        s_q[0] = Q;
+      }
+    }
+    __syncthreads();
+  };
+
+  reduce_publish();
+  if (!gb_sync(gbar)) return;
+  int slot = phase & 1;  // slot of the columns just published
+  all_reduce();
+  if (tid == 0) {
+    s_scale = sqrt(s_q[0]);
+    s_res = s_scale > 0.0 ? 1.0 : 0.0;
+    if (b == 0) {
+      P.hist[0] = s_res;
+      *P.scale_out = s_scale;
+    }
+  }
+  __syncthreads();
+  const double scale = s_scale;
+  const double guard = DBL_EPSILON * fmax(scale, DBL_MIN);
+  int kk = 0;
+  for (;;) {
+    const long long I = s_idx;
+    const bool have = I >= 0 && I < (long long)qm * a;
+    const int pi = have ? (int)(I / a) : 0, pj = have ? (int)(I % a) : 0;
+    if (have) {
+      const double* cand = P.cand + ((size_t)slot * nwg + pj / cpw) * qm;  // the pivot column's owner
+      for (int r = tid; r < qm; r += kBlock) s_u[r] = cand[r];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_go = (have && kk < P.limit && s_res > P.stop && isfinite(scale)) ? 1 : 0;
+      if (s_go) {
+        s_piv = s_u[pi];
+        if (!(fabs(s_piv) > guard)) s_go = 0;
+      }
+    }
+    __syncthreads();
+    if (!s_go) break;
+    const double piv = s_piv;
+    // own share of the pivot row, v = R[pi, c] / piv
+#pragma unroll
+    for (int j = 0; j < kAcaRegPer; ++j)
+      if (rr[j] == pi) s_v[cc[j]] = R[j] / piv;
+    __syncthreads();
+    // factor columns
+    if (b == 0)
+      for (int r = tid; r < qm; r += kBlock) P.U[r + (size_t)qm * kk] = s_u[r];
+    for (int cl = tid; cl < nc; cl += kBlock) P.V[c0 + cl + (size_t)a * kk] = s_v[cl];
+    if (b == 0 && tid == 0) {
+      P.pivots[2 * kk] = pi;
+      P.pivots[2 * kk + 1] = pj;
+    }
+#pragma unroll
+    for (int j = 0; j < kAcaRegPer; ++j)
+      if (rr[j] >= 0) R[j] = __dsub_rn(R[j], __dmul_rn(s_u[rr[j]], s_v[cc[j]]));
+    reduce_publish();
+    if (!gb_sync(gbar)) return;
+    slot = phase & 1;
+    all_reduce();
+    ++kk;
+    if (tid == 0) {
+      s_res = sqrt(s_q[0]) / scale;
+      if (b == 0) P.hist[kk] = s_res;
+    }
+    __syncthreads();
   }
   if (b == 0 && tid == 0) *P.rank_out = kk;
 }
@@ -862,6 +1059,16 @@ hipError_t launch_sort_gather(const int* rowptr, int nb, int* key_col, int* key_
 }
 hipError_t aca_occupancy(int* blocks) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_aca, kBlock, 0);
+}
+hipError_t aca_reg_occupancy(int* blocks) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_aca_reg, kBlock, 0);
+}
+hipError_t launch_aca_reg(const AcaParams& P, hipStream_t st) {
+  if (P.qm > kAcaRegQm) return hipErrorInvalidValue;
+  const int cpw = (P.a + P.nwg - 1) / P.nwg;
+  if ((long long)P.qm * cpw > (long long)kAcaRegPer * kBlock || cpw > kAcaRegCols) return hipErrorInvalidValue;
+  void* args[] = {(void*)&P};
+  return hipLaunchCooperativeKernel((const void*)k_aca_reg, dim3(P.nwg), dim3(kBlock), args, 0, st);
 }
 hipError_t launch_aca(const AcaParams& P, hipStream_t st) {
   void* args[] = {(void*)&P};

@@ -158,7 +158,7 @@ struct hsc_handle {
   // buffers
   Buf row_count, rowptr, cursor, active_flag, act_prefix, act_node, act_dof;
   Buf key_col, key_src, colidx, val64, val32, invD64, invD32;
-  Buf A, E, R0, R1, U, V, hist, pivots, acapart, Binv, P, s1, s2, s3, W1, C, Lm, W64, W32;
+  Buf A, E, R0, R1, U, V, hist, pivots, acapart, acacand, Binv, P, s1, s2, s3, W1, C, Lm, W64, W32;
   Buf rhs, r, p0, p1, z0, z1, q, q64, yfull, small, ybuf, misc, ints;
   // persistent-launch scratch
   Buf barrier, errflag, respart, profbuf;  // barrier/errflag: setup kernels (ACA)
@@ -934,15 +934,24 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   double* dmisc = h->misc.as<double>();
   HIPCHK(hipMemsetAsync(h->hist.p, 0, sizeof(double) * (lim1 + 1), st));
   if (limit > 0) {
-    int occ = 0;
-    HIPCHK(hsc::aca_occupancy(&occ));
     const DeviceInfo di = device_info(h->device);
     const long long total = (long long)qm * a;
-    // one workgroup per CU at most: each ACA step is one grid barrier plus a
-    // pass over E, so fewer, fuller workgroups are cheaper
-    int nwg = (int)std::min<long long>(di.cu, std::max<long long>(1, (total + 2047) / 2048));
-    nwg = std::max(1, std::min(nwg, std::max(occ, 1) * di.cu));
+    // register-resident residual when it fits in one workgroup per CU,
+    // else the residual streamed through memory each step
+    int occ_reg = 0;
+    HIPCHK(hsc::aca_reg_occupancy(&occ_reg));
+    const int nwg_reg = occ_reg > 0 ? hsc::aca_reg_grid(qm, a, di.cu) : 0;
+    int nwg = nwg_reg;
+    if (nwg == 0) {
+      int occ = 0;
+      HIPCHK(hsc::aca_occupancy(&occ));
+      // one workgroup per CU at most: each ACA step is one grid barrier plus
+      // a pass over E, so fewer, fuller workgroups are cheaper
+      nwg = (int)std::min<long long>(di.cu, std::max<long long>(1, (total + 2047) / 2048));
+      nwg = std::max(1, std::min(nwg, std::max(occ, 1) * di.cu));
+    }
     HIPCHK(h->acapart.ensure(sizeof(double) * 8 * nwg));  // two halves, see k_aca
+    if (nwg_reg) HIPCHK(h->acacand.ensure(sizeof(double) * 2 * (size_t)nwg * qm));
     HIPCHK(h->barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
     HIPCHK(h->errflag.ensure(64));
     HIPCHK(hipMemsetAsync(h->barrier.p, 0, sizeof(unsigned) * hsc::kBarrierWords, st));
@@ -960,13 +969,14 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     ap.hist = h->hist.as<double>();
     ap.pivots = h->pivots.as<int>();
     ap.part = h->acapart.as<double>();
+    ap.cand = nwg_reg ? h->acacand.as<double>() : nullptr;
     ap.rank_out = rank_dev;
     ap.scale_out = dmisc + 10;
     ap.barrier = h->barrier.as<unsigned>();
     ap.error = h->errflag.as<int>();
     ap.timeout_ticks = kTimeoutTicks;
     ap.nwg = nwg;
-    HIPCHK(hsc::launch_aca(ap, st));
+    HIPCHK(nwg_reg ? hsc::launch_aca_reg(ap, st) : hsc::launch_aca(ap, st));
   }
   // ---- ABD inverse and P
   HIPCHK(hsc::launch_body_inverse(h->A.as<double>(), qm, h->Binv.as<double>(), err + 9, st));
