@@ -92,12 +92,26 @@ __global__ void k_classify(SplitParams S) {
 // Single-workgroup exclusive scan: out[i] = sum_{j<i} in[i], out[n] = total.
 // If mul != 1 the output is scaled (used to turn a node count into W rows).
 __global__ __launch_bounds__(1024) void k_scan(const int* in, int* out, int n) {
+  constexpr int kPer = 32;  // entries per thread held in registers (n <= 32768)
   __shared__ int s[1024];
   const int tid = threadIdx.x;
   const int per = (n + 1023) / 1024;
   const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
+  const bool reg = n > 0 && per <= kPer;
+  int v[kPer];
   int sum = 0;
-  for (int i = b0; i < b1; ++i) sum += in[i];
+  if (reg) {
+    // all loads issued together (indices clamped, masked by a select)
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int x = in[min(b0 + j, n - 1)];
+      v[j] = b0 + j < b1 ? x : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) sum += v[j];
+  } else {
+    for (int i = b0; i < b1; ++i) sum += in[i];
+  }
   s[tid] = sum;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -107,9 +121,17 @@ __global__ __launch_bounds__(1024) void k_scan(const int* in, int* out, int n) {
     __syncthreads();
   }
   int run = tid ? s[tid - 1] : 0;
-  for (int i = b0; i < b1; ++i) {
-    out[i] = run;
-    run += in[i];
+  if (reg) {
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (b0 + j < b1) out[b0 + j] = run;
+      run += v[j];
+    }
+  } else {
+    for (int i = b0; i < b1; ++i) {
+      out[i] = run;
+      run += in[i];
+    }
   }
   if (tid == 1023) out[n] = s[1023];
 }
@@ -163,40 +185,42 @@ __global__ void k_place(PlaceParams S) {
   }
 }
 
-// Sort each BSR row by (column, source) -- deterministic slot order.
-__global__ void k_sort_rows(const int* rowptr, int nb, int* key_col, int* key_src) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  const int p0 = rowptr[i], p1 = rowptr[i + 1];
-  for (int p = p0 + 1; p < p1; ++p) {
-    const int kc = key_col[p], ks = key_src[p];
-    int q = p - 1;
-    while (q >= p0 && (key_col[q] > kc || (key_col[q] == kc && key_src[q] > ks))) {
-      key_col[q + 1] = key_col[q];
-      key_src[q + 1] = key_src[q];
-      --q;
+// Sort each BSR row by (column, source) -- a deterministic slot order -- and
+// gather its blocks into SoA (component c = r + 3*col of the 3x3), FP64 and
+// FP32.  One wave per row: each lane ranks its entry against every entry of
+// the row (keys exchanged by shuffles; (column, source) is unique within a
+// row) and writes it to its sorted slot.
+__global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ rowptr, int nb,
+                                                     const int* __restrict__ key_col, const int* __restrict__ key_src,
+                                                     const double* __restrict__ vals, int nnzb, int* colidx,
+                                                     double* v64, float* v32) {
+  const int row = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (row >= nb) return;  // (uniform over the wave)
+  const int p0 = rowptr[row], len = rowptr[row + 1] - p0;
+  auto key = [&](int e) { return ((long long)key_col[p0 + e] << 32) | (long long)(unsigned)key_src[p0 + e]; };
+  for (int b0 = 0; b0 < len; b0 += 64) {
+    const int e = b0 + lane;
+    const long long mine = e < len ? key(e) : 0x7fffffffffffffffLL;
+    int rank = 0;
+    for (int c0 = 0; c0 < len; c0 += 64) {
+      const long long kj = c0 + lane < len ? key(c0 + lane) : 0x7fffffffffffffffLL;
+      const int cn = min(64, len - c0);
+      for (int j = 0; j < cn; ++j) rank += __shfl(kj, j, 64) < mine;
     }
-    key_col[q + 1] = kc;
-    key_src[q + 1] = ks;
+    if (e < len) {
+      const int dst = p0 + rank;
+      const int src = (int)(mine & 0xffffffffLL);
+      const double* v = vals + 9 * (size_t)(src >> 1);
+      const bool tr = src & 1;
+      colidx[dst] = (int)(mine >> 32);
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+          const double bv = tr ? v[c + 3 * r] : v[r + 3 * c];
+          v64[(size_t)(r + 3 * c) * nnzb + dst] = bv;
+          v32[(size_t)(r + 3 * c) * nnzb + dst] = (float)bv;
+        }
+    }
   }
-}
-
-// Gather block values into SoA (component c = r + 3*col of the 3x3) FP64/FP32.
-__global__ void k_gather(const double* vals, const int* key_col, const int* key_src, int nnzb,
-                         int* colidx, double* v64, float* v32) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nnzb) return;
-  const int src = key_src[p];
-  const int t = src >> 1;
-  const bool tr = src & 1;
-  const double* v = vals + 9 * (size_t)t;
-  colidx[p] = key_col[p];
-  for (int c = 0; c < 3; ++c)
-    for (int r = 0; r < 3; ++r) {
-      const double b = tr ? v[c + 3 * r] : v[r + 3 * c];
-      v64[(size_t)(r + 3 * c) * nnzb + p] = b;
-      v32[(size_t)(r + 3 * c) * nnzb + p] = (float)b;
-    }
 }
 
 // ---------------------------------------------------------------- ACA
@@ -843,14 +867,45 @@ __global__ void k_make_W(const double* V, const double* Lm, int a, int k, double
   if (e >= (long long)a * k) return;
   const int j = (int)(e / k), l = (int)(e % k);
   double s = 0.0;
+#pragma unroll 8
   for (int m = l; m < k; ++m) s += V[j + (size_t)a * m] * Lm[(size_t)m * k + l];
   W64[e] = s;
   W32[e] = (float)s;
 }
 
+// y[0:rows) = M x for a row-major M (ld = row stride) or, with cols_major,
+// M[i][m] = Mt[i * ld + m] read as columns of a column-major matrix (the same
+// addressing): one wave per row, lanes over the inner index (coalesced), RB
+// rows per wave in flight together; lane-strided partial sums and a
+// butterfly, so the order is fixed.  Called by a whole workgroup.
+template <int RB>
+__device__ __forceinline__ void wave_matvec(const double* __restrict__ M, int ld, int rows, int inner,
+                                            const double* x, double* y) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int i0 = wave * RB; i0 < rows; i0 += nw * RB) {
+    double acc[RB];
+    const double* Mr[RB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      acc[u] = 0.0;
+      Mr[u] = M + (size_t)min(i0 + u, rows - 1) * ld;  // (rows past the end: loaded, not stored)
+    }
+    for (int m = lane; m < inner; m += 64) {
+      const double xm = x[m];
+#pragma unroll
+      for (int u = 0; u < RB; ++u) acc[u] += Mr[u][m] * xm;
+    }
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      const double t = wave_sum(acc[u]);
+      if (lane == 0 && i0 + u < rows) y[i0 + u] = t;
+    }
+  }
+}
+
 // y = L^-1 (U^T P b_q)  (reduced right-hand side, core.py:163-164;
 // linear_hsc.cu:351-357 with V c = W L^-1 c).  One workgroup: b_q, P b_q and
-// U^T P b_q in LDS, one thread per output entry (inner index ascending); the
+// U^T P b_q in LDS (matrix-vector products one wave per output entry); the
 // forward substitution is column-oriented in one wave (lane j owns rows j,
 // j+64, ...; after y_i is final every later row subtracts L[j][i] y_i), with
 // L's lower triangle in LDS when it fits (k <= 64).
@@ -870,19 +925,9 @@ __global__ __launch_bounds__(512) void k_rhs_small(const double* P, const double
   if (lds_L)
     for (int e = threadIdx.x; e < k * k; e += blockDim.x) s_L[e] = Lm[e];
   __syncthreads();
-  for (int i = threadIdx.x; i < qm; i += blockDim.x) {  // P b_q
-    const double* row = P + (size_t)i * qm;
-    double s = 0.0;
-    for (int m = 0; m < qm; ++m) s += row[m] * bq[m];
-    pb[i] = s;
-  }
+  wave_matvec<6>(P, qm, qm, qm, bq, pb);  // P b_q
   __syncthreads();
-  for (int l = threadIdx.x; l < k; l += blockDim.x) {  // c = U^T P b_q
-    const double* col = U + (size_t)qm * l;
-    double s = 0.0;
-    for (int i = 0; i < qm; ++i) s += col[i] * pb[i];
-    s_c[l] = s;
-  }
+  wave_matvec<6>(U, qm, k, qm, pb, s_c);  // c = U^T P b_q (U column-major: column l at U + qm l)
   __syncthreads();
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
@@ -927,24 +972,24 @@ __global__ __launch_bounds__(256) void k_backsub_vt(const double* V, const int* 
   __shared__ double red[256 / 64];
   const int l = blockIdx.x;
   double s = 0.0;
+#pragma unroll 4
   for (int j = threadIdx.x; j < a; j += blockDim.x) s += V[j + (size_t)a * l] * x[fo + act_dof[j]];
   const double tot = block_sum_t0(s, red);
   if (threadIdx.x == 0) vt[l] = tot;
 }
 __global__ __launch_bounds__(512) void k_backsub(const double* P, const double* U, const double* b, double* x,
                                                  int qo, int qm, int k, double* tmp) {
+  __shared__ double s_vt[kMaxRankLds];
+  for (int l = threadIdx.x; l < k; l += blockDim.x) s_vt[l] = tmp[l];
+  __syncthreads();
   for (int i = threadIdx.x; i < qm; i += blockDim.x) {
     double s = b[qo + i];
-    for (int l = 0; l < k; ++l) s -= U[i + (size_t)qm * l] * tmp[l];
+#pragma unroll 8
+    for (int l = 0; l < k; ++l) s -= U[i + (size_t)qm * l] * s_vt[l];
     tmp[k + i] = s;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < qm; i += blockDim.x) {  // one thread per row, inner index ascending
-    const double* row = P + (size_t)i * qm;
-    double s = 0.0;
-    for (int m = 0; m < qm; ++m) s += row[m] * tmp[k + m];
-    x[qo + i] = s;
-  }
+  wave_matvec<6>(P, qm, qm, qm, tmp + k, x + qo);  // P (b_q - U V^T dx)
 }
 
 // full H x from the split parts: y_fem = D x_f (precomputed in yf) + E^T x_q,
@@ -959,6 +1004,7 @@ __global__ void k_full_residual_fem(const double* yf, const double* E, const int
     const int an = act_node[i / 3];
     if (an >= 0) {
       const int col = an + i % 3;
+#pragma unroll 8
       for (int q = 0; q < qm; ++q) hx += E[(size_t)q * a + col] * x[qo + q];
     }
     const double bi = b[fo + i];
@@ -1051,10 +1097,9 @@ hipError_t launch_place(const PlaceParams& S, hipStream_t st) {
 }
 hipError_t launch_sort_gather(const int* rowptr, int nb, int* key_col, int* key_src, const double* vals, int nnzb,
                               int* colidx, double* v64, float* v32, hipStream_t st) {
-  if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sort_rows, dim3(cdiv(nb, 128)), dim3(128), 0, st, rowptr, nb, key_col, key_src);
-  if (nnzb > 0)
-    hipLaunchKernelGGL(k_gather, dim3(cdiv(nnzb, 256)), dim3(256), 0, st, vals, key_col, key_src, nnzb, colidx, v64, v32);
+  if (nb == 0 || nnzb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_gather, dim3(cdiv((long long)nb * 64, 256)), dim3(256), 0, st, rowptr, nb, key_col,
+                     key_src, vals, nnzb, colidx, v64, v32);
   return hipGetLastError();
 }
 hipError_t aca_occupancy(int* blocks) {
@@ -1063,12 +1108,17 @@ hipError_t aca_occupancy(int* blocks) {
 hipError_t aca_reg_occupancy(int* blocks) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_aca_reg, kBlock, 0);
 }
+// Launched as an ordinary kernel: the caller keeps the grid within the
+// co-resident bound (one workgroup per CU at most), so every workgroup gets a
+// CU once whatever else runs has drained, and every grid wait is bounded.  A
+// cooperative launch would first wait for the setup's D-side work on the
+// other stream, which this launch overlaps.
 hipError_t launch_aca_reg(const AcaParams& P, hipStream_t st) {
   if (P.qm > kAcaRegQm) return hipErrorInvalidValue;
   const int cpw = (P.a + P.nwg - 1) / P.nwg;
   if ((long long)P.qm * cpw > (long long)kAcaRegPer * kBlock || cpw > kAcaRegCols) return hipErrorInvalidValue;
-  void* args[] = {(void*)&P};
-  return hipLaunchCooperativeKernel((const void*)k_aca_reg, dim3(P.nwg), dim3(kBlock), args, 0, st);
+  hipLaunchKernelGGL(k_aca_reg, dim3(P.nwg), dim3(kBlock), 0, st, P);
+  return hipGetLastError();
 }
 hipError_t launch_aca(const AcaParams& P, hipStream_t st) {
   void* args[] = {(void*)&P};

@@ -920,15 +920,11 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                       h->rowptr.as<int>(), h->cursor.as<int>(), h->key_col.as<int>(), h->key_src.as<int>(),
                       h->act_node.as<int>(), a, h->E.as<double>()};
   HIPCHK(hsc::launch_place(pp, st));
-  // D side on the second stream, beside ACA and the ABD inverse: the row
-  // sort, the value gather and (Chebyshev) the power iteration
+  // the D side (row sort, value gather, power iteration) depends on nothing
+  // after this point; it is submitted to the second stream once ACA and the
+  // ABD inverse are queued on this one, so that it runs beside them
   HIPCHK(hipEventRecord(h->aux_ev, st));
-  HIPCHK(hipStreamWaitEvent(h->aux, h->aux_ev, 0));
-  AuxJoin join{h->aux};
-  HIPCHK(hsc::launch_sort_gather(h->rowptr.as<int>(), nb, h->key_col.as<int>(), h->key_src.as<int>(), Av->values,
-                                 nnzb, h->colidx.as<int>(), h->val64.as<double>(), h->val32.as<float>(), h->aux));
-  ss->lam2 = 0.0;
-  if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
+  AuxJoin join{nullptr};
   // ---- ACA
   int* rank_dev = err + 8;
   double* dmisc = h->misc.as<double>();
@@ -985,6 +981,12 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                      h->cfg.neumann_order, h->cfg.omega, h->cfg.abd_inverse_tol, dmisc};
   if (h->cfg.abd_inverse == HSC_ABD_NEUMANN && h->cfg.adaptive_abd_inverse) abp.mode = 2;
   HIPCHK(hsc::launch_abd_inverse(abp, st));
+  HIPCHK(hipStreamWaitEvent(h->aux, h->aux_ev, 0));
+  join.s = h->aux;
+  HIPCHK(hsc::launch_sort_gather(h->rowptr.as<int>(), nb, h->key_col.as<int>(), h->key_src.as<int>(), Av->values,
+                                 nnzb, h->colidx.as<int>(), h->val64.as<double>(), h->val32.as<float>(), h->aux));
+  ss->lam2 = 0.0;
+  if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
   ss->aca_err = 0;
   HIPCHK(hipMemcpyAsync(ss->ints2, err + 8, sizeof(ss->ints2), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&ss->max_coupled, err + 12, sizeof(int), hipMemcpyDeviceToHost, st));
