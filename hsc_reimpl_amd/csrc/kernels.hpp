@@ -98,6 +98,14 @@ hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks)
 hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st);
 const void* pcg_kernel_fn(int G, bool mixed, bool res);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);  // bytes % 4 == 0; a kernel, safe in graphs
+// up to kZeroListMax buffers zeroed by one kernel launch (each bytes % 4 == 0)
+constexpr int kZeroListMax = 12;
+struct ZeroList {
+  void* p[kZeroListMax];
+  long long start[kZeroListMax + 1];
+  int n;
+};
+hipError_t launch_zero_multi(const void* const* ptrs, const size_t* bytes, int n, hipStream_t st);
 hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st);
 hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double* omegas, const double* r, double* z,
                                double* tmp, hipStream_t st);

@@ -1390,6 +1390,35 @@ __global__ void k_zero_u32(unsigned* p, long long n) {
     p[i] = 0u;
 }
 
+// several buffers zeroed by one launch: segment s covers 32-bit words
+// [start[s], start[s+1]) of the concatenation
+__global__ void k_zero_multi(ZeroList L) {
+  const long long total = L.start[L.n];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    int s = 0;
+    while (i >= L.start[s + 1]) ++s;
+    static_cast<unsigned*>(L.p[s])[i - L.start[s]] = 0u;
+  }
+}
+
+hipError_t launch_zero_multi(const void* const* ptrs, const size_t* bytes, int n, hipStream_t st) {
+  if (n < 0 || n > kZeroListMax) return hipErrorInvalidValue;
+  ZeroList L{};
+  L.n = n;
+  L.start[0] = 0;
+  for (int s = 0; s < n; ++s) {
+    if (bytes[s] % 4 != 0) return hipErrorInvalidValue;
+    L.p[s] = const_cast<void*>(ptrs[s]);
+    L.start[s + 1] = L.start[s] + (long long)(bytes[s] / 4);
+  }
+  const long long total = L.start[n];
+  if (total == 0) return hipSuccess;
+  const int grid = (int)std::min<long long>((total + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_zero_multi, dim3(grid), dim3(256), 0, st, L);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
   if (bytes % 4 != 0) return hipErrorInvalidValue;
   const long long n = (long long)(bytes / 4);

@@ -873,11 +873,11 @@ __global__ void k_make_W(const double* V, const double* Lm, int a, int k, double
   W32[e] = (float)s;
 }
 
-// y[0:rows) = M x for a row-major M (ld = row stride) or, with cols_major,
-// M[i][m] = Mt[i * ld + m] read as columns of a column-major matrix (the same
-// addressing): one wave per row, lanes over the inner index (coalesced), RB
-// rows per wave in flight together; lane-strided partial sums and a
-// butterfly, so the order is fixed.  Called by a whole workgroup.
+// y[0:rows) = M x with M[i][m] at M[i * ld + m] (a row-major matrix, or the
+// transpose of a column-major one): one wave per row, lanes over the inner
+// index (coalesced), the loads of RB rows in flight together; lane-strided
+// partial sums and a butterfly, so the order is fixed.  Called by a whole
+// workgroup.
 template <int RB>
 __device__ __forceinline__ void wave_matvec(const double* __restrict__ M, int ld, int rows, int inner,
                                             const double* x, double* y) {

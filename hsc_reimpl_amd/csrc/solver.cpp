@@ -524,11 +524,21 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
 // (multiplied by beta = 0 in the first iteration, so it must be finite).
 // Fill kernels, not memsets, so that the sequence can be captured in a graph.
 void reset_pcg(hsc_handle* const* hs, int nsys, PcgScratch& sc, hipStream_t st) {
-  HIPCHK(hsc::launch_zero(sc.barrier.p, sizeof(unsigned) * hsc::kBarrierWords, st));
-  HIPCHK(hsc::launch_zero(sc.errflag.p, 64, st));
-  for (int s = 0; s < nsys; ++s)
-    HIPCHK(hsc::launch_zero(hs[s]->p0.p, sizeof(double) * std::max(hs[s]->fn, 1), st));
-  if (sc.nflags > 0) HIPCHK(hsc::launch_zero(sc.flags.p, sizeof(unsigned) * sc.nflags, st));
+  // one launch per kZeroListMax buffers
+  std::vector<const void*> zp = {sc.barrier.p, sc.errflag.p};
+  std::vector<size_t> zb = {sizeof(unsigned) * hsc::kBarrierWords, 64};
+  if (sc.nflags > 0) {
+    zp.push_back(sc.flags.p);
+    zb.push_back(sizeof(unsigned) * sc.nflags);
+  }
+  for (int s = 0; s < nsys; ++s) {
+    zp.push_back(hs[s]->p0.p);
+    zb.push_back(sizeof(double) * std::max(hs[s]->fn, 1));
+  }
+  for (size_t o = 0; o < zp.size(); o += hsc::kZeroListMax) {
+    const int cnt = (int)std::min<size_t>(hsc::kZeroListMax, zp.size() - o);
+    HIPCHK(hsc::launch_zero_multi(zp.data() + o, zb.data() + o, cnt, st));
+  }
 }
 
 // Copy the per-system outputs back (pinned) and wait; raises on a device
@@ -843,13 +853,12 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->ints.ensure(sizeof(int) * 32));
   HIPCHK(h->stage.ensure(sizeof(SetupStage)));
   SetupStage* ss = h->stage.as<SetupStage>();
-  HIPCHK(hipMemsetAsync(h->row_count.p, 0, sizeof(int) * nb, st));
-  HIPCHK(hipMemsetAsync(h->cursor.p, 0, sizeof(int) * nb, st));
-  HIPCHK(hipMemsetAsync(h->active_flag.p, 0, sizeof(int) * nb, st));
-  HIPCHK(hipMemsetAsync(h->invD64.p, 0, sizeof(double) * 9 * nb, st));
-  HIPCHK(hipMemsetAsync(h->invD32.p, 0, sizeof(float) * 9 * nb, st));
-  HIPCHK(hipMemsetAsync(h->A.p, 0, sizeof(double) * qm * qm, st));
-  HIPCHK(hipMemsetAsync(h->ints.p, 0, sizeof(int) * 32, st));
+  {
+    const void* zp[] = {h->row_count.p, h->cursor.p, h->active_flag.p, h->invD64.p, h->invD32.p, h->A.p, h->ints.p};
+    const size_t zb[] = {sizeof(int) * nb, sizeof(int) * nb, sizeof(int) * nb, sizeof(double) * 9 * nb,
+                         sizeof(float) * 9 * nb, sizeof(double) * qm * qm, sizeof(int) * 32};
+    HIPCHK(hsc::launch_zero_multi(zp, zb, 7, st));
+  }
   int* err = h->ints.as<int>();
   hsc::SplitParams sp{Av->row_indices, Av->col_indices, Av->values, Av->triplet_count, qo, qm, fo, fn,
                       h->row_count.as<int>(), h->active_flag.as<int>(), h->invD64.as<double>(),
@@ -910,7 +919,15 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     HIPCHK(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&h->aux_ev, hipEventDisableTiming));
   }
-  if (a > 0) HIPCHK(hipMemsetAsync(h->E.p, 0, sizeof(double) * (size_t)qm * a, st));
+  HIPCHK(h->barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
+  HIPCHK(h->errflag.ensure(64));
+  {
+    // E (filled by k_place), the ACA history, grid-barrier counters and error flag
+    const void* zp[] = {h->hist.p, h->barrier.p, h->errflag.p, h->E.p};
+    const size_t zb[] = {sizeof(double) * (lim1 + 1), sizeof(unsigned) * hsc::kBarrierWords, 64,
+                         sizeof(double) * (size_t)qm * a};
+    HIPCHK(hsc::launch_zero_multi(zp, zb, a > 0 ? 4 : 3, st));
+  }
   HIPCHK(hsc::launch_active(h->active_flag.as<int>(), h->act_prefix.as<int>(), nb, h->act_node.as<int>(),
                             h->act_dof.as<int>(), st));
   // coupled nodes per 128-node tile (err[12]): sizes the LDS cache of W rows
@@ -928,7 +945,6 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   // ---- ACA
   int* rank_dev = err + 8;
   double* dmisc = h->misc.as<double>();
-  HIPCHK(hipMemsetAsync(h->hist.p, 0, sizeof(double) * (lim1 + 1), st));
   if (limit > 0) {
     const DeviceInfo di = device_info(h->device);
     const long long total = (long long)qm * a;
@@ -948,10 +964,6 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     }
     HIPCHK(h->acapart.ensure(sizeof(double) * 8 * nwg));  // two halves, see k_aca
     if (nwg_reg) HIPCHK(h->acacand.ensure(sizeof(double) * 2 * (size_t)nwg * qm));
-    HIPCHK(h->barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
-    HIPCHK(h->errflag.ensure(64));
-    HIPCHK(hipMemsetAsync(h->barrier.p, 0, sizeof(unsigned) * hsc::kBarrierWords, st));
-    HIPCHK(hipMemsetAsync(h->errflag.p, 0, 64, st));
     hsc::AcaParams ap{};
     ap.E = h->E.as<double>();
     ap.R0 = h->R0.as<double>();
