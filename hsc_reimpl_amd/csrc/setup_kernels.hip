@@ -396,8 +396,8 @@ constexpr int kAcaRegCols = 1024;  // columns per workgroup (pivot-row share sta
 int aca_reg_grid(int qm, int a, int max_wg) {
   if (qm <= 0 || a <= 0 || qm > kAcaRegQm || max_wg <= 0) return 0;
   const long long total = (long long)qm * a;
-  // about 12 entries per thread: few workgroups keep the barrier and the
-  // reduction over partials short, and the per-step update stays small
+  // about 12 entries per thread (measured: 12, 16 and 20 per thread take the
+  // same time at 144 x 6,300; the step is bound by the grid barrier)
   int nwg = (int)std::min<long long>(max_wg, std::max<long long>(1, (total + 12LL * kBlock - 1) / (12LL * kBlock)));
   for (; nwg <= max_wg; ++nwg) {
     const int cpw = (a + nwg - 1) / nwg;
@@ -423,19 +423,20 @@ __global__ __launch_bounds__(kBlock) void k_aca_reg(AcaParams P) {
   if (!gb_init(gbar, P.barrier, P.error, P.timeout_ticks, (unsigned)nwg)) return;
   int phase = 0;
   double R[kAcaRegPer];
-  int rr[kAcaRegPer], cc[kAcaRegPer];  // row and local column of each entry (row -1: none)
+  int rc[kAcaRegPer];  // (row << 10) | local column of each entry, -1: none
 #pragma unroll
   for (int j = 0; j < kAcaRegPer; ++j) {
     const int le = tid + j * kBlock;
-    rr[j] = -1;
-    cc[j] = 0;
+    rc[j] = -1;
     R[j] = 0.0;
     if (le < cnt) {
-      rr[j] = le / nc;
-      cc[j] = le - rr[j] * nc;
-      R[j] = P.E[(size_t)rr[j] * a + c0 + cc[j]];
+      const int r = le / nc, c = le - r * nc;
+      rc[j] = (r << 10) | c;
+      R[j] = P.E[(size_t)r * a + c0 + c];
     }
   }
+  auto rr = [&](int j) { return rc[j] >> 10; };  // (-1 for no entry)
+  auto cc = [&](int j) { return rc[j] & 1023; };
 
   // argmax |R| (lowest row-major index on ties) and sum of squares of the
   // owned entries; the partial into slot `phase`, then the column of the
@@ -445,8 +446,8 @@ __global__ __launch_bounds__(kBlock) void k_aca_reg(AcaParams P) {
     long long mi = 0x7fffffffffffffffLL;
 #pragma unroll
     for (int j = 0; j < kAcaRegPer; ++j) {
-      if (rr[j] >= 0) {
-        argmax_combine(m, mi, fabs(R[j]), (long long)rr[j] * a + c0 + cc[j]);
+      if (rc[j] >= 0) {
+        argmax_combine(m, mi, fabs(R[j]), (long long)rr(j) * a + c0 + cc(j));
         q += R[j] * R[j];
       }
     }
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_aca_reg(AcaParams P) {
       double* cand = P.cand + ((size_t)(phase & 1) * nwg + b) * qm;
 #pragma unroll
       for (int j = 0; j < kAcaRegPer; ++j)
-        if (rr[j] >= 0 && cc[j] == bc) cand[rr[j]] = R[j];
+        if (rc[j] >= 0 && cc(j) == bc) cand[rr(j)] = R[j];
     }
   };
   // every workgroup combines all partials the same way (as in k_aca)
@@ -549,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_aca_reg(AcaParams P) {
     // own share of the pivot row, v = R[pi, c] / piv
 #pragma unroll
     for (int j = 0; j < kAcaRegPer; ++j)
-      if (rr[j] == pi) s_v[cc[j]] = R[j] / piv;
+      if (rr(j) == pi) s_v[cc(j)] = R[j] / piv;
     __syncthreads();
     // factor columns
     if (b == 0)
@@ -561,7 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_aca_reg(AcaParams P) {
     }
 #pragma unroll
     for (int j = 0; j < kAcaRegPer; ++j)
-      if (rr[j] >= 0) R[j] = __dsub_rn(R[j], __dmul_rn(s_u[rr[j]], s_v[cc[j]]));
+      if (rc[j] >= 0) R[j] = __dsub_rn(R[j], __dmul_rn(s_u[rr(j)], s_v[cc(j)]));
     reduce_publish();
     if (!gb_sync(gbar)) return;
     slot = phase & 1;
