@@ -1196,7 +1196,7 @@ out:
 // (plain launches: one tile per workgroup; used by the parity entry points)
 template <int G>
 __global__ __launch_bounds__(kBlock) void op_Dx64(SysDesc S, const double* x, double* y) {
-  tile_Dx64<G>(S, blockIdx.x, x, y);
+  tile_Dx64<G, StreamU<G, false>::value>(S, blockIdx.x, x, y);
 }
 
 // One power-iteration step for lambda_max(Dinv D), fused: z = Dinv D (x s)
@@ -1240,7 +1240,7 @@ __global__ __launch_bounds__(kBlock) void op_power_step(SysDesc S, const double*
 template <int G, typename VT>
 __global__ __launch_bounds__(kBlock) void op_neumann_step(SysDesc S, const VT* zin, VT* zout, double omega) {
   double rz = 0.0;
-  tile_N<G, VT>(S, blockIdx.x, zin, zout, (VT)omega, false, rz);
+  tile_N<G, VT, StreamU<G, false>::value>(S, blockIdx.x, zin, zout, (VT)omega, false, rz);
 }
 
 template <typename VT>
@@ -1488,6 +1488,31 @@ hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, h
     case 16: hipLaunchKernelGGL(op_Dx64<16>, dim3(tiles), dim3(kBlock), 0, st, S, x, y); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// one launch of a standalone operator kernel (bandwidth measurements):
+// 0 = FP64 y = D x, 1 = FP64 pass y = x + w Dinv (r - D x), 2 = the pass in FP32
+hipError_t launch_timed_op(int kernel, int G, const SysDesc& S, const void* x, void* y, hipStream_t st) {
+  const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
+  if (tiles == 0) return hipSuccess;
+#define HSC_TIMED(GG)                                                                                          \
+  if (kernel == 0)                                                                                             \
+    hipLaunchKernelGGL(op_Dx64<GG>, dim3(tiles), dim3(kBlock), 0, st, S, (const double*)x, (double*)y);        \
+  else if (kernel == 1)                                                                                        \
+    hipLaunchKernelGGL((op_neumann_step<GG, double>), dim3(tiles), dim3(kBlock), 0, st, S, (const double*)x,   \
+                       (double*)y, 0.5);                                                                       \
+  else                                                                                                         \
+    hipLaunchKernelGGL((op_neumann_step<GG, float>), dim3(tiles), dim3(kBlock), 0, st, S, (const float*)x,     \
+                       (float*)y, 0.5);
+  if (kernel < 0 || kernel > 2) return hipErrorInvalidValue;
+  switch (G) {
+    case 4: HSC_TIMED(4) break;
+    case 8: HSC_TIMED(8) break;
+    case 16: HSC_TIMED(16) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef HSC_TIMED
   return hipGetLastError();
 }
 

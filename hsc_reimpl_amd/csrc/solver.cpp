@@ -1771,6 +1771,42 @@ int32_t hsc_apply_precond(hsc_handle* h, const double* r, double* z, hsc_stream 
   });
 }
 
+int32_t hsc_time_kernel(hsc_handle* h, int32_t kernel, int32_t lanes_per_row, int32_t repeats, hsc_stream stream,
+                        double* ms_per_launch) {
+  if (!h || !ms_per_launch || kernel < 0 || kernel > 2 || repeats < 1 ||
+      !(lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16))
+    return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    hipStream_t st = (hipStream_t)stream;
+    hsc::SysDesc S = make_desc(h);
+    // the handle's own vectors as operands (zeroed; the arithmetic does not
+    // depend on the values)
+    const size_t vb = sizeof(double) * (size_t)h->fn;
+    const void* zp[] = {h->p0.p, h->p1.p, h->r.p};
+    const size_t zb[] = {vb, vb, vb};
+    HIPCHK(hsc::launch_zero_multi(zp, zb, 3, st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipError_t le = hipSuccess;
+    (void)hipEventRecord(e0, st);
+    for (int i = 0; i < repeats && le == hipSuccess; ++i)
+      le = hsc::launch_timed_op(kernel, lanes_per_row, S, h->p0.p, h->p1.p, st);
+    (void)hipEventRecord(e1, st);
+    hipError_t se = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (le == hipSuccess && se == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(le);
+    HIPCHK(se);
+    *ms_per_launch = ms / repeats;
+    return HSC_OK;
+  });
+}
+
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   if (!h || !o) return HSC_ERR_INVALID_ARGUMENT;
   std::memset(o, 0, sizeof(*o));

@@ -198,6 +198,17 @@ class Handle:
         self._check(self.lib.hsc_get_setup_info(self.h, C.byref(si)))
         return si.as_dict()
 
+    TIMED_KERNELS = {"spmv64": 0, "pass64": 1, "pass32": 2}
+
+    def time_kernel(self, kernel: str, lanes_per_row: int = 8, repeats: int = 50, stream=None) -> float:
+        """ms per launch of one standalone operator kernel on this handle's D
+        (hsc_time_kernel): "spmv64" (y = D x), "pass64" / "pass32" (one
+        block-Jacobi pass in FP64 / FP32)."""
+        ms = C.c_double()
+        self._check(self.lib.hsc_time_kernel(self.h, self.TIMED_KERNELS[kernel], lanes_per_row, repeats,
+                                             _stream(stream), C.byref(ms)))
+        return ms.value
+
     def factors(self) -> dict:
         import numpy as np
         info = self.setup_info()

@@ -341,6 +341,30 @@ def torus_system(seed=1, n_tori=7, n_bodies=12, youngs=(1e10,) * 4 + (1e8,) * 3,
                    bodies, couplings, rng, info)
 
 
+def replicate_fem(sysm, copies, seed=0):
+    """`copies` block-diagonal copies of a generated system's FEM part, its
+    ABD part kept once and coupled to the first copy only: a larger system
+    with the same per-row structure (bandwidth measurements with the working
+    set beyond the 256 MB Infinity Cache).  Needs the generator's layout
+    (FEM blocks first, then the ABD block rows)."""
+    if sysm.fo != 0 or sysm.qo != sysm.fn:
+        raise ValueError("expects FEM blocks first, then ABD")
+    nb = sysm.fn // 3
+    r, c, v = sysm.rows.astype(np.int64), sysm.cols.astype(np.int64), sysm.vals
+    ff = c < nb                      # FEM-FEM (rows <= cols)
+    shift = (copies - 1) * nb        # ABD block rows move past the copies
+    rows = [r[ff] + k * nb for k in range(copies)] + [r[~ff] + np.where(r[~ff] >= nb, shift, 0)]
+    cols = [c[ff] + k * nb for k in range(copies)] + [c[~ff] + shift]
+    vals = [v[ff]] * copies + [v[~ff]]
+    R = np.concatenate(rows).astype(np.int32)
+    C = np.concatenate(cols).astype(np.int32)
+    V = np.ascontiguousarray(np.concatenate(vals))
+    fn = copies * sysm.fn
+    b = np.random.default_rng(seed).normal(size=fn + sysm.qm)
+    info = dict(sysm.info, copies=copies, nb=copies * nb, n=fn, N=fn + sysm.qm, bcoo_nnz=int(len(R)))
+    return HSCSystem(R, C, V, b, fo=0, fn=fn, qo=fn, qm=sysm.qm, info=info)
+
+
 def batch_systems(seed=2, count=64, bodies=(1, 16), tori=(1, 8), tets=(2000, 15000),
                   ranks=(4, 16), youngs=(1e8, 1e10), progress=None):
     """Uneven batch: per system U{1..16} bodies, U{1..8} tori of

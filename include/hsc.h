@@ -195,6 +195,17 @@ typedef struct hsc_setup_info {
   double lambda_max;             /* power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights; else 0) */
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
+
+/* ---- measurement (no reference counterpart) ----
+ * Times `repeats` back-to-back launches of one standalone operator kernel on
+ * the set-up handle's D, between HIP events on `stream`: kernel 0 = FP64
+ * y = D x, 1 = one FP64 block-Jacobi pass y = x + w Dinv (r - D x), 2 = the
+ * same pass with FP32 values, inverse blocks and vectors (FP64 r), with
+ * `lanes_per_row` (4, 8, 16) lanes per block row.  The operands are the
+ * handle's own scratch vectors (zeroed first; their contents are lost).
+ * *ms_per_launch = elapsed time / repeats. */
+int32_t hsc_time_kernel(hsc_handle* h, int32_t kernel, int32_t lanes_per_row, int32_t repeats, hsc_stream stream,
+                        double* ms_per_launch);
 int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* P,
                         int32_t* active, double* aca_history);
 

@@ -63,6 +63,8 @@ def test_invalid_config_rejected(lib):
     c.poly_interval_ratio = 1.0
     assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
     assert lib.hsc_solve(None, None, None, None, None) == _abi.HSC_ERR_INVALID_ARGUMENT
+    ms = C.c_double()
+    assert lib.hsc_time_kernel(None, 0, 8, 1, None, C.byref(ms)) == _abi.HSC_ERR_INVALID_ARGUMENT
 
 
 def test_jsonl_row_matches_reference_schema(lib, tmp_path):

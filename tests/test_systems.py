@@ -53,3 +53,18 @@ def test_from_dense_roundtrip():
     D, A, E, H, b, _ = hsc_core.manufactured_system(1e4, nx=4, ny=2, nz=2)
     s = systems.from_dense(D, A, E, b)
     np.testing.assert_allclose(s.dense(), H, rtol=0, atol=1e-13 * np.abs(H).max())
+
+
+def test_replicate_fem_is_block_diagonal_with_one_coupling():
+    """replicate_fem: the FEM block repeated on the diagonal, the ABD block and
+    its coupling kept once (to the first copy), nothing between copies."""
+    s = systems.cube_system(seed=0)
+    R = systems.replicate_fem(s, 3)
+    assert (R.fn, R.qm, R.qo) == (3 * s.fn, s.qm, 3 * s.fn)
+    assert np.all(R.rows <= R.cols)
+    H, HR, n = s.dense(), R.dense(), s.fn
+    for k in range(3):
+        np.testing.assert_array_equal(HR[k * n:(k + 1) * n, k * n:(k + 1) * n], H[:n, :n])
+    np.testing.assert_array_equal(HR[3 * n:, 3 * n:], H[n:, n:])
+    np.testing.assert_array_equal(HR[:n, 3 * n:], H[:n, n:])
+    assert not HR[n:3 * n, 3 * n:].any() and not HR[:n, n:3 * n].any()
