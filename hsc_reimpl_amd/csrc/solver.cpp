@@ -1225,7 +1225,7 @@ void enqueue_finish(hsc_handle* h, const double* b, double* x, hipStream_t st) {
 }
 
 // Acceptance (linear_hsc.cu:400-415) from out = {|b - Hx|^2, |b|^2, b.x}.
-int judge_finish(hsc_handle* h, hsc_stats* s, const double out[3]) {
+int evaluate_finish(hsc_handle* h, hsc_stats* s, const double out[3]) {
   const double bn = std::sqrt(out[1]);
   s->descent = out[2];
   s->full_relative_residual = bn > 0 ? std::sqrt(out[0]) / bn : 0.0;
@@ -1251,7 +1251,7 @@ int finish_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStr
   double out[3];
   HIPCHK(hipMemcpyAsync(out, h->misc.as<double>() + 20, sizeof(out), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  return judge_finish(h, s, out);
+  return evaluate_finish(h, s, out);
 }
 
 int status_from_out(hsc_handle* h, const hsc::SysOut& o) {
@@ -1313,7 +1313,7 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   s->true_residual_checks = out[0].checks;
   s->restarts = out[0].restarts;
   // the PCG's own failure message takes precedence over the acceptance one
-  int c2 = judge_finish(h, s, fin.data());
+  int c2 = evaluate_finish(h, s, fin.data());
   int code = status_from_out(h, out[0]);
   if (code == HSC_OK) code = c2;
   s->status = code;
@@ -1514,7 +1514,7 @@ int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count, const doubl
       s->schur_relative_residual = out[i].rel;
       s->true_residual_checks = out[i].checks;
       s->restarts = out[i].restarts;
-      int c2 = judge_finish(h, s, fin.data() + 3 * i);
+      int c2 = evaluate_finish(h, s, fin.data() + 3 * i);
       int code = status_from_out(h, out[i]);
       if (code == HSC_OK) code = c2;
       s->status = code;
@@ -1686,7 +1686,7 @@ int32_t hsc_batch_stats(hsc_batch* bt, hsc_stats* stats, hsc_stream stream) {
       s->schur_relative_residual = out[i].rel;
       s->true_residual_checks = out[i].checks;
       s->restarts = out[i].restarts;
-      const int c2 = judge_finish(h, s, res.data() + 3 * i);
+      const int c2 = evaluate_finish(h, s, res.data() + 3 * i);
       int code = status_from_out(h, out[i]);
       if (code == HSC_OK) code = c2;
       s->status = code;
