@@ -98,6 +98,10 @@ hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks)
 hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st);
 const void* pcg_kernel_fn(int G, bool mixed, bool res);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);  // bytes % 4 == 0; a kernel, safe in graphs
+// general 3x3-block triplets -> upper-triangle BCOO of unique blocks, sorted by
+// (row, col), duplicates summed in input order (assemble.hip); synchronises st
+hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, long long count, int fold,
+                         int* orow, int* ocol, double* ovals, long long* out_count, hipStream_t st);
 // one launch of a standalone operator kernel: 0 FP64 D x, 1 FP64 pass, 2 FP32 pass
 hipError_t launch_timed_op(int kernel, int G, const SysDesc& S, const void* x, void* y, hipStream_t st);
 // up to kZeroListMax buffers zeroed by one kernel launch (each bytes % 4 == 0)

@@ -1771,6 +1771,19 @@ int32_t hsc_apply_precond(hsc_handle* h, const double* r, double* z, hsc_stream 
   });
 }
 
+int32_t hsc_assemble_bcoo(const int32_t* rows, const int32_t* cols, const double* values, int64_t count,
+                          int32_t fold_lower, int32_t* out_rows, int32_t* out_cols, double* out_values,
+                          int64_t* out_count, hsc_stream stream) {
+  if (!out_count || count < 0 || (count > 0 && (!rows || !cols || !values || !out_rows || !out_cols || !out_values)))
+    return HSC_ERR_INVALID_ARGUMENT;
+  if (count >= (1LL << 31)) return HSC_ERR_CAPACITY;
+  long long n = 0;
+  const hipError_t e = hsc::assemble_bcoo(rows, cols, values, count, fold_lower ? 1 : 0, out_rows, out_cols,
+                                          out_values, &n, (hipStream_t)stream);
+  *out_count = n;
+  return e == hipSuccess ? HSC_OK : HSC_ERR_HIP;
+}
+
 int32_t hsc_time_kernel(hsc_handle* h, int32_t kernel, int32_t lanes_per_row, int32_t repeats, hsc_stream stream,
                         double* ms_per_launch) {
   if (!h || !ms_per_launch || kernel < 0 || kernel > 2 || repeats < 1 ||

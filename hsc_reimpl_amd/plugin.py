@@ -119,6 +119,30 @@ class BCOO3:
                    sysm.N // 3)
 
 
+def assemble_bcoo(rows: torch.Tensor, cols: torch.Tensor, values: torch.Tensor, block_dim: int,
+                  fold_lower: bool = True, stream=None) -> BCOO3:
+    """General 3x3-block triplets on the device (duplicates allowed) -> the
+    upper-triangle BCOO of unique blocks the plugin takes (hsc_assemble_bcoo;
+    the caller-side ge2sym + convert of global_linear_system.cu:232-234)."""
+    lib = _abi.load()
+    n = int(rows.numel())
+    dev = rows.device
+    r = rows.to(torch.int32).contiguous()
+    c = cols.to(torch.int32).contiguous()
+    v = values.to(torch.float64).reshape(n, 9).contiguous()
+    orow = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    ocol = torch.empty_like(orow)
+    oval = torch.empty((max(n, 1), 9), dtype=torch.float64, device=dev)
+    cnt = C.c_int64()
+    code = lib.hsc_assemble_bcoo(_ptr(r) if n else None, _ptr(c) if n else None, _ptr(v) if n else None, n,
+                                 1 if fold_lower else 0, _ptr(orow), _ptr(ocol), _ptr(oval), C.byref(cnt),
+                                 _stream(stream))
+    if code != _abi.HSC_OK:
+        raise HSCError(code, "hsc_assemble_bcoo failed")
+    k = cnt.value
+    return BCOO3(orow[:k].clone(), ocol[:k].clone(), oval[:k].clone(), block_dim)
+
+
 @dataclass
 class SolvingInfo:
     """Mirror of GlobalLinearSystem::SolvingInfo (global_linear_system.h:227-245

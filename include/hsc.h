@@ -196,6 +196,21 @@ typedef struct hsc_setup_info {
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 
+/* ---- assembly before the path ----
+ * Replaces the caller-side conversion that produces the plugin's input
+ * (libuipc MatrixConverter::ge2sym + convert, global_linear_system.cu:232-234):
+ * `count` general 3x3-block triplets (block row, block column, 9 values
+ * column-major; duplicates allowed) -> the upper-triangle BCOO of unique
+ * blocks the plugin receives, sorted by (row, column), duplicates summed in
+ * input order.  A lower block (row > column) is folded onto (column, row)
+ * transposed when fold_lower = 1, or dropped when 0 (the input holds both
+ * triangles of a symmetric H).  All arrays are device memory; the outputs
+ * need `count` entries; *out_count receives the number of unique blocks.
+ * Synchronises `stream`. */
+int32_t hsc_assemble_bcoo(const int32_t* rows, const int32_t* cols, const double* values, int64_t count,
+                          int32_t fold_lower, int32_t* out_rows, int32_t* out_cols, double* out_values,
+                          int64_t* out_count, hsc_stream stream);
+
 /* ---- measurement (no reference counterpart) ----
  * Times `repeats` back-to-back launches of one standalone operator kernel on
  * the set-up handle's D, between HIP events on `stream`: kernel 0 = FP64
