@@ -98,6 +98,23 @@ hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks)
 hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st);
 const void* pcg_kernel_fn(int G, bool mixed, bool res);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);  // bytes % 4 == 0; a kernel, safe in graphs
+// per-tet log-Neo-Hookean Hessian (SPD-projected when project != 0) pulled
+// back to 10 node-block triplets per tet (fem_hessian.hip)
+struct FemHessianParams {
+  const int* tets;        // [ntet][4] node indices
+  const double* x;        // [nodes][3] positions
+  const double* Dm_inv;   // [ntet][9] column-major inverse rest edge matrices
+  const double* mu;       // [ntet] Lame parameters
+  const double* lambda;
+  const double* vol;      // [ntet] rest volumes
+  double dt2;             // scale dt^2 (with the volume)
+  long long ntet;
+  int project;
+  int* orow;              // [10 ntet] block rows (the smaller node index)
+  int* ocol;
+  double* oval;           // [10 ntet][9] column-major blocks
+};
+hipError_t launch_fem_hessian(const FemHessianParams& P, hipStream_t st);
 // general 3x3-block triplets -> upper-triangle BCOO of unique blocks, sorted by
 // (row, col), duplicates summed in input order (assemble.hip); synchronises st
 hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, long long count, int fold,

@@ -1771,6 +1771,20 @@ int32_t hsc_apply_precond(hsc_handle* h, const double* r, double* z, hsc_stream 
   });
 }
 
+int32_t hsc_fem_hessian(const int32_t* tets, const double* x, const double* Dm_inv, const double* mu,
+                        const double* lambda, const double* volumes, double dt, int64_t n_tets, int32_t project_spd,
+                        int32_t* out_rows, int32_t* out_cols, double* out_values, hsc_stream stream) {
+  if (n_tets < 0 || (n_tets > 0 && (!tets || !x || !Dm_inv || !mu || !lambda || !volumes || !out_rows ||
+                                    !out_cols || !out_values)))
+    return HSC_ERR_INVALID_ARGUMENT;
+  if (n_tets > (1LL << 27)) return HSC_ERR_CAPACITY;
+  hsc::FemHessianParams P{tets, x, Dm_inv, mu, lambda, volumes, dt * dt, n_tets, project_spd ? 1 : 0,
+                          out_rows, out_cols, out_values};
+  hipError_t e = hsc::launch_fem_hessian(P, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? HSC_OK : HSC_ERR_HIP;
+}
+
 int32_t hsc_assemble_bcoo(const int32_t* rows, const int32_t* cols, const double* values, int64_t count,
                           int32_t fold_lower, int32_t* out_rows, int32_t* out_cols, double* out_values,
                           int64_t* out_count, hsc_stream stream) {

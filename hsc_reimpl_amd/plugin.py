@@ -119,6 +119,32 @@ class BCOO3:
                    sysm.N // 3)
 
 
+def fem_hessian(tets: torch.Tensor, x: torch.Tensor, Dm_inv: torch.Tensor, mu: torch.Tensor, lam: torch.Tensor,
+                volumes: torch.Tensor, dt: float = 1.0, project_spd: bool = True, stream=None):
+    """Per-tet log-Neo-Hookean Hessian blocks on the device (hsc_fem_hessian):
+    returns (rows, cols, values (10 n, 9) column-major), 10 general triplets
+    per tet for assemble_bcoo.  Dm_inv: (n, 3, 3) row-major tensors (stored
+    column-major for the kernel)."""
+    lib = _abi.load()
+    n = int(tets.shape[0])
+    dev = tets.device
+    t = tets.to(torch.int32).contiguous()
+    xx = x.to(torch.float64).contiguous()
+    dm = Dm_inv.to(torch.float64).transpose(1, 2).contiguous()  # column-major 3x3 per tet
+    m = mu.to(torch.float64).contiguous()
+    la = lam.to(torch.float64).contiguous()
+    vo = volumes.to(torch.float64).contiguous()
+    rows = torch.empty(max(10 * n, 1), dtype=torch.int32, device=dev)
+    cols = torch.empty_like(rows)
+    vals = torch.empty((max(10 * n, 1), 9), dtype=torch.float64, device=dev)
+    if n:
+        code = lib.hsc_fem_hessian(_ptr(t), _ptr(xx), _ptr(dm), _ptr(m), _ptr(la), _ptr(vo), float(dt), n,
+                                   1 if project_spd else 0, _ptr(rows), _ptr(cols), _ptr(vals), _stream(stream))
+        if code != _abi.HSC_OK:
+            raise HSCError(code, "hsc_fem_hessian failed")
+    return rows[:10 * n], cols[:10 * n], vals[:10 * n]
+
+
 def assemble_bcoo(rows: torch.Tensor, cols: torch.Tensor, values: torch.Tensor, block_dim: int,
                   fold_lower: bool = True, stream=None) -> BCOO3:
     """General 3x3-block triplets on the device (duplicates allowed) -> the

@@ -196,6 +196,20 @@ typedef struct hsc_setup_info {
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 
+/* ---- FEM Hessian generator (before the path) ----
+ * The per-tet log-Neo-Hookean Hessian of the reference's FEM constitution
+ * (log_neo_hookean_function.h:93-150, log_neo_hookean_3d.cu:62-135 of the
+ * overlays): F = Ds Dm^-1 (Ds = [x1-x0, x2-x0, x3-x0]), the 9x9 d2Psi/dF2 with
+ * vec(F) column-major, projected SPD by bounded cyclic Jacobi when
+ * project_spd = 1, scaled by volume * dt^2 and pulled back to the 10 node
+ * blocks a <= b of each tet.  Writes 10 * n_tets general triplets (row = the
+ * smaller node index, 3x3 column-major) for hsc_assemble_bcoo.  tets:
+ * [n_tets][4]; x: [nodes][3]; Dm_inv: [n_tets][9] column-major; mu, lambda,
+ * volumes: [n_tets].  Device memory; synchronises `stream`. */
+int32_t hsc_fem_hessian(const int32_t* tets, const double* x, const double* Dm_inv, const double* mu,
+                        const double* lambda, const double* volumes, double dt, int64_t n_tets, int32_t project_spd,
+                        int32_t* out_rows, int32_t* out_cols, double* out_values, hsc_stream stream);
+
 /* ---- assembly before the path ----
  * Replaces the caller-side conversion that produces the plugin's input
  * (libuipc MatrixConverter::ge2sym + convert, global_linear_system.cu:232-234):
