@@ -139,6 +139,61 @@ def assemble_timing(sysm, dev, reps=5):
             "what": "general triplets (both triangles, shuffled) -> upper-triangle BCOO (ge2sym + convert)"}
 
 
+def fem_generator_timing(dev, reps=5, sample=64):
+    """hsc_fem_hessian + hsc_assemble_bcoo on the 78,400 tets of the vehicle
+    system's 7 tori (11,200 Tet4 each, E 1e10 on 4 and 1e8 on 3, nu 0.45),
+    nodes displaced by N(0, 2 mm) so that the SPD projection acts: median wall
+    time of `reps` calls each; a sample of tets checked against the oracle
+    (1e-10 per block), whose own time on that sample is the CPU figure."""
+    import numpy as np
+    import torch
+    from hsc_reimpl_amd import plugin, systems
+    from oracle import fem
+    rng = np.random.default_rng(0)
+    nodes0, tets0, _ = systems.torus_mesh(0.45, 0.12, 0.06, 70, 16, 2)
+    youngs = (1e10,) * 4 + (1e8,) * 3
+    nodes = np.concatenate([nodes0 + np.array([1.2 * t, 0.0, 0.0]) for t in range(7)])
+    tets = np.concatenate([tets0 + t * len(nodes0) for t in range(7)])
+    E = np.repeat(np.array(youngs), len(tets0))
+    nu = 0.45
+    mu, lam = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    X = nodes[tets]
+    Dm = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], -1)
+    Dinv = np.linalg.inv(Dm)
+    vol = np.abs(np.linalg.det(Dm)) / 6
+    x = nodes + 0.002 * rng.normal(size=nodes.shape)
+    t = lambda a, d: torch.as_tensor(np.ascontiguousarray(a), dtype=d, device=dev)  # noqa: E731
+    args = (t(tets, torch.int32), t(x, torch.float64), t(Dinv, torch.float64), t(mu, torch.float64),
+            t(lam, torch.float64), t(vol, torch.float64))
+    rows, cols, vals = plugin.fem_hessian(*args, dt=0.01)
+    A = plugin.assemble_bcoo(rows, cols, vals, len(nodes))
+    tg, ta = [], []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rows, cols, vals = plugin.fem_hessian(*args, dt=0.01)
+        t1 = time.perf_counter()
+        A = plugin.assemble_bcoo(rows, cols, vals, len(nodes))
+        t2 = time.perf_counter()
+        tg.append(t1 - t0)
+        ta.append(t2 - t1)
+    vh = vals.cpu().numpy()
+    ok = True
+    t0 = time.perf_counter()
+    for e in rng.choice(len(tets), sample, replace=False):
+        for o, (a, b, B) in enumerate(fem.tet_blocks(x[tets[e]], Dinv[e], mu[e], lam[e], vol[e], dt=0.01)):
+            tet = tets[e]
+            ref = B if tet[a] <= tet[b] else B.T
+            ok &= bool(np.abs(vh[10 * e + o].reshape(3, 3, order="F") - ref).max() <= 1e-10 * np.abs(ref).max())
+    cpu_per_tet = (time.perf_counter() - t0) / sample
+    g_ms, a_ms = 1e3 * float(np.median(tg)), 1e3 * float(np.median(ta))
+    return {"tets": int(len(tets)), "triplets": int(10 * len(tets)), "blocks_assembled": int(A.rows.numel()),
+            "generate_ms": round(g_ms, 3), "assemble_ms": round(a_ms, 3),
+            "tets_per_s": round(len(tets) / (g_ms * 1e-3), 1), "matches_oracle_sample": ok,
+            "cpu_oracle_tets_per_s": round(1.0 / cpu_per_tet, 1),
+            "cpu_sample": f"{sample} tets through the oracle (pure Python Jacobi), one thread"}
+
+
 def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
     """Oracle (sparse NumPy/SciPy restatement of the reference algorithm):
     full solves (setup + PCG + back-substitution) of the same system, single
@@ -397,6 +452,7 @@ def main():
                       "converged": r["plain"]["status"] == 0 and r["learned"]["status"] == 0}
     bandwidth = beyond_mall(sysm, dev)
     assembly = assemble_timing(sysm, dev)
+    generator = fem_generator_timing(dev)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sysm, order, omega, rtol)
@@ -435,6 +491,7 @@ def main():
         "mixed_stiffness_sweep_2178_dof": sweep,
         "bandwidth_beyond_mall": bandwidth,
         "assembly_before_path": assembly,
+        "fem_hessian_generator": generator,
     }
     print(json.dumps(line), flush=True)
 
