@@ -194,6 +194,56 @@ def fem_generator_timing(dev, reps=5, sample=64):
             "cpu_sample": f"{sample} tets through the oracle (pure Python Jacobi), one thread"}
 
 
+def coupling_timing(dev, reps=7, sample=256):
+    """hsc_abd_fem_coupling on random contact blocks between 12 bodies of
+    2,000 ABD vertices and the vehicle system's 23,520 FEM nodes: 8,400
+    blocks (its 2,100 coupled rim nodes x 4 ABD vertices each -> 33,600
+    triplets) and 1M blocks (the kernel's bandwidth: 4 + 4 + 72 + 4 + 24 +
+    4 + 4 bytes read and 4 x 80 written per contact, 436 B), median wall time
+    of `reps` calls through the C ABI; a sample of contacts checked against
+    the oracle (bitwise), whose own time on it is the CPU figure."""
+    import numpy as np
+    import torch
+    from hsc_reimpl_amd import plugin
+    from oracle import coupling
+    rng = np.random.default_rng(0)
+    nb, vpb, nf = 12, 2000, 7 * 3360
+    v2b = np.repeat(np.arange(nb), vpb).astype(np.int32)
+    xbar = rng.uniform(-0.3, 0.3, size=(nb * vpb, 3))
+    body_fixed = np.zeros(nb, dtype=np.int32)
+    vertex_fixed = np.zeros(nf, dtype=np.int32)
+    t = lambda a, d: torch.as_tensor(np.ascontiguousarray(a), dtype=d, device=dev)  # noqa: E731
+    common = (t(v2b, torch.int32), t(xbar, torch.float64), t(body_fixed, torch.int32), t(vertex_fixed, torch.int32),
+              0, 4 * nb)
+    out = {}
+    for name, n in (("vehicle", 8400), ("1M", 1 << 20)):
+        av = rng.integers(0, nb * vpb, n).astype(np.int32)
+        fv = rng.integers(0, nf, n).astype(np.int32)
+        H = rng.normal(size=(n, 3, 3))
+        args = (t(av, torch.int32), t(fv, torch.int32), t(H, torch.float64)) + common
+        plugin.abd_fem_coupling(*args)
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rows, cols, vals = plugin.abd_fem_coupling(*args)
+            ts.append(time.perf_counter() - t0)
+        ms = 1e3 * float(np.median(ts))
+        pick = rng.choice(n, sample, replace=False)
+        t0 = time.perf_counter()
+        r, c, v = coupling.coupling_blocks(av[pick], fv[pick], H[pick], v2b, xbar, body_fixed, vertex_fixed, 0,
+                                           4 * nb)
+        cpu = (time.perf_counter() - t0) / sample
+        idx = (4 * pick[:, None] + np.arange(4)[None]).ravel()
+        ok = bool(np.array_equal(vals[torch.as_tensor(idx, device=dev)].cpu().numpy(), v)
+                  and np.array_equal(rows[torch.as_tensor(idx, device=dev)].cpu().numpy(), r))
+        out[name] = {"contacts": n, "triplets": 4 * n, "ms": round(ms, 4),
+                     "contacts_per_s": round(n / (ms * 1e-3), 1), "GB_per_s_wall": round(436 * n / (ms * 1e-3) / 1e9, 1),
+                     "matches_oracle_sample": ok, "cpu_oracle_contacts_per_s": round(1.0 / cpu, 1)}
+    out["cpu_sample"] = f"{sample} contacts per case through the oracle (NumPy per contact), one thread"
+    return out
+
+
 def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
     """Oracle (sparse NumPy/SciPy restatement of the reference algorithm):
     full solves (setup + PCG + back-substitution) of the same system, single
@@ -453,6 +503,7 @@ def main():
     bandwidth = beyond_mall(sysm, dev)
     assembly = assemble_timing(sysm, dev)
     generator = fem_generator_timing(dev)
+    coupling_t = coupling_timing(dev)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sysm, order, omega, rtol)
@@ -492,6 +543,7 @@ def main():
         "bandwidth_beyond_mall": bandwidth,
         "assembly_before_path": assembly,
         "fem_hessian_generator": generator,
+        "abd_fem_coupling": coupling_t,
     }
     print(json.dumps(line), flush=True)
 

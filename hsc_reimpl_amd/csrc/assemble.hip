@@ -79,7 +79,49 @@ __global__ void k_asm_reduce(const u64* __restrict__ keys, const int* __restrict
 
 int grid_for(long long n) { return (int)std::min<long long>((n + 255) / 256, 4096); }
 
+// ABD-FEM coupling (abd_fem_linear_subsystem.cu:18-61): contact block H (3x3,
+// ABD vertex a, FEM vertex f) -> J(xbar_a)^T H (12x3), written as the 4
+// blocks (4 body + k, f), zero when the body or the vertex is fixed.  With
+// x = p + A xbar and q = (p, A's rows), x_i = p_i + A_i . xbar, so block 0 is
+// H and block k >= 1 (DOFs of row k-1 of A) is xbar outer H's row k-1: entry
+// (m, l) = xbar[m] H(k-1, l) -- the layout of J^T g in the gravity setup at
+// affine_body_dynamics.cu:609-613 (segment 3+3k = g[k] xbar).
+__global__ void k_abd_fem_coupling(AbdFemCouplingParams P) {
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < P.count;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int av = P.abd_vertex[c], fv = P.fem_vertex[c];
+    const int body = av >= 0 && av < P.n_abd_vertices ? P.v2b[av] : -1;
+    if (body < 0 || body >= P.n_bodies || fv < 0 || fv >= P.n_fem_vertices) {
+      atomicOr(P.err, 1);
+      continue;
+    }
+    const bool zero = P.body_fixed[body] != 0 || P.vertex_fixed[fv] != 0;
+    double H[9], xb[3];  // H column-major
+#pragma unroll
+    for (int q = 0; q < 9; ++q) H[q] = zero ? 0.0 : P.hess[9 * c + q];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) xb[m] = P.xbar[3 * (long long)av + m];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const long long o = 4 * c + k;
+      P.orow[o] = P.abd_block_offset + 4 * body + k;
+      P.ocol[o] = P.fem_block_offset + fv;
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          P.oval[9 * o + m + 3 * l] = k == 0 ? H[m + 3 * l] : xb[m] * H[(k - 1) + 3 * l];
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_abd_fem_coupling(const AbdFemCouplingParams& P, hipStream_t st) {
+  if (P.count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_abd_fem_coupling, dim3(grid_for(P.count)), dim3(256), 0, st, P);
+  return hipGetLastError();
+}
 
 hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, long long count, int fold,
                          int* orow, int* ocol, double* ovals, long long* out_count, hipStream_t st) {

@@ -115,6 +115,24 @@ struct FemHessianParams {
   double* oval;           // [10 ntet][9] column-major blocks
 };
 hipError_t launch_fem_hessian(const FemHessianParams& P, hipStream_t st);
+struct AbdFemCouplingParams {
+  const int* abd_vertex;    // [count] ABD vertex per contact block
+  const int* fem_vertex;    // [count] FEM vertex per contact block
+  const double* hess;       // [count][9] column-major 3x3 (ABD rows, FEM columns)
+  const int* v2b;           // [n_abd_vertices] body of each ABD vertex
+  const double* xbar;       // [n_abd_vertices][3] rest positions
+  const int* body_fixed;    // [n_bodies]
+  const int* vertex_fixed;  // [n_fem_vertices]
+  long long count;
+  int n_abd_vertices, n_fem_vertices, n_bodies;
+  int abd_block_offset, fem_block_offset;
+  int* err;                 // set to 1 on an index out of range
+  int* orow;                // [4 count]
+  int* ocol;
+  double* oval;             // [4 count][9]
+};
+// launch only (no synchronisation), assemble.hip
+hipError_t launch_abd_fem_coupling(const AbdFemCouplingParams& P, hipStream_t st);
 // general 3x3-block triplets -> upper-triangle BCOO of unique blocks, sorted by
 // (row, col), duplicates summed in input order (assemble.hip); synchronises st
 hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, long long count, int fold,

@@ -1785,6 +1785,41 @@ int32_t hsc_fem_hessian(const int32_t* tets, const double* x, const double* Dm_i
   return e == hipSuccess ? HSC_OK : HSC_ERR_HIP;
 }
 
+int32_t hsc_abd_fem_coupling(const int32_t* abd_vertex, const int32_t* fem_vertex, const double* hess,
+                             int64_t count, const int32_t* v2b, const double* xbar, int32_t n_abd_vertices,
+                             const int32_t* body_fixed, int32_t n_bodies, const int32_t* vertex_fixed,
+                             int32_t n_fem_vertices, int32_t abd_block_offset, int32_t fem_block_offset,
+                             int32_t* out_rows, int32_t* out_cols, double* out_values, hsc_stream stream) {
+  if (count < 0 || n_abd_vertices < 0 || n_bodies < 0 || n_fem_vertices < 0 || abd_block_offset < 0 ||
+      fem_block_offset < 0)
+    return HSC_ERR_INVALID_ARGUMENT;
+  if (count == 0) return HSC_OK;
+  if (!abd_vertex || !fem_vertex || !hess || !v2b || !xbar || !body_fixed || !vertex_fixed || !out_rows ||
+      !out_cols || !out_values)
+    return HSC_ERR_INVALID_ARGUMENT;
+  if (count > (1LL << 28) || (long long)abd_block_offset + 4LL * n_bodies > INT32_MAX ||
+      (long long)fem_block_offset + n_fem_vertices > INT32_MAX)
+    return HSC_ERR_CAPACITY;
+  hipStream_t st = (hipStream_t)stream;
+  int* err = nullptr;
+  int herr = 0;
+  hipError_t e = hipMallocAsync((void**)&err, sizeof(int), st);
+  if (e == hipSuccess) e = hipMemsetAsync(err, 0, sizeof(int), st);
+  if (e == hipSuccess) {
+    hsc::AbdFemCouplingParams P{abd_vertex,     fem_vertex,     hess,     v2b,      xbar,
+                                body_fixed,     vertex_fixed,   count,    n_abd_vertices,
+                                n_fem_vertices, n_bodies,       abd_block_offset,
+                                fem_block_offset, err,          out_rows, out_cols, out_values};
+    e = hsc::launch_abd_fem_coupling(P, st);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (err) (void)hipFreeAsync(err, st);
+  const hipError_t se = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = se;
+  if (e != hipSuccess) return HSC_ERR_HIP;
+  return herr ? HSC_ERR_INVALID_ARGUMENT : HSC_OK;
+}
+
 int32_t hsc_assemble_bcoo(const int32_t* rows, const int32_t* cols, const double* values, int64_t count,
                           int32_t fold_lower, int32_t* out_rows, int32_t* out_cols, double* out_values,
                           int64_t* out_count, hsc_stream stream) {

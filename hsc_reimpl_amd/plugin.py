@@ -145,6 +145,38 @@ def fem_hessian(tets: torch.Tensor, x: torch.Tensor, Dm_inv: torch.Tensor, mu: t
     return rows[:10 * n], cols[:10 * n], vals[:10 * n]
 
 
+def abd_fem_coupling(abd_vertex: torch.Tensor, fem_vertex: torch.Tensor, hess: torch.Tensor,
+                     v2b: torch.Tensor, xbar: torch.Tensor, body_fixed: torch.Tensor, vertex_fixed: torch.Tensor,
+                     abd_block_offset: int, fem_block_offset: int, stream=None):
+    """ABD-FEM contact blocks -> the coupling triplets of the plugin's input on
+    the device (hsc_abd_fem_coupling; abd_fem_linear_subsystem.cu:18-61):
+    returns (rows, cols, values (4 n, 9) column-major), 4 triplets per contact
+    block for assemble_bcoo.  hess: (n, 3, 3) row-major contact blocks (ABD
+    vertex rows, FEM vertex columns); xbar: (abd vertices, 3) rest positions;
+    body_fixed / vertex_fixed: per body / per FEM vertex flags."""
+    lib = _abi.load()
+    n = int(abd_vertex.numel())
+    dev = abd_vertex.device
+    av = abd_vertex.to(torch.int32).contiguous()
+    fv = fem_vertex.to(torch.int32).contiguous()
+    h = hess.to(torch.float64).reshape(n, 3, 3).transpose(1, 2).contiguous()  # column-major 3x3
+    vb = v2b.to(torch.int32).contiguous()
+    xb = xbar.to(torch.float64).reshape(-1, 3).contiguous()
+    bf = body_fixed.to(torch.int32).contiguous()
+    vf = vertex_fixed.to(torch.int32).contiguous()
+    rows = torch.empty(max(4 * n, 1), dtype=torch.int32, device=dev)
+    cols = torch.empty_like(rows)
+    vals = torch.empty((max(4 * n, 1), 9), dtype=torch.float64, device=dev)
+    if n:
+        code = lib.hsc_abd_fem_coupling(_ptr(av), _ptr(fv), _ptr(h), n, _ptr(vb), _ptr(xb), int(vb.numel()),
+                                        _ptr(bf), int(bf.numel()), _ptr(vf), int(vf.numel()),
+                                        int(abd_block_offset), int(fem_block_offset), _ptr(rows), _ptr(cols),
+                                        _ptr(vals), _stream(stream))
+        if code != _abi.HSC_OK:
+            raise HSCError(code, "hsc_abd_fem_coupling failed")
+    return rows[:4 * n], cols[:4 * n], vals[:4 * n]
+
+
 def assemble_bcoo(rows: torch.Tensor, cols: torch.Tensor, values: torch.Tensor, block_dim: int,
                   fold_lower: bool = True, stream=None) -> BCOO3:
     """General 3x3-block triplets on the device (duplicates allowed) -> the

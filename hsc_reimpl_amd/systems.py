@@ -408,6 +408,76 @@ def batch_systems(seed=2, count=64, bodies=(1, 16), tori=(1, 8), tets=(2000, 150
     return out
 
 
+def _affine_jacobian(xbar):
+    """3 x 12 dx/dq of x = p + A xbar, q = (p, rows of A)."""
+    J = np.zeros((3, 12))
+    J[:, :3] = np.eye(3)
+    for i in range(3):
+        J[i, 3 + 3 * i:6 + 3 * i] = xbar
+    return J
+
+
+def contact_scene(seed=4, nx=6, ny=6, nz=3, h=0.02, E=1e8, nu=0.45, n_bodies=3, verts_per_body=40,
+                  n_contacts=600, fixed_body=None, n_fixed_vertices=0):
+    """ABD bodies touching an FEM box through point contacts, split as the
+    coupling assembly sees it: the contact list (ABD vertex, FEM vertex,
+    3x3 contact block H = -K_c, the ABD-row / FEM-column block of the
+    penalty 0.5 (x_a - x_f)^T K_c (x_a - x_f)) and the general triplets of
+    everything else (ABD blocks m_v sum J^T J + kappa diag(0, I9) + sum
+    J^T K_c J, FEM stiffness + m I + sum K_c), ABD block rows first (qo = 0,
+    fo = qm).  K_c = k_n n n^T + k_t (I - n n^T), n a random unit normal."""
+    rng = np.random.default_rng(seed)
+    nodes, tets = box_mesh(nx, ny, nz, h)
+    nf = len(nodes)
+    I, J, B = fem_blocks(nodes, tets, E, nu)
+    kf = E * h
+    diag = np.flatnonzero(I == J)
+    pos = np.empty(nf, dtype=np.int64)
+    pos[I[diag]] = diag
+    B[diag] += 0.05 * kf * np.eye(3)[None]
+    v2b = np.repeat(np.arange(n_bodies), verts_per_body).astype(np.int32)
+    xbar = rng.uniform(-0.05, 0.05, size=(len(v2b), 3))
+    top = np.flatnonzero(np.isclose(nodes[:, 2], nodes[:, 2].max()))
+    av = rng.integers(0, len(v2b), n_contacts).astype(np.int32)
+    fv = rng.choice(top, n_contacts).astype(np.int32)
+    nrm = rng.normal(size=(n_contacts, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    kn = kf * rng.uniform(0.5, 2.0, n_contacts)
+    K = (kn[:, None, None] * (0.9 * nrm[:, :, None] * nrm[:, None, :] + 0.1 * np.eye(3)[None]))
+    body_fixed = np.zeros(n_bodies, dtype=np.int32)
+    if fixed_body is not None:
+        body_fixed[fixed_body] = 1
+    vertex_fixed = np.zeros(nf, dtype=np.int32)
+    vertex_fixed[rng.choice(top, n_fixed_vertices, replace=False)] = 1
+    # ABD blocks
+    A = np.zeros((n_bodies, 12, 12))
+    mv = 0.05 * kf
+    for v in range(len(v2b)):
+        Jv = _affine_jacobian(xbar[v])
+        A[v2b[v]] += mv * Jv.T @ Jv
+    A[:, 3:, 3:] += 0.5 * kf * np.eye(9)[None]
+    for c in range(n_contacts):
+        Jc = _affine_jacobian(xbar[av[c]])
+        A[v2b[av[c]]] += Jc.T @ K[c] @ Jc
+        B[pos[fv[c]]] += K[c]
+    qm = 12 * n_bodies
+    rows, cols, vals = [], [], []
+    for t in range(n_bodies):
+        for bi in range(4):
+            for bj in range(bi, 4):
+                rows.append(4 * t + bi)
+                cols.append(4 * t + bj)
+                vals.append(A[t, 3 * bi:3 * bi + 3, 3 * bj:3 * bj + 3].T.ravel())
+    up = I <= J
+    R = np.concatenate([np.asarray(rows), I[up] + qm // 3]).astype(np.int32)
+    C = np.concatenate([np.asarray(cols), J[up] + qm // 3]).astype(np.int32)
+    V = np.concatenate([np.asarray(vals), B[up].transpose(0, 2, 1).reshape(-1, 9)])
+    return dict(rows=R, cols=C, vals=np.ascontiguousarray(V), abd_vertex=av, fem_vertex=fv, hess=-K,
+                v2b=v2b, xbar=xbar, body_fixed=body_fixed, vertex_fixed=vertex_fixed, qo=0, qm=qm, fo=qm,
+                fn=3 * nf, abd_block_offset=0, fem_block_offset=qm // 3, N=qm + 3 * nf,
+                b=rng.normal(size=qm + 3 * nf))
+
+
 def from_dense(D, A, E, b):
     """Pack a dense FEM-first system (core.py layout: D n x n, A m x m,
     E m x n) into plugin input format; zero blocks are dropped."""

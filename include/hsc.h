@@ -210,6 +210,24 @@ int32_t hsc_fem_hessian(const int32_t* tets, const double* x, const double* Dm_i
                         const double* lambda, const double* volumes, double dt, int64_t n_tets, int32_t project_spd,
                         int32_t* out_rows, int32_t* out_cols, double* out_values, hsc_stream stream);
 
+/* ---- ABD-FEM coupling blocks (before the path) ----
+ * The reference's off-diagonal subsystem assembly
+ * (abd_fem_linear_subsystem.cu:18-61): each contact block H (3x3 column-major,
+ * rows on ABD vertex abd_vertex[i], columns on FEM vertex fem_vertex[i])
+ * becomes J^T H (12x3) with J = d x / d q the affine Jacobian at the vertex's
+ * rest position xbar (x = p + A xbar, q = (p, rows of A)), written as 4
+ * triplets (abd_block_offset + 4 body + k, fem_block_offset + fem vertex),
+ * values zero when body_fixed[body] or vertex_fixed[fem vertex] is nonzero.
+ * v2b, xbar: [n_abd_vertices](x3); body_fixed: [n_bodies]; vertex_fixed:
+ * [n_fem_vertices]; outputs: 4 * count triplets, the E blocks of the plugin's
+ * input for hsc_assemble_bcoo.  An index out of range returns
+ * HSC_ERR_INVALID_ARGUMENT.  Device memory; synchronises `stream`. */
+int32_t hsc_abd_fem_coupling(const int32_t* abd_vertex, const int32_t* fem_vertex, const double* hess,
+                             int64_t count, const int32_t* v2b, const double* xbar, int32_t n_abd_vertices,
+                             const int32_t* body_fixed, int32_t n_bodies, const int32_t* vertex_fixed,
+                             int32_t n_fem_vertices, int32_t abd_block_offset, int32_t fem_block_offset,
+                             int32_t* out_rows, int32_t* out_cols, double* out_values, hsc_stream stream);
+
 /* ---- assembly before the path ----
  * Replaces the caller-side conversion that produces the plugin's input
  * (libuipc MatrixConverter::ge2sym + convert, global_linear_system.cu:232-234):
