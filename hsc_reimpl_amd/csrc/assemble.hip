@@ -86,32 +86,52 @@ int grid_for(long long n) { return (int)std::min<long long>((n + 255) / 256, 409
 // H and block k >= 1 (DOFs of row k-1 of A) is xbar outer H's row k-1: entry
 // (m, l) = xbar[m] H(k-1, l) -- the layout of J^T g in the gravity setup at
 // affine_body_dynamics.cu:609-613 (segment 3+3k = g[k] xbar).
-__global__ void k_abd_fem_coupling(AbdFemCouplingParams P) {
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < P.count;
-       c += (long long)gridDim.x * blockDim.x) {
-    const int av = P.abd_vertex[c], fv = P.fem_vertex[c];
-    const int body = av >= 0 && av < P.n_abd_vertices ? P.v2b[av] : -1;
-    if (body < 0 || body >= P.n_bodies || fv < 0 || fv >= P.n_fem_vertices) {
-      atomicOr(P.err, 1);
-      continue;
+//
+// A workgroup takes tiles of kCplTile contacts: their H (contiguous), xbar,
+// block row / column and flags are staged in LDS, then the tile's 36 output
+// values and 4 index pairs per contact are written as contiguous runs (one
+// element per lane), so every store is coalesced.
+constexpr int kCplTile = 64;
+constexpr int kCplThreads = 256;
+
+__global__ __launch_bounds__(kCplThreads) void k_abd_fem_coupling(AbdFemCouplingParams P) {
+  __shared__ double sH[kCplTile * 9];
+  __shared__ double sX[kCplTile * 3];
+  __shared__ int sRow[kCplTile], sCol[kCplTile], sZero[kCplTile];
+  const int tid = threadIdx.x;
+  for (long long t = blockIdx.x; t * kCplTile < P.count; t += gridDim.x) {
+    const long long c0 = t * kCplTile;
+    const int nc = (int)(P.count - c0 < kCplTile ? P.count - c0 : kCplTile);
+    for (int i = tid; i < 9 * nc; i += kCplThreads) sH[i] = P.hess[9 * c0 + i];
+    if (tid < nc) {
+      const int av = P.abd_vertex[c0 + tid], fv = P.fem_vertex[c0 + tid];
+      const int body = av >= 0 && av < P.n_abd_vertices ? P.v2b[av] : -1;
+      const bool ok = body >= 0 && body < P.n_bodies && fv >= 0 && fv < P.n_fem_vertices;
+      if (!ok) atomicOr(P.err, 1);
+      sRow[tid] = ok ? P.abd_block_offset + 4 * body : -1;
+      sCol[tid] = ok ? P.fem_block_offset + fv : -1;
+      sZero[tid] = !ok || P.body_fixed[body] != 0 || P.vertex_fixed[fv] != 0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) sX[3 * tid + m] = ok ? P.xbar[3 * (long long)av + m] : 0.0;
     }
-    const bool zero = P.body_fixed[body] != 0 || P.vertex_fixed[fv] != 0;
-    double H[9], xb[3];  // H column-major
-#pragma unroll
-    for (int q = 0; q < 9; ++q) H[q] = zero ? 0.0 : P.hess[9 * c + q];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) xb[m] = P.xbar[3 * (long long)av + m];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const long long o = 4 * c + k;
-      P.orow[o] = P.abd_block_offset + 4 * body + k;
-      P.ocol[o] = P.fem_block_offset + fv;
-#pragma unroll
-      for (int l = 0; l < 3; ++l)
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-          P.oval[9 * o + m + 3 * l] = k == 0 ? H[m + 3 * l] : xb[m] * H[(k - 1) + 3 * l];
+    __syncthreads();
+    double* ov = P.oval + 36 * c0;
+    for (int e = tid; e < 36 * nc; e += kCplThreads) {
+      const int cl = e / 36, r = e - 36 * cl, k = r / 9, q = r - 9 * k;
+      const int m = q % 3, l = q / 3;
+      double v = 0.0;
+      if (!sZero[cl]) {
+        const double h = sH[9 * cl + (k == 0 ? q : (k - 1) + 3 * l)];
+        v = k == 0 ? h : sX[3 * cl + m] * h;
+      }
+      ov[e] = v;
     }
+    for (int e = tid; e < 4 * nc; e += kCplThreads) {
+      const int cl = e >> 2;
+      P.orow[4 * c0 + e] = sRow[cl] < 0 ? -1 : sRow[cl] + (e & 3);
+      P.ocol[4 * c0 + e] = sCol[cl];
+    }
+    __syncthreads();  // the next tile overwrites the staging
   }
 }
 
@@ -119,7 +139,9 @@ __global__ void k_abd_fem_coupling(AbdFemCouplingParams P) {
 
 hipError_t launch_abd_fem_coupling(const AbdFemCouplingParams& P, hipStream_t st) {
   if (P.count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_abd_fem_coupling, dim3(grid_for(P.count)), dim3(256), 0, st, P);
+  const long long tiles = (P.count + kCplTile - 1) / kCplTile;
+  const int grid = (int)std::min<long long>(tiles, 8192);
+  hipLaunchKernelGGL(k_abd_fem_coupling, dim3(grid), dim3(kCplThreads), 0, st, P);
   return hipGetLastError();
 }
 

@@ -1801,10 +1801,26 @@ int32_t hsc_abd_fem_coupling(const int32_t* abd_vertex, const int32_t* fem_verte
       (long long)fem_block_offset + n_fem_vertices > INT32_MAX)
     return HSC_ERR_CAPACITY;
   hipStream_t st = (hipStream_t)stream;
-  int* err = nullptr;
-  int herr = 0;
-  hipError_t e = hipMallocAsync((void**)&err, sizeof(int), st);
-  if (e == hipSuccess) e = hipMemsetAsync(err, 0, sizeof(int), st);
+  // the index-error word: one device int and one pinned host int per device,
+  // allocated on the first call and kept; calls are serialised by the lock
+  static std::mutex mu;
+  static std::map<int, std::pair<int*, int*>> flags;
+  std::lock_guard<std::mutex> lock(mu);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return HSC_ERR_HIP;
+  auto& f = flags[dev];
+  if (!f.first) {
+    e = hipMalloc((void**)&f.first, sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&f.second, sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      if (f.first) (void)hipFree(f.first);
+      f = {nullptr, nullptr};
+      return HSC_ERR_HIP;
+    }
+  }
+  int* err = f.first;
+  e = hipMemsetAsync(err, 0, sizeof(int), st);
   if (e == hipSuccess) {
     hsc::AbdFemCouplingParams P{abd_vertex,     fem_vertex,     hess,     v2b,      xbar,
                                 body_fixed,     vertex_fixed,   count,    n_abd_vertices,
@@ -1812,12 +1828,11 @@ int32_t hsc_abd_fem_coupling(const int32_t* abd_vertex, const int32_t* fem_verte
                                 fem_block_offset, err,          out_rows, out_cols, out_values};
     e = hsc::launch_abd_fem_coupling(P, st);
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st);
-  if (err) (void)hipFreeAsync(err, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(f.second, err, sizeof(int), hipMemcpyDeviceToHost, st);
   const hipError_t se = hipStreamSynchronize(st);
   if (e == hipSuccess) e = se;
   if (e != hipSuccess) return HSC_ERR_HIP;
-  return herr ? HSC_ERR_INVALID_ARGUMENT : HSC_OK;
+  return *f.second ? HSC_ERR_INVALID_ARGUMENT : HSC_OK;
 }
 
 int32_t hsc_assemble_bcoo(const int32_t* rows, const int32_t* cols, const double* values, int64_t count,

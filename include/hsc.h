@@ -221,7 +221,8 @@ int32_t hsc_fem_hessian(const int32_t* tets, const double* x, const double* Dm_i
  * v2b, xbar: [n_abd_vertices](x3); body_fixed: [n_bodies]; vertex_fixed:
  * [n_fem_vertices]; outputs: 4 * count triplets, the E blocks of the plugin's
  * input for hsc_assemble_bcoo.  An index out of range returns
- * HSC_ERR_INVALID_ARGUMENT.  Device memory; synchronises `stream`. */
+ * HSC_ERR_INVALID_ARGUMENT.  Device memory; synchronises `stream`; calls in
+ * one process are serialised (they share a per-device error word). */
 int32_t hsc_abd_fem_coupling(const int32_t* abd_vertex, const int32_t* fem_vertex, const double* hess,
                              int64_t count, const int32_t* v2b, const double* xbar, int32_t n_abd_vertices,
                              const int32_t* body_fixed, int32_t n_bodies, const int32_t* vertex_fixed,
