@@ -42,39 +42,39 @@ __global__ void k_asm_heads(const u64* __restrict__ keys, long long count, int* 
     flag[j] = keys[j] != kDropKey && (j == 0 || keys[j] != keys[j - 1]) ? 1 : 0;
 }
 
-// each head sums its run of equal keys in input order (a folded lower block
-// transposed: column-major v[r + 3c] lands on (c, r))
-__global__ void k_asm_reduce(const u64* __restrict__ keys, const int* __restrict__ idx,
-                             const int* __restrict__ flag, const int* __restrict__ pos, long long count,
-                             const int* __restrict__ rows, const int* __restrict__ cols,
-                             const double* __restrict__ vals, int* __restrict__ orow, int* __restrict__ ocol,
-                             double* __restrict__ ovals) {
-  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < count;
-       j += (long long)gridDim.x * blockDim.x) {
+// each head sums its run of equal keys in input order, one lane per value
+// component (9 lanes per sorted entry, kRedHeads entries per workgroup), so
+// every sum keeps the sequential order and the 9 lanes read each 72-byte block
+// contiguously (a folded lower block is transposed: component o = c + 3r of
+// the output takes r + 3c of the input)
+constexpr int kRedHeads = 64;
+
+__global__ __launch_bounds__(9 * kRedHeads) void k_asm_reduce(
+    const u64* __restrict__ keys, const int* __restrict__ idx, const int* __restrict__ flag,
+    const int* __restrict__ pos, long long count, const int* __restrict__ rows, const int* __restrict__ cols,
+    const double* __restrict__ vals, int* __restrict__ orow, int* __restrict__ ocol, double* __restrict__ ovals) {
+  const int h = threadIdx.x / 9, o = threadIdx.x - 9 * h;
+  const int ot = o / 3 + 3 * (o % 3);
+  for (long long j = (long long)blockIdx.x * kRedHeads + h; j < count; j += (long long)gridDim.x * kRedHeads) {
     if (!flag[j]) continue;
     const u64 key = keys[j];
-    double s[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) s[q] = 0.0;
+    double s = 0.0;
     for (long long e = j; e < count && keys[e] == key; ++e) {
       const int i = idx[e];
-      const double* v = vals + 9 * (size_t)i;
-      if (rows[i] > cols[i]) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int r = 0; r < 3; ++r) s[c + 3 * r] += v[r + 3 * c];
-      } else {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) s[q] += v[q];
-      }
+      s += vals[9 * (size_t)i + (rows[i] > cols[i] ? ot : o)];
     }
     const int p = pos[j];
-    orow[p] = (int)(key >> 32);
-    ocol[p] = (int)(key & 0xffffffffull);
-#pragma unroll
-    for (int q = 0; q < 9; ++q) ovals[9 * (size_t)p + q] = s[q];
+    ovals[9 * (size_t)p + o] = s;
+    if (o == 0) {
+      orow[p] = (int)(key >> 32);
+      ocol[p] = (int)(key & 0xffffffffull);
+    }
   }
+}
+
+// number of kept blocks = exclusive-scan total (one thread)
+__global__ void k_asm_count(const int* __restrict__ flag, const int* __restrict__ pos, int n, int* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = pos[n - 1] + flag[n - 1];
 }
 
 int grid_for(long long n) { return (int)std::min<long long>((n + 255) / 256, 4096); }
@@ -157,15 +157,23 @@ hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, l
   size_t sort_bytes = 0, scan_bytes = 0;
   hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, i0, i1, n, 0, 64, st);
   if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flag, pos, n, st);
-  const size_t tmp_bytes = std::max(sort_bytes, scan_bytes);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&k0, sizeof(u64) * n, st);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&k1, sizeof(u64) * n, st);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&i0, sizeof(int) * n, st);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&i1, sizeof(int) * n, st);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&flag, sizeof(int) * n, st);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&pos, sizeof(int) * n, st);
-  if (e == hipSuccess) e = hipMallocAsync(&tmp, std::max<size_t>(tmp_bytes, 16), st);
-  int last[2] = {0, 0};
+  const size_t tmp_bytes = std::max<size_t>(std::max(sort_bytes, scan_bytes), 16);
+  // one arena for the keys, indices, flags, positions and hipcub's workspace
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t kb = up(sizeof(u64) * n), ib = up(sizeof(int) * n);
+  char* arena = nullptr;
+  if (e == hipSuccess) e = hipMallocAsync((void**)&arena, 2 * kb + 4 * ib + up(tmp_bytes) + 256, st);
+  if (e == hipSuccess) {
+    k0 = (u64*)arena;
+    k1 = (u64*)(arena + kb);
+    i0 = (int*)(arena + 2 * kb);
+    i1 = (int*)(arena + 2 * kb + ib);
+    flag = (int*)(arena + 2 * kb + 2 * ib);
+    pos = (int*)(arena + 2 * kb + 3 * ib);
+    tmp = arena + 2 * kb + 4 * ib;
+  }
+  int* dcount = arena ? (int*)(arena + 2 * kb + 4 * ib + up(tmp_bytes)) : nullptr;
+  int hcount = 0;
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_asm_keys, dim3(grid_for(n)), dim3(256), 0, st, rows, cols, count, fold, k0, i0);
     e = hipGetLastError();
@@ -177,17 +185,20 @@ hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, l
   }
   if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, flag, pos, n, st);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_asm_reduce, dim3(grid_for(n)), dim3(256), 0, st, k1, i1, flag, pos, count, rows, cols,
+    const int grid = (int)std::min<long long>((count + kRedHeads - 1) / kRedHeads, 8192);
+    hipLaunchKernelGGL(k_asm_reduce, dim3(grid), dim3(9 * kRedHeads), 0, st, k1, i1, flag, pos, count, rows, cols,
                        vals, orow, ocol, ovals);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(&last[0], pos + n - 1, sizeof(int), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(&last[1], flag + n - 1, sizeof(int), hipMemcpyDeviceToHost, st);
-  for (void* p : {(void*)k0, (void*)k1, (void*)i0, (void*)i1, (void*)flag, (void*)pos, tmp})
-    if (p) (void)hipFreeAsync(p, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_asm_count, dim3(1), dim3(64), 0, st, flag, pos, n, dcount);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&hcount, dcount, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (arena) (void)hipFreeAsync(arena, st);
   const hipError_t se = hipStreamSynchronize(st);
   if (e == hipSuccess) e = se;
-  if (e == hipSuccess) *out_count = (long long)last[0] + last[1];
+  if (e == hipSuccess) *out_count = hcount;
   return e;
 }
 
