@@ -26,6 +26,15 @@ constexpr int kBreakdown = 7, kNotConverged = 8;
 __device__ __forceinline__ double combine(double z, double p, double beta) { return fma(beta, p, z); }
 __device__ __forceinline__ float combine(float z, float p, float beta) { return fmaf(beta, p, z); }
 
+// The same pointer in the global address space: loads through it compile to
+// global_load (vector memory counter only) instead of flat_load, which the
+// compiler emits when a pointer's origin (a select between buffers, a struct
+// field) hides that it is global, and which also holds LDS waits back.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
 template <typename MT> __device__ __forceinline__ const MT* mat_vals(const SysDesc& S);
 template <> __device__ __forceinline__ const double* mat_vals<double>(const SysDesc& S) { return S.val64; }
 template <> __device__ __forceinline__ const float* mat_vals<float>(const SysDesc& S) { return S.val32; }
@@ -47,14 +56,16 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
                                          int lane, Src src, MT& a0, MT& a1, MT& a2) {
   a0 = a1 = a2 = MT(0);
   if (row < S.nb) {
-    const int p0 = S.rowptr[row], p1 = S.rowptr[row + 1];
+    const auto rowptr = gbl(S.rowptr);
+    const auto colidx = gbl(S.colidx);
+    const int p0 = rowptr[row], p1 = rowptr[row + 1];
     const int nnzb = S.nnzb;
     if constexpr (U == 1) {
       for (int p = p0 + lane; p < p1; p += G) {
-        const int c = S.colidx[p];
+        const int c = colidx[p];
         MT x0, x1, x2;
         src(c, x0, x1, x2);
-        const MT* v = val + p;
+        const auto v = gbl(val) + p;
         const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
         const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
         const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
@@ -68,14 +79,15 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           pp[u] = min(pb + u * G, p1 - 1);  // rows are never empty (diagonal block)
-          cc[u] = S.colidx[pp[u]];
+          cc[u] = colidx[pp[u]];
         }
         MT x[U][3], v[U][9];
+        const auto vg = gbl(val);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           src(cc[u], x[u][0], x[u][1], x[u][2]);
 #pragma unroll
-          for (int e = 0; e < 9; ++e) v[u][e] = val[pp[u] + (size_t)e * nnzb];
+          for (int e = 0; e < 9; ++e) v[u][e] = vg[pp[u] + (size_t)e * nnzb];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -232,11 +244,12 @@ __device__ __forceinline__ void row_spmv_res(const SysDesc& S, const MT* __restr
       }
     }
     const int nnzb = S.nnzb;
+    const auto colidx = gbl(S.colidx);
     for (int p = rr.p0 + lane + U * G; p < rr.p1; p += G) {
-      const int c = S.colidx[p];
+      const int c = colidx[p];
       MT x0, x1, x2;
       src(c, x0, x1, x2);
-      const MT* v = val + p;
+      const auto v = gbl(val) + p;
       const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
       const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
       const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
@@ -267,18 +280,20 @@ __device__ __forceinline__ void tile_N_res(const SysDesc& S, int row, const RowR
   const bool own = row < S.nb && lane < 3;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0;
   VT zo = VT(0);
+  const auto zg = gbl(zin);
   if (own) {
-    r0 = S.r[3 * row];
-    r1 = S.r[3 * row + 1];
-    r2 = S.r[3 * row + 2];
-    zo = zin[3 * row + lane];
+    const auto rg = gbl(S.r);
+    r0 = rg[3 * row];
+    r1 = rg[3 * row + 1];
+    r2 = rg[3 * row + 2];
+    zo = zg[3 * row + lane];
   }
   VT a0, a1, a2;
   row_spmv_res<G, VT>(S, mat_vals<VT>(S), rr, lane,
                       [&](int c, VT& x0, VT& x1, VT& x2) {
-                        x0 = zin[3 * c];
-                        x1 = zin[3 * c + 1];
-                        x2 = zin[3 * c + 2];
+                        x0 = zg[3 * c];
+                        x1 = zg[3 * c + 1];
+                        x2 = zg[3 * c + 2];
                       },
                       a0, a1, a2);
   if (own) {
@@ -297,16 +312,18 @@ __device__ __forceinline__ void tile_A_res(const SysDesc& S, int row, const RowR
   const int lane = threadIdx.x % G;
   const bool own = row < S.nb && lane < 3;
   VT zo = VT(0), po = VT(0);
+  const auto zg = gbl(z);
+  const auto pg = gbl(pold);
   if (own) {
-    zo = z[3 * row + lane];
-    po = pold[3 * row + lane];
+    zo = zg[3 * row + lane];
+    po = pg[3 * row + lane];
   }
   VT a0, a1, a2;
   row_spmv_res<G, VT>(S, mat_vals<VT>(S), rr, lane,
                       [&](int c, VT& x0, VT& x1, VT& x2) {
-                        x0 = combine(z[3 * c], pold[3 * c], beta);
-                        x1 = combine(z[3 * c + 1], pold[3 * c + 1], beta);
-                        x2 = combine(z[3 * c + 2], pold[3 * c + 2], beta);
+                        x0 = combine(zg[3 * c], pg[3 * c], beta);
+                        x1 = combine(zg[3 * c + 1], pg[3 * c + 1], beta);
+                        x2 = combine(zg[3 * c + 2], pg[3 * c + 2], beta);
                       },
                       a0, a1, a2);
   if (own) {
