@@ -1801,14 +1801,16 @@ int32_t hsc_abd_fem_coupling(const int32_t* abd_vertex, const int32_t* fem_verte
       (long long)fem_block_offset + n_fem_vertices > INT32_MAX)
     return HSC_ERR_CAPACITY;
   hipStream_t st = (hipStream_t)stream;
-  // the index-error word: one device int and one pinned host int per device,
-  // allocated on the first call and kept; calls are serialised by the lock
+  // the index-error word: one device int and one pinned host int per device
+  // (the device of `stream`), allocated on the first call and kept; calls are
+  // serialised by the lock
   static std::mutex mu;
   static std::map<int, std::pair<int*, int*>> flags;
   std::lock_guard<std::mutex> lock(mu);
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
+  hipDevice_t dev = 0;
+  hipError_t e = hipStreamGetDevice(st, &dev);
   if (e != hipSuccess) return HSC_ERR_HIP;
+  DeviceGuard g(dev);
   auto& f = flags[dev];
   if (!f.first) {
     e = hipMalloc((void**)&f.first, sizeof(int));
