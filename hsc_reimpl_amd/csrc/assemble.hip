@@ -42,39 +42,45 @@ __global__ void k_asm_heads(const u64* __restrict__ keys, long long count, int* 
     flag[j] = keys[j] != kDropKey && (j == 0 || keys[j] != keys[j - 1]) ? 1 : 0;
 }
 
-// each head sums its run of equal keys in input order, one lane per value
-// component (9 lanes per sorted entry, kRedHeads entries per workgroup), so
-// every sum keeps the sequential order and the 9 lanes read each 72-byte block
-// contiguously (a folded lower block is transposed: component o = c + 3r of
-// the output takes r + 3c of the input)
+// the sorted positions of the kept blocks' first entries, in output order
+// (heads[pos[j]] = j), and their count (the exclusive-scan total)
+__global__ void k_asm_compact(const int* __restrict__ flag, const int* __restrict__ pos, long long count,
+                              int* __restrict__ heads, int* __restrict__ nheads) {
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < count;
+       j += (long long)gridDim.x * blockDim.x) {
+    if (flag[j]) heads[pos[j]] = (int)j;
+    if (j == count - 1) nheads[0] = pos[j] + flag[j];
+  }
+}
+
+// each kept block sums its run of equal keys in input order, one lane per
+// value component (9 lanes per block, kRedHeads blocks per workgroup), so
+// every sum keeps the sequential order and the 9 lanes read each 72-byte
+// input block contiguously (a folded lower block is transposed: component
+// o = c + 3r of the output takes r + 3c of the input)
 constexpr int kRedHeads = 64;
 
 __global__ __launch_bounds__(9 * kRedHeads) void k_asm_reduce(
-    const u64* __restrict__ keys, const int* __restrict__ idx, const int* __restrict__ flag,
-    const int* __restrict__ pos, long long count, const int* __restrict__ rows, const int* __restrict__ cols,
+    const int* __restrict__ heads, const int* __restrict__ nheads, const u64* __restrict__ keys,
+    const int* __restrict__ idx, long long count, const int* __restrict__ rows, const int* __restrict__ cols,
     const double* __restrict__ vals, int* __restrict__ orow, int* __restrict__ ocol, double* __restrict__ ovals) {
   const int h = threadIdx.x / 9, o = threadIdx.x - 9 * h;
   const int ot = o / 3 + 3 * (o % 3);
-  for (long long j = (long long)blockIdx.x * kRedHeads + h; j < count; j += (long long)gridDim.x * kRedHeads) {
-    if (!flag[j]) continue;
+  const long long nh = nheads[0];
+  for (long long t = (long long)blockIdx.x * kRedHeads + h; t < nh; t += (long long)gridDim.x * kRedHeads) {
+    const long long j = heads[t];
     const u64 key = keys[j];
     double s = 0.0;
     for (long long e = j; e < count && keys[e] == key; ++e) {
       const int i = idx[e];
       s += vals[9 * (size_t)i + (rows[i] > cols[i] ? ot : o)];
     }
-    const int p = pos[j];
-    ovals[9 * (size_t)p + o] = s;
+    ovals[9 * (size_t)t + o] = s;
     if (o == 0) {
-      orow[p] = (int)(key >> 32);
-      ocol[p] = (int)(key & 0xffffffffull);
+      orow[t] = (int)(key >> 32);
+      ocol[t] = (int)(key & 0xffffffffull);
     }
   }
-}
-
-// number of kept blocks = exclusive-scan total (one thread)
-__global__ void k_asm_count(const int* __restrict__ flag, const int* __restrict__ pos, int n, int* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = pos[n - 1] + flag[n - 1];
 }
 
 int grid_for(long long n) { return (int)std::min<long long>((n + 255) / 256, 4096); }
@@ -162,7 +168,7 @@ hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, l
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t kb = up(sizeof(u64) * n), ib = up(sizeof(int) * n);
   char* arena = nullptr;
-  if (e == hipSuccess) e = hipMallocAsync((void**)&arena, 2 * kb + 4 * ib + up(tmp_bytes) + 256, st);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&arena, 2 * kb + 5 * ib + up(tmp_bytes) + 256, st);
   if (e == hipSuccess) {
     k0 = (u64*)arena;
     k1 = (u64*)(arena + kb);
@@ -172,7 +178,8 @@ hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, l
     pos = (int*)(arena + 2 * kb + 3 * ib);
     tmp = arena + 2 * kb + 4 * ib;
   }
-  int* dcount = arena ? (int*)(arena + 2 * kb + 4 * ib + up(tmp_bytes)) : nullptr;
+  int* heads = arena ? (int*)(arena + 2 * kb + 4 * ib + up(tmp_bytes)) : nullptr;
+  int* dcount = arena ? (int*)(arena + 2 * kb + 5 * ib + up(tmp_bytes)) : nullptr;
   int hcount = 0;
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_asm_keys, dim3(grid_for(n)), dim3(256), 0, st, rows, cols, count, fold, k0, i0);
@@ -185,13 +192,13 @@ hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, l
   }
   if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, flag, pos, n, st);
   if (e == hipSuccess) {
-    const int grid = (int)std::min<long long>((count + kRedHeads - 1) / kRedHeads, 8192);
-    hipLaunchKernelGGL(k_asm_reduce, dim3(grid), dim3(9 * kRedHeads), 0, st, k1, i1, flag, pos, count, rows, cols,
-                       vals, orow, ocol, ovals);
+    hipLaunchKernelGGL(k_asm_compact, dim3(grid_for(n)), dim3(256), 0, st, flag, pos, count, heads, dcount);
     e = hipGetLastError();
   }
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_asm_count, dim3(1), dim3(64), 0, st, flag, pos, n, dcount);
+    const int grid = (int)std::min<long long>((count + kRedHeads - 1) / kRedHeads, 1024);
+    hipLaunchKernelGGL(k_asm_reduce, dim3(grid), dim3(9 * kRedHeads), 0, st, heads, dcount, k1, i1, count, rows,
+                       cols, vals, orow, ocol, ovals);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(&hcount, dcount, sizeof(int), hipMemcpyDeviceToHost, st);

@@ -35,6 +35,16 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const 
   return (const __attribute__((address_space(1))) T*)p;
 }
 
+// gbl() when the kernel's values are FP64 (MT = double), the pointer as it is
+// otherwise: FP64 solves measured 5% faster with global loads in the row
+// SpMVs, passes and stage A; the FP32 persistent kernel gained nothing (1-5%
+// slower in the runs measured), so its passes keep the compiler's choice
+template <typename MT, typename T>
+__device__ __forceinline__ auto gsel(const T* p) {
+  if constexpr (sizeof(MT) == 8) return gbl(p);
+  else return p;
+}
+
 template <typename MT> __device__ __forceinline__ const MT* mat_vals(const SysDesc& S);
 template <> __device__ __forceinline__ const double* mat_vals<double>(const SysDesc& S) { return S.val64; }
 template <> __device__ __forceinline__ const float* mat_vals<float>(const SysDesc& S) { return S.val32; }
@@ -56,8 +66,8 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
                                          int lane, Src src, MT& a0, MT& a1, MT& a2) {
   a0 = a1 = a2 = MT(0);
   if (row < S.nb) {
-    const auto rowptr = gbl(S.rowptr);
-    const auto colidx = gbl(S.colidx);
+    const auto rowptr = gsel<MT>(S.rowptr);
+    const auto colidx = gsel<MT>(S.colidx);
     const int p0 = rowptr[row], p1 = rowptr[row + 1];
     const int nnzb = S.nnzb;
     if constexpr (U == 1) {
@@ -65,7 +75,7 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
         const int c = colidx[p];
         MT x0, x1, x2;
         src(c, x0, x1, x2);
-        const auto v = gbl(val) + p;
+        const auto v = gsel<MT>(val) + p;
         const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
         const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
         const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
@@ -82,7 +92,7 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
           cc[u] = colidx[pp[u]];
         }
         MT x[U][3], v[U][9];
-        const auto vg = gbl(val);
+        const auto vg = gsel<MT>(val);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           src(cc[u], x[u][0], x[u][1], x[u][2]);
@@ -244,12 +254,12 @@ __device__ __forceinline__ void row_spmv_res(const SysDesc& S, const MT* __restr
       }
     }
     const int nnzb = S.nnzb;
-    const auto colidx = gbl(S.colidx);
+    const auto colidx = gsel<MT>(S.colidx);
     for (int p = rr.p0 + lane + U * G; p < rr.p1; p += G) {
       const int c = colidx[p];
       MT x0, x1, x2;
       src(c, x0, x1, x2);
-      const auto v = gbl(val) + p;
+      const auto v = gsel<MT>(val) + p;
       const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
       const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
       const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
@@ -280,9 +290,9 @@ __device__ __forceinline__ void tile_N_res(const SysDesc& S, int row, const RowR
   const bool own = row < S.nb && lane < 3;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0;
   VT zo = VT(0);
-  const auto zg = gbl(zin);
+  const auto zg = gsel<VT>(zin);
   if (own) {
-    const auto rg = gbl(S.r);
+    const auto rg = gsel<VT>(S.r);
     r0 = rg[3 * row];
     r1 = rg[3 * row + 1];
     r2 = rg[3 * row + 2];
@@ -312,8 +322,8 @@ __device__ __forceinline__ void tile_A_res(const SysDesc& S, int row, const RowR
   const int lane = threadIdx.x % G;
   const bool own = row < S.nb && lane < 3;
   VT zo = VT(0), po = VT(0);
-  const auto zg = gbl(z);
-  const auto pg = gbl(pold);
+  const auto zg = gsel<VT>(z);
+  const auto pg = gsel<VT>(pold);
   if (own) {
     zo = zg[3 * row + lane];
     po = pg[3 * row + lane];
