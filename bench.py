@@ -244,6 +244,39 @@ def coupling_timing(dev, reps=7, sample=256):
     return out
 
 
+def newton_replay(sysm, dev):
+    """The Newton step's use of the direction (hsc_reimpl_amd.replay) on the
+    78k-tet system: plain FP64 and the new solver at rtol 1e-8 and at
+    libuipc's default tol_rate 1e-3 -- descent, full residual, the model line
+    search's alpha and the model decrease as a fraction of the exact Newton
+    step's.  Exact step: a plain FP64 solve at rtol 1e-12 (SciPy's direct
+    solve if that does not converge)."""
+    import numpy as np
+    import torch
+    from hsc_reimpl_amd import plugin, replay
+    A = plugin.BCOO3.from_system(sysm, dev)
+    b = torch.as_tensor(sysm.b, device=dev)
+    h = plugin.Handle(plugin.make_config({"linear_system/tol_rate": 1e-12, "linear_system/hsc/max_iter": 20000}))
+    h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
+    x = torch.zeros_like(b)
+    st = h.solve(b, x, check=False)
+    exact = {"how": "plain FP64 solve, rtol 1e-12", "status": st["status_name"],
+             "full_relative_residual": st["full_relative_residual"]}
+    x_ref = x.cpu().numpy()
+    if st["status"] != 0:
+        import scipy.sparse.linalg as spla
+        x_ref = spla.spsolve(sysm.sparse().tocsc(), np.asarray(sysm.b))
+        exact = {"how": "SciPy spsolve (the rtol 1e-12 solve ended with " + st["status_name"] + ")"}
+    keep = ("status", "pcg_iterations", "descent_ok", "residual_ok", "full_relative_residual", "alpha", "trials",
+            "accepted", "decrease_vs_exact")
+    rows = {}
+    for tol in (1e-8, 1e-3):
+        for r in replay.replay(sysm, ["plain_fp64", "new"], {"linear_system/tol_rate": tol}, dev, x_exact=x_ref):
+            rows[f"{r['variant']} @ tol {tol:g}"] = {k: r[k] for k in keep}
+    return {"exact_step": exact, "directions": rows,
+            "not_replayed": "feasible-step / CCD / CFL caps and the true energy (scene geometry)"}
+
+
 def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
     """Oracle (sparse NumPy/SciPy restatement of the reference algorithm):
     full solves (setup + PCG + back-substitution) of the same system, single
@@ -504,6 +537,7 @@ def main():
     assembly = assemble_timing(sysm, dev)
     generator = fem_generator_timing(dev)
     coupling_t = coupling_timing(dev)
+    replay_r = newton_replay(sysm, dev)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sysm, order, omega, rtol)
@@ -544,6 +578,7 @@ def main():
         "assembly_before_path": assembly,
         "fem_hessian_generator": generator,
         "abd_fem_coupling": coupling_t,
+        "newton_replay": replay_r,
     }
     print(json.dumps(line), flush=True)
 
