@@ -277,25 +277,98 @@ def newton_replay(sysm, dev):
             "not_replayed": "feasible-step / CCD / CFL caps and the true energy (scene geometry)"}
 
 
-def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
-    """Oracle (sparse NumPy/SciPy restatement of the reference algorithm):
-    full solves (setup + PCG + back-substitution) of the same system, single
-    thread, repeated until about `budget_s` of CPU work; median rate."""
+def _sparse_solves(sysm, order, omega, rtol, budget_s, max_solves):
     from oracle import sparse as osp
     import numpy as np
     n = sysm.fn
-    times, iters = [], 0
-    while len(times) < max_solves and sum(times) < budget_s:
+    setup, solve, iters = [], [], 0
+    while len(setup) < max_solves and sum(setup) + sum(solve) < budget_s:
         t0 = time.perf_counter()
         ref = osp.SparseHSC.from_system(sysm, order=order, omega=omega)
+        t1 = time.perf_counter()
         _, _, st = ref.solve(sysm.b[:n], sysm.b[n:], rtol=rtol, maxiter=20000)
-        times.append(time.perf_counter() - t0)
+        t2 = time.perf_counter()
+        setup.append(t1 - t0)
+        solve.append(t2 - t1)
         iters = st["iterations"]
-    med = float(np.median(times))
-    return {"value": 1.0 / med, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"{len(times)} full solves (setup + {iters} PCG iterations each) of the same 78k-tet system, "
-                      f"median {med:.2f} s, {sum(times):.1f} s total; SciPy sparse matvecs, BLAS pinned to 1 thread",
-            "iterations": iters, "cpu": platform.processor() or platform.machine()}
+    tot = [a + b for a, b in zip(setup, solve)]
+    return float(np.median(tot)), float(np.median(setup)), float(np.median(solve)), len(tot), sum(tot), iters
+
+
+def _dense_c1(reps=5):
+    """The dense restatement (core.py's HSC: dense D, the Cholesky check of
+    the Schur complement, exact A^-1 as the plugin path) on the 2,178-DOF
+    system: ACA tol 1e-6, Neumann order 2, omega 0.6 (0.7 makes the order-2
+    series indefinite on it), rtol 1e-8; construction
+    and solve timed separately, median of `reps` after one warm-up."""
+    from hsc_reimpl_amd import systems
+    from oracle import hsc_core
+    import numpy as np
+    s = systems.cube_system(seed=0)
+    H = s.dense()
+    n = s.fn
+    D, A, E = H[:n, :n], H[n:, n:], H[n:, :n]
+    setup, solve, iters = [], [], 0
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        hsc = hsc_core.HSC(D, A, E, order=2, omega=0.6, aca_rtol=1e-6, exact_affine=True)
+        t1 = time.perf_counter()
+        _, st = hsc.solve(s.b, rtol=1e-8, maxiter=5000)
+        t2 = time.perf_counter()
+        if r:
+            setup.append(t1 - t0)
+            solve.append(t2 - t1)
+        iters = st["iterations"]
+    return {"threads": int(os.environ.get("OPENBLAS_NUM_THREADS", "0")),
+            "construct_s": round(float(np.median(setup)), 4),
+            "solve_s": round(float(np.median(solve)), 4), "pcg_iterations": iters,
+            "solves_per_s": round(1.0 / (float(np.median(setup)) + float(np.median(solve))), 3)}
+
+
+def cpu_all_cores(order, omega, rtol, budget_s=5.0, max_solves=3):
+    """The CPU measurements with the BLAS / OpenMP pools at the process's
+    thread setting (a child process started with OMP_NUM_THREADS,
+    OPENBLAS_NUM_THREADS and MKL_NUM_THREADS set before NumPy loads): the
+    sparse restatement on the 78k-tet system and the dense one on the
+    2,178-DOF system.  CPU only; nothing here touches the GPU."""
+    from hsc_reimpl_amd import systems
+    sysm = systems.torus_system(seed=1)
+    med, m_setup, m_solve, k, _, _ = _sparse_solves(sysm, order, omega, rtol, budget_s, max_solves)
+    return {"threads": int(os.environ.get("OPENBLAS_NUM_THREADS", "0")), "solves_per_s": round(1.0 / med, 4),
+            "setup_s": round(m_setup, 3), "solve_s": round(m_solve, 3), "solves": k,
+            "note": "SciPy's sparse products run on one thread whatever the BLAS pool",
+            "dense_2178_dof": _dense_c1()}
+
+
+def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
+    """Oracle (sparse NumPy/SciPy restatement of the reference algorithm):
+    full solves (setup + PCG + back-substitution) of the same system, single
+    thread, repeated until about `budget_s` of CPU work; median rate (the
+    reported value).  Beside it: the dense core.py-form restatement on the
+    2,178-DOF system at one thread, and both again with the thread pools at
+    the box's CPU share, in a child process whose pools are set before NumPy
+    loads (this process keeps its single-thread pools)."""
+    import subprocess
+    import numpy as np
+    med, m_setup, m_solve, k, tot, iters = _sparse_solves(sysm, order, omega, rtol, budget_s, max_solves)
+    out = {"value": 1.0 / med, "unit": "solves/s", "cores": 1, "kind": "port",
+           "sample": f"{k} full solves (setup + {iters} PCG iterations each) of the same 78k-tet system, "
+                     f"median {med:.2f} s, {tot:.1f} s total; SciPy sparse matvecs, BLAS pinned to 1 thread",
+           "iterations": iters, "cpu": platform.processor() or platform.machine(),
+           "setup_s": round(m_setup, 3), "solve_s": round(m_solve, 3),
+           "dense_2178_dof_one_thread": dict(_dense_c1(), what="core.py-form dense restatement "
+                                                               "(oracle.hsc_core.HSC) on the 2,178-DOF system")}
+    allc = max(1, min(os.cpu_count() or 1, 16))
+    env = dict(os.environ, OMP_NUM_THREADS=str(allc), OPENBLAS_NUM_THREADS=str(allc), MKL_NUM_THREADS=str(allc))
+    try:
+        res = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-only"], env=env,
+                             capture_output=True, text=True, timeout=240)
+        line = [x for x in res.stdout.splitlines() if x.startswith("{")]
+        out["all_cores"] = json.loads(line[-1]) if res.returncode == 0 and line else {
+            "threads": allc, "error": f"exit {res.returncode}: {res.stderr[-300:]}"}
+    except subprocess.TimeoutExpired:
+        out["all_cores"] = {"threads": allc, "error": "timed out after 240 s"}
+    return out
 
 
 def run_batch64(args, dev, order, omega, rtol):
@@ -406,7 +479,13 @@ def main():
     ap.add_argument("--wg-per-cu", type=int, default=1)
     ap.add_argument("--schedule", type=int, default=1)
     ap.add_argument("--preconditioner", default="chebyshev", choices=["neumann", "chebyshev"])
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="print the CPU baseline's measurements at the inherited thread setting and exit "
+                         "(the default run starts this as a child process; no GPU use)")
     args = ap.parse_args()
+    if args.cpu_only:
+        print(json.dumps(cpu_all_cores(5, 0.7, 1e-8)), flush=True)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: the HSC solve of one system runs on one MI355X and no "
               "multi-GPU figure is produced", file=sys.stderr)
