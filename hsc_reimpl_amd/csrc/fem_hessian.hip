@@ -15,20 +15,31 @@ namespace hsc {
 
 namespace {
 
-constexpr int kFemBlock = 32;  // tets per workgroup (the 9x9 matrices live in LDS)
+constexpr int kFemBlock = 64;  // tets per workgroup (one wave)
 
 __device__ __forceinline__ double det3(const double F[9]) {  // column-major F[i + 3j]
   return F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) + F[6] * (F[1] * F[5] - F[4] * F[2]);
 }
 
-__global__ __launch_bounds__(kFemBlock) void k_fem_hessian(FemHessianParams P) {
-  __shared__ double sH[81][kFemBlock];
-  __shared__ double sQ[81][kFemBlock];
+// packed upper-triangle index of the symmetric 9x9 entry (r, c)
+__host__ __device__ constexpr int pk(int r, int c) {
+  return r <= c ? r * 9 - r * (r - 1) / 2 + (c - r) : c * 9 - c * (c - 1) / 2 + (r - c);
+}
+
+// The 9x9 Hessian lives in LDS as its 45 upper-triangle entries (one column
+// of sH per tet); the Jacobi eigenvectors Q live in registers, every loop
+// over them unrolled so that all indices are compile-time constants.  23 KB
+// of LDS per 64-tet workgroup and at most 256 VGPRs (two waves per SIMD, a
+// few values spilled) put 512 tets in flight per CU, which is what hides the
+// latency of the rotations' dependent LDS updates (measured on 78,400 tets:
+// 1.09 ms with both matrices in LDS and 96 tets per CU, 0.32 ms at one wave
+// per SIMD, 0.25 ms at two).
+__global__ __launch_bounds__(kFemBlock, 2) void k_fem_hessian(FemHessianParams P) {
+  __shared__ double sH[45][kFemBlock];
   const int t = threadIdx.x;
   const long long e = (long long)blockIdx.x * kFemBlock + t;
   if (e >= P.ntet) return;  // (no workgroup barrier below)
-  auto H = [&](int r, int c) -> double& { return sH[r + 9 * c][t]; };
-  auto Q = [&](int r, int c) -> double& { return sQ[r + 9 * c][t]; };
+  auto H = [&](int r, int c) -> double& { return sH[pk(r, c)][t]; };
   int v[4];
   for (int a = 0; a < 4; ++a) v[a] = P.tets[4 * e + a];
   const double* Di = P.Dm_inv + 9 * e;  // column-major Dm^-1
@@ -51,24 +62,34 @@ __global__ __launch_bounds__(kFemBlock) void k_fem_hessian(FemHessianParams P) {
       G[i + 3 * j] = (F[i1 + 3 * j1] * F[i2 + 3 * j2] - F[i1 + 3 * j2] * F[i2 + 3 * j1]) / J;
     }
   const double q = lambda * log(J) - mu;
-  for (int a = 0; a < 3; ++a)
-    for (int i = 0; i < 3; ++i)
-      for (int b = 0; b < 3; ++b)
-        for (int j = 0; j < 3; ++j)
-          H(i + 3 * a, j + 3 * b) =
-              mu * (i == j && a == b ? 1.0 : 0.0) + lambda * G[i + 3 * a] * G[j + 3 * b] - q * G[i + 3 * b] * G[j + 3 * a];
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int c = r; c < 9; ++c) {
+      const int i = r % 3, a = r / 3, j = c % 3, b = c / 3;
+      H(r, c) = mu * (i == j && a == b ? 1.0 : 0.0) + lambda * G[i + 3 * a] * G[j + 3 * b] -
+                q * G[i + 3 * b] * G[j + 3 * a];
+    }
   if (P.project) {
     // bounded cyclic Jacobi eigendecomposition, eigenvalues clamped at 0
+    double Q[9][9];
     double nrm = 0.0;
-    for (int k = 0; k < 81; ++k) {
-      Q(k % 9, k / 9) = (k % 9 == k / 9) ? 1.0 : 0.0;
-      const double h = sH[k][t];
-      nrm += h * h;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) Q[r][c] = r == c ? 1.0 : 0.0;
+#pragma unroll
+      for (int c = r; c < 9; ++c) {
+        const double h = H(r, c);
+        nrm += (r == c ? 1.0 : 2.0) * h * h;
+      }
     }
     const double threshold = 1e-13 * sqrt(nrm);
     for (int sweep = 0; sweep < 32; ++sweep) {
       double maximum = 0.0;
+#pragma unroll
       for (int p = 0; p < 8; ++p)
+#pragma unroll
         for (int qq = p + 1; qq < 9; ++qq) {
           const double apq = H(p, qq);
           maximum = fmax(maximum, fabs(apq));
@@ -79,28 +100,30 @@ __global__ __launch_bounds__(kFemBlock) void k_fem_hessian(FemHessianParams P) {
           H(p, p) -= tt * apq;
           H(qq, qq) += tt * apq;
           H(p, qq) = 0.0;
-          H(qq, p) = 0.0;
+#pragma unroll
           for (int k = 0; k < 9; ++k) {
             if (k != p && k != qq) {
               const double kp = H(k, p), kq = H(k, qq);
               H(k, p) = c * kp - s * kq;
-              H(p, k) = H(k, p);
               H(k, qq) = s * kp + c * kq;
-              H(qq, k) = H(k, qq);
             }
-            const double qp = Q(k, p), qk = Q(k, qq);
-            Q(k, p) = c * qp - s * qk;
-            Q(k, qq) = s * qp + c * qk;
+            const double qp = Q[k][p], qk = Q[k][qq];
+            Q[k][p] = c * qp - s * qk;
+            Q[k][qq] = s * qp + c * qk;
           }
         }
       if (maximum <= threshold) break;
     }
     double w[9];
+#pragma unroll
     for (int i = 0; i < 9; ++i) w[i] = fmax(H(i, i), 0.0);
+#pragma unroll
     for (int i = 0; i < 9; ++i)
-      for (int j = 0; j < 9; ++j) {
+#pragma unroll
+      for (int j = i; j < 9; ++j) {
         double s = 0.0;
-        for (int k = 0; k < 9; ++k) s += Q(i, k) * w[k] * Q(j, k);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s += Q[i][k] * w[k] * Q[j][k];
         H(i, j) = s;  // (only Q and w are read here)
       }
   }
@@ -114,16 +137,27 @@ __global__ __launch_bounds__(kFemBlock) void k_fem_hessian(FemHessianParams P) {
     gN[0][p] = -(gN[1][p] + gN[2][p] + gN[3][p]);
   }
   long long o = 10 * e;
+#pragma unroll
   for (int i = 0; i < 4; ++i)
+#pragma unroll
     for (int j = i; j < 4; ++j) {
-      const int l = v[i] <= v[j] ? i : j, r = v[i] <= v[j] ? j : i;
-      P.orow[o] = v[l];
-      P.ocol[o] = v[r];
+      const bool keep = v[i] <= v[j];
+      P.orow[o] = keep ? v[i] : v[j];
+      P.ocol[o] = keep ? v[j] : v[i];
+      // block (l, r) with l the smaller node: (gN_l, gN_r) = keep ? (gN_i, gN_j) : (gN_j, gN_i)
+#pragma unroll
       for (int k = 0; k < 3; ++k)
+#pragma unroll
         for (int m = 0; m < 3; ++m) {
           double s = 0.0;
+#pragma unroll
           for (int p = 0; p < 3; ++p)
-            for (int qq = 0; qq < 3; ++qq) s += gN[l][p] * gN[r][qq] * H(k + 3 * p, m + 3 * qq);
+#pragma unroll
+            for (int qq = 0; qq < 3; ++qq) {
+              const double gl = keep ? gN[i][p] : gN[j][p];
+              const double gr = keep ? gN[j][qq] : gN[i][qq];
+              s += gl * gr * H(k + 3 * p, m + 3 * qq);
+            }
           P.oval[9 * o + k + 3 * m] = scale * s;  // column-major block
         }
       ++o;

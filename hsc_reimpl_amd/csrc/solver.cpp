@@ -1782,6 +1782,7 @@ int32_t hsc_fem_hessian(const int32_t* tets, const double* x, const double* Dm_i
                           out_rows, out_cols, out_values};
   hipError_t e = hsc::launch_fem_hessian(P, (hipStream_t)stream);
   if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) std::fprintf(stderr, "hsc_fem_hessian: %s\n", hipGetErrorString(e));
   return e == hipSuccess ? HSC_OK : HSC_ERR_HIP;
 }
 
@@ -1833,7 +1834,10 @@ int32_t hsc_abd_fem_coupling(const int32_t* abd_vertex, const int32_t* fem_verte
   if (e == hipSuccess) e = hipMemcpyAsync(f.second, err, sizeof(int), hipMemcpyDeviceToHost, st);
   const hipError_t se = hipStreamSynchronize(st);
   if (e == hipSuccess) e = se;
-  if (e != hipSuccess) return HSC_ERR_HIP;
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "hsc_abd_fem_coupling: %s\n", hipGetErrorString(e));
+    return HSC_ERR_HIP;
+  }
   return *f.second ? HSC_ERR_INVALID_ARGUMENT : HSC_OK;
 }
 
@@ -1846,6 +1850,7 @@ int32_t hsc_assemble_bcoo(const int32_t* rows, const int32_t* cols, const double
   long long n = 0;
   const hipError_t e = hsc::assemble_bcoo(rows, cols, values, count, fold_lower ? 1 : 0, out_rows, out_cols,
                                           out_values, &n, (hipStream_t)stream);
+  if (e != hipSuccess) std::fprintf(stderr, "hsc_assemble_bcoo: %s\n", hipGetErrorString(e));
   *out_count = n;
   return e == hipSuccess ? HSC_OK : HSC_ERR_HIP;
 }
