@@ -418,18 +418,24 @@ def _affine_jacobian(xbar):
 
 
 def contact_scene(seed=4, nx=6, ny=6, nz=3, h=0.02, E=1e8, nu=0.45, n_bodies=3, verts_per_body=40,
-                  n_contacts=600, fixed_body=None, n_fixed_vertices=0):
+                  n_contacts=600, fixed_body=None, n_fixed_vertices=0, fem_stiffness=True):
     """ABD bodies touching an FEM box through point contacts, split as the
     coupling assembly sees it: the contact list (ABD vertex, FEM vertex,
     3x3 contact block H = -K_c, the ABD-row / FEM-column block of the
     penalty 0.5 (x_a - x_f)^T K_c (x_a - x_f)) and the general triplets of
     everything else (ABD blocks m_v sum J^T J + kappa diag(0, I9) + sum
     J^T K_c J, FEM stiffness + m I + sum K_c), ABD block rows first (qo = 0,
-    fo = qm).  K_c = k_n n n^T + k_t (I - n n^T), n a random unit normal."""
+    fo = qm).  K_c = k_n n n^T + k_t (I - n n^T), n a random unit normal.
+    With fem_stiffness=False the FEM triplets hold only the mass and contact
+    diagonal terms (the caller adds the elastic Hessian, e.g. from the
+    device generator); the mesh is returned as "nodes" and "tets"."""
     rng = np.random.default_rng(seed)
     nodes, tets = box_mesh(nx, ny, nz, h)
     nf = len(nodes)
     I, J, B = fem_blocks(nodes, tets, E, nu)
+    if not fem_stiffness:
+        I = J = np.arange(nf, dtype=np.int64)
+        B = np.zeros((nf, 3, 3))
     kf = E * h
     diag = np.flatnonzero(I == J)
     pos = np.empty(nf, dtype=np.int64)
@@ -475,7 +481,7 @@ def contact_scene(seed=4, nx=6, ny=6, nz=3, h=0.02, E=1e8, nu=0.45, n_bodies=3, 
     return dict(rows=R, cols=C, vals=np.ascontiguousarray(V), abd_vertex=av, fem_vertex=fv, hess=-K,
                 v2b=v2b, xbar=xbar, body_fixed=body_fixed, vertex_fixed=vertex_fixed, qo=0, qm=qm, fo=qm,
                 fn=3 * nf, abd_block_offset=0, fem_block_offset=qm // 3, N=qm + 3 * nf,
-                b=rng.normal(size=qm + 3 * nf))
+                b=rng.normal(size=qm + 3 * nf), nodes=nodes, tets=tets)
 
 
 def from_dense(D, A, E, b):
