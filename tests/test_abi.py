@@ -67,6 +67,38 @@ def test_invalid_config_rejected(lib):
     assert lib.hsc_time_kernel(None, 0, 8, 1, None, C.byref(ms)) == _abi.HSC_ERR_INVALID_ARGUMENT
 
 
+def test_stateless_entry_points_validate_before_any_device_call(lib):
+    """hsc_fem_hessian, hsc_assemble_bcoo and hsc_abd_fem_coupling reject
+    bad sizes and null pointers, and sizes past their limits, before they
+    touch the device (the dummy pointer is never dereferenced)."""
+    buf = (C.c_double * 16)()
+    d = C.cast(buf, C.c_void_p)
+    INV, CAP = _abi.HSC_ERR_INVALID_ARGUMENT, _abi.HSC_ERR_CAPACITY
+    # FEM generator: negative count, nulls with a positive count, past 2^27 tets
+    assert lib.hsc_fem_hessian(d, d, d, d, d, d, 1.0, -1, 1, d, d, d, None) == INV
+    assert lib.hsc_fem_hessian(None, d, d, d, d, d, 1.0, 5, 1, d, d, d, None) == INV
+    assert lib.hsc_fem_hessian(d, d, d, d, d, d, 1.0, (1 << 27) + 1, 1, d, d, d, None) == CAP
+    # assembly: no count pointer, negative count, nulls, past 2^31 - 1 triplets
+    n = C.c_int64()
+    assert lib.hsc_assemble_bcoo(d, d, d, 4, 1, d, d, d, None, None) == INV
+    assert lib.hsc_assemble_bcoo(d, d, d, -1, 1, d, d, d, C.byref(n), None) == INV
+    assert lib.hsc_assemble_bcoo(None, d, d, 4, 1, d, d, d, C.byref(n), None) == INV
+    assert lib.hsc_assemble_bcoo(d, d, d, 1 << 31, 1, d, d, d, C.byref(n), None) == CAP
+    # coupling: negative sizes / offsets, nulls, past 2^28 contacts, block indices past int32
+    args = dict(av=d, fv=d, h=d, count=4, v2b=d, xb=d, nv=8, bf=d, nb=2, vf=d, nf=10, ao=0, fo=8)
+
+    def cpl(**kw):
+        a = dict(args, **kw)
+        return lib.hsc_abd_fem_coupling(a["av"], a["fv"], a["h"], a["count"], a["v2b"], a["xb"], a["nv"], a["bf"],
+                                        a["nb"], a["vf"], a["nf"], a["ao"], a["fo"], d, d, d, None)
+    for kw in ({"count": -1}, {"nv": -1}, {"nb": -1}, {"nf": -1}, {"ao": -1}, {"fo": -1}, {"av": None},
+               {"xb": None}, {"vf": None}):
+        assert cpl(**kw) == INV, kw
+    assert cpl(count=(1 << 28) + 1) == CAP
+    assert cpl(nb=1 << 30) == CAP
+    assert cpl(count=0) == _abi.HSC_OK          # nothing to do, no device call
+
+
 def test_jsonl_row_matches_reference_schema(lib, tmp_path):
     s = _abi.Stats()
     s.fem_dofs, s.abd_dofs, s.active_fem_dofs, s.rank = 74508, 144, 819, 84
