@@ -35,6 +35,21 @@ def test_matches_oracle_bitwise(fold):
     np.testing.assert_array_equal(out.values.cpu().numpy(), v)
 
 
+def test_single_triplet_and_all_dropped():
+    """One lower triplet folds onto (col, row) transposed; with fold_lower
+    off, an input of lower triplets only assembles to nothing."""
+    v = np.arange(9, dtype=np.float64)[None, :]
+    out = plugin.assemble_bcoo(_dev([5], torch.int32), _dev([2], torch.int32), _dev(v, torch.float64), 6)
+    assert out.rows.tolist() == [2] and out.cols.tolist() == [5]
+    np.testing.assert_array_equal(out.values.cpu().numpy()[0], v[0].reshape(3, 3, order="F").T.ravel(order="F"))
+    rng = np.random.default_rng(9)
+    rows = rng.integers(1, 30, 300)
+    cols = rows - rng.integers(1, rows + 1)          # strictly lower
+    out = plugin.assemble_bcoo(_dev(rows, torch.int32), _dev(cols, torch.int32),
+                               _dev(rng.normal(size=(300, 9)), torch.float64), 30, fold_lower=False)
+    assert out.rows.numel() == 0
+
+
 def test_empty_input():
     e = torch.empty(0, dtype=torch.int32, device=DEV)
     out = plugin.assemble_bcoo(e, e, torch.empty((0, 9), dtype=torch.float64, device=DEV), 1)

@@ -97,6 +97,16 @@ def test_index_out_of_range_is_an_error_and_the_device_stays_usable():
     assert rows.numel() == 400 and bool(torch.isfinite(vals).all())
 
 
+def test_single_contact_matches_oracle():
+    sc = systems.contact_scene(seed=7, n_contacts=1)
+    rows, cols, vals = _device_coupling(sc, 3, 50)
+    r, c, v = coupling.coupling_blocks(sc["abd_vertex"], sc["fem_vertex"], sc["hess"], sc["v2b"], sc["xbar"],
+                                       sc["body_fixed"], sc["vertex_fixed"], 3, 50)
+    np.testing.assert_array_equal(rows.cpu().numpy(), r)
+    np.testing.assert_array_equal(cols.cpu().numpy(), c)
+    np.testing.assert_array_equal(vals.cpu().numpy(), v)
+
+
 def test_empty_input():
     sc = systems.contact_scene(seed=6, n_contacts=0)
     rows, cols, vals = _device_coupling(sc)
