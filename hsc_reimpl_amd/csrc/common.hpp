@@ -18,6 +18,19 @@ constexpr int kNbrMaxTiles = 1024;   // row tiles per system for which neighbour
 // Per-system descriptor for the persistent PCG kernel and the operator
 // kernels.  All pointers are device pointers.  VT buffers (p, z, q) hold
 // double (FP64 mode) or float (mixed mode).
+// Matrix values of the block-row storage: the 9 components of 64 consecutive
+// blocks form one chunk, component-major inside it ([p / 64][c][p % 64]), so a
+// lane group's loads of one component are contiguous (coalesced) and the 9
+// component streams of a chunk are one contiguous 4.6 KB (FP64) region.
+constexpr int kValChunk = 64;
+__host__ __device__ __forceinline__ size_t val_index(int p, int c) {
+  return (size_t)(p >> 6) * (9 * kValChunk) + (size_t)c * kValChunk + (size_t)(p & (kValChunk - 1));
+}
+// entries to allocate for nnzb blocks (the last chunk padded)
+__host__ __device__ __forceinline__ size_t val_count(long long nnzb) {
+  return (size_t)((nnzb + kValChunk - 1) / kValChunk) * 9 * kValChunk;
+}
+
 struct SysDesc {
   int nb, n, a, k, nnzb;
   // global tile ranges
@@ -34,7 +47,7 @@ struct SysDesc {
   // matrix
   const int* rowptr;
   const int* colidx;
-  const double* val64;    // SoA: component c (col-major 3x3 index) at c*nnzb + p
+  const double* val64;    // component c (col-major 3x3 index) of block p at val_index(p, c)
   const float* val32;
   const double* invD64;   // [nb*9] row-major inverse diagonal blocks
   const float* invD32;

@@ -69,16 +69,15 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
     const auto rowptr = gsel<MT>(S.rowptr);
     const auto colidx = gsel<MT>(S.colidx);
     const int p0 = rowptr[row], p1 = rowptr[row + 1];
-    const int nnzb = S.nnzb;
     if constexpr (U == 1) {
       for (int p = p0 + lane; p < p1; p += G) {
         const int c = colidx[p];
         MT x0, x1, x2;
         src(c, x0, x1, x2);
-        const auto v = gsel<MT>(val) + p;
-        const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
-        const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
-        const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
+        const auto v = gsel<MT>(val) + val_index(p, 0);
+        const MT v0 = v[0], v1 = v[kValChunk], v2 = v[2 * kValChunk];
+        const MT v3 = v[3 * kValChunk], v4 = v[4 * kValChunk], v5 = v[5 * kValChunk];
+        const MT v6 = v[6 * kValChunk], v7 = v[7 * kValChunk], v8 = v[8 * kValChunk];
         a0 += v0 * x0 + v3 * x1 + v6 * x2;
         a1 += v1 * x0 + v4 * x1 + v7 * x2;
         a2 += v2 * x0 + v5 * x1 + v8 * x2;
@@ -97,7 +96,7 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
         for (int u = 0; u < U; ++u) {
           src(cc[u], x[u][0], x[u][1], x[u][2]);
 #pragma unroll
-          for (int e = 0; e < 9; ++e) v[u][e] = vg[pp[u] + (size_t)e * nnzb];
+          for (int e = 0; e < 9; ++e) v[u][e] = vg[val_index(pp[u], e)];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -217,7 +216,6 @@ __device__ __forceinline__ void row_preload(const SysDesc& S, const MT* __restri
   if (row >= S.nb) return;
   rr.p0 = S.rowptr[row];
   rr.p1 = S.rowptr[row + 1];
-  const int nnzb = S.nnzb;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int p = rr.p0 + lane + u * G;
@@ -225,7 +223,7 @@ __device__ __forceinline__ void row_preload(const SysDesc& S, const MT* __restri
     if (p < rr.p1) {
       rr.c[u] = S.colidx[p];
 #pragma unroll
-      for (int e = 0; e < 9; ++e) rr.v[u][e] = val[p + (size_t)e * nnzb];
+      for (int e = 0; e < 9; ++e) rr.v[u][e] = val[val_index(p, e)];
     }
   }
   if (lane < 3) {
@@ -253,16 +251,15 @@ __device__ __forceinline__ void row_spmv_res(const SysDesc& S, const MT* __restr
         a2 += rr.v[u][2] * x[u][0] + rr.v[u][5] * x[u][1] + rr.v[u][8] * x[u][2];
       }
     }
-    const int nnzb = S.nnzb;
     const auto colidx = gsel<MT>(S.colidx);
     for (int p = rr.p0 + lane + U * G; p < rr.p1; p += G) {
       const int c = colidx[p];
       MT x0, x1, x2;
       src(c, x0, x1, x2);
-      const auto v = gsel<MT>(val) + p;
-      const MT v0 = v[0], v1 = v[nnzb], v2 = v[2 * nnzb];
-      const MT v3 = v[3 * nnzb], v4 = v[4 * nnzb], v5 = v[5 * nnzb];
-      const MT v6 = v[6 * nnzb], v7 = v[7 * nnzb], v8 = v[8 * nnzb];
+      const auto v = gsel<MT>(val) + val_index(p, 0);
+      const MT v0 = v[0], v1 = v[kValChunk], v2 = v[2 * kValChunk];
+      const MT v3 = v[3 * kValChunk], v4 = v[4 * kValChunk], v5 = v[5 * kValChunk];
+      const MT v6 = v[6 * kValChunk], v7 = v[7 * kValChunk], v8 = v[8 * kValChunk];
       a0 += v0 * x0 + v3 * x1 + v6 * x2;
       a1 += v1 * x0 + v4 * x1 + v7 * x2;
       a2 += v2 * x0 + v5 * x1 + v8 * x2;
@@ -1349,7 +1346,7 @@ __global__ void k_node_features(const int* rowptr, const int* colidx, const doub
   double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (pd >= 0)
     for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) B[3 * r + c] = val64[(size_t)(r + 3 * c) * nnzb + pd];
+      for (int c = 0; c < 3; ++c) B[3 * r + c] = val64[val_index(pd, r + 3 * c)];
   double s = (B[0] + B[4] + B[8]) / 3.0;
   if (!(s > 0.0)) s = 1.0;
   for (int e = 0; e < 9; ++e) Dfeat[9 * i + e] = (float)(B[e] / s);

@@ -216,8 +216,8 @@ __global__ __launch_bounds__(256) void k_sort_gather(const int* __restrict__ row
       for (int c = 0; c < 3; ++c)
         for (int r = 0; r < 3; ++r) {
           const double bv = tr ? v[c + 3 * r] : v[r + 3 * c];
-          v64[(size_t)(r + 3 * c) * nnzb + dst] = bv;
-          v32[(size_t)(r + 3 * c) * nnzb + dst] = (float)bv;
+          v64[val_index(dst, r + 3 * c)] = bv;
+          v32[val_index(dst, r + 3 * c)] = (float)bv;
         }
     }
   }

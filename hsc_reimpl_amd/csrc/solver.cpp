@@ -888,8 +888,8 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->key_col.ensure(sizeof(int) * nnzb));
   HIPCHK(h->key_src.ensure(sizeof(int) * nnzb));
   HIPCHK(h->colidx.ensure(sizeof(int) * nnzb));
-  HIPCHK(h->val64.ensure(sizeof(double) * 9 * (size_t)nnzb));
-  HIPCHK(h->val32.ensure(sizeof(float) * 9 * (size_t)nnzb));
+  HIPCHK(h->val64.ensure(sizeof(double) * hsc::val_count(nnzb)));
+  HIPCHK(h->val32.ensure(sizeof(float) * hsc::val_count(nnzb)));
   HIPCHK(h->E.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
   HIPCHK(h->R0.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
   HIPCHK(h->R1.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
