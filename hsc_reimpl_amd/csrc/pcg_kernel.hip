@@ -182,10 +182,14 @@ __device__ __forceinline__ void tile_N(const SysDesc& S, int tl, const VT* __res
 // column indices and values -- and the row's inverse diagonal block are
 // loaded once per launch and kept in registers; every Neumann pass then
 // costs one gather latency instead of the row-pointer -> index -> gather
-// chain.  Blocks beyond the first U per lane are streamed as before.
+// chain.  Blocks beyond the first U per lane are streamed as before, and a
+// lane holding such a tail adds a dependent load to the whole pass: FP32 keeps
+// 5 blocks per lane at 4 lanes per row (20 per row, the 78k-tet system's
+// longest row is 19; 251 VGPRs, no spills), which took its passes' streamed
+// tails from 4.5% of the blocks to none and the PCG launch from 3.91 to 3.60 ms.
 template <typename MT>
 struct ResidentU {
-  static constexpr int value = 4;
+  static constexpr int value = 5;
 };
 template <>
 struct ResidentU<double> {
