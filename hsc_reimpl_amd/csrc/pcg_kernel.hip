@@ -187,13 +187,15 @@ __device__ __forceinline__ void tile_N(const SysDesc& S, int tl, const VT* __res
 // 5 blocks per lane at 4 lanes per row (20 per row, the 78k-tet system's
 // longest row is 19; 251 VGPRs, no spills), which took its passes' streamed
 // tails from 4.5% of the blocks to none and the PCG launch from 3.91 to 3.60 ms.
+// FP64 keeps 3 (12 per row, 249 VGPRs; streamed tails 35% -> 17% of the
+// blocks, plain FP64 PCG 14.5 -> 13.6 ms); 4 would pass 256 VGPRs.
 template <typename MT>
 struct ResidentU {
   static constexpr int value = 5;
 };
 template <>
 struct ResidentU<double> {
-  static constexpr int value = 2;
+  static constexpr int value = 3;
 };
 
 template <int G, typename MT>
