@@ -548,6 +548,51 @@ __device__ __forceinline__ bool chunk_empty(int c, int nC, int T) {
   return tile_of_chunk(c, nC, T) == tile_of_chunk(c + 1, nC, T);
 }
 
+// reduce_chunks_all in two halves: the first kPreN chunk loads of every lane
+// issued early (so that they overlap other work), the sums later in the same
+// order as reduce_chunks_all adds them
+constexpr int kPreN = 3;
+struct ChunkPre {
+  double x[kPreN];
+  int c0, c1, s, l, L;
+  bool act;
+};
+template <typename Rng, typename Want>
+__device__ __forceinline__ void chunks_prefetch(const double* __restrict__ part, int nsys, int nC, Rng rng,
+                                                Want want, ChunkPre& pf) {
+  int L = 64;
+  while (L > 1 && L * nsys > kBlock) L >>= 1;
+  pf.L = L;
+  pf.s = threadIdx.x / L;
+  pf.l = threadIdx.x % L;
+  pf.act = pf.s < nsys && want(pf.s);
+  pf.c0 = 0;
+  pf.c1 = -1;
+  if (pf.act) rng(pf.s, pf.c0, pf.c1);
+#pragma unroll
+  for (int i = 0; i < kPreN; ++i) {
+    const int c = pf.c0 + pf.l + i * L;
+    pf.x[i] = pf.act && c <= pf.c1 ? part[(size_t)pf.s * nC + c] : 0.0;
+  }
+}
+__device__ __forceinline__ void chunks_finish(const double* __restrict__ part, int nC, int T, const ChunkPre& pf,
+                                              double* out) {
+  double v = 0.0;
+  if (pf.act) {
+#pragma unroll
+    for (int i = 0; i < kPreN; ++i) {
+      const int c = pf.c0 + pf.l + i * pf.L;
+      if (c <= pf.c1) v += chunk_empty(c, nC, T) ? 0.0 : pf.x[i];
+    }
+    for (int c = pf.c0 + pf.l + kPreN * pf.L; c <= pf.c1; c += pf.L) {
+      const double x = part[(size_t)pf.s * nC + c];
+      v += chunk_empty(c, nC, T) ? 0.0 : x;
+    }
+  }
+  for (int m = pf.L >> 1; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  if (pf.act && pf.l == 0) out[pf.s] = v;
+}
+
 // Per-system sums of chunk partials for every system at once: groups of L
 // lanes (L = 64 for up to 8 systems, halved until all systems fit in the
 // workgroup) take one system each, every lane strides the system's chunks
@@ -972,6 +1017,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // groups added in order), else one thread per entry.  The stage-A loads are
   // issued before the first wait, so all latencies overlap.
   auto reduce_t = [&](auto want, bool with_pA) {
+    ChunkPre preA;
+    if (with_pA) chunks_prefetch(P.partA, nsys, P.nA, rngA, all_sys, preA);
     const int T = P.t_total;
     const int ng = T > 0 && T <= kBlock ? kBlock / T : 1;
     auto sum_entry = [&](int q, int g0, int gs) {
@@ -999,7 +1046,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     } else {
       for (int q = tid; q < T; q += kBlock) s_t[q] = sum_entry(q, 0, 1);
     }
-    if (with_pA) reduce_chunks_all(P.partA, nsys, P.nA, P.rt_total, rngA, all_sys, s_v1);
+    if (with_pA) chunks_finish(P.partA, P.nA, P.rt_total, preA, s_v1);
     __syncthreads();
     if (T > 0 && T <= kBlock) {
       if (tid < T) {
