@@ -10,7 +10,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libhsc_mi355x.so"
 
-HSC_ABI_VERSION = 1
+HSC_ABI_VERSION = 2
 
 STATUS = {
     0: "HSC_OK", 1: "HSC_ERR_INVALID_ARGUMENT", 2: "HSC_ERR_FEM_DIAGONAL",
@@ -18,6 +18,7 @@ STATUS = {
     6: "HSC_ERR_NONFINITE_FACTORS", 7: "HSC_ERR_BREAKDOWN", 8: "HSC_ERR_NOT_CONVERGED",
     9: "HSC_ERR_NOT_DESCENT", 10: "HSC_ERR_FULL_RESIDUAL", 11: "HSC_ERR_HIP",
     12: "HSC_ERR_NOT_SETUP", 13: "HSC_ERR_DEVICE_TIMEOUT", 14: "HSC_ERR_CAPACITY",
+    15: "HSC_ERR_STALE_BATCH",
 }
 globals().update({v: k for k, v in STATUS.items()})
 
@@ -50,11 +51,14 @@ class Stats(C.Structure):
         ("aca_relative_error", C.c_double), ("pcg_iterations", C.c_int32),
         ("schur_relative_residual", C.c_double), ("full_relative_residual", C.c_double),
         ("setup_ms", C.c_double), ("total_ms", C.c_double),
+        ("active_mesh_contacts", C.c_int32), ("active_ground_contacts", C.c_int32),
         ("adaptive_abd_inverse", C.c_int32), ("neumann_doublings", C.c_int32),
         ("abd_inverse_residual", C.c_double), ("full_residual_limit", C.c_double),
         ("accepted", C.c_int32), ("fallback", C.c_int32), ("status", C.c_int32),
         ("precision", C.c_int32), ("preconditioner", C.c_int32), ("true_residual_checks", C.c_int32),
         ("restarts", C.c_int32), ("pcg_ms", C.c_double), ("descent", C.c_double),
+        ("gbps", C.c_double), ("roofline_fraction", C.c_double), ("fp64_oracle_error", C.c_double),
+        ("learned_used", C.c_int32), ("fallback_reason", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -88,6 +92,11 @@ SIGNATURES = {
     "hsc_do_solve": (C.c_int32, [C.c_void_p, C.POINTER(BCOO3View), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                  C.c_void_p, C.c_void_p, C.POINTER(Stats), C.c_void_p]),
     "hsc_append_jsonl": (C.c_int32, [C.POINTER(Stats), C.c_char_p, C.c_int32, C.c_int32]),
+    "hsc_preconditioner_name": (C.c_char_p, [C.c_int32]),
+    "hsc_precision_name": (C.c_char_p, [C.c_int32]),
+    "hsc_tune_file": (C.c_int32, [C.c_char_p]),
+    "hsc_tune": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "hsc_alloc_count": (C.c_int64, []),
     "hsc_solve_batched": (C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_void_p), C.POINTER(Stats), C.c_void_p]),
     "hsc_batch_create": (C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p),

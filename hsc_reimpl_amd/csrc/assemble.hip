@@ -12,6 +12,11 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <cstdio>
+#include <map>
+#include <mutex>
+
+#include "../../include/hsc.h"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -151,40 +156,45 @@ hipError_t launch_abd_fem_coupling(const AbdFemCouplingParams& P, hipStream_t st
   return hipGetLastError();
 }
 
-hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, long long count, int fold,
-                         int* orow, int* ocol, double* ovals, long long* out_count, hipStream_t st) {
-  *out_count = 0;
-  if (count <= 0) return hipSuccess;
-  if (count >= (1LL << 31)) return hipErrorInvalidValue;
+// Workspace of one assembly of n triplets: two key and two index arrays
+// (sort input and output), head flags, their scan, the compacted heads,
+// hipcub's temporary storage and the head count, each 256-byte aligned.
+namespace {
+size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+hipError_t asm_layout(int n, hipStream_t st, AsmLayout* L) {
+  u64* nk = nullptr;
+  int* ni = nullptr;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, L->sort_bytes, nk, nk, ni, ni, n, 0, 64, st);
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, L->scan_bytes, ni, ni, n, st);
+  L->tmp = up256(std::max<size_t>(std::max(L->sort_bytes, L->scan_bytes), 16));
+  L->kb = up256(sizeof(u64) * (size_t)n);
+  L->ib = up256(sizeof(int) * (size_t)n);
+  L->total = 2 * L->kb + 5 * L->ib + L->tmp + 256;
+  return e;
+}
+
+// The assembly in a caller-provided workspace of at least L.total bytes
+// (nothing in it needs to start at zero: every array is written before it is
+// read).  Synchronises st; *out_count = number of unique kept blocks.
+hipError_t assemble_in(const int* rows, const int* cols, const double* vals, long long count, int fold, int* orow,
+                       int* ocol, double* ovals, long long* out_count, char* arena, const AsmLayout& L,
+                       hipStream_t st) {
   const int n = (int)count;
-  u64 *k0 = nullptr, *k1 = nullptr;
-  int *i0 = nullptr, *i1 = nullptr, *flag = nullptr, *pos = nullptr;
-  void* tmp = nullptr;
-  size_t sort_bytes = 0, scan_bytes = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, i0, i1, n, 0, 64, st);
-  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flag, pos, n, st);
-  const size_t tmp_bytes = std::max<size_t>(std::max(sort_bytes, scan_bytes), 16);
-  // one arena for the keys, indices, flags, positions and hipcub's workspace
-  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t kb = up(sizeof(u64) * n), ib = up(sizeof(int) * n);
-  char* arena = nullptr;
-  if (e == hipSuccess) e = hipMallocAsync((void**)&arena, 2 * kb + 5 * ib + up(tmp_bytes) + 256, st);
-  if (e == hipSuccess) {
-    k0 = (u64*)arena;
-    k1 = (u64*)(arena + kb);
-    i0 = (int*)(arena + 2 * kb);
-    i1 = (int*)(arena + 2 * kb + ib);
-    flag = (int*)(arena + 2 * kb + 2 * ib);
-    pos = (int*)(arena + 2 * kb + 3 * ib);
-    tmp = arena + 2 * kb + 4 * ib;
-  }
-  int* heads = arena ? (int*)(arena + 2 * kb + 4 * ib + up(tmp_bytes)) : nullptr;
-  int* dcount = arena ? (int*)(arena + 2 * kb + 5 * ib + up(tmp_bytes)) : nullptr;
+  u64* k0 = (u64*)arena;
+  u64* k1 = (u64*)(arena + L.kb);
+  int* i0 = (int*)(arena + 2 * L.kb);
+  int* i1 = (int*)(arena + 2 * L.kb + L.ib);
+  int* flag = (int*)(arena + 2 * L.kb + 2 * L.ib);
+  int* pos = (int*)(arena + 2 * L.kb + 3 * L.ib);
+  void* tmp = arena + 2 * L.kb + 4 * L.ib;
+  int* heads = (int*)(arena + 2 * L.kb + 4 * L.ib + L.tmp);
+  int* dcount = (int*)(arena + 2 * L.kb + 5 * L.ib + L.tmp);
+  size_t sort_bytes = L.sort_bytes, scan_bytes = L.scan_bytes;
   int hcount = 0;
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_asm_keys, dim3(grid_for(n)), dim3(256), 0, st, rows, cols, count, fold, k0, i0);
-    e = hipGetLastError();
-  }
+  hipLaunchKernelGGL(k_asm_keys, dim3(grid_for(n)), dim3(256), 0, st, rows, cols, count, fold, k0, i0);
+  hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, k0, k1, i0, i1, n, 0, 64, st);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_asm_heads, dim3(grid_for(n)), dim3(256), 0, st, k1, count, flag);
@@ -202,11 +212,56 @@ hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, l
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(&hcount, dcount, sizeof(int), hipMemcpyDeviceToHost, st);
-  if (arena) (void)hipFreeAsync(arena, st);
   const hipError_t se = hipStreamSynchronize(st);
   if (e == hipSuccess) e = se;
   if (e == hipSuccess) *out_count = hcount;
   return e;
 }
 
+namespace {
+
+// Library-held workspace of hsc_assemble_bcoo, one per device, grown on
+// demand and kept for the process; calls are serialised by its lock.
+struct AsmArena {
+  char* p = nullptr;
+  size_t cap = 0;
+};
+std::mutex g_asm_mu;
+std::map<int, AsmArena> g_asm_arena;
+
+}  // namespace
+
+hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, long long count, int fold,
+                         int* orow, int* ocol, double* ovals, long long* out_count, hipStream_t st) {
+  *out_count = 0;
+  if (count <= 0) return hipSuccess;
+  if (count >= (1LL << 31)) return hipErrorInvalidValue;
+  AsmLayout L;
+  hipError_t e = asm_layout((int)count, st, &L);
+  if (e != hipSuccess) return e;
+  int dev = 0;
+  e = hipStreamGetDevice(st, &dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(g_asm_mu);
+  AsmArena& a = g_asm_arena[dev];
+  if (a.cap < L.total) {
+    if (a.p) {
+      e = hipStreamSynchronize(st);
+      if (e == hipSuccess) e = hipFree(a.p);
+      a.p = nullptr;
+      a.cap = 0;
+      if (e != hipSuccess) return e;
+    }
+    e = hipMalloc((void**)&a.p, L.total);
+    if (e != hipSuccess) {
+      a.p = nullptr;
+      return e;
+    }
+    note_alloc();
+    a.cap = L.total;
+  }
+  return assemble_in(rows, cols, vals, count, fold, orow, ocol, ovals, out_count, a.p, L, st);
+}
+
 }  // namespace hsc
+

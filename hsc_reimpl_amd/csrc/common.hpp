@@ -5,6 +5,10 @@
 
 namespace hsc {
 
+// Counts one device or pinned-host allocation made by the library
+// (hsc_alloc_count reports the total).
+void note_alloc();
+
 constexpr int kBlock = 512;        // threads per workgroup (8 wave64)
 constexpr int kNodeTileMax = 512;  // nodes per elementwise tile (one node per thread; runtime 128 or 512)
 constexpr int kLrTile = 128;       // low-rank rows (W rows) per tile

@@ -135,8 +135,20 @@ struct AbdFemCouplingParams {
 hipError_t launch_abd_fem_coupling(const AbdFemCouplingParams& P, hipStream_t st);
 // general 3x3-block triplets -> upper-triangle BCOO of unique blocks, sorted by
 // (row, col), duplicates summed in input order (assemble.hip); synchronises st
+// (workspace: library-held, one per device, grown on demand; counted by note_alloc)
 hipError_t assemble_bcoo(const int* rows, const int* cols, const double* vals, long long count, int fold,
                          int* orow, int* ocol, double* ovals, long long* out_count, hipStream_t st);
+// workspace layout of one assembly of n triplets (hipcub's size queries only:
+// no allocation) and the assembly itself in a caller-provided workspace of
+// at least total bytes (nothing in it needs to start at zero)
+struct AsmLayout {
+  size_t kb = 0, ib = 0, tmp = 0, total = 0;
+  size_t sort_bytes = 0, scan_bytes = 0;
+};
+hipError_t asm_layout(int n, hipStream_t st, AsmLayout* L);
+hipError_t assemble_in(const int* rows, const int* cols, const double* vals, long long count, int fold, int* orow,
+                       int* ocol, double* ovals, long long* out_count, char* arena, const AsmLayout& L,
+                       hipStream_t st);
 // one launch of a standalone operator kernel: 0 FP64 D x, 1 FP64 pass, 2 FP32 pass
 hipError_t launch_timed_op(int kernel, int G, const SysDesc& S, const void* x, void* y, hipStream_t st);
 // up to kZeroListMax buffers zeroed by one kernel launch (each bytes % 4 == 0)

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/hsc.h"
+#include "common.hpp"
 #include "mlp.hpp"
 
 namespace hsc {
@@ -14,7 +15,9 @@ namespace hsc {
 namespace {
 template <class T>
 hipError_t dmalloc(T** p, size_t n) {
-  return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1));
+  const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1));
+  if (e == hipSuccess) note_alloc();
+  return e;
 }
 #define TRY(x)                           \
   do {                                   \

@@ -22,9 +22,20 @@
 #include "learned.hpp"
 #include "mlp.hpp"
 
+#include <atomic>
 #include <cstring>
+#include <fstream>
 #include <functional>
 #include <memory>
+#include <sstream>
+
+#include <sys/stat.h>
+
+namespace {
+std::atomic<long long> g_alloc_count{0};
+}  // namespace
+
+void hsc::note_alloc() { g_alloc_count.fetch_add(1, std::memory_order_relaxed); }
 
 namespace {
 
@@ -43,7 +54,10 @@ struct Buf {
     p = nullptr;
     cap = 0;
     hipError_t e = hipMalloc(&p, bytes);
-    if (e == hipSuccess) cap = bytes;
+    if (e == hipSuccess) {
+      cap = bytes;
+      hsc::note_alloc();
+    }
     return e;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
@@ -63,7 +77,10 @@ struct HostBuf {
     p = nullptr;
     cap = 0;
     hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
-    if (e == hipSuccess) cap = bytes;
+    if (e == hipSuccess) {
+      cap = bytes;
+      hsc::note_alloc();
+    }
     return e;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
@@ -121,18 +138,71 @@ DeviceInfo device_info(int dev) {
   return d;
 }
 
-// autotune table: (nb bucket, nnzb per row bucket, rank, precision) -> (G, wg per CU)
+// autotune table: size class -> launch shape.  The class is the device (CU
+// count), log2 of the nodes, blocks per row, rank / 16, precision, batch or
+// single, preconditioner family (Neumann; Chebyshev and learned share the
+// polynomial passes), preconditioner order and pass sync.
 struct TuneKey {
-  int nb_log2, deg, kbucket, mixed, batch;
-  bool operator<(const TuneKey& o) const {
-    return std::tie(nb_log2, deg, kbucket, mixed, batch) < std::tie(o.nb_log2, o.deg, o.kbucket, o.mixed, o.batch);
-  }
+  int cu, nb_log2, deg, kbucket, mixed, batch, poly, order, psync;
+  auto tie() const { return std::tie(cu, nb_log2, deg, kbucket, mixed, batch, poly, order, psync); }
+  bool operator<(const TuneKey& o) const { return tie() < o.tie(); }
 };
 std::mutex g_tune_mu;
 struct TuneVal {
   int G, wgpcu, sched;
 };
 std::map<TuneKey, TuneVal> g_tune;
+// the file the table is loaded from and appended to ("" = memory only)
+std::string g_tune_path;
+bool g_tune_path_set = false, g_tune_loaded = false;
+constexpr const char* kTuneMagic = "hsc-tune-v2";
+
+std::string default_tune_path() {
+  if (const char* e = std::getenv("HSC_TUNE_FILE")) return e;
+  if (const char* home = std::getenv("HOME")) return std::string(home) + "/.cache/hsc_mi355x/tune_v2.txt";
+  return "";
+}
+
+// Load the table file once (caller holds g_tune_mu).  Lines:
+// "hsc-tune-v2 cu nb_log2 deg kbucket mixed batch poly order psync G wgpcu sched";
+// anything else is skipped.
+void load_tune_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  if (!g_tune_path_set) {
+    g_tune_path = default_tune_path();
+    g_tune_path_set = true;
+  }
+  if (g_tune_path.empty()) return;
+  std::ifstream f(g_tune_path);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string magic;
+    TuneKey k{};
+    TuneVal v{};
+    if (!(is >> magic) || magic != kTuneMagic) continue;
+    if (!(is >> k.cu >> k.nb_log2 >> k.deg >> k.kbucket >> k.mixed >> k.batch >> k.poly >> k.order >> k.psync >> v.G >>
+          v.wgpcu >> v.sched))
+      continue;
+    if (!(v.G == 4 || v.G == 8 || v.G == 16) || v.wgpcu < 1 || v.wgpcu > 4 || v.sched < 1 || v.sched > 2) continue;
+    g_tune[k] = v;
+  }
+}
+
+// Append one entry to the table file (caller holds g_tune_mu); an unwritable
+// path leaves the table in memory only.
+void save_tune_locked(const TuneKey& k, const TuneVal& v) {
+  if (g_tune_path.empty()) return;
+  // create the missing parent directories (mkdir -p of the path's directory)
+  for (size_t i = g_tune_path.find('/', 1); i != std::string::npos; i = g_tune_path.find('/', i + 1))
+    (void)::mkdir(g_tune_path.substr(0, i).c_str(), 0755);
+  FILE* f = std::fopen(g_tune_path.c_str(), "a");
+  if (!f) return;
+  std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d\n", kTuneMagic, k.cu, k.nb_log2, k.deg, k.kbucket, k.mixed,
+               k.batch, k.poly, k.order, k.psync, v.G, v.wgpcu, v.sched);
+  std::fclose(f);
+}
 
 }  // namespace
 
@@ -146,6 +216,9 @@ struct hsc_handle {
   bool blockdiag = true, setup_ok = false;
   double aca_err = 0.0, abd_res = 0.0, setup_ms = 0.0;
   int doublings = 0;
+  int abd_mode = 0;                 // ABD inverse that ran: 0 exact, 1 Neumann polynomial, 2 with doubling
+  unsigned long long gen = 0;       // bumped by every setup and reconfiguration (captured batches check it)
+  bool launch_ready = false;        // G/wgpcu/sched chosen and the launch scratch sized for them
   int G = 8, wgpcu = 1, sched = 1;
   bool pass_sync_used = false;  // the last PCG launch synchronised its passes per tile
   int max_coupled128 = 0;  // most coupled nodes in one 128-node tile
@@ -541,14 +614,23 @@ void reset_pcg(hsc_handle* const* hs, int nsys, PcgScratch& sc, hipStream_t st) 
   }
 }
 
+// Pinned host staging of collect_pcg for nsys systems, and the launch's
+// timing events.
+size_t host_out_offset(int nsys) { return (sizeof(hsc::SysOut) * nsys + 64 + 15) / 16 * 16; }
+void ensure_host_scratch(PcgScratch& sc, int nsys) {
+  HIPCHK(sc.host.ensure(host_out_offset(nsys) + sizeof(double) * 3 * (size_t)nsys));
+  if (!sc.ev0) HIPCHK(hipEventCreate(&sc.ev0));
+  if (!sc.ev1) HIPCHK(hipEventCreate(&sc.ev1));
+}
+
 // Copy the per-system outputs back (pinned) and wait; raises on a device
 // timeout.  With fin_hs, also each handle's acceptance sums (|b - Hx|^2, |b|^2,
 // b.x) into *fin, fetched in the same synchronisation.
 std::vector<hsc::SysOut> collect_pcg(PcgScratch& sc, int nsys, hipStream_t st, hsc_handle* const* fin_hs = nullptr,
                                      std::vector<double>* fin = nullptr) {
   const size_t out_bytes = sizeof(hsc::SysOut) * nsys;
-  const size_t fin_off = (out_bytes + 64 + 15) / 16 * 16;
-  HIPCHK(sc.host.ensure(fin_off + sizeof(double) * 3 * (size_t)nsys));
+  const size_t fin_off = host_out_offset(nsys);
+  ensure_host_scratch(sc, nsys);
   unsigned char* hb = sc.host.as<unsigned char>();
   int* derr = reinterpret_cast<int*>(hb + out_bytes);
   double* hfin = reinterpret_cast<double*>(hb + fin_off);
@@ -587,8 +669,7 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
     P.prof = h0->profbuf.as<long long>();
   }
   PcgScratch& sc = h0->pcg;
-  if (!sc.ev0) HIPCHK(hipEventCreate(&sc.ev0));
-  if (!sc.ev1) HIPCHK(hipEventCreate(&sc.ev1));
+  ensure_host_scratch(sc, nsys);
   hipEvent_t e0 = sc.ev0, e1 = sc.ev1;
   HIPCHK(hipEventRecord(e0, st));
   hipError_t le = hsc::launch_pcg(plan.G, plan.mixed, plan.res, P, plan.shmem, st);
@@ -651,33 +732,43 @@ TuneKey tune_key(hsc_handle* const* hs, int nsys) {
     nnz += hs[s]->nnzb;
     kmax = std::max(kmax, hs[s]->k);
   }
-  const bool mixed = hs[0]->cfg.precision == HSC_PRECISION_MIXED;
-  return TuneKey{(int)std::log2((double)std::max(nbt, 1LL)), (int)(nnz / std::max(nbt, 1LL)), kmax / 16,
-                 mixed ? 1 : 0, nsys > 1 ? 1 : 0};
+  const hsc_config& c = hs[0]->cfg;
+  const bool mixed = c.precision == HSC_PRECISION_MIXED;
+  return TuneKey{device_info(hs[0]->device).cu,
+                 (int)std::log2((double)std::max(nbt, 1LL)),
+                 (int)(nnz / std::max(nbt, 1LL)),
+                 kmax / 16,
+                 mixed ? 1 : 0,
+                 nsys > 1 ? 1 : 0,
+                 c.preconditioner == HSC_PRECOND_NEUMANN ? 0 : 1,
+                 c.neumann_order,
+                 c.pass_sync};
 }
 
 bool pinned(const hsc_config& c) { return c.lanes_per_row && c.wg_per_cu && c.schedule; }
 
-bool tune_cached(hsc_handle* const* hs, int nsys) {
-  if (pinned(hs[0]->cfg)) return true;
+// The table's entry for the class, with the config's pinned fields applied;
+// false when the class has no entry (and the config pins not everything).
+bool tune_lookup(hsc_handle* const* hs, int nsys, LaunchCfg* out) {
+  const hsc_config& c = hs[0]->cfg;
+  if (pinned(c)) {
+    *out = {c.lanes_per_row, c.wg_per_cu, c.schedule};
+    return true;
+  }
   std::lock_guard<std::mutex> g(g_tune_mu);
-  return g_tune.count(tune_key(hs, nsys)) > 0;
+  load_tune_locked();
+  auto it = g_tune.find(tune_key(hs, nsys));
+  if (it == g_tune.end()) return false;
+  *out = {c.lanes_per_row ? c.lanes_per_row : it->second.G, c.wg_per_cu ? c.wg_per_cu : it->second.wgpcu,
+          c.schedule ? c.schedule : it->second.sched};
+  return true;
 }
 
-LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
-  hsc_handle* h0 = hs[0];
-  const hsc_config& c = h0->cfg;
-  if (pinned(c)) return {c.lanes_per_row, c.wg_per_cu, c.schedule};
-  const TuneKey key = tune_key(hs, nsys);
-  {
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
-      return {c.lanes_per_row ? c.lanes_per_row : it->second.G, c.wg_per_cu ? c.wg_per_cu : it->second.wgpcu,
-              c.schedule ? c.schedule : it->second.sched};
-    }
-  }
-  // time a few PCG iterations per candidate; the real solve re-initialises x, r
+// Time a few PCG iterations per candidate launch shape and keep the fastest
+// in the table (and its file).  The systems' rhs, r and x must hold a
+// right-hand side the timing runs may start from (they are overwritten).
+LaunchCfg tune_now(hsc_handle* const* hs, int nsys, hipStream_t st) {
+  const hsc_config& c = hs[0]->cfg;
   const int Gs[3] = {4, 8, 16};
   double best = 1e300;
   LaunchCfg bestc{8, 1, 1};
@@ -702,14 +793,49 @@ LaunchCfg tuned_config(hsc_handle* const* hs, int nsys, hipStream_t st) {
           }
         } catch (const HscError& e) {
           if (e.code == HSC_ERR_CAPACITY) continue;  // candidate does not fit: try the next
-          throw;                                     // a HIP or device error ends the solve
+          throw;                                     // a HIP or device error ends the tuning
         }
       }
     }
   }
-  std::lock_guard<std::mutex> g(g_tune_mu);
-  g_tune[key] = {bestc.G, bestc.wgpcu, bestc.sched};
+  if (!pinned(c)) {
+    const TuneKey key = tune_key(hs, nsys);
+    const TuneVal v{bestc.G, bestc.wgpcu, bestc.sched};
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    load_tune_locked();
+    g_tune[key] = v;
+    save_tune_locked(key, v);
+  }
   return bestc;
+}
+
+// Launch shape and scratch of a handle's persistent PCG, fixed at setup so
+// that a solve allocates nothing and runs no tuning: the shape comes from the
+// tune table, or is timed now (from a pseudo-random right-hand side, with
+// the handle's spare vector as x) when the class is new; then every buffer of
+// that launch is sized by planning it once.
+void prepare_launch(hsc_handle* h, hipStream_t st, bool retune = false) {
+  hsc_handle* hs[1] = {h};
+  h->launch_ready = false;
+  h->xsol = h->yfull.as<double>();
+  LaunchCfg lc{};
+  if (retune || !tune_lookup(hs, 1, &lc)) {
+    HIPCHK(hsc::launch_hash_fill(h->rhs.as<double>(), h->fn, 0x2545f491u, st));
+    HIPCHK(hipMemcpyAsync(h->r.p, h->rhs.p, sizeof(double) * h->fn, hipMemcpyDeviceToDevice, st));
+    lc = tune_now(hs, 1, st);
+  }
+  {
+    PcgPlan plan = plan_pcg(hs, 1, lc, 0, st, h->pcg);
+    ensure_host_scratch(h->pcg, 1);
+    static const bool stage_timing = std::getenv("HSC_STAGE_TIMING") != nullptr;
+    if (stage_timing) HIPCHK(h->profbuf.ensure(sizeof(long long) * 32 * plan.P.nwg));
+    HIPCHK(hipStreamSynchronize(st));  // the plan's table uploads read its host copies
+  }
+  h->xsol = nullptr;
+  h->G = lc.G;
+  h->wgpcu = lc.wgpcu;
+  h->sched = lc.sched;
+  h->launch_ready = true;
 }
 
 // ------------------------------------------------------------------ setup
@@ -823,6 +949,8 @@ struct AuxJoin {
 void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, int fn, hipStream_t st) {
   auto t0 = Clock::now();
   h->setup_ok = false;
+  h->launch_ready = false;
+  ++h->gen;
   for (bool& ok : h->tnbr_ok) ok = false;
   if (!Av) throw HscError{HSC_ERR_INVALID_ARGUMENT, "null matrix view"};
   const long long N = 3 * Av->block_dim;
@@ -993,6 +1121,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                      h->cfg.neumann_order, h->cfg.omega, h->cfg.abd_inverse_tol, dmisc};
   if (h->cfg.abd_inverse == HSC_ABD_NEUMANN && h->cfg.adaptive_abd_inverse) abp.mode = 2;
   HIPCHK(hsc::launch_abd_inverse(abp, st));
+  h->abd_mode = abp.mode;
   HIPCHK(hipStreamWaitEvent(h->aux, h->aux_ev, 0));
   join.s = h->aux;
   HIPCHK(hsc::launch_sort_gather(h->rowptr.as<int>(), nb, h->key_col.as<int>(), h->key_src.as<int>(), Av->values,
@@ -1036,6 +1165,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   }
   HIPCHK(hipStreamSynchronize(h->aux));
   set_poly_weights(h, ss->lam2);
+  if (h->cfg.preconditioner != HSC_PRECOND_LEARNED) prepare_launch(h, st);
   h->setup_ok = true;
   h->setup_ms = ms_since(t0);
 }
@@ -1084,7 +1214,7 @@ int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   HIPCHK(h->ptmp.ensure(sizeof(double) * n));
   HIPCHK(h->dotbuf.ensure(sizeof(double) * 300));
   hsc::SysDesc S = make_desc(h);
-  const int G = h->G;
+  const int G = 8;  // a fixed lane split: the same sums whatever shape the persistent launch was tuned to
   double* r = h->r.as<double>();
   double* rhs = h->rhs.as<double>();
   double* p = h->p0.as<double>();
@@ -1195,6 +1325,34 @@ int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   return status;
 }
 
+// Algorithmic HBM bytes of one Schur-PCG iteration of h's system (every
+// matrix and vector element read or written once per pass).  FP64: an SpMV
+// pass is 76 B per block (values + column), 4 B per row pointer and 16 B per
+// DOF (one vector gathered, one written); each further preconditioner pass
+// adds the inverse diagonal blocks (72 B per node) and r; stage A reads p_old
+// and writes p; the low-rank apply reads W twice plus its indices; stage B
+// reads q, p, r, x, the inverse blocks and writes r, x, z.  Mixed: 40 B per
+// block, 36 B per inverse block, FP32 p, z, q (r and x stay FP64).
+double pcg_bytes_per_iteration(const hsc_handle* h) {
+  const double nb = h->nb, n = 3.0 * h->nb, nnzb = h->nnzb, a = h->k > 0 ? h->a : 0, k = h->k;
+  const int order = h->cfg.neumann_order;
+  if (h->cfg.precision != HSC_PRECISION_MIXED) {
+    const double spmv = 76 * nnzb + 4 * (nb + 1) + 16 * n;
+    return (order - 1) * (spmv + 72 * nb + 8 * n) + spmv + 16 * n + 2 * 8 * a * k + 4 * a + 6 * 8 * n + 72 * nb;
+  }
+  const double spmv = 40 * nnzb + 4 * (nb + 1) + 8 * n;
+  return (order - 1) * (spmv + 36 * nb + 8 * n) + spmv + 8 * n + 2 * 4 * a * k + 4 * a + 4 * 8 * n + 2 * 4 * n +
+         36 * nb + 4 * n;
+}
+
+constexpr double kHbmPeakGBs = 8000.0;  // MI355X HBM3E
+
+// gbps / roofline_fraction of a timed launch
+void fill_rate(const hsc_handle* h, hsc_stats* s) {
+  s->gbps = s->pcg_ms > 0 ? pcg_bytes_per_iteration(h) * s->pcg_iterations / (s->pcg_ms * 1e-3) / 1e9 : 0.0;
+  s->roofline_fraction = s->gbps / kHbmPeakGBs;
+}
+
 void fill_common_stats(hsc_handle* h, hsc_stats* s) {
   std::memset(s, 0, sizeof(*s));
   s->fem_dofs = h->fn;
@@ -1202,8 +1360,9 @@ void fill_common_stats(hsc_handle* h, hsc_stats* s) {
   s->active_fem_dofs = h->a;
   s->rank = h->k;
   s->aca_relative_error = h->aca_err;
-  s->adaptive_abd_inverse = h->cfg.adaptive_abd_inverse;
+  s->adaptive_abd_inverse = h->abd_mode == 2 ? 1 : 0;
   s->neumann_doublings = h->doublings;
+  s->fp64_oracle_error = -1.0;
   s->abd_inverse_residual = h->abd_res;
   s->full_residual_limit = h->cfg.full_residual_limit;
   s->precision = h->cfg.precision;
@@ -1268,6 +1427,13 @@ int status_from_out(hsc_handle* h, const hsc::SysOut& o) {
   return o.status;
 }
 
+// Reduced right-hand side and initial residual of h's system (rhs = r).
+void enqueue_rhs(hsc_handle* h, const double* b, double* x, hipStream_t st) {
+  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
+                         h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
+                         h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
+}
+
 int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStream_t st, Clock::time_point t0) {
   if (!h->setup_ok) {
     h->err = "hsc_solve before a successful hsc_setup";
@@ -1290,17 +1456,10 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
     return code;
   }
   hsc_handle* hs[1] = {h};
+  // a reconfiguration after setup that changed the launch re-plans it here
+  if (!h->launch_ready) prepare_launch(h, st);
   h->xsol = x + h->fo;
-  if (!tune_cached(hs, 1)) {
-    // the tuning runs start from the real right-hand side
-    HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
-                           h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
-                           h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
-  }
-  LaunchCfg lc = tuned_config(hs, 1, st);
-  h->G = lc.G;
-  h->wgpcu = lc.wgpcu;
-  h->sched = lc.sched;
+  const LaunchCfg lc{h->G, h->wgpcu, h->sched};
   HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
                          h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
                          h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
@@ -1309,6 +1468,7 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   auto out = run_pcg(hs, 1, lc, 0, st, &pms, [&] { enqueue_finish(h, b, x, st); }, &fin);
   s->pcg_ms = pms;
   s->pcg_iterations = out[0].iterations;
+  fill_rate(h, s);
   s->schur_relative_residual = out[0].rel;
   s->true_residual_checks = out[0].checks;
   s->restarts = out[0].restarts;
@@ -1417,8 +1577,14 @@ int32_t hsc_set_config(hsc_handle* h, const hsc_config* cfg) {
                         cfg->abd_inverse_tol != h->cfg.abd_inverse_tol ||
                         cfg->preconditioner != h->cfg.preconditioner ||
                         cfg->poly_interval_ratio != h->cfg.poly_interval_ratio;
+  // the launch shape and its scratch depend on these (re-planned by the next solve)
+  const bool relaunch = cfg->precision != h->cfg.precision || cfg->lanes_per_row != h->cfg.lanes_per_row ||
+                        cfg->wg_per_cu != h->cfg.wg_per_cu || cfg->schedule != h->cfg.schedule ||
+                        cfg->pass_sync != h->cfg.pass_sync;
   h->cfg = *cfg;
+  ++h->gen;  // a captured batch of this handle is stale now
   if (refactor) h->setup_ok = false;
+  if (relaunch) h->launch_ready = false;
   return HSC_OK;
 }
 
@@ -1480,14 +1646,13 @@ int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count, const doubl
   hipStream_t st = (hipStream_t)stream;
   auto t0 = Clock::now();
   return guarded(h0, [&] {
-    for (int i = 0; i < count; ++i) {
-      hsc_handle* h = handles[i];
-      HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b[i], h->qo, h->qm, h->fo,
-                             h->fn, h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
-                             h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x[i], st));
-    }
     for (int i = 0; i < count; ++i) handles[i]->xsol = x[i] + handles[i]->fo;
-    LaunchCfg lc = tuned_config(handles, count, st);
+    LaunchCfg lc{};
+    if (!tune_lookup(handles, count, &lc)) {
+      // the timing runs start from the real right-hand sides
+      for (int i = 0; i < count; ++i) enqueue_rhs(handles[i], b[i], x[i], st);
+      lc = tune_now(handles, count, st);
+    }
     for (int i = 0; i < count; ++i) {
       hsc_handle* h = handles[i];
       HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b[i], h->qo, h->qm, h->fo,
@@ -1536,6 +1701,7 @@ struct hsc_batch {
   std::vector<double*> x;
   PcgScratch sc;  // own launch scratch: eager solves on the handles do not touch the captured tables
   PcgPlan plan;
+  std::vector<unsigned long long> gens;  // each handle's setup generation at capture
   LaunchCfg lc{8, 1, 1};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1569,11 +1735,21 @@ int validate_batch(hsc_handle* const* handles, int32_t count, const double* cons
   return HSC_OK;
 }
 
-void enqueue_rhs(hsc_handle* h, const double* b, double* x, hipStream_t st) {
-  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
-                         h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
-                         h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
+// HSC_ERR_STALE_BATCH when a handle was set up again or reconfigured after
+// the capture (its buffers, rank and tile ranges may no longer be the ones
+// the graph holds), HSC_ERR_NOT_SETUP when its last setup failed.
+int batch_stale(hsc_batch* bt) {
+  for (size_t i = 0; i < bt->hs.size(); ++i) {
+    hsc_handle* h = bt->hs[i];
+    if (h->gen != bt->gens[i]) {
+      bt->hs[0]->err = "captured batch: system " + std::to_string(i) + " was set up or reconfigured after the capture";
+      return HSC_ERR_STALE_BATCH;
+    }
+    if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  }
+  return HSC_OK;
 }
+
 
 }  // namespace
 
@@ -1595,9 +1771,11 @@ int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double
     bt->b.assign(b, b + count);
     bt->x.assign(x, x + count);
     for (int i = 0; i < count; ++i) handles[i]->xsol = x[i] + handles[i]->fo;
-    if (!tune_cached(handles, count))
+    if (!tune_lookup(handles, count, &bt->lc)) {
       for (int i = 0; i < count; ++i) enqueue_rhs(handles[i], b[i], x[i], st);
-    bt->lc = tuned_config(handles, count, st);
+      bt->lc = tune_now(handles, count, st);
+    }
+    for (int i = 0; i < count; ++i) bt->gens.push_back(handles[i]->gen);
     bt->plan = plan_pcg(handles, count, bt->lc, 0, st, bt->sc);
     HIPCHK(hipStreamSynchronize(st));
     hipStream_t cap = nullptr;
@@ -1657,9 +1835,9 @@ int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double
 
 int32_t hsc_batch_launch(hsc_batch* bt, hsc_stream stream) {
   if (!bt || !bt->exec) return HSC_ERR_INVALID_ARGUMENT;
+  const int stale = batch_stale(bt);
+  if (stale != HSC_OK) return stale;
   DeviceGuard g(bt->device);
-  for (auto* h : bt->hs)
-    if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
   const hipError_t e = hipGraphLaunch(bt->exec, (hipStream_t)stream);
   if (e != hipSuccess) {
     bt->hs[0]->err = std::string("graph launch: ") + hipGetErrorString(e);
@@ -1670,6 +1848,8 @@ int32_t hsc_batch_launch(hsc_batch* bt, hsc_stream stream) {
 
 int32_t hsc_batch_stats(hsc_batch* bt, hsc_stats* stats, hsc_stream stream) {
   if (!bt || !stats) return HSC_ERR_INVALID_ARGUMENT;
+  const int stale = batch_stale(bt);
+  if (stale != HSC_OK) return stale;
   DeviceGuard g(bt->device);
   hipStream_t st = (hipStream_t)stream;
   const int n = (int)bt->hs.size();
@@ -1709,28 +1889,73 @@ int32_t hsc_batch_info(const hsc_batch* bt, int32_t* lanes_per_row, int32_t* wg_
 
 void hsc_batch_destroy(hsc_batch* bt) { delete bt; }
 
+const char* hsc_preconditioner_name(int32_t p) {
+  switch (p) {
+    case HSC_PRECOND_NEUMANN: return "neumann";
+    case HSC_PRECOND_LEARNED: return "learned";
+    case HSC_PRECOND_CHEBYSHEV: return "chebyshev";
+    default: return "unknown";
+  }
+}
+
+const char* hsc_precision_name(int32_t p) {
+  switch (p) {
+    case HSC_PRECISION_FP64: return "fp64";
+    case HSC_PRECISION_MIXED: return "mixed";
+    default: return "unknown";
+  }
+}
+
 int32_t hsc_append_jsonl(const hsc_stats* s, const char* path, int32_t frame, int32_t newton) {
   if (!s || !path) return HSC_ERR_INVALID_ARGUMENT;
   FILE* f = std::fopen(path, "a");
   if (!f) return HSC_ERR_INVALID_ARGUMENT;
-  // field order of linear_hsc.cu:421-431; the contact counters are not known
-  // at this boundary and are written as 0
+  // the reference's fields in the order of linear_hsc.cu:421-431, then the additions
+  char oracle[40];
+  if (s->fp64_oracle_error >= 0 && std::isfinite(s->fp64_oracle_error))
+    std::snprintf(oracle, sizeof oracle, "%.17g", s->fp64_oracle_error);
+  else
+    std::snprintf(oracle, sizeof oracle, "null");
   std::fprintf(f,
                "{\"frame\":%d,\"newton\":%d,\"fem_dofs\":%d,\"abd_dofs\":%d,\"active_fem_dofs\":%d,\"rank\":%d,"
                "\"aca_relative_error\":%.17g,\"pcg_iterations\":%d,\"schur_relative_residual\":%.17g,"
-               "\"full_relative_residual\":%.17g,\"setup_ms\":%.6g,\"total_ms\":%.6g,\"active_mesh_contacts\":0,"
-               "\"active_ground_contacts\":0,\"adaptive_abd_inverse\":%s,\"neumann_doublings\":%d,"
+               "\"full_relative_residual\":%.17g,\"setup_ms\":%.6g,\"total_ms\":%.6g,\"active_mesh_contacts\":%d,"
+               "\"active_ground_contacts\":%d,\"adaptive_abd_inverse\":%s,\"neumann_doublings\":%d,"
                "\"abd_inverse_residual\":%.17g,\"full_residual_limit\":%.17g,\"accepted\":%s,"
                "\"solver\":\"linear_hsc\",\"fallback\":false,\"precision\":\"%s\",\"preconditioner\":\"%s\","
-               "\"learned_fallback\":%s,\"status\":%d,\"pcg_ms\":%.6g}\n",
+               "\"learned_used\":%s,\"learned_fallback\":%s,\"fallback_reason\":%d,\"status\":%d,\"pcg_ms\":%.6g,"
+               "\"gbps\":%.6g,\"roofline_fraction\":%.6g,\"fp64_oracle_error\":%s}\n",
                frame, newton, s->fem_dofs, s->abd_dofs, s->active_fem_dofs, s->rank, s->aca_relative_error,
                s->pcg_iterations, s->schur_relative_residual, s->full_relative_residual, s->setup_ms, s->total_ms,
-               s->adaptive_abd_inverse ? "true" : "false", s->neumann_doublings, s->abd_inverse_residual,
-               s->full_residual_limit, s->accepted ? "true" : "false", s->precision ? "mixed" : "fp64",
-               s->preconditioner ? "learned" : "neumann", s->fallback ? "true" : "false", s->status, s->pcg_ms);
+               s->active_mesh_contacts, s->active_ground_contacts, s->adaptive_abd_inverse ? "true" : "false",
+               s->neumann_doublings, s->abd_inverse_residual, s->full_residual_limit, s->accepted ? "true" : "false",
+               hsc_precision_name(s->precision), hsc_preconditioner_name(s->preconditioner),
+               s->learned_used ? "true" : "false", s->fallback ? "true" : "false", s->fallback_reason, s->status,
+               s->pcg_ms, s->gbps, s->roofline_fraction, oracle);
   std::fclose(f);
   return HSC_OK;
 }
+
+int32_t hsc_tune_file(const char* path) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune_path = path ? path : "";
+  g_tune_path_set = true;
+  g_tune_loaded = false;  // (re)load from the new file; entries in memory are kept
+  load_tune_locked();
+  return HSC_OK;
+}
+
+int32_t hsc_tune(hsc_handle* h, hsc_stream stream) {
+  if (!h) return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    prepare_launch(h, (hipStream_t)stream, true);
+    return HSC_OK;
+  });
+}
+
+int64_t hsc_alloc_count(void) { return g_alloc_count.load(); }
 
 int32_t hsc_apply_D(hsc_handle* h, const double* x, double* y, hsc_stream stream) {
   if (!h || !x || !y) return HSC_ERR_INVALID_ARGUMENT;
@@ -1815,7 +2040,9 @@ int32_t hsc_abd_fem_coupling(const int32_t* abd_vertex, const int32_t* fem_verte
   auto& f = flags[dev];
   if (!f.first) {
     e = hipMalloc((void**)&f.first, sizeof(int));
+    if (e == hipSuccess) hsc::note_alloc();
     if (e == hipSuccess) e = hipHostMalloc((void**)&f.second, sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) hsc::note_alloc();
     if (e != hipSuccess) {
       if (f.first) (void)hipFree(f.first);
       f = {nullptr, nullptr};

@@ -97,11 +97,18 @@ class GraphMLP:
         if code:
             raise plugin.HSCError(code, "hsc_mlp_create")
         self.h = h
+        plugin.register_live("mlp", self)
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "h", None):
             self.lib.hsc_mlp_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def set_params(self, theta: np.ndarray):
         t = np.ascontiguousarray(theta, dtype=np.float32)

@@ -18,13 +18,36 @@ status code -- the reference's UIPC_ASSERT failures, one code each.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 from pathlib import Path
 
 import torch
 
 from . import _abi
+
+# Library objects still alive at interpreter exit are destroyed by an atexit
+# hook (registered after torch's, so it runs first), while the HIP runtime is
+# certainly up: destructors left to module teardown could otherwise call HIP
+# after a profiler or the runtime has finalised.  Batches go before the
+# handles they hold, then networks, then handles.
+_LIVE = {"batch": weakref.WeakSet(), "mlp": weakref.WeakSet(), "handle": weakref.WeakSet()}
+
+
+def register_live(kind: str, obj) -> None:
+    _LIVE[kind].add(obj)
+
+
+@atexit.register
+def _close_live() -> None:
+    for kind in ("batch", "mlp", "handle"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
 
 # libuipc scene-config keys of the plugin and their defaults
 CONFIG_DEFAULTS = {
@@ -217,6 +240,10 @@ class SolvingInfo:
     iter_count: int = 0
     frame: int = 0
     newton: int = 0
+    # contact counters of the JSONL row: the reference reads them from its
+    # contact systems (linear_hsc.cu:410-414); the caller supplies them here
+    active_mesh_contacts: int = 0
+    active_ground_contacts: int = 0
 
 
 class Handle:
@@ -231,6 +258,7 @@ class Handle:
         if code != 0:
             raise HSCError(code, "hsc_create")
         self.h = h
+        register_live("handle", self)
 
     def _check(self, code: int) -> None:
         if code != 0:
@@ -344,6 +372,7 @@ class Batch:
             msg = self.lib.hsc_last_error(handles[0].h)
             raise HSCError(code, msg.decode() if msg else "hsc_batch_create")
         self.b = out
+        register_live("batch", self)
 
     def launch(self, stream=None):
         code = self.lib.hsc_batch_launch(self.b, _stream(stream))
@@ -406,6 +435,8 @@ class LinearHSC:
         code = lib.hsc_do_solve(self.handle.h, C.byref(v), info.qo, info.qm, info.fo, info.fn,
                                 C.c_void_p(_ptr(info.b)), C.c_void_p(_ptr(info.x)), C.byref(st), _stream(None))
         info.iter_count = st.pcg_iterations
+        st.active_mesh_contacts = int(info.active_mesh_contacts)
+        st.active_ground_contacts = int(info.active_ground_contacts)
         if self.workspace is not None:
             self.workspace.mkdir(parents=True, exist_ok=True)
             lib.hsc_append_jsonl(C.byref(st), str(self.workspace / "hsc_stats.jsonl").encode(),

@@ -365,11 +365,12 @@ def replicate_fem(sysm, copies, seed=0):
     return HSCSystem(R, C, V, b, fo=0, fn=fn, qo=fn, qm=sysm.qm, info=info)
 
 
-def batch_systems(seed=2, count=64, bodies=(1, 16), tori=(1, 8), tets=(2000, 15000),
-                  ranks=(4, 16), youngs=(1e8, 1e10), progress=None):
-    """Uneven batch: per system U{1..16} bodies, U{1..8} tori of
-    log-U[2k, 15k] tets, coupling rank U{4..16} (capped at 12 per hub
-    block, the most a 12-row block can carry)."""
+def batch_specs(seed=2, count=64, bodies=(1, 16), tori=(1, 8), tets=(2000, 15000), ranks=(4, 16),
+                youngs=(1e8, 1e10)):
+    """Per-system parameters of the uneven batch, drawn in order from one
+    stream: U{1..16} bodies, U{1..8} tori of log-U[2k, 15k] tets (E
+    log-uniform), coupling rank U{4..16}, and the seed of the system's own
+    stream (mesh barriers, bodies, couplings, right-hand side)."""
     rng = np.random.default_rng(seed)
     out = []
     for s in range(count):
@@ -378,33 +379,64 @@ def batch_systems(seed=2, count=64, bodies=(1, 16), tori=(1, 8), tets=(2000, 150
         rank = int(rng.integers(ranks[0], ranks[1] + 1))
         target = _log_uniform(rng, tets[0], tets[1], ntor)
         ys = _log_uniform(rng, youngs[0], youngs[1], ntor)
-        sub = np.random.default_rng(int(rng.integers(1 << 31)))
-        Is, Js, Bs, rims, ks, base, nt = [], [], [], [], [], 0, 0
-        for t in range(ntor):
-            n_phi, n_rho = 12, 2
-            n_theta = max(4, 2 * int(round(target[t] / (5 * n_phi * n_rho) / 2)))
-            n_outer = n_theta * n_phi
-            nodes, tt, I, J, B, rim, k_s = _torus_block(
-                sub, 0.45, 0.12, 0.06, n_theta, n_phi, n_rho, ys[t], 0.45,
-                min(300, n_outer // 4), 0.05, (1e3, 1e6), (1.2 * t, 0.0, 0.0))
-            Is.append(I + base)
-            Js.append(J + base)
-            Bs.append(B)
-            rims.append(rim + base)
-            ks.append(k_s)
-            nt += len(tt)
-            base += len(nodes)
-        bods = [_spd_body(sub) for _ in range(nbod)]
-        couplings = []
-        for t in range(ntor):
-            hub = t % nbod
-            Eb = _coupling_block(sub, min(rank, 12), 3 * len(rims[t]), ks[t], bods[hub])
-            couplings.append((hub, rims[t], Eb))
-        info = dict(kind="batch", index=s, tets=int(nt), tori=ntor, bodies=nbod, rank=rank)
-        out.append(_finish(base, np.concatenate(Is), np.concatenate(Js), np.concatenate(Bs),
-                           bods, couplings, sub, info))
-        if progress:
-            progress(s, out[-1])
+        sub_seed = int(rng.integers(1 << 31))
+        out.append(dict(index=s, bodies=nbod, tori=ntor, rank=rank, tets=target, youngs=ys, seed=sub_seed))
+    return out
+
+
+def batch_system(spec):
+    """One system of the uneven batch from its spec (see batch_specs); the
+    coupling rank is capped at 12 per hub block, the most a 12-row block can
+    carry."""
+    sub = np.random.default_rng(spec["seed"])
+    Is, Js, Bs, rims, ks, base, nt = [], [], [], [], [], 0, 0
+    for t in range(spec["tori"]):
+        n_phi, n_rho = 12, 2
+        n_theta = max(4, 2 * int(round(spec["tets"][t] / (5 * n_phi * n_rho) / 2)))
+        n_outer = n_theta * n_phi
+        nodes, tt, I, J, B, rim, k_s = _torus_block(
+            sub, 0.45, 0.12, 0.06, n_theta, n_phi, n_rho, spec["youngs"][t], 0.45,
+            min(300, n_outer // 4), 0.05, (1e3, 1e6), (1.2 * t, 0.0, 0.0))
+        Is.append(I + base)
+        Js.append(J + base)
+        Bs.append(B)
+        rims.append(rim + base)
+        ks.append(k_s)
+        nt += len(tt)
+        base += len(nodes)
+    bods = [_spd_body(sub) for _ in range(spec["bodies"])]
+    couplings = []
+    for t in range(spec["tori"]):
+        hub = t % spec["bodies"]
+        Eb = _coupling_block(sub, min(spec["rank"], 12), 3 * len(rims[t]), ks[t], bods[hub])
+        couplings.append((hub, rims[t], Eb))
+    info = dict(kind="batch", index=spec["index"], tets=int(nt), tori=spec["tori"], bodies=spec["bodies"],
+                rank=spec["rank"])
+    return _finish(base, np.concatenate(Is), np.concatenate(Js), np.concatenate(Bs), bods, couplings, sub, info)
+
+
+def map_processes(fn, items, workers):
+    """fn over items in `workers` fresh processes (spawned, so no parent
+    state such as an initialised GPU runtime is inherited), results in
+    order; workers <= 1 maps in this process."""
+    items = list(items)
+    if workers <= 1 or len(items) <= 1:
+        return [fn(x) for x in items]
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=min(workers, len(items)), mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(fn, items))
+
+
+def batch_systems(seed=2, count=64, bodies=(1, 16), tori=(1, 8), tets=(2000, 15000),
+                  ranks=(4, 16), youngs=(1e8, 1e10), progress=None, workers=1):
+    """The uneven batch: `count` systems from batch_specs, built in `workers`
+    processes (the same systems for any worker count)."""
+    specs = batch_specs(seed, count, bodies, tori, tets, ranks, youngs)
+    out = map_processes(batch_system, specs, workers)
+    if progress:
+        for s, sysm in enumerate(out):
+            progress(s, sysm)
     return out
 
 

@@ -11,8 +11,11 @@
  *
  * Threading: one handle = one host thread = one HIP stream at a time.  Calls
  * on one handle are not thread-safe.  All device buffers are caller-owned
- * unless stated; hsc_setup allocates the handle's workspaces (re-used, grown
- * only when a bigger system arrives), hsc_solve allocates nothing.
+ * unless stated.  hsc_setup allocates the handle's workspaces (re-used, grown
+ * only when a bigger system arrives), picks the persistent-PCG launch shape
+ * (timed once per size class and kept in the tune table, see hsc_tune_file)
+ * and sizes that launch's scratch; hsc_solve then allocates nothing and runs
+ * no tuning (hsc_alloc_count lets a caller check this).
  */
 #ifndef HSC_MI355X_H
 #define HSC_MI355X_H
@@ -24,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HSC_ABI_VERSION 1
+#define HSC_ABI_VERSION 2
 
 /* Status codes: one per failure mode of the reference plugin. */
 enum hsc_status {
@@ -42,7 +45,9 @@ enum hsc_status {
   HSC_ERR_HIP = 11,               /* HIP runtime error: linear_hsc.cu:34-37 */
   HSC_ERR_NOT_SETUP = 12,         /* solve before setup (no reference counterpart) */
   HSC_ERR_DEVICE_TIMEOUT = 13,    /* a persistent kernel's bounded wait expired (no counterpart) */
-  HSC_ERR_CAPACITY = 14           /* a system exceeds a compiled-in limit (no counterpart) */
+  HSC_ERR_CAPACITY = 14,          /* a system exceeds a compiled-in limit (no counterpart) */
+  HSC_ERR_STALE_BATCH = 15        /* a captured batch's handle was set up or reconfigured after
+                                     the capture (no counterpart) */
 };
 
 /* Arithmetic of the bulk (matrix/vector) traffic. */
@@ -68,6 +73,11 @@ enum hsc_preconditioner {
                                      [b/poly_interval_ratio, b], b = 1.2 x a power-iteration estimate
                                      of lambda_max(Dinv D) (no counterpart) */
 };
+
+/* Name of a preconditioner / precision as written to the JSONL row
+ * ("neumann", "learned", "chebyshev"; "fp64", "mixed"); "unknown" otherwise. */
+const char* hsc_preconditioner_name(int32_t preconditioner);
+const char* hsc_precision_name(int32_t precision);
 
 /* Solver configuration.  The first eight fields are the reference's scene
  * config keys and defaults (integration.patch:34-55; linear_system/tol_rate
@@ -123,7 +133,13 @@ typedef struct hsc_stats {
   int32_t pcg_iterations;
   double schur_relative_residual, full_relative_residual;
   double setup_ms, total_ms;
-  int32_t adaptive_abd_inverse, neumann_doublings;
+  int32_t active_mesh_contacts;  /* contact counters of the row (linear_hsc.cu:410-414): not known to the
+                                    solver; a solve writes 0 and the caller fills them before
+                                    hsc_append_jsonl */
+  int32_t active_ground_contacts;
+  int32_t adaptive_abd_inverse;  /* 1 when the adaptive doubling of the ABD inverse ran (abd_inverse =
+                                    NEUMANN with adaptive_abd_inverse = 1), else 0 */
+  int32_t neumann_doublings;
   double abd_inverse_residual, full_residual_limit;
   int32_t accepted, fallback;    /* fallback: learned preconditioner fell back to Neumann */
   int32_t status;                /* enum hsc_status of this solve */
@@ -131,6 +147,14 @@ typedef struct hsc_stats {
   int32_t precision, preconditioner, true_residual_checks, restarts;
   double pcg_ms;                 /* device time of the persistent PCG launch */
   double descent;                /* b^T x */
+  double gbps;                   /* algorithmic bytes of the PCG launch / pcg_ms, GB/s (0 when not timed) */
+  double roofline_fraction;      /* gbps / 8000 (MI355X HBM peak) */
+  double fp64_oracle_error;      /* |x - x_ref| / |x_ref| against an FP64 reference solution: not known to
+                                    the solver; a solve writes -1 (JSONL null) and a caller that has a
+                                    reference fills it before hsc_append_jsonl */
+  int32_t learned_used;          /* 1 when learned parameters shaped this solve's preconditioner */
+  int32_t fallback_reason;       /* 0 none; 1 no learned weights loaded; 2 network output rejected
+                                    (non-finite or outside its bounds): plain preconditioner used */
 } hsc_stats;
 
 typedef struct hsc_handle hsc_handle;
@@ -166,13 +190,33 @@ int32_t hsc_do_solve(hsc_handle* h, const hsc_bcoo3_view* A, int32_t qo, int32_t
                      hsc_stats* stats, hsc_stream stream);
 
 /* Append one JSONL row in the reference's hsc_stats.jsonl schema
- * (linear_hsc.cu:420-432) to `path`; frame/newton label the row. */
+ * (linear_hsc.cu:420-432) to `path`; frame/newton label the row.  After the
+ * reference's fields come precision, preconditioner (by name), the learned
+ * fields, status, pcg_ms, gbps, roofline_fraction and fp64_oracle_error
+ * (null when negative). */
 int32_t hsc_append_jsonl(const hsc_stats* s, const char* path, int32_t frame, int32_t newton);
+
+/* ---- launch-shape tuning and allocation accounting (no reference counterpart) ----
+ * The tune table maps a size class (nodes, blocks per row, rank, precision,
+ * batch, preconditioner, order, pass sync, device) to the fastest persistent
+ * PCG launch shape.  It is loaded from and appended to the file named here
+ * (default: $HSC_TUNE_FILE, else $HOME/.cache/hsc_mi355x/tune_v2.txt; NULL or
+ * "" = keep it in memory only).  hsc_tune runs the timing for h's current
+ * system now (hsc_setup does it too when the class is new).  hsc_alloc_count
+ * is the number of device and pinned-host allocations the library has made
+ * in this process. */
+int32_t hsc_tune_file(const char* path);
+int32_t hsc_tune(hsc_handle* h, hsc_stream stream);
+int64_t hsc_alloc_count(void);
 
 /* ---- batched solves (no reference counterpart: many independent systems
  * of uneven size in ONE persistent launch) ----
  * handles[i] must each have been hsc_setup; all share one config precision
- * and preconditioner. b[i], x[i]: device vectors of system i. */
+ * and preconditioner. b[i], x[i]: device vectors of system i.  The first call
+ * for a batch size class times its launch shapes (the tune table), and the
+ * launch scratch lives in handles[0], grown when a larger batch arrives: a
+ * caller that cannot allocate captures the batch once (hsc_batch_create) and
+ * replays it. */
 int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count,
                           const double* const* b, double* const* x,
                           hsc_stats* stats, hsc_stream stream);
@@ -265,7 +309,9 @@ int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* 
  * synchronisation); hsc_batch_stats waits for it and fills one hsc_stats per
  * system (pcg_ms is not measured inside a replay and reads 0).  The batch is
  * valid until one of its handles is set up again, reconfigured or
- * destroyed.  Handles must share device, precision, neumann_order, omega and
+ * destroyed: after a setup or reconfiguration hsc_batch_launch and
+ * hsc_batch_stats return HSC_ERR_STALE_BATCH before any device work (a
+ * destroyed handle cannot be detected; destroy the batch first).  Handles must share device, precision, neumann_order, omega and
  * preconditioner (Neumann or Chebyshev; with Chebyshev the batch uses the
  * weights of the handle with the largest spectrum bound, valid for all). */
 typedef struct hsc_batch hsc_batch;

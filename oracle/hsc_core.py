@@ -115,6 +115,53 @@ class Neumann:
         return float(np.max(np.abs(np.linalg.eigvals(T))))
 
 
+class AbdInverseError(ArithmeticError):
+    """The adaptive ABD inverse missed its tolerance or diverged
+    (linear_hsc.cu:292-297 of the reference)."""
+
+
+def neumann_adaptive(A, order=5, omega=0.7, tol=1e-5, adaptive=True, max_doublings=48, strict=True):
+    """ABD inverse of the drop-in plugin (linear_hsc.cu:263-297 of the reference).
+
+    P_r = sum_{k<r} (I - w B^-1 A)^k w B^-1 with B the 12x12 diagonal blocks
+    of A (each Cholesky-checked), symmetrised; then, when `adaptive`, while
+    ||I - A P||_F / sqrt(qm) > tol and fewer than `max_doublings` doublings
+    were made: P <- P + P (I - A P), symmetrised.  Raises AbdInverseError when
+    a doubling gives a non-finite P or residual, or (when `strict`) when the
+    tolerance is still missed.  Returns (P, doublings, residual).
+    """
+    qm = A.shape[0]
+    if qm % 12:
+        raise ValueError("ABD block size must be a multiple of 12")
+    Binv = np.zeros_like(A)
+    for i in range(0, qm, 12):
+        blk = A[i:i + 12, i:i + 12]
+        np.linalg.cholesky(blk)
+        Binv[i:i + 12, i:i + 12] = np.linalg.solve(blk, np.eye(12))
+    eye = np.eye(qm)
+    T = eye - omega * Binv @ A
+    term = omega * Binv
+    P = term.copy()
+    for _ in range(1, order):
+        term = T @ term
+        P += term
+    P = 0.5 * (P + P.T)
+    R = eye - A @ P
+    res = float(np.linalg.norm(R) / np.sqrt(qm))
+    doublings = 0
+    while adaptive and res > tol and doublings < max_doublings:
+        P = P + P @ R
+        P = 0.5 * (P + P.T)
+        R = eye - A @ P
+        res = float(np.linalg.norm(R) / np.sqrt(qm))
+        doublings += 1
+        if not (np.all(np.isfinite(P)) and np.isfinite(res)):
+            raise AbdInverseError(f"ABD Neumann extension diverged at doubling {doublings}")
+    if strict and adaptive and res > tol:
+        raise AbdInverseError(f"ABD inverse residual {res} exceeds {tol} after {doublings} doublings")
+    return P, doublings, res
+
+
 def pcg(matvec, b, preconditioner=None, rtol=1e-10, maxiter=1000):
     """PCG with the TRUE relative residual tested every iteration (core.py:95-138).
 
@@ -213,12 +260,15 @@ class HSC:
     """
 
     def __init__(self, D, A, E, order=5, omega=0.7, aca_rtol=None,
-                 exact_affine=False, aca_max_rank=None, check_schur=True):
+                 exact_affine=False, aca_max_rank=None, check_schur=True, affine_P=None):
         self.D, self.A, self.E = D, A, E
         self.factors = aca(E, rtol=aca_rtol, max_rank=aca_max_rank)
         self.U, self.V = self.factors.U, self.factors.V
         self.affine_neumann = Neumann(A, 12, order, omega)
-        if exact_affine:
+        if affine_P is not None:
+            # an explicit A^-1 approximation (the drop-in plugin's adaptive P)
+            self.affine_inverse = lambda v: affine_P @ v
+        elif exact_affine:
             self.affine_inverse = lambda v: np.linalg.solve(A, v)
         else:
             self.affine_inverse = self.affine_neumann

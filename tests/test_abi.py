@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported(lib):
 
 
 def test_abi_version_and_defaults(lib):
-    assert lib.hsc_abi_version() == 1
+    assert lib.hsc_abi_version() == 2
     c = _abi.Config()
     lib.hsc_config_default(C.byref(c))
     # integration.patch:34-55 and scene_default_config.cpp:146-151 of the reference
@@ -104,8 +104,12 @@ def test_jsonl_row_matches_reference_schema(lib, tmp_path):
     s.fem_dofs, s.abd_dofs, s.active_fem_dofs, s.rank = 74508, 144, 819, 84
     s.pcg_iterations, s.schur_relative_residual, s.full_relative_residual = 499, 9.8e-5, 1.4e-7
     s.accepted, s.adaptive_abd_inverse, s.full_residual_limit = 1, 1, 0.1
+    s.active_mesh_contacts, s.active_ground_contacts = 1234, 56
+    s.preconditioner, s.precision = _abi.PRECONDITIONER["chebyshev"], _abi.PRECISION["mixed"]
+    s.pcg_ms, s.gbps, s.roofline_fraction, s.fp64_oracle_error = 3.5, 2500.0, 0.3125, -1.0
     p = tmp_path / "hsc_stats.jsonl"
     assert lib.hsc_append_jsonl(C.byref(s), str(p).encode(), 1, 0) == 0
+    s.fp64_oracle_error, s.preconditioner, s.learned_used = 2.5e-9, _abi.PRECONDITIONER["learned"], 1
     assert lib.hsc_append_jsonl(C.byref(s), str(p).encode(), 1, 1) == 0
     rows = [json.loads(line) for line in p.read_text().splitlines()]
     # field list of linear_hsc.cu:421-431 and the consumer storage.py:48-61 of the reference
@@ -118,3 +122,28 @@ def test_jsonl_row_matches_reference_schema(lib, tmp_path):
     assert rows[0]["accepted"] is True and rows[0]["adaptive_abd_inverse"] is True
     # the consumer's acceptance rule
     assert not (rows[0]["solver"] != "linear_hsc" or rows[0]["fallback"] or not rows[0]["accepted"])
+    # contact counters supplied by the caller, names of what ran, rate and oracle-error fields
+    assert (rows[0]["active_mesh_contacts"], rows[0]["active_ground_contacts"]) == (1234, 56)
+    assert rows[0]["preconditioner"] == "chebyshev" and rows[0]["precision"] == "mixed"
+    assert rows[1]["preconditioner"] == "learned" and rows[1]["learned_used"] is True
+    assert rows[0]["gbps"] == 2500.0 and rows[0]["roofline_fraction"] == 0.3125
+    assert rows[0]["fp64_oracle_error"] is None and rows[1]["fp64_oracle_error"] == 2.5e-9
+
+
+def test_names_of_modes(lib):
+    names = {v: lib.hsc_preconditioner_name(v).decode() for v in _abi.PRECONDITIONER.values()}
+    assert names == {v: k for k, v in _abi.PRECONDITIONER.items()}
+    assert lib.hsc_preconditioner_name(7) == b"unknown"
+    assert [lib.hsc_precision_name(v).decode() for v in (0, 1)] == ["fp64", "mixed"]
+
+
+def test_alloc_count_and_tune_file_without_a_device(lib, tmp_path):
+    # host-only calls: the counter reads, and a table file is loaded (lines
+    # of another format are skipped) or an absent one is tolerated
+    assert lib.hsc_alloc_count() >= 0
+    f = tmp_path / "tune.txt"
+    f.write_text("garbage line\nhsc-tune-v2 256 14 11 3 1 0 1 16 1 4 2 1\n")
+    assert lib.hsc_tune_file(str(f).encode()) == 0
+    assert lib.hsc_tune_file(str(tmp_path / "absent.txt").encode()) == 0
+    assert lib.hsc_tune_file(None) == 0
+    assert lib.hsc_tune(None, None) == _abi.HSC_ERR_INVALID_ARGUMENT

@@ -174,3 +174,39 @@ def test_pass_sync_batch_bitwise():
     for a, b in zip(res[0][0], res[1][0]):
         assert np.array_equal(a, b)
     assert res[0][1] == res[1][1]
+
+
+@pytest.mark.parametrize("shape", ["tuned", "pinned"])
+def test_benchmarked_solver_on_vehicle_system(shape):
+    """The configuration bench.py reports as `value`, on the 78k-tet system:
+    mixed precision, 16 Chebyshev-weighted passes on [b/1000, b], neighbour
+    pass sync, the tuner's launch shape (and the latency shape the tuner
+    picked when it was measured: 4 lanes per row, 2 workgroups per CU,
+    schedule 1).  The true Schur residual (FP64, recomputed on the CPU from
+    the oracle's operators) is <= 1e-8, x is within 1e-7 of the FP64 sparse
+    direct solve, and the iteration count is within 2 of 43, the measured
+    count.  The spectrum bound b = 1.2 x the estimate stays above
+    lambda_max(Dinv D) from SciPy (the preconditioner is SPD)."""
+    import scipy.sparse.linalg as spla
+    s = systems.torus_system(seed=1)
+    scene = dict(C2_SCENE, **{"linear_system/hsc/neumann_order": 16})
+    ext = dict(precision="mixed", residual_replace_every=25, preconditioner="chebyshev", poly_interval_ratio=1000.0)
+    if shape == "pinned":
+        ext.update(LATENCY)
+    h = _setup(s, scene, **ext)
+    x, st = _solve(h, s)
+    info = h.setup_info()
+    assert st["status"] == 0 and st["pcg_iterations"] == pytest.approx(43, abs=2), st["pcg_iterations"]
+    if shape == "pinned":
+        assert info["pass_sync"] == 1 and (info["lanes_per_row"], info["schedule"]) == (4, 1)
+    ref = osp.SparseHSC.from_system(s, order=5, omega=0.7)
+    n = s.fn
+    rhs = ref.reduced_rhs(s.b[:n], s.b[n:])
+    assert np.linalg.norm(rhs - ref.matvec(x[:n])) / np.linalg.norm(rhs) <= 1e-8 * 1.001
+    xd = osp.direct_solve(s)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) <= 1e-7
+    D = ref.D
+    J = osp.BlockJacobi3(D, 1.0)
+    op = spla.LinearOperator(D.shape, matvec=lambda v: J(D @ v))
+    lam = abs(spla.eigs(op, k=1, which="LM", return_eigenvectors=False, tol=1e-8)[0])
+    assert 1.2 * info["lambda_max"] >= lam and info["lambda_max"] <= lam * (1 + 1e-9), (info["lambda_max"], lam)

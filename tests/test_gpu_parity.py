@@ -313,9 +313,33 @@ def test_plugin_do_solve_writes_jsonl(c1, tmp_path):
     A = plugin.BCOO3.from_system(c1, DEV)
     b = torch.as_tensor(c1.b, device=DEV)
     info = plugin.SolvingInfo(x=torch.zeros_like(b), b=b, A=A, qo=c1.qo, qm=c1.qm, fo=c1.fo, fn=c1.fn,
-                              frame=3, newton=1)
+                              frame=3, newton=1, active_mesh_contacts=17, active_ground_contacts=4)
     st = ls.do_solve(info)
     assert info.iter_count == st["pcg_iterations"] > 0
     row = json.loads((tmp_path / "hsc_stats.jsonl").read_text().splitlines()[-1])
     assert row["frame"] == 3 and row["newton"] == 1 and row["accepted"] is True and row["solver"] == "linear_hsc"
     assert row["pcg_iterations"] == st["pcg_iterations"] and row["setup_ms"] > 0
+    assert (row["active_mesh_contacts"], row["active_ground_contacts"]) == (17, 4)
+    # default ABD inverse is the exact one: no doubling ran
+    assert row["preconditioner"] == "neumann" and row["precision"] == "fp64"
+    assert row["adaptive_abd_inverse"] is False and row["neumann_doublings"] == 0
+    assert row["gbps"] > 0 and row["fp64_oracle_error"] is None
+
+
+@pytest.mark.parametrize("pre", ["neumann", "chebyshev"])
+@pytest.mark.parametrize("abd", ["exact", "neumann"])
+def test_jsonl_names_what_ran(c1, tmp_path, pre, abd):
+    """The row names the preconditioner and precision that ran, and
+    adaptive_abd_inverse is true exactly when the doubling of the ABD inverse
+    ran (abd_inverse = neumann with adaptive_abd_inverse = 1)."""
+    ls = plugin.LinearHSC(workspace=tmp_path, aca_rtol=1e-6, preconditioner=pre, abd_inverse=abd,
+                          precision="mixed")
+    ls.do_build(dict(C1_SCENE, **{"linear_system/hsc/neumann_order": 5 if pre == "chebyshev" else 2}))
+    A = plugin.BCOO3.from_system(c1, DEV)
+    b = torch.as_tensor(c1.b, device=DEV)
+    info = plugin.SolvingInfo(x=torch.zeros_like(b), b=b, A=A, qo=c1.qo, qm=c1.qm, fo=c1.fo, fn=c1.fn)
+    st = ls.do_solve(info)
+    row = json.loads((tmp_path / "hsc_stats.jsonl").read_text().splitlines()[-1])
+    assert row["preconditioner"] == pre and row["precision"] == "mixed" and row["status"] == 0
+    assert row["adaptive_abd_inverse"] is (abd == "neumann")
+    assert row["neumann_doublings"] == st["neumann_doublings"] and (row["neumann_doublings"] > 0) == (abd == "neumann")

@@ -156,3 +156,58 @@ def test_pcg_dropin_agrees_with_core_schedule():
         _, info = hsc_core.pcg(solver.matvec, rhs, solver.deformable_preconditioner, 1e-10, 1000)
         _, it, rel = hsc_core.pcg_dropin(solver.matvec, rhs, solver.deformable_preconditioner, 1e-10, 1000)
         assert abs(it - info["iterations"]) <= 1 and rel <= 1e-10
+
+
+def test_adaptive_abd_inverse_closed_form_and_squaring():
+    """The drop-in plugin's ABD inverse (linear_hsc.cu:263-297).  One body:
+    B = A^-1, so P_r = (1 - (1-w)^r) A^-1 and d doublings give
+    (1 - (1-w)^(r 2^d)) A^-1 with residual (1-w)^(r 2^d).  Coupled bodies:
+    each doubling squares I - A P (P + P(I - AP) = 2P - PAP is symmetric, so
+    the symmetrisation changes nothing), so the residual falls quadratically
+    to the tolerance.  A dropped doubling term, a wrong sign or order in the
+    product, or a residual normalised differently fails one of these."""
+    rng = np.random.default_rng(11)
+    M = rng.normal(size=(12, 12))
+    A = M @ M.T + 12 * np.eye(12)
+    Ainv = np.linalg.inv(A)
+    for tol, d_expect in ((1e-2, 0), (1e-5, 1), (1e-9, 2)):
+        P, d, res = hsc_core.neumann_adaptive(A, order=5, omega=0.7, tol=tol)
+        assert d == d_expect, (tol, d)
+        np.testing.assert_allclose(P, (1 - 0.3 ** (5 * 2 ** d)) * Ainv, rtol=1e-9, atol=1e-15)
+        assert res == pytest.approx(0.3 ** (5 * 2 ** d), rel=1e-6)
+    # not adaptive: the fixed-order polynomial equals the Neumann oracle
+    P0, d0, _ = hsc_core.neumann_adaptive(A, order=3, omega=0.6, tol=1e-12, adaptive=False)
+    assert d0 == 0
+    np.testing.assert_allclose(P0, hsc_core.Neumann(A, 12, 3, 0.6)(np.eye(12)), rtol=1e-12, atol=1e-15)
+    # coupled bodies (the reference fixtures couple bodies 0-1 and 1-2)
+    f = load_fixture(1e6)
+    Af = f["A"]
+    eye = np.eye(len(Af))
+    res_hist = []
+    for dd in range(3):
+        # (beyond two doublings the residual reaches the round-off floor of
+        # A P, about 1e-9 here, where squaring no longer describes it)
+        P, d, res = hsc_core.neumann_adaptive(Af, order=2, omega=0.7, tol=0.0, max_doublings=dd, strict=False)
+        assert d == dd
+        res_hist.append(res)
+        if dd:
+            R = eye - Af @ Pprev
+            assert np.linalg.norm(eye - Af @ P - R @ R) <= 1e-6 * np.linalg.norm(R @ R) + 1e-8
+        Pprev = P
+    assert res_hist[2] < res_hist[1] < res_hist[0]
+    P, d, res = hsc_core.neumann_adaptive(Af, order=5, omega=0.7, tol=1e-5)
+    assert res <= 1e-5 and d >= 1
+    np.testing.assert_allclose(P, np.linalg.inv(Af), rtol=0, atol=1e-4 * np.abs(np.linalg.inv(Af)).max())
+
+
+def test_adaptive_abd_inverse_failure_modes():
+    """A tolerance below round-off is missed after 48 doublings (the
+    plugin's 'exceeds ... after 48 doublings' failure); a non-SPD body block
+    fails the Cholesky check."""
+    f = load_fixture(1e4)
+    with pytest.raises(hsc_core.AbdInverseError, match="after 48 doublings"):
+        hsc_core.neumann_adaptive(f["A"], tol=1e-300)
+    bad = f["A"].copy()
+    bad[0, 0] = -1.0
+    with pytest.raises(np.linalg.LinAlgError):
+        hsc_core.neumann_adaptive(bad)
