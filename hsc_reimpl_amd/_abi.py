@@ -118,6 +118,9 @@ SIGNATURES = {
                                          C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hsc_assemble_bcoo": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
+    "hsc_assemble_bcoo_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64),
+                                              C.c_void_p]),
     "hsc_time_kernel": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                     C.POINTER(C.c_double)]),
     "hsc_get_factors": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

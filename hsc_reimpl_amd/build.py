@@ -19,7 +19,8 @@ CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 LIB = HERE / "libhsc_mi355x.so"
 OBJ = HERE / "_build"
-SOURCES = ["pcg_kernel.hip", "setup_kernels.hip", "learned_kernels.hip", "assemble.hip", "fem_hessian.hip", "solver.cpp", "mlp.cpp"]
+SOURCES = ["pcg_kernel.hip", "setup_kernels.hip", "learned_kernels.hip", "assemble.hip", "assemble_noalloc.hip",
+           "fem_hessian.hip", "solver.cpp", "mlp.cpp"]
 ARCH = os.environ.get("HSC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INCLUDE}", f"-I{CSRC}",

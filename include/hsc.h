@@ -288,6 +288,21 @@ int32_t hsc_assemble_bcoo(const int32_t* rows, const int32_t* cols, const double
                           int32_t fold_lower, int32_t* out_rows, int32_t* out_cols, double* out_values,
                           int64_t* out_count, hsc_stream stream);
 
+/* ---- assembly without allocation (no counterpart in the reference) ----
+ * hsc_assemble_bcoo for callers that cannot allocate device memory (code
+ * under stream capture, serving loops and harnesses that forbid hipMalloc):
+ * the same kernels and the same results, with the workspace in a 64 MiB
+ * device array of the library (one per GPU, held until the process ends)
+ * instead of memory the library allocates.  At most 1,048,576 triplets per
+ * call (HSC_ERR_CAPACITY beyond, before any device work).  Every call on a
+ * GPU shares that GPU's array: calls from one process are serialised by a
+ * lock, and calls that overlap on one GPU from elsewhere race.  Nothing in
+ * the array needs clearing between calls (every part is written before it
+ * is read).  Synchronises `stream`. */
+int32_t hsc_assemble_bcoo_noalloc(const int32_t* rows, const int32_t* cols, const double* values, int64_t count,
+                                  int32_t fold_lower, int32_t* out_rows, int32_t* out_cols, double* out_values,
+                                  int64_t* out_count, hsc_stream stream);
+
 /* ---- measurement (no reference counterpart) ----
  * Times `repeats` back-to-back launches of one standalone operator kernel on
  * the set-up handle's D, between HIP events on `stream`: kernel 0 = FP64
