@@ -8,16 +8,19 @@ factors; persistent Schur PCG; back-substitution; acceptance checks) with the
 Hessian and b already resident in HBM.
 
 Prints ONE JSON line.  `value` = solves/s of the new solver: FP32 bulk /
-FP64 reductions with an FP64 true residual every 25 iterations, and 16
-block-Jacobi passes weighted by the Chebyshev roots of [b/1000, b] (Leja
-order) with b from a power-iteration spectrum bound.  The plain FP64 solver (the
-reference algorithm: Neumann series of order 5, omega 0.7, FP64) is timed
-in the same run and reported beside it, with the mixed-precision Neumann
-and FP64 Chebyshev variants.
+FP64 reductions with an FP64 true residual every 25 iterations, and the
+learned preconditioner -- 16 block-Jacobi passes weighted by the Chebyshev
+roots (Leja order) of the interval [b/ratio, b] whose ratio the committed
+graph MLP picks at setup from D's node features, b from a power-iteration
+spectrum bound.  The plain FP64 solver (the reference algorithm: Neumann
+series of order 5, omega 0.7, FP64) is timed in the same run and reported
+beside it, with the same passes at the hand-picked fixed ratio 1000, the
+mixed-precision Neumann and the FP64 Chebyshev variants.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import platform
@@ -492,7 +495,8 @@ def main():
     ap.add_argument("--lanes-per-row", type=int, default=4)
     ap.add_argument("--wg-per-cu", type=int, default=1)
     ap.add_argument("--schedule", type=int, default=1)
-    ap.add_argument("--preconditioner", default="chebyshev", choices=["neumann", "chebyshev"])
+    ap.add_argument("--preconditioner", default="learned", choices=["neumann", "chebyshev", "learned"],
+                    help="preconditioner of the --traffic-probe solve")
     ap.add_argument("--cpu-only", action="store_true",
                     help="print the CPU baseline's measurements at the inherited thread setting and exit "
                          "(the default run starts this as a child process; no GPU use)")
@@ -524,12 +528,16 @@ def main():
 
     new_scene = dict(scene, **{"linear_system/hsc/neumann_order": NEW_PASSES})
     if args.traffic_probe:
-        cheb = args.preconditioner == "chebyshev"
-        ext = {"poly_interval_ratio": NEW_RATIO} if cheb else {}
-        h = plugin.Handle(plugin.make_config(new_scene if cheb else scene, precision="mixed",
+        poly = args.preconditioner != "neumann"
+        ext = {"poly_interval_ratio": NEW_RATIO} if args.preconditioner == "chebyshev" else {}
+        h = plugin.Handle(plugin.make_config(new_scene if poly else scene, precision="mixed",
                                              residual_replace_every=25, lanes_per_row=args.lanes_per_row,
                                              wg_per_cu=args.wg_per_cu, schedule=args.schedule,
                                              preconditioner=args.preconditioner, **ext))
+        if args.preconditioner == "learned":
+            from hsc_reimpl_amd import learned
+            w = torch.as_tensor(learned.load_weights(), device=dev)
+            h._check(h.lib.hsc_set_learned_weights(h.h, C.c_void_p(w.data_ptr()), w.numel() * 4))
         h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
         x = torch.zeros_like(b)
         st = h.solve(b, x)
@@ -544,17 +552,23 @@ def main():
         print(json.dumps({"probe": "traffic", "pcg_iterations": st["pcg_iterations"], "pcg_ms": st["pcg_ms"],
                           "bytes_per_iteration": pcg_bytes_per_iteration(info["fem_nodes"], info["nnzb"],
                                                                          info["active_fem_dofs"], info["rank"],
-                                                                         NEW_PASSES if cheb else order,
+                                                                         NEW_PASSES if poly else order,
                                                                          mixed=True),
                           "copy_bytes": nbytes}), flush=True)
         return
+
+    from hsc_reimpl_amd import learned
+    theta = learned.load_weights()
 
     def run(precision, preconditioner="neumann"):
         ext = {"precision": "mixed", "residual_replace_every": 25} if precision == "mixed" else {}
         ext["preconditioner"] = preconditioner
         if preconditioner == "chebyshev":
             ext["poly_interval_ratio"] = NEW_RATIO
-        h = plugin.Handle(plugin.make_config(new_scene if preconditioner == "chebyshev" else scene, **ext))
+        h = plugin.Handle(plugin.make_config(scene if preconditioner == "neumann" else new_scene, **ext))
+        if preconditioner == "learned":
+            w = torch.as_tensor(theta, device=dev)
+            h._check(h.lib.hsc_set_learned_weights(h.h, C.c_void_p(w.data_ptr()), w.numel() * 4))
         x = torch.zeros_like(b)
         for _ in range(args.warmup):
             h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
@@ -578,7 +592,8 @@ def main():
 
     plain = run("fp64")                      # the reference algorithm: Neumann series, FP64
     mixed = run("mixed")                     # + FP32 bulk / FP64 reductions
-    new = run("mixed", "chebyshev")          # + 16 Chebyshev-weighted passes
+    new = run("mixed", "learned")            # + 16 passes on the network's interval
+    cheb = run("mixed", "chebyshev")         # the same passes on the fixed interval ratio 1000
     cheb64 = run("fp64", "chebyshev")
     info = plain["info"]
     info_new = new["info"]
@@ -617,15 +632,19 @@ def main():
                                              "mixed": res["mixed"][0]["pcg_iterations"]},
                       "mixed_rel_diff_vs_fp64": diff,
                       "schur_relative_residual_mixed": res["mixed"][0]["schur_relative_residual"]})
-    learned_c1 = None
-    from hsc_reimpl_amd import learned
-    if learned.WEIGHTS.exists():
-        r = learned.evaluate(learned.load_weights(), ["c1"], device=dev)[0]
-        learned_c1 = {"system": "2,178-DOF system (Neumann order 2, omega 0.6, rtol 1e-8)",
-                      "loop": "host-driven flexible PCG, same operators, with and without the graph MLP",
-                      "iterations": {"plain": r["plain"]["iterations"], "learned": r["learned"]["iterations"]},
-                      "ms": {"plain": round(r["plain"]["ms"], 2), "learned": round(r["learned"]["ms"], 2)},
-                      "converged": r["plain"]["status"] == 0 and r["learned"]["status"] == 0}
+    # what the network gains on the 2,178-DOF system and a held-out one of its
+    # family: iterations and device ms beside the plain Neumann series and the
+    # fixed intervals
+    rows = learned.evaluate(theta, {"c1": systems.cube_system(seed=0), "cube200": learned.family_system(200)},
+                            device=dev, reps=3)
+    learned_c1 = {"systems": "2,178-DOF system and seed 200 of its family (Neumann order 2, omega 0.6; "
+                             "polynomials at 16 passes; mixed precision, rtol 1e-8)",
+                  "rows": [{"system": r["system"],
+                            "iterations": {k: r[k]["iterations"] for k in
+                                           ("neumann", "chebyshev_100", "chebyshev_1000", "learned")},
+                            "pcg_ms": {k: round(r[k]["pcg_ms"], 3) for k in
+                                       ("neumann", "chebyshev_100", "chebyshev_1000", "learned")},
+                            "learned_ratio": round(r["learned"]["ratio"], 1)} for r in rows]}
     bandwidth = beyond_mall(sysm, dev)
     assembly = assemble_timing(sysm, dev)
     generator = fem_generator_timing(dev)
@@ -642,16 +661,21 @@ def main():
         "config": {"workload": "78k-tet vehicle-scale HSC system (7 tori x 11,200 Tet4, 12 ABD bodies)",
                    "fem_nodes": nb, "fem_dofs": 3 * nb, "abd_dofs": info["abd_dofs"], "nnzb": nnzb,
                    "active_fem_dofs": a, "rank": k, "neumann_order": order, "omega": omega, "rtol": rtol,
-                   "new_passes": NEW_PASSES, "new_interval_ratio": NEW_RATIO, "pass_sync": info_new["pass_sync"],
+                   "new_passes": NEW_PASSES, "new_interval_ratio": round(info_new["poly_ratio"], 1),
+                   "new_interval_from": "learned graph MLP", "pass_sync": info_new["pass_sync"],
                    "lanes_per_row": info_new["lanes_per_row"], "wg_per_cu": info_new["workgroups"],
                    "schedule": info_new["schedule"]},
-        "new": "mixed precision (FP32 bulk, FP64 reductions and true residual every 25) + 16 block-Jacobi "
-               "passes weighted by the Chebyshev roots of [b/1000, b] in Leja order, b = 1.2 x a power-iteration "
-               "bound; neighbour-tile flags between passes",
+        "new": "learned + mixed precision: FP32 bulk, FP64 reductions and true residual every 25; 16 block-"
+               "Jacobi passes weighted by the Chebyshev roots of [b/ratio, b] in Leja order, the ratio picked at "
+               "setup by the graph MLP from D's node features, b = 1.2 x a power-iteration bound; neighbour-tile "
+               "flags between passes",
         "pcg_iterations": {"new": new["pcg_iterations"], "plain_fp64": plain["pcg_iterations"],
+                           "mixed_chebyshev_1000": cheb["pcg_iterations"],
                            "mixed_neumann": mixed["pcg_iterations"], "fp64_chebyshev": cheb64["pcg_iterations"]},
         "plain_fp64": {"solves_per_s": round(plain["solves_per_s"], 3), "ms_per_solve": round(plain["ms_per_solve"], 3),
                        "pcg_ms": round(plain["pcg_ms"], 3), "pcg_gbs": round(achieved64, 1)},
+        "mixed_chebyshev_1000": {"solves_per_s": round(cheb["solves_per_s"], 3),
+                                 "ms_per_solve": round(cheb["ms_per_solve"], 3), "pcg_ms": round(cheb["pcg_ms"], 3)},
         "mixed_neumann": {"solves_per_s": round(mixed["solves_per_s"], 3),
                           "ms_per_solve": round(mixed["ms_per_solve"], 3), "pcg_ms": round(mixed["pcg_ms"], 3)},
         "fp64_chebyshev": {"solves_per_s": round(cheb64["solves_per_s"], 3),
@@ -662,10 +686,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "pcg_persistent<G, float, true> (whole Schur PCG of the new solver, one cooperative launch)",
+                     "kernel_name_in_trace": "void hsc::pcg_persistent<%d, float, true>(hsc::PcgParams)"
+                                             % info_new["lanes_per_row"],
                      "bytes_per_iteration": bpi, "iterations": new["pcg_iterations"],
                      "kernel_ms": round(new["pcg_ms"], 3)},
         "cpu_baseline": cpu,
-        "learned_preconditioner": learned_c1,
+        "learned_preconditioner_2178_dof": learned_c1,
         "mixed_stiffness_sweep_2178_dof": sweep,
         "bandwidth_beyond_mall": bandwidth,
         "assembly_before_path": assembly,

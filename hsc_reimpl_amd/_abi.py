@@ -72,10 +72,14 @@ class SetupInfo(C.Structure):
         ("workgroups", C.c_int32), ("neumann_doublings", C.c_int32), ("abd_block_diagonal", C.c_int32),
         ("aca_relative_error", C.c_double), ("abd_inverse_residual", C.c_double),
         ("schedule", C.c_int32), ("pass_sync", C.c_int32), ("lambda_max", C.c_double),
+        ("poly_ratio", C.c_double), ("learned_out", C.c_double * 3), ("learned_used", C.c_int32),
+        ("fallback_reason", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        out = {name: getattr(self, name) for name, _ in self._fields_}
+        out["learned_out"] = list(self.learned_out)
+        return out
 
 
 # exported symbol -> (restype, argtypes)
@@ -126,12 +130,8 @@ SIGNATURES = {
     "hsc_get_factors": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),
     "hsc_set_learned_weights": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
-    "hsc_solve_traced": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
-                                     C.c_int32, C.POINTER(C.c_int32), C.POINTER(Stats), C.c_void_p]),
     "hsc_get_node_graph": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
-    "hsc_get_node_features": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
-    "hsc_energy_targets": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
-                                       C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hsc_get_learned_features": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "hsc_mlp_param_count": (C.c_int32, []),
     "hsc_mlp_create": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
                                    C.POINTER(C.c_void_p)]),

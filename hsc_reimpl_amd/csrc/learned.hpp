@@ -40,11 +40,11 @@ hipError_t launch_mlp_loss(const float* O, const float* T, const float* den, int
 hipError_t launch_mlp_adam(float* theta, const float* grad, float* m, float* v, int count, float lr, int step,
                            const unsigned char* frozen, hipStream_t st);
 hipError_t launch_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt, hipStream_t st);
-hipError_t launch_mlp_features(const float* Dfeat, const double* r, double rho, int nb, bf16_t* X, hipStream_t st);
-hipError_t launch_energy_w(const float* O, const float* sdiag, double rho, const double* U, int nb, double* w,
-                           hipStream_t st);
-hipError_t launch_energy_T(const float* O, const float* sdiag, double rho, const double* Sw, double inv_den, int nb,
-                           float* T, hipStream_t st);
-hipError_t launch_mlp_apply(const float* O, const float* sdiag, double rho, int nb, double* z, hipStream_t st);
+// input features of the learned polynomial (nb x 16 bf16, from D only) and
+// the node mean of the network's 3 outputs (deterministic)
+hipError_t launch_poly_features(const int* rowptr, const int* colidx, const double* val64, const float* Dfeat,
+                                const float* sdiag, const int* act_node, int nb, float lam_host, const double* lam2,
+                                bf16_t* X, hipStream_t st);
+hipError_t launch_mean_rows3(const float* O, int nb, double* out, hipStream_t st);
 
 }  // namespace hsc

@@ -211,53 +211,66 @@ __global__ void k_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt)
   Wt[(size_t)n * K + k] = v;
 }
 
-// learned correction inside the preconditioner:
-// features x_i = [Dfeat_i (9), r_i/rho (3), 0 x 4] in bf16
-__global__ void k_mlp_features(const float* Dfeat /* nb x 9 */, const double* r, double rho, int nb, bf16_t* X) {
+// Per-node input of the learned polynomial (one thread per node), from D
+// only: x_i = [D_ii / s_i (9, row-major), log10(s_i / mean of the
+// neighbours' s_j) / 2, sum_{j != i} |D_ij|_F / |D_ii|_F / 4, degree / 16,
+// coupled (1 when node i carries W rows), log10(nb) / 6, lambda / 4, 0] in
+// bf16, s_i = tr(D_ii) / 3, lambda the spectrum estimate of Dinv D (the
+// square root of *lam2 when lam2 is given, else lam_host).
+__global__ void k_poly_features(const int* rowptr, const int* colidx, const double* val64, const float* Dfeat,
+                                const float* sdiag, const int* act_node, int nb, float lam_host,
+                                const double* lam2, bf16_t* X) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
-  for (int c = 0; c < 9; ++c) X[(size_t)i * 16 + c] = f2bf(Dfeat[i * 9 + c]);
-  for (int c = 0; c < 3; ++c) X[(size_t)i * 16 + 9 + c] = f2bf((float)(r[3 * i + c] / rho));
-  for (int c = 12; c < 16; ++c) X[(size_t)i * 16 + c] = 0;
+  const float lam = lam2 ? (float)sqrt(*lam2) : lam_host;
+  double ssum = 0.0, off = 0.0, dnorm = 0.0;
+  int deg = 0;
+  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+    const int j = colidx[p];
+    double f2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const double v = val64[val_index(p, c)];
+      f2 += v * v;
+    }
+    if (j == i) {
+      dnorm = sqrt(f2);
+    } else {
+      off += sqrt(f2);
+      ssum += (double)sdiag[j];
+      ++deg;
+    }
+  }
+  bf16_t* x = X + (size_t)i * 16;
+  for (int c = 0; c < 9; ++c) x[c] = f2bf(Dfeat[9 * i + c]);
+  const double si = (double)sdiag[i];
+  x[9] = f2bf(deg > 0 && ssum > 0.0 ? (float)(0.5 * log10(si * deg / ssum)) : 0.f);
+  x[10] = f2bf(dnorm > 0.0 ? (float)(0.25 * off / dnorm) : 0.f);
+  x[11] = f2bf((float)deg / 16.f);
+  x[12] = f2bf(act_node[i] >= 0 ? 1.f : 0.f);
+  x[13] = f2bf((float)(log10((double)nb) / 6.0));
+  x[14] = f2bf(0.25f * lam);
+  x[15] = 0;
 }
 
-// z_i += (rho / s_i) o_i
-__global__ void k_mlp_apply(const float* O, const float* sdiag, double rho, int nb, double* z) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  const double sc = rho / (double)sdiag[i];
-  for (int c = 0; c < 3; ++c) z[3 * i + c] += sc * (double)O[(size_t)i * 3 + c];
-}
-
-// energy-norm training signal: w = U - (rho / s_i) o_i
-__global__ void k_energy_w(const float* O, const float* sdiag, double rho, const double* U, int nb, double* w) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 3 * nb) return;
-  const double o = O ? (double)O[e] : 0.0;
-  w[e] = U[e] - rho / (double)sdiag[e / 3] * o;
-}
-
-// T = o + (rho / s_i) (S w)_i / den
-__global__ void k_energy_T(const float* O, const float* sdiag, double rho, const double* Sw, double inv_den, int nb,
-                           float* T) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 3 * nb) return;
-  const double o = O ? (double)O[e] : 0.0;
-  T[e] = (float)(o + rho / (double)sdiag[e / 3] * Sw[e] * inv_den);
+// Node mean of the 3 output columns, one workgroup in a fixed order:
+// out[c] = sum_i O[i][c] / nb (deterministic).
+__global__ __launch_bounds__(256) void k_mean_rows3(const float* O, int nb, double* out) {
+  __shared__ double red[3][256];
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += 256)
+    for (int c = 0; c < 3; ++c) a[c] += (double)O[(size_t)i * 3 + c];
+  for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = a[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[threadIdx.x] = nb > 0 ? red[threadIdx.x][0] / (double)nb : 0.0;
 }
 
 static inline int cdivl(long long a, long long b) { return (int)((a + b - 1) / b); }
-
-hipError_t launch_energy_w(const float* O, const float* sdiag, double rho, const double* U, int nb, double* w,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(k_energy_w, dim3(cdivl(3LL * nb, 256)), dim3(256), 0, st, O, sdiag, rho, U, nb, w);
-  return hipGetLastError();
-}
-hipError_t launch_energy_T(const float* O, const float* sdiag, double rho, const double* Sw, double inv_den, int nb,
-                           float* T, hipStream_t st) {
-  hipLaunchKernelGGL(k_energy_T, dim3(cdivl(3LL * nb, 256)), dim3(256), 0, st, O, sdiag, rho, Sw, inv_den, nb, T);
-  return hipGetLastError();
-}
 
 hipError_t launch_mlp_gemm(const MlpGemm& g, hipStream_t st) {
   if (g.M == 0) return hipSuccess;
@@ -308,12 +321,16 @@ hipError_t launch_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt,
   hipLaunchKernelGGL(k_mlp_cast, dim3(cdivl(K * N, 256)), dim3(256), 0, st, W, K, N, Wb, Wt);
   return hipGetLastError();
 }
-hipError_t launch_mlp_features(const float* Dfeat, const double* r, double rho, int nb, bf16_t* X, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_features, dim3(cdivl(nb, 256)), dim3(256), 0, st, Dfeat, r, rho, nb, X);
+hipError_t launch_poly_features(const int* rowptr, const int* colidx, const double* val64, const float* Dfeat,
+                                const float* sdiag, const int* act_node, int nb, float lam_host, const double* lam2,
+                                bf16_t* X, hipStream_t st) {
+  if (nb > 0)
+    hipLaunchKernelGGL(k_poly_features, dim3(cdivl(nb, 256)), dim3(256), 0, st, rowptr, colidx, val64, Dfeat, sdiag,
+                       act_node, nb, lam_host, lam2, X);
   return hipGetLastError();
 }
-hipError_t launch_mlp_apply(const float* O, const float* sdiag, double rho, int nb, double* z, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_apply, dim3(cdivl(nb, 256)), dim3(256), 0, st, O, sdiag, rho, nb, z);
+hipError_t launch_mean_rows3(const float* O, int nb, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_mean_rows3, dim3(1), dim3(256), 0, st, O, nb, out);
   return hipGetLastError();
 }
 

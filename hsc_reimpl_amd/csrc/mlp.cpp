@@ -38,6 +38,8 @@ hipError_t MlpNet::init(int nb, const int* nbr_ptr_dev, const int* nbr_dev, int 
                         hipStream_t st) {
   using L = MlpLayout;
   nb_ = nb;
+  cap_nb_ = nb;
+  cap_edges_ = n_edges;
   max_batch_ = max_batch;
   const size_t M = (size_t)nb * max_batch;
   TRY(dmalloc(&ptr_, nb + 1));
@@ -85,6 +87,14 @@ hipError_t MlpNet::set_params(const float* theta, bool device_ptr, hipStream_t s
 hipError_t MlpNet::get_params(float* theta_host, hipStream_t st) {
   TRY(hipMemcpyAsync(theta_host, theta_, sizeof(float) * MlpLayout::count, hipMemcpyDeviceToHost, st));
   return hipStreamSynchronize(st);
+}
+
+hipError_t MlpNet::set_graph(int nb, const int* nbr_ptr_dev, const int* nbr_dev, int n_edges, hipStream_t st) {
+  if (!fits(nb, n_edges)) return hipErrorInvalidValue;
+  nb_ = nb;
+  TRY(hipMemcpyAsync(ptr_, nbr_ptr_dev, sizeof(int) * (nb + 1), hipMemcpyDefault, st));
+  if (n_edges) TRY(hipMemcpyAsync(nbr_, nbr_dev, sizeof(int) * n_edges, hipMemcpyDefault, st));
+  return hipSuccess;
 }
 
 hipError_t MlpNet::forward(const bf16_t* X, int B, float* O, hipStream_t st) {

@@ -34,6 +34,9 @@ class MlpNet {
 
   // graph: neighbour CSR without self loops (device pointers are copied)
   hipError_t init(int nb, const int* nbr_ptr_dev, const int* nbr_dev, int n_edges, int max_batch, hipStream_t st);
+  // another graph of at most the nodes and edges init was given (copied; no allocation)
+  hipError_t set_graph(int nb, const int* nbr_ptr_dev, const int* nbr_dev, int n_edges, hipStream_t st);
+  bool fits(int nb, int n_edges) const { return nb <= cap_nb_ && n_edges <= cap_edges_; }
   hipError_t set_params(const float* theta, bool device_ptr, hipStream_t st);
   hipError_t get_params(float* theta_host, hipStream_t st);
   hipError_t forward(const bf16_t* X, int B, float* O, hipStream_t st);
@@ -45,7 +48,7 @@ class MlpNet {
 
  private:
   hipError_t cast_all(hipStream_t st);
-  int nb_ = 0, max_batch_ = 0, step_ = 0;
+  int nb_ = 0, max_batch_ = 0, step_ = 0, cap_nb_ = 0, cap_edges_ = 0;
   int* ptr_ = nullptr;
   int* nbr_ = nullptr;
   float* theta_ = nullptr;

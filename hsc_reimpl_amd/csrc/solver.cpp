@@ -237,11 +237,16 @@ struct hsc_handle {
   Buf barrier, errflag, respart, profbuf;  // barrier/errflag: setup kernels (ACA)
   PcgScratch pcg;                           // the persistent PCG launch of hsc_solve / hsc_solve_batched
   // learned preconditioner: graph MLP over D's node graph
-  Buf learned, nbr_ptr, nbr, dfeat, sdiag, xfeat, omlp, ptmp, dotbuf, ebuf;
+  Buf learned, nbr_ptr, nbr, dfeat, sdiag, xfeat, omlp, lpred;
+  double poly_ratio = 0.0;       // interval ratio b / a of the polynomial passes (Chebyshev or learned; else 0)
+  double learned_out[3] = {0.0, 0.0, 0.0};  // the network's channel means at the last setup
+  int learned_used = 0;          // the last setup's passes follow the network's interval
+  int fallback_reason = 0;       // 2: the network's output was rejected (Neumann weights used)
   Buf pwr;  // power iteration (Chebyshev weights)
   long long learned_bytes = 0;
+  long long learned_version = 0;       // bumped by each hsc_set_learned_weights
+  long long mlp_weights_version = -1;  // the version the network's bf16 copies were cast from
   std::unique_ptr<hsc::MlpNet> mlp;
-  int mlp_nb = -1;
   // setup: a second stream runs the D-side work (row sort, value gather,
   // power iteration) beside ACA; pinned staging for the setup's small results
   hipStream_t aux = nullptr;
@@ -286,6 +291,10 @@ int validate_config(const hsc_config& c, std::string& why) {
   }
   if (c.wg_per_cu < 0 || c.wg_per_cu > 4) {
     why = "wg_per_cu must be in 0..4";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
+  if (c.preconditioner == HSC_PRECOND_LEARNED && c.neumann_order > hsc::kMaxPolySteps) {
+    why = "learned preconditioner: neumann_order <= 32 required";
     return HSC_ERR_INVALID_ARGUMENT;
   }
   if (c.preconditioner == HSC_PRECOND_CHEBYSHEV &&
@@ -850,11 +859,13 @@ void prepare_launch(hsc_handle* h, hipStream_t st, bool retune = false) {
 // 1 - mu / mu_j of the residual polynomial keeps the product below 1.
 constexpr int kPowerSteps = 12;
 
-// Device buffer of the power iteration: x, z (n each), two partial arrays.
+// Device buffer of the power iteration: x, z (n each), two partial arrays,
+// the result |Dinv D x_11|^2.
 size_t power_buffer_doubles(int n, int nb) {
   const size_t cap = (size_t)std::max(hsc::power_step_tiles(8, nb), 256);
   return 2 * (size_t)n + 2 * cap + 1;
 }
+const double* power_lam2_dev(const hsc_handle* h);
 
 // Enqueue the power iteration for lambda_max(Dinv D) on st (D's rows and
 // inverse diagonal blocks ready there): 12 fused steps from a fixed
@@ -885,18 +896,40 @@ void enqueue_power_iteration(hsc_handle* h, hipStream_t st, double* lam2_host) {
   HIPCHK(hipMemcpyAsync(lam2_host, lam2, sizeof(double), hipMemcpyDeviceToHost, st));
 }
 
-// Per-pass weights: omega for the Neumann series; for the Chebyshev
-// preconditioner the reciprocal Chebyshev roots of [b / ratio, b] with
-// b = 1.2 x sqrt(lam2) (see the comment above).
-void set_poly_weights(hsc_handle* h, double lam2) {
+int interval_from_prediction(hsc_handle* h, const double pred[3], double* ratio);
+void enqueue_prediction(hsc_handle* h, hipStream_t st, const double* lam2_dev, double* pred_host);
+
+// Per-pass weights: omega for the Neumann series (and for the learned
+// preconditioner when its network's output is rejected); for the Chebyshev
+// and learned preconditioners the reciprocal Chebyshev roots of [b / ratio,
+// b] with b = 1.2 x sqrt(lam2) (see the comment above), the ratio from the
+// config (Chebyshev) or from the network's node means `pred` (learned).
+void set_poly_weights(hsc_handle* h, double lam2, const double* pred) {
   const int m = h->cfg.neumann_order;
   for (int j = 0; j < hsc::kMaxPolySteps; ++j) h->omegas[j] = h->cfg.omega;
   h->lambda_max = 0.0;
-  if (h->cfg.preconditioner != HSC_PRECOND_CHEBYSHEV) return;
+  h->poly_ratio = 0.0;
+  h->learned_used = 0;
+  h->fallback_reason = 0;
+  if (h->cfg.preconditioner == HSC_PRECOND_NEUMANN) return;
   const double lam = std::sqrt(lam2);
   if (!(lam > 0) || !std::isfinite(lam)) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: spectrum estimate failed"};
   h->lambda_max = lam;
-  const double b = 1.2 * lam, a = b / h->cfg.poly_interval_ratio;
+  double ratio = h->cfg.poly_interval_ratio;
+  if (h->cfg.preconditioner == HSC_PRECOND_LEARNED) {
+    const int why = interval_from_prediction(h, pred, &ratio);
+    if (why) {
+      // the plain Neumann series, its weight capped at 2 / b so that every
+      // factor 1 - w mu stays in [-1, 1) on (0, b]: SPD at any order
+      h->fallback_reason = why;
+      const double w = std::min(h->cfg.omega, 2.0 / (1.2 * lam));
+      for (int j = 0; j < hsc::kMaxPolySteps; ++j) h->omegas[j] = w;
+      return;
+    }
+    h->learned_used = 1;
+  }
+  h->poly_ratio = ratio;
+  const double b = 1.2 * lam, a = b / ratio;
   std::vector<double> mu(m);
   for (int j = 0; j < m; ++j) mu[j] = 0.5 * (b + a) + 0.5 * (b - a) * std::cos((2.0 * j + 1.0) * M_PI / (2.0 * m));
   std::sort(mu.begin(), mu.end());
@@ -934,6 +967,7 @@ struct SetupStage {
   int aca_err, max_coupled;
   double dm[4];
   double lam2, fst;
+  double pred[4];  // learned: the network's node-mean outputs
   double hist[hsc::kMaxRankLds + 1];
 };
 
@@ -1041,7 +1075,8 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->ybuf.ensure(sizeof(double) * lim1));
   HIPCHK(h->misc.ensure(sizeof(double) * 64));
   HIPCHK(h->respart.ensure(sizeof(double) * 3 * ((fn + hsc::kBlock - 1) / hsc::kBlock + qm + 1)));
-  const bool cheb = h->cfg.preconditioner == HSC_PRECOND_CHEBYSHEV;
+  // the polynomial preconditioners (Chebyshev, learned) need the spectrum bound
+  const bool cheb = h->cfg.preconditioner != HSC_PRECOND_NEUMANN;
   if (cheb) HIPCHK(h->pwr.ensure(sizeof(double) * power_buffer_doubles(fn, nb)));
   if (!h->aux) {
     HIPCHK(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
@@ -1128,6 +1163,9 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                                  nnzb, h->colidx.as<int>(), h->val64.as<double>(), h->val32.as<float>(), h->aux));
   ss->lam2 = 0.0;
   if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
+  // learned: the network runs on the same stream behind the power iteration
+  // (it reads the spectrum bound on the device), hidden behind ACA
+  if (h->cfg.preconditioner == HSC_PRECOND_LEARNED) enqueue_prediction(h, h->aux, power_lam2_dev(h), ss->pred);
   ss->aca_err = 0;
   HIPCHK(hipMemcpyAsync(ss->ints2, err + 8, sizeof(ss->ints2), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&ss->max_coupled, err + 12, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1164,24 +1202,23 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     if (ss->fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
   }
   HIPCHK(hipStreamSynchronize(h->aux));
-  set_poly_weights(h, ss->lam2);
-  if (h->cfg.preconditioner != HSC_PRECOND_LEARNED) prepare_launch(h, st);
+  set_poly_weights(h, ss->lam2, ss->pred);
+  prepare_launch(h, st);
   h->setup_ok = true;
   h->setup_ms = ms_since(t0);
 }
 
-// ------------------------------------------------------------------ learned / traced PCG
-// Host-driven (flexible) PCG over the same operators (D, W W^T, Neumann(D)):
-// used for the learned preconditioner (Polak-Ribiere beta, per-iteration
-// fallback to the plain Neumann z when the corrected z is not a descent
-// direction) and for harvesting (r_k, x_k) snapshots of the plain solver.
-struct FpcgOpts {
-  bool learned = false;
-  double* trace_r = nullptr;
-  double* trace_x = nullptr;
-  int every = 0, max_slots = 0;
-  int* nslots = nullptr;
-};
+// ------------------------------------------------------------------ learned polynomial
+// The learned preconditioner keeps the Chebyshev-weighted block-Jacobi
+// passes (a fixed SPD polynomial of Dinv D, so plain PCG applies) and lets a
+// graph MLP choose the polynomial's interval: at setup the network reads
+// per-node features of D, its 3 outputs are averaged over the nodes, and the
+// mean predicts log10 of the interval ratio b / a for degrees 8, 12 and 16
+// (channel value = log10(ratio) - 2.5; other degrees interpolate between
+// channels, beyond 16 the last one).  An output that is not finite or lies
+// outside log10(ratio) in [1, 4.5] is rejected and the plain Neumann series
+// (weight omega every pass) is used instead.
+constexpr double kLearnedCenter = 2.5, kLearnedMinLog = 1.0, kLearnedMaxLog = 4.5;
 
 void ensure_graph(hsc_handle* h, hipStream_t st) {
   const int nb = h->nb, ne = h->nnzb - h->nb;
@@ -1194,135 +1231,66 @@ void ensure_graph(hsc_handle* h, hipStream_t st) {
                                    h->dfeat.as<float>(), h->sdiag.as<float>(), st));
 }
 
+// The handle's network over its current node graph (re-pointed at each setup;
+// reallocated only for a larger graph), with the loaded weights (cast to
+// bf16 again only when they changed).
 void ensure_mlp(hsc_handle* h, hipStream_t st) {
   if (h->learned_bytes != (long long)sizeof(float) * hsc::MlpLayout::count)
     throw HscError{HSC_ERR_INVALID_ARGUMENT, "learned preconditioner selected but no weights set"};
   ensure_graph(h, st);
-  if (!h->mlp || h->mlp_nb != h->nb) {
+  const int ne = h->nnzb - h->nb;
+  if (!h->mlp || !h->mlp->fits(h->nb, ne)) {
     h->mlp.reset(new hsc::MlpNet());
-    HIPCHK(h->mlp->init(h->nb, h->nbr_ptr.as<int>(), h->nbr.as<int>(), h->nnzb - h->nb, 1, st));
-    h->mlp_nb = h->nb;
+    HIPCHK(h->mlp->init(h->nb, h->nbr_ptr.as<int>(), h->nbr.as<int>(), ne, 1, st));
+    h->mlp_weights_version = -1;
+  } else {
+    HIPCHK(h->mlp->set_graph(h->nb, h->nbr_ptr.as<int>(), h->nbr.as<int>(), ne, st));
   }
-  HIPCHK(h->mlp->set_params(h->learned.as<float>(), true, st));
+  if (h->mlp_weights_version != h->learned_version) {
+    HIPCHK(h->mlp->set_params(h->learned.as<float>(), true, st));
+    h->mlp_weights_version = h->learned_version;
+  }
   HIPCHK(h->xfeat.ensure(sizeof(hsc::bf16_t) * 16 * h->nb));
   HIPCHK(h->omlp.ensure(sizeof(float) * 3 * h->nb));
+  HIPCHK(h->lpred.ensure(sizeof(double) * 4));
 }
 
-int fpcg_solve(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStream_t st, const FpcgOpts& o) {
-  auto t0 = Clock::now();
-  const int n = h->fn;
-  HIPCHK(h->ptmp.ensure(sizeof(double) * n));
-  HIPCHK(h->dotbuf.ensure(sizeof(double) * 300));
-  hsc::SysDesc S = make_desc(h);
-  const int G = 8;  // a fixed lane split: the same sums whatever shape the persistent launch was tuned to
-  double* r = h->r.as<double>();
-  double* rhs = h->rhs.as<double>();
-  double* p = h->p0.as<double>();
-  double* Ap = h->q.as<double>();
-  double* z = h->z0.as<double>();
-  double* zold = h->z1.as<double>();
-  double* tmp = h->ptmp.as<double>();
-  double* xf = x + h->fo;
-  double* part = h->dotbuf.as<double>();
-  double* dout = part + 290;
-  auto dot = [&](const double* a, const double* c) {
-    HIPCHK(hsc::launch_dot(a, c, n, part, dout, st));
-    double v = 0.0;
-    HIPCHK(hipMemcpyAsync(&v, dout, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return v;
-  };
-  auto schur = [&](const double* in, double* out) {
-    HIPCHK(hsc::launch_op_Dx64(G, S, in, out, st));
-    HIPCHK(hsc::launch_op_lowrank64(S, in, out, h->ybuf.as<double>(), st));
-  };
-  int fallbacks = 0;
-  bool corrected = false;  // the current z carries a learned correction
-  // z = M(r) (+ learned correction); returns r.z
-  auto precondition = [&](double rr) {
-    HIPCHK(hsc::launch_op_precond64(G, S, h->cfg.neumann_order, h->omegas, r, z, tmp, st));
-    corrected = false;
-    if (!o.learned) return dot(r, z);
-    HIPCHK(hipMemcpyAsync(tmp, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    const double rho = std::sqrt(rr / n) > 0 ? std::sqrt(rr / n) : 1.0;
-    HIPCHK(hsc::launch_mlp_features(h->dfeat.as<float>(), r, rho, h->nb, h->xfeat.as<hsc::bf16_t>(), st));
-    HIPCHK(h->mlp->forward(h->xfeat.as<hsc::bf16_t>(), 1, h->omlp.as<float>(), st));
-    HIPCHK(hsc::launch_mlp_apply(h->omlp.as<float>(), h->sdiag.as<float>(), rho, h->nb, z, st));
-    const double rz = dot(r, z);
-    if (!(std::isfinite(rz) && rz > 0)) {  // fall back to the plain Neumann z
-      ++fallbacks;
-      HIPCHK(hipMemcpyAsync(z, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-      return dot(r, z);
-    }
-    corrected = true;
-    return rz;
-  };
-  if (o.learned) ensure_mlp(h, st);
-  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, n, h->k,
-                         h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(), h->ybuf.as<double>(),
-                         rhs, r, x, st));
-  const double nb2 = dot(rhs, rhs);
-  const double normb = std::sqrt(nb2);
-  double rel = normb > 0 ? 1.0 : 0.0;
-  int it = 0, restarts = 0, checks = 0, slots = 0, status = HSC_OK;
-  const double tol = h->cfg.tol_rate;
-  bool prev_corrected = false;
-  if (normb > 0) {
-    double rz = precondition(nb2);
-    prev_corrected = corrected;
-    HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    while (it < h->cfg.max_iter && rel > tol) {
-      schur(p, Ap);
-      const double pAp = dot(p, Ap);
-      if (!(std::isfinite(pAp) && pAp > 0 && std::isfinite(rz) && rz > 0)) {
-        status = HSC_ERR_BREAKDOWN;
-        break;
-      }
-      const double alpha = rz / pAp;
-      HIPCHK(hsc::launch_update_xr(xf, r, p, Ap, alpha, n, st));
-      ++it;
-      if (o.every > 0 && it % o.every == 0 && slots < o.max_slots) {
-        HIPCHK(hipMemcpyAsync(o.trace_r + (size_t)slots * n, r, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipMemcpyAsync(o.trace_x + (size_t)slots * n, xf, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        ++slots;
-      }
-      double rr = dot(r, r);
-      rel = std::sqrt(rr) / normb;
-      bool restart = false;
-      if (rel <= tol) {  // one true residual check (linear_hsc.cu:375-387)
-        ++checks;
-        schur(xf, Ap);
-        HIPCHK(hipMemcpyAsync(r, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hsc::launch_axpby(r, Ap, -1.0, 1.0, n, st));
-        rr = dot(r, r);
-        rel = std::sqrt(rr) / normb;
-        if (rel <= tol) break;
-        restart = true;
-        ++restarts;
-      }
-      HIPCHK(hipMemcpyAsync(zold, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-      const double rzn = precondition(rr);
-      double beta = 0.0;
-      if (!restart) {
-        // Polak-Ribiere (flexible) whenever the preconditioner varied; with
-        // every correction rejected this is exactly the plain PCG recurrence
-        if (corrected || prev_corrected) beta = (rzn - dot(r, zold)) / rz;
-        else beta = rzn / rz;
-      }
-      prev_corrected = corrected;
-      HIPCHK(hsc::launch_axpby(p, z, 1.0, beta, n, st));
-      rz = rzn;
-    }
-    if (status == HSC_OK && rel > tol) status = HSC_ERR_NOT_CONVERGED;
-  }
-  if (o.nslots) *o.nslots = slots;
-  s->pcg_iterations = it;
-  s->schur_relative_residual = rel;
-  s->restarts = restarts;
-  s->true_residual_checks = checks;
-  s->fallback = fallbacks > 0;
-  s->pcg_ms = ms_since(t0);
-  return status;
+// The network's input features of h's system (graph and D's diagonal
+// features built on st first); the spectrum bound is read from lam2_dev
+// (|Dinv D x|^2 of the power iteration) when given, else h->lambda_max.
+void enqueue_learned_features(hsc_handle* h, hipStream_t st, const double* lam2_dev) {
+  HIPCHK(hsc::launch_poly_features(h->rowptr.as<int>(), h->colidx.as<int>(), h->val64.as<double>(),
+                                   h->dfeat.as<float>(), h->sdiag.as<float>(), h->act_node.as<int>(), h->nb,
+                                   (float)h->lambda_max, lam2_dev, h->xfeat.as<hsc::bf16_t>(), st));
+}
+
+// Enqueue the network on h's system on st (behind D's rows and the power
+// iteration there): features, forward, node mean of the 3 outputs, copied
+// to pred_host (pinned) once st drains.
+void enqueue_prediction(hsc_handle* h, hipStream_t st, const double* lam2_dev, double* pred_host) {
+  ensure_mlp(h, st);
+  enqueue_learned_features(h, st, lam2_dev);
+  HIPCHK(h->mlp->forward(h->xfeat.as<hsc::bf16_t>(), 1, h->omlp.as<float>(), st));
+  HIPCHK(hsc::launch_mean_rows3(h->omlp.as<float>(), h->nb, h->lpred.as<double>(), st));
+  HIPCHK(hipMemcpyAsync(pred_host, h->lpred.p, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+}
+
+// The interval ratio for h's degree from the network's 3 node means; returns
+// 0 (accepted, *ratio set) or the fallback reason 2 (rejected).
+int interval_from_prediction(hsc_handle* h, const double pred[3], double* ratio) {
+  for (int c = 0; c < 3; ++c) h->learned_out[c] = pred[c];
+  const double t = std::min(std::max((h->cfg.neumann_order - 8) / 4.0, 0.0), 2.0);
+  const int c0 = (int)std::floor(t), c1 = std::min(c0 + 1, 2);
+  const double u = (1.0 - (t - c0)) * pred[c0] + (t - c0) * pred[c1];
+  const double lg = u + kLearnedCenter;
+  if (!std::isfinite(lg) || lg < kLearnedMinLog || lg > kLearnedMaxLog) return 2;
+  *ratio = std::pow(10.0, lg);
+  return 0;
+}
+
+const double* power_lam2_dev(const hsc_handle* h) {
+  const size_t cap = (size_t)std::max(hsc::power_step_tiles(8, h->nb), 256);
+  return h->pwr.as<double>() + 2 * (size_t)h->fn + 2 * cap;
 }
 
 // Algorithmic HBM bytes of one Schur-PCG iteration of h's system (every
@@ -1367,6 +1335,9 @@ void fill_common_stats(hsc_handle* h, hsc_stats* s) {
   s->full_residual_limit = h->cfg.full_residual_limit;
   s->precision = h->cfg.precision;
   s->preconditioner = h->cfg.preconditioner;
+  s->learned_used = h->learned_used;
+  s->fallback = h->fallback_reason != 0;
+  s->fallback_reason = h->fallback_reason;
   s->setup_ms = h->setup_ms;
 }
 
@@ -1443,18 +1414,6 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   hsc_stats tmp;
   if (!s) s = &tmp;
   fill_common_stats(h, s);
-  if (h->cfg.preconditioner == HSC_PRECOND_LEARNED) {
-    FpcgOpts o;
-    o.learned = true;
-    int code = fpcg_solve(h, b, x, s, st, o);
-    if (code == HSC_ERR_BREAKDOWN) h->err = "HSC flexible PCG breakdown. No fallback.";
-    if (code == HSC_ERR_NOT_CONVERGED) h->err = "HSC flexible PCG did not converge";
-    int c2 = finish_solve(h, b, x, s, st);
-    if (code == HSC_OK) code = c2;
-    s->status = code;
-    s->total_ms = ms_since(t0);
-    return code;
-  }
   hsc_handle* hs[1] = {h};
   // a reconfiguration after setup that changed the launch re-plans it here
   if (!h->launch_ready) prepare_launch(h, st);
@@ -1726,7 +1685,6 @@ int validate_batch(hsc_handle* const* handles, int32_t count, const double* cons
     if (!b[i] || !x[i]) return HSC_ERR_INVALID_ARGUMENT;
     if (handles[i]->device != h0->device || handles[i]->cfg.precision != h0->cfg.precision ||
         handles[i]->cfg.neumann_order != h0->cfg.neumann_order || handles[i]->cfg.omega != h0->cfg.omega ||
-        handles[i]->cfg.preconditioner == HSC_PRECOND_LEARNED ||
         handles[i]->cfg.preconditioner != h0->cfg.preconditioner)
       return HSC_ERR_INVALID_ARGUMENT;
     for (int j = 0; j < i; ++j)
@@ -2136,6 +2094,10 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   o->abd_block_diagonal = h->blockdiag ? 1 : 0;
   o->aca_relative_error = h->aca_err;
   o->abd_inverse_residual = h->abd_res;
+  o->poly_ratio = h->poly_ratio;
+  for (int c = 0; c < 3; ++c) o->learned_out[c] = h->learned_out[c];
+  o->learned_used = h->learned_used;
+  o->fallback_reason = h->fallback_reason;
   return h->setup_ok ? HSC_OK : HSC_ERR_NOT_SETUP;
 }
 
@@ -2166,32 +2128,8 @@ int32_t hsc_set_learned_weights(hsc_handle* h, const void* w, int64_t bytes) {
     HIPCHK(h->learned.ensure((size_t)std::max<int64_t>(bytes, 4)));
     if (bytes) HIPCHK(hipMemcpy(h->learned.p, w, (size_t)bytes, hipMemcpyDefault));
     h->learned_bytes = bytes;
+    ++h->learned_version;
     return HSC_OK;
-  });
-}
-
-int32_t hsc_solve_traced(hsc_handle* h, const double* b, double* x, double* trace_r, double* trace_x, int32_t every,
-                         int32_t max_slots, int32_t* nslots, hsc_stats* stats, hsc_stream stream) {
-  if (!h || !b || !x || (every > 0 && (!trace_r || !trace_x || max_slots < 0))) return HSC_ERR_INVALID_ARGUMENT;
-  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
-  DeviceGuard g(h->device);
-  auto t0 = Clock::now();
-  hsc_stats tmp;
-  hsc_stats* s = stats ? stats : &tmp;
-  return guarded(h, [&] {
-    fill_common_stats(h, s);
-    FpcgOpts o;
-    o.trace_r = trace_r;
-    o.trace_x = trace_x;
-    o.every = every;
-    o.max_slots = max_slots;
-    o.nslots = nslots;
-    int code = fpcg_solve(h, b, x, s, (hipStream_t)stream, o);
-    int c2 = finish_solve(h, b, x, s, (hipStream_t)stream);
-    if (code == HSC_OK) code = c2;
-    s->status = code;
-    s->total_ms = ms_since(t0);
-    return code;
   });
 }
 
@@ -2210,44 +2148,15 @@ int32_t hsc_get_node_graph(hsc_handle* h, int32_t* nbr_ptr, int32_t* nbr, int64_
   });
 }
 
-int32_t hsc_get_node_features(hsc_handle* h, float* dfeat, float* sdiag) {
-  if (!h || !dfeat || !sdiag) return HSC_ERR_INVALID_ARGUMENT;
+int32_t hsc_get_learned_features(hsc_handle* h, uint16_t* X) {
+  if (!h || !X) return HSC_ERR_INVALID_ARGUMENT;
   if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
   DeviceGuard g(h->device);
   return guarded(h, [&] {
     ensure_graph(h, nullptr);
-    HIPCHK(hipMemcpy(dfeat, h->dfeat.p, sizeof(float) * 9 * h->nb, hipMemcpyDefault));
-    HIPCHK(hipMemcpy(sdiag, h->sdiag.p, sizeof(float) * h->nb, hipMemcpyDefault));
-    return HSC_OK;
-  });
-}
-
-int32_t hsc_energy_targets(hsc_handle* h, const float* O, const double* U, const double* rho, const double* den,
-                           int32_t B, float* T, double* energy, hsc_stream stream) {
-  if (!h || !U || !rho || !energy || B <= 0 || (T && !den)) return HSC_ERR_INVALID_ARGUMENT;
-  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
-  DeviceGuard g(h->device);
-  return guarded(h, [&] {
-    hipStream_t st = (hipStream_t)stream;
-    const int n = h->fn, nb = h->nb;
-    ensure_graph(h, st);
-    HIPCHK(h->ebuf.ensure(sizeof(double) * (2 * (size_t)n + B)));
-    HIPCHK(h->dotbuf.ensure(sizeof(double) * 300));
-    double* w = h->ebuf.as<double>();
-    double* Sw = w + n;
-    double* en = Sw + n;
-    hsc::SysDesc S = make_desc(h);
-    for (int b = 0; b < B; ++b) {
-      if (T && !(den[b] > 0)) throw HscError{HSC_ERR_INVALID_ARGUMENT, "energy targets: den must be positive"};
-      const float* Ob = O ? O + (size_t)b * n : nullptr;
-      HIPCHK(hsc::launch_energy_w(Ob, h->sdiag.as<float>(), rho[b], U + (size_t)b * n, nb, w, st));
-      HIPCHK(hsc::launch_op_Dx64(h->G, S, w, Sw, st));
-      HIPCHK(hsc::launch_op_lowrank64(S, w, Sw, h->ybuf.as<double>(), st));
-      HIPCHK(hsc::launch_dot(w, Sw, n, h->dotbuf.as<double>(), en + b, st));
-      if (T) HIPCHK(hsc::launch_energy_T(Ob, h->sdiag.as<float>(), rho[b], Sw, 1.0 / den[b], nb, T + (size_t)b * n, st));
-    }
-    HIPCHK(hipMemcpyAsync(energy, en, sizeof(double) * B, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(h->xfeat.ensure(sizeof(hsc::bf16_t) * 16 * h->nb));
+    enqueue_learned_features(h, nullptr, nullptr);
+    HIPCHK(hipMemcpy(X, h->xfeat.p, sizeof(hsc::bf16_t) * 16 * h->nb, hipMemcpyDefault));
     return HSC_OK;
   });
 }

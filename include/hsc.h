@@ -236,7 +236,12 @@ typedef struct hsc_setup_info {
   double aca_relative_error, abd_inverse_residual;
   int32_t schedule;              /* lanes_per_row, workgroups (per CU), schedule: the last solve's launch */
   int32_t pass_sync;             /* 1 when the last solve's preconditioner passes synchronised per tile */
-  double lambda_max;             /* power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights; else 0) */
+  double lambda_max;             /* power-iteration estimate of lambda_max(Dinv D) (Chebyshev or learned; else 0) */
+  double poly_ratio;             /* interval ratio b/a of the polynomial passes (Chebyshev: the config's; learned:
+                                    the network's); 0 for the Neumann series or the learned fallback */
+  double learned_out[3];         /* learned: the network's node-mean outputs (log10(b/a) - 2.5 at degrees 8, 12, 16) */
+  int32_t learned_used;          /* 1 when the network's interval is in use */
+  int32_t fallback_reason;       /* 2 when the network's output was rejected (Neumann series used) */
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 
@@ -339,38 +344,41 @@ int32_t hsc_batch_info(const hsc_batch* batch, int32_t* lanes_per_row, int32_t* 
 void hsc_batch_destroy(hsc_batch* batch);
 
 /* ---- learned preconditioner (no reference counterpart) ----
- * A per-node graph MLP over D's node graph (2 message-passing layers, hidden
- * 64, BF16 weights/activations, FP32 accumulation) corrects the Neumann z
- * inside flexible PCG (Polak-Ribiere beta); when the corrected z is not a
- * descent direction the plain Neumann z is used for that iteration and
- * hsc_stats.fallback is set.  Selected by hsc_config.preconditioner. */
+ * preconditioner = HSC_PRECOND_LEARNED keeps the Chebyshev-weighted
+ * block-Jacobi passes of HSC_PRECOND_CHEBYSHEV (neumann_order passes, one
+ * fixed SPD polynomial of Dinv D per solve, so the persistent PCG runs
+ * unchanged, in either precision) and lets a per-node graph MLP over D's
+ * node graph (2 message-passing layers, hidden 64, BF16 weights and
+ * activations, FP32 accumulation) choose the polynomial's interval: at
+ * setup the network reads each node's features of D (hsc_get_learned_features),
+ * and the node mean of its 3 outputs predicts log10(b/a) - 2.5 for degrees
+ * 8, 12 and 16 (other degrees interpolate).  An output that is not finite
+ * or outside log10(b/a) in [1, 4.5] is rejected: the plain Neumann series
+ * is used (weight min(omega, 2/b), SPD at any order) and the stats report
+ * fallback = 1, fallback_reason = 2.  hsc_set_learned_weights loads the
+ * network's hsc_mlp_param_count() float32 parameters (host or device
+ * pointer); setup without them fails with HSC_ERR_INVALID_ARGUMENT. */
 int32_t hsc_set_learned_weights(hsc_handle* h, const void* theta_f32, int64_t bytes);
 
-/* Host-driven PCG of the plain solver (same operators) that stores r_k and
- * the FEM part of x_k every `every` iterations into trace_r/trace_x (device,
- * max_slots x fem_dofs doubles each): residual/error pairs for training. */
-int32_t hsc_solve_traced(hsc_handle* h, const double* b, double* x, double* trace_r, double* trace_x,
-                         int32_t every, int32_t max_slots, int32_t* nslots, hsc_stats* stats, hsc_stream stream);
-
-/* Node graph of D (neighbour CSR without self; device or host destination)
- * and per-node features D_ii / s_i (row-major 3x3) and s_i = tr(D_ii)/3. */
+/* Node graph of D (neighbour CSR without self; device or host destination). */
 int32_t hsc_get_node_graph(hsc_handle* h, int32_t* nbr_ptr, int32_t* nbr, int64_t* n_edges);
-int32_t hsc_get_node_features(hsc_handle* h, float* dfeat, float* sdiag);
 
-/* Energy-norm training signal of the learned correction.  For sample b
- * (U: [B x fem_dofs] = e - Neumann(r) of training pair b; O: [B x fem_dofs]
- * fp32 network outputs or null for zero; rho, den: host arrays of B):
- *   w_b = U_b - (rho_b / s_i) o_b,  energy_b = w_b^T S w_b  (host out),
- *   T_b = o_b + (rho_b / s_i) (S w_b)_i / den_b  (device out, if T != null),
- * so that hsc_mlp_train_step(X, T, den = 1) descends mean_b energy_b / den_b. */
-int32_t hsc_energy_targets(hsc_handle* h, const float* O, const double* U, const double* rho, const double* den,
-                           int32_t B, float* T, double* energy, hsc_stream stream);
+/* The network's input for h's set-up system: fem_nodes x 16 bfloat16 values
+ * per node (device or host destination): D_ii / s_i (9, row-major, s_i =
+ * tr(D_ii) / 3), log10(s_i / mean of the neighbours' s_j) / 2,
+ * sum_{j != i} |D_ij|_F / |D_ii|_F / 4, neighbour count / 16, 1 when the
+ * node is coupled to the ABD block (else 0), log10(fem_nodes) / 6,
+ * lambda_max / 4 (the setup's spectrum estimate of Dinv D; 0 for the
+ * Neumann preconditioner), 0. */
+int32_t hsc_get_learned_features(hsc_handle* h, uint16_t* X);
 
 /* The network object used for training on the GPU (hand-written BF16
- * forward/backward kernels + Adam).  X: [B x nb x 16] bf16 features
- * (D_ii/s_i, r_i/rms(r), 0 x 4); T: [B x nb x 3] fp32 targets; den: [B]
- * per-sample normalisers; O: [B x nb x 3] fp32 outputs.  Parameters are a
- * flat float32 vector of hsc_mlp_param_count() values. */
+ * forward/backward kernels + Adam) over one node graph (for training, the
+ * disjoint union of the training systems' graphs).  X: [B x nb x 16] bf16
+ * features; T: [B x nb x 3] fp32 targets; den: [B] per-sample normalisers
+ * (loss = mean_b sum_i |o_bi - t_bi|^2 / den_b); O: [B x nb x 3] fp32
+ * outputs.  Parameters are a flat float32 vector of hsc_mlp_param_count()
+ * values. */
 typedef struct hsc_mlp hsc_mlp;
 int32_t hsc_mlp_param_count(void);
 int32_t hsc_mlp_create(int32_t device, int32_t nb, const int32_t* nbr_ptr, const int32_t* nbr, int64_t n_edges,

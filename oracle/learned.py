@@ -7,16 +7,16 @@ the oracle itself is pinned by finite differences of its loss
 
 Network (per node i of a mesh graph N(i) = tet-edge neighbours, self
 excluded):
-    x_i  = [D_ii / s_i (9, row-major), r_i / rho (3), 0 x 4]   s_i = tr(D_ii)/3, rho = rms(r)
+    x_i  = poly_features(...)_i (16 values of D, see there)
     h0   = relu(x W0 + b0)                       W0: 16 x 64
     m1_i = mean_{j in N(i)} h0_j
     h1   = relu(h0 Ws1 + m1 Wn1 + b1)            64 x 64 each
     m2   = mean_{j in N(i)} h1_j
     h2   = relu(h1 Ws2 + m2 Wn2 + b2)
     o    = h2 Wo + bo                            Wo: 64 x 3 (padded to 64 x 32 on the GPU)
-    z_i  = zN_i + (rho / s_i) o_i                zN = Neumann(D) r
-Training pairs (r, e) with S e = r; target t_i = (e_i - zN_i) s_i / rho;
-per-sample loss sum_i |o_i - t_i|^2 / sum_i |t_i|^2, batch loss = mean.
+The node mean of o predicts log10(b/a) - 2.5 of the Chebyshev interval of
+the block-Jacobi passes at degrees 8, 12, 16 (predict_log_ratio).
+Loss of a batch: mean over samples of sum_i |o_i - t_i|^2 / den.
 
 With ``bf16=True`` the weights and every stored activation (x, h0, m1, h1,
 m2, h2) are rounded to bfloat16 (round to nearest even) exactly where the
@@ -50,9 +50,9 @@ def init_params(seed=0):
         if name.startswith("b"):
             p[name] = np.zeros(shp)
         else:
-            fan_in = shp[0] if name != "W0" else 12
+            fan_in = shp[0] if name != "W0" else 15
             p[name] = rng.normal(size=shp) * np.sqrt(2.0 / fan_in) * (0.1 if name == "Wo" else 1.0)
-    p["W0"][12:] = 0.0
+    p["W0"][15:] = 0.0
     return p
 
 
@@ -66,17 +66,58 @@ def graph_from_bsr(rowptr, colidx):
     return np.asarray(nbr_ptr, dtype=np.int64), np.asarray(nbr, dtype=np.int64)
 
 
-def node_features(Dblocks, r):
-    """Dblocks: (nb,3,3) diagonal blocks; r: (..., nb*3) residuals.
-    Returns x (..., nb, 16), s (nb,), rho (...,)."""
-    s = np.trace(Dblocks, axis1=1, axis2=2) / 3.0
-    r = np.asarray(r).reshape(r.shape[:-1] + (-1, 3))
-    rho = np.sqrt(np.mean(r ** 2, axis=(-2, -1)))
-    rho = np.where(rho > 0, rho, 1.0)
-    x = np.zeros(r.shape[:-1] + (IN,))
-    x[..., :9] = (Dblocks / s[:, None, None]).reshape(-1, 9)
-    x[..., 9:12] = r / rho[..., None, None]
-    return x, s, rho
+def poly_features(D, active_nodes, lam):
+    """Per-node input of the learned polynomial from the full symmetric FEM
+    block D (SciPy sparse, 3x3 node blocks), the coupled nodes and the
+    spectrum estimate lam of Dinv D: x_i = [D_ii / s_i (9, row-major),
+    log10(s_i / mean_{j in N(i)} s_j) / 2, sum_{j in N(i)} |D_ij|_F / |D_ii|_F / 4,
+    |N(i)| / 16, coupled, log10(nb) / 6, lam / 4, 0], s_i = tr(D_ii) / 3,
+    N(i) = the other block columns stored in block row i.  Returns x (nb, 16)
+    and the neighbour CSR (nbr_ptr, nbr)."""
+    Db = D.tobsr(blocksize=(3, 3))
+    Db.sort_indices()
+    nb = Db.shape[0] // 3
+    ptr, idx, blk = Db.indptr, Db.indices, Db.data
+    x = np.zeros((nb, IN))
+    diag = np.zeros((nb, 3, 3))
+    for i in range(nb):
+        for p in range(ptr[i], ptr[i + 1]):
+            if idx[p] == i:
+                diag[i] = blk[p]
+    s = np.trace(diag, axis1=1, axis2=2) / 3.0
+    s = np.where(s > 0, s, 1.0)
+    act = np.zeros(nb, dtype=bool)
+    act[np.asarray(active_nodes, dtype=np.int64)] = True
+    nbr_ptr, nbr = [0], []
+    for i in range(nb):
+        cols = [int(idx[p]) for p in range(ptr[i], ptr[i + 1]) if idx[p] != i]
+        offn = sum(np.linalg.norm(blk[p]) for p in range(ptr[i], ptr[i + 1]) if idx[p] != i)
+        nbr.extend(cols)
+        nbr_ptr.append(len(nbr))
+        deg = len(cols)
+        x[i, :9] = (diag[i] / s[i]).ravel()
+        ms = np.mean(s[cols]) if deg else 0.0
+        x[i, 9] = 0.5 * np.log10(s[i] / ms) if deg and ms > 0 else 0.0
+        dn = np.linalg.norm(diag[i])
+        x[i, 10] = 0.25 * offn / dn if dn > 0 else 0.0
+        x[i, 11] = deg / 16.0
+        x[i, 12] = 1.0 if act[i] else 0.0
+        x[i, 13] = np.log10(nb) / 6.0
+        x[i, 14] = 0.25 * lam
+    return x, np.asarray(nbr_ptr, dtype=np.int64), np.asarray(nbr, dtype=np.int64)
+
+
+def predict_log_ratio(p, x, nbr_ptr, nbr, degree, bf=True):
+    """log10(b/a) the network picks for `degree` passes: the node means of its
+    3 outputs are log10(b/a) - 2.5 at degrees 8, 12 and 16; other degrees
+    interpolate linearly between neighbouring channels (clamped to [8, 16]).
+    Returns (log10 ratio, channel means)."""
+    o, _ = forward(p, x, nbr_ptr, nbr, bf=bf)
+    u = o.reshape(-1, OUT).mean(axis=0)
+    t = min(max((degree - 8) / 4.0, 0.0), 2.0)
+    c0 = int(np.floor(t))
+    c1 = min(c0 + 1, 2)
+    return float((1 - (t - c0)) * u[c0] + (t - c0) * u[c1] + 2.5), u
 
 
 def aggregate(h, nbr_ptr, nbr):

@@ -37,7 +37,7 @@ def test_gradients_match_finite_differences():
         p[k] = p[k] + 0.05 * rng.normal(size=p[k].shape)
     B, nb = 2, len(ptr) - 1
     x = rng.normal(size=(B, nb, L.IN))
-    x[..., 12:] = 0.0
+    x[..., 15:] = 0.0
     t = rng.normal(size=(B, nb, 3))
     loss, g, _ = L.loss_and_grad(p, x, t, ptr, idx)
     eps = 1e-6
@@ -63,3 +63,43 @@ def test_aggregate_adjoint():
     lhs = np.sum(L.aggregate(h, ptr, idx) * g)
     rhs = np.sum(h * L.aggregate_T(g, ptr, idx))
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_poly_features_closed_form():
+    """A 3-node chain with D_ii = a_i I and D_ij = c I (i, j adjacent): every
+    feature has a closed form (s_i = a_i; neighbour mean of s; the off-diagonal
+    Frobenius sum over |D_ii|_F = deg c / a_i)."""
+    import scipy.sparse as sp
+    a = np.array([4.0, 10.0, 25.0])
+    c = -1.5
+    D = np.zeros((9, 9))
+    for i in range(3):
+        D[3 * i:3 * i + 3, 3 * i:3 * i + 3] = a[i] * np.eye(3)
+    for i, j in ((0, 1), (1, 2)):
+        D[3 * i:3 * i + 3, 3 * j:3 * j + 3] = c * np.eye(3)
+        D[3 * j:3 * j + 3, 3 * i:3 * i + 3] = c * np.eye(3)
+    x, ptr, nbr = L.poly_features(sp.csr_matrix(D), [2], lam=2.0)
+    assert ptr.tolist() == [0, 1, 3, 4] and nbr.tolist() == [1, 0, 2, 1]
+    np.testing.assert_allclose(x[:, :9], np.tile(np.eye(3).ravel(), (3, 1)))
+    np.testing.assert_allclose(x[:, 9], 0.5 * np.log10([4 / 10, 10 / 14.5, 25 / 10]))
+    np.testing.assert_allclose(x[:, 10], 0.25 * np.array([1, 2, 1]) * abs(c) / a)
+    np.testing.assert_allclose(x[:, 11], [1 / 16, 2 / 16, 1 / 16])
+    assert x[:, 12].tolist() == [0, 0, 1]
+    np.testing.assert_allclose(x[:, 13], np.log10(3) / 6)
+    np.testing.assert_allclose(x[:, 14], 0.5)
+    assert np.all(x[:, 15] == 0)
+
+
+def test_predict_log_ratio_channels():
+    """The node mean of the outputs picks the channel of the degree (8, 12,
+    16) and interpolates between them: a network whose output is its bias
+    (all weights zero) returns bias + 2.5 exactly."""
+    p = L.init_params(0)
+    for k in p:
+        p[k] = np.zeros_like(p[k])
+    p["bo"] = np.array([-0.5, 0.25, 1.0])
+    ptr, idx = _tiny_graph()
+    x = np.random.default_rng(0).normal(size=(len(ptr) - 1, L.IN))
+    for deg, want in ((4, 2.0), (8, 2.0), (10, 2.375), (12, 2.75), (14, 3.125), (16, 3.5), (30, 3.5)):
+        lg, u = L.predict_log_ratio(p, x, ptr, idx, deg, bf=False)
+        assert lg == want and np.allclose(u, p["bo"])
