@@ -342,6 +342,17 @@ def _dense_c1(reps=5):
             "solves_per_s": round(1.0 / (float(np.median(setup)) + float(np.median(solve))), 3)}
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), else the machine type."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_all_cores(order, omega, rtol, budget_s=5.0, max_solves=3):
     """The CPU measurements with the BLAS / OpenMP pools at the process's
     thread setting (a child process started with OMP_NUM_THREADS,
@@ -371,7 +382,7 @@ def cpu_baseline(sysm, order, omega, rtol, budget_s=10.0, max_solves=8):
     out = {"value": 1.0 / med, "unit": "solves/s", "cores": 1, "kind": "port",
            "sample": f"{k} full solves (setup + {iters} PCG iterations each) of the same 78k-tet system, "
                      f"median {med:.2f} s, {tot:.1f} s total; SciPy sparse matvecs, BLAS pinned to 1 thread",
-           "iterations": iters, "cpu": platform.processor() or platform.machine(),
+           "iterations": iters, "cpu": cpu_model(), "nproc": os.cpu_count(),
            "setup_s": round(m_setup, 3), "solve_s": round(m_solve, 3),
            "dense_2178_dof_one_thread": dict(_dense_c1(), what="core.py-form dense restatement "
                                                                "(oracle.hsc_core.HSC) on the 2,178-DOF system")}
@@ -401,8 +412,8 @@ def run_batch64(args, dev, order, omega, rtol):
     from hsc_reimpl_amd import plugin, systems
 
     t0 = time.perf_counter()
-    syss = systems.batch_systems(seed=2, progress=lambda i, s: (i % 8 == 7) and print(
-        f"[batch64] generated {i + 1} systems ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True))
+    syss = systems.batch_systems(seed=2, workers=min(16, os.cpu_count() or 1))
+    print(f"[batch64] generated 64 systems ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
     scene = {"linear_system/tol_rate": rtol, "linear_system/hsc/neumann_order": order,
              "linear_system/hsc/omega": omega, "linear_system/hsc/max_iter": 20000}
     cfg = dict(precision="mixed", residual_replace_every=25, preconditioner="chebyshev",

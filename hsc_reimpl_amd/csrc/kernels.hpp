@@ -174,16 +174,10 @@ hipError_t launch_power_step(int G, const SysDesc& S, const double* x, const dou
                              double* part, hipStream_t st);
 // fixed-order sum of np partials into *out
 hipError_t launch_sum_partials(const double* part, int np, double* out, hipStream_t st);
-hipError_t launch_normalize(double* x, const double* z, int n, const double* nrm2, double* out, int it,
-                            hipStream_t st);
 hipError_t launch_op_lowrank64(const SysDesc& S, const double* x, double* y, double* t, hipStream_t st);
 
-hipError_t launch_dot(const double* a, const double* b, int n, double* part, double* out, hipStream_t st);
 // partials of a.b only (*np_out <= 256 of them, into part)
 hipError_t launch_dot_partial(const double* a, const double* b, int n, double* part, int* np_out, hipStream_t st);
-hipError_t launch_update_xr(double* x, double* r, const double* p, const double* Ap, double alpha, int n,
-                            hipStream_t st);
-hipError_t launch_axpby(double* y, const double* x, double a, double b, int n, hipStream_t st);
 hipError_t launch_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr, hipStream_t st);
 hipError_t launch_node_features(const int* rowptr, const int* colidx, const double* val64, int nnzb, int nb,
                                 float* Dfeat, float* sdiag, hipStream_t st);

@@ -1365,18 +1365,6 @@ __global__ void k_dot_final(const double* part, int np, double* out) {
   if (threadIdx.x == 0) *out = s;
 }
 // x += alpha p ; r -= alpha Ap
-__global__ void k_update_xr(double* x, double* r, const double* p, const double* Ap, double alpha, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    x[i] += alpha * p[i];
-    r[i] -= alpha * Ap[i];
-  }
-}
-// y = a x + b y
-__global__ void k_axpby(double* y, const double* x, double a, double b, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = a * x[i] + b * y[i];
-}
 
 // neighbour CSR of the BSR pattern without the diagonal block (one per row)
 __global__ void k_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr) {
@@ -1410,21 +1398,6 @@ hipError_t launch_dot_partial(const double* a, const double* b, int n, double* p
   const int np = min(256, max(1, (n + kBlock - 1) / kBlock));
   hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(kBlock), 0, st, a, b, n, part);
   *np_out = np;
-  return hipGetLastError();
-}
-hipError_t launch_dot(const double* a, const double* b, int n, double* part, double* out, hipStream_t st) {
-  const int np = min(256, max(1, (n + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(kBlock), 0, st, a, b, n, part);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, st, part, np, out);
-  return hipGetLastError();
-}
-hipError_t launch_update_xr(double* x, double* r, const double* p, const double* Ap, double alpha, int n,
-                            hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_update_xr, dim3((n + 255) / 256), dim3(256), 0, st, x, r, p, Ap, alpha, n);
-  return hipGetLastError();
-}
-hipError_t launch_axpby(double* y, const double* x, double a, double b, int n, hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, st, y, x, a, b, n);
   return hipGetLastError();
 }
 hipError_t launch_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr, hipStream_t st) {
@@ -1712,19 +1685,6 @@ __global__ void k_hash_fill(double* x, int n, unsigned seed) {
 
 // power-iteration step on the device: out[it] = |z| (from nrm2 = |z|^2),
 // x = z / |z|; a zero or non-finite norm leaves x untouched
-__global__ void k_normalize(double* x, const double* z, int n, const double* nrm2, double* out, int it) {
-  const double nz = sqrt(*nrm2);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) out[it] = nz;
-  if (i >= n || !(nz > 0.0) || !isfinite(nz)) return;
-  x[i] = z[i] / nz;
-}
-
-hipError_t launch_normalize(double* x, const double* z, int n, const double* nrm2, double* out, int it,
-                            hipStream_t st) {
-  hipLaunchKernelGGL(k_normalize, dim3((n + 255) / 256), dim3(256), 0, st, x, z, n, nrm2, out, it);
-  return hipGetLastError();
-}
 
 hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_hash_fill, dim3((n + 255) / 256), dim3(256), 0, st, x, n, seed);

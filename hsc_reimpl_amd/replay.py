@@ -34,8 +34,10 @@ import numpy as np
 VARIANTS = {
     "plain_fp64": {},
     "mixed_neumann": {"precision": "mixed"},
-    "new": {"precision": "mixed", "preconditioner": "chebyshev", "neumann_order": 16,
-            "poly_interval_ratio": 1000.0},
+    # the new solver: mixed precision, 16 passes on the interval the committed network picks
+    "new": {"precision": "mixed", "preconditioner": "learned", "neumann_order": 16},
+    "chebyshev_1000": {"precision": "mixed", "preconditioner": "chebyshev", "neumann_order": 16,
+                       "poly_interval_ratio": 1000.0},
 }
 
 
@@ -95,6 +97,12 @@ def replay(sysm, variants=None, scene=None, device="cuda:0", x_exact=None, max_i
     for name, extra in variants.items():
         cfg = plugin.make_config(scene, **extra)
         h = plugin.Handle(cfg)
+        if extra.get("preconditioner") == "learned":
+            import ctypes as C
+
+            from . import learned
+            w = torch.as_tensor(learned.load_weights(), device=device)
+            h._check(h.lib.hsc_set_learned_weights(h.h, C.c_void_p(w.data_ptr()), w.numel() * 4))
         h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
         x = torch.zeros_like(bd)
         st = h.solve(bd, x, check=False)
