@@ -135,6 +135,31 @@ struct AcaParams {
   int* error;
   long long timeout_ticks;
   int nwg;
+  const int* skip;        // nonzero: the block-by-body ACA has produced the factors (exit at once)
 };
+
+// ACA by ABD body (see k_aca_blocks): scratch of one setup
+struct AcaBlkParams {
+  const double* E;        // qm x a row-major
+  int qm, a, limit;
+  double stop;            // relative-residual stop of the ACA (the body path requires 0)
+  int* colbody;           // [a] body of column c, -1 none, -2 several
+  double* colsq;          // [a] |E[:, c]|^2
+  int* plan;              // [4 + bodies]: ok, K, ..., per-body column counts
+  double* scale;          // [1] ||E||_F
+  double* ublk;           // [bodies x 144] pivot columns (12 rows) of each body's steps
+  double* vtmp;           // [a x 12] pivot-row entries by column and step
+  double* hblk;           // [bodies x 13] residual |R_b|^2 after 0..rank steps
+  int* pblk;              // [bodies x 24] pivots (global row, global column)
+  int* rank_b;            // [bodies]
+  // outputs in k_aca's format
+  double* U;              // qm x limit column-major
+  double* V;              // a x limit column-major
+  double* hist;           // limit + 1
+  int* pivots;            // limit x 2
+  int* rank_out;
+  double* scale_out;
+};
+constexpr int kAcaBlkMaxBodies = 128;
 
 }  // namespace hsc

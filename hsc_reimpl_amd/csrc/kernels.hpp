@@ -73,6 +73,8 @@ hipError_t launch_sort_gather(const int* rowptr, int nb, int* key_col, int* key_
                               int* colidx, double* v64, float* v32, hipStream_t st);
 hipError_t aca_occupancy(int* blocks);
 hipError_t launch_aca(const AcaParams& P, hipStream_t st);
+// ACA by ABD body when E is block diagonal by body (plan[0] = 1; else nothing)
+hipError_t launch_aca_blocks(const AcaBlkParams& P, hipStream_t st);
 // register-resident ACA: grid size for (qm, a) within max_wg workgroups, 0 when it does not fit
 int aca_reg_grid(int qm, int a, int max_wg);
 hipError_t aca_reg_occupancy(int* blocks);

@@ -43,8 +43,9 @@ hipError_t launch_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt,
 // input features of the learned polynomial (nb x 16 bf16, from D only) and
 // the node mean of the network's 3 outputs (deterministic)
 hipError_t launch_poly_features(const int* rowptr, const int* colidx, const double* val64, const float* Dfeat,
-                                const float* sdiag, const int* act_node, int nb, float lam_host, const double* lam2,
-                                bf16_t* X, hipStream_t st);
-hipError_t launch_mean_rows3(const float* O, int nb, double* out, hipStream_t st);
+                                const float* sdiag, const int* act_node, int nb, bf16_t* X, hipStream_t st);
+// partials: mean_rows3_parts(nb) x 3 doubles
+int mean_rows3_parts(int nb);
+hipError_t launch_mean_rows3(const float* O, int nb, double* part, double* out, hipStream_t st);
 
 }  // namespace hsc

@@ -214,15 +214,13 @@ __global__ void k_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt)
 // Per-node input of the learned polynomial (one thread per node), from D
 // only: x_i = [D_ii / s_i (9, row-major), log10(s_i / mean of the
 // neighbours' s_j) / 2, sum_{j != i} |D_ij|_F / |D_ii|_F / 4, degree / 16,
-// coupled (1 when node i carries W rows), log10(nb) / 6, lambda / 4, 0] in
-// bf16, s_i = tr(D_ii) / 3, lambda the spectrum estimate of Dinv D (the
-// square root of *lam2 when lam2 is given, else lam_host).
+// coupled (1 when node i carries W rows), log10(nb) / 6, 0, 0] in bf16,
+// s_i = tr(D_ii) / 3.  Nothing depends on the spectrum estimate, so the
+// network runs beside the power iteration.
 __global__ void k_poly_features(const int* rowptr, const int* colidx, const double* val64, const float* Dfeat,
-                                const float* sdiag, const int* act_node, int nb, float lam_host,
-                                const double* lam2, bf16_t* X) {
+                                const float* sdiag, const int* act_node, int nb, bf16_t* X) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
-  const float lam = lam2 ? (float)sqrt(*lam2) : lam_host;
   double ssum = 0.0, off = 0.0, dnorm = 0.0;
   int deg = 0;
   for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
@@ -249,16 +247,19 @@ __global__ void k_poly_features(const int* rowptr, const int* colidx, const doub
   x[11] = f2bf((float)deg / 16.f);
   x[12] = f2bf(act_node[i] >= 0 ? 1.f : 0.f);
   x[13] = f2bf((float)(log10((double)nb) / 6.0));
-  x[14] = f2bf(0.25f * lam);
+  x[14] = 0;
   x[15] = 0;
 }
 
-// Node mean of the 3 output columns, one workgroup in a fixed order:
-// out[c] = sum_i O[i][c] / nb (deterministic).
-__global__ __launch_bounds__(256) void k_mean_rows3(const float* O, int nb, double* out) {
+// Node mean of the 3 output columns in a fixed order: workgroup w sums the
+// nodes [w * 1024, (w + 1) * 1024) into part[w][3], then one workgroup sums
+// the partials in order: out[c] = sum_i O[i][c] / nb (deterministic).
+constexpr int kMeanChunk = 1024;
+__global__ __launch_bounds__(256) void k_mean_rows3_part(const float* O, int nb, double* part) {
   __shared__ double red[3][256];
+  const int i0 = blockIdx.x * kMeanChunk, i1 = min(nb, i0 + kMeanChunk);
   double a[3] = {0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < nb; i += 256)
+  for (int i = i0 + threadIdx.x; i < i1; i += 256)
     for (int c = 0; c < 3; ++c) a[c] += (double)O[(size_t)i * 3 + c];
   for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = a[c];
   __syncthreads();
@@ -267,7 +268,13 @@ __global__ __launch_bounds__(256) void k_mean_rows3(const float* O, int nb, doub
       for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 3) out[threadIdx.x] = nb > 0 ? red[threadIdx.x][0] / (double)nb : 0.0;
+  if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ void k_mean_rows3_final(const double* part, int np, int nb, double* out) {
+  if (threadIdx.x >= 3) return;
+  double s = 0.0;
+  for (int w = 0; w < np; ++w) s += part[3 * w + threadIdx.x];
+  out[threadIdx.x] = nb > 0 ? s / (double)nb : 0.0;
 }
 
 static inline int cdivl(long long a, long long b) { return (int)((a + b - 1) / b); }
@@ -322,15 +329,17 @@ hipError_t launch_mlp_cast(const float* W, int K, int N, bf16_t* Wb, bf16_t* Wt,
   return hipGetLastError();
 }
 hipError_t launch_poly_features(const int* rowptr, const int* colidx, const double* val64, const float* Dfeat,
-                                const float* sdiag, const int* act_node, int nb, float lam_host, const double* lam2,
-                                bf16_t* X, hipStream_t st) {
+                                const float* sdiag, const int* act_node, int nb, bf16_t* X, hipStream_t st) {
   if (nb > 0)
     hipLaunchKernelGGL(k_poly_features, dim3(cdivl(nb, 256)), dim3(256), 0, st, rowptr, colidx, val64, Dfeat, sdiag,
-                       act_node, nb, lam_host, lam2, X);
+                       act_node, nb, X);
   return hipGetLastError();
 }
-hipError_t launch_mean_rows3(const float* O, int nb, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_mean_rows3, dim3(1), dim3(256), 0, st, O, nb, out);
+int mean_rows3_parts(int nb) { return std::max(1, cdivl(nb, kMeanChunk)); }
+hipError_t launch_mean_rows3(const float* O, int nb, double* part, double* out, hipStream_t st) {
+  const int np = mean_rows3_parts(nb);
+  hipLaunchKernelGGL(k_mean_rows3_part, dim3(np), dim3(256), 0, st, O, nb, part);
+  hipLaunchKernelGGL(k_mean_rows3_final, dim3(1), dim3(64), 0, st, part, np, nb, out);
   return hipGetLastError();
 }
 

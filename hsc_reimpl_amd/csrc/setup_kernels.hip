@@ -247,6 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
   const long long total = (long long)P.qm * P.a;
   const long long gthreads = (long long)P.nwg * kBlock;
   const long long g = (long long)b * kBlock + tid;
+  if (P.skip && *P.skip) return;  // uniform: the factors came from the block-by-body ACA
   GridBarrier gbar;
   if (!gb_init(gbar, P.barrier, P.error, P.timeout_ticks, (unsigned)P.nwg)) return;
   // partials alternate between two halves of P.part: a workgroup rewrites a
@@ -380,6 +381,276 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
   if (b == 0 && tid == 0) *P.rank_out = kk;
 }
 
+// ---------------------------------------------------------------- ACA by ABD body
+// When every active FEM column of E couples to one ABD body only (each tire
+// rim to its own hub, as in the vehicle systems), E is block diagonal once
+// its columns are grouped by body, and a rank-1 step of full-pivot ACA
+// changes only the block that holds its pivot.  The global pivot sequence is
+// then an interleaving of the blocks' own full-pivot sequences (the largest
+// entry of a block is the same element whether it is sought in the block or
+// in E, ties broken by the same row-major index), with the same pivot rows
+// and columns and the same arithmetic per entry.  Each block is therefore
+// factored by its own workgroup, with its residual in registers and no grid
+// barrier, and the factors are concatenated body by body (one column order;
+// V C V^T is the same).  The stop guard is the global eps x ||E||_F.  The
+// path is taken when the ACA has no relative-residual stop, no column
+// touches two bodies, the blocks cannot reach the rank limit together
+// (sum over bodies of min(12, columns) <= limit) and every block fits in
+// registers (12 x columns <= kAcaBlkPer x kBlock); otherwise k_aca or
+// k_aca_reg factor the whole E (they exit at once when this path ran).
+constexpr int kAcaBlkPer = 32;                         // residual entries per thread
+constexpr int kAcaBlkCols = kAcaBlkPer * kBlock / 12;  // columns per body
+
+// body of column c (-1: no nonzero, -2: nonzeros in two bodies) and |E[:, c]|^2;
+// the 12 rows of a body are loaded together
+__global__ void k_col_bodies(AcaBlkParams P) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.a) return;
+  int body = -1;
+  double q = 0.0;
+  for (int bb = 0; bb < P.qm / 12; ++bb) {
+    const double* e = P.E + (size_t)(12 * bb) * P.a + c;
+    double v[12];
+#pragma unroll
+    for (int r = 0; r < 12; ++r) v[r] = e[(size_t)r * P.a];
+    bool nz = false;
+#pragma unroll
+    for (int r = 0; r < 12; ++r) {
+      q += v[r] * v[r];
+      nz = nz || v[r] != 0.0;
+    }
+    if (nz) body = (body == -1 || body == bb) ? bb : -2;
+  }
+  P.colbody[c] = body;
+  P.colsq[c] = q;
+}
+
+// one workgroup: ||E||_F (fixed order), columns per body, and the decision
+// plan[0] = 1 when the body path applies
+__global__ __launch_bounds__(kBlock) void k_aca_plan(AcaBlkParams P) {
+  __shared__ int s_cnt[kAcaBlkMaxBodies];
+  __shared__ int s_multi;
+  __shared__ double s_red[kBlock / 64];
+  const int tid = threadIdx.x, nbody = P.qm / 12;
+  for (int bb = tid; bb < kAcaBlkMaxBodies; bb += kBlock) s_cnt[bb] = 0;
+  if (tid == 0) s_multi = 0;
+  __syncthreads();
+  double q = 0.0;
+  for (int c = tid; c < P.a; c += kBlock) {
+    q += P.colsq[c];
+    const int bb = P.colbody[c];
+    if (bb == -2) s_multi = 1;
+    else if (bb >= 0 && bb < kAcaBlkMaxBodies) atomicAdd(&s_cnt[bb], 1);
+  }
+  const double tot = block_sum_t0(q, s_red);
+  __syncthreads();
+  if (tid == 0) {
+    P.plan[1] = 0;
+    int ok = P.stop == 0.0 && !s_multi && nbody <= kAcaBlkMaxBodies;
+    long long kmax = 0;
+    for (int bb = 0; bb < nbody && ok; ++bb) {
+      if (s_cnt[bb] > kAcaBlkCols) ok = 0;
+      kmax += s_cnt[bb] < 12 ? s_cnt[bb] : 12;
+    }
+    if (kmax > P.limit) ok = 0;
+    P.plan[0] = ok;
+    *P.scale = sqrt(tot);
+  }
+  for (int bb = tid; bb < nbody && bb < kAcaBlkMaxBodies; bb += kBlock) P.plan[4 + bb] = s_cnt[bb];
+}
+
+// one workgroup per body: full-pivot ACA of the body's 12 x n block
+__global__ __launch_bounds__(kBlock) void k_aca_blocks(AcaBlkParams P) {
+  __shared__ int s_cols[kAcaBlkCols];
+  __shared__ int s_wcnt[kBlock / 64];
+  __shared__ double s_m[kBlock / 64], s_q[kBlock / 64];
+  __shared__ long long s_i[kBlock / 64];
+  __shared__ double s_u[12], s_v[kAcaBlkCols];
+  __shared__ double s_piv;
+  __shared__ int s_pil, s_pjl, s_n;
+  __shared__ long long s_best;
+  __shared__ double s_bestv;
+  if (!P.plan[0]) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int body = blockIdx.x, r0 = 12 * body, a = P.a;
+  // the body's columns in ascending order
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  for (int base = 0; base < a; base += kBlock) {
+    const int c = base + tid;
+    const bool mine = c < a && P.colbody[c] == body;
+    const unsigned long long m = __ballot(mine);
+    if (lane == 0) s_wcnt[wave] = __popcll(m);
+    __syncthreads();
+    int off = s_n;
+    for (int w = 0; w < wave; ++w) off += s_wcnt[w];
+    if (mine) s_cols[off + __popcll(m & ((1ull << lane) - 1ull))] = c;
+    __syncthreads();
+    if (tid == 0) {
+      int t = s_n;
+      for (int w = 0; w < kBlock / 64; ++w) t += s_wcnt[w];
+      s_n = t;
+    }
+    __syncthreads();
+  }
+  const int n = s_n, cnt = 12 * n;
+  double R[kAcaBlkPer];
+#pragma unroll
+  for (int j = 0; j < kAcaBlkPer; ++j) {
+    const int le = tid + j * kBlock;
+    R[j] = le < cnt ? P.E[(size_t)(r0 + le / n) * a + s_cols[le % n]] : 0.0;
+  }
+  const double scale = *P.scale;
+  const double guard = DBL_EPSILON * fmax(scale, DBL_MIN);
+  // argmax |R| (lowest global row-major index on ties) and |R|^2 of the block
+  auto reduce = [&](double& best_v, long long& best_i, double& sq) {
+    double m = -1.0, q = 0.0;
+    long long mi = 0x7fffffffffffffffLL;
+#pragma unroll
+    for (int j = 0; j < kAcaBlkPer; ++j) {
+      const int le = tid + j * kBlock;
+      if (le < cnt) {
+        argmax_combine(m, mi, fabs(R[j]), (long long)(r0 + le / n) * a + s_cols[le % n]);
+        q += R[j] * R[j];
+      }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double m2 = __shfl_xor(m, off, 64);
+      const long long i2 = __shfl_xor(mi, off, 64);
+      argmax_combine(m, mi, m2, i2);
+      q += __shfl_xor(q, off, 64);
+    }
+    if (lane == 0) {
+      s_m[wave] = m;
+      s_i[wave] = mi;
+      s_q[wave] = q;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double M = s_m[0], Q = s_q[0];
+      long long I = s_i[0];
+      for (int w = 1; w < kBlock / 64; ++w) {
+        argmax_combine(M, I, s_m[w], s_i[w]);
+        Q += s_q[w];
+      }
+      s_bestv = M;
+      s_best = I;
+      s_q[0] = Q;
+    }
+    __syncthreads();
+    best_v = s_bestv;
+    best_i = s_best;
+    sq = s_q[0];
+    __syncthreads();
+  };
+  double bv, sq;
+  long long bi;
+  reduce(bv, bi, sq);
+  if (tid == 0) P.hblk[body * 13] = sq;
+  int k = 0;
+  while (n > 0 && k < 12 && bi != 0x7fffffffffffffffLL) {
+    // the pivot's owner publishes its value and block position
+#pragma unroll
+    for (int j = 0; j < kAcaBlkPer; ++j) {
+      const int le = tid + j * kBlock;
+      if (le < cnt && (long long)(r0 + le / n) * a + s_cols[le % n] == bi) {
+        s_piv = R[j];
+        s_pil = le / n;
+        s_pjl = le % n;
+      }
+    }
+    __syncthreads();
+    const double piv = s_piv;
+    if (!(fabs(piv) > guard)) break;  // uniform
+    const int pil = s_pil, pjl = s_pjl;
+#pragma unroll
+    for (int j = 0; j < kAcaBlkPer; ++j) {
+      const int le = tid + j * kBlock;
+      if (le < cnt) {
+        const int r = le / n, c = le % n;
+        if (c == pjl) s_u[r] = R[j];
+        if (r == pil) s_v[c] = R[j] / piv;
+      }
+    }
+    __syncthreads();
+    if (tid < 12) P.ublk[(size_t)body * 144 + k * 12 + tid] = s_u[tid];
+    for (int c = tid; c < n; c += kBlock) P.vtmp[(size_t)s_cols[c] * 12 + k] = s_v[c];
+    if (tid == 0) {
+      P.pblk[body * 24 + 2 * k] = r0 + pil;
+      P.pblk[body * 24 + 2 * k + 1] = s_cols[pjl];
+    }
+#pragma unroll
+    for (int j = 0; j < kAcaBlkPer; ++j) {
+      const int le = tid + j * kBlock;
+      if (le < cnt) R[j] = __dsub_rn(R[j], __dmul_rn(s_u[le / n], s_v[le % n]));
+    }
+    ++k;
+    reduce(bv, bi, sq);
+    if (tid == 0) P.hblk[body * 13 + k] = sq;
+  }
+  if (tid == 0) {
+    P.rank_b[body] = k;
+    // a 13th pivot above the guard (rounding left in a 12-row block): the
+    // finish kernel hands the factorisation back to the whole-E ACA
+    if (k == 12 && bi != 0x7fffffffffffffffLL && bv > guard) P.plan[1] = 1;
+  }
+}
+
+// the factors in k_aca's format: U, V concatenated body by body, pivots,
+// the residual history in that order, rank and scale
+__global__ __launch_bounds__(256) void k_aca_blocks_finish(AcaBlkParams P) {
+  __shared__ int s_off[kAcaBlkMaxBodies + 1];
+  if (!P.plan[0]) return;
+  const int tid = threadIdx.x, nbody = P.qm / 12;
+  if (P.plan[1]) {  // a block needed more than 12 steps: the whole-E ACA runs instead
+    if (blockIdx.x == 0 && tid == 0) P.plan[0] = 0;
+    return;
+  }
+  if (tid == 0) {
+    int o = 0;
+    for (int bb = 0; bb < nbody; ++bb) {
+      s_off[bb] = o;
+      o += P.rank_b[bb];
+    }
+    s_off[nbody] = o;
+  }
+  __syncthreads();
+  const int K = s_off[nbody], qm = P.qm, a = P.a;
+  const long long nu = (long long)qm * K, nv = (long long)a * K;
+  for (long long e = (long long)blockIdx.x * blockDim.x + tid; e < nu + nv; e += (long long)gridDim.x * blockDim.x) {
+    if (e < nu) {
+      const int g = (int)(e / qm), r = (int)(e % qm), bb = r / 12;
+      const int kk = g - s_off[bb];
+      P.U[e] = (kk >= 0 && kk < P.rank_b[bb]) ? P.ublk[(size_t)bb * 144 + kk * 12 + r % 12] : 0.0;
+    } else {
+      const long long f = e - nu;
+      const int g = (int)(f / a), c = (int)(f % a), bb = P.colbody[c];
+      const int kk = bb >= 0 ? g - s_off[bb] : -1;
+      P.V[f] = (bb >= 0 && kk >= 0 && kk < P.rank_b[bb]) ? P.vtmp[(size_t)c * 12 + kk] : 0.0;
+    }
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    const double scale = *P.scale;
+    double done = 0.0, rest = 0.0;
+    for (int bb = 0; bb < nbody; ++bb) rest += P.hblk[bb * 13];
+    P.hist[0] = scale > 0.0 ? 1.0 : 0.0;
+    int g = 0;
+    for (int bb = 0; bb < nbody; ++bb) {
+      const int rk = P.rank_b[bb];
+      rest -= P.hblk[bb * 13];
+      for (int kk = 0; kk < rk; ++kk, ++g) {
+        P.pivots[2 * g] = P.pblk[bb * 24 + 2 * kk];
+        P.pivots[2 * g + 1] = P.pblk[bb * 24 + 2 * kk + 1];
+        const double tot = done + P.hblk[bb * 13 + kk + 1] + fmax(rest, 0.0);
+        P.hist[g + 1] = scale > 0.0 ? sqrt(fmax(tot, 0.0)) / scale : 0.0;
+      }
+      done += P.hblk[bb * 13 + rk];
+    }
+    *P.rank_out = K;
+    *P.scale_out = scale;
+  }
+}
+
 // ---------------------------------------------------------------- ACA, register-resident residual
 // The same steps and arithmetic as k_aca with the residual held in
 // registers: workgroup b owns the columns [b*cpw, (b+1)*cpw) of R (all qm
@@ -419,6 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_aca_reg(AcaParams P) {
   const int cpw = (a + nwg - 1) / nwg;
   const int c0 = min(a, b * cpw), nc = min(a, c0 + cpw) - c0;
   const int cnt = qm * nc;
+  if (P.skip && *P.skip) return;  // uniform: the factors came from the block-by-body ACA
   GridBarrier gbar;
   if (!gb_init(gbar, P.barrier, P.error, P.timeout_ticks, (unsigned)nwg)) return;
   int phase = 0;
@@ -1119,6 +1391,19 @@ hipError_t launch_aca_reg(const AcaParams& P, hipStream_t st) {
   const int cpw = (P.a + P.nwg - 1) / P.nwg;
   if ((long long)P.qm * cpw > (long long)kAcaRegPer * kBlock || cpw > kAcaRegCols) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_aca_reg, dim3(P.nwg), dim3(kBlock), 0, st, P);
+  return hipGetLastError();
+}
+// the body path: column bodies, plan, one workgroup per body, finish (every
+// kernel after the plan does nothing when the plan declines)
+hipError_t launch_aca_blocks(const AcaBlkParams& P, hipStream_t st) {
+  if (P.a <= 0 || P.qm <= 0) return hipSuccess;
+  const int nbody = P.qm / 12;
+  hipLaunchKernelGGL(k_col_bodies, dim3(cdiv(P.a, 256)), dim3(256), 0, st, P);
+  hipLaunchKernelGGL(k_aca_plan, dim3(1), dim3(kBlock), 0, st, P);
+  if (nbody <= kAcaBlkMaxBodies) {
+    hipLaunchKernelGGL(k_aca_blocks, dim3(nbody), dim3(kBlock), 0, st, P);
+    hipLaunchKernelGGL(k_aca_blocks_finish, dim3(256), dim3(256), 0, st, P);
+  }
   return hipGetLastError();
 }
 hipError_t launch_aca(const AcaParams& P, hipStream_t st) {

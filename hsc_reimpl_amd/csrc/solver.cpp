@@ -232,6 +232,7 @@ struct hsc_handle {
   Buf row_count, rowptr, cursor, active_flag, act_prefix, act_node, act_dof;
   Buf key_col, key_src, colidx, val64, val32, invD64, invD32;
   Buf A, E, R0, R1, U, V, hist, pivots, acapart, acacand, Binv, P, s1, s2, s3, W1, C, Lm, W64, W32;
+  Buf acab, acabi;  // the ACA-by-body path's scratch (doubles, ints)
   Buf rhs, r, p0, p1, z0, z1, q, q64, yfull, small, ybuf, misc, ints;
   // persistent-launch scratch
   Buf barrier, errflag, respart, profbuf;  // barrier/errflag: setup kernels (ACA)
@@ -251,11 +252,17 @@ struct hsc_handle {
   // power iteration) beside ACA; pinned staging for the setup's small results
   hipStream_t aux = nullptr;
   hipEvent_t aux_ev = nullptr;
+  // learned: a third stream runs the network on D beside the power iteration
+  hipStream_t aux2 = nullptr;
+  hipEvent_t d_ev = nullptr;
   HostBuf stage;
   ~hsc_handle() {
     if (aux) (void)hipStreamSynchronize(aux);
+    if (aux2) (void)hipStreamSynchronize(aux2);
     if (aux_ev) (void)hipEventDestroy(aux_ev);
+    if (d_ev) (void)hipEventDestroy(d_ev);
     if (aux) (void)hipStreamDestroy(aux);
+    if (aux2) (void)hipStreamDestroy(aux2);
   }
 };
 
@@ -865,7 +872,6 @@ size_t power_buffer_doubles(int n, int nb) {
   const size_t cap = (size_t)std::max(hsc::power_step_tiles(8, nb), 256);
   return 2 * (size_t)n + 2 * cap + 1;
 }
-const double* power_lam2_dev(const hsc_handle* h);
 
 // Enqueue the power iteration for lambda_max(Dinv D) on st (D's rows and
 // inverse diagonal blocks ready there): 12 fused steps from a fixed
@@ -897,7 +903,7 @@ void enqueue_power_iteration(hsc_handle* h, hipStream_t st, double* lam2_host) {
 }
 
 int interval_from_prediction(hsc_handle* h, const double pred[3], double* ratio);
-void enqueue_prediction(hsc_handle* h, hipStream_t st, const double* lam2_dev, double* pred_host);
+void enqueue_prediction(hsc_handle* h, hipStream_t st, double* pred_host);
 
 // Per-pass weights: omega for the Neumann series (and for the learned
 // preconditioner when its network's output is rejected); for the Chebyshev
@@ -1082,6 +1088,11 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     HIPCHK(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&h->aux_ev, hipEventDisableTiming));
   }
+  const bool learned = h->cfg.preconditioner == HSC_PRECOND_LEARNED;
+  if (learned && !h->aux2) {
+    HIPCHK(hipStreamCreateWithFlags(&h->aux2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->d_ev, hipEventDisableTiming));
+  }
   HIPCHK(h->barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
   HIPCHK(h->errflag.ensure(64));
   {
@@ -1147,6 +1158,34 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     ap.error = h->errflag.as<int>();
     ap.timeout_ticks = kTimeoutTicks;
     ap.nwg = nwg;
+    // ACA by ABD body first when E is block diagonal by body (each block by
+    // one workgroup, no grid barrier); the whole-E kernel then exits at once
+    const int nbody = qm / 12, nbb = std::min(nbody, hsc::kAcaBlkMaxBodies);
+    HIPCHK(h->acab.ensure(sizeof(double) * ((size_t)a + 1 + 157 * (size_t)nbb)));
+    HIPCHK(h->acabi.ensure(sizeof(int) * ((size_t)a + 4 + nbody + 25 * (size_t)nbb)));
+    hsc::AcaBlkParams bp{};
+    bp.E = h->E.as<double>();
+    bp.qm = qm;
+    bp.a = a;
+    bp.limit = limit;
+    bp.stop = h->cfg.aca_rtol;
+    bp.colsq = h->acab.as<double>();
+    bp.scale = bp.colsq + a;
+    bp.ublk = bp.scale + 1;
+    bp.hblk = bp.ublk + 144 * (size_t)nbb;
+    bp.vtmp = h->R0.as<double>();  // a x 12 <= qm x a
+    bp.colbody = h->acabi.as<int>();
+    bp.plan = bp.colbody + a;
+    bp.pblk = bp.plan + 4 + nbody;
+    bp.rank_b = bp.pblk + 24 * (size_t)nbb;
+    bp.U = ap.U;
+    bp.V = ap.V;
+    bp.hist = ap.hist;
+    bp.pivots = ap.pivots;
+    bp.rank_out = ap.rank_out;
+    bp.scale_out = ap.scale_out;
+    HIPCHK(hsc::launch_aca_blocks(bp, st));
+    ap.skip = bp.plan;
     HIPCHK(nwg_reg ? hsc::launch_aca_reg(ap, st) : hsc::launch_aca(ap, st));
   }
   // ---- ABD inverse and P
@@ -1161,11 +1200,17 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   join.s = h->aux;
   HIPCHK(hsc::launch_sort_gather(h->rowptr.as<int>(), nb, h->key_col.as<int>(), h->key_src.as<int>(), Av->values,
                                  nnzb, h->colidx.as<int>(), h->val64.as<double>(), h->val32.as<float>(), h->aux));
+  AuxJoin join2{nullptr};
+  if (learned) {
+    // the network needs D only: it runs on a third stream beside the power
+    // iteration, both hidden behind ACA
+    HIPCHK(hipEventRecord(h->d_ev, h->aux));
+    HIPCHK(hipStreamWaitEvent(h->aux2, h->d_ev, 0));
+    join2.s = h->aux2;
+    enqueue_prediction(h, h->aux2, ss->pred);
+  }
   ss->lam2 = 0.0;
   if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
-  // learned: the network runs on the same stream behind the power iteration
-  // (it reads the spectrum bound on the device), hidden behind ACA
-  if (h->cfg.preconditioner == HSC_PRECOND_LEARNED) enqueue_prediction(h, h->aux, power_lam2_dev(h), ss->pred);
   ss->aca_err = 0;
   HIPCHK(hipMemcpyAsync(ss->ints2, err + 8, sizeof(ss->ints2), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&ss->max_coupled, err + 12, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1202,6 +1247,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     if (ss->fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
   }
   HIPCHK(hipStreamSynchronize(h->aux));
+  if (learned) HIPCHK(hipStreamSynchronize(h->aux2));
   set_poly_weights(h, ss->lam2, ss->pred);
   prepare_launch(h, st);
   h->setup_ok = true;
@@ -1252,27 +1298,27 @@ void ensure_mlp(hsc_handle* h, hipStream_t st) {
   }
   HIPCHK(h->xfeat.ensure(sizeof(hsc::bf16_t) * 16 * h->nb));
   HIPCHK(h->omlp.ensure(sizeof(float) * 3 * h->nb));
-  HIPCHK(h->lpred.ensure(sizeof(double) * 4));
+  HIPCHK(h->lpred.ensure(sizeof(double) * (4 + 3 * (size_t)hsc::mean_rows3_parts(h->nb))));
 }
 
 // The network's input features of h's system (graph and D's diagonal
-// features built on st first); the spectrum bound is read from lam2_dev
-// (|Dinv D x|^2 of the power iteration) when given, else h->lambda_max.
-void enqueue_learned_features(hsc_handle* h, hipStream_t st, const double* lam2_dev) {
+// features built on st first).
+void enqueue_learned_features(hsc_handle* h, hipStream_t st) {
   HIPCHK(hsc::launch_poly_features(h->rowptr.as<int>(), h->colidx.as<int>(), h->val64.as<double>(),
                                    h->dfeat.as<float>(), h->sdiag.as<float>(), h->act_node.as<int>(), h->nb,
-                                   (float)h->lambda_max, lam2_dev, h->xfeat.as<hsc::bf16_t>(), st));
+                                   h->xfeat.as<hsc::bf16_t>(), st));
 }
 
-// Enqueue the network on h's system on st (behind D's rows and the power
-// iteration there): features, forward, node mean of the 3 outputs, copied
-// to pred_host (pinned) once st drains.
-void enqueue_prediction(hsc_handle* h, hipStream_t st, const double* lam2_dev, double* pred_host) {
+// Enqueue the network on h's system on st (behind D's rows there):
+// features, forward, node mean of the 3 outputs, copied to pred_host
+// (pinned) once st drains.
+void enqueue_prediction(hsc_handle* h, hipStream_t st, double* pred_host) {
   ensure_mlp(h, st);
-  enqueue_learned_features(h, st, lam2_dev);
+  enqueue_learned_features(h, st);
   HIPCHK(h->mlp->forward(h->xfeat.as<hsc::bf16_t>(), 1, h->omlp.as<float>(), st));
-  HIPCHK(hsc::launch_mean_rows3(h->omlp.as<float>(), h->nb, h->lpred.as<double>(), st));
-  HIPCHK(hipMemcpyAsync(pred_host, h->lpred.p, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+  double* lp = h->lpred.as<double>();
+  HIPCHK(hsc::launch_mean_rows3(h->omlp.as<float>(), h->nb, lp + 4, lp, st));
+  HIPCHK(hipMemcpyAsync(pred_host, lp, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
 }
 
 // The interval ratio for h's degree from the network's 3 node means; returns
@@ -1288,10 +1334,6 @@ int interval_from_prediction(hsc_handle* h, const double pred[3], double* ratio)
   return 0;
 }
 
-const double* power_lam2_dev(const hsc_handle* h) {
-  const size_t cap = (size_t)std::max(hsc::power_step_tiles(8, h->nb), 256);
-  return h->pwr.as<double>() + 2 * (size_t)h->fn + 2 * cap;
-}
 
 // Algorithmic HBM bytes of one Schur-PCG iteration of h's system (every
 // matrix and vector element read or written once per pass).  FP64: an SpMV
@@ -2155,7 +2197,7 @@ int32_t hsc_get_learned_features(hsc_handle* h, uint16_t* X) {
   return guarded(h, [&] {
     ensure_graph(h, nullptr);
     HIPCHK(h->xfeat.ensure(sizeof(hsc::bf16_t) * 16 * h->nb));
-    enqueue_learned_features(h, nullptr, nullptr);
+    enqueue_learned_features(h, nullptr);
     HIPCHK(hipMemcpy(X, h->xfeat.p, sizeof(hsc::bf16_t) * 16 * h->nb, hipMemcpyDefault));
     return HSC_OK;
   });

@@ -53,13 +53,13 @@ def param_count() -> int:
 
 
 def init_params(seed=0) -> np.ndarray:
-    """Flat float32 parameters in the library's layout (He init over the 15
+    """Flat float32 parameters in the library's layout (He init over the 14
     used inputs; output layer scaled by 0.1; padded rows/columns zero)."""
     rng = np.random.default_rng(seed)
     HID, IN, OUTP = 64, 16, 32
     parts = []
-    W0 = rng.normal(size=(IN, HID)) * np.sqrt(2.0 / 15)
-    W0[15:] = 0.0
+    W0 = rng.normal(size=(IN, HID)) * np.sqrt(2.0 / 14)
+    W0[14:] = 0.0
     parts += [W0, np.zeros(HID)]
     for _ in range(2):
         parts += [rng.normal(size=(HID, HID)) * np.sqrt(2.0 / HID), rng.normal(size=(HID, HID)) * np.sqrt(2.0 / HID),

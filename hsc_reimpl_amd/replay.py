@@ -41,6 +41,22 @@ VARIANTS = {
 }
 
 
+def make_handle(scene: dict, extra: dict, device="cuda:0", **more):
+    """A handle of one variant (make_config keys + extras); the learned
+    preconditioner gets the committed network's weights."""
+    from . import plugin
+    h = plugin.Handle(plugin.make_config(scene, **extra, **more))
+    if extra.get("preconditioner") == "learned":
+        import ctypes as C
+
+        import torch
+
+        from . import learned
+        w = torch.as_tensor(learned.load_weights(), device=device)
+        h._check(h.lib.hsc_set_learned_weights(h.h, C.c_void_p(w.data_ptr()), w.numel() * 4))
+    return h
+
+
 def model_line_search(bdx: float, dHd: float, max_iter: int = 8) -> dict:
     """The line search of advance_ipc.cu:410-440 on the quadratic model
     E(alpha) - E0 = -alpha bdx + alpha^2 dHd / 2: alpha = 1, halved after each
@@ -95,14 +111,8 @@ def replay(sysm, variants=None, scene=None, device="cuda:0", x_exact=None, max_i
     bd = torch.as_tensor(b, device=device)
     rows = []
     for name, extra in variants.items():
-        cfg = plugin.make_config(scene, **extra)
-        h = plugin.Handle(cfg)
-        if extra.get("preconditioner") == "learned":
-            import ctypes as C
-
-            from . import learned
-            w = torch.as_tensor(learned.load_weights(), device=device)
-            h._check(h.lib.hsc_set_learned_weights(h.h, C.c_void_p(w.data_ptr()), w.numel() * 4))
+        h = make_handle(scene, extra, device)
+        cfg = h.config
         h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
         x = torch.zeros_like(bd)
         st = h.solve(bd, x, check=False)

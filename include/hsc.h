@@ -367,9 +367,8 @@ int32_t hsc_get_node_graph(hsc_handle* h, int32_t* nbr_ptr, int32_t* nbr, int64_
  * per node (device or host destination): D_ii / s_i (9, row-major, s_i =
  * tr(D_ii) / 3), log10(s_i / mean of the neighbours' s_j) / 2,
  * sum_{j != i} |D_ij|_F / |D_ii|_F / 4, neighbour count / 16, 1 when the
- * node is coupled to the ABD block (else 0), log10(fem_nodes) / 6,
- * lambda_max / 4 (the setup's spectrum estimate of Dinv D; 0 for the
- * Neumann preconditioner), 0. */
+ * node is coupled to the ABD block (else 0), log10(fem_nodes) / 6, 0, 0.
+ * Only D enters: the network runs at setup beside the power iteration. */
 int32_t hsc_get_learned_features(hsc_handle* h, uint16_t* X);
 
 /* The network object used for training on the GPU (hand-written BF16

@@ -50,9 +50,9 @@ def init_params(seed=0):
         if name.startswith("b"):
             p[name] = np.zeros(shp)
         else:
-            fan_in = shp[0] if name != "W0" else 15
+            fan_in = shp[0] if name != "W0" else 14
             p[name] = rng.normal(size=shp) * np.sqrt(2.0 / fan_in) * (0.1 if name == "Wo" else 1.0)
-    p["W0"][15:] = 0.0
+    p["W0"][14:] = 0.0
     return p
 
 
@@ -66,12 +66,12 @@ def graph_from_bsr(rowptr, colidx):
     return np.asarray(nbr_ptr, dtype=np.int64), np.asarray(nbr, dtype=np.int64)
 
 
-def poly_features(D, active_nodes, lam):
+def poly_features(D, active_nodes):
     """Per-node input of the learned polynomial from the full symmetric FEM
-    block D (SciPy sparse, 3x3 node blocks), the coupled nodes and the
-    spectrum estimate lam of Dinv D: x_i = [D_ii / s_i (9, row-major),
-    log10(s_i / mean_{j in N(i)} s_j) / 2, sum_{j in N(i)} |D_ij|_F / |D_ii|_F / 4,
-    |N(i)| / 16, coupled, log10(nb) / 6, lam / 4, 0], s_i = tr(D_ii) / 3,
+    block D (SciPy sparse, 3x3 node blocks) and the coupled nodes:
+    x_i = [D_ii / s_i (9, row-major), log10(s_i / mean_{j in N(i)} s_j) / 2,
+    sum_{j in N(i)} |D_ij|_F / |D_ii|_F / 4, |N(i)| / 16, coupled,
+    log10(nb) / 6, 0, 0], s_i = tr(D_ii) / 3,
     N(i) = the other block columns stored in block row i.  Returns x (nb, 16)
     and the neighbour CSR (nbr_ptr, nbr)."""
     Db = D.tobsr(blocksize=(3, 3))
@@ -103,7 +103,6 @@ def poly_features(D, active_nodes, lam):
         x[i, 11] = deg / 16.0
         x[i, 12] = 1.0 if act[i] else 0.0
         x[i, 13] = np.log10(nb) / 6.0
-        x[i, 14] = 0.25 * lam
     return x, np.asarray(nbr_ptr, dtype=np.int64), np.asarray(nbr, dtype=np.int64)
 
 

@@ -50,9 +50,9 @@ def _fem_graph(s):
     return nbrs
 
 
-def _oracle_features(s, lam):
+def _oracle_features(s):
     D, _, _, act = osp.split_bcoo(s.rows, s.cols, s.vals, s.qo, s.qm, s.fo, s.fn)
-    return OL.poly_features(D, np.unique(np.asarray(act) // 3), lam)
+    return OL.poly_features(D, np.unique(np.asarray(act) // 3))
 
 
 def test_node_graph_and_features(c1_handle):
@@ -66,12 +66,11 @@ def test_node_graph_and_features(c1_handle):
     for i, want in enumerate(nbrs):
         assert sorted(nbr[ptr[i]:ptr[i + 1]].tolist()) == sorted(want), i
     X = learned.node_features(h, DEV).float().cpu().numpy()
-    lam = h.setup_info()["lambda_max"]
-    xo, optr, onbr = _oracle_features(s, lam)
+    xo, optr, onbr = _oracle_features(s)
     np.testing.assert_array_equal(optr, ptr)
     np.testing.assert_allclose(X, xo, rtol=2.0 ** -8, atol=1e-7)
     assert X[:, 12].sum() == 64          # the 8 x 8 rim patch is coupled
-    assert np.all(X[:, 15] == 0) and np.all(X[:, 11] > 0)
+    assert np.all(X[:, 14:] == 0) and np.all(X[:, 11] > 0)
 
 
 def _feature_batch(h, B, seed):
@@ -79,7 +78,7 @@ def _feature_batch(h, B, seed):
     X = learned.node_features(h, DEV).float()
     Xb = X[None].repeat(B, 1, 1) + torch.as_tensor(0.1 * rng.normal(size=(B,) + tuple(X.shape)), device=DEV,
                                                    dtype=torch.float32)
-    Xb[..., 15] = 0.0
+    Xb[..., 14:] = 0.0
     return Xb.to(torch.bfloat16).contiguous(), rng
 
 
@@ -130,8 +129,8 @@ def test_mlp_gradients_match_oracle(c1_handle):
     G = learned.to_oracle_params(g)
     for k in gref:
         a, b = G[k], gref[k]
-        if k == "W0":  # row 15 sees a zero feature
-            a, b = a[:15], b[:15]
+        if k == "W0":  # rows 14, 15 see zero features
+            a, b = a[:14], b[:14]
         err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         assert err <= 5e-2, (k, err)
     # padded output columns carry no gradient
@@ -182,7 +181,7 @@ def test_learned_interval_and_preconditioner_match_oracle(degree):
     h = learned.poly_handle(s, degree, preconditioner="learned", weights=theta, device=DEV)
     info = h.setup_info()
     assert info["learned_used"] == 1 and info["fallback_reason"] == 0
-    xo, optr, onbr = _oracle_features(s, info["lambda_max"])
+    xo, optr, onbr = _oracle_features(s)
     lg, u = OL.predict_log_ratio(learned.to_oracle_params(theta), OL.bf16(xo), optr, onbr, degree)
     np.testing.assert_allclose(info["learned_out"], u, atol=2e-2)
     t = min(max((degree - 8) / 4, 0), 2)

@@ -343,3 +343,28 @@ def test_jsonl_names_what_ran(c1, tmp_path, pre, abd):
     assert row["preconditioner"] == pre and row["precision"] == "mixed" and row["status"] == 0
     assert row["adaptive_abd_inverse"] is (abd == "neumann")
     assert row["neumann_doublings"] == st["neumann_doublings"] and (row["neumann_doublings"] > 0) == (abd == "neumann")
+
+
+def test_aca_by_body_equals_whole_e_aca():
+    """The 78k-tet system's E couples each torus rim to its own hub, so the
+    setup factors it body by body (one workgroup per body).  Against the
+    whole-E full-pivot ACA (forced with a relative stop of 1e-300, which it
+    never reaches before the round-off guard): the same rank, U V^T equal to
+    1e-13 of |E|, the same residual, and solutions within 1e-10 of each
+    other."""
+    s = systems.torus_system(seed=1)
+    scene = {"linear_system/tol_rate": 1e-8, "linear_system/hsc/max_iter": 20000}
+    hb, _ = _setup(s, scene)
+    hg, _ = _setup(s, scene, aca_rtol=1e-300)
+    Fb, Fg = hb.factors(), hg.factors()
+    assert Fb["rank"] == Fg["rank"] == 56
+    ub, ug = Fb["U"] @ Fb["V"].T, Fg["U"] @ Fg["V"].T
+    assert np.abs(ub - ug).max() <= 1e-13 * np.abs(ug).max()
+    assert Fb["aca_relative_error"] == pytest.approx(Fg["aca_relative_error"], rel=1e-6, abs=1e-12)
+    # both histories end at the same residual; the body path's is in body order
+    assert Fb["aca_history"][-1] == pytest.approx(Fg["aca_history"][-1], rel=1e-6, abs=1e-12)
+    assert np.all(np.diff(Fb["aca_history"]) <= 0)
+    xb, stb = _solve(hb, s)
+    xg, stg = _solve(hg, s)
+    assert stb["pcg_iterations"] == pytest.approx(stg["pcg_iterations"], abs=1)
+    assert np.linalg.norm(xb - xg) / np.linalg.norm(xg) <= 1e-10
