@@ -31,8 +31,8 @@ DEV = "cuda:0"
 
 @pytest.fixture(scope="module")
 def c1_handle():
-    """The 2,178-DOF system set up with the Chebyshev passes (so the setup
-    has a spectrum estimate, one of the network's inputs)."""
+    """The 2,178-DOF system set up with the Chebyshev passes at a fixed
+    interval (the network's inputs depend on D only)."""
     s = systems.cube_system(seed=0)
     h = learned.poly_handle(s, 16, 300.0, device=DEV)
     return s, h
