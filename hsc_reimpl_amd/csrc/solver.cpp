@@ -1523,7 +1523,7 @@ void hsc_config_default(hsc_config* c) {
   c->full_residual_limit = 0.1;
   c->omega = 0.7;
   c->tol_rate = 1e-3;
-  c->abd_inverse = HSC_ABD_EXACT;
+  c->abd_inverse = HSC_ABD_NEUMANN;  // the plugin's polynomial (+ doubling: adaptive_abd_inverse)
   c->precision = HSC_PRECISION_FP64;
   c->residual_replace_every = 25;
   c->true_residual_every_iteration = 0;

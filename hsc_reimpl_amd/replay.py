@@ -32,12 +32,14 @@ import sys
 import numpy as np
 
 VARIANTS = {
+    # the plugin's algorithm with its defaults (block Neumann ABD inverse, doubled)
     "plain_fp64": {},
     "mixed_neumann": {"precision": "mixed"},
-    # the new solver: mixed precision, 16 passes on the interval the committed network picks
-    "new": {"precision": "mixed", "preconditioner": "learned", "neumann_order": 16},
+    # the new solver: mixed precision, 16 passes on the interval the committed
+    # network picks, exact ABD inverse
+    "new": {"precision": "mixed", "preconditioner": "learned", "neumann_order": 16, "abd_inverse": "exact"},
     "chebyshev_1000": {"precision": "mixed", "preconditioner": "chebyshev", "neumann_order": 16,
-                       "poly_interval_ratio": 1000.0},
+                       "poly_interval_ratio": 1000.0, "abd_inverse": "exact"},
 }
 
 

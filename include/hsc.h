@@ -60,15 +60,19 @@ enum hsc_precision {
 /* How A^-1 (the ABD block inverse) is formed. */
 enum hsc_abd_inverse {
   HSC_ABD_EXACT = 0,              /* FP64 Cholesky of A (core.py:149-150 exact_affine=True) */
-  HSC_ABD_NEUMANN = 1,            /* fixed-order block Neumann polynomial (core.py:74-92;
-                                     linear_hsc.cu:264-275 with adaptive_abd_inverse = 0) */
+  HSC_ABD_NEUMANN = 1,            /* the plugin's inverse (default): block Neumann polynomial of
+                                     neumann_order terms and weight omega (core.py:74-92;
+                                     linear_hsc.cu:264-275), doubled to abd_inverse_tol when
+                                     adaptive_abd_inverse = 1 (linear_hsc.cu:276-297) */
   HSC_ABD_NEUMANN_ADAPTIVE = 2    /* polynomial + doubling to abd_inverse_tol (linear_hsc.cu:276-297) */
 };
 
 /* Preconditioner. */
 enum hsc_preconditioner {
   HSC_PRECOND_NEUMANN = 0,        /* block-Jacobi Neumann series of D (linear_hsc.cu:330-343) */
-  HSC_PRECOND_LEARNED = 1,        /* Neumann + graph-MLP correction, flexible PCG (no counterpart) */
+  HSC_PRECOND_LEARNED = 1,        /* the CHEBYSHEV passes on the interval ratio a graph MLP picks
+                                     from D's node features at setup; needs hsc_set_learned_weights
+                                     (no counterpart) */
   HSC_PRECOND_CHEBYSHEV = 2       /* same passes, per-pass weights from the Chebyshev roots on
                                      [b/poly_interval_ratio, b], b = 1.2 x a power-iteration estimate
                                      of lambda_max(Dinv D) (no counterpart) */
@@ -92,7 +96,7 @@ typedef struct hsc_config {
   double omega;                  /* linear_system/hsc/omega               (0.7)  */
   double tol_rate;               /* linear_system/tol_rate                (1e-3) */
   /* --- additions --- */
-  int32_t abd_inverse;           /* enum hsc_abd_inverse (HSC_ABD_EXACT)          */
+  int32_t abd_inverse;           /* enum hsc_abd_inverse (HSC_ABD_NEUMANN)        */
   int32_t precision;             /* enum hsc_precision (HSC_PRECISION_FP64)       */
   int32_t residual_replace_every;/* mixed: FP64 true-residual period (25); 0 = off */
   int32_t true_residual_every_iteration; /* 1 = stop test on the true residual each

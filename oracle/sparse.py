@@ -145,15 +145,27 @@ def chebyshev_weights(lam_est, m, ratio, safety=1.2):
 
 
 class SparseHSC:
-    """HSC with exact FP64 A^-1 (Cholesky) and sparse D (core.py:141-168 with
-    exact_affine=True).  E is the compacted qm x active coupling."""
+    """HSC with sparse D (core.py:141-168).  E is the compacted qm x active
+    coupling.  The ABD inverse is either exact FP64 (Cholesky; core.py's
+    exact_affine=True) or, with abd_inverse="neumann", the drop-in plugin's
+    block Neumann polynomial of the same order and omega as the FEM passes,
+    doubled while ||I - A P||_F / sqrt(qm) > abd_tol when `adaptive`
+    (linear_hsc.cu:263-297, hsc_core.neumann_adaptive)."""
 
-    def __init__(self, D, A, E, active, order=5, omega=0.7, aca_rtol=None, aca_max_rank=None):
+    def __init__(self, D, A, E, active, order=5, omega=0.7, aca_rtol=None, aca_max_rank=None,
+                 abd_inverse="exact", adaptive=True, abd_tol=1e-5):
         self.D, self.A, self.E, self.active = D, A, E, np.asarray(active)
         self.n = D.shape[0]
         self.factors = hsc_core.aca(E, rtol=aca_rtol, max_rank=aca_max_rank)
         self.U, self.Vc = self.factors.U, self.factors.V
-        self.cho = sla.cho_factor(A)
+        if abd_inverse == "exact":
+            self.cho, self.P, self.doublings, self.abd_residual = sla.cho_factor(A), None, 0, 0.0
+        elif abd_inverse == "neumann":
+            self.cho = None
+            self.P, self.doublings, self.abd_residual = hsc_core.neumann_adaptive(
+                np.asarray(A), order=order, omega=omega, tol=abd_tol, adaptive=adaptive)
+        else:
+            raise ValueError(abd_inverse)
         self.precond_op = SparseNeumann(D, order, omega)
         C = self.U.T @ self.Ainv(self.U)
         self.C = 0.5 * (C + C.T)
@@ -168,7 +180,7 @@ class SparseHSC:
         return self.precond_op(r)
 
     def Ainv(self, v):
-        return sla.cho_solve(self.cho, v)
+        return sla.cho_solve(self.cho, v) if self.P is None else self.P @ v
 
     def Vt(self, p):
         return self.Vc.T @ p[self.active]

@@ -41,6 +41,9 @@ def test_abi_version_and_defaults(lib):
     assert (c.max_rank, c.neumann_order, c.max_iter, c.adaptive_abd_inverse) == (128, 5, 2000, 1)
     assert (c.abd_inverse_tol, c.full_residual_limit, c.omega, c.tol_rate) == (1e-5, 0.1, 0.7, 1e-3)
     assert c.residual_replace_every == 25 and c.check_acceptance == 1
+    # the plugin's ABD inverse: block Neumann polynomial, doubled (linear_hsc.cu:263-297)
+    assert c.abd_inverse == _abi.ABD_INVERSE["neumann"] and c.precision == _abi.PRECISION["fp64"]
+    assert c.preconditioner == _abi.PRECONDITIONER["neumann"]
     # extensions: autotuned schedule, Chebyshev interval ratio, neighbour-tile pass sync
     assert (c.schedule, c.poly_interval_ratio, c.pass_sync) == (0, 100.0, 1)
 

@@ -19,6 +19,7 @@ DEV = "cuda:0"
 
 
 def _handles(syss, scene, **ext):
+    ext.setdefault("abd_inverse", "exact")  # solutions are compared with direct solves
     hs, As = [], []
     for s in syss:
         h = plugin.Handle(plugin.make_config(scene, **ext))

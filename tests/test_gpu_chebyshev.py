@@ -30,6 +30,7 @@ C2_SCENE = {"linear_system/tol_rate": 1e-8, "linear_system/hsc/max_iter": 20000}
 
 
 def _setup(sysm, scene, **ext):
+    ext.setdefault("abd_inverse", "exact")  # solutions are compared with direct solves
     h = plugin.Handle(plugin.make_config(scene, **ext))
     A = plugin.BCOO3.from_system(sysm, DEV)
     h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)

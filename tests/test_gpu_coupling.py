@@ -68,7 +68,7 @@ def test_contact_scene_assembles_to_the_pullback_and_solves():
     x_ref = np.linalg.solve(H, sc["b"])
     scene = {"linear_system/tol_rate": 1e-8, "linear_system/hsc/neumann_order": 2,
              "linear_system/hsc/omega": 0.6, "linear_system/hsc/max_iter": 5000}
-    h = plugin.Handle(plugin.make_config(scene, aca_rtol=1e-6))
+    h = plugin.Handle(plugin.make_config(scene, aca_rtol=1e-6, abd_inverse="exact"))
     h.setup(A, sc["qo"], sc["qm"], sc["fo"], sc["fn"])
     info = h.setup_info()
     assert info["rank"] <= 24          # two free bodies: rank <= 24

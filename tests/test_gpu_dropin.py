@@ -71,6 +71,52 @@ def test_adaptive_abd_inverse_matches_oracle(stiffness, order):
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-8
 
 
+@pytest.mark.parametrize("case", ["c1", "c2"])
+def test_default_config_is_the_plugin_algorithm(case):
+    """hsc_config_default as the plugin's do_build leaves it (block Neumann
+    ABD inverse doubled to 1e-5, Neumann series of D, FP64, drop-in stopping
+    rule), through LinearHSC.do_solve, on the 2,178-DOF system (order 2,
+    omega 0.6) and the 78k-tet system (order 5, omega 0.7): the same
+    doublings as the oracle, PCG iterations within 1 of the oracle's HSC on
+    that P, solution within 1e-7 of it; against a sparse direct solve the
+    solution carries the ABD inverse's own error (measured ~6e-6, bounded
+    here by 5e-5) and is accepted."""
+    import json
+    import tempfile
+    from oracle import sparse as osp
+    if case == "c1":
+        s, order, omega = systems.cube_system(seed=0), 2, 0.6
+        scene = {"linear_system/tol_rate": 1e-8, "linear_system/hsc/neumann_order": 2,
+                 "linear_system/hsc/omega": 0.6, "linear_system/hsc/max_iter": 5000}
+    else:
+        s, order, omega = systems.torus_system(seed=1), 5, 0.7
+        scene = {"linear_system/tol_rate": 1e-8, "linear_system/hsc/max_iter": 20000}
+    with tempfile.TemporaryDirectory() as ws:
+        ls = plugin.LinearHSC(workspace=ws)
+        ls.do_build(scene)
+        A = plugin.BCOO3.from_system(s, DEV)
+        b = torch.as_tensor(s.b, device=DEV)
+        info = plugin.SolvingInfo(x=torch.zeros_like(b), b=b, A=A, qo=s.qo, qm=s.qm, fo=s.fo, fn=s.fn)
+        st = ls.do_solve(info)
+        row = json.loads(open(f"{ws}/hsc_stats.jsonl").read().splitlines()[-1])
+    x = info.x.cpu().numpy()
+    assert row["accepted"] is True and row["adaptive_abd_inverse"] is True
+    D, Ad, E, act = osp.split_bcoo(s.rows, s.cols, s.vals, s.qo, s.qm, s.fo, s.fn)
+    ref = osp.SparseHSC(D, Ad, E, act, order=order, omega=omega, abd_inverse="neumann")
+    assert st["neumann_doublings"] == ref.doublings >= 1
+    assert st["abd_inverse_residual"] == pytest.approx(ref.abd_residual, rel=1e-4)
+    n = s.fn
+    dx, dq, ost = ref.solve(s.b[:n], s.b[n:], rtol=1e-8, maxiter=20000)
+    xo = np.concatenate([dx, dq])
+    assert abs(st["pcg_iterations"] - ost["iterations"]) <= 1, (st["pcg_iterations"], ost["iterations"])
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-7
+    xd = osp.direct_solve(s)
+    err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+    print(f"{case}: doublings {st['neumann_doublings']} ABD residual {st['abd_inverse_residual']:.2e} "
+          f"iterations {st['pcg_iterations']} (oracle {ost['iterations']}) error vs direct {err:.2e}")
+    assert err <= 5e-5
+
+
 def test_abd_inverse_tolerance_below_reach():
     """A tolerance below round-off is missed after 48 doublings:
     HSC_ERR_ABD_INVERSE (linear_hsc.cu:295-297), as in the oracle."""
@@ -140,12 +186,12 @@ def test_full_residual_limit_is_enforced():
     assert 1e-3 < full_ref < 0.1
     scene = {"linear_system/tol_rate": 1e-10, "linear_system/hsc/max_rank": 2,
              "linear_system/hsc/full_residual_limit": 1e-3}
-    h = _setup(sysm, scene)
+    h = _setup(sysm, scene, abd_inverse="exact")
     x, st = _solve(h, sysm.b, check=False)
     assert st["status_name"] == "HSC_ERR_FULL_RESIDUAL" and st["accepted"] == 0
     assert st["full_relative_residual"] == pytest.approx(full_ref, rel=1e-6)
     assert "exceeds" in h.lib.hsc_last_error(h.h).decode()
-    h2 = _setup(sysm, dict(scene, **{"linear_system/hsc/full_residual_limit": 0.1}))
+    h2 = _setup(sysm, dict(scene, **{"linear_system/hsc/full_residual_limit": 0.1}), abd_inverse="exact")
     _, st2 = _solve(h2, sysm.b)
     assert st2["status"] == 0 and st2["accepted"] == 1
 

@@ -25,6 +25,11 @@ DEV = "cuda:0"
 
 
 def _setup(sysm, scene=None, **ext):
+    """A set-up handle.  Unless a test names another, the ABD inverse is the
+    exact one, so that solutions compare with direct solves at FP64
+    tolerances; the plugin's default (block Neumann + doubling) is compared
+    with its oracle in test_gpu_dropin.py."""
+    ext.setdefault("abd_inverse", "exact")
     cfg = plugin.make_config(scene or {}, **ext)
     h = plugin.Handle(cfg)
     A = plugin.BCOO3.from_system(sysm, DEV)
@@ -320,9 +325,10 @@ def test_plugin_do_solve_writes_jsonl(c1, tmp_path):
     assert row["frame"] == 3 and row["newton"] == 1 and row["accepted"] is True and row["solver"] == "linear_hsc"
     assert row["pcg_iterations"] == st["pcg_iterations"] and row["setup_ms"] > 0
     assert (row["active_mesh_contacts"], row["active_ground_contacts"]) == (17, 4)
-    # default ABD inverse is the exact one: no doubling ran
+    # the plugin's default ABD inverse: the block Neumann polynomial, doubled
     assert row["preconditioner"] == "neumann" and row["precision"] == "fp64"
-    assert row["adaptive_abd_inverse"] is False and row["neumann_doublings"] == 0
+    assert row["adaptive_abd_inverse"] is True and row["neumann_doublings"] >= 1
+    assert row["abd_inverse_residual"] <= 1e-5
     assert row["gbps"] > 0 and row["fp64_oracle_error"] is None
 
 

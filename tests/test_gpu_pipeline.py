@@ -84,7 +84,8 @@ def test_device_pipeline_matches_cpu_and_solves():
              "linear_system/hsc/omega": 0.6, "linear_system/hsc/max_iter": 5000}
     b = _t(sc["b"], torch.float64)
     for name in ("plain_fp64", "new"):
-        h = replay.make_handle(scene, replay.VARIANTS[name], DEV, aca_rtol=1e-6)
+        # exact ABD inverse: the solutions are compared with a dense solve at 1e-7
+        h = replay.make_handle(scene, dict(replay.VARIANTS[name], abd_inverse="exact"), DEV, aca_rtol=1e-6)
         h.setup(A, sc["qo"], sc["qm"], sc["fo"], sc["fn"])
         xs = torch.zeros_like(b)
         st = h.solve(b, xs)
