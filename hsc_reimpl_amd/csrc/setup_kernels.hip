@@ -996,6 +996,101 @@ __device__ void wg_symmetrize(double* P, int n) {
 }
 
 
+// The block Neumann polynomial and its doubling when A has no inter-body
+// blocks: every matrix of the recurrence (B^-1 A, the terms, P, I - A P) is
+// then block diagonal, so each product is one 12 x 12 product per body.  The
+// blocks are kept compact, entry (i, jj) of block row i at i * 12 + jj; the
+// sums run over the block's 12 columns in the order the dense product takes
+// them (its other terms are exact zeros).  The doubling stops on the norm of
+// the whole I - A P, as in the dense form.  Returns the residual.
+__device__ double wg_abd_neumann_blockdiag(const AbdParams& S, int& doublings, double* red) {
+  const int n = S.qm, ne = n * 12;
+  double* term = S.s1;  // compact
+  double* tmp = S.s2;   // compact
+  double* Pc = S.s3;    // compact
+  double* R = S.P;      // compact I - A P while iterating; P written dense at the end
+  auto blk = [&](const double* M, int i, int jj) {  // dense M's entry (i, o + jj), o = block start
+    return M[(size_t)i * n + 12 * (i / 12) + jj];
+  };
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    const double v = S.omega * S.Binv[e];  // Binv is [body][12][12] = compact
+    term[e] = v;
+    Pc[e] = v;
+  }
+  __syncthreads();
+  for (int k = 1; k < S.order; ++k) {
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {  // tmp = A term
+      const int i = e / 12, jj = e % 12, o = 12 * (i / 12);
+      double s = 0.0;
+      for (int m = 0; m < 12; ++m) s += blk(S.A, i, m) * term[(size_t)(o + m) * 12 + jj];
+      tmp[e] = s;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {  // term -= w B^-1 tmp; P += term
+      const int i = e / 12, jj = e % 12, o = 12 * (i / 12);
+      double s = 0.0;
+      for (int m = 0; m < 12; ++m) s += S.Binv[(size_t)i * 12 + m] * tmp[(size_t)(o + m) * 12 + jj];
+      const double t = term[e] - S.omega * s;
+      term[e] = t;
+      Pc[e] += t;
+    }
+    __syncthreads();
+  }
+  auto symmetrize = [&]() {
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const int i = e / 12, jj = e % 12, o = 12 * (i / 12), ii = i - o;
+      if (jj > ii) {
+        const size_t t = (size_t)(o + jj) * 12 + ii;
+        const double s = 0.5 * (Pc[e] + Pc[t]);
+        Pc[e] = s;
+        Pc[t] = s;
+      }
+    }
+    __syncthreads();
+  };
+  __shared__ double s_res;
+  auto residual = [&]() {  // R = I - A P, its norm / sqrt(n)
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const int i = e / 12, jj = e % 12, o = 12 * (i / 12);
+      double s = (i - o == jj) ? 1.0 : 0.0;
+      for (int m = 0; m < 12; ++m) s -= blk(S.A, i, m) * Pc[(size_t)(o + m) * 12 + jj];
+      R[e] = s;
+      acc += s * s;
+    }
+    __syncthreads();
+    const double tot = block_sum_t0(acc, red);
+    if (threadIdx.x == 0) s_res = sqrt(tot) / sqrt((double)n);
+    __syncthreads();
+    return s_res;
+  };
+  symmetrize();
+  double res = residual();
+  if (S.mode == 2) {
+    while (res > S.tol && doublings < 48) {
+      for (int e = threadIdx.x; e < ne; e += blockDim.x) {  // tmp = P (I - A P)
+        const int i = e / 12, jj = e % 12, o = 12 * (i / 12);
+        double s = 0.0;
+        for (int m = 0; m < 12; ++m) s += Pc[(size_t)i * 12 + m] * R[(size_t)(o + m) * 12 + jj];
+        tmp[e] = s;
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < ne; e += blockDim.x) Pc[e] += tmp[e];
+      __syncthreads();
+      symmetrize();
+      res = residual();
+      ++doublings;
+      if (!isfinite(res)) break;
+    }
+  }
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e % n;
+    S.P[e] = (i / 12 == j / 12) ? Pc[(size_t)i * 12 + (j % 12)] : 0.0;
+  }
+  __syncthreads();
+  return res;
+}
+
 // A^-1 approximation P (one workgroup): exact Cholesky inverse, or the block
 // Neumann polynomial P_r = sum_{k<r} (I - w B^-1 A)^k w B^-1 symmetrised, then
 // optional doubling P <- P + P (I - A P) until ||I-AP||_F/sqrt(qm) <= tol
@@ -1004,6 +1099,17 @@ __global__ __launch_bounds__(1024) void k_abd_inverse(AbdParams S) {
   __shared__ double red[1024 / 64];
   const int n = S.qm, nn = n * n;
   int status = 0, doublings = 0;
+  if (S.mode != 0 && S.blockdiag) {
+    const double res = wg_abd_neumann_blockdiag(S, doublings, red);
+    if (S.mode == 2 && !(res <= S.tol)) status = 5;
+    if (!isfinite(res)) status = status ? status : 6;
+    if (threadIdx.x == 0) {
+      S.out[0] = res;
+      S.out[1] = (double)doublings;
+      S.out[2] = (double)status;
+    }
+    return;
+  }
   if (S.mode == 0) {
     if (S.blockdiag) {
       for (int e = threadIdx.x; e < nn; e += blockDim.x) {
