@@ -180,8 +180,7 @@ hipError_t launch_op_lowrank64(const SysDesc& S, const double* x, double* y, dou
 
 // partials of a.b only (*np_out <= 256 of them, into part)
 hipError_t launch_dot_partial(const double* a, const double* b, int n, double* part, int* np_out, hipStream_t st);
-hipError_t launch_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr, hipStream_t st);
-hipError_t launch_node_features(const int* rowptr, const int* colidx, const double* val64, int nnzb, int nb,
-                                float* Dfeat, float* sdiag, hipStream_t st);
+hipError_t launch_node_graph(const int* rowptr, const int* colidx, const double* val64, int nb, int* nbr_ptr,
+                             int* nbr, float* Dfeat, float* sdiag, hipStream_t st);
 
 }  // namespace hsc

@@ -92,6 +92,7 @@ hipError_t MlpNet::get_params(float* theta_host, hipStream_t st) {
 hipError_t MlpNet::set_graph(int nb, const int* nbr_ptr_dev, const int* nbr_dev, int n_edges, hipStream_t st) {
   if (!fits(nb, n_edges)) return hipErrorInvalidValue;
   nb_ = nb;
+  gptr_ = gnbr_ = nullptr;
   TRY(hipMemcpyAsync(ptr_, nbr_ptr_dev, sizeof(int) * (nb + 1), hipMemcpyDefault, st));
   if (n_edges) TRY(hipMemcpyAsync(nbr_, nbr_dev, sizeof(int) * n_edges, hipMemcpyDefault, st));
   return hipSuccess;
@@ -113,7 +114,7 @@ hipError_t MlpNet::forward(const bf16_t* X, int B, float* O, hipStream_t st) {
   g.relu = 1;
   g.Y = H0_;
   TRY(launch_mlp_gemm(g, st));
-  TRY(launch_mlp_aggregate(H0_, ptr_, nbr_, nb_, M, M1_, st));
+  TRY(launch_mlp_aggregate(H0_, gp(), gn(), nb_, M, M1_, st));
   g.A[0] = H0_;
   g.A[1] = M1_;
   g.K[0] = g.K[1] = L::HID;
@@ -123,7 +124,7 @@ hipError_t MlpNet::forward(const bf16_t* X, int B, float* O, hipStream_t st) {
   g.bias = theta_ + L::ob1;
   g.Y = H1_;
   TRY(launch_mlp_gemm(g, st));
-  TRY(launch_mlp_aggregate(H1_, ptr_, nbr_, nb_, M, M2_, st));
+  TRY(launch_mlp_aggregate(H1_, gp(), gn(), nb_, M, M2_, st));
   g.A[0] = H1_;
   g.A[1] = M2_;
   g.Bt[0] = wt_ + L::oWs2;
@@ -175,7 +176,7 @@ hipError_t MlpNet::backward(const bf16_t* X, const float* T, const float* den, i
   g.Bt[0] = wb_ + L::oWn2;
   g.Yf = G2_;
   TRY(launch_mlp_gemm(g, st));
-  TRY(launch_mlp_aggT_mask(G1_, G2_, H1_, ptr_, nbr_, nb_, M, dA1_, st));
+  TRY(launch_mlp_aggT_mask(G1_, G2_, H1_, gp(), gn(), nb_, M, dA1_, st));
   // layer 1
   TRY(wgrad(H0_, 64, 64, dA1_, 64, 64, L::oWs1));
   TRY(wgrad(M1_, 64, 64, dA1_, 64, 64, L::oWn1));
@@ -187,7 +188,7 @@ hipError_t MlpNet::backward(const bf16_t* X, const float* T, const float* den, i
   g.Bt[0] = wb_ + L::oWn1;
   g.Yf = G2_;
   TRY(launch_mlp_gemm(g, st));
-  TRY(launch_mlp_aggT_mask(G1_, G2_, H0_, ptr_, nbr_, nb_, M, dA0_, st));
+  TRY(launch_mlp_aggT_mask(G1_, G2_, H0_, gp(), gn(), nb_, M, dA0_, st));
   // layer 0
   TRY(wgrad(X, 16, 16, dA0_, 64, 64, L::oW0));
   TRY(launch_mlp_colsum(dA0_, M, 64, 64, kRowsPerChunk, part_, grad_ + L::ob0, st));

@@ -37,6 +37,13 @@ class MlpNet {
   // another graph of at most the nodes and edges init was given (copied; no allocation)
   hipError_t set_graph(int nb, const int* nbr_ptr_dev, const int* nbr_dev, int n_edges, hipStream_t st);
   bool fits(int nb, int n_edges) const { return nb <= cap_nb_ && n_edges <= cap_edges_; }
+  // use a graph the caller keeps alive and unchanged while the net uses it
+  // (not copied; the forward and backward passes read it in place)
+  void borrow_graph(int nb, const int* nbr_ptr_dev, const int* nbr_dev) {
+    nb_ = nb;
+    gptr_ = nbr_ptr_dev;
+    gnbr_ = nbr_dev;
+  }
   hipError_t set_params(const float* theta, bool device_ptr, hipStream_t st);
   hipError_t get_params(float* theta_host, hipStream_t st);
   hipError_t forward(const bf16_t* X, int B, float* O, hipStream_t st);
@@ -51,6 +58,10 @@ class MlpNet {
   int nb_ = 0, max_batch_ = 0, step_ = 0, cap_nb_ = 0, cap_edges_ = 0;
   int* ptr_ = nullptr;
   int* nbr_ = nullptr;
+  const int* gptr_ = nullptr;  // the graph in use: borrowed, else ptr_ / nbr_
+  const int* gnbr_ = nullptr;
+  const int* gp() const { return gptr_ ? gptr_ : ptr_; }
+  const int* gn() const { return gptr_ ? gnbr_ : nbr_; }
   float* theta_ = nullptr;
   float* grad_ = nullptr;
   float* m_ = nullptr;

@@ -1366,24 +1366,22 @@ __global__ void k_dot_final(const double* part, int np, double* out) {
 }
 // x += alpha p ; r -= alpha Ap
 
-// neighbour CSR of the BSR pattern without the diagonal block (one per row)
-__global__ void k_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr) {
+// One thread per block row i of D: the neighbour CSR of the BSR pattern
+// without the diagonal block (nbr_ptr[i] = rowptr[i] - i, the row's other
+// columns in order) and the row's diagonal features Dfeat_i = D_ii / s_i
+// (row-major 3x3), sdiag_i = s_i = tr(D_ii) / 3 (1 when not positive).
+__global__ void k_node_graph(const int* rowptr, const int* colidx, const double* val64, int nb, int* nbr_ptr,
+                             int* nbr, float* Dfeat, float* sdiag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > nb) return;
   nbr_ptr[i] = rowptr[i] - i;
   if (i == nb) return;
-  int o = rowptr[i] - i;
-  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p)
-    if (colidx[p] != i) nbr[o++] = colidx[p];
-}
-// Dfeat_i = D_ii / s_i (row-major 3x3), s_i = tr(D_ii)/3
-__global__ void k_node_features(const int* rowptr, const int* colidx, const double* val64, int nnzb, int nb,
-                                float* Dfeat, float* sdiag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  int pd = -1;
-  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p)
-    if (colidx[p] == i) pd = p;
+  int o = rowptr[i] - i, pd = -1;
+  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+    const int c = colidx[p];
+    if (c != i) nbr[o++] = c;
+    else pd = p;
+  }
   double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (pd >= 0)
     for (int r = 0; r < 3; ++r)
@@ -1400,15 +1398,10 @@ hipError_t launch_dot_partial(const double* a, const double* b, int n, double* p
   *np_out = np;
   return hipGetLastError();
 }
-hipError_t launch_graph(const int* rowptr, const int* colidx, int nb, int* nbr_ptr, int* nbr, hipStream_t st) {
-  hipLaunchKernelGGL(k_graph, dim3((nb + 1 + 255) / 256), dim3(256), 0, st, rowptr, colidx, nb, nbr_ptr, nbr);
-  return hipGetLastError();
-}
-hipError_t launch_node_features(const int* rowptr, const int* colidx, const double* val64, int nnzb, int nb,
-                                float* Dfeat, float* sdiag, hipStream_t st) {
-  if (nb > 0)
-    hipLaunchKernelGGL(k_node_features, dim3((nb + 255) / 256), dim3(256), 0, st, rowptr, colidx, val64, nnzb, nb,
-                       Dfeat, sdiag);
+hipError_t launch_node_graph(const int* rowptr, const int* colidx, const double* val64, int nb, int* nbr_ptr,
+                             int* nbr, float* Dfeat, float* sdiag, hipStream_t st) {
+  hipLaunchKernelGGL(k_node_graph, dim3((nb + 1 + 255) / 256), dim3(256), 0, st, rowptr, colidx, val64, nb, nbr_ptr,
+                     nbr, Dfeat, sdiag);
   return hipGetLastError();
 }
 

@@ -396,10 +396,10 @@ __global__ __launch_bounds__(kBlock) void k_aca(AcaParams P) {
 // path is taken when the ACA has no relative-residual stop, no column
 // touches two bodies, the blocks cannot reach the rank limit together
 // (sum over bodies of min(12, columns) <= limit) and every block fits in
-// registers (12 x columns <= kAcaBlkPer x kBlock); otherwise k_aca or
+// registers (columns <= kAcaBlkCpt x kBlock); otherwise k_aca or
 // k_aca_reg factor the whole E (they exit at once when this path ran).
-constexpr int kAcaBlkPer = 32;                         // residual entries per thread
-constexpr int kAcaBlkCols = kAcaBlkPer * kBlock / 12;  // columns per body
+constexpr int kAcaBlkCpt = 3;                   // block columns per thread (12 rows each)
+constexpr int kAcaBlkCols = kAcaBlkCpt * kBlock;  // columns per body
 
 // body of column c (-1: no nonzero, -2: nonzeros in two bodies) and |E[:, c]|^2;
 // the 12 rows of a body are loaded together
@@ -459,17 +459,22 @@ __global__ __launch_bounds__(kBlock) void k_aca_plan(AcaBlkParams P) {
   for (int bb = tid; bb < nbody && bb < kAcaBlkMaxBodies; bb += kBlock) P.plan[4 + bb] = s_cnt[bb];
 }
 
-// one workgroup per body: full-pivot ACA of the body's 12 x n block
+// one workgroup per body: full-pivot ACA of the body's 12 x n block.  Thread
+// t owns the block's columns t, t + kBlock, ... (all 12 rows of each) in
+// registers, so the pivot row's entries v_c = R[pil][c] / piv stay with
+// their owners and only the pivot column (12 values) goes through LDS.  The
+// block's entries keep the whole-E order: a column's rank among the body's
+// columns follows its global index, so (row, local column) orders entries as
+// the global row-major index does (the argmax tie-break).
 __global__ __launch_bounds__(kBlock) void k_aca_blocks(AcaBlkParams P) {
   __shared__ int s_cols[kAcaBlkCols];
   __shared__ int s_wcnt[kBlock / 64];
   __shared__ double s_m[kBlock / 64], s_q[kBlock / 64];
-  __shared__ long long s_i[kBlock / 64];
-  __shared__ double s_u[12], s_v[kAcaBlkCols];
+  __shared__ int s_i[kBlock / 64];
+  __shared__ double s_u[12];
   __shared__ double s_piv;
-  __shared__ int s_pil, s_pjl, s_n;
-  __shared__ long long s_best;
-  __shared__ double s_bestv;
+  __shared__ int s_n, s_best;
+  __shared__ double s_bestv, s_sq;
   if (!P.plan[0]) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int body = blockIdx.x, r0 = 12 * body, a = P.a;
@@ -493,31 +498,47 @@ __global__ __launch_bounds__(kBlock) void k_aca_blocks(AcaBlkParams P) {
     }
     __syncthreads();
   }
-  const int n = s_n, cnt = 12 * n;
-  double R[kAcaBlkPer];
+  const int n = s_n;
+  constexpr int kNone = 0x7fffffff;
+  double R[kAcaBlkCpt][12];
+  int gcol[kAcaBlkCpt];
 #pragma unroll
-  for (int j = 0; j < kAcaBlkPer; ++j) {
-    const int le = tid + j * kBlock;
-    R[j] = le < cnt ? P.E[(size_t)(r0 + le / n) * a + s_cols[le % n]] : 0.0;
+  for (int j = 0; j < kAcaBlkCpt; ++j) {
+    const int c = tid + j * kBlock;
+    gcol[j] = c < n ? s_cols[c] : -1;
+#pragma unroll
+    for (int r = 0; r < 12; ++r) R[j][r] = c < n ? P.E[(size_t)(r0 + r) * a + gcol[j]] : 0.0;
   }
   const double scale = *P.scale;
   const double guard = DBL_EPSILON * fmax(scale, DBL_MIN);
-  // argmax |R| (lowest global row-major index on ties) and |R|^2 of the block
-  auto reduce = [&](double& best_v, long long& best_i, double& sq) {
+  // argmax |R| (lowest (row, column) on ties; key = row * 2048 + local column)
+  // and |R|^2 of the block, left in s_bestv / s_best / s_sq
+  auto reduce = [&]() {
     double m = -1.0, q = 0.0;
-    long long mi = 0x7fffffffffffffffLL;
+    int mi = kNone;
 #pragma unroll
-    for (int j = 0; j < kAcaBlkPer; ++j) {
-      const int le = tid + j * kBlock;
-      if (le < cnt) {
-        argmax_combine(m, mi, fabs(R[j]), (long long)(r0 + le / n) * a + s_cols[le % n]);
-        q += R[j] * R[j];
+    for (int j = 0; j < kAcaBlkCpt; ++j) {
+      const int c = tid + j * kBlock;
+      if (c < n) {
+#pragma unroll
+        for (int r = 0; r < 12; ++r) {
+          const double v = fabs(R[j][r]);
+          const int key = (r << 11) | c;
+          if (v > m || (v == m && key < mi)) {
+            m = v;
+            mi = key;
+          }
+          q += R[j][r] * R[j][r];
+        }
       }
     }
     for (int off = 32; off >= 1; off >>= 1) {
       const double m2 = __shfl_xor(m, off, 64);
-      const long long i2 = __shfl_xor(mi, off, 64);
-      argmax_combine(m, mi, m2, i2);
+      const int i2 = __shfl_xor(mi, off, 64);
+      if (m2 > m || (m2 == m && i2 < mi)) {
+        m = m2;
+        mi = i2;
+      }
       q += __shfl_xor(q, off, 64);
     }
     if (lane == 0) {
@@ -528,89 +549,105 @@ __global__ __launch_bounds__(kBlock) void k_aca_blocks(AcaBlkParams P) {
     __syncthreads();
     if (tid == 0) {
       double M = s_m[0], Q = s_q[0];
-      long long I = s_i[0];
+      int I = s_i[0];
       for (int w = 1; w < kBlock / 64; ++w) {
-        argmax_combine(M, I, s_m[w], s_i[w]);
+        if (s_m[w] > M || (s_m[w] == M && s_i[w] < I)) {
+          M = s_m[w];
+          I = s_i[w];
+        }
         Q += s_q[w];
       }
       s_bestv = M;
       s_best = I;
-      s_q[0] = Q;
+      s_sq = Q;
     }
     __syncthreads();
-    best_v = s_bestv;
-    best_i = s_best;
-    sq = s_q[0];
-    __syncthreads();
   };
-  double bv, sq;
-  long long bi;
-  reduce(bv, bi, sq);
-  if (tid == 0) P.hblk[body * 13] = sq;
+  reduce();
+  if (tid == 0) P.hblk[body * 13] = s_sq;
   int k = 0;
-  while (n > 0 && k < 12 && bi != 0x7fffffffffffffffLL) {
-    // the pivot's owner publishes its value and block position
+  while (n > 0 && k < 12 && s_best != kNone) {
+    const int bi = s_best, pil = bi >> 11, pjl = bi & 2047;
+    // the pivot column's owner publishes it (and the pivot)
+    if (pjl % kBlock == tid) {
+      const int j = pjl / kBlock;
 #pragma unroll
-    for (int j = 0; j < kAcaBlkPer; ++j) {
-      const int le = tid + j * kBlock;
-      if (le < cnt && (long long)(r0 + le / n) * a + s_cols[le % n] == bi) {
-        s_piv = R[j];
-        s_pil = le / n;
-        s_pjl = le % n;
-      }
+      for (int jj = 0; jj < kAcaBlkCpt; ++jj)
+        if (jj == j) {
+#pragma unroll
+          for (int r = 0; r < 12; ++r) s_u[r] = R[jj][r];
+        }
+      s_piv = s_u[pil];
     }
     __syncthreads();
     const double piv = s_piv;
     if (!(fabs(piv) > guard)) break;  // uniform
-    const int pil = s_pil, pjl = s_pjl;
-#pragma unroll
-    for (int j = 0; j < kAcaBlkPer; ++j) {
-      const int le = tid + j * kBlock;
-      if (le < cnt) {
-        const int r = le / n, c = le % n;
-        if (c == pjl) s_u[r] = R[j];
-        if (r == pil) s_v[c] = R[j] / piv;
-      }
-    }
-    __syncthreads();
     if (tid < 12) P.ublk[(size_t)body * 144 + k * 12 + tid] = s_u[tid];
-    for (int c = tid; c < n; c += kBlock) P.vtmp[(size_t)s_cols[c] * 12 + k] = s_v[c];
     if (tid == 0) {
       P.pblk[body * 24 + 2 * k] = r0 + pil;
       P.pblk[body * 24 + 2 * k + 1] = s_cols[pjl];
     }
+    double u[12];
 #pragma unroll
-    for (int j = 0; j < kAcaBlkPer; ++j) {
-      const int le = tid + j * kBlock;
-      if (le < cnt) R[j] = __dsub_rn(R[j], __dmul_rn(s_u[le / n], s_v[le % n]));
+    for (int r = 0; r < 12; ++r) u[r] = s_u[r];
+#pragma unroll
+    for (int j = 0; j < kAcaBlkCpt; ++j) {
+      if (tid + j * kBlock < n) {
+        double pr = R[j][0];
+#pragma unroll
+        for (int r = 1; r < 12; ++r) pr = r == pil ? R[j][r] : pr;
+        const double v = pr / piv;
+        P.vtmp[(size_t)gcol[j] * 12 + k] = v;
+#pragma unroll
+        for (int r = 0; r < 12; ++r) R[j][r] = __dsub_rn(R[j][r], __dmul_rn(u[r], v));
+      }
     }
     ++k;
-    reduce(bv, bi, sq);
-    if (tid == 0) P.hblk[body * 13 + k] = sq;
+    __syncthreads();  // s_u / s_piv are rewritten by the next step
+    reduce();
+    if (tid == 0) P.hblk[body * 13 + k] = s_sq;
   }
   if (tid == 0) {
     P.rank_b[body] = k;
     // a 13th pivot above the guard (rounding left in a 12-row block): the
     // finish kernel hands the factorisation back to the whole-E ACA
-    if (k == 12 && bi != 0x7fffffffffffffffLL && bv > guard) P.plan[1] = 1;
+    if (k == 12 && s_best != kNone && s_bestv > guard) P.plan[1] = 1;
   }
 }
 
 // the factors in k_aca's format: U, V concatenated body by body, pivots,
-// the residual history in that order, rank and scale
+// the residual history in that order, rank and scale.  Per-body ranks and
+// residuals are staged in LDS; one thread then forms the running sums over
+// the bodies (the history's fixed order) and every pivot is written by its
+// own thread.
 __global__ __launch_bounds__(256) void k_aca_blocks_finish(AcaBlkParams P) {
-  __shared__ int s_off[kAcaBlkMaxBodies + 1];
+  __shared__ int s_off[kAcaBlkMaxBodies + 1], s_rk[kAcaBlkMaxBodies];
+  __shared__ double s_h0[kAcaBlkMaxBodies], s_hl[kAcaBlkMaxBodies];
+  __shared__ double s_done[kAcaBlkMaxBodies], s_rest[kAcaBlkMaxBodies];
   if (!P.plan[0]) return;
   const int tid = threadIdx.x, nbody = P.qm / 12;
   if (P.plan[1]) {  // a block needed more than 12 steps: the whole-E ACA runs instead
     if (blockIdx.x == 0 && tid == 0) P.plan[0] = 0;
     return;
   }
+  for (int bb = tid; bb < nbody; bb += blockDim.x) {
+    const int rk = P.rank_b[bb];
+    s_rk[bb] = rk;
+    s_h0[bb] = P.hblk[bb * 13];
+    s_hl[bb] = P.hblk[bb * 13 + rk];
+  }
+  __syncthreads();
   if (tid == 0) {
     int o = 0;
+    double rest = 0.0, done = 0.0;
+    for (int bb = 0; bb < nbody; ++bb) rest += s_h0[bb];
     for (int bb = 0; bb < nbody; ++bb) {
       s_off[bb] = o;
-      o += P.rank_b[bb];
+      o += s_rk[bb];
+      rest -= s_h0[bb];
+      s_done[bb] = done;
+      s_rest[bb] = rest;
+      done += s_hl[bb];
     }
     s_off[nbody] = o;
   }
@@ -621,33 +658,30 @@ __global__ __launch_bounds__(256) void k_aca_blocks_finish(AcaBlkParams P) {
     if (e < nu) {
       const int g = (int)(e / qm), r = (int)(e % qm), bb = r / 12;
       const int kk = g - s_off[bb];
-      P.U[e] = (kk >= 0 && kk < P.rank_b[bb]) ? P.ublk[(size_t)bb * 144 + kk * 12 + r % 12] : 0.0;
+      P.U[e] = (kk >= 0 && kk < s_rk[bb]) ? P.ublk[(size_t)bb * 144 + kk * 12 + r % 12] : 0.0;
     } else {
       const long long f = e - nu;
       const int g = (int)(f / a), c = (int)(f % a), bb = P.colbody[c];
       const int kk = bb >= 0 ? g - s_off[bb] : -1;
-      P.V[f] = (bb >= 0 && kk >= 0 && kk < P.rank_b[bb]) ? P.vtmp[(size_t)c * 12 + kk] : 0.0;
+      P.V[f] = (bb >= 0 && kk >= 0 && kk < s_rk[bb]) ? P.vtmp[(size_t)c * 12 + kk] : 0.0;
     }
   }
-  if (blockIdx.x == 0 && tid == 0) {
+  if (blockIdx.x == 0) {
     const double scale = *P.scale;
-    double done = 0.0, rest = 0.0;
-    for (int bb = 0; bb < nbody; ++bb) rest += P.hblk[bb * 13];
-    P.hist[0] = scale > 0.0 ? 1.0 : 0.0;
-    int g = 0;
-    for (int bb = 0; bb < nbody; ++bb) {
-      const int rk = P.rank_b[bb];
-      rest -= P.hblk[bb * 13];
-      for (int kk = 0; kk < rk; ++kk, ++g) {
-        P.pivots[2 * g] = P.pblk[bb * 24 + 2 * kk];
-        P.pivots[2 * g + 1] = P.pblk[bb * 24 + 2 * kk + 1];
-        const double tot = done + P.hblk[bb * 13 + kk + 1] + fmax(rest, 0.0);
-        P.hist[g + 1] = scale > 0.0 ? sqrt(fmax(tot, 0.0)) / scale : 0.0;
-      }
-      done += P.hblk[bb * 13 + rk];
+    for (int g = tid; g < K; g += blockDim.x) {
+      int bb = 0;
+      while (s_off[bb + 1] <= g) ++bb;
+      const int kk = g - s_off[bb];
+      P.pivots[2 * g] = P.pblk[bb * 24 + 2 * kk];
+      P.pivots[2 * g + 1] = P.pblk[bb * 24 + 2 * kk + 1];
+      const double tot = s_done[bb] + P.hblk[bb * 13 + kk + 1] + fmax(s_rest[bb], 0.0);
+      P.hist[g + 1] = scale > 0.0 ? sqrt(fmax(tot, 0.0)) / scale : 0.0;
     }
-    *P.rank_out = K;
-    *P.scale_out = scale;
+    if (tid == 0) {
+      P.hist[0] = scale > 0.0 ? 1.0 : 0.0;
+      *P.rank_out = K;
+      *P.scale_out = scale;
+    }
   }
 }
 

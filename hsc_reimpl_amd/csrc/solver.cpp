@@ -1111,9 +1111,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                       h->rowptr.as<int>(), h->cursor.as<int>(), h->key_col.as<int>(), h->key_src.as<int>(),
                       h->act_node.as<int>(), a, h->E.as<double>()};
   HIPCHK(hsc::launch_place(pp, st));
-  // the D side (row sort, value gather, power iteration) depends on nothing
-  // after this point; it is submitted to the second stream once ACA and the
-  // ABD inverse are queued on this one, so that it runs beside them
+  // the D side waits for this point only
   HIPCHK(hipEventRecord(h->aux_ev, st));
   AuxJoin join{nullptr};
   // ---- ACA
@@ -1196,6 +1194,17 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   if (h->cfg.abd_inverse == HSC_ABD_NEUMANN && h->cfg.adaptive_abd_inverse) abp.mode = 2;
   HIPCHK(hsc::launch_abd_inverse(abp, st));
   h->abd_mode = abp.mode;
+  ss->aca_err = 0;
+  HIPCHK(hipMemcpyAsync(ss->ints2, err + 8, sizeof(ss->ints2), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&ss->max_coupled, err + 12, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(ss->dm, dmisc, sizeof(ss->dm), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(ss->hist, h->hist.as<double>(), sizeof(double) * (lim1 + 1), hipMemcpyDeviceToHost, st));
+  if (limit > 0) HIPCHK(hipMemcpyAsync(&ss->aca_err, h->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  // The D side (row sort and value gather, then the network and the power
+  // iteration) depends on nothing after k_place.  It is submitted once the
+  // main stream's short ACA / ABD-inverse chain and its read-backs are
+  // queued, so that the host's launch time for the two longer chains
+  // overlaps that chain's execution and the ACA kernels start first.
   HIPCHK(hipStreamWaitEvent(h->aux, h->aux_ev, 0));
   join.s = h->aux;
   HIPCHK(hsc::launch_sort_gather(h->rowptr.as<int>(), nb, h->key_col.as<int>(), h->key_src.as<int>(), Av->values,
@@ -1203,7 +1212,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   AuxJoin join2{nullptr};
   if (learned) {
     // the network needs D only: it runs on a third stream beside the power
-    // iteration, both hidden behind ACA
+    // iteration
     HIPCHK(hipEventRecord(h->d_ev, h->aux));
     HIPCHK(hipStreamWaitEvent(h->aux2, h->d_ev, 0));
     join2.s = h->aux2;
@@ -1211,12 +1220,6 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   }
   ss->lam2 = 0.0;
   if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
-  ss->aca_err = 0;
-  HIPCHK(hipMemcpyAsync(ss->ints2, err + 8, sizeof(ss->ints2), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&ss->max_coupled, err + 12, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(ss->dm, dmisc, sizeof(ss->dm), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(ss->hist, h->hist.as<double>(), sizeof(double) * (lim1 + 1), hipMemcpyDeviceToHost, st));
-  if (limit > 0) HIPCHK(hipMemcpyAsync(&ss->aca_err, h->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   const int* ints2 = ss->ints2;
   const double* dm = ss->dm;
@@ -1272,14 +1275,15 @@ void ensure_graph(hsc_handle* h, hipStream_t st) {
   HIPCHK(h->nbr.ensure(sizeof(int) * std::max(ne, 1)));
   HIPCHK(h->dfeat.ensure(sizeof(float) * 9 * nb));
   HIPCHK(h->sdiag.ensure(sizeof(float) * nb));
-  HIPCHK(hsc::launch_graph(h->rowptr.as<int>(), h->colidx.as<int>(), nb, h->nbr_ptr.as<int>(), h->nbr.as<int>(), st));
-  HIPCHK(hsc::launch_node_features(h->rowptr.as<int>(), h->colidx.as<int>(), h->val64.as<double>(), h->nnzb, nb,
-                                   h->dfeat.as<float>(), h->sdiag.as<float>(), st));
+  HIPCHK(hsc::launch_node_graph(h->rowptr.as<int>(), h->colidx.as<int>(), h->val64.as<double>(), nb,
+                                 h->nbr_ptr.as<int>(), h->nbr.as<int>(), h->dfeat.as<float>(), h->sdiag.as<float>(),
+                                 st));
 }
 
-// The handle's network over its current node graph (re-pointed at each setup;
-// reallocated only for a larger graph), with the loaded weights (cast to
-// bf16 again only when they changed).
+// The handle's network over its current node graph (the handle's own
+// neighbour CSR, read in place; the net is reallocated only for a larger
+// graph), with the loaded weights (cast to bf16 again only when they
+// changed).
 void ensure_mlp(hsc_handle* h, hipStream_t st) {
   if (h->learned_bytes != (long long)sizeof(float) * hsc::MlpLayout::count)
     throw HscError{HSC_ERR_INVALID_ARGUMENT, "learned preconditioner selected but no weights set"};
@@ -1289,9 +1293,8 @@ void ensure_mlp(hsc_handle* h, hipStream_t st) {
     h->mlp.reset(new hsc::MlpNet());
     HIPCHK(h->mlp->init(h->nb, h->nbr_ptr.as<int>(), h->nbr.as<int>(), ne, 1, st));
     h->mlp_weights_version = -1;
-  } else {
-    HIPCHK(h->mlp->set_graph(h->nb, h->nbr_ptr.as<int>(), h->nbr.as<int>(), ne, st));
   }
+  h->mlp->borrow_graph(h->nb, h->nbr_ptr.as<int>(), h->nbr.as<int>());
   if (h->mlp_weights_version != h->learned_version) {
     HIPCHK(h->mlp->set_params(h->learned.as<float>(), true, st));
     h->mlp_weights_version = h->learned_version;
