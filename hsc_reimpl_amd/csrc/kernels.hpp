@@ -56,7 +56,8 @@ struct FactorParams {
   const double* U;   // qm x ldk col-major (ld = qm)
   const double* V;   // a x ldk col-major (ld = a)
   int qm, a, k;
-  double* W1;        // qm x k scratch
+  double* W1;        // qm x k scratch (P U)
+  double* W1T;       // k x qm: (P U)^T, kept for the right-hand side of every solve
   double* C;         // k x k (out: C; then L in lower triangle of Lm)
   double* Lm;        // k x k lower factor
   double* out;       // [0] status (0 ok, 6 nonfinite / not SPD)
@@ -64,7 +65,7 @@ struct FactorParams {
 
 // setup_kernels.hip
 hipError_t launch_split(const SplitParams& S, hipStream_t st);
-hipError_t launch_scan(const int* in, int* out, int n, hipStream_t st);
+hipError_t launch_scan2(const int* in0, int* out0, const int* in1, int* out1, int n, hipStream_t st);
 hipError_t launch_active(const int* flag, const int* prefix, int nb, int* act_node, int* act_dof, hipStream_t st);
 // *out = max over tiles of `tile` consecutive nodes of the active-node count (*out zeroed before)
 hipError_t launch_max_tile_active(const int* act_node, int nb, int tile, int* out, hipStream_t st);
@@ -83,9 +84,8 @@ hipError_t launch_body_inverse(const double* A, int qm, double* Binv, int* err, 
 hipError_t launch_abd_inverse(const AbdParams& S, hipStream_t st);
 hipError_t launch_coupling_core(const FactorParams& S, hipStream_t st);
 hipError_t launch_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32, hipStream_t st);
-hipError_t launch_rhs(const double* P, const double* U, const double* Lm, const double* b, int qo, int qm, int fo,
-                      int n, int k, const int* act_node, const double* W64, double* tmp, double* y, double* rhs,
-                      double* r, double* x, hipStream_t st);
+hipError_t launch_rhs(const double* W1T, const double* V, const double* b, int qo, int qm, int fo, int n, int k,
+                      int a, const int* act_node, double* y, double* rhs, double* r, double* x, hipStream_t st);
 hipError_t launch_backsub(const double* P, const double* U, const double* V, const int* act_dof, const double* b,
                           double* x, int qo, int qm, int fo, int a, int k, double* tmp, hipStream_t st);
 hipError_t launch_full_residual(const double* yf, const double* E, const double* A, const int* act_node,

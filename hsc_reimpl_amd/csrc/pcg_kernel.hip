@@ -12,6 +12,9 @@
 // same fixed order, so all of them take the same decisions and the result is
 // bitwise reproducible.  Grid barriers have bounded waits.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "common.hpp"
 #include "device_util.hpp"
@@ -1507,9 +1510,35 @@ hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, siz
   return hipErrorInvalidValue;
 }
 
-hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks) {
+static hipError_t pcg_occupancy_query(int G, bool mixed, bool res, size_t shmem, int* blocks) {
   HSC_PCG_DISPATCH(occ_t, blocks, shmem);
   return hipErrorInvalidValue;
+}
+
+// the runtime's occupancy query costs tens of microseconds of host time; its
+// answer depends only on the device, the kernel and the LDS size, so it is
+// kept per process
+hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int, size_t>, int> cache;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const auto key = std::make_tuple(dev, G, (int)mixed, (int)res, shmem);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      *blocks = it->second;
+      return hipSuccess;
+    }
+  }
+  e = pcg_occupancy_query(G, mixed, res, shmem, blocks);
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = *blocks;
+  }
+  return e;
 }
 
 template <int G, typename VT, bool RES>

@@ -5,6 +5,8 @@
 // back-substitution and acceptance checks.
 // Reference semantics: linear_hsc.cu:61-91,195-304,349-435 and
 // core.py:22-50,53-92,141-168 (reproduction/ of the reference).
+#include <map>
+#include <mutex>
 #include "common.hpp"
 #include "device_util.hpp"
 #include "kernels.hpp"
@@ -91,7 +93,7 @@ __global__ void k_classify(SplitParams S) {
 
 // Single-workgroup exclusive scan: out[i] = sum_{j<i} in[i], out[n] = total.
 // If mul != 1 the output is scaled (used to turn a node count into W rows).
-__global__ __launch_bounds__(1024) void k_scan(const int* in, int* out, int n) {
+__device__ void wg_scan_strided(const int* in, int* out, int n) {
   constexpr int kPer = 32;  // entries per thread held in registers (n <= 32768)
   __shared__ int s[1024];
   const int tid = threadIdx.x;
@@ -134,6 +136,65 @@ __global__ __launch_bounds__(1024) void k_scan(const int* in, int* out, int n) {
     }
   }
   if (tid == 1023) out[n] = s[1023];
+}
+
+
+// Exclusive scan of n ints by one workgroup of 1024 threads (out[n] = the
+// total).  Each of the 16 waves owns a contiguous run of 64-entry chunks,
+// loads them coalesced and all at once, scans each chunk across its lanes
+// and carries the chunk totals; the waves' totals are then scanned in LDS.
+// Runs longer than 32 chunks per wave (n > 32768) take the strided path.
+__device__ void wg_scan(const int* in, int* out, int n) {
+  constexpr int kWaves = 1024 / 64, kCh = 32;
+  __shared__ int s_tot[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int seg = (((n + kWaves - 1) / kWaves) + 63) & ~63;
+  const int nch = seg / 64;
+  if (n <= 0 || nch > kCh) {
+    wg_scan_strided(in, out, n);
+    return;
+  }
+  const int base = w * seg;
+  int v[kCh];
+#pragma unroll
+  for (int j = 0; j < kCh; ++j) {
+    const int idx = base + 64 * j + lane;
+    const int x = in[min(idx, n - 1)];
+    v[j] = (j < nch && idx < n) ? x : 0;
+  }
+  int carry = 0;
+#pragma unroll
+  for (int j = 0; j < kCh; ++j) {
+    if (j < nch) {
+      int x = v[j];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+      }
+      v[j] = carry + x - v[j];  // exclusive within the run
+      carry += __shfl(x, 63, 64);
+    }
+  }
+  if (lane == 0) s_tot[w] = carry;
+  __syncthreads();
+  int off = 0;
+  for (int q = 0; q < w; ++q) off += s_tot[q];
+#pragma unroll
+  for (int j = 0; j < kCh; ++j) {
+    const int idx = base + 64 * j + lane;
+    if (j < nch && idx < n) out[idx] = off + v[j];
+  }
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int q = 0; q < kWaves; ++q) t += s_tot[q];
+    out[n] = t;
+  }
+}
+
+// blockIdx.x scans (in0 -> out0) or (in1 -> out1), n entries each
+__global__ __launch_bounds__(1024) void k_scan2(const int* in0, int* out0, const int* in1, int* out1, int n) {
+  wg_scan(blockIdx.x ? in1 : in0, blockIdx.x ? out1 : out0, n);
 }
 
 // act_node[i] = 3*prefix (first W row of the node) or -1; act_dof[W row] = FEM DOF
@@ -886,7 +947,7 @@ __global__ __launch_bounds__(kBlock) void k_aca_reg(AcaParams P) {
 // ---------------------------------------------------------------- small dense LA
 // Per-body 12x12 Cholesky + inverse of the diagonal ABD blocks
 // (core.py:53-71 BlockInverse; linear_hsc.cu:264-271).  One wave per body.
-__global__ void k_body_inverse(const double* A, int qm, double* Binv, int* err) {
+__global__ __launch_bounds__(64) void k_body_inverse(const double* A, int qm, double* Binv, int* err) {
   __shared__ double L[12][13];
   __shared__ int ok;
   const int body = blockIdx.x, tid = threadIdx.x, o = 12 * body;
@@ -913,18 +974,23 @@ __global__ void k_body_inverse(const double* A, int qm, double* Binv, int* err) 
   }
   __syncthreads();
   if (!ok) return;
-  if (tid < 12) {  // column tid of (L L^T)^-1
+  if (tid < 12) {  // column tid of (L L^T)^-1 (loops unrolled: y stays in registers)
     double y[12];
+#pragma unroll
     for (int i = 0; i < 12; ++i) {
       double s = (i == tid) ? 1.0 : 0.0;
+#pragma unroll
       for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
       y[i] = s / L[i][i];
     }
+#pragma unroll
     for (int i = 11; i >= 0; --i) {
       double s = y[i];
+#pragma unroll
       for (int k = i + 1; k < 12; ++k) s -= L[k][i] * y[k];
       y[i] = s / L[i][i];
     }
+#pragma unroll
     for (int i = 0; i < 12; ++i) Binv[(size_t)body * 144 + i * 12 + tid] = y[i];
   }
 }
@@ -1222,7 +1288,8 @@ __global__ __launch_bounds__(1024) void k_abd_inverse(AbdParams S) {
 // transA (A stored column-major with leading dim lda, i.e. A^T row-major).
 // B is addressed as B[k*ldb_k + n*ldb_n] so both layouts are covered.
 __global__ __launch_bounds__(256) void k_gemm_small(const double* A, int lda, int transA, const double* B,
-                                                    int ldb_k, int ldb_n, double* C, int ldc, int M, int N, int K) {
+                                                    int ldb_k, int ldb_n, double* C, int ldc, int M, int N, int K,
+                                                    double* Ct) {
   __shared__ double As[16][17], Bs[16][17];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
@@ -1237,14 +1304,16 @@ __global__ __launch_bounds__(256) void k_gemm_small(const double* A, int lda, in
     for (int kk = 0; kk < 16; ++kk) acc += As[ty][kk] * Bs[kk][tx];
     __syncthreads();
   }
-  if (row < M && col < N) C[(size_t)row * ldc + col] = acc;
+  if (row < M && col < N) {
+    C[(size_t)row * ldc + col] = acc;
+    if (Ct) Ct[(size_t)col * M + row] = acc;  // C^T (N x M row-major) too
+  }
 }
 
 constexpr int kLdsChol = 80;  // C of rank <= 80 is factored in LDS (50 KB)
-
-// C = U^T P U symmetrised (linear_hsc.cu:298-300) and its Cholesky C = L L^T.
-// (the products W1 = P U and C = U^T W1 are done by k_gemm_small before this)
-__global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
+// C = U^T P U symmetrised (linear_hsc.cu:298-300), copied to Lm; 6 when an
+// entry is not finite, else 0.
+__device__ int coupling_prelude(const FactorParams& S) {
   const int k = S.k;
   __syncthreads();
   wg_symmetrize(S.C, k);
@@ -1258,7 +1327,14 @@ __global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
   __syncthreads();
   if (bad) atomicOr(&s_bad, 1);
   __syncthreads();
-  int status = s_bad ? 6 : 0;
+  return s_bad ? 6 : 0;
+}
+
+// C = U^T P U symmetrised and its Cholesky C = L L^T.  (The products
+// W1 = P U and C = U^T W1 are done by k_gemm_small before this.)
+__global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
+  const int k = S.k;
+  int status = coupling_prelude(S);
   if (!status) {
     if (k <= kLdsChol) {
       __shared__ double Ls[kLdsChol * kLdsChol];
@@ -1274,7 +1350,32 @@ __global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
   if (threadIdx.x == 0) S.out[0] = (double)status;
 }
 
-// W = V L (a x k row-major, FP64 and FP32): V C V^T = W W^T
+// W = V L (a x k row-major, FP64 and FP32): V C V^T = W W^T.  Each entry
+// W[j][l] = sum over m = l .. k-1, in increasing order, of V[j][m] L[m][l].
+// k <= 64: a workgroup takes 64 rows of V (column-major, loaded coalesced)
+// and L into LDS and writes its 64 x k block of W row-major (coalesced);
+// larger ranks: one thread per entry.
+constexpr int kMakeWRows = 64, kMakeWMaxK = 64;
+__global__ __launch_bounds__(256) void k_make_W_lds(const double* V, const double* Lm, int a, int k, double* W64,
+                                                    float* W32) {
+  __shared__ double s_V[kMakeWMaxK][kMakeWRows + 1];
+  __shared__ double s_L[kMakeWMaxK * kMakeWMaxK];
+  const int j0 = blockIdx.x * kMakeWRows, rows = min(kMakeWRows, a - j0);
+  for (int e = threadIdx.x; e < k * kMakeWRows; e += blockDim.x) {
+    const int m = e / kMakeWRows, jj = e % kMakeWRows;
+    s_V[m][jj] = jj < rows ? V[j0 + jj + (size_t)a * m] : 0.0;
+  }
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) s_L[e] = Lm[e];
+  __syncthreads();
+  for (int o = threadIdx.x; o < rows * k; o += blockDim.x) {
+    const int jj = o / k, l = o % k;
+    double s = 0.0;
+#pragma unroll 4
+    for (int m = l; m < k; ++m) s += s_V[m][jj] * s_L[m * k + l];
+    W64[(size_t)j0 * k + o] = s;
+    W32[(size_t)j0 * k + o] = (float)s;
+  }
+}
 __global__ void k_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)a * k) return;
@@ -1316,61 +1417,34 @@ __device__ __forceinline__ void wave_matvec(const double* __restrict__ M, int ld
   }
 }
 
-// y = L^-1 (U^T P b_q)  (reduced right-hand side, core.py:163-164;
-// linear_hsc.cu:351-357 with V c = W L^-1 c).  One workgroup: b_q, P b_q and
-// U^T P b_q in LDS (matrix-vector products one wave per output entry); the
-// forward substitution is column-oriented in one wave (lane j owns rows j,
-// j+64, ...; after y_i is final every later row subtracts L[j][i] y_i), with
-// L's lower triangle in LDS when it fits (k <= 64).
-__global__ __launch_bounds__(512) void k_rhs_small(const double* P, const double* U, const double* Lm,
-                                                   const double* b, int qo, int qm, int k, double* tmp, double* y) {
+// c = U^T P b_q = (P U)^T b_q  (the coupling part of the reduced
+// right-hand side, core.py:163-164; linear_hsc.cu:351-357), with (P U)^T
+// kept from setup (W1T, k x qm row-major).  One workgroup: b_q in LDS, one
+// wave per output entry.
+__global__ __launch_bounds__(512) void k_rhs_small(const double* W1T, const double* b, int qo, int qm, int k,
+                                                   double* y) {
   constexpr int kMaxQm = 1536;  // ABD DOFs staged in LDS (128 bodies)
   __shared__ double s_b[kMaxQm];
-  __shared__ double s_pb[kMaxQm];
-  __shared__ double s_c[kMaxRankLds];
-  __shared__ double s_L[64 * 64];
   const bool lds_b = qm <= kMaxQm;
-  const double* bq = lds_b ? s_b : b + qo;
-  double* pb = lds_b ? s_pb : tmp;
   if (lds_b)
     for (int i = threadIdx.x; i < qm; i += blockDim.x) s_b[i] = b[qo + i];
-  const bool lds_L = k <= 64;
-  if (lds_L)
-    for (int e = threadIdx.x; e < k * k; e += blockDim.x) s_L[e] = Lm[e];
   __syncthreads();
-  wave_matvec<6>(P, qm, qm, qm, bq, pb);  // P b_q
-  __syncthreads();
-  wave_matvec<6>(U, qm, k, qm, pb, s_c);  // c = U^T P b_q (U column-major: column l at U + qm l)
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    const double* L = lds_L ? s_L : Lm;
-    for (int i = 0; i < k; ++i) {
-      double yi = 0.0;
-      if (lane == (i & 63)) {
-        yi = s_c[i] / L[(size_t)i * k + i];
-        s_c[i] = yi;
-      }
-      yi = __shfl(yi, i & 63, 64);
-      for (int j = i + 1 + ((lane - (i + 1)) & 63); j < k; j += 64) s_c[j] -= L[(size_t)j * k + i] * yi;
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < k; i += blockDim.x) y[i] = s_c[i];
+  wave_matvec<4>(W1T, qm, k, qm, lds_b ? s_b : b + qo, y);
 }
 
-// rhs = b_x - W y ; r = rhs ; x_fem = 0
-__global__ void k_rhs_vec(const double* b, int fo, int n, const int* act_node, const double* W64, int k,
-                          const double* y, double* rhs, double* r, double* x) {
+// rhs = b_x - V c (V: a x k column-major, row j = the j-th coupled FEM DOF);
+// r = rhs; x_fem = 0
+__global__ void k_rhs_vec(const double* b, int fo, int n, const int* act_node, const double* V, int a, int k,
+                          const double* c, double* rhs, double* r, double* x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double v = b[fo + i];
   const int an = act_node[i / 3];
   if (an >= 0 && k > 0) {
-    const double* w = W64 + (size_t)(an + i % 3) * k;
+    const double* vr = V + an + i % 3;
     double g = 0.0;
-    for (int l = 0; l < k; ++l) g += w[l] * y[l];
+#pragma unroll 8
+    for (int l = 0; l < k; ++l) g += vr[(size_t)a * l] * c[l];
     v -= g;
   }
   rhs[i] = v;
@@ -1487,8 +1561,8 @@ hipError_t launch_split(const SplitParams& S, hipStream_t st) {
   hipLaunchKernelGGL(k_classify, dim3(cdiv(S.count, 256)), dim3(256), 0, st, S);
   return hipGetLastError();
 }
-hipError_t launch_scan(const int* in, int* out, int n, hipStream_t st) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, in, out, n);
+hipError_t launch_scan2(const int* in0, int* out0, const int* in1, int* out1, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan2, dim3(2), dim3(1024), 0, st, in0, out0, in1, out1, n);
   return hipGetLastError();
 }
 hipError_t launch_active(const int* flag, const int* prefix, int nb, int* act_node, int* act_dof, hipStream_t st) {
@@ -1515,12 +1589,26 @@ hipError_t launch_sort_gather(const int* rowptr, int nb, int* key_col, int* key_
                      key_src, vals, nnzb, colidx, v64, v32);
   return hipGetLastError();
 }
-hipError_t aca_occupancy(int* blocks) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_aca, kBlock, 0);
+// occupancy of a kernel on the current device, queried once per process
+// (the runtime's query costs tens of microseconds of host time)
+static hipError_t cached_occupancy(int which, const void* fn, int* blocks) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, int> cache;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find({dev, which});
+  if (it != cache.end()) {
+    *blocks = it->second;
+    return hipSuccess;
+  }
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, fn, kBlock, 0);
+  if (e == hipSuccess) cache[{dev, which}] = *blocks;
+  return e;
 }
-hipError_t aca_reg_occupancy(int* blocks) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_aca_reg, kBlock, 0);
-}
+hipError_t aca_occupancy(int* blocks) { return cached_occupancy(0, (const void*)k_aca, blocks); }
+hipError_t aca_reg_occupancy(int* blocks) { return cached_occupancy(1, (const void*)k_aca_reg, blocks); }
 // Launched as an ordinary kernel: the caller keeps the grid within the
 // co-resident bound (one workgroup per CU at most), so every workgroup gets a
 // CU once whatever else runs has drained, and every grid wait is bounded.  A
@@ -1563,24 +1651,27 @@ hipError_t launch_coupling_core(const FactorParams& S, hipStream_t st) {
   const int n = S.qm, k = S.k;
   // W1 = P U  (P row-major n x n; U column-major n x k: U[m + n*l])
   hipLaunchKernelGGL(k_gemm_small, dim3((k + 15) / 16, (n + 15) / 16), dim3(256), 0, st, S.P, n, 0, S.U, 1, n,
-                     S.W1, k, n, k, n);
+                     S.W1, k, n, k, n, S.W1T);
   // C = U^T W1 (column-major U read as row-major U^T with lda = n; W1 row-major n x k)
   hipLaunchKernelGGL(k_gemm_small, dim3((k + 15) / 16, (k + 15) / 16), dim3(256), 0, st, S.U, n, 0, S.W1, k, 1,
-                     S.C, k, k, k, n);
+                     S.C, k, k, k, n, nullptr);
+  // rank <= 64: one wave factors C in registers; else a workgroup in LDS
   hipLaunchKernelGGL(k_coupling_core, dim3(1), dim3(1024), 0, st, S);
   return hipGetLastError();
 }
 hipError_t launch_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32, hipStream_t st) {
   if ((long long)a * k == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_make_W, dim3(cdiv((long long)a * k, 256)), dim3(256), 0, st, V, Lm, a, k, W64, W32);
+  if (k <= kMakeWMaxK)
+    hipLaunchKernelGGL(k_make_W_lds, dim3(cdiv(a, kMakeWRows)), dim3(256), 0, st, V, Lm, a, k, W64, W32);
+  else
+    hipLaunchKernelGGL(k_make_W, dim3(cdiv((long long)a * k, 256)), dim3(256), 0, st, V, Lm, a, k, W64, W32);
   return hipGetLastError();
 }
-hipError_t launch_rhs(const double* P, const double* U, const double* Lm, const double* b, int qo, int qm, int fo,
-                      int n, int k, const int* act_node, const double* W64, double* tmp, double* y, double* rhs,
-                      double* r, double* x, hipStream_t st) {
-  if (k > 0) hipLaunchKernelGGL(k_rhs_small, dim3(1), dim3(512), 0, st, P, U, Lm, b, qo, qm, k, tmp, y);
+hipError_t launch_rhs(const double* W1T, const double* V, const double* b, int qo, int qm, int fo, int n, int k,
+                      int a, const int* act_node, double* y, double* rhs, double* r, double* x, hipStream_t st) {
+  if (k > 0) hipLaunchKernelGGL(k_rhs_small, dim3(1), dim3(512), 0, st, W1T, b, qo, qm, k, y);
   if (n > 0)
-    hipLaunchKernelGGL(k_rhs_vec, dim3(cdiv(n, 256)), dim3(256), 0, st, b, fo, n, act_node, W64, k, y, rhs, r, x);
+    hipLaunchKernelGGL(k_rhs_vec, dim3(cdiv(n, 256)), dim3(256), 0, st, b, fo, n, act_node, V, a, k, y, rhs, r, x);
   return hipGetLastError();
 }
 hipError_t launch_backsub(const double* P, const double* U, const double* V, const int* act_dof, const double* b,

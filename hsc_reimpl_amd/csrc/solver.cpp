@@ -1032,8 +1032,8 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                       h->row_count.as<int>(), h->active_flag.as<int>(), h->invD64.as<double>(),
                       h->invD32.as<float>(), h->A.as<double>(), err};
   HIPCHK(hsc::launch_split(sp, st));
-  HIPCHK(hsc::launch_scan(h->row_count.as<int>(), h->rowptr.as<int>(), nb, st));
-  HIPCHK(hsc::launch_scan(h->active_flag.as<int>(), h->act_prefix.as<int>(), nb, st));
+  HIPCHK(hsc::launch_scan2(h->row_count.as<int>(), h->rowptr.as<int>(), h->active_flag.as<int>(),
+                           h->act_prefix.as<int>(), nb, st));
   HIPCHK(hipMemcpyAsync(ss->herr, err, sizeof(ss->herr), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&ss->nnzb, h->rowptr.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&ss->nact, h->act_prefix.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1070,7 +1070,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->s1.ensure(sizeof(double) * qm * qm));
   HIPCHK(h->s2.ensure(sizeof(double) * qm * qm));
   HIPCHK(h->s3.ensure(sizeof(double) * qm * qm));
-  HIPCHK(h->W1.ensure(sizeof(double) * (size_t)qm * lim1));
+  HIPCHK(h->W1.ensure(sizeof(double) * 2 * (size_t)qm * lim1));  // P U, then (P U)^T
   HIPCHK(h->C.ensure(sizeof(double) * (size_t)lim1 * lim1));
   HIPCHK(h->Lm.ensure(sizeof(double) * (size_t)lim1 * lim1));
   HIPCHK(h->W64.ensure(sizeof(double) * (size_t)std::max(a, 1) * lim1));
@@ -1240,7 +1240,8 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   h->k = rank;
   if (rank > 0) {
     hsc::FactorParams fp{h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), qm, a, rank,
-                         h->W1.as<double>(), h->C.as<double>(), h->Lm.as<double>(), dmisc + 4};
+                         h->W1.as<double>(), h->W1.as<double>() + (size_t)qm * rank, h->C.as<double>(),
+                         h->Lm.as<double>(), dmisc + 4};
     HIPCHK(hsc::launch_coupling_core(fp, st));
     HIPCHK(hsc::launch_make_W(h->V.as<double>(), h->Lm.as<double>(), a, rank, h->W64.as<double>(),
                               h->W32.as<float>(), st));
@@ -1445,9 +1446,9 @@ int status_from_out(hsc_handle* h, const hsc::SysOut& o) {
 
 // Reduced right-hand side and initial residual of h's system (rhs = r).
 void enqueue_rhs(hsc_handle* h, const double* b, double* x, hipStream_t st) {
-  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
-                         h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
-                         h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
+  HIPCHK(hsc::launch_rhs(h->W1.as<double>() + (size_t)h->qm * h->k, h->V.as<double>(), b, h->qo, h->qm, h->fo,
+                         h->fn, h->k, h->a, h->act_node.as<int>(), h->ybuf.as<double>(), h->rhs.as<double>(),
+                         h->r.as<double>(), x, st));
 }
 
 int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStream_t st, Clock::time_point t0) {
@@ -1464,9 +1465,7 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   if (!h->launch_ready) prepare_launch(h, st);
   h->xsol = x + h->fo;
   const LaunchCfg lc{h->G, h->wgpcu, h->sched};
-  HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b, h->qo, h->qm, h->fo, h->fn,
-                         h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
-                         h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x, st));
+  enqueue_rhs(h, b, x, st);
   float pms = 0.f;
   std::vector<double> fin;
   auto out = run_pcg(hs, 1, lc, 0, st, &pms, [&] { enqueue_finish(h, b, x, st); }, &fin);
@@ -1659,9 +1658,7 @@ int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count, const doubl
     }
     for (int i = 0; i < count; ++i) {
       hsc_handle* h = handles[i];
-      HIPCHK(hsc::launch_rhs(h->P.as<double>(), h->U.as<double>(), h->Lm.as<double>(), b[i], h->qo, h->qm, h->fo,
-                             h->fn, h->k, h->act_node.as<int>(), h->W64.as<double>(), h->small.as<double>(),
-                             h->ybuf.as<double>(), h->rhs.as<double>(), h->r.as<double>(), x[i], st));
+      enqueue_rhs(h, b[i], x[i], st);
     }
     float pms = 0.f;
     std::vector<double> fin;
