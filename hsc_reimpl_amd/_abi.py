@@ -36,7 +36,7 @@ class Config(C.Structure):
         ("true_residual_every_iteration", C.c_int32), ("aca_rtol", C.c_double),
         ("lanes_per_row", C.c_int32), ("wg_per_cu", C.c_int32), ("preconditioner", C.c_int32),
         ("check_acceptance", C.c_int32), ("schedule", C.c_int32), ("poly_interval_ratio", C.c_double),
-        ("pass_sync", C.c_int32), ("reserved0", C.c_int32),
+        ("pass_sync", C.c_int32), ("pipelined", C.c_int32),
     ]
 
 
@@ -73,11 +73,11 @@ class SetupInfo(C.Structure):
         ("aca_relative_error", C.c_double), ("abd_inverse_residual", C.c_double),
         ("schedule", C.c_int32), ("pass_sync", C.c_int32), ("lambda_max", C.c_double),
         ("poly_ratio", C.c_double), ("learned_out", C.c_double * 3), ("learned_used", C.c_int32),
-        ("fallback_reason", C.c_int32),
+        ("fallback_reason", C.c_int32), ("pipelined", C.c_int32), ("reserved1", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        out = {name: getattr(self, name) for name, _ in self._fields_}
+        out = {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved1"}
         out["learned_out"] = list(self.learned_out)
         return out
 

@@ -10,6 +10,7 @@ namespace hsc {
 void note_alloc();
 
 constexpr int kBlock = 512;        // threads per workgroup (8 wave64)
+constexpr int kPipeRows = 128;     // rows (= nodes) of a resident tile in the pipelined PCG loop (G = 4)
 constexpr int kNodeTileMax = 512;  // nodes per elementwise tile (one node per thread; runtime 128 or 512)
 constexpr int kLrTile = 128;       // low-rank rows (W rows) per tile
 constexpr int kMaxSys = 128;       // systems per persistent launch
@@ -106,6 +107,7 @@ struct PcgParams {
   double* partE;          // [2 * nsys * nF]  (rr, rz from node tiles)
   double* partF;          // [nsys * nF]      (rz from Neumann stages)
   double* partL;          // [sum over systems (cL1-cL0+1)*k]
+  double* partR;          // pipelined loop: W^T v of each row workgroup's coupled nodes [nF * t_total]
   unsigned* barrier;      // grid barrier counter (zeroed before launch)
   int* error;             // device error flag
   long long timeout_ticks;// bounded wait (s_memrealtime ticks, 100 MHz)

@@ -95,10 +95,11 @@ hipError_t launch_full_residual(const double* yf, const double* E, const double*
 // res: the latency-schedule kernel (register-resident rows and node tile);
 // otherwise the throughput-schedule kernel (streaming SpMV with several
 // blocks per lane in flight)
-hipError_t launch_pcg(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st);
-hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks);
-hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st);
-const void* pcg_kernel_fn(int G, bool mixed, bool res);
+// pipe: the pipelined loop (one grid barrier per iteration; G = 4, latency schedule only)
+hipError_t launch_pcg(int G, bool mixed, bool res, bool pipe, const PcgParams& P, size_t shmem, hipStream_t st);
+hipError_t pcg_occupancy(int G, bool mixed, bool res, bool pipe, size_t shmem, int* blocks);
+hipError_t launch_pcg_plain(int G, bool mixed, bool res, bool pipe, const PcgParams& P, size_t shmem, hipStream_t st);
+const void* pcg_kernel_fn(int G, bool mixed, bool res, bool pipe);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);  // bytes % 4 == 0; a kernel, safe in graphs
 // per-tet log-Neo-Hookean Hessian (SPD-projected when project != 0) pulled
 // back to 10 node-block triplets per tet (fem_hessian.hip)

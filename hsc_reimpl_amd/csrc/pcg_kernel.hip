@@ -624,6 +624,48 @@ __device__ __forceinline__ void reduce_chunks_all(const double* __restrict__ par
   if (act && l == 0) out[s] = v;
 }
 
+// Four reductions of reduce_chunks_all's kind over one chunk range per
+// system (arrays a, b with nA chunks per system; f, e with nF), their loads
+// in flight together; each sum is added in reduce_chunks_all's order.
+template <typename Rng>
+__device__ __forceinline__ void reduce4_chunks(const double* __restrict__ f, int nF, const double* __restrict__ a,
+                                               const double* __restrict__ b2, int nA, const double* __restrict__ e,
+                                               int nsys, int T, Rng rng, double* of, double* oa, double* ob,
+                                               double* oe) {
+  int L = 64;
+  while (L > 1 && L * nsys > kBlock) L >>= 1;
+  const int tid = threadIdx.x;
+  const int s = tid / L, l = tid % L;
+  double vf = 0.0, va = 0.0, vb = 0.0, ve = 0.0;
+  const bool act = s < nsys;
+  if (act) {
+    int c0, c1;
+    rng(s, c0, c1);
+#pragma unroll 2
+    for (int c = c0 + l; c <= c1; c += L) {
+      const double xf = f[(size_t)s * nF + c], xa = a[(size_t)s * nA + c], xb = b2[(size_t)s * nA + c];
+      const double xe = e[(size_t)s * nF + c];
+      const bool emp = chunk_empty(c, nF, T);
+      vf += emp ? 0.0 : xf;
+      va += emp ? 0.0 : xa;
+      vb += emp ? 0.0 : xb;
+      ve += emp ? 0.0 : xe;
+    }
+  }
+  for (int m = L >> 1; m >= 1; m >>= 1) {
+    vf += __shfl_xor(vf, m, 64);
+    va += __shfl_xor(va, m, 64);
+    vb += __shfl_xor(vb, m, 64);
+    ve += __shfl_xor(ve, m, 64);
+  }
+  if (act && l == 0) {
+    of[s] = vf;
+    oa[s] = va;
+    ob[s] = vb;
+    oe[s] = ve;
+  }
+}
+
 // |t_s|^2 for every system at once (same grouping as reduce_chunks_all)
 template <typename Want>
 __device__ __forceinline__ void sum_tt_all(const SysDesc* sys, int nsys, const double* t, Want want, double* out) {
@@ -641,7 +683,7 @@ __device__ __forceinline__ void sum_tt_all(const SysDesc* sys, int nsys, const d
 }
 
 // ---------------------------------------------------------------- kernel
-template <int G, typename VT, bool RES>
+template <int G, typename VT, bool RES, bool PIPE = false>
 __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   constexpr int SU = StreamU<G, RES>::value;
   extern __shared__ double s_t[];  // t vectors (sum of ranks)
@@ -795,7 +837,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     pre_ok = true;
   };
 
-  auto node_stage = [&](int kind) {
+  auto node_stage = [&](int kind, const double* tx) {
     double rr = 0.0, rz = 0.0;
     for_tiles(sys, nsys, trE, 1,
               [&](int s, int tl) {
@@ -813,7 +855,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                     lowrank_apply(S.k, an, w_vals<VT>(S), s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt,
                                   resE ? s_wc : (const VT*)nullptr);
                   else
-                    lowrank_apply(S.k, an, S.W64, s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
+                    lowrank_apply(S.k, an, S.W64, tx + S.t_off, g, s_list, s_slot, s_g, &s_cnt);
                 }
                 if (!valid) return;
                 const int i3 = 3 * i;
@@ -1019,7 +1061,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // workgroup (sum k <= kBlock) by ng groups of threads (every ng-th chunk,
   // groups added in order), else one thread per entry.  The stage-A loads are
   // issued before the first wait, so all latencies overlap.
-  auto reduce_t = [&](auto want, bool with_pA) {
+  auto reduce_t = [&](auto want, bool with_pA, double* tdst) {
     ChunkPre preA;
     if (with_pA) chunks_prefetch(P.partA, nsys, P.nA, rngA, all_sys, preA);
     const int T = P.t_total;
@@ -1047,7 +1089,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       const int q = tid % T, grp = tid / T;
       if (grp < ng) s_tred[grp * T + q] = sum_entry(q, grp, ng);
     } else {
-      for (int q = tid; q < T; q += kBlock) s_t[q] = sum_entry(q, 0, 1);
+      for (int q = tid; q < T; q += kBlock) tdst[q] = sum_entry(q, 0, 1);
     }
     if (with_pA) chunks_finish(P.partA, P.nA, P.rt_total, preA, s_v1);
     __syncthreads();
@@ -1055,11 +1097,11 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       if (tid < T) {
         double acc = 0.0;
         for (int g = 0; g < ng; ++g) acc += s_tred[g * T + tid];
-        s_t[tid] = acc;
+        tdst[tid] = acc;
       }
       __syncthreads();
     }
-    sum_tt_all(sys, nsys, s_t, want, s_tt);
+    sum_tt_all(sys, nsys, tdst, want, s_tt);
     __syncthreads();
   };
 
@@ -1095,7 +1137,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   };
 
   // ================================================================ init
-  node_stage(0);
+  node_stage(0, s_t);
   if (!sync_at(0)) goto out;
   reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE, all_sys, s_v1);
   __syncthreads();
@@ -1108,6 +1150,368 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     }
   }
   __syncthreads();
+  if constexpr (PIPE) {
+    // ============================================================ pipelined loop
+    // One grid barrier per iteration.  The passes end with a neighbour-tile
+    // wait, so D z (the D part of S z) is formed by each row workgroup for
+    // its own rows without a grid barrier; with q_old = D p_old kept in FP64,
+    //   q = D p = D z + beta q_old,
+    //   p.D p = z.D z + 2 beta z.q_old + beta^2 p_old.D p_old,
+    //   t = W^T p = W^T z + beta W^T p_old,
+    // so the dots and W^T z of one iteration are reduced at one barrier
+    // together with r.z (the last pass) and |r|^2 (the previous update).
+    // The residual check is on the r the preconditioner just used, so
+    // convergence is seen after one more preconditioner application than in
+    // the standard loop, with the same iterate.  A true-residual check
+    // (convergence or replacement) that keeps a system running recomputes
+    // its z from the new r before its next update.
+    __shared__ double s_dz[kPipeRows * 3];  // D z of the resident rows (row layout -> node layout)
+    __shared__ double s_zdz[kMaxSys], s_zq[kMaxSys], s_rr[kMaxSys], s_pdp[kMaxSys];
+    __shared__ int s_first[kMaxSys], s_redo[kMaxSys], s_skip[kMaxSys];
+    double* s_tz = s_t + P.t_total;      // W^T z
+    double* s_tx = s_t + 2 * P.t_total;  // W^T x (true residual)
+    double* partA2 = P.partA + (size_t)nsys * P.nA;
+    const int lrow = tid / G;
+    for (int s = tid; s < nsys; s += kBlock) {
+      s_first[s] = 1;
+      s_redo[s] = 0;
+      s_skip[s] = 0;
+      s_pdp[s] = 0.0;
+      s_rz[s] = 0.0;
+    }
+    __syncthreads();
+    auto pipe_wait = [&]() -> bool { return P.pass_sync ? pass_wait() : sync_at(1); };
+    // W^T v over the coupled nodes of this row workgroup's node tile (v = the
+    // last pass's z, or x for a true residual), written to partR for the
+    // reduction at the next barrier.  Each wave sums its consecutive nodes in
+    // order and the waves' sums are added in order, so the result does not
+    // depend on scheduling.  (The low-rank workgroups idle in this loop.)
+    __shared__ double s_vc[kPipeRows * 3];
+    __shared__ int s_an[kPipeRows];
+    if (tid < kPipeRows) s_an[tid] = (resE && tid < P.node_tile) ? anE : -1;
+    __syncthreads();
+    auto rows_Wt = [&](bool use_x) {
+      if (!resF) return;
+      const SysDesc& S = sys[sF];
+      const int k = S.k;
+      if (k == 0 || s_st[sF] != ST_RUN || (use_x && s_chk[sF] == CHK_NONE)) return;
+      if (tid < kPipeRows) {
+        const int i = iE;
+        const bool cpl = s_an[tid] >= 0 && i < S.nb;
+        for (int d = 0; d < 3; ++d)
+          s_vc[3 * tid + d] = !cpl ? 0.0 : (use_x ? S.x[3 * i + d] : (double)((const VT*)S.z[zc])[3 * i + d]);
+      }
+      __syncthreads();
+      constexpr int NW = kBlock / 64;
+      constexpr int per = kPipeRows / NW;
+      const int wv = tid >> 6, lane = tid & 63;
+      double* dst = P.partR + (size_t)S.t_off * P.nF + (size_t)cF * k;
+      for (int l0 = 0; l0 < k; l0 += 64) {
+        const int l = l0 + lane;
+        double acc = 0.0;
+        if (l < k) {
+          for (int u = 0; u < per; ++u) {
+            const int node = wv * per + u;
+            const int an = s_an[node];
+            if (an < 0) continue;
+            double w0, w1, w2;
+            if (use_x) {
+              w0 = S.W64[(size_t)an * k + l];
+              w1 = S.W64[(size_t)(an + 1) * k + l];
+              w2 = S.W64[(size_t)(an + 2) * k + l];
+            } else if (s_wc) {
+              const VT* wr = s_wc + (size_t)s_slot[node] * 3 * k;
+              w0 = (double)wr[l];
+              w1 = (double)wr[k + l];
+              w2 = (double)wr[2 * k + l];
+            } else {
+              const VT* W = w_vals<VT>(S);
+              w0 = (double)W[(size_t)an * k + l];
+              w1 = (double)W[(size_t)(an + 1) * k + l];
+              w2 = (double)W[(size_t)(an + 2) * k + l];
+            }
+            acc += w0 * s_vc[3 * node] + w1 * s_vc[3 * node + 1] + w2 * s_vc[3 * node + 2];
+          }
+        }
+        s_lr[wv][lane] = acc;
+        __syncthreads();
+        if (tid < 64 && l < k) {
+          double t = 0.0;
+          for (int w = 0; w < NW; ++w) t += s_lr[w][tid];
+          dst[l] = t;
+        }
+        __syncthreads();
+      }
+    };
+    // t entries of the systems selected by `want` from the row workgroups'
+    // partials: entry q of system s sums its chunks cF0..cF1 in order (groups
+    // of threads take every ng-th chunk; the groups are added in order)
+    auto reduce_rows_t = [&](auto want, double* tdst) {
+      const int T = P.t_total;
+      const int ng = T > 0 && T <= kBlock ? kBlock / T : 1;
+      auto sum_entry = [&](int q, int g0, int gs) {
+        int s = 0, hi = nsys - 1;
+        while (s < hi) {
+          const int mid = (s + hi + 1) >> 1;
+          if (sys[mid].t_off <= q) s = mid;
+          else hi = mid - 1;
+        }
+        const SysDesc& S = sys[s];
+        double v = 0.0;
+        if (want(s) && S.k > 0) {
+          const int l = q - S.t_off;
+          const double* src = P.partR + (size_t)S.t_off * P.nF;
+#pragma unroll 8
+          for (int c = S.cF0 + g0; c <= S.cF1; c += gs) v += src[(size_t)c * S.k + l];
+        }
+        return v;
+      };
+      if (T > 0 && T <= kBlock) {
+        const int q = tid % T, grp = tid / T;
+        if (grp < ng) s_tred[grp * T + q] = sum_entry(q, grp, ng);
+      } else {
+        for (int q = tid; q < T; q += kBlock) tdst[q] = sum_entry(q, 0, 1);
+      }
+      __syncthreads();
+      if (T > 0 && T <= kBlock) {
+        if (tid < T) {
+          double acc = 0.0;
+          for (int g = 0; g < ng; ++g) acc += s_tred[g * T + tid];
+          tdst[tid] = acc;
+        }
+      }
+      __syncthreads();
+    };
+    for (;;) {
+      if (tid == 0) {
+        int any = 0;
+        for (int s = 0; s < nsys; ++s) any |= (s_st[s] == ST_RUN);
+        s_any = any;
+      }
+      __syncthreads();
+      if (!s_any) break;
+
+      // ---------------- passes 1 .. order-1, each after the neighbours' previous one
+      if (!pipe_wait()) goto out;
+      for (int j = 1; j < order; ++j) {
+        const bool last = j == order - 1;
+        double rz = 0.0;
+        if (resF) {
+          const SysDesc& S = sys[sF];
+          if (s_st[sF] == ST_RUN) {
+            const VT* zi = (const VT*)S.z[zc];
+            VT* zn = (VT*)S.z[zc ^ 1];
+            if (P.pass_sync)
+              tile_N_res<G, VT, true>(S, rowF, rrF, zi, zn, step_w(j), last, rz);
+            else
+              tile_N_res<G, VT>(S, rowF, rrF, zi, zn, step_w(j), last, rz);
+          }
+          if (last) {
+            const double c = block_sum_t0(rz, s_red);
+            if (s_st[sF] == ST_RUN && tid == 0) P.partF[(size_t)sF * P.nF + cF] = c;
+          }
+        }
+        zc ^= 1;
+        if (!pipe_wait()) goto out;
+      }
+
+      // ---------------- D z and W^T z on the resident rows
+      if (resF) {
+        const SysDesc& S = sys[sF];
+        double zdz = 0.0, zq = 0.0;
+        if (s_st[sF] == ST_RUN) {
+          const auto zg = gsel<VT>((const VT*)S.z[zc]);
+          const int lane = tid % G;
+          VT a0, a1, a2;
+          row_spmv_res<G, VT>(S, mat_vals<VT>(S), rrF, lane,
+                              [&](int c, VT& x0, VT& x1, VT& x2) {
+                                x0 = zg[3 * c];
+                                x1 = zg[3 * c + 1];
+                                x2 = zg[3 * c + 2];
+                              },
+                              a0, a1, a2);
+          if (rowF < S.nb && lane < 3) {
+            const int idx = 3 * rowF + lane;
+            const VT dz = pick3(lane, a0, a1, a2);
+            const double zo = (double)zg[idx];
+            zdz = zo * (double)dz;
+            if (!s_first[sF]) zq = zo * ((const double*)S.q)[idx];
+            s_dz[3 * lrow + lane] = (double)dz;
+          }
+        }
+        const double d0 = block_sum_t0(zdz, s_red);
+        const double d1 = block_sum_t0(zq, s_red);
+        if (s_st[sF] == ST_RUN && tid == 0) {
+          P.partA[(size_t)sF * P.nA + cA] = d0;
+          partA2[(size_t)sF * P.nA + cA] = d1;
+        }
+        rows_Wt(false);
+      }
+      if (!sync_at(2)) goto out;
+
+      // ---------------- reductions (every workgroup, fixed order)
+      // r.z, z.D z, z.q_old and |r|^2 (the aligned split chunks rows and
+      // nodes alike: one chunk range per system), then W^T z
+      reduce4_chunks(P.partF, P.nF, P.partA, partA2, P.nA, partE_rr, nsys, P.rt_total, rngF, s_v0, s_zdz, s_zq,
+                     s_rr);
+      reduce_rows_t([&](int s) { return s_st[s] == ST_RUN; }, s_tz);
+      if (tid == 0) {
+        int any = 0;
+        for (int s = 0; s < nsys; ++s) {
+          s_chk[s] = CHK_NONE;
+          s_redo[s] = 0;
+          if (s_st[s] != ST_RUN || s_skip[s]) continue;
+          const SysDesc& S = sys[s];
+          const double rel = sqrt(s_rr[s]) / s_normb[s];
+          s_rel[s] = rel;
+          int ck = CHK_NONE;
+          if (rel <= S.tol) ck = CHK_CONVERGE;
+          else if (P.replace_every > 0 && s_it[s] > 0 && (s_it[s] % P.replace_every) == 0) ck = CHK_REPLACE;
+          s_chk[s] = ck;
+          any |= (ck != CHK_NONE);
+        }
+        s_any = any;
+      }
+      __syncthreads();
+
+      // ---------------- true residual of the checked systems (T1, T2)
+      if (s_any) {
+        if (resF) {
+          const SysDesc& S = sys[sF];
+          if (s_st[sF] == ST_RUN && s_chk[sF] != CHK_NONE) tile_Dx64<G, SU>(S, trF.t0 - S.rt_begin, S.x, S.q64);
+          rows_Wt(true);
+        }
+        if (!sync_at(4)) goto out;
+        reduce_rows_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, s_tx);
+        node_stage(2, s_tx);
+        if (!sync_at(5)) goto out;
+        reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE,
+                          [&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, s_v1);
+        __syncthreads();
+        if (tid == 0) {
+          for (int s = 0; s < nsys; ++s) {
+            if (s_st[s] != ST_RUN || s_chk[s] == CHK_NONE) continue;
+            const SysDesc& S = sys[s];
+            const double rel = sqrt(s_v1[s]) / s_normb[s];
+            s_nchk[s] += 1;
+            s_rel[s] = rel;
+            if (rel <= S.tol) {
+              s_st[s] = ST_DONE;
+            } else {
+              // r replaced by the true residual and z0 recomputed from it:
+              // the passes run again before the next update (after a failed
+              // convergence check the iteration restarts with beta = 0)
+              if (s_chk[s] == CHK_CONVERGE) {
+                s_first[s] = 1;
+                s_nrs[s] += 1;
+              }
+              s_redo[s] = 1;
+              s_skip[s] = 1;
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (tid == 0) {
+        for (int s = 0; s < nsys; ++s)
+          if (s_st[s] == ST_RUN && !s_redo[s] && s_it[s] >= sys[s].max_iter) s_st[s] = kNotConverged;
+      }
+      __syncthreads();
+
+      // ---------------- beta, t = W^T p, alpha
+      for (int s = tid; s < nsys; s += kBlock)
+        if (s_st[s] == ST_RUN && !s_redo[s]) s_beta[s] = s_first[s] ? 0.0 : s_v0[s] / s_rz[s];
+      __syncthreads();
+      for (int s = 0; s < nsys; ++s) {
+        if (s_st[s] != ST_RUN || s_redo[s]) continue;
+        const int off = sys[s].t_off, k = sys[s].k;
+        const double be = s_beta[s];
+        const bool first = s_first[s];
+        for (int l = tid; l < k; l += kBlock) s_t[off + l] = first ? s_tz[off + l] : s_tz[off + l] + be * s_t[off + l];
+      }
+      __syncthreads();
+      sum_tt_all(sys, nsys, s_t, [&](int s) { return s_st[s] == ST_RUN && !s_redo[s]; }, s_tt);
+      __syncthreads();
+      for (int s = tid; s < nsys; s += kBlock) {
+        if (s_st[s] != ST_RUN || s_redo[s]) continue;
+        const double be = s_beta[s], rz = s_v0[s];
+        const double pdp = s_first[s] ? s_zdz[s] : s_zdz[s] + be * (2.0 * s_zq[s] + be * s_pdp[s]);
+        const double pSp = pdp - s_tt[s];
+        if (!(isfinite(pSp) && pSp > 0.0 && isfinite(rz) && rz > 0.0)) {
+          s_st[s] = kBreakdown;
+        } else {
+          s_alpha[s] = rz / pSp;
+          s_rz[s] = rz;
+          s_pdp[s] = pdp;
+        }
+      }
+      __syncthreads();
+
+      // ---------------- update on the resident node tile; z0 = w Dinv r
+      {
+        double rr = 0.0;
+        for_tiles(sys, nsys, trE, 1,
+                  [&](int s, int tl) {
+                    const SysDesc& S = sys[s];
+                    if (s_st[s] != ST_RUN || s_redo[s]) return;
+                    const int i = tl * P.node_tile + tid;
+                    const bool valid = tid < P.node_tile && i < S.nb;
+                    double g[3] = {0.0, 0.0, 0.0};
+                    if (S.k > 0)
+                      lowrank_apply(S.k, anE, w_vals<VT>(S), s_t + S.t_off, g, s_list, s_slot, s_g, &s_cnt, s_wc);
+                    if (!valid) return;
+                    const int i3 = 3 * i;
+                    VT* z = (VT*)S.z[zc];
+                    VT* pv = (VT*)S.p[0];
+                    double* qv = (double*)S.q;
+                    const double al = s_alpha[s], bed = s_beta[s];
+                    const VT be = (VT)bed;
+                    const bool first = s_first[s];
+                    double rv[3];
+                    for (int d = 0; d < 3; ++d) {
+                      const VT zd = z[i3 + d];
+                      const VT pn = first ? zd : combine(zd, pv[i3 + d], be);
+                      const double dz = s_dz[3 * tid + d];
+                      const double qn = first ? dz : fma(bed, qv[i3 + d], dz);
+                      pv[i3 + d] = pn;
+                      qv[i3 + d] = qn;
+                      rv[d] = S.r[i3 + d] - al * (qn - g[d]);
+                      S.x[i3 + d] += al * (double)pn;
+                    }
+                    for (int d = 0; d < 3; ++d) S.r[i3 + d] = rv[d];
+                    rr += rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+                    const VT* ig = inv_vals<VT>(S) + 9 * i;
+                    const VT r0 = (VT)rv[0], r1 = (VT)rv[1], r2 = (VT)rv[2];
+                    const VT z0 = omega * (ig[0] * r0 + ig[1] * r1 + ig[2] * r2);
+                    const VT z1 = omega * (ig[3] * r0 + ig[4] * r1 + ig[5] * r2);
+                    const VT z2 = omega * (ig[6] * r0 + ig[7] * r1 + ig[8] * r2);
+                    if (P.pass_sync) {
+                      zst<true>(z + i3, z0);
+                      zst<true>(z + i3 + 1, z1);
+                      zst<true>(z + i3 + 2, z2);
+                    } else {
+                      z[i3] = z0;
+                      z[i3 + 1] = z1;
+                      z[i3 + 2] = z2;
+                    }
+                  },
+                  [&](int s) {
+                    const double a = block_sum_t0(rr, s_red);
+                    if (s_st[s] == ST_RUN && !s_redo[s] && tid == 0) partE_rr[(size_t)s * P.nF + cF] = a;
+                    rr = 0.0;
+                  });
+      }
+      if (tid == 0) {
+        for (int s = 0; s < nsys; ++s) {
+          if (s_st[s] != ST_RUN || s_redo[s]) continue;
+          s_it[s] += 1;
+          s_first[s] = 0;
+          s_skip[s] = 0;
+        }
+      }
+      __syncthreads();
+    }
+  } else {
   if (!neumann_steps()) goto out;
   reduce_rz();
   for (int s = tid; s < nsys; s += kBlock) {
@@ -1155,7 +1559,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     pc ^= 1;
     if (!sync_at(2)) goto out;
     prefetch_B();
-    reduce_t([&](int s) { return s_st[s] == ST_RUN; }, true);
+    reduce_t([&](int s) { return s_st[s] == ST_RUN; }, true, s_t);
     for (int s = tid; s < nsys; s += kBlock) {
       if (s_st[s] != ST_RUN) continue;
       const double pAp = s_v1[s] - s_tt[s];
@@ -1170,7 +1574,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
 
     // ---------------- stage B
     if (P.prof && tid == 0) P.prof[(size_t)b * 32 + 12] += rt_ticks() - t_last;  // B: reductions part
-    node_stage(1);
+    node_stage(1, s_t);
     pre_ok = false;
     if (!sync_at(3)) goto out;
     reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE, all_sys, s_v1);
@@ -1209,8 +1613,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   [&](int) {});
       }
       if (!sync_at(4)) goto out;
-      reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, false);
-      node_stage(2);
+      reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, false, s_t);
+      node_stage(2, s_t);
       if (!sync_at(5)) goto out;
       reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE,
                         [&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, s_v1);
@@ -1254,6 +1658,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     }
     __syncthreads();
   }
+  }  // standard loop
 
   }  // started
 out:
@@ -1409,10 +1814,10 @@ hipError_t launch_node_graph(const int* rowptr, const int* colidx, const double*
 }
 
 // explicit instantiations via host launchers
-template <int G, typename VT, bool RES>
+template <int G, typename VT, bool RES, bool PIPE = false>
 static hipError_t launch_pcg_t(const PcgParams& P, size_t shmem, hipStream_t st) {
   void* args[] = {(void*)&P};
-  return hipLaunchCooperativeKernel((const void*)pcg_persistent<G, VT, RES>, dim3(P.nwg), dim3(kBlock), args,
+  return hipLaunchCooperativeKernel((const void*)pcg_persistent<G, VT, RES, PIPE>, dim3(P.nwg), dim3(kBlock), args,
                                     shmem, st);
 }
 
@@ -1420,15 +1825,15 @@ static hipError_t launch_pcg_t(const PcgParams& P, size_t shmem, hipStream_t st)
 // whose kernel node is then marked cooperative); the grid never exceeds the
 // co-resident capacity computed from the occupancy, and every grid wait is
 // bounded
-template <int G, typename VT, bool RES>
+template <int G, typename VT, bool RES, bool PIPE = false>
 static hipError_t launch_pcg_plain_t(const PcgParams& P, size_t shmem, hipStream_t st) {
-  hipLaunchKernelGGL((pcg_persistent<G, VT, RES>), dim3(P.nwg), dim3(kBlock), shmem, st, P);
+  hipLaunchKernelGGL((pcg_persistent<G, VT, RES, PIPE>), dim3(P.nwg), dim3(kBlock), shmem, st, P);
   return hipGetLastError();
 }
 
-template <int G, typename VT, bool RES>
+template <int G, typename VT, bool RES, bool PIPE = false>
 static hipError_t occ_t(int* blocks, size_t shmem) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pcg_persistent<G, VT, RES>, kBlock, shmem);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, pcg_persistent<G, VT, RES, PIPE>, kBlock, shmem);
 }
 
 __global__ void k_zero_u32(unsigned* p, long long n) {
@@ -1477,7 +1882,12 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
 // dispatch over lanes per row (4/8/16), precision and schedule
 #define HSC_PCG_DISPATCH(FN, ...)                                                       \
   do {                                                                                  \
-    if (mixed) {                                                                        \
+    if (pipe) {                                                                         \
+      if (G == 4 && res) {                                                              \
+        if (mixed) return FN<4, float, true, true>(__VA_ARGS__);                        \
+        return FN<4, double, true, true>(__VA_ARGS__);                                  \
+      }                                                                                 \
+    } else if (mixed) {                                                                        \
       if (res) {                                                                        \
         if (G == 4) return FN<4, float, true>(__VA_ARGS__);                             \
         if (G == 8) return FN<8, float, true>(__VA_ARGS__);                             \
@@ -1500,17 +1910,18 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
     }                                                                                   \
   } while (0)
 
-hipError_t launch_pcg(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st) {
+hipError_t launch_pcg(int G, bool mixed, bool res, bool pipe, const PcgParams& P, size_t shmem, hipStream_t st) {
   HSC_PCG_DISPATCH(launch_pcg_t, P, shmem, st);
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_pcg_plain(int G, bool mixed, bool res, const PcgParams& P, size_t shmem, hipStream_t st) {
+hipError_t launch_pcg_plain(int G, bool mixed, bool res, bool pipe, const PcgParams& P, size_t shmem,
+                            hipStream_t st) {
   HSC_PCG_DISPATCH(launch_pcg_plain_t, P, shmem, st);
   return hipErrorInvalidValue;
 }
 
-static hipError_t pcg_occupancy_query(int G, bool mixed, bool res, size_t shmem, int* blocks) {
+static hipError_t pcg_occupancy_query(int G, bool mixed, bool res, bool pipe, size_t shmem, int* blocks) {
   HSC_PCG_DISPATCH(occ_t, blocks, shmem);
   return hipErrorInvalidValue;
 }
@@ -1518,13 +1929,13 @@ static hipError_t pcg_occupancy_query(int G, bool mixed, bool res, size_t shmem,
 // the runtime's occupancy query costs tens of microseconds of host time; its
 // answer depends only on the device, the kernel and the LDS size, so it is
 // kept per process
-hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks) {
+hipError_t pcg_occupancy(int G, bool mixed, bool res, bool pipe, size_t shmem, int* blocks) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int, int, size_t>, int> cache;
+  static std::map<std::tuple<int, int, int, int, int, size_t>, int> cache;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  const auto key = std::make_tuple(dev, G, (int)mixed, (int)res, shmem);
+  const auto key = std::make_tuple(dev, G, (int)mixed, (int)res, (int)pipe, shmem);
   {
     std::lock_guard<std::mutex> g(mu);
     auto it = cache.find(key);
@@ -1533,7 +1944,7 @@ hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks)
       return hipSuccess;
     }
   }
-  e = pcg_occupancy_query(G, mixed, res, shmem, blocks);
+  e = pcg_occupancy_query(G, mixed, res, pipe, shmem, blocks);
   if (e == hipSuccess) {
     std::lock_guard<std::mutex> g(mu);
     cache[key] = *blocks;
@@ -1541,12 +1952,12 @@ hipError_t pcg_occupancy(int G, bool mixed, bool res, size_t shmem, int* blocks)
   return e;
 }
 
-template <int G, typename VT, bool RES>
+template <int G, typename VT, bool RES, bool PIPE = false>
 static const void* kernel_fn_t() {
-  return (const void*)pcg_persistent<G, VT, RES>;
+  return (const void*)pcg_persistent<G, VT, RES, PIPE>;
 }
 
-const void* pcg_kernel_fn(int G, bool mixed, bool res) {
+const void* pcg_kernel_fn(int G, bool mixed, bool res, bool pipe) {
   HSC_PCG_DISPATCH(kernel_fn_t);
   return nullptr;
 }

@@ -92,7 +92,7 @@ struct HostBuf {
 // Scratch of one persistent PCG launch: system descriptors and outputs,
 // chunk tables, reduction partials, grid-barrier counters, error flag.
 struct PcgScratch {
-  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, barrier, errflag, flags;
+  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, partR, barrier, errflag, flags;
   int nflags = 0;  // row-tile pass flags zeroed before each launch (0: point-to-point pass sync off)
   HostBuf host;    // pinned copies of the outputs and the error flag
   // last uploaded descriptor and chunk tables (an identical plan is not uploaded again)
@@ -221,6 +221,7 @@ struct hsc_handle {
   bool launch_ready = false;        // G/wgpcu/sched chosen and the launch scratch sized for them
   int G = 8, wgpcu = 1, sched = 1;
   bool pass_sync_used = false;  // the last PCG launch synchronised its passes per tile
+  bool pipe_used = false;       // the last PCG launch ran the pipelined loop
   int max_coupled128 = 0;  // most coupled nodes in one 128-node tile
   // row-tile neighbour lists for 128, 64 and 32 rows per tile (lanes per row 4, 8, 16)
   Buf tnbr[3], tnbr_bits;
@@ -313,6 +314,10 @@ int validate_config(const hsc_config& c, std::string& why) {
     why = "pass_sync must be 0 or 1";
     return HSC_ERR_INVALID_ARGUMENT;
   }
+  if (c.pipelined < 0 || c.pipelined > 1) {
+    why = "pipelined must be 0 or 1";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
   if (c.schedule < 0 || c.schedule > 2) {
     why = "schedule must be 0, 1 or 2";
     return HSC_ERR_INVALID_ARGUMENT;
@@ -397,6 +402,7 @@ struct PcgPlan {
   int G = 8;
   bool mixed = false;
   bool res = true;  // latency-schedule kernel
+  bool pipe = false;  // the pipelined loop
   std::vector<hsc::SysDesc> S;
   std::vector<int> ch;
 };
@@ -454,9 +460,19 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
       wc_bytes = 0;
     }
   }
-  const size_t shmem = t_bytes + wc_bytes;
+  // the pipelined loop (requested by the config) runs when every row tile
+  // and its 128 nodes get their own workgroup: 4 lanes per row, latency
+  // split, polynomial passes, recursive-residual stopping
+  const bool pipe_req = h0->cfg.pipelined && G == 4 && lc.sched != 2 && node_tile == R && h0->cfg.neumann_order >= 2 &&
+                        !h0->cfg.true_residual_every_iteration && hsc::kPipeRows == R;
+  size_t shmem = t_bytes + wc_bytes;
   int occ = 0;
-  HIPCHK(hsc::pcg_occupancy(G, mixed, lc.sched != 2, shmem, &occ));
+  HIPCHK(hsc::pcg_occupancy(G, mixed, lc.sched != 2, false, shmem, &occ));
+  // (the pipelined loop keeps two more t vectors in LDS: W^T z and W^T x)
+  const size_t shmem_pipe = 3 * t_bytes + wc_bytes;
+  int occ_pipe = 0;
+  if (pipe_req) HIPCHK(hsc::pcg_occupancy(G, mixed, true, true, shmem_pipe, &occ_pipe));
+  if (pipe_req) occ = std::min(occ, occ_pipe);
   if (occ <= 0) throw HscError{HSC_ERR_CAPACITY, "persistent PCG kernel does not fit on a CU"};
   const int cap = occ * di.cu;
   // Stage A runs row tiles (SpMV) on nA workgroups and low-rank tiles (W^T p)
@@ -537,15 +553,22 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
     }
     HIPCHK(sc.flags.ensure(sizeof(unsigned) * (size_t)rt));
   }
+  const bool pipe = pipe_req && aligned && !thr && nA == rt;
   sc.nflags = psync ? rt : 0;
-  for (int s = 0; s < nsys; ++s) hs[s]->pass_sync_used = psync;
+  plan.pipe = pipe;
+  if (pipe) shmem = shmem_pipe;
+  for (int s = 0; s < nsys; ++s) {
+    hs[s]->pass_sync_used = psync;
+    hs[s]->pipe_used = pipe;
+  }
   HIPCHK(sc.sysdesc.ensure(sizeof(hsc::SysDesc) * nsys));
   HIPCHK(sc.sysout.ensure(sizeof(hsc::SysOut) * nsys));
   HIPCHK(sc.chunks.ensure(sizeof(int) * 3 * nwg));
-  HIPCHK(sc.partA.ensure(sizeof(double) * (size_t)nsys * nA));
+  HIPCHK(sc.partA.ensure(sizeof(double) * 2 * (size_t)nsys * nA));  // (second half: the pipelined loop's z.q_old)
   HIPCHK(sc.partE.ensure(sizeof(double) * 2 * (size_t)nsys * nF));
   HIPCHK(sc.partF.ensure(sizeof(double) * (size_t)nsys * nF));
   HIPCHK(sc.partL.ensure(sizeof(double) * (size_t)std::max(lrp, 1)));
+  if (pipe) HIPCHK(sc.partR.ensure(sizeof(double) * (size_t)nF * std::max(t_total, 1)));
   HIPCHK(sc.barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
   HIPCHK(sc.errflag.ensure(64));
   std::vector<int>& ch = plan.ch;
@@ -592,7 +615,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   P.rows_per_tile = R;
   P.node_tile = node_tile;
   P.wc_rows = wc_rows;
-  P.wc_off = (int)t_bytes;
+  P.wc_off = (int)(pipe ? 3 * t_bytes : t_bytes);
   P.chunkA = sc.chunks.as<int>();
   P.chunkF = sc.chunks.as<int>() + nwg;
   P.chunkL = sc.chunks.as<int>() + 2 * nwg;
@@ -600,6 +623,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   P.partE = sc.partE.as<double>();
   P.partF = sc.partF.as<double>();
   P.partL = sc.partL.as<double>();
+  P.partR = pipe ? sc.partR.as<double>() : nullptr;
   P.barrier = sc.barrier.as<unsigned>();
   P.error = sc.errflag.as<int>();
   P.timeout_ticks = kTimeoutTicks;
@@ -688,7 +712,7 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   ensure_host_scratch(sc, nsys);
   hipEvent_t e0 = sc.ev0, e1 = sc.ev1;
   HIPCHK(hipEventRecord(e0, st));
-  hipError_t le = hsc::launch_pcg(plan.G, plan.mixed, plan.res, P, plan.shmem, st);
+  hipError_t le = hsc::launch_pcg(plan.G, plan.mixed, plan.res, plan.pipe, P, plan.shmem, st);
   HIPCHK(hipEventRecord(e1, st));
   if (le != hipSuccess) {
     (void)hipStreamSynchronize(st);
@@ -758,7 +782,7 @@ TuneKey tune_key(hsc_handle* const* hs, int nsys) {
                  nsys > 1 ? 1 : 0,
                  c.preconditioner == HSC_PRECOND_NEUMANN ? 0 : 1,
                  c.neumann_order,
-                 c.pass_sync};
+                 c.pass_sync | (c.pipelined << 1)};
 }
 
 bool pinned(const hsc_config& c) { return c.lanes_per_row && c.wg_per_cu && c.schedule; }
@@ -1537,7 +1561,7 @@ void hsc_config_default(hsc_config* c) {
   c->check_acceptance = 1;
   c->poly_interval_ratio = 100.0;
   c->pass_sync = 1;
-  c->reserved0 = 0;
+  c->pipelined = 0;
 }
 
 int32_t hsc_create(const hsc_config* cfg, int32_t device, hsc_handle** out) {
@@ -1583,7 +1607,7 @@ int32_t hsc_set_config(hsc_handle* h, const hsc_config* cfg) {
   // the launch shape and its scratch depend on these (re-planned by the next solve)
   const bool relaunch = cfg->precision != h->cfg.precision || cfg->lanes_per_row != h->cfg.lanes_per_row ||
                         cfg->wg_per_cu != h->cfg.wg_per_cu || cfg->schedule != h->cfg.schedule ||
-                        cfg->pass_sync != h->cfg.pass_sync;
+                        cfg->pass_sync != h->cfg.pass_sync || cfg->pipelined != h->cfg.pipelined;
   h->cfg = *cfg;
   ++h->gen;  // a captured batch of this handle is stale now
   if (refactor) h->setup_ok = false;
@@ -1786,7 +1810,8 @@ int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double
       try {
         for (int i = 0; i < count; ++i) enqueue_rhs(handles[i], b[i], x[i], cap);
         reset_pcg(handles, count, bt->sc, cap);
-        HIPCHK(hsc::launch_pcg_plain(bt->plan.G, bt->plan.mixed, bt->plan.res, bt->plan.P, bt->plan.shmem, cap));
+        HIPCHK(hsc::launch_pcg_plain(bt->plan.G, bt->plan.mixed, bt->plan.res, bt->plan.pipe, bt->plan.P,
+                                     bt->plan.shmem, cap));
         for (int i = 0; i < count; ++i) enqueue_finish(handles[i], b[i], x[i], cap);
       } catch (const HscError& e) {
         what = e.msg;
@@ -1801,7 +1826,7 @@ int32_t hsc_batch_create(hsc_handle* const* handles, int32_t count, const double
     (void)hipStreamDestroy(cap);
     if (!what.empty()) throw HscError{HSC_ERR_HIP, what};
     // mark the persistent PCG node cooperative (co-residency of its grid)
-    const void* fn = hsc::pcg_kernel_fn(bt->plan.G, bt->plan.mixed, bt->plan.res);
+    const void* fn = hsc::pcg_kernel_fn(bt->plan.G, bt->plan.mixed, bt->plan.res, bt->plan.pipe);
     size_t nn = 0;
     HIPCHK(hipGraphGetNodes(bt->graph, nullptr, &nn));
     std::vector<hipGraphNode_t> nodes(nn);
@@ -2131,6 +2156,7 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   o->workgroups = h->wgpcu;
   o->schedule = h->sched;
   o->pass_sync = h->pass_sync_used ? 1 : 0;
+  o->pipelined = h->pipe_used ? 1 : 0;
   o->lambda_max = h->lambda_max;
   o->neumann_doublings = h->doublings;
   o->abd_block_diagonal = h->blockdiag ? 1 : 0;

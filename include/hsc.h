@@ -114,7 +114,10 @@ typedef struct hsc_config {
   int32_t pass_sync;             /* latency schedule, order > 2: 1 (default) = each row tile waits only for
                                     its neighbour tiles between preconditioner passes; 0 = grid barriers.
                                     Same arithmetic either way (bitwise-equal results) */
-  int32_t reserved0;
+  int32_t pipelined;             /* 1 = the pipelined PCG loop where it applies (latency schedule, 4 lanes per
+                                    row, Chebyshev/learned or Neumann passes of order >= 2, recursive-residual
+                                    stopping): one grid barrier per iteration; q = D p kept by recurrence in
+                                    FP64 (no counterpart; default 0 = the reference's loop) */
 } hsc_config;
 
 /* Device view of the assembled Hessian: the stored triangle of 3x3 blocks
@@ -246,6 +249,8 @@ typedef struct hsc_setup_info {
   double learned_out[3];         /* learned: the network's node-mean outputs (log10(b/a) - 2.5 at degrees 8, 12, 16) */
   int32_t learned_used;          /* 1 when the network's interval is in use */
   int32_t fallback_reason;       /* 2 when the network's output was rejected (Neumann series used) */
+  int32_t pipelined;             /* 1 when the last solve's launch ran the pipelined PCG loop */
+  int32_t reserved1;
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 
