@@ -55,7 +55,8 @@ struct FactorParams {
   const double* P;   // qm x qm row-major
   const double* U;   // qm x ldk col-major (ld = qm)
   const double* V;   // a x ldk col-major (ld = a)
-  int qm, a, k;
+  int qm, a, k;         // k: the largest rank (buffer sizing); the rank itself is *kdev
+  const int* kdev;       // rank found by ACA (device; clamped to [0, k] by the kernels)
   double* W1;        // qm x k scratch (P U)
   double* W1T;       // k x qm: (P U)^T, kept for the right-hand side of every solve
   double* C;         // k x k (out: C; then L in lower triangle of Lm)
@@ -83,7 +84,9 @@ hipError_t launch_aca_reg(const AcaParams& P, hipStream_t st);
 hipError_t launch_body_inverse(const double* A, int qm, double* Binv, int* err, hipStream_t st);
 hipError_t launch_abd_inverse(const AbdParams& S, hipStream_t st);
 hipError_t launch_coupling_core(const FactorParams& S, hipStream_t st);
-hipError_t launch_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32, hipStream_t st);
+// W = V L for the rank *kdev (<= kmax), read on the device
+hipError_t launch_make_W(const double* V, const double* Lm, int a, const int* kdev, int kmax, double* W64, float* W32,
+                         hipStream_t st);
 hipError_t launch_rhs(const double* W1T, const double* V, const double* b, int qo, int qm, int fo, int n, int k,
                       int a, const int* act_node, double* y, double* rhs, double* r, double* x, hipStream_t st);
 hipError_t launch_backsub(const double* P, const double* U, const double* V, const int* act_dof, const double* b,

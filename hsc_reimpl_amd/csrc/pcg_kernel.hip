@@ -837,7 +837,10 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     pre_ok = true;
   };
 
-  auto node_stage = [&](int kind, const double* tx) {
+  // tsel: the t vector of the true residual's low-rank term (0: s_t; 2: the
+  // pipelined loop's W^T x copy), chosen by index so its loads stay LDS loads
+  auto node_stage = [&](int kind, int tsel) {
+    const double* tx = s_t + (tsel ? 2 * P.t_total : 0);
     double rr = 0.0, rz = 0.0;
     for_tiles(sys, nsys, trE, 1,
               [&](int s, int tl) {
@@ -1061,7 +1064,9 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // workgroup (sum k <= kBlock) by ng groups of threads (every ng-th chunk,
   // groups added in order), else one thread per entry.  The stage-A loads are
   // issued before the first wait, so all latencies overlap.
-  auto reduce_t = [&](auto want, bool with_pA, double* tdst) {
+  // tsel: destination s_t + tsel * t_total (0: t; 1, 2: the pipelined loop's copies)
+  auto reduce_t = [&](auto want, bool with_pA, int tsel) {
+    double* tdst = s_t + tsel * P.t_total;
     ChunkPre preA;
     if (with_pA) chunks_prefetch(P.partA, nsys, P.nA, rngA, all_sys, preA);
     const int T = P.t_total;
@@ -1137,7 +1142,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   };
 
   // ================================================================ init
-  node_stage(0, s_t);
+  node_stage(0, 0);
   if (!sync_at(0)) goto out;
   reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE, all_sys, s_v1);
   __syncthreads();
@@ -1169,7 +1174,6 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     __shared__ double s_zdz[kMaxSys], s_zq[kMaxSys], s_rr[kMaxSys], s_pdp[kMaxSys];
     __shared__ int s_first[kMaxSys], s_redo[kMaxSys], s_skip[kMaxSys];
     double* s_tz = s_t + P.t_total;      // W^T z
-    double* s_tx = s_t + 2 * P.t_total;  // W^T x (true residual)
     double* partA2 = P.partA + (size_t)nsys * P.nA;
     const int lrow = tid / G;
     for (int s = tid; s < nsys; s += kBlock) {
@@ -1246,7 +1250,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     // t entries of the systems selected by `want` from the row workgroups'
     // partials: entry q of system s sums its chunks cF0..cF1 in order (groups
     // of threads take every ng-th chunk; the groups are added in order)
-    auto reduce_rows_t = [&](auto want, double* tdst) {
+    auto reduce_rows_t = [&](auto want, int tsel) {
+      double* tdst = s_t + tsel * P.t_total;
       const int T = P.t_total;
       const int ng = T > 0 && T <= kBlock ? kBlock / T : 1;
       auto sum_entry = [&](int q, int g0, int gs) {
@@ -1354,7 +1359,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       // nodes alike: one chunk range per system), then W^T z
       reduce4_chunks(P.partF, P.nF, P.partA, partA2, P.nA, partE_rr, nsys, P.rt_total, rngF, s_v0, s_zdz, s_zq,
                      s_rr);
-      reduce_rows_t([&](int s) { return s_st[s] == ST_RUN; }, s_tz);
+      reduce_rows_t([&](int s) { return s_st[s] == ST_RUN; }, 1);
       if (tid == 0) {
         int any = 0;
         for (int s = 0; s < nsys; ++s) {
@@ -1382,8 +1387,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
           rows_Wt(true);
         }
         if (!sync_at(4)) goto out;
-        reduce_rows_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, s_tx);
-        node_stage(2, s_tx);
+        reduce_rows_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, 2);
+        node_stage(2, 2);
         if (!sync_at(5)) goto out;
         reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE,
                           [&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, s_v1);
@@ -1559,7 +1564,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     pc ^= 1;
     if (!sync_at(2)) goto out;
     prefetch_B();
-    reduce_t([&](int s) { return s_st[s] == ST_RUN; }, true, s_t);
+    reduce_t([&](int s) { return s_st[s] == ST_RUN; }, true, 0);
     for (int s = tid; s < nsys; s += kBlock) {
       if (s_st[s] != ST_RUN) continue;
       const double pAp = s_v1[s] - s_tt[s];
@@ -1574,7 +1579,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
 
     // ---------------- stage B
     if (P.prof && tid == 0) P.prof[(size_t)b * 32 + 12] += rt_ticks() - t_last;  // B: reductions part
-    node_stage(1, s_t);
+    node_stage(1, 0);
     pre_ok = false;
     if (!sync_at(3)) goto out;
     reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE, all_sys, s_v1);
@@ -1613,8 +1618,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   [&](int) {});
       }
       if (!sync_at(4)) goto out;
-      reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, false, s_t);
-      node_stage(2, s_t);
+      reduce_t([&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, false, 0);
+      node_stage(2, 0);
       if (!sync_at(5)) goto out;
       reduce_chunks_all(partE_rr, nsys, P.nF, P.et_total, rngE,
                         [&](int s) { return s_st[s] == ST_RUN && s_chk[s] != CHK_NONE; }, s_v1);

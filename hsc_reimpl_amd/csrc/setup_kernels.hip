@@ -1287,9 +1287,8 @@ __global__ __launch_bounds__(1024) void k_abd_inverse(AbdParams S) {
 // workgroup staged through LDS.  op(A) = A (row-major, lda) or A^T when
 // transA (A stored column-major with leading dim lda, i.e. A^T row-major).
 // B is addressed as B[k*ldb_k + n*ldb_n] so both layouts are covered.
-__global__ __launch_bounds__(256) void k_gemm_small(const double* A, int lda, int transA, const double* B,
-                                                    int ldb_k, int ldb_n, double* C, int ldc, int M, int N, int K,
-                                                    double* Ct) {
+__device__ __forceinline__ void gemm_small_tile(const double* A, int lda, int transA, const double* B, int ldb_k,
+                                                int ldb_n, double* C, int ldc, int M, int N, int K, double* Ct) {
   __shared__ double As[16][17], Bs[16][17];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
@@ -1310,11 +1309,34 @@ __global__ __launch_bounds__(256) void k_gemm_small(const double* A, int lda, in
   }
 }
 
+__global__ __launch_bounds__(256) void k_gemm_small(const double* A, int lda, int transA, const double* B,
+                                                    int ldb_k, int ldb_n, double* C, int ldc, int M, int N, int K,
+                                                    double* Ct) {
+  gemm_small_tile(A, lda, transA, B, ldb_k, ldb_n, C, ldc, M, N, K, Ct);
+}
+
+// the rank ACA found, read on the device (the coupling kernels are queued
+// before the host has read it back)
+__device__ __forceinline__ int factor_rank(const FactorParams& S) { return min(max(*S.kdev, 0), S.k); }
+
+// W1 = P U (qm x k row-major) and W1T = W1^T (k x qm); tiles past the rank exit
+__global__ __launch_bounds__(256) void k_coupling_PU(FactorParams S) {
+  const int k = factor_rank(S), n = S.qm;
+  if ((int)blockIdx.x * 16 >= k) return;  // uniform
+  gemm_small_tile(S.P, n, 0, S.U, 1, n, S.W1, k, n, k, n, S.W1T);
+}
+// C = U^T W1 (k x k row-major; U column-major read as row-major U^T)
+__global__ __launch_bounds__(256) void k_coupling_UtW(FactorParams S) {
+  const int k = factor_rank(S), n = S.qm;
+  if ((int)blockIdx.x * 16 >= k || (int)blockIdx.y * 16 >= k) return;  // uniform
+  gemm_small_tile(S.U, n, 0, S.W1, k, 1, S.C, k, k, k, n, nullptr);
+}
+
 constexpr int kLdsChol = 80;  // C of rank <= 80 is factored in LDS (50 KB)
 // C = U^T P U symmetrised (linear_hsc.cu:298-300), copied to Lm; 6 when an
 // entry is not finite, else 0.
 __device__ int coupling_prelude(const FactorParams& S) {
-  const int k = S.k;
+  const int k = factor_rank(S);
   __syncthreads();
   wg_symmetrize(S.C, k);
   int bad = 0;
@@ -1333,7 +1355,7 @@ __device__ int coupling_prelude(const FactorParams& S) {
 // C = U^T P U symmetrised and its Cholesky C = L L^T.  (The products
 // W1 = P U and C = U^T W1 are done by k_gemm_small before this.)
 __global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
-  const int k = S.k;
+  const int k = factor_rank(S);
   int status = coupling_prelude(S);
   if (!status) {
     if (k <= kLdsChol) {
@@ -1356,35 +1378,38 @@ __global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
 // and L into LDS and writes its 64 x k block of W row-major (coalesced);
 // larger ranks: one thread per entry.
 constexpr int kMakeWRows = 64, kMakeWMaxK = 64;
-__global__ __launch_bounds__(256) void k_make_W_lds(const double* V, const double* Lm, int a, int k, double* W64,
-                                                    float* W32) {
+__global__ __launch_bounds__(256) void k_make_W(const double* V, const double* Lm, int a, const int* kdev, int kmax,
+                                                double* W64, float* W32) {
   __shared__ double s_V[kMakeWMaxK][kMakeWRows + 1];
   __shared__ double s_L[kMakeWMaxK * kMakeWMaxK];
+  const int k = min(max(*kdev, 0), kmax);
   const int j0 = blockIdx.x * kMakeWRows, rows = min(kMakeWRows, a - j0);
-  for (int e = threadIdx.x; e < k * kMakeWRows; e += blockDim.x) {
-    const int m = e / kMakeWRows, jj = e % kMakeWRows;
-    s_V[m][jj] = jj < rows ? V[j0 + jj + (size_t)a * m] : 0.0;
-  }
-  for (int e = threadIdx.x; e < k * k; e += blockDim.x) s_L[e] = Lm[e];
-  __syncthreads();
-  for (int o = threadIdx.x; o < rows * k; o += blockDim.x) {
-    const int jj = o / k, l = o % k;
-    double s = 0.0;
+  if (k == 0) return;
+  if (k <= kMakeWMaxK) {
+    for (int e = threadIdx.x; e < k * kMakeWRows; e += blockDim.x) {
+      const int m = e / kMakeWRows, jj = e % kMakeWRows;
+      s_V[m][jj] = jj < rows ? V[j0 + jj + (size_t)a * m] : 0.0;
+    }
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) s_L[e] = Lm[e];
+    __syncthreads();
+    for (int o = threadIdx.x; o < rows * k; o += blockDim.x) {
+      const int jj = o / k, l = o % k;
+      double s = 0.0;
 #pragma unroll 4
-    for (int m = l; m < k; ++m) s += s_V[m][jj] * s_L[m * k + l];
-    W64[(size_t)j0 * k + o] = s;
-    W32[(size_t)j0 * k + o] = (float)s;
-  }
-}
-__global__ void k_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)a * k) return;
-  const int j = (int)(e / k), l = (int)(e % k);
-  double s = 0.0;
+      for (int m = l; m < k; ++m) s += s_V[m][jj] * s_L[m * k + l];
+      W64[(size_t)j0 * k + o] = s;
+      W32[(size_t)j0 * k + o] = (float)s;
+    }
+  } else {
+    for (int o = threadIdx.x; o < rows * k; o += blockDim.x) {
+      const int j = j0 + o / k, l = o % k;
+      double s = 0.0;
 #pragma unroll 8
-  for (int m = l; m < k; ++m) s += V[j + (size_t)a * m] * Lm[(size_t)m * k + l];
-  W64[e] = s;
-  W32[e] = (float)s;
+      for (int m = l; m < k; ++m) s += V[j + (size_t)a * m] * Lm[(size_t)m * k + l];
+      W64[(size_t)j0 * k + o] = s;
+      W32[(size_t)j0 * k + o] = (float)s;
+    }
+  }
 }
 
 // y[0:rows) = M x with M[i][m] at M[i * ld + m] (a row-major matrix, or the
@@ -1648,23 +1673,17 @@ hipError_t launch_abd_inverse(const AbdParams& S, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_coupling_core(const FactorParams& S, hipStream_t st) {
-  const int n = S.qm, k = S.k;
-  // W1 = P U  (P row-major n x n; U column-major n x k: U[m + n*l])
-  hipLaunchKernelGGL(k_gemm_small, dim3((k + 15) / 16, (n + 15) / 16), dim3(256), 0, st, S.P, n, 0, S.U, 1, n,
-                     S.W1, k, n, k, n, S.W1T);
-  // C = U^T W1 (column-major U read as row-major U^T with lda = n; W1 row-major n x k)
-  hipLaunchKernelGGL(k_gemm_small, dim3((k + 15) / 16, (k + 15) / 16), dim3(256), 0, st, S.U, n, 0, S.W1, k, 1,
-                     S.C, k, k, k, n, nullptr);
-  // rank <= 64: one wave factors C in registers; else a workgroup in LDS
+  const int n = S.qm, k = S.k;  // grids for the largest rank; the kernels read the rank
+  if (k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_coupling_PU, dim3((k + 15) / 16, (n + 15) / 16), dim3(256), 0, st, S);
+  hipLaunchKernelGGL(k_coupling_UtW, dim3((k + 15) / 16, (k + 15) / 16), dim3(256), 0, st, S);
   hipLaunchKernelGGL(k_coupling_core, dim3(1), dim3(1024), 0, st, S);
   return hipGetLastError();
 }
-hipError_t launch_make_W(const double* V, const double* Lm, int a, int k, double* W64, float* W32, hipStream_t st) {
-  if ((long long)a * k == 0) return hipSuccess;
-  if (k <= kMakeWMaxK)
-    hipLaunchKernelGGL(k_make_W_lds, dim3(cdiv(a, kMakeWRows)), dim3(256), 0, st, V, Lm, a, k, W64, W32);
-  else
-    hipLaunchKernelGGL(k_make_W, dim3(cdiv((long long)a * k, 256)), dim3(256), 0, st, V, Lm, a, k, W64, W32);
+hipError_t launch_make_W(const double* V, const double* Lm, int a, const int* kdev, int kmax, double* W64, float* W32,
+                         hipStream_t st) {
+  if (a <= 0 || kmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_make_W, dim3(cdiv(a, kMakeWRows)), dim3(256), 0, st, V, Lm, a, kdev, kmax, W64, W32);
   return hipGetLastError();
 }
 hipError_t launch_rhs(const double* W1T, const double* V, const double* b, int qo, int qm, int fo, int n, int k,

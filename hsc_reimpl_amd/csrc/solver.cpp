@@ -1218,6 +1218,19 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   if (h->cfg.abd_inverse == HSC_ABD_NEUMANN && h->cfg.adaptive_abd_inverse) abp.mode = 2;
   HIPCHK(hsc::launch_abd_inverse(abp, st));
   h->abd_mode = abp.mode;
+  // ---- C = U^T P U, its Cholesky C = L L^T and W = V L, for the rank ACA
+  // found: the kernels read it on the device, so the chain follows ACA
+  // without a round trip to the host (grids sized for the largest rank)
+  ss->fst = 0.0;
+  if (limit > 0) {
+    hsc::FactorParams fp{h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), qm, a, limit, rank_dev,
+                         h->W1.as<double>(), h->W1.as<double>() + (size_t)qm * limit, h->C.as<double>(),
+                         h->Lm.as<double>(), dmisc + 4};
+    HIPCHK(hsc::launch_coupling_core(fp, st));
+    HIPCHK(hsc::launch_make_W(h->V.as<double>(), h->Lm.as<double>(), a, rank_dev, limit, h->W64.as<double>(),
+                              h->W32.as<float>(), st));
+    HIPCHK(hipMemcpyAsync(&ss->fst, dmisc + 4, sizeof(double), hipMemcpyDeviceToHost, st));
+  }
   ss->aca_err = 0;
   HIPCHK(hipMemcpyAsync(ss->ints2, err + 8, sizeof(ss->ints2), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&ss->max_coupled, err + 12, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1262,18 +1275,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   if (abd_status == 6) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite Neumann/Schur factors"};
   h->aca_err = (limit > 0) ? ss->hist[rank] : 0.0;
   h->k = rank;
-  if (rank > 0) {
-    hsc::FactorParams fp{h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), qm, a, rank,
-                         h->W1.as<double>(), h->W1.as<double>() + (size_t)qm * rank, h->C.as<double>(),
-                         h->Lm.as<double>(), dmisc + 4};
-    HIPCHK(hsc::launch_coupling_core(fp, st));
-    HIPCHK(hsc::launch_make_W(h->V.as<double>(), h->Lm.as<double>(), a, rank, h->W64.as<double>(),
-                              h->W32.as<float>(), st));
-    ss->fst = 0.0;
-    HIPCHK(hipMemcpyAsync(&ss->fst, dmisc + 4, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (ss->fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
-  }
+  if (rank > 0 && ss->fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
   HIPCHK(hipStreamSynchronize(h->aux));
   if (learned) HIPCHK(hipStreamSynchronize(h->aux2));
   set_poly_weights(h, ss->lam2, ss->pred);
@@ -1470,7 +1472,7 @@ int status_from_out(hsc_handle* h, const hsc::SysOut& o) {
 
 // Reduced right-hand side and initial residual of h's system (rhs = r).
 void enqueue_rhs(hsc_handle* h, const double* b, double* x, hipStream_t st) {
-  HIPCHK(hsc::launch_rhs(h->W1.as<double>() + (size_t)h->qm * h->k, h->V.as<double>(), b, h->qo, h->qm, h->fo,
+  HIPCHK(hsc::launch_rhs(h->W1.as<double>() + (size_t)h->qm * h->limit, h->V.as<double>(), b, h->qo, h->qm, h->fo,
                          h->fn, h->k, h->a, h->act_node.as<int>(), h->ybuf.as<double>(), h->rhs.as<double>(),
                          h->r.as<double>(), x, st));
 }
