@@ -73,11 +73,11 @@ class SetupInfo(C.Structure):
         ("aca_relative_error", C.c_double), ("abd_inverse_residual", C.c_double),
         ("schedule", C.c_int32), ("pass_sync", C.c_int32), ("lambda_max", C.c_double),
         ("poly_ratio", C.c_double), ("learned_out", C.c_double * 3), ("learned_used", C.c_int32),
-        ("fallback_reason", C.c_int32), ("pipelined", C.c_int32), ("reserved1", C.c_int32),
+        ("fallback_reason", C.c_int32), ("pipelined", C.c_int32), ("pass_pairs", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        out = {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved1"}
+        out = {name: getattr(self, name) for name, _ in self._fields_}
         out["learned_out"] = list(self.learned_out)
         return out
 

@@ -20,6 +20,39 @@ constexpr int kMaxPolySteps = 32;    // per-step weights of the polynomial preco
 constexpr int kNbrStride = 64;       // row-tile neighbour list: [count, up to 63 tiles]; count -1 = every tile
 constexpr int kNbrMaxTiles = 1024;   // row tiles per system for which neighbour lists are built
 
+// Pass pairs (FP32 resident rows, 4 lanes per row, 128-row tiles): a row
+// tile computes pass j on its rows and on its halo rows (the rows of other
+// tiles that its rows reference), keeps both in LDS and computes pass j+1 on
+// its rows from there, so it synchronises with its neighbours once per two
+// passes.  The tile's node set lists the nodes those passes read: its own
+// rows (slots 0..127), its halo rows (128..127+nh), then the other columns
+// of the halo rows; each is gathered once per pair.  Per tile, a halo pack in
+// device memory is the image of the LDS region the kernel copies it to at
+// launch:
+//   [0, 16)        int  nh (halo rows), nset (node-set size), nblk (halo blocks), 0
+//   [16, 2064)     int2 per halo row {length, first halo block}
+//   [2064, 11280)  float per halo row: its 3x3 inverse diagonal block, row-major
+//   [11280, 16400) ushort [u][thread]: set slot of the column of block u of each own-row lane
+//   [16400, 18448) int  node of set slots 128.. (halo rows, then their other columns)
+//   [18448, ..)    ushort set slot of each halo block's column (16-byte padded)
+//   [.., +36 hbl)  float the 9 value components of each halo block, block after block
+// followed in LDS (not in the pack) by the scratch of a pair, one float4 per
+// node: the set's z, the halo rows' r and the pass values of the own and halo
+// rows.
+constexpr int kHaloRows = 256;       // halo rows per row tile
+constexpr int kHaloSet = 640;        // node-set slots per row tile
+constexpr int kHaloU = 5;            // blocks per lane of a row in pair mode (rows of at most 20 blocks)
+constexpr int kHaloTileRows = 128;   // rows per tile in pair mode (4 lanes per row, 512 threads)
+constexpr int kHaloOffRows = 16;
+constexpr int kHaloOffInv = kHaloOffRows + 8 * kHaloRows;
+constexpr int kHaloOffSlot = kHaloOffInv + 36 * kHaloRows;
+constexpr int kHaloOffSet = kHaloOffSlot + 2 * kHaloU * 512;
+constexpr int kHaloOffBlk = kHaloOffSet + 4 * (kHaloSet - kHaloTileRows);
+__host__ __device__ constexpr long long halo_val_off(int hbl) { return kHaloOffBlk + (2LL * hbl + 15) / 16 * 16; }
+__host__ __device__ constexpr long long halo_pack_bytes(int hbl) { return (halo_val_off(hbl) + 36LL * hbl + 255) / 256 * 256; }
+constexpr int kHaloScratchBytes = 16 * (kHaloSet + kHaloRows + kHaloTileRows + kHaloRows);
+constexpr int kHaloBaseBytes = 256;  // per-system header before the tile packs: int fail flag (nonzero: no pairs)
+
 // Per-system descriptor for the persistent PCG kernel and the operator
 // kernels.  All pointers are device pointers.  VT buffers (p, z, q) hold
 // double (FP64 mode) or float (mixed mode).
@@ -69,6 +102,8 @@ struct SysDesc {
   void* q;
   double* q64;            // FP64 D x for the true residual (== q in FP64 mode)
   const int* nbr;         // [row tiles * kNbrStride] neighbour lists (local tile ids), or null
+  const int* nbr2;        // two-hop neighbour lists (pass pairs), same layout, or null
+  const unsigned char* halo;  // pass pairs: kHaloBaseBytes header, then one halo pack per row tile, or null
   double tol;
   int max_iter;
   int pad_;
@@ -80,6 +115,8 @@ struct SysOut {
   int status;             // hsc_status
   int checks;             // true-residual evaluations
   int restarts;
+  int pairs;              // 1 when the launch ran its preconditioner passes in pairs
+  int pad_;
   double rel;             // final relative residual (true when converged)
   double normb;
 };
@@ -114,6 +151,12 @@ struct PcgParams {
   int t_total;            // LDS doubles for t vectors
   int pass_sync;          // 1: passes before the last publish per-tile flags and wait on neighbour tiles
   unsigned* pass_flags;   // [rt_total] last completed pass of each row tile (zeroed before launch)
+  int sys_bytes;          // dynamic LDS bytes of the system descriptors (the low-rank node sums follow)
+  int sg_bytes;           // dynamic LDS bytes of the low-rank node sums (node_tile x 3 doubles; the t vectors follow)
+  int halo;               // 1: pass pairs when every system's halo packs were built
+  int halo_off;           // byte offset of the halo region from the t vectors (after them and the W-row cache)
+  int hbl;                // pack blocks per halo pack
+  long long halo_stride;  // bytes per halo pack (halo_pack_bytes(hbl))
   long long* prof;        // optional per-workgroup stage timers [nwg * 32] (null = off):
                           // site si: [2si] ticks computing before barrier si, [2si+1] ticks waiting in it
 };

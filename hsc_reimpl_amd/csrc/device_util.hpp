@@ -21,11 +21,49 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Sum over groups of G consecutive lanes.
+// v of lane (lane ^ M) for M in {1, 2, 4, 8}, through DPP lane moves inside
+// the VALU (no LDS permute round trip): quad_perm for 1 and 2; 4 = half-row
+// mirror (lane ^ 7) then the quad reversal (lane ^ 3); 8 = row mirror
+// (lane ^ 15) then the half-row mirror (lane ^ 7).
+template <int M>
+__device__ __forceinline__ int dpp_xor_i32(int v) {
+  static_assert(M == 1 || M == 2 || M == 4 || M == 8, "DPP xor distance");
+  if constexpr (M == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+  if constexpr (M == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+  if constexpr (M == 4) {
+    const int t = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);             // row_half_mirror
+    return __builtin_amdgcn_mov_dpp(t, 0x1B, 0xF, 0xF, true);                      // quad_perm [3,2,1,0]
+  }
+  if constexpr (M == 8) {
+    const int t = __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true);             // row_mirror
+    return __builtin_amdgcn_mov_dpp(t, 0x141, 0xF, 0xF, true);                     // row_half_mirror
+  }
+  return v;
+}
+template <int M>
+__device__ __forceinline__ float dpp_xor(float v) {
+  return __int_as_float(dpp_xor_i32<M>(__float_as_int(v)));
+}
+template <int M>
+__device__ __forceinline__ double dpp_xor(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_xor_i32<M>((int)(unsigned)(b & 0xffffffffll));
+  const int hi = dpp_xor_i32<M>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over groups of G consecutive lanes: v += v(lane ^ m) for m = G/2 .. 1,
+// so every lane of a group gets the bitwise-same value (each step adds two
+// commutative partial sums).  Distances up to 8 move through DPP.
 template <int G, typename T>
 __device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-  for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
+  static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "group size");
+  if constexpr (G == 64) v += __shfl_xor(v, 32, 64);
+  if constexpr (G >= 32) v += __shfl_xor(v, 16, G);
+  if constexpr (G >= 16) v += dpp_xor<8>(v);
+  if constexpr (G >= 8) v += dpp_xor<4>(v);
+  if constexpr (G >= 4) v += dpp_xor<2>(v);
+  if constexpr (G >= 2) v += dpp_xor<1>(v);
   return v;
 }
 

@@ -58,6 +58,30 @@ template <typename MT> __device__ __forceinline__ const MT* w_vals(const SysDesc
 template <> __device__ __forceinline__ const double* w_vals<double>(const SysDesc& S) { return S.W64; }
 template <> __device__ __forceinline__ const float* w_vals<float>(const SysDesc& S) { return S.W32; }
 
+// One block of a lane's row sum: a += B x when `in` (a select).  The fused
+// multiply-adds are written out, so every code path that computes a row
+// (registers, LDS or memory operands) rounds the same way: the sums are
+// bitwise the same whatever the compiler's contraction choices.
+template <typename MT>
+__device__ __forceinline__ void blk_step(bool in, const MT (&v)[9], MT x0, MT x1, MT x2, MT& a0, MT& a1, MT& a2) {
+  const MT b0 = a0 + fma(v[6], x2, fma(v[3], x1, v[0] * x0));
+  const MT b1 = a1 + fma(v[7], x2, fma(v[4], x1, v[1] * x0));
+  const MT b2 = a2 + fma(v[8], x2, fma(v[5], x1, v[2] * x0));
+  a0 = in ? b0 : a0;
+  a1 = in ? b1 : a1;
+  a2 = in ? b2 : a2;
+}
+
+// Output component of a preconditioner pass: z + w d.(r - D z)_row, with d
+// the component's row of the inverse diagonal block and a = (D z)_row
+// (fused multiply-adds written out, as in blk_step).
+template <typename VT>
+__device__ __forceinline__ VT pass_out(VT zo, VT w, VT d0, VT d1, VT d2, double r0, double r1, double r2, VT a0,
+                                       VT a1, VT a2) {
+  const VT e0 = (VT)r0 - a0, e1 = (VT)r1 - a1, e2 = (VT)r2 - a2;
+  return fma(w, fma(d2, e2, fma(d1, e1, d0 * e0)), zo);
+}
+
 // y_row = sum_j D_ij src(j) over one block row with G lanes (lane strides the
 // row's 3x3 blocks); every lane of the group gets the full 3-vector.  With
 // U > 1 each round puts U blocks per lane in flight at once (column indices
@@ -78,12 +102,9 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
         MT x0, x1, x2;
         src(c, x0, x1, x2);
         const auto v = gsel<MT>(val) + val_index(p, 0);
-        const MT v0 = v[0], v1 = v[kValChunk], v2 = v[2 * kValChunk];
-        const MT v3 = v[3 * kValChunk], v4 = v[4 * kValChunk], v5 = v[5 * kValChunk];
-        const MT v6 = v[6 * kValChunk], v7 = v[7 * kValChunk], v8 = v[8 * kValChunk];
-        a0 += v0 * x0 + v3 * x1 + v6 * x2;
-        a1 += v1 * x0 + v4 * x1 + v7 * x2;
-        a2 += v2 * x0 + v5 * x1 + v8 * x2;
+        const MT vb[9] = {v[0],             v[kValChunk],     v[2 * kValChunk], v[3 * kValChunk], v[4 * kValChunk],
+                          v[5 * kValChunk], v[6 * kValChunk], v[7 * kValChunk], v[8 * kValChunk]};
+        blk_step(true, vb, x0, x1, x2, a0, a1, a2);
       }
     } else {
       for (int pb = p0 + lane; pb < p1; pb += G * U) {
@@ -102,13 +123,7 @@ __device__ __forceinline__ void row_spmv(const SysDesc& S, const MT* __restrict_
           for (int e = 0; e < 9; ++e) v[u][e] = vg[val_index(pp[u], e)];
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (pb + u * G < p1) {
-            a0 += v[u][0] * x[u][0] + v[u][3] * x[u][1] + v[u][6] * x[u][2];
-            a1 += v[u][1] * x[u][0] + v[u][4] * x[u][1] + v[u][7] * x[u][2];
-            a2 += v[u][2] * x[u][0] + v[u][5] * x[u][1] + v[u][8] * x[u][2];
-          }
-        }
+        for (int u = 0; u < U; ++u) blk_step(pb + u * G < p1, v[u], x[u][0], x[u][1], x[u][2], a0, a1, a2);
       }
     }
   }
@@ -171,9 +186,8 @@ __device__ __forceinline__ void tile_N(const SysDesc& S, int tl, const VT* __res
   if (row < S.nb && lane < 3) {
     const int i3 = 3 * row;
     const double r0 = S.r[i3], r1 = S.r[i3 + 1], r2 = S.r[i3 + 2];
-    const VT e0 = (VT)r0 - a0, e1 = (VT)r1 - a1, e2 = (VT)r2 - a2;
     const VT* inv = inv_vals<VT>(S) + 9 * row + 3 * lane;
-    const VT out = zin[i3 + lane] + omega * (inv[0] * e0 + inv[1] * e1 + inv[2] * e2);
+    const VT out = pass_out(zin[i3 + lane], omega, inv[0], inv[1], inv[2], r0, r1, r2, a0, a1, a2);
     zout[i3 + lane] = out;
     if (last) rz += pick3(lane, r0, r1, r2) * (double)out;
   }
@@ -242,36 +256,34 @@ __device__ __forceinline__ void row_preload(const SysDesc& S, const MT* __restri
   }
 }
 
-// row_spmv with the first U blocks of each lane from registers
-template <int G, typename MT, typename Src>
+// row_spmv with the first U blocks of each lane from registers.  The gathers
+// of the resident blocks are issued unconditionally (absent blocks gather
+// the row itself, rows past the system column 0, with zero values), then
+// `pre` issues the caller's own loads of the row, and only then is anything
+// accumulated: every load of a pass is in flight together, one memory
+// latency per pass.  Blocks beyond the row's end are skipped by a select,
+// so each lane adds exactly its blocks p0+lane, p0+lane+G, ... in order.
+template <int G, typename MT, typename Src, typename Pre>
 __device__ __forceinline__ void row_spmv_res(const SysDesc& S, const MT* __restrict__ val, const RowRes<G, MT>& rr,
-                                             int lane, Src src, MT& a0, MT& a1, MT& a2) {
+                                             int lane, Src src, Pre pre, MT& a0, MT& a1, MT& a2) {
   constexpr int U = RowRes<G, MT>::U;
   a0 = a1 = a2 = MT(0);
-  if (rr.p1 > rr.p0) {
-    MT x[U][3];
+  MT x[U][3];
 #pragma unroll
-    for (int u = 0; u < U; ++u) src(rr.c[u], x[u][0], x[u][1], x[u][2]);
+  for (int u = 0; u < U; ++u) src(rr.c[u], x[u][0], x[u][1], x[u][2]);
+  pre();
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (rr.p0 + lane + u * G < rr.p1) {
-        a0 += rr.v[u][0] * x[u][0] + rr.v[u][3] * x[u][1] + rr.v[u][6] * x[u][2];
-        a1 += rr.v[u][1] * x[u][0] + rr.v[u][4] * x[u][1] + rr.v[u][7] * x[u][2];
-        a2 += rr.v[u][2] * x[u][0] + rr.v[u][5] * x[u][1] + rr.v[u][8] * x[u][2];
-      }
-    }
+  for (int u = 0; u < U; ++u) blk_step(rr.p0 + lane + u * G < rr.p1, rr.v[u], x[u][0], x[u][1], x[u][2], a0, a1, a2);
+  if (rr.p0 + lane + U * G < rr.p1) {
     const auto colidx = gsel<MT>(S.colidx);
     for (int p = rr.p0 + lane + U * G; p < rr.p1; p += G) {
       const int c = colidx[p];
       MT x0, x1, x2;
       src(c, x0, x1, x2);
       const auto v = gsel<MT>(val) + val_index(p, 0);
-      const MT v0 = v[0], v1 = v[kValChunk], v2 = v[2 * kValChunk];
-      const MT v3 = v[3 * kValChunk], v4 = v[4 * kValChunk], v5 = v[5 * kValChunk];
-      const MT v6 = v[6 * kValChunk], v7 = v[7 * kValChunk], v8 = v[8 * kValChunk];
-      a0 += v0 * x0 + v3 * x1 + v6 * x2;
-      a1 += v1 * x0 + v4 * x1 + v7 * x2;
-      a2 += v2 * x0 + v5 * x1 + v8 * x2;
+      const MT vb[9] = {v[0],             v[kValChunk],     v[2 * kValChunk], v[3 * kValChunk], v[4 * kValChunk],
+                        v[5 * kValChunk], v[6 * kValChunk], v[7 * kValChunk], v[8 * kValChunk]};
+      blk_step(true, vb, x0, x1, x2, a0, a1, a2);
     }
   }
   a0 = group_sum<G>(a0);
@@ -287,23 +299,20 @@ __device__ __forceinline__ void zst(VT* p, VT v) {
   else *p = v;
 }
 
-// Neumann step on the resident row tile (same arithmetic as tile_N)
+// Neumann step on the resident row tile (same arithmetic as tile_N).  The
+// row's r and z are loaded after the gathers are issued, by every lane (a
+// lane that owns no output reads row 0), so no branch holds them back.
 template <int G, typename VT, bool WT = false>
 __device__ __forceinline__ void tile_N_res(const SysDesc& S, int row, const RowRes<G, VT>& rr,
                                            const VT* __restrict__ zin, VT* __restrict__ zout, VT omega, bool last,
                                            double& rz) {
   const int lane = threadIdx.x % G;
   const bool own = row < S.nb && lane < 3;
-  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-  VT zo = VT(0);
-  const auto zg = gsel<VT>(zin);
-  if (own) {
-    const auto rg = gsel<VT>(S.r);
-    r0 = rg[3 * row];
-    r1 = rg[3 * row + 1];
-    r2 = rg[3 * row + 2];
-    zo = zg[3 * row + lane];
-  }
+  const int ro = own ? row : 0, lo = own ? lane : 0;
+  double r0, r1, r2;
+  VT zo;
+  const auto zg = gbl(zin);
+  const auto rg = gbl(S.r);
   VT a0, a1, a2;
   row_spmv_res<G, VT>(S, mat_vals<VT>(S), rr, lane,
                       [&](int c, VT& x0, VT& x1, VT& x2) {
@@ -311,35 +320,42 @@ __device__ __forceinline__ void tile_N_res(const SysDesc& S, int row, const RowR
                         x1 = zg[3 * c + 1];
                         x2 = zg[3 * c + 2];
                       },
+                      [&]() {
+                        r0 = rg[3 * ro];
+                        r1 = rg[3 * ro + 1];
+                        r2 = rg[3 * ro + 2];
+                        zo = zg[3 * ro + lo];
+                      },
                       a0, a1, a2);
   if (own) {
-    const VT e0 = (VT)r0 - a0, e1 = (VT)r1 - a1, e2 = (VT)r2 - a2;
-    const VT out = zo + omega * (rr.d0 * e0 + rr.d1 * e1 + rr.d2 * e2);
+    const VT out = pass_out(zo, omega, rr.d0, rr.d1, rr.d2, r0, r1, r2, a0, a1, a2);
     zst<WT>(zout + 3 * row + lane, out);
     if (last) rz += pick3(lane, r0, r1, r2) * (double)out;
   }
 }
 
-// stage A on the resident row tile (same arithmetic as tile_A)
+// stage A on the resident row tile (same arithmetic as tile_A; loads as in
+// tile_N_res)
 template <int G, typename VT>
 __device__ __forceinline__ void tile_A_res(const SysDesc& S, int row, const RowRes<G, VT>& rr,
                                            const VT* __restrict__ z, const VT* __restrict__ pold,
                                            VT* __restrict__ pnew, VT* __restrict__ q, VT beta, double& dot) {
   const int lane = threadIdx.x % G;
   const bool own = row < S.nb && lane < 3;
-  VT zo = VT(0), po = VT(0);
-  const auto zg = gsel<VT>(z);
-  const auto pg = gsel<VT>(pold);
-  if (own) {
-    zo = zg[3 * row + lane];
-    po = pg[3 * row + lane];
-  }
+  const int io = own ? 3 * row + lane : 0;
+  VT zo, po;
+  const auto zg = gbl(z);
+  const auto pg = gbl(pold);
   VT a0, a1, a2;
   row_spmv_res<G, VT>(S, mat_vals<VT>(S), rr, lane,
                       [&](int c, VT& x0, VT& x1, VT& x2) {
                         x0 = combine(zg[3 * c], pg[3 * c], beta);
                         x1 = combine(zg[3 * c + 1], pg[3 * c + 1], beta);
                         x2 = combine(zg[3 * c + 2], pg[3 * c + 2], beta);
+                      },
+                      [&]() {
+                        zo = zg[io];
+                        po = pg[io];
                       },
                       a0, a1, a2);
   if (own) {
@@ -686,7 +702,15 @@ __device__ __forceinline__ void sum_tt_all(const SysDesc* sys, int nsys, const d
 template <int G, typename VT, bool RES, bool PIPE = false>
 __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   constexpr int SU = StreamU<G, RES>::value;
-  extern __shared__ double s_t[];  // t vectors (sum of ranks)
+  // pass pairs: FP32 resident rows at 4 lanes per row, standard loop
+  constexpr bool kPairs = G == 4 && RES && !PIPE && sizeof(VT) == 4;
+  // dynamic LDS: the system descriptors, the low-rank sums of the node
+  // tile's coupled nodes, then the t vectors (sum of ranks), the coupled W
+  // rows of the resident node tile and the halo region
+  extern __shared__ __align__(16) double s_dyn[];
+  SysDesc* const s_sys = reinterpret_cast<SysDesc*>(s_dyn);
+  double(*const s_g)[3] = reinterpret_cast<double(*)[3]>(reinterpret_cast<char*>(s_dyn) + P.sys_bytes);
+  double* const s_t = reinterpret_cast<double*>(reinterpret_cast<char*>(s_dyn) + P.sys_bytes + P.sg_bytes);
   __shared__ double s_rz[kMaxSys], s_alpha[kMaxSys], s_beta[kMaxSys], s_normb[kMaxSys];
   __shared__ double s_rel[kMaxSys], s_v0[kMaxSys], s_v1[kMaxSys], s_tt[kMaxSys];
   __shared__ int s_st[kMaxSys], s_it[kMaxSys], s_chk[kMaxSys], s_rs[kMaxSys];
@@ -695,9 +719,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   __shared__ double s_lr[kBlock / 64][64];
   __shared__ int s_any, s_pfail;
   __shared__ int s_list[kNodeTileMax], s_slot[kNodeTileMax], s_cnt;
-  __shared__ double s_g[kNodeTileMax][3];
   __shared__ double s_tred[kBlock];
-  __shared__ SysDesc s_sys[kMaxSys];
+  __shared__ int s_pairs;
 
   // the descriptors live in LDS for the launch: after every grid barrier's
   // acquire they are read without a trip to memory
@@ -797,6 +820,34 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     }
     __syncthreads();
   }
+  // pass pairs: on when every system's halo packs were built (the same
+  // decision in every workgroup); each row workgroup copies its tile's pack
+  // into LDS for the launch
+  unsigned char* const s_halo = reinterpret_cast<unsigned char*>(s_t) + P.halo_off;
+  if constexpr (kPairs) {
+    if (tid == 0) {
+      int ok = P.halo;
+      for (int s = 0; s < nsys && ok; ++s)
+        ok = sys[s].halo != nullptr && sys[s].nbr2 != nullptr && *reinterpret_cast<const int*>(sys[s].halo) == 0;
+      s_pairs = ok;
+    }
+    __syncthreads();
+    if (s_pairs && resF) {
+      const SysDesc& S = sys[sF];
+      const unsigned char* pk = S.halo + kHaloBaseBytes + (size_t)(trF.t0 - S.rt_begin) * P.halo_stride;
+      const int nblk = reinterpret_cast<const int*>(pk)[2];
+      for (int e = tid; e < kHaloOffBlk / 16; e += kBlock)
+        reinterpret_cast<int4*>(s_halo)[e] = reinterpret_cast<const int4*>(pk)[e];
+      // the halo blocks' column slots (ushort pairs), then their 9 value planes
+      for (int e = tid; e < (nblk + 1) / 2; e += kBlock)
+        reinterpret_cast<int*>(s_halo + kHaloOffBlk)[e] = reinterpret_cast<const int*>(pk + kHaloOffBlk)[e];
+      const size_t voff = halo_val_off(P.hbl);
+      for (int e = tid; e < 9 * nblk; e += kBlock)
+        reinterpret_cast<float*>(s_halo + voff)[e] = reinterpret_cast<const float*>(pk + voff)[e];
+      __syncthreads();
+    }
+  }
+
   int zc = 0, pc = 0;
   const int order = P.order;
   double* partE_rr = P.partE;
@@ -961,7 +1012,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // row workgroups of the aligned split take part; the low-rank workgroups
   // idle until the next grid barrier.
   unsigned pseq = 0;
-  auto pass_wait = [&]() -> bool {
+  auto pass_wait = [&](bool two_hop) -> bool {
     pseq += 1;
     if (!resF) return true;
     long long t_before = 0;
@@ -976,7 +1027,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     }
     __syncthreads();
     if (s_st[sF] == ST_RUN) {
-      const int* L = S.nbr + (size_t)tl * kNbrStride;
+      const int* L = (two_hop ? S.nbr2 : S.nbr) + (size_t)tl * kNbrStride;
       const int cnt = L[0];
       const int n = cnt >= 0 ? cnt : S.rt_end - S.rt_begin;
       const long long t0 = rt_ticks();
@@ -1000,9 +1051,173 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     return ok;
   };
 
+  // ---- pass pairs: passes j and j+1 on the resident row tile --------------
+  // The z of the tile's node set (own rows, halo rows, their other columns)
+  // is gathered once into LDS, with the halo rows' r; pass j then runs on
+  // the own rows and the halo rows from LDS, and pass j+1 on the own rows
+  // from pass j's values in LDS, stored to zout.  A halo row is computed
+  // exactly as its owner computes it (same 4-lane split of its blocks, same
+  // order, same helpers), so pass j+1 sees the bits the owner's pass j
+  // produced.  The set reaches two tile steps; the caller waits on the
+  // two-hop neighbour tiles between pairs.
+  // (r is the same in every pass of an iteration: the own rows' r stays in
+  // registers and the halo rows' r in LDS from the first pair on)
+  double pr0 = 0.0, pr1 = 0.0, pr2 = 0.0;
+  auto pair_passes = [&](const SysDesc& S, const VT* zin, VT* zout, VT w1, VT w2, bool wt, bool last2, bool first,
+                         double& rz) {
+    if constexpr (kPairs) {
+      constexpr int R = kHaloTileRows;
+      const int lane = tid & 3, g = tid >> 2;
+      const bool own = rowF < S.nb && lane < 3;
+      const int ro = own ? rowF : 0, lo = own ? lane : 0;
+      const int r0t = rowF - g;  // first row of the tile
+      const int* hdr = reinterpret_cast<const int*>(s_halo);
+      const int nh = hdr[0], nset = hdr[1];
+      const int2* hrows = reinterpret_cast<const int2*>(s_halo + kHaloOffRows);
+      const float* hinv = reinterpret_cast<const float*>(s_halo + kHaloOffInv);
+      const unsigned short* oslot = reinterpret_cast<const unsigned short*>(s_halo + kHaloOffSlot);
+      const int* setn = reinterpret_cast<const int*>(s_halo + kHaloOffSet);
+      const unsigned short* bslot = reinterpret_cast<const unsigned short*>(s_halo + kHaloOffBlk);
+      const float* hval = reinterpret_cast<const float*>(s_halo + halo_val_off(P.hbl));
+      float4* zs = reinterpret_cast<float4*>(s_halo + halo_pack_bytes(P.hbl));
+      float4* rs = zs + kHaloSet;
+      float4* zl = rs + kHaloRows;
+      const auto zg = gbl(zin);
+      const auto rg = gbl(S.r);
+      // gather z at set slots tid and tid + 512 (and, in the first pair of
+      // an iteration, r of the own row and of the halo row at slot tid)
+      {
+        const int k0 = tid, k1 = tid + kBlock;
+        const bool v0 = k0 < nset && (k0 >= R || r0t + k0 < S.nb);
+        const bool v1 = k1 < nset;
+        const int n0 = v0 ? (k0 < R ? r0t + k0 : setn[k0 - R]) : 0;
+        const int n1 = v1 ? setn[k1 - R] : 0;
+        const VT z0a = zg[3 * n0], z0b = zg[3 * n0 + 1], z0c = zg[3 * n0 + 2];
+        const VT z1a = zg[3 * n1], z1b = zg[3 * n1 + 1], z1c = zg[3 * n1 + 2];
+        if (first) {
+          const bool hr0 = k0 >= R && k0 < R + nh;
+          const double h0 = rg[3 * n0], h1 = rg[3 * n0 + 1], h2 = rg[3 * n0 + 2];
+          pr0 = rg[3 * ro];
+          pr1 = rg[3 * ro + 1];
+          pr2 = rg[3 * ro + 2];
+          if (hr0) rs[k0 - R] = make_float4((VT)h0, (VT)h1, (VT)h2, 0.0f);
+        }
+        if (v0) zs[k0] = make_float4(z0a, z0b, z0c, 0.0f);
+        if (v1) zs[k1] = make_float4(z1a, z1b, z1c, 0.0f);
+      }
+      __syncthreads();
+      const double r0 = pr0, r1 = pr1, r2 = pr2;
+      // pass j on the own rows
+      {
+        float4 x[kHaloU];
+#pragma unroll
+        for (int u = 0; u < kHaloU; ++u) x[u] = zs[oslot[u * kBlock + tid]];
+        VT a0 = VT(0), a1 = VT(0), a2 = VT(0);
+#pragma unroll
+        for (int u = 0; u < kHaloU; ++u)
+          blk_step(rrF.p0 + lane + u * G < rrF.p1, rrF.v[u], x[u].x, x[u].y, x[u].z, a0, a1, a2);
+        a0 = group_sum<G>(a0);
+        a1 = group_sum<G>(a1);
+        a2 = group_sum<G>(a2);
+        const float4 zo = zs[g];
+        if (own)
+          reinterpret_cast<float*>(zl + g)[lane] =
+              pass_out(lane == 0 ? zo.x : (lane == 1 ? zo.y : zo.z), w1, rrF.d0, rrF.d1, rrF.d2, r0, r1, r2, a0, a1,
+                       a2);
+      }
+      // pass j on the halo rows (4 lanes per row, rows g and 128 + g)
+#pragma unroll
+      for (int q = 0; q < kHaloRows / R; ++q) {
+        if (q * R >= nh) break;  // (the same for the whole workgroup)
+        const int i = q * R + g;
+        const bool hv = i < nh;
+        const int2 rw = hv ? hrows[i] : make_int2(0, 0);
+        // the blocks' slots, then their z, then their values: the LDS reads
+        // of a level are issued together
+        int pb[kHaloU], sl[kHaloU];
+#pragma unroll
+        for (int u = 0; u < kHaloU; ++u) {
+          const int bi = lane + 4 * u;
+          pb[u] = bi < rw.x ? rw.y + bi : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kHaloU; ++u) sl[u] = bslot[pb[u]];
+        float4 xb[kHaloU];
+#pragma unroll
+        for (int u = 0; u < kHaloU; ++u) xb[u] = zs[sl[u]];
+        VT b0 = VT(0), b1 = VT(0), b2 = VT(0);
+#pragma unroll
+        for (int u = 0; u < kHaloU; ++u) {
+          const float* vp = hval + 9 * (size_t)pb[u];
+          VT v[9];
+#pragma unroll
+          for (int e = 0; e < 9; ++e) v[e] = vp[e];
+          blk_step(lane + 4 * u < rw.x, v, xb[u].x, xb[u].y, xb[u].z, b0, b1, b2);
+        }
+        b0 = group_sum<G>(b0);
+        b1 = group_sum<G>(b1);
+        b2 = group_sum<G>(b2);
+        if (hv && lane < 3) {
+          const float* d = hinv + 9 * i + 3 * lane;
+          const float4 rr = rs[i];
+          const float4 zo = zs[R + i];
+          reinterpret_cast<float*>(zl + R + i)[lane] =
+              pass_out(lane == 0 ? zo.x : (lane == 1 ? zo.y : zo.z), w1, d[0], d[1], d[2], (double)rr.x,
+                       (double)rr.y, (double)rr.z, b0, b1, b2);
+        }
+      }
+      __syncthreads();
+      // pass j+1 on the own rows, from pass j's values
+      float4 x[kHaloU];
+#pragma unroll
+      for (int u = 0; u < kHaloU; ++u) x[u] = zl[oslot[u * kBlock + tid]];
+      const float4 zo2 = zl[g];
+      VT a0 = VT(0), a1 = VT(0), a2 = VT(0);
+#pragma unroll
+      for (int u = 0; u < kHaloU; ++u)
+        blk_step(rrF.p0 + lane + u * G < rrF.p1, rrF.v[u], x[u].x, x[u].y, x[u].z, a0, a1, a2);
+      a0 = group_sum<G>(a0);
+      a1 = group_sum<G>(a1);
+      a2 = group_sum<G>(a2);
+      if (own) {
+        const VT out = pass_out(lane == 0 ? zo2.x : (lane == 1 ? zo2.y : zo2.z), w2, rrF.d0, rrF.d1, rrF.d2, r0, r1,
+                                r2, a0, a1, a2);
+        if (wt)
+          zst<true>(zout + 3 * rowF + lane, out);
+        else
+          zout[3 * rowF + lane] = out;
+        if (last2) rz += pick3(lane, r0, r1, r2) * (double)out;
+      }
+    }
+  };
+
   // ---- Neumann steps C1..C(order-1) -----------------------------------
+  // (in pair mode: pairs (1, 2), (3, 4), ..., then a single last pass when
+  // order - 1 is odd)
   auto neumann_steps = [&]() -> bool {
-    for (int j = 1; j < order; ++j) {
+    for (int j = 1; j < order;) {
+      if (kPairs && s_pairs && j + 1 < order) {
+        const bool last2 = j + 1 == order - 1;
+        double rz = 0.0;
+        if (resF) {
+          const SysDesc& S = sys[sF];
+          if (s_st[sF] == ST_RUN)
+            pair_passes(S, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], step_w(j), step_w(j + 1), !last2, last2, j == 1,
+                        rz);
+          if (last2) {
+            const double c = block_sum_t0(rz, s_red);
+            if (s_st[sF] == ST_RUN && tid == 0) P.partF[(size_t)sF * P.nF + cF] = c;
+          }
+        }
+        zc ^= 1;
+        if (!last2) {
+          if (!pass_wait(true)) return false;
+        } else {
+          if (!sync_at(1)) return false;
+        }
+        j += 2;
+        continue;
+      }
       const bool last = (j == order - 1);
       double rz = 0.0;
       auto flushF = [&](int s) {
@@ -1036,10 +1251,11 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       }
       zc ^= 1;
       if (P.pass_sync && !last) {
-        if (!pass_wait()) return false;
+        if (!pass_wait(kPairs && s_pairs)) return false;
       } else {
         if (!sync_at(1)) return false;
       }
+      ++j;
     }
     return true;
   };
@@ -1184,7 +1400,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       s_rz[s] = 0.0;
     }
     __syncthreads();
-    auto pipe_wait = [&]() -> bool { return P.pass_sync ? pass_wait() : sync_at(1); };
+    auto pipe_wait = [&]() -> bool { return P.pass_sync ? pass_wait(false) : sync_at(1); };
     // W^T v over the coupled nodes of this row workgroup's node tile (v = the
     // last pass's z, or x for a true residual), written to partR for the
     // reduction at the next barrier.  Each wave sums its consecutive nodes in
@@ -1325,8 +1541,13 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
         const SysDesc& S = sys[sF];
         double zdz = 0.0, zq = 0.0;
         if (s_st[sF] == ST_RUN) {
-          const auto zg = gsel<VT>((const VT*)S.z[zc]);
+          const auto zg = gbl((const VT*)S.z[zc]);
+          const auto qg = gbl((const double*)S.q);
           const int lane = tid % G;
+          const bool own = rowF < S.nb && lane < 3;
+          const int io = own ? 3 * rowF + lane : 0;
+          VT zr;
+          double qo;
           VT a0, a1, a2;
           row_spmv_res<G, VT>(S, mat_vals<VT>(S), rrF, lane,
                               [&](int c, VT& x0, VT& x1, VT& x2) {
@@ -1334,13 +1555,16 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                                 x1 = zg[3 * c + 1];
                                 x2 = zg[3 * c + 2];
                               },
+                              [&]() {
+                                zr = zg[io];
+                                qo = qg[io];
+                              },
                               a0, a1, a2);
-          if (rowF < S.nb && lane < 3) {
-            const int idx = 3 * rowF + lane;
+          if (own) {
             const VT dz = pick3(lane, a0, a1, a2);
-            const double zo = (double)zg[idx];
+            const double zo = (double)zr;
             zdz = zo * (double)dz;
-            if (!s_first[sF]) zq = zo * ((const double*)S.q)[idx];
+            if (!s_first[sF]) zq = zo * qo;
             s_dz[3 * lrow + lane] = (double)dz;
           }
         }
@@ -1675,6 +1899,8 @@ out:
       o.status = s_st[s] == ST_DONE ? 0 : (s_st[s] == ST_RUN ? HSC_ERR_DEVICE_TIMEOUT_CODE : s_st[s]);
       o.checks = s_nchk[s];
       o.restarts = s_nrs[s];
+      o.pairs = kPairs ? s_pairs : 0;
+      o.pad_ = 0;
       o.rel = s_rel[s];
       o.normb = s_normb[s];
       P.out[s] = o;
@@ -1926,6 +2152,19 @@ hipError_t launch_pcg_plain(int G, bool mixed, bool res, bool pipe, const PcgPar
   return hipErrorInvalidValue;
 }
 
+template <int G, typename VT, bool RES, bool PIPE = false>
+static hipError_t lds_t(size_t* bytes) {
+  hipFuncAttributes a;
+  const hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(pcg_persistent<G, VT, RES, PIPE>));
+  if (e == hipSuccess) *bytes = a.sharedSizeBytes;
+  return e;
+}
+
+hipError_t pcg_static_lds(int G, bool mixed, bool res, bool pipe, size_t* bytes) {
+  HSC_PCG_DISPATCH(lds_t, bytes);
+  return hipErrorInvalidValue;
+}
+
 static hipError_t pcg_occupancy_query(int G, bool mixed, bool res, bool pipe, size_t shmem, int* blocks) {
   HSC_PCG_DISPATCH(occ_t, blocks, shmem);
   return hipErrorInvalidValue;
@@ -2086,6 +2325,282 @@ hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, i
   const int words = (rtl + 31) / 32;
   hipLaunchKernelGGL(k_tile_nbr_bits, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, nb, R, words, bits);
   hipLaunchKernelGGL(k_tile_nbr_list, dim3(rtl), dim3(kBlock), 0, st, bits, rtl, words, list);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- pass-pair tables
+// Two-hop row-tile neighbour lists: list2[t] = the tiles within two steps of
+// t in the (symmetric) one-step lists, ascending, without t; count -1 (wait
+// on every tile) when a one-step list is -1 or more than kNbrStride - 1
+// tiles result.  One wave per tile.
+__global__ __launch_bounds__(64) void k_tile_nbr2(const int* __restrict__ list, int rtl, int* __restrict__ list2) {
+  __shared__ unsigned s_b[kNbrMaxTiles / 32];
+  __shared__ int s_all;
+  const int t = blockIdx.x, lane = threadIdx.x;
+  for (int w = lane; w < kNbrMaxTiles / 32; w += 64) s_b[w] = 0u;
+  if (lane == 0) s_all = 0;
+  __syncthreads();
+  const int* L = list + (size_t)t * kNbrStride;
+  const int c = L[0];
+  if (c < 0) {
+    if (lane == 0) s_all = 1;
+  } else {
+    for (int i = lane; i < c; i += 64) {
+      const int u = L[1 + i];
+      atomicOr(&s_b[u >> 5], 1u << (u & 31));
+      const int* Lu = list + (size_t)u * kNbrStride;
+      const int cu = Lu[0];
+      if (cu < 0) {
+        s_all = 1;
+      } else {
+        for (int k = 0; k < cu; ++k) {
+          const int v = Lu[1 + k];
+          atomicOr(&s_b[v >> 5], 1u << (v & 31));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s_b[t >> 5] &= ~(1u << (t & 31));
+    int pos = 0;
+    int* out = list2 + (size_t)t * kNbrStride;
+    for (int w = 0; w < (rtl + 31) / 32; ++w) {
+      unsigned bits = s_b[w];
+      while (bits) {
+        const int bi = __ffs(bits) - 1;
+        if (pos < kNbrStride - 1) out[1 + pos] = 32 * w + bi;
+        ++pos;
+        bits &= bits - 1u;
+      }
+    }
+    out[0] = (s_all || pos > kNbrStride - 1) ? -1 : pos;
+  }
+}
+
+// Sorted, distinct node lists through a bitmap over [lo, lo + 32 words): the
+// exclusive prefix of the words' popcounts (contiguous word runs per thread),
+// so a member's rank is pre[w] + popc(bits below it).  Returns the count.
+__device__ int bitmap_prefix(const unsigned* bits, int* pre, int words, int* tsum) {
+  const int tid = threadIdx.x;
+  const int per = (words + kBlock - 1) / kBlock;
+  const int w0 = min(words, tid * per), w1 = min(words, w0 + per);
+  int mine = 0;
+  for (int w = w0; w < w1; ++w) mine += __popc(bits[w]);
+  tsum[tid] = mine;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int i = 0; i < kBlock; ++i) {
+      const int v = tsum[i];
+      tsum[i] = acc;
+      acc += v;
+    }
+    tsum[kBlock] = acc;
+  }
+  __syncthreads();
+  int acc = tsum[tid];
+  for (int w = w0; w < w1; ++w) {
+    pre[w] = acc;
+    acc += __popc(bits[w]);
+  }
+  __syncthreads();
+  return tsum[kBlock];
+}
+__device__ __forceinline__ bool bitmap_has(const unsigned* bits, int lo, int words, int c) {
+  const int o = c - lo;
+  return o >= 0 && (o >> 5) < words && ((bits[o >> 5] >> (o & 31)) & 1u);
+}
+__device__ __forceinline__ int bitmap_rank(const unsigned* bits, const int* pre, int lo, int c) {
+  const int o = c - lo, w = o >> 5;
+  return pre[w] + __popc(bits[w] & ((1u << (o & 31)) - 1u));
+}
+// block min / max / or over the workgroup (every thread gets the result)
+__device__ void block_minmaxor(int& lo, int& hi, int& f, int* red) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    lo = min(lo, __shfl_xor(lo, m, 64));
+    hi = max(hi, __shfl_xor(hi, m, 64));
+    f |= __shfl_xor(f, m, 64);
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[3 * wv] = lo;
+    red[3 * wv + 1] = hi;
+    red[3 * wv + 2] = f;
+  }
+  __syncthreads();
+  lo = red[0];
+  hi = red[1];
+  f = red[2];
+  for (int w = 1; w < kBlock / 64; ++w) {
+    lo = min(lo, red[3 * w]);
+    hi = max(hi, red[3 * w + 1]);
+    f |= red[3 * w + 2];
+  }
+  __syncthreads();
+}
+
+// Halo pack of row tile t (layout in common.hpp); one workgroup per tile.
+// H1 (the halo rows) = the distinct columns outside the tile that its rows
+// reference, ascending; H2 = the distinct columns of the halo rows outside
+// the tile and H1, ascending.  A tile that does not fit (more than kHaloRows
+// halo rows or kHaloSet set slots or hbl halo blocks, a row of more than
+// 4 kHaloU blocks, a column range past a bitmap) sets the system's fail
+// word, and the solve keeps one synchronisation per pass.
+constexpr int kHaloBitmapWords = 4096;  // column range of 131,072 nodes
+__global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                      const float* __restrict__ val32,
+                                                      const float* __restrict__ invD32, int nb, int hbl,
+                                                      unsigned char* __restrict__ base, long long stride) {
+  __shared__ unsigned s_b1[kHaloBitmapWords], s_b2[kHaloBitmapWords];
+  __shared__ int s_p1[kHaloBitmapWords], s_p2[kHaloBitmapWords];
+  __shared__ int s_tsum[kBlock + 1];
+  __shared__ int s_h1[kHaloRows], s_len[kHaloRows], s_lp0[kHaloRows];
+  __shared__ int s_red[3 * (kBlock / 64)];
+  constexpr int R = kHaloTileRows;
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int r0 = t * R, r1 = min(nb, r0 + R);
+  const int p0 = rowptr[r0], p1 = rowptr[r1];
+  int* fail = reinterpret_cast<int*>(base);
+  unsigned char* pk = base + kHaloBaseBytes + (size_t)t * stride;
+  auto give_up = [&]() {
+    if (tid == 0) atomicOr(fail, 1);
+  };
+
+  // H1: columns of the tile's rows outside the tile
+  int lo1 = 0x7fffffff, hi1 = -1, bad = 0;
+  for (int p = p0 + tid; p < p1; p += kBlock) {
+    const int c = colidx[p];
+    if (c < r0 || c >= r1) {
+      lo1 = min(lo1, c);
+      hi1 = max(hi1, c);
+    }
+  }
+  for (int r = r0 + tid; r < r1; r += kBlock) bad |= (rowptr[r + 1] - rowptr[r] > 4 * kHaloU) ? 1 : 0;
+  block_minmaxor(lo1, hi1, bad, s_red);
+  const int words1 = hi1 >= lo1 ? (hi1 - lo1 + 1 + 31) / 32 : 0;
+  if (bad || words1 > kHaloBitmapWords) return give_up();
+  for (int w = tid; w < words1; w += kBlock) s_b1[w] = 0u;
+  __syncthreads();
+  for (int p = p0 + tid; p < p1; p += kBlock) {
+    const int c = colidx[p];
+    if (c < r0 || c >= r1) atomicOr(&s_b1[(c - lo1) >> 5], 1u << ((c - lo1) & 31));
+  }
+  __syncthreads();
+  const int nh = bitmap_prefix(s_b1, s_p1, words1, s_tsum);
+  if (nh > kHaloRows) return give_up();
+  for (int w = tid; w < words1; w += kBlock) {
+    unsigned bits = s_b1[w];
+    int idx = s_p1[w];
+    while (bits) {
+      const int bi = __ffs(bits) - 1;
+      s_h1[idx++] = lo1 + 32 * w + bi;
+      bits &= bits - 1u;
+    }
+  }
+  __syncthreads();
+  // halo rows' lengths, their blocks' offsets, and H2's range
+  int lo2 = 0x7fffffff, hi2 = -1;
+  bad = 0;
+  if (tid < nh) {
+    const int h = s_h1[tid];
+    const int q0 = rowptr[h], q1 = rowptr[h + 1];
+    bad = q1 - q0 > 4 * kHaloU;
+    s_len[tid] = q1 - q0;
+    for (int p = q0; p < q1; ++p) {
+      const int c = colidx[p];
+      if ((c < r0 || c >= r1) && !bitmap_has(s_b1, lo1, words1, c)) {
+        lo2 = min(lo2, c);
+        hi2 = max(hi2, c);
+      }
+    }
+  }
+  block_minmaxor(lo2, hi2, bad, s_red);
+  const int words2 = hi2 >= lo2 ? (hi2 - lo2 + 1 + 31) / 32 : 0;
+  if (bad || words2 > kHaloBitmapWords) return give_up();
+  if (tid == 0) {
+    int acc = 0;
+    for (int i = 0; i < nh; ++i) {
+      s_lp0[i] = acc;
+      acc += s_len[i];
+    }
+    s_tsum[0] = acc;
+  }
+  for (int w = tid; w < words2; w += kBlock) s_b2[w] = 0u;
+  __syncthreads();
+  const int nblk = s_tsum[0];
+  if (nblk > hbl) return give_up();
+  if (tid < nh) {
+    const int h = s_h1[tid];
+    for (int p = rowptr[h]; p < rowptr[h + 1]; ++p) {
+      const int c = colidx[p];
+      if ((c < r0 || c >= r1) && !bitmap_has(s_b1, lo1, words1, c))
+        atomicOr(&s_b2[(c - lo2) >> 5], 1u << ((c - lo2) & 31));
+    }
+  }
+  __syncthreads();
+  const int n2 = bitmap_prefix(s_b2, s_p2, words2, s_tsum);
+  const int nset = R + nh + n2;
+  if (nset > kHaloSet) return give_up();
+  auto slot_of = [&](int c) -> int {
+    if (c >= r0 && c < r1) return c - r0;
+    if (bitmap_has(s_b1, lo1, words1, c)) return R + bitmap_rank(s_b1, s_p1, lo1, c);
+    return R + nh + bitmap_rank(s_b2, s_p2, lo2, c);
+  };
+  // the pack
+  if (tid == 0) {
+    int* hdr = reinterpret_cast<int*>(pk);
+    hdr[0] = nh;
+    hdr[1] = nset;
+    hdr[2] = nblk;
+    hdr[3] = 0;
+  }
+  int* setn = reinterpret_cast<int*>(pk + kHaloOffSet);
+  for (int w = tid; w < words2; w += kBlock) {
+    unsigned bits = s_b2[w];
+    int idx = nh + s_p2[w];
+    while (bits) {
+      const int bi = __ffs(bits) - 1;
+      setn[idx++] = lo2 + 32 * w + bi;
+      bits &= bits - 1u;
+    }
+  }
+  if (tid < nh) {
+    const int h = s_h1[tid];
+    setn[tid] = h;
+    reinterpret_cast<int2*>(pk + kHaloOffRows)[tid] = make_int2(s_len[tid], s_lp0[tid]);
+    float* inv = reinterpret_cast<float*>(pk + kHaloOffInv) + 9 * tid;
+    for (int e = 0; e < 9; ++e) inv[e] = invD32[9 * (size_t)h + e];
+    unsigned short* bslot = reinterpret_cast<unsigned short*>(pk + kHaloOffBlk);
+    float* val = reinterpret_cast<float*>(pk + halo_val_off(hbl));
+    const int lp0 = s_lp0[tid], q0 = rowptr[h];
+    for (int b = 0; b < s_len[tid]; ++b) {
+      bslot[lp0 + b] = (unsigned short)slot_of(colidx[q0 + b]);
+      for (int e = 0; e < 9; ++e) val[9 * (size_t)(lp0 + b) + e] = val32[val_index(q0 + b, e)];
+    }
+  }
+  // set slot of the column of block u of each own-row lane (4 lanes per row);
+  // a lane past its row's end gathers its own row (value 0 in the sum)
+  {
+    const int g = tid >> 2, l = tid & 3, row = r0 + g;
+    unsigned short* oslot = reinterpret_cast<unsigned short*>(pk + kHaloOffSlot);
+    const int q0 = row < r1 ? rowptr[row] : 0, q1 = row < r1 ? rowptr[row + 1] : 0;
+    for (int u = 0; u < kHaloU; ++u) {
+      const int p = q0 + l + 4 * u;
+      oslot[u * kBlock + tid] = (unsigned short)(p < q1 ? slot_of(colidx[p]) : g);
+    }
+  }
+}
+
+hipError_t launch_tile_pairs(const int* rowptr, const int* colidx, const float* val32, const float* invD32, int nb,
+                             const int* list, int* list2, int hbl, unsigned char* halo, hipStream_t st) {
+  const int rtl = (nb + kHaloTileRows - 1) / kHaloTileRows;
+  if (rtl <= 0 || rtl > kNbrMaxTiles || hbl < 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned*>(halo),
+                     (long long)(kHaloBaseBytes / 4));
+  hipLaunchKernelGGL(k_tile_nbr2, dim3(rtl), dim3(64), 0, st, list, rtl, list2);
+  hipLaunchKernelGGL(k_tile_halo, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, val32, invD32, nb, hbl, halo,
+                     halo_pack_bytes(hbl));
   return hipGetLastError();
 }
 

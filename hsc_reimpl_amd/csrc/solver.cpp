@@ -122,6 +122,7 @@ constexpr long long kTimeoutTicks = 100LL * 1000 * 1000 * 20;  // 20 s at 100 MH
 
 struct DeviceInfo {
   int cu = 0;
+  int lds_max = 65536;  // LDS bytes one workgroup may allocate
 };
 
 DeviceInfo device_info(int dev) {
@@ -132,7 +133,10 @@ DeviceInfo device_info(int dev) {
   if (it != cache.end()) return it->second;
   hipDeviceProp_t prop;
   DeviceInfo d;
-  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) d.cu = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    d.cu = prop.multiProcessorCount;
+    if (prop.sharedMemPerBlock > 0) d.lds_max = (int)prop.sharedMemPerBlock;
+  }
   if (d.cu <= 0) d.cu = 1;
   cache[dev] = d;
   return d;
@@ -155,16 +159,16 @@ std::map<TuneKey, TuneVal> g_tune;
 // the file the table is loaded from and appended to ("" = memory only)
 std::string g_tune_path;
 bool g_tune_path_set = false, g_tune_loaded = false;
-constexpr const char* kTuneMagic = "hsc-tune-v2";
+constexpr const char* kTuneMagic = "hsc-tune-v3";
 
 std::string default_tune_path() {
   if (const char* e = std::getenv("HSC_TUNE_FILE")) return e;
-  if (const char* home = std::getenv("HOME")) return std::string(home) + "/.cache/hsc_mi355x/tune_v2.txt";
+  if (const char* home = std::getenv("HOME")) return std::string(home) + "/.cache/hsc_mi355x/tune_v3.txt";
   return "";
 }
 
 // Load the table file once (caller holds g_tune_mu).  Lines:
-// "hsc-tune-v2 cu nb_log2 deg kbucket mixed batch poly order psync G wgpcu sched";
+// "hsc-tune-v3 cu nb_log2 deg kbucket mixed batch poly order psync G wgpcu sched";
 // anything else is skipped.
 void load_tune_locked() {
   if (g_tune_loaded) return;
@@ -222,10 +226,15 @@ struct hsc_handle {
   int G = 8, wgpcu = 1, sched = 1;
   bool pass_sync_used = false;  // the last PCG launch synchronised its passes per tile
   bool pipe_used = false;       // the last PCG launch ran the pipelined loop
+  bool pairs_used = false;      // the last PCG launch ran pass pairs (set up for them; after a solve, as run)
   int max_coupled128 = 0;  // most coupled nodes in one 128-node tile
   // row-tile neighbour lists for 128, 64 and 32 rows per tile (lanes per row 4, 8, 16)
   Buf tnbr[3], tnbr_bits;
   bool tnbr_ok[3] = {false, false, false};
+  // pass pairs (128-row tiles): two-hop neighbour lists and halo packs, built
+  // for pack capacity pairs_hbl (-1: not built since the last setup)
+  Buf tnbr2, halo;
+  int pairs_hbl = -1;
   double lambda_max = 0.0;  // power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights)
   double omegas[hsc::kMaxPolySteps] = {};  // per-pass preconditioner weights
   double* xsol = nullptr;  // FEM part of the caller's x during a solve
@@ -310,8 +319,8 @@ int validate_config(const hsc_config& c, std::string& why) {
     why = "Chebyshev preconditioner: neumann_order <= 32 and poly_interval_ratio > 1 required";
     return HSC_ERR_INVALID_ARGUMENT;
   }
-  if (c.pass_sync < 0 || c.pass_sync > 1) {
-    why = "pass_sync must be 0 or 1";
+  if (c.pass_sync < 0 || c.pass_sync > 2) {
+    why = "pass_sync must be 0, 1 or 2";
     return HSC_ERR_INVALID_ARGUMENT;
   }
   if (c.pipelined < 0 || c.pipelined > 1) {
@@ -454,22 +463,26 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
       wc_rows = std::max(wc_rows, hs[s]->max_coupled128);
       kmax = std::max(kmax, hs[s]->k);
     }
-    wc_bytes = (size_t)wc_rows * 3 * kmax * (mixed ? sizeof(float) : sizeof(double));
+    wc_bytes = ((size_t)wc_rows * 3 * kmax * (mixed ? sizeof(float) : sizeof(double)) + 15) / 16 * 16;
     if (wc_rows == 0 || kmax == 0 || wc_bytes > 64 * 1024) {
       wc_rows = 0;
       wc_bytes = 0;
     }
   }
+  // dynamic LDS starts with the system descriptors and the node tile's
+  // low-rank sums
+  const size_t sys_bytes = (sizeof(hsc::SysDesc) * nsys + 15) / 16 * 16;
+  const size_t sg_bytes = ((size_t)node_tile * 3 * sizeof(double) + 15) / 16 * 16;
   // the pipelined loop (requested by the config) runs when every row tile
   // and its 128 nodes get their own workgroup: 4 lanes per row, latency
   // split, polynomial passes, recursive-residual stopping
   const bool pipe_req = h0->cfg.pipelined && G == 4 && lc.sched != 2 && node_tile == R && h0->cfg.neumann_order >= 2 &&
                         !h0->cfg.true_residual_every_iteration && hsc::kPipeRows == R;
-  size_t shmem = t_bytes + wc_bytes;
+  size_t shmem = sys_bytes + sg_bytes + t_bytes + wc_bytes;
   int occ = 0;
   HIPCHK(hsc::pcg_occupancy(G, mixed, lc.sched != 2, false, shmem, &occ));
   // (the pipelined loop keeps two more t vectors in LDS: W^T z and W^T x)
-  const size_t shmem_pipe = 3 * t_bytes + wc_bytes;
+  const size_t shmem_pipe = sys_bytes + sg_bytes + 3 * t_bytes + wc_bytes;
   int occ_pipe = 0;
   if (pipe_req) HIPCHK(hsc::pcg_occupancy(G, mixed, true, true, shmem_pipe, &occ_pipe));
   if (pipe_req) occ = std::min(occ, occ_pipe);
@@ -557,9 +570,51 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   sc.nflags = psync ? rt : 0;
   plan.pipe = pipe;
   if (pipe) shmem = shmem_pipe;
+  // pass pairs (pass_sync = 2): one system, FP32 resident rows at 4 lanes per
+  // row, at least two passes.  The halo region takes the LDS the kernel
+  // leaves free, and the pack capacity (hbl blocks) follows from it.
+  bool pairs = false;
+  int hbl = 0;
+  const size_t halo_off = t_bytes + wc_bytes;  // relative to the t vectors (16-byte aligned)
+  if (psync && !pipe && h0->cfg.pass_sync == 2 && nsys == 1 && G == 4 && mixed && R == hsc::kHaloTileRows &&
+      h0->cfg.neumann_order >= 3) {
+    size_t stat = 0;
+    HIPCHK(hsc::pcg_static_lds(G, mixed, true, false, &stat));
+    const long long avail = (long long)di.lds_max - (long long)stat - (long long)(sys_bytes + sg_bytes + halo_off) -
+                            hsc::kHaloScratchBytes - hsc::kHaloOffBlk - 512;
+    hbl = (int)std::min<long long>(avail / 38, (long long)4 * hsc::kHaloU * hsc::kHaloRows);
+    while (hbl > 0 && (long long)(sys_bytes + sg_bytes + halo_off) + hsc::halo_pack_bytes(hbl) + hsc::kHaloScratchBytes +
+                              (long long)stat > (long long)di.lds_max)
+      --hbl;
+    if (hbl >= 64) {
+      const size_t shmem_pairs =
+          sys_bytes + sg_bytes + halo_off + (size_t)hsc::halo_pack_bytes(hbl) + hsc::kHaloScratchBytes;
+      int occ_pairs = 0;
+      HIPCHK(hsc::pcg_occupancy(G, mixed, true, false, shmem_pairs, &occ_pairs));
+      if (occ_pairs >= 1 && nwg <= std::min(lc.wgpcu, occ_pairs) * di.cu) {
+        pairs = true;
+        shmem = shmem_pairs;
+      }
+    }
+  }
+  if (pairs) {
+    hsc_handle* h = h0;
+    if (h->pairs_hbl != hbl) {
+      const int rtl = (h->nb + R - 1) / R;
+      HIPCHK(h->tnbr2.ensure(sizeof(int) * (size_t)rtl * hsc::kNbrStride));
+      HIPCHK(h->halo.ensure((size_t)hsc::kHaloBaseBytes + (size_t)rtl * (size_t)hsc::halo_pack_bytes(hbl)));
+      HIPCHK(hsc::launch_tile_pairs(h->rowptr.as<int>(), h->colidx.as<int>(), h->val32.as<float>(),
+                                    h->invD32.as<float>(), h->nb, h->tnbr[0].as<int>(), h->tnbr2.as<int>(), hbl,
+                                    h->halo.as<unsigned char>(), st));
+      h->pairs_hbl = hbl;
+    }
+    S[0].nbr2 = h->tnbr2.as<int>();
+    S[0].halo = h->halo.as<unsigned char>();
+  }
   for (int s = 0; s < nsys; ++s) {
     hs[s]->pass_sync_used = psync;
     hs[s]->pipe_used = pipe;
+    hs[s]->pairs_used = pairs;
   }
   HIPCHK(sc.sysdesc.ensure(sizeof(hsc::SysDesc) * nsys));
   HIPCHK(sc.sysout.ensure(sizeof(hsc::SysOut) * nsys));
@@ -590,6 +645,12 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   hsc::PcgParams& P = plan.P;
   P.pass_sync = psync ? 1 : 0;
   P.pass_flags = psync ? sc.flags.as<unsigned>() : nullptr;
+  P.sys_bytes = (int)sys_bytes;
+  P.sg_bytes = (int)sg_bytes;
+  P.halo = pairs ? 1 : 0;
+  P.halo_off = (int)halo_off;
+  P.hbl = hbl;
+  P.halo_stride = pairs ? hsc::halo_pack_bytes(hbl) : 0;
   P.sys = sc.sysdesc.as<hsc::SysDesc>();
   P.out = sc.sysout.as<hsc::SysOut>();
   P.nsys = nsys;
@@ -687,6 +748,8 @@ std::vector<hsc::SysOut> collect_pcg(PcgScratch& sc, int nsys, hipStream_t st, h
   std::vector<hsc::SysOut> out(nsys);
   std::memcpy(out.data(), hb, out_bytes);
   if (fin_hs && fin) fin->assign(hfin, hfin + 3 * (size_t)nsys);
+  if (fin_hs)
+    for (int i = 0; i < nsys; ++i) fin_hs[i]->pairs_used = out[i].pairs != 0;
   return out;
 }
 
@@ -720,6 +783,7 @@ std::vector<hsc::SysOut> run_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, 
   }
   if (after) after();
   std::vector<hsc::SysOut> out = collect_pcg(sc, nsys, st, fin ? hs : nullptr, fin);
+  for (int s = 0; s < nsys; ++s) hs[s]->pairs_used = out[s].pairs != 0;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   if (pcg_ms) *pcg_ms = ms;
@@ -782,7 +846,7 @@ TuneKey tune_key(hsc_handle* const* hs, int nsys) {
                  nsys > 1 ? 1 : 0,
                  c.preconditioner == HSC_PRECOND_NEUMANN ? 0 : 1,
                  c.neumann_order,
-                 c.pass_sync | (c.pipelined << 1)};
+                 c.pass_sync | (c.pipelined << 2)};
 }
 
 bool pinned(const hsc_config& c) { return c.lanes_per_row && c.wg_per_cu && c.schedule; }
@@ -1016,6 +1080,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   h->launch_ready = false;
   ++h->gen;
   for (bool& ok : h->tnbr_ok) ok = false;
+  h->pairs_hbl = -1;
   if (!Av) throw HscError{HSC_ERR_INVALID_ARGUMENT, "null matrix view"};
   const long long N = 3 * Av->block_dim;
   // linear_hsc.cu:197-200
@@ -1562,7 +1627,7 @@ void hsc_config_default(hsc_config* c) {
   c->preconditioner = HSC_PRECOND_NEUMANN;
   c->check_acceptance = 1;
   c->poly_interval_ratio = 100.0;
-  c->pass_sync = 1;
+  c->pass_sync = 2;
   c->pipelined = 0;
 }
 
@@ -2159,6 +2224,7 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   o->schedule = h->sched;
   o->pass_sync = h->pass_sync_used ? 1 : 0;
   o->pipelined = h->pipe_used ? 1 : 0;
+  o->pass_pairs = h->pairs_used ? 1 : 0;
   o->lambda_max = h->lambda_max;
   o->neumann_doublings = h->doublings;
   o->abd_block_diagonal = h->blockdiag ? 1 : 0;

@@ -111,9 +111,13 @@ typedef struct hsc_config {
                                     SpMV row tiles, 128-node elementwise tiles); 2 = throughput (grid covers
                                     row and low-rank tiles, split by tile count, 512-node tiles) */
   double poly_interval_ratio;    /* HSC_PRECOND_CHEBYSHEV: b / a of the fitted interval (100) */
-  int32_t pass_sync;             /* latency schedule, order > 2: 1 (default) = each row tile waits only for
-                                    its neighbour tiles between preconditioner passes; 0 = grid barriers.
-                                    Same arithmetic either way (bitwise-equal results) */
+  int32_t pass_sync;             /* latency schedule, order > 2: 1 = each row tile waits only for its
+                                    neighbour tiles between preconditioner passes; 0 = grid barriers;
+                                    2 (default) = pass pairs where they apply (one system, mixed precision,
+                                    4 lanes per row, rows of at most 20 blocks): each row tile also computes
+                                    its halo rows' pass in LDS and waits on its two-hop neighbour tiles once
+                                    per two passes, else as 1.
+                                    Same arithmetic in every mode (bitwise-equal results) */
   int32_t pipelined;             /* 1 = the pipelined PCG loop where it applies (latency schedule, 4 lanes per
                                     row, Chebyshev/learned or Neumann passes of order >= 2, recursive-residual
                                     stopping): one grid barrier per iteration; q = D p kept by recurrence in
@@ -207,7 +211,7 @@ int32_t hsc_append_jsonl(const hsc_stats* s, const char* path, int32_t frame, in
  * The tune table maps a size class (nodes, blocks per row, rank, precision,
  * batch, preconditioner, order, pass sync, device) to the fastest persistent
  * PCG launch shape.  It is loaded from and appended to the file named here
- * (default: $HSC_TUNE_FILE, else $HOME/.cache/hsc_mi355x/tune_v2.txt; NULL or
+ * (default: $HSC_TUNE_FILE, else $HOME/.cache/hsc_mi355x/tune_v3.txt; NULL or
  * "" = keep it in memory only).  hsc_tune runs the timing for h's current
  * system now (hsc_setup does it too when the class is new).  hsc_alloc_count
  * is the number of device and pinned-host allocations the library has made
@@ -250,7 +254,8 @@ typedef struct hsc_setup_info {
   int32_t learned_used;          /* 1 when the network's interval is in use */
   int32_t fallback_reason;       /* 2 when the network's output was rejected (Neumann series used) */
   int32_t pipelined;             /* 1 when the last solve's launch ran the pipelined PCG loop */
-  int32_t reserved1;
+  int32_t pass_pairs;            /* 1 when the last solve ran its preconditioner passes in pairs (pass_sync 2,
+                                    and every row tile's halo fit: otherwise one wait per pass) */
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 

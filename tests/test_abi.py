@@ -45,7 +45,7 @@ def test_abi_version_and_defaults(lib):
     assert c.abd_inverse == _abi.ABD_INVERSE["neumann"] and c.precision == _abi.PRECISION["fp64"]
     assert c.preconditioner == _abi.PRECONDITIONER["neumann"]
     # extensions: autotuned schedule, Chebyshev interval ratio, neighbour-tile pass sync
-    assert (c.schedule, c.poly_interval_ratio, c.pass_sync) == (0, 100.0, 1)
+    assert (c.schedule, c.poly_interval_ratio, c.pass_sync) == (0, 100.0, 2)
 
 
 def test_invalid_config_rejected(lib):
@@ -57,7 +57,7 @@ def test_invalid_config_rejected(lib):
     lib.hsc_config_default(C.byref(c))
     c.neumann_order = 0
     assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
-    for field, bad in (("schedule", 3), ("pass_sync", 2), ("pass_sync", -1), ("lanes_per_row", 6)):
+    for field, bad in (("schedule", 3), ("pass_sync", 3), ("pass_sync", -1), ("lanes_per_row", 6)):
         lib.hsc_config_default(C.byref(c))
         setattr(c, field, bad)
         assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT, field
@@ -145,7 +145,7 @@ def test_alloc_count_and_tune_file_without_a_device(lib, tmp_path):
     # of another format are skipped) or an absent one is tolerated
     assert lib.hsc_alloc_count() >= 0
     f = tmp_path / "tune.txt"
-    f.write_text("garbage line\nhsc-tune-v2 256 14 11 3 1 0 1 16 1 4 2 1\n")
+    f.write_text("garbage line\nhsc-tune-v3 256 14 11 3 1 0 1 16 1 4 2 1\n")
     assert lib.hsc_tune_file(str(f).encode()) == 0
     assert lib.hsc_tune_file(str(tmp_path / "absent.txt").encode()) == 0
     assert lib.hsc_tune_file(None) == 0

@@ -180,8 +180,8 @@ def test_pass_sync_batch_bitwise():
 @pytest.mark.parametrize("shape", ["tuned", "pinned"])
 def test_benchmarked_solver_on_vehicle_system(shape):
     """The configuration bench.py reports as `value`, on the 78k-tet system:
-    mixed precision, 16 Chebyshev-weighted passes on [b/1000, b], neighbour
-    pass sync, the tuner's launch shape (and the latency shape the tuner
+    mixed precision, 16 Chebyshev-weighted passes on [b/1000, b], pass pairs
+    (the default pass sync), the tuner's launch shape (and the latency shape the tuner
     picked when it was measured: 4 lanes per row, 2 workgroups per CU,
     schedule 1).  The true Schur residual (FP64, recomputed on the CPU from
     the oracle's operators) is <= 1e-8, x is within 1e-7 of the FP64 sparse
@@ -200,6 +200,7 @@ def test_benchmarked_solver_on_vehicle_system(shape):
     assert st["status"] == 0 and st["pcg_iterations"] == pytest.approx(43, abs=2), st["pcg_iterations"]
     if shape == "pinned":
         assert info["pass_sync"] == 1 and (info["lanes_per_row"], info["schedule"]) == (4, 1)
+        assert info["pass_pairs"] == 1  # the default pass_sync = 2 runs the passes in pairs
     ref = osp.SparseHSC.from_system(s, order=5, omega=0.7)
     n = s.fn
     rhs = ref.reduced_rhs(s.b[:n], s.b[n:])
