@@ -12,15 +12,6 @@ __device__ __forceinline__ long long rt_ticks() {
   return (long long)__builtin_amdgcn_s_memrealtime();
 }
 
-// Butterfly sum over the 64 lanes; every lane gets the bitwise-same value
-// (each step adds two commutative partial sums).
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
-
 // v of lane (lane ^ M) for M in {1, 2, 4, 8}, through DPP lane moves inside
 // the VALU (no LDS permute round trip): quad_perm for 1 and 2; 4 = half-row
 // mirror (lane ^ 7) then the quad reversal (lane ^ 3); 8 = row mirror
@@ -64,6 +55,16 @@ __device__ __forceinline__ T group_sum(T v) {
   if constexpr (G >= 8) v += dpp_xor<4>(v);
   if constexpr (G >= 4) v += dpp_xor<2>(v);
   if constexpr (G >= 2) v += dpp_xor<1>(v);
+  return v;
+}
+
+// Butterfly sum over the 64 lanes; every lane gets the bitwise-same value
+// (each step adds two commutative partial sums).  (LDS permutes: DPP moves
+// measured slower here in FP64, as the reductions' critical path.)
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
   return v;
 }
 
