@@ -687,12 +687,15 @@ def main():
                    "active_fem_dofs": a, "rank": k, "neumann_order": order, "omega": omega, "rtol": rtol,
                    "new_passes": NEW_PASSES, "new_interval_ratio": round(info_new["poly_ratio"], 1),
                    "new_interval_from": "learned graph MLP", "pass_sync": info_new["pass_sync"],
+                   "pass_pairs": info_new["pass_pairs"],
                    "lanes_per_row": info_new["lanes_per_row"], "wg_per_cu": info_new["workgroups"],
                    "schedule": info_new["schedule"]},
         "new": "learned + mixed precision: FP32 bulk, FP64 reductions and true residual every 25; 16 block-"
                "Jacobi passes weighted by the Chebyshev roots of [b/ratio, b] in Leja order, the ratio picked at "
-               "setup by the graph MLP from D's node features, b = 1.2 x a power-iteration bound; neighbour-tile "
-               "flags between passes",
+               "setup by the graph MLP from D's node features, b = 1.2 x a power-iteration bound; passes in pairs: "
+               "each row tile gathers its node set once, computes its own and halo rows' pass in LDS, then the next "
+               "pass; between pairs the tiles exchange tagged pass values (value and round in one atomic word) "
+               "instead of waiting on flags",
         "pcg_iterations": {"new": new["pcg_iterations"], "plain_fp64": plain["pcg_iterations"],
                            "mixed_chebyshev_1000": cheb["pcg_iterations"],
                            "mixed_neumann": mixed["pcg_iterations"], "fp64_chebyshev": cheb64["pcg_iterations"]},

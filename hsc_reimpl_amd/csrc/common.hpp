@@ -23,8 +23,8 @@ constexpr int kNbrMaxTiles = 1024;   // row tiles per system for which neighbour
 // Pass pairs (FP32 resident rows, 4 lanes per row, 128-row tiles): a row
 // tile computes pass j on its rows and on its halo rows (the rows of other
 // tiles that its rows reference), keeps both in LDS and computes pass j+1 on
-// its rows from there, so it synchronises with its neighbours once per two
-// passes.  The tile's node set lists the nodes those passes read: its own
+// its rows from there, so it reads its neighbours' values once per two
+// passes (as tagged words, see the kernel).  The tile's node set lists the nodes those passes read: its own
 // rows (slots 0..127), its halo rows (128..127+nh), then the other columns
 // of the halo rows; each is gathered once per pair.  Per tile, a halo pack in
 // device memory is the image of the LDS region the kernel copies it to at
@@ -102,8 +102,8 @@ struct SysDesc {
   void* q;
   double* q64;            // FP64 D x for the true residual (== q in FP64 mode)
   const int* nbr;         // [row tiles * kNbrStride] neighbour lists (local tile ids), or null
-  const int* nbr2;        // two-hop neighbour lists (pass pairs), same layout, or null
   const unsigned char* halo;  // pass pairs: kHaloBaseBytes header, then one halo pack per row tile, or null
+  unsigned long long* zt;     // pass pairs: two buffers of 3 nb tagged pass values (FP32 bits | round << 32)
   double tol;
   int max_iter;
   int pad_;

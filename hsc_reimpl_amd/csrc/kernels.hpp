@@ -173,11 +173,10 @@ hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st);
 // list: rtl*kNbrStride ints, rtl = ceil(nb/R) <= kNbrMaxTiles)
 hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, int R, unsigned* bits, int* list,
                                  hipStream_t st);
-// pass pairs of a system (128-row tiles): two-hop lists list2 from the
-// one-hop lists, and the halo packs (kHaloBaseBytes header with the fail
-// word, then halo_pack_bytes(hbl) per tile) into halo
+// pass pairs of a system (128-row tiles): the halo packs (kHaloBaseBytes
+// header with the fail word, then halo_pack_bytes(hbl) per tile) into halo
 hipError_t launch_tile_pairs(const int* rowptr, const int* colidx, const float* val32, const float* invD32, int nb,
-                             const int* list, int* list2, int hbl, unsigned char* halo, hipStream_t st);
+                             int hbl, unsigned char* halo, hipStream_t st);
 // static LDS bytes of the persistent PCG kernel variant
 hipError_t pcg_static_lds(int G, bool mixed, bool res, bool pipe, size_t* bytes);
 // power iteration: z = Dinv D x / |x| with |x|^2 the sum of part_in[0:np_in];

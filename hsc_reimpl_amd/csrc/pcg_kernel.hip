@@ -828,7 +828,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     if (tid == 0) {
       int ok = P.halo;
       for (int s = 0; s < nsys && ok; ++s)
-        ok = sys[s].halo != nullptr && sys[s].nbr2 != nullptr && *reinterpret_cast<const int*>(sys[s].halo) == 0;
+        ok = sys[s].halo != nullptr && sys[s].zt != nullptr && *reinterpret_cast<const int*>(sys[s].halo) == 0;
       s_pairs = ok;
     }
     __syncthreads();
@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // row workgroups of the aligned split take part; the low-rank workgroups
   // idle until the next grid barrier.
   unsigned pseq = 0;
-  auto pass_wait = [&](bool two_hop) -> bool {
+  auto pass_wait = [&]() -> bool {
     pseq += 1;
     if (!resF) return true;
     long long t_before = 0;
@@ -1027,7 +1027,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
     }
     __syncthreads();
     if (s_st[sF] == ST_RUN) {
-      const int* L = (two_hop ? S.nbr2 : S.nbr) + (size_t)tl * kNbrStride;
+      const int* L = S.nbr + (size_t)tl * kNbrStride;
       const int cnt = L[0];
       const int n = cnt >= 0 ? cnt : S.rt_end - S.rt_begin;
       const long long t0 = rt_ticks();
@@ -1055,16 +1055,30 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // The z of the tile's node set (own rows, halo rows, their other columns)
   // is gathered once into LDS, with the halo rows' r; pass j then runs on
   // the own rows and the halo rows from LDS, and pass j+1 on the own rows
-  // from pass j's values in LDS, stored to zout.  A halo row is computed
-  // exactly as its owner computes it (same 4-lane split of its blocks, same
-  // order, same helpers), so pass j+1 sees the bits the owner's pass j
-  // produced.  The set reaches two tile steps; the caller waits on the
-  // two-hop neighbour tiles between pairs.
+  // from pass j's values in LDS.  A halo row is computed exactly as its
+  // owner computes it (same 4-lane split of its blocks, same order, same
+  // helpers), so pass j+1 sees the bits the owner's pass j produced.
+  //
+  // Between pairs the tiles exchange tagged values instead of waiting on
+  // each other: a pass value is published as one 64-bit atomic word (its
+  // FP32 bits, the round number above them), stored write-through without
+  // waiting, and a pair's gather re-reads each word it needs until the tag
+  // is the round it expects.  The word is single-copy atomic, so a matching
+  // tag proves its value.  Rounds alternate between two buffers, and a tile
+  // cannot overwrite a buffer before every reader is done with it: a tile
+  // that reads this tile's rows has a row in this tile's node set (the set
+  // reaches two steps, and the pattern is symmetric), so this tile's next
+  // gather waits for that reader's output of the round that followed its read.
+  // zin: plain input (the first step of an iteration, after a grid barrier),
+  // else the tagged input zti of round rin; zout: plain output (the last
+  // pair), else the tagged output zto of round rout.  Returns false when a
+  // bounded wait expired (error flag set).
   // (r is the same in every pass of an iteration: the own rows' r stays in
   // registers and the halo rows' r in LDS from the first pair on)
   double pr0 = 0.0, pr1 = 0.0, pr2 = 0.0;
-  auto pair_passes = [&](const SysDesc& S, const VT* zin, VT* zout, VT w1, VT w2, bool wt, bool last2, bool first,
-                         double& rz) {
+  auto pair_passes = [&](const SysDesc& S, const VT* zin, const unsigned long long* zti, unsigned rin, VT* zout,
+                         unsigned long long* zto, unsigned rout, VT w1, VT w2, bool last2, bool first,
+                         double& rz) -> bool {
     if constexpr (kPairs) {
       constexpr int R = kHaloTileRows;
       const int lane = tid & 3, g = tid >> 2;
@@ -1082,18 +1096,56 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       float4* zs = reinterpret_cast<float4*>(s_halo + halo_pack_bytes(P.hbl));
       float4* rs = zs + kHaloSet;
       float4* zl = rs + kHaloRows;
-      const auto zg = gbl(zin);
       const auto rg = gbl(S.r);
-      // gather z at set slots tid and tid + 512 (and, in the first pair of
-      // an iteration, r of the own row and of the halo row at slot tid)
+      // gather: set slots tid and tid + 512 (and, in the first pair of an
+      // iteration, r of the own row and of the halo row at slot tid)
       {
         const int k0 = tid, k1 = tid + kBlock;
         const bool v0 = k0 < nset && (k0 >= R || r0t + k0 < S.nb);
         const bool v1 = k1 < nset;
         const int n0 = v0 ? (k0 < R ? r0t + k0 : setn[k0 - R]) : 0;
         const int n1 = v1 ? setn[k1 - R] : 0;
-        const VT z0a = zg[3 * n0], z0b = zg[3 * n0 + 1], z0c = zg[3 * n0 + 2];
-        const VT z1a = zg[3 * n1], z1b = zg[3 * n1 + 1], z1c = zg[3 * n1 + 2];
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c = a;
+        if (zti == nullptr) {
+          const auto zg = gbl(zin);
+          a = make_float4(zg[3 * n0], zg[3 * n0 + 1], zg[3 * n0 + 2], 0.0f);
+          c = make_float4(zg[3 * n1], zg[3 * n1 + 1], zg[3 * n1 + 2], 0.0f);
+        } else {
+          bool need0 = v0, need1 = v1, fail = false;
+          const long long t0 = rt_ticks();
+          for (;;) {
+            unsigned long long w[6];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+              w[e] = __hip_atomic_load(zti + 3 * n0 + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              w[3 + e] = __hip_atomic_load(zti + 3 * n1 + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (need0 && (unsigned)(w[0] >> 32) == rin && (unsigned)(w[1] >> 32) == rin &&
+                (unsigned)(w[2] >> 32) == rin) {
+              a = make_float4(__uint_as_float((unsigned)w[0]), __uint_as_float((unsigned)w[1]),
+                              __uint_as_float((unsigned)w[2]), 0.0f);
+              need0 = false;
+            }
+            if (need1 && (unsigned)(w[3] >> 32) == rin && (unsigned)(w[4] >> 32) == rin &&
+                (unsigned)(w[5] >> 32) == rin) {
+              c = make_float4(__uint_as_float((unsigned)w[3]), __uint_as_float((unsigned)w[4]),
+                              __uint_as_float((unsigned)w[5]), 0.0f);
+              need1 = false;
+            }
+            if (!need0 && !need1) break;
+            if (__hip_atomic_load(P.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+              fail = true;
+              break;
+            }
+            if (rt_ticks() - t0 > P.timeout_ticks) {
+              __hip_atomic_store(P.error, HSC_ERR_DEVICE_TIMEOUT_CODE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              fail = true;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          if (fail) s_pfail = 1;
+        }
         if (first) {
           const bool hr0 = k0 >= R && k0 < R + nh;
           const double h0 = rg[3 * n0], h1 = rg[3 * n0 + 1], h2 = rg[3 * n0 + 2];
@@ -1102,10 +1154,11 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
           pr2 = rg[3 * ro + 2];
           if (hr0) rs[k0 - R] = make_float4((VT)h0, (VT)h1, (VT)h2, 0.0f);
         }
-        if (v0) zs[k0] = make_float4(z0a, z0b, z0c, 0.0f);
-        if (v1) zs[k1] = make_float4(z1a, z1b, z1c, 0.0f);
+        if (v0) zs[k0] = a;
+        if (v1) zs[k1] = c;
       }
       __syncthreads();
+      if (s_pfail) return false;
       const double r0 = pr0, r1 = pr1, r2 = pr2;
       // pass j on the own rows
       {
@@ -1182,11 +1235,46 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       if (own) {
         const VT out = pass_out(lane == 0 ? zo2.x : (lane == 1 ? zo2.y : zo2.z), w2, rrF.d0, rrF.d1, rrF.d2, r0, r1,
                                 r2, a0, a1, a2);
-        if (wt)
-          zst<true>(zout + 3 * rowF + lane, out);
+        if (zto)
+          __hip_atomic_store(zto + 3 * rowF + lane, ((unsigned long long)rout << 32) | __float_as_uint(out),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else
           zout[3 * rowF + lane] = out;
         if (last2) rz += pick3(lane, r0, r1, r2) * (double)out;
+      }
+    }
+    return true;
+  };
+
+  // the first pass alone (an odd pass count in pair mode): on the own rows
+  // from the plain input, published as tagged values of round rout
+  auto single_tagged = [&](const SysDesc& S, const VT* zin, unsigned long long* zto, unsigned rout, VT w) {
+    if constexpr (kPairs) {
+      const int lane = tid & 3;
+      const bool own = rowF < S.nb && lane < 3;
+      const int ro = own ? rowF : 0, lo = own ? lane : 0;
+      double r0, r1, r2;
+      VT zo;
+      const auto zg = gbl(zin);
+      const auto rg = gbl(S.r);
+      VT a0, a1, a2;
+      row_spmv_res<G, VT>(S, mat_vals<VT>(S), rrF, lane,
+                          [&](int c, VT& x0, VT& x1, VT& x2) {
+                            x0 = zg[3 * c];
+                            x1 = zg[3 * c + 1];
+                            x2 = zg[3 * c + 2];
+                          },
+                          [&]() {
+                            r0 = rg[3 * ro];
+                            r1 = rg[3 * ro + 1];
+                            r2 = rg[3 * ro + 2];
+                            zo = zg[3 * ro + lo];
+                          },
+                          a0, a1, a2);
+      if (own) {
+        const VT out = pass_out(zo, w, rrF.d0, rrF.d1, rrF.d2, r0, r1, r2, a0, a1, a2);
+        __hip_atomic_store(zto + 3 * rowF + lane, ((unsigned long long)rout << 32) | __float_as_uint(out),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   };
@@ -1194,30 +1282,58 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   // ---- Neumann steps C1..C(order-1) -----------------------------------
   // (in pair mode: pairs (1, 2), (3, 4), ..., then a single last pass when
   // order - 1 is odd)
+  // pass-pair rounds: every tagged write round of the launch has its own
+  // number (1, 2, ...), and round r writes buffer r & 1
+  unsigned round = 0;
   auto neumann_steps = [&]() -> bool {
-    for (int j = 1; j < order;) {
-      if (kPairs && s_pairs && j + 1 < order) {
+    if (kPairs && s_pairs && order >= 3) {
+      // pairs (1, 2), ... or, for an odd pass count, pass 1 alone and then
+      // pairs (2, 3), ...: the first step reads the plain z, the last pair
+      // writes the plain z, the steps between exchange tagged values.  A
+      // failed bounded wait of any thread sets s_pfail (cleared here).
+      if (tid == 0) s_pfail = 0;
+      __syncthreads();
+      const int n = order - 1;
+      int j = 1;
+      const unsigned long long* zti = nullptr;
+      unsigned rin = 0;
+      const size_t zt_stride = 3 * (size_t)(resF ? sys[sF].nb : 0);
+      if (n & 1) {
+        round += 1;
+        if (resF && s_st[sF] == ST_RUN) {
+          const SysDesc& S = sys[sF];
+          single_tagged(S, (const VT*)S.z[zc], S.zt + (round & 1) * zt_stride, round, step_w(1));
+        }
+        zti = resF ? sys[sF].zt + (round & 1) * zt_stride : nullptr;
+        rin = round;
+        j = 2;
+      }
+      for (; j < order; j += 2) {
         const bool last2 = j + 1 == order - 1;
+        const bool first = j <= 2;
+        unsigned long long* zto = nullptr;
+        if (!last2) round += 1;
         double rz = 0.0;
+        bool ok = true;
         if (resF) {
           const SysDesc& S = sys[sF];
+          if (!last2) zto = S.zt + (round & 1) * zt_stride;
           if (s_st[sF] == ST_RUN)
-            pair_passes(S, (const VT*)S.z[zc], (VT*)S.z[zc ^ 1], step_w(j), step_w(j + 1), !last2, last2, j == 1,
-                        rz);
+            ok = pair_passes(S, (const VT*)S.z[zc], zti, rin, (VT*)S.z[zc ^ 1], zto, round, step_w(j), step_w(j + 1),
+                             last2, first, rz);
           if (last2) {
             const double c = block_sum_t0(rz, s_red);
             if (s_st[sF] == ST_RUN && tid == 0) P.partF[(size_t)sF * P.nF + cF] = c;
           }
         }
-        zc ^= 1;
-        if (!last2) {
-          if (!pass_wait(true)) return false;
-        } else {
-          if (!sync_at(1)) return false;
-        }
-        j += 2;
-        continue;
+        if (!ok) return false;
+        zti = zto;
+        rin = round;
       }
+      zc ^= 1;
+      return sync_at(1);
+    }
+    for (int j = 1; j < order;) {
       const bool last = (j == order - 1);
       double rz = 0.0;
       auto flushF = [&](int s) {
@@ -1251,7 +1367,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       }
       zc ^= 1;
       if (P.pass_sync && !last) {
-        if (!pass_wait(kPairs && s_pairs)) return false;
+        if (!pass_wait()) return false;
       } else {
         if (!sync_at(1)) return false;
       }
@@ -1400,7 +1516,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       s_rz[s] = 0.0;
     }
     __syncthreads();
-    auto pipe_wait = [&]() -> bool { return P.pass_sync ? pass_wait(false) : sync_at(1); };
+    auto pipe_wait = [&]() -> bool { return P.pass_sync ? pass_wait() : sync_at(1); };
     // W^T v over the coupled nodes of this row workgroup's node tile (v = the
     // last pass's z, or x for a true residual), written to partR for the
     // reduction at the next barrier.  Each wave sums its consecutive nodes in
@@ -2329,55 +2445,6 @@ hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, i
 }
 
 // ---------------------------------------------------------------- pass-pair tables
-// Two-hop row-tile neighbour lists: list2[t] = the tiles within two steps of
-// t in the (symmetric) one-step lists, ascending, without t; count -1 (wait
-// on every tile) when a one-step list is -1 or more than kNbrStride - 1
-// tiles result.  One wave per tile.
-__global__ __launch_bounds__(64) void k_tile_nbr2(const int* __restrict__ list, int rtl, int* __restrict__ list2) {
-  __shared__ unsigned s_b[kNbrMaxTiles / 32];
-  __shared__ int s_all;
-  const int t = blockIdx.x, lane = threadIdx.x;
-  for (int w = lane; w < kNbrMaxTiles / 32; w += 64) s_b[w] = 0u;
-  if (lane == 0) s_all = 0;
-  __syncthreads();
-  const int* L = list + (size_t)t * kNbrStride;
-  const int c = L[0];
-  if (c < 0) {
-    if (lane == 0) s_all = 1;
-  } else {
-    for (int i = lane; i < c; i += 64) {
-      const int u = L[1 + i];
-      atomicOr(&s_b[u >> 5], 1u << (u & 31));
-      const int* Lu = list + (size_t)u * kNbrStride;
-      const int cu = Lu[0];
-      if (cu < 0) {
-        s_all = 1;
-      } else {
-        for (int k = 0; k < cu; ++k) {
-          const int v = Lu[1 + k];
-          atomicOr(&s_b[v >> 5], 1u << (v & 31));
-        }
-      }
-    }
-  }
-  __syncthreads();
-  if (lane == 0) {
-    s_b[t >> 5] &= ~(1u << (t & 31));
-    int pos = 0;
-    int* out = list2 + (size_t)t * kNbrStride;
-    for (int w = 0; w < (rtl + 31) / 32; ++w) {
-      unsigned bits = s_b[w];
-      while (bits) {
-        const int bi = __ffs(bits) - 1;
-        if (pos < kNbrStride - 1) out[1 + pos] = 32 * w + bi;
-        ++pos;
-        bits &= bits - 1u;
-      }
-    }
-    out[0] = (s_all || pos > kNbrStride - 1) ? -1 : pos;
-  }
-}
-
 // Sorted, distinct node lists through a bitmap over [lo, lo + 32 words): the
 // exclusive prefix of the words' popcounts (contiguous word runs per thread),
 // so a member's rank is pre[w] + popc(bits below it).  Returns the count.
@@ -2593,12 +2660,11 @@ __global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ ro
 }
 
 hipError_t launch_tile_pairs(const int* rowptr, const int* colidx, const float* val32, const float* invD32, int nb,
-                             const int* list, int* list2, int hbl, unsigned char* halo, hipStream_t st) {
+                             int hbl, unsigned char* halo, hipStream_t st) {
   const int rtl = (nb + kHaloTileRows - 1) / kHaloTileRows;
   if (rtl <= 0 || rtl > kNbrMaxTiles || hbl < 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned*>(halo),
                      (long long)(kHaloBaseBytes / 4));
-  hipLaunchKernelGGL(k_tile_nbr2, dim3(rtl), dim3(64), 0, st, list, rtl, list2);
   hipLaunchKernelGGL(k_tile_halo, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, val32, invD32, nb, hbl, halo,
                      halo_pack_bytes(hbl));
   return hipGetLastError();

@@ -92,8 +92,9 @@ struct HostBuf {
 // Scratch of one persistent PCG launch: system descriptors and outputs,
 // chunk tables, reduction partials, grid-barrier counters, error flag.
 struct PcgScratch {
-  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, partR, barrier, errflag, flags;
+  Buf sysdesc, sysout, chunks, partA, partE, partF, partL, partR, barrier, errflag, flags, ztag;
   int nflags = 0;  // row-tile pass flags zeroed before each launch (0: point-to-point pass sync off)
+  size_t ztag_bytes = 0;  // pass pairs' tagged values zeroed before each launch (0: no pairs)
   HostBuf host;    // pinned copies of the outputs and the error flag
   // last uploaded descriptor and chunk tables (an identical plan is not uploaded again)
   std::vector<unsigned char> up_sys, up_ch;
@@ -231,9 +232,9 @@ struct hsc_handle {
   // row-tile neighbour lists for 128, 64 and 32 rows per tile (lanes per row 4, 8, 16)
   Buf tnbr[3], tnbr_bits;
   bool tnbr_ok[3] = {false, false, false};
-  // pass pairs (128-row tiles): two-hop neighbour lists and halo packs, built
-  // for pack capacity pairs_hbl (-1: not built since the last setup)
-  Buf tnbr2, halo;
+  // pass pairs (128-row tiles): halo packs, built for pack capacity
+  // pairs_hbl (-1: not built since the last setup)
+  Buf halo;
   int pairs_hbl = -1;
   double lambda_max = 0.0;  // power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights)
   double omegas[hsc::kMaxPolySteps] = {};  // per-pass preconditioner weights
@@ -568,6 +569,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   }
   const bool pipe = pipe_req && aligned && !thr && nA == rt;
   sc.nflags = psync ? rt : 0;
+  sc.ztag_bytes = 0;
   plan.pipe = pipe;
   if (pipe) shmem = shmem_pipe;
   // pass pairs (pass_sync = 2): one system, FP32 resident rows at 4 lanes per
@@ -601,15 +603,16 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
     hsc_handle* h = h0;
     if (h->pairs_hbl != hbl) {
       const int rtl = (h->nb + R - 1) / R;
-      HIPCHK(h->tnbr2.ensure(sizeof(int) * (size_t)rtl * hsc::kNbrStride));
       HIPCHK(h->halo.ensure((size_t)hsc::kHaloBaseBytes + (size_t)rtl * (size_t)hsc::halo_pack_bytes(hbl)));
       HIPCHK(hsc::launch_tile_pairs(h->rowptr.as<int>(), h->colidx.as<int>(), h->val32.as<float>(),
-                                    h->invD32.as<float>(), h->nb, h->tnbr[0].as<int>(), h->tnbr2.as<int>(), hbl,
-                                    h->halo.as<unsigned char>(), st));
+                                    h->invD32.as<float>(), h->nb, hbl, h->halo.as<unsigned char>(), st));
       h->pairs_hbl = hbl;
     }
-    S[0].nbr2 = h->tnbr2.as<int>();
+    // two buffers of tagged pass values, zeroed before each launch
+    sc.ztag_bytes = sizeof(unsigned long long) * 2 * 3 * (size_t)h->nb;
+    HIPCHK(sc.ztag.ensure(sc.ztag_bytes));
     S[0].halo = h->halo.as<unsigned char>();
+    S[0].zt = sc.ztag.as<unsigned long long>();
   }
   for (int s = 0; s < nsys; ++s) {
     hs[s]->pass_sync_used = psync;
@@ -704,6 +707,10 @@ void reset_pcg(hsc_handle* const* hs, int nsys, PcgScratch& sc, hipStream_t st) 
   if (sc.nflags > 0) {
     zp.push_back(sc.flags.p);
     zb.push_back(sizeof(unsigned) * sc.nflags);
+  }
+  if (sc.ztag_bytes > 0) {
+    zp.push_back(sc.ztag.p);
+    zb.push_back(sc.ztag_bytes);
   }
   for (int s = 0; s < nsys; ++s) {
     zp.push_back(hs[s]->p0.p);

@@ -115,8 +115,8 @@ typedef struct hsc_config {
                                     neighbour tiles between preconditioner passes; 0 = grid barriers;
                                     2 (default) = pass pairs where they apply (one system, mixed precision,
                                     4 lanes per row, rows of at most 20 blocks): each row tile also computes
-                                    its halo rows' pass in LDS and waits on its two-hop neighbour tiles once
-                                    per two passes, else as 1.
+                                    its halo rows' pass in LDS, so it reads its neighbours' values once per
+                                    two passes, as tagged words (no flags or waits), else as 1.
                                     Same arithmetic in every mode (bitwise-equal results) */
   int32_t pipelined;             /* 1 = the pipelined PCG loop where it applies (latency schedule, 4 lanes per
                                     row, Chebyshev/learned or Neumann passes of order >= 2, recursive-residual

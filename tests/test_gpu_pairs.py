@@ -2,17 +2,19 @@
 
 Each row tile computes preconditioner pass j for its own rows and for its
 halo rows (the rows of other tiles its rows reference) in LDS, then pass j+1
-for its own rows from there, and waits on its two-hop neighbour tiles once
-per pair.  A halo row is computed with the owner's lane split, order and
-helpers, so the preconditioner is the same operator to the last bit:
+for its own rows from there; between pairs the tiles exchange their pass
+values as tagged 64-bit words (value and round number), re-read until the
+tag matches, instead of waiting on flags.  A halo row is computed with the
+owner's lane split, order and helpers, so the preconditioner is the same
+operator to the last bit:
 
 * on the 78k-tet system (mixed precision, Chebyshev-weighted passes, ratio
   1000) x, the iteration count and the residuals are bitwise equal to the
   per-pass neighbour sync and to grid barriers, for a pair count that ends at
   the grid barrier (3 and 5 passes-plus-one) and one followed by a single last
   pass (16), and the pairs actually ran;
-* repeated solves are bitwise identical (the waits order every read after
-  the write it needs);
+* repeated solves are bitwise identical (every value is read with the tag
+  of the round that wrote it);
 * a system whose tile halo does not fit (a row of more than 20 blocks) falls
   back to one wait per pass on the device, bit for bit the per-pass result;
 * the 2,178-DOF system runs pairs too, and converges to the direct solution.
