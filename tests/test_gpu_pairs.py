@@ -10,9 +10,9 @@ operator to the last bit:
 
 * on the 78k-tet system (mixed precision, Chebyshev-weighted passes, ratio
   1000) x, the iteration count and the residuals are bitwise equal to the
-  per-pass neighbour sync and to grid barriers, for a pair count that ends at
-  the grid barrier (3 and 5 passes-plus-one) and one followed by a single last
-  pass (16), and the pairs actually ran;
+  per-pass neighbour sync and to grid barriers, for 2 passes (one pair, no
+  tagged exchange), 4 (two pairs, one tagged round) and 15 (pass 1 alone, then
+  7 pairs: 7 tagged rounds per iteration), and the pairs actually ran;
 * repeated solves are bitwise identical (every value is read with the tag
   of the round that wrote it);
 * a system whose tile halo does not fit (a row of more than 20 blocks) falls
