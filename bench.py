@@ -68,8 +68,11 @@ def beyond_mall(sysm, dev, copies=32):
     """The standalone SpMV / block-Jacobi pass kernels with the working set
     beyond the 256 MB Infinity Cache: `copies` block-diagonal copies of the
     78k-tet system's FEM part (systems.replicate_fem), each kernel timed over
-    40 back-to-back launches with HIP events (hsc_time_kernel) at its best
-    lanes-per-row, beside a measured 1 GiB device-to-device copy."""
+    40 back-to-back launches with HIP events (hsc_time_kernel): the streaming
+    kernels the library's operators run (chunks per wavefront: the library's
+    choice), and beside them the row mapping of the persistent solve at its
+    best lanes-per-row; with a measured 1 GiB device-to-device copy for
+    scale."""
     import torch
     from hsc_reimpl_amd import plugin, systems
     big = systems.replicate_fem(sysm, copies)
@@ -86,12 +89,18 @@ def beyond_mall(sysm, dev, copies=32):
                "pass32": ("the same pass, FP32 values/vectors, FP64 r", 40 * nnzb + 4 * (nb + 1) + 16 * n + 36 * nb)}
     out = {"system": f"{copies} copies of the 78k-tet system's FEM part", "fem_nodes": nb, "nnzb": nnzb,
            "working_set_MB": {k: round(v[1] / 1e6, 1) for k, v in kernels.items()}, "kernels": {}}
+    out["row_mapped"] = {}
     for k, (what, nbytes) in kernels.items():
-        best = min((h.time_kernel(k, G, repeats=40), G) for G in (4, 8, 16))
-        ms, G = best
+        h.time_kernel(k, 0, repeats=5)
+        ms = h.time_kernel(k, 0, repeats=40)
         gbs = nbytes / (ms * 1e-3) / 1e9
-        out["kernels"][k] = {"what": what, "lanes_per_row": G, "us": round(1e3 * ms, 1), "bytes": nbytes,
-                             "GB/s": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
+        out["kernels"][k] = {"what": what, "mapping": "streaming: 64-block chunks per wavefront, segmented scan",
+                             "us": round(1e3 * ms, 1), "bytes": nbytes, "GB/s": round(gbs, 1),
+                             "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
+        ms, G = min((h.time_kernel(k + "_rows", G, repeats=40), G) for G in (4, 8, 16))
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out["row_mapped"][k] = {"lanes_per_row": G, "us": round(1e3 * ms, 1), "GB/s": round(gbs, 1),
+                                "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 3)}
     del h, A
     src = torch.empty((1 << 30) // 4, dtype=torch.float32, device=dev).fill_(1.0)
     dst = torch.empty_like(src)
@@ -104,6 +113,7 @@ def beyond_mall(sysm, dev, copies=32):
     torch.cuda.synchronize()
     copy_gbs = 5 * 2 * (1 << 30) / (e0.elapsed_time(e1) * 1e-3) / 1e9
     out["device_copy_1GiB_GB/s"] = round(copy_gbs, 1)
+
     for v in out["kernels"].values():
         v["frac_of_copy"] = round(v["GB/s"] / copy_gbs, 3)
     return out
@@ -501,6 +511,10 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one mixed solve with the launch shape pinned (no tuner launches), then a 256 MiB "
                          "device copy: the program profiled by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--bandwidth-probe", action="store_true",
+                    help="one launch of each streaming operator kernel on 32 copies of the 78k-tet system's "
+                         "FEM part, then a 256 MiB device copy: the program profiled by rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE for the kernels' traffic")
     ap.add_argument("--workload", choices=["78k-tet", "batch64"], default="78k-tet",
                     help="78k-tet: the headline single-system solve; batch64: 64 uneven systems (seed 2) "
                          "in one captured persistent launch vs one at a time")
@@ -539,6 +553,26 @@ def main():
              "linear_system/hsc/omega": omega, "linear_system/hsc/max_iter": 20000}
 
     new_scene = dict(scene, **{"linear_system/hsc/neumann_order": NEW_PASSES})
+    if args.bandwidth_probe:
+        big = systems.replicate_fem(sysm, 32)
+        Ab = plugin.BCOO3.from_system(big, dev)
+        h = plugin.Handle(plugin.make_config({"linear_system/tol_rate": rtol}, precision="mixed"))
+        h.setup(Ab, big.qo, big.qm, big.fo, big.fn)
+        i = h.setup_info()
+        nb, nnzb = i["fem_nodes"], i["nnzb"]
+        ms = {k: h.time_kernel(k, 0, repeats=1) for k in ("spmv64", "pass64", "pass32")}
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        n = 3 * nb
+        alg = {"spmv64": 76 * nnzb + 4 * (nb + 1) + 16 * n, "pass64": 76 * nnzb + 4 * (nb + 1) + 24 * n + 72 * nb,
+               "pass32": 40 * nnzb + 4 * (nb + 1) + 16 * n + 36 * nb}
+        print(json.dumps({"probe": "bandwidth", "fem_nodes": nb, "nnzb": nnzb, "algorithmic_bytes": alg,
+                          "ms_single_launch": ms, "copy_bytes": nbytes}), flush=True)
+        return
     if args.traffic_probe:
         poly = args.preconditioner != "neumann"
         ext = {"poly_interval_ratio": NEW_RATIO} if args.preconditioner == "chebyshev" else {}

@@ -10,7 +10,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libhsc_mi355x.so"
 
-HSC_ABI_VERSION = 2
+HSC_ABI_VERSION = 3
 
 STATUS = {
     0: "HSC_OK", 1: "HSC_ERR_INVALID_ARGUMENT", 2: "HSC_ERR_FEM_DIAGONAL",
@@ -36,7 +36,8 @@ class Config(C.Structure):
         ("true_residual_every_iteration", C.c_int32), ("aca_rtol", C.c_double),
         ("lanes_per_row", C.c_int32), ("wg_per_cu", C.c_int32), ("preconditioner", C.c_int32),
         ("check_acceptance", C.c_int32), ("schedule", C.c_int32), ("poly_interval_ratio", C.c_double),
-        ("pass_sync", C.c_int32), ("pipelined", C.c_int32),
+        ("pass_sync", C.c_int32), ("pipelined", C.c_int32), ("operator_mapping", C.c_int32),
+        ("reserved0", C.c_int32),
     ]
 
 
@@ -74,6 +75,7 @@ class SetupInfo(C.Structure):
         ("schedule", C.c_int32), ("pass_sync", C.c_int32), ("lambda_max", C.c_double),
         ("poly_ratio", C.c_double), ("learned_out", C.c_double * 3), ("learned_used", C.c_int32),
         ("fallback_reason", C.c_int32), ("pipelined", C.c_int32), ("pass_pairs", C.c_int32),
+        ("streaming_operators", C.c_int32), ("reserved0", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -113,6 +115,8 @@ SIGNATURES = {
     "hsc_apply_D": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hsc_apply_schur": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hsc_apply_precond": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hsc_apply_pass": (C.c_int32, [C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
     "hsc_get_setup_info": (C.c_int32, [C.c_void_p, C.POINTER(SetupInfo)]),
     "hsc_fem_hessian": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_double, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -109,6 +109,17 @@ struct SysDesc {
   int pad_;
 };
 
+// Chunk tables of the streaming (strip) operator kernels: for chunk k of 64
+// blocks, bit j of hbits[k] is set when block 64k + j is the first block of
+// a row, and hrow[k] counts the rows that start before the chunk
+// (hrow[nch] = nb).  Wave w of `nwave` walks chunks [w nch / nwave,
+// (w + 1) nch / nwave); nwave 0: the launcher's choice.
+struct StripDesc {
+  const unsigned long long* hbits;  // [nch]
+  const int* hrow;                  // [nch + 1]
+  int nch, nwave;
+};
+
 // Per-system outputs of the persistent PCG kernel.
 struct SysOut {
   int iterations;

@@ -155,8 +155,9 @@ hipError_t asm_layout(int n, hipStream_t st, AsmLayout* L);
 hipError_t assemble_in(const int* rows, const int* cols, const double* vals, long long count, int fold, int* orow,
                        int* ocol, double* ovals, long long* out_count, char* arena, const AsmLayout& L,
                        hipStream_t st);
-// one launch of a standalone operator kernel: 0 FP64 D x, 1 FP64 pass, 2 FP32 pass
-hipError_t launch_timed_op(int kernel, int G, const SysDesc& S, const void* x, void* y, hipStream_t st);
+// one launch of a row-mapped operator kernel (G lanes per block row): 0 FP64 D x,
+// 1 FP64 pass, 2 FP32 pass (y = x + w Dinv (r - D x), r = S.r)
+hipError_t launch_row_op(int kernel, int G, const SysDesc& S, const void* x, void* y, double w, hipStream_t st);
 // up to kZeroListMax buffers zeroed by one kernel launch (each bytes % 4 == 0)
 constexpr int kZeroListMax = 12;
 struct ZeroList {
@@ -165,9 +166,16 @@ struct ZeroList {
   int n;
 };
 hipError_t launch_zero_multi(const void* const* ptrs, const size_t* bytes, int n, hipStream_t st);
-hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st);
-hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double* omegas, const double* r, double* z,
-                               double* tmp, hipStream_t st);
+// streaming (strip) operator kernels: chunk tables, and one launch of
+// 0 FP64 D x, 1 FP64 pass, 2 FP32 pass (y = x + w Dinv (r - D x))
+hipError_t launch_chunk_heads(const int* rowptr, int nb, int nnzb, unsigned long long* hbits, int* hrow,
+                              hipStream_t st);
+hipError_t launch_strip_op(int kernel, const SysDesc& S, const StripDesc& T, const void* x, void* y, double w,
+                           hipStream_t st);
+// the preconditioner's passes, FP64: streaming pass kernels (T) or row-mapped
+// ones with G lanes per row (T null)
+hipError_t launch_op_precond64(const StripDesc* T, int G, const SysDesc& S, int order, const double* omegas,
+                               const double* r, double* z, double* tmp, hipStream_t st);
 hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st);
 // row-tile neighbour lists for R rows per tile (bits: ceil(rtl/32)*rtl words,
 // list: rtl*kNbrStride ints, rtl = ceil(nb/R) <= kNbrMaxTiles)

@@ -2102,6 +2102,261 @@ __global__ __launch_bounds__(kBlock) void op_lowrank64(SysDesc S, const double* 
   }
 }
 
+// ---------------------------------------------------------------- chunk-strip operator kernels
+// The standalone D x and preconditioner pass stream the matrix once per
+// launch: their rate is set by how the loads map onto memory.  Here a wave
+// owns the rows whose first block lies in its `cpw` chunks of 64 blocks and
+// walks their blocks chunk by chunk, lane j on block 64k + j, so the column
+// index and each of the 9 value components of a chunk is one contiguous,
+// aligned 64-element load (the component-major chunk layout), no lane idles
+// on a short row, and a lane's x gather is one 12-byte (FP32) or 16 + 8-byte
+// (FP64) load.  Row sums come out of a head-segmented inclusive scan across
+// the wave (DPP row shifts by 1, 2, 4, 8, then the row-15 and row-31
+// broadcasts), and the running sum of a row that continues past the chunk's
+// end is carried into lane 0 of the next chunk, so rows of any length are
+// summed in a fixed order.  Two per-chunk tables built at setup give the
+// structure: the chunk's head bits (bit j: block 64k + j is the first block
+// of a row) and the number of rows that start before the chunk.  Lanes
+// outside the wave's blocks load a clamped in-range address (a cache line a
+// lane in range loads anyway) and contribute zero.
+
+// v of another lane through a DPP move (CTRL, row mask RM); 0 where the
+// source lane is out of range or the row is masked off
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, 0xF, false));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RM, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int CTRL, int RM, typename T>
+__device__ __forceinline__ void seg_step(bool take, T& a0, T& a1, T& a2) {
+  const T t0 = dpp_mov<CTRL, RM>(a0), t1 = dpp_mov<CTRL, RM>(a1), t2 = dpp_mov<CTRL, RM>(a2);
+  a0 = take ? a0 + t0 : a0;
+  a1 = take ? a1 + t1 : a1;
+  a2 = take ? a2 + t2 : a2;
+}
+
+// Inclusive scan of (a0, a1, a2) over the lanes of each segment; s0 is the
+// first lane of this lane's segment.  Every lane's result is a fixed tree of
+// its segment's values.
+template <typename T>
+__device__ __forceinline__ void seg_scan3(int lane, int s0, T& a0, T& a1, T& a2) {
+  const int rl = lane & 15;
+  seg_step<0x111, 0xF>(rl >= 1 && lane - 1 >= s0, a0, a1, a2);  // row_shr:1
+  seg_step<0x112, 0xF>(rl >= 2 && lane - 2 >= s0, a0, a1, a2);  // row_shr:2
+  seg_step<0x114, 0xF>(rl >= 4 && lane - 4 >= s0, a0, a1, a2);  // row_shr:4
+  seg_step<0x118, 0xF>(rl >= 8 && lane - 8 >= s0, a0, a1, a2);  // row_shr:8
+  // row_bcast:15 into rows 1 and 3 (from lanes 15 and 47)
+  seg_step<0x142, 0xA>(((lane >> 4) & 1) != 0 && s0 <= (lane | 15) - 16, a0, a1, a2);
+  // row_bcast:31 into rows 2 and 3 (from lane 31)
+  seg_step<0x143, 0xC>(lane >= 32 && s0 <= 31, a0, a1, a2);
+}
+
+template <typename T>
+__device__ __forceinline__ T lane63(T v) {
+  if constexpr (sizeof(T) == 4) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+  }
+}
+
+// x[3c .. 3c+2] as one load of 12 bytes (FP32) or 16 + 8 bytes (FP64)
+template <typename VT>
+struct __attribute__((aligned(sizeof(VT)))) Vec3 {
+  VT v0, v1, v2;
+};
+template <typename VT>
+__device__ __forceinline__ Vec3<VT> gather3(const VT* __restrict__ x, int c) {
+  Vec3<VT> t;
+  __builtin_memcpy(&t, x + 3 * (size_t)c, sizeof(t));
+  return t;
+}
+
+// MODE 0: y = D x (FP64).  MODE 1: y = x + w Dinv (r - D x) (VT values and
+// vectors, FP64 r), the arithmetic of tile_N per row.
+//
+// The lane that ends a row writes the row's sum into a 128-row ring in LDS;
+// each time 64 rows are finished the wave runs their epilogue with one lane
+// per row (consecutive rows: r, z and the inverse blocks are read, and y
+// written, as contiguous streams instead of by the few row-ending lanes of
+// each chunk).  Workgroups take contiguous runs of chunks per XCD (workgroup
+// b runs on XCD b % 8), so the x lines that neighbouring rows share are
+// fetched into one L2; the value and index streams, read once per launch,
+// are loaded non-temporally so they do not evict x.  At most 72 VGPRs (7
+// waves per SIMD): the FP64 pass at 80 ran 6% slower.
+constexpr int kStripBlock = 256;  // one wave per SIMD per workgroup: occupancy in steps of one wave
+
+template <int MODE, typename VT>
+__global__ __launch_bounds__(kStripBlock) __attribute__((amdgpu_waves_per_eu(7))) void op_strip(
+    SysDesc S, StripDesc T, const VT* __restrict__ x, VT* __restrict__ y, VT w) {
+  __shared__ VT s_sum[kStripBlock / 64][128][3];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int nwg = (int)gridDim.x, xc = (int)blockIdx.x & 7, q = nwg >> 3, rem = nwg & 7;
+  const int b = xc * q + min(xc, rem) + ((int)blockIdx.x >> 3);
+  const int wv = b * (kStripBlock / 64) + wl;
+  if (wv >= T.nwave) return;
+  const int c0 = (int)((long long)wv * T.nch / T.nwave), c1 = (int)((long long)(wv + 1) * T.nch / T.nwave);
+  const int rb = T.hrow[c0], re = T.hrow[c1];
+  if (rb >= re) return;  // a row longer than the wave's chunks: its owner sums it
+  const int hb = S.rowptr[rb], he = S.rowptr[re];
+  const VT* val = mat_vals<VT>(S);
+  const auto inv = gbl(inv_vals<VT>(S));
+  const auto rg = gbl(S.r);
+  int rdone = rb;  // rows [rb, rdone) have had their epilogue
+  auto epilogue = [&](int cnt) {
+    if (lane < cnt) {
+      const int row = rdone + lane;
+      const int sl = (row - rb) & 127;
+      const VT a0 = s_sum[wl][sl][0], a1 = s_sum[wl][sl][1], a2 = s_sum[wl][sl][2];
+      if constexpr (MODE == 0) {
+        y[3 * row] = a0;
+        y[3 * row + 1] = a1;
+        y[3 * row + 2] = a2;
+      } else {
+        const double r0 = rg[3 * row], r1 = rg[3 * row + 1], r2 = rg[3 * row + 2];
+        const Vec3<VT> zr = gather3(x, row);
+        VT d[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) d[e] = inv[9 * (size_t)row + e];
+        y[3 * row] = pass_out(zr.v0, w, d[0], d[1], d[2], r0, r1, r2, a0, a1, a2);
+        y[3 * row + 1] = pass_out(zr.v1, w, d[3], d[4], d[5], r0, r1, r2, a0, a1, a2);
+        y[3 * row + 2] = pass_out(zr.v2, w, d[6], d[7], d[8], r0, r1, r2, a0, a1, a2);
+      }
+    }
+    rdone += cnt;
+  };
+  VT cy0 = VT(0), cy1 = VT(0), cy2 = VT(0);
+  const int kb = hb >> 6, ke = (he - 1) >> 6;
+  for (int k = kb; k <= ke; ++k) {
+    const unsigned long long H = T.hbits[k];
+    const unsigned long long Hn = k + 1 < T.nch ? T.hbits[k + 1] : 1ull;
+    const int p = (k << 6) + lane;
+    const bool valid = p >= hb && p < he;
+    const int pc = min(max(p, hb), he - 1);
+    const unsigned long long m = H & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int s0 = m ? 63 - __clzll((long long)m) : 0;  // first lane of this lane's row in the chunk
+    const int row = min(max(T.hrow[k] - 1 + __popcll(m), 0), S.nb - 1);
+    const bool next_head = lane < 63 ? ((H >> (lane + 1)) & 1ull) != 0 : (Hn & 1ull) != 0;
+    const bool end = valid && (next_head || p + 1 == he);
+    const int c = __builtin_nontemporal_load(S.colidx + pc);
+    VT v[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) v[e] = __builtin_nontemporal_load(val + val_index(pc, e));
+    const Vec3<VT> xv = gather3(x, c);
+    const bool carry_in = lane == 0 && (H & 1ull) == 0;
+    VT a0 = carry_in ? cy0 : VT(0), a1 = carry_in ? cy1 : VT(0), a2 = carry_in ? cy2 : VT(0);
+    blk_step(valid, v, xv.v0, xv.v1, xv.v2, a0, a1, a2);
+    seg_scan3(lane, s0, a0, a1, a2);
+    cy0 = lane63(a0);
+    cy1 = lane63(a1);
+    cy2 = lane63(a2);
+    if (end) {
+      const int sl = (row - rb) & 127;
+      s_sum[wl][sl][0] = a0;
+      s_sum[wl][sl][1] = a1;
+      s_sum[wl][sl][2] = a2;
+    }
+    // the ring's writes before any lane's read of them (LDS operations of
+    // one wave complete in order)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // rows finished through this chunk: those that start in it or before,
+    // less the last one unless the next chunk starts with a row
+    const int fin = k == ke ? re : min(re, (Hn & 1ull) ? T.hrow[k + 1] : T.hrow[k + 1] - 1);
+    while (fin - rdone >= 64) epilogue(64);
+    if (k == ke && fin > rdone) epilogue(fin - rdone);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// head bits and rows-before of each chunk: one thread per chunk k <= nch
+// (hrow[nch] = nb closes the table)
+__global__ __launch_bounds__(kBlock) void k_chunk_heads(const int* __restrict__ rowptr, int nb, int nch,
+                                                        unsigned long long* __restrict__ hbits,
+                                                        int* __restrict__ hrow) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k > nch) return;
+  const long long lo = 64ll * k;
+  int a = 0, b = nb;  // first row whose first block is >= lo
+  while (a < b) {
+    const int mid = (a + b) >> 1;
+    if (rowptr[mid] < lo) a = mid + 1;
+    else b = mid;
+  }
+  hrow[k] = a;
+  if (k == nch) return;
+  unsigned long long bits = 0;
+  for (int r = a; r < nb && rowptr[r] < lo + 64; ++r) bits |= 1ull << (rowptr[r] - lo);
+  hbits[k] = bits;
+}
+
+// Waves of a strip launch when the caller leaves it open: as many as fit on
+// the device at once (occupancy x CUs), at most one per chunk, so the grid
+// runs in one round with equal shares of chunks and no tail of late
+// workgroups (on 8.3M blocks: 5.1-5.2 TB/s, against 4.9-5.2 for fixed runs of
+// 8 to 16 chunks per wave, whose best length differed by kernel).
+static int strip_resident_waves(int kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> g(mu);
+  const auto key = std::make_pair(dev, kernel);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0, blocks = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const void* fn = kernel == 0   ? (const void*)op_strip<0, double>
+                   : kernel == 1 ? (const void*)op_strip<1, double>
+                                 : (const void*)op_strip<1, float>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, kStripBlock, 0) != hipSuccess) return 0;
+  const int waves = std::max(1, blocks) * cus * (kStripBlock / 64);
+  cache[key] = waves;
+  return waves;
+}
+
+hipError_t launch_chunk_heads(const int* rowptr, int nb, int nnzb, unsigned long long* hbits, int* hrow,
+                              hipStream_t st) {
+  const int nch = (nnzb + 63) / 64;
+  hipLaunchKernelGGL(k_chunk_heads, dim3((nch + kBlock) / kBlock), dim3(kBlock), 0, st, rowptr, nb, nch, hbits,
+                     hrow);
+  return hipGetLastError();
+}
+
+hipError_t launch_strip_op(int kernel, const SysDesc& S, const StripDesc& T, const void* x, void* y, double w,
+                           hipStream_t st) {
+  if (T.nch == 0) return hipSuccess;
+  if (kernel < 0 || kernel > 2) return hipErrorInvalidValue;
+  StripDesc D = T;
+  if (D.nwave <= 0) {
+    const int res = strip_resident_waves(kernel);
+    if (res <= 0) return hipErrorInvalidDevice;
+    D.nwave = std::min(D.nch, res);
+  }
+  D.nwave = std::min(D.nwave, D.nch);
+  const dim3 grid((D.nwave + kStripBlock / 64 - 1) / (kStripBlock / 64));
+  if (kernel == 0)
+    hipLaunchKernelGGL((op_strip<0, double>), grid, dim3(kStripBlock), 0, st, S, D, (const double*)x, (double*)y,
+                       0.0);
+  else if (kernel == 1)
+    hipLaunchKernelGGL((op_strip<1, double>), grid, dim3(kStripBlock), 0, st, S, D, (const double*)x, (double*)y,
+                       w);
+  else
+    hipLaunchKernelGGL((op_strip<1, float>), grid, dim3(kStripBlock), 0, st, S, D, (const float*)x, (float*)y,
+                       (float)w);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- vector kernels
 // (host-driven flexible PCG of the learned preconditioner)
 __global__ __launch_bounds__(kBlock) void k_dot_partial(const double* a, const double* b, int n, double* part) {
@@ -2322,21 +2577,9 @@ const void* pcg_kernel_fn(int G, bool mixed, bool res, bool pipe) {
   return nullptr;
 }
 
-hipError_t launch_op_Dx64(int G, const SysDesc& S, const double* x, double* y, hipStream_t st) {
-  const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
-  if (tiles == 0) return hipSuccess;
-  switch (G) {
-    case 4: hipLaunchKernelGGL(op_Dx64<4>, dim3(tiles), dim3(kBlock), 0, st, S, x, y); break;
-    case 8: hipLaunchKernelGGL(op_Dx64<8>, dim3(tiles), dim3(kBlock), 0, st, S, x, y); break;
-    case 16: hipLaunchKernelGGL(op_Dx64<16>, dim3(tiles), dim3(kBlock), 0, st, S, x, y); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-// one launch of a standalone operator kernel (bandwidth measurements):
-// 0 = FP64 y = D x, 1 = FP64 pass y = x + w Dinv (r - D x), 2 = the pass in FP32
-hipError_t launch_timed_op(int kernel, int G, const SysDesc& S, const void* x, void* y, hipStream_t st) {
+// one launch of a row-mapped operator kernel (G lanes per block row): 0 =
+// FP64 y = D x, 1 = FP64 pass y = x + w Dinv (r - D x), 2 = the pass in FP32
+hipError_t launch_row_op(int kernel, int G, const SysDesc& S, const void* x, void* y, double w, hipStream_t st) {
   const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
   if (tiles == 0) return hipSuccess;
 #define HSC_TIMED(GG)                                                                                          \
@@ -2344,10 +2587,10 @@ hipError_t launch_timed_op(int kernel, int G, const SysDesc& S, const void* x, v
     hipLaunchKernelGGL(op_Dx64<GG>, dim3(tiles), dim3(kBlock), 0, st, S, (const double*)x, (double*)y);        \
   else if (kernel == 1)                                                                                        \
     hipLaunchKernelGGL((op_neumann_step<GG, double>), dim3(tiles), dim3(kBlock), 0, st, S, (const double*)x,   \
-                       (double*)y, 0.5);                                                                       \
+                       (double*)y, w);                                                                         \
   else                                                                                                         \
     hipLaunchKernelGGL((op_neumann_step<GG, float>), dim3(tiles), dim3(kBlock), 0, st, S, (const float*)x,     \
-                       (float*)y, 0.5);
+                       (float*)y, w);
   if (kernel < 0 || kernel > 2) return hipErrorInvalidValue;
   switch (G) {
     case 4: HSC_TIMED(4) break;
@@ -2359,9 +2602,8 @@ hipError_t launch_timed_op(int kernel, int G, const SysDesc& S, const void* x, v
   return hipGetLastError();
 }
 
-hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double* omegas, const double* r,
-                               double* z, double* tmp, hipStream_t st) {
-  const int tiles = (S.nb + (kBlock / G) - 1) / (kBlock / G);
+hipError_t launch_op_precond64(const StripDesc* T, int G, const SysDesc& S, int order, const double* omegas,
+                               const double* r, double* z, double* tmp, hipStream_t st) {
   const int ntile = (S.nb + kBlock - 1) / kBlock;
   if (S.nb == 0) return hipSuccess;
   auto w = [&](int j) { return omegas[j < kMaxPolySteps ? j : kMaxPolySteps - 1]; };
@@ -2372,12 +2614,9 @@ hipError_t launch_op_precond64(int G, const SysDesc& S, int order, const double*
   SysDesc S2 = S;
   S2.r = const_cast<double*>(r);
   for (int j = 1; j < order; ++j) {
-    switch (G) {
-      case 4: hipLaunchKernelGGL((op_neumann_step<4, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], w(j)); break;
-      case 8: hipLaunchKernelGGL((op_neumann_step<8, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], w(j)); break;
-      case 16: hipLaunchKernelGGL((op_neumann_step<16, double>), dim3(tiles), dim3(kBlock), 0, st, S2, bufs[cur], bufs[cur ^ 1], w(j)); break;
-      default: return hipErrorInvalidValue;
-    }
+    const hipError_t e = T ? launch_strip_op(1, S2, *T, bufs[cur], bufs[cur ^ 1], w(j), st)
+                           : launch_row_op(1, G, S2, bufs[cur], bufs[cur ^ 1], w(j), st);
+    if (e != hipSuccess) return e;
     cur ^= 1;
   }
   return hipGetLastError();

@@ -236,6 +236,10 @@ struct hsc_handle {
   // pairs_hbl (-1: not built since the last setup)
   Buf halo;
   int pairs_hbl = -1;
+  // chunk tables of the streaming operator kernels (strip_nch -1: not built
+  // since the last setup)
+  Buf hbits, hrow;
+  int strip_nch = -1;
   double lambda_max = 0.0;  // power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights)
   double omegas[hsc::kMaxPolySteps] = {};  // per-pass preconditioner weights
   double* xsol = nullptr;  // FEM part of the caller's x during a solve
@@ -328,6 +332,10 @@ int validate_config(const hsc_config& c, std::string& why) {
     why = "pipelined must be 0 or 1";
     return HSC_ERR_INVALID_ARGUMENT;
   }
+  if (c.operator_mapping < 0 || c.operator_mapping > 2) {
+    why = "operator_mapping must be 0, 1 or 2";
+    return HSC_ERR_INVALID_ARGUMENT;
+  }
   if (c.schedule < 0 || c.schedule > 2) {
     why = "schedule must be 0, 1 or 2";
     return HSC_ERR_INVALID_ARGUMENT;
@@ -364,6 +372,42 @@ hsc::SysDesc make_desc(hsc_handle* h) {
   S.tol = h->cfg.tol_rate;
   S.max_iter = h->cfg.max_iter;
   return S;
+}
+
+// chunk tables of the streaming operator kernels (built by the setup)
+hsc::StripDesc strip_desc(const hsc_handle* h) {
+  const int nch = (h->nnzb + 63) / 64;
+  if (h->strip_nch != nch) throw HscError{HSC_ERR_NOT_SETUP, "chunk tables not built"};
+  return hsc::StripDesc{h->hbits.as<unsigned long long>(), h->hrow.as<int>(), nch, 0};
+}
+
+// Algorithmic bytes of one standalone operator launch (0 FP64 D x, 1 FP64
+// pass, 2 FP32 pass): values and column indices, row pointers, x gathered
+// once, y written, and for a pass r and the inverse diagonal blocks.
+long long operator_bytes(int kernel, long long nb, long long nnzb) {
+  const long long n = 3 * nb;
+  if (kernel == 0) return 76 * nnzb + 4 * (nb + 1) + 16 * n;
+  if (kernel == 1) return 76 * nnzb + 4 * (nb + 1) + 24 * n + 72 * nb;
+  return 40 * nnzb + 4 * (nb + 1) + 16 * n + 36 * nb;
+}
+
+// Whether a standalone operator runs the streaming kernels.  By working set
+// (operator_mapping 0): from 256 MiB on, beyond the 256 MB Infinity Cache,
+// the streaming kernels are faster (on 8.3M blocks: 128 vs 171 us for D x);
+// below it the row mapping's repeated reads hit the cache and its shorter
+// chain of dependent loads wins (on 0.26M blocks: 5.8 vs 9.6 us).
+bool streaming_ops(const hsc_handle* h, int kernel) {
+  if (h->cfg.operator_mapping == 1) return false;
+  if (h->cfg.operator_mapping == 2) return true;
+  return operator_bytes(kernel, h->nb, h->nnzb) >= (256ll << 20);
+}
+
+// One standalone operator launch with the handle's mapping: row mapping at
+// 16 lanes per row (8 for the FP32 pass, measured fastest at each size)
+void launch_operator(const hsc_handle* h, int kernel, const hsc::SysDesc& S, const void* x, void* y, double w,
+                     hipStream_t st) {
+  if (streaming_ops(h, kernel)) HIPCHK(hsc::launch_strip_op(kernel, S, strip_desc(h), x, y, w, st));
+  else HIPCHK(hsc::launch_row_op(kernel, kernel == 2 ? 8 : 16, S, x, y, w, st));
 }
 
 // XCD-aware chunk order: workgroup b runs on XCD b % 8; chunks are numbered
@@ -1088,6 +1132,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   ++h->gen;
   for (bool& ok : h->tnbr_ok) ok = false;
   h->pairs_hbl = -1;
+  h->strip_nch = -1;
   if (!Av) throw HscError{HSC_ERR_INVALID_ARGUMENT, "null matrix view"};
   const long long N = 3 * Av->block_dim;
   // linear_hsc.cu:197-200
@@ -1154,6 +1199,9 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->colidx.ensure(sizeof(int) * nnzb));
   HIPCHK(h->val64.ensure(sizeof(double) * hsc::val_count(nnzb)));
   HIPCHK(h->val32.ensure(sizeof(float) * hsc::val_count(nnzb)));
+  const int nch = (nnzb + 63) / 64;
+  HIPCHK(h->hbits.ensure(sizeof(unsigned long long) * (size_t)std::max(nch, 1)));
+  HIPCHK(h->hrow.ensure(sizeof(int) * (size_t)(nch + 1)));
   HIPCHK(h->E.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
   HIPCHK(h->R0.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
   HIPCHK(h->R1.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
@@ -1318,6 +1366,9 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   join.s = h->aux;
   HIPCHK(hsc::launch_sort_gather(h->rowptr.as<int>(), nb, h->key_col.as<int>(), h->key_src.as<int>(), Av->values,
                                  nnzb, h->colidx.as<int>(), h->val64.as<double>(), h->val32.as<float>(), h->aux));
+  HIPCHK(hsc::launch_chunk_heads(h->rowptr.as<int>(), nb, nnzb, h->hbits.as<unsigned long long>(),
+                                 h->hrow.as<int>(), h->aux));
+  h->strip_nch = nch;
   AuxJoin join2{nullptr};
   if (learned) {
     // the network needs D only: it runs on a third stream beside the power
@@ -1492,7 +1543,7 @@ void enqueue_finish(hsc_handle* h, const double* b, double* x, hipStream_t st) {
   HIPCHK(hsc::launch_backsub(h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), h->act_dof.as<int>(), b, x,
                              h->qo, h->qm, h->fo, h->a, h->k, h->small.as<double>(), st));
   hsc::SysDesc S = make_desc(h);
-  HIPCHK(hsc::launch_op_Dx64(8, S, x + h->fo, h->yfull.as<double>(), st));
+  launch_operator(h, 0, S, x + h->fo, h->yfull.as<double>(), 0.0, st);
   HIPCHK(hsc::launch_full_residual(h->yfull.as<double>(), h->E.as<double>(), h->A.as<double>(),
                                    h->act_node.as<int>(), h->act_dof.as<int>(), h->a, h->qm, b, x, h->fo, h->qo,
                                    h->fn, h->respart.as<double>(), h->misc.as<double>() + 20, st));
@@ -1636,6 +1687,8 @@ void hsc_config_default(hsc_config* c) {
   c->poly_interval_ratio = 100.0;
   c->pass_sync = 2;
   c->pipelined = 0;
+  c->operator_mapping = 0;
+  c->reserved0 = 0;
 }
 
 int32_t hsc_create(const hsc_config* cfg, int32_t device, hsc_handle** out) {
@@ -2062,8 +2115,9 @@ int32_t hsc_apply_D(hsc_handle* h, const double* x, double* y, hsc_stream stream
   DeviceGuard g(h->device);
   return guarded(h, [&] {
     hsc::SysDesc S = make_desc(h);
-    HIPCHK(hsc::launch_op_Dx64(h->G, S, x, y, (hipStream_t)stream));
-    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    hipStream_t st = (hipStream_t)stream;
+    launch_operator(h, 0, S, x, y, 0.0, st);
+    HIPCHK(hipStreamSynchronize(st));
     return HSC_OK;
   });
 }
@@ -2075,7 +2129,7 @@ int32_t hsc_apply_schur(hsc_handle* h, const double* x, double* y, hsc_stream st
   return guarded(h, [&] {
     hsc::SysDesc S = make_desc(h);
     hipStream_t st = (hipStream_t)stream;
-    HIPCHK(hsc::launch_op_Dx64(h->G, S, x, y, st));
+    launch_operator(h, 0, S, x, y, 0.0, st);
     HIPCHK(hsc::launch_op_lowrank64(S, x, y, h->ybuf.as<double>(), st));
     HIPCHK(hipStreamSynchronize(st));
     return HSC_OK;
@@ -2089,7 +2143,26 @@ int32_t hsc_apply_precond(hsc_handle* h, const double* r, double* z, hsc_stream 
   return guarded(h, [&] {
     hsc::SysDesc S = make_desc(h);
     hipStream_t st = (hipStream_t)stream;
-    HIPCHK(hsc::launch_op_precond64(h->G, S, h->cfg.neumann_order, h->omegas, r, z, h->z1.as<double>(), st));
+    const hsc::StripDesc T = strip_desc(h);
+    HIPCHK(hsc::launch_op_precond64(streaming_ops(h, 1) ? &T : nullptr, 16, S, h->cfg.neumann_order, h->omegas, r,
+                                    z, h->z1.as<double>(), st));
+    HIPCHK(hipStreamSynchronize(st));
+    return HSC_OK;
+  });
+}
+
+int32_t hsc_apply_pass(hsc_handle* h, int32_t precision, double w, const void* x, const double* r, void* y,
+                       hsc_stream stream) {
+  if (!h || !x || !r || !y || (precision != HSC_PRECISION_FP64 && precision != HSC_PRECISION_MIXED) ||
+      !std::isfinite(w))
+    return HSC_ERR_INVALID_ARGUMENT;
+  if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
+  DeviceGuard g(h->device);
+  return guarded(h, [&] {
+    hsc::SysDesc S = make_desc(h);
+    S.r = const_cast<double*>(r);
+    hipStream_t st = (hipStream_t)stream;
+    launch_operator(h, precision == HSC_PRECISION_FP64 ? 1 : 2, S, x, y, w, st);
     HIPCHK(hipStreamSynchronize(st));
     return HSC_OK;
   });
@@ -2183,9 +2256,10 @@ int32_t hsc_assemble_bcoo(const int32_t* rows, const int32_t* cols, const double
 
 int32_t hsc_time_kernel(hsc_handle* h, int32_t kernel, int32_t lanes_per_row, int32_t repeats, hsc_stream stream,
                         double* ms_per_launch) {
-  if (!h || !ms_per_launch || kernel < 0 || kernel > 2 || repeats < 1 ||
-      !(lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16))
+  if (!h || !ms_per_launch || kernel < 0 || kernel > 5 || repeats < 1) return HSC_ERR_INVALID_ARGUMENT;
+  if (kernel < 3 && !(lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16))
     return HSC_ERR_INVALID_ARGUMENT;
+  if (kernel >= 3 && (lanes_per_row < 0 || lanes_per_row > 64)) return HSC_ERR_INVALID_ARGUMENT;
   if (!h->setup_ok) return HSC_ERR_NOT_SETUP;
   DeviceGuard g(h->device);
   return guarded(h, [&] {
@@ -2202,8 +2276,14 @@ int32_t hsc_time_kernel(hsc_handle* h, int32_t kernel, int32_t lanes_per_row, in
     HIPCHK(hipEventCreate(&e1));
     hipError_t le = hipSuccess;
     (void)hipEventRecord(e0, st);
+    hsc::StripDesc T{};
+    if (kernel >= 3) {
+      T = strip_desc(h);
+      if (lanes_per_row > 0) T.nwave = (T.nch + lanes_per_row - 1) / lanes_per_row;
+    }
     for (int i = 0; i < repeats && le == hipSuccess; ++i)
-      le = hsc::launch_timed_op(kernel, lanes_per_row, S, h->p0.p, h->p1.p, st);
+      le = kernel < 3 ? hsc::launch_row_op(kernel, lanes_per_row, S, h->p0.p, h->p1.p, 0.5, st)
+                      : hsc::launch_strip_op(kernel - 3, S, T, h->p0.p, h->p1.p, 0.5, st);
     (void)hipEventRecord(e1, st);
     hipError_t se = hipEventSynchronize(e1);
     float ms = 0.f;
@@ -2232,6 +2312,7 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   o->pass_sync = h->pass_sync_used ? 1 : 0;
   o->pipelined = h->pipe_used ? 1 : 0;
   o->pass_pairs = h->pairs_used ? 1 : 0;
+  o->streaming_operators = streaming_ops(h, 0) ? 1 : 0;
   o->lambda_max = h->lambda_max;
   o->neumann_doublings = h->doublings;
   o->abd_block_diagonal = h->blockdiag ? 1 : 0;

@@ -78,7 +78,7 @@ def make_config(scene: dict | None = None, **extra) -> _abi.Config:
     """Build an hsc_config from libuipc-style keys plus extension fields
     (abd_inverse, precision, residual_replace_every,
     true_residual_every_iteration, aca_rtol, lanes_per_row, wg_per_cu, schedule,
-    poly_interval_ratio, pass_sync,
+    poly_interval_ratio, pass_sync, pipelined, operator_mapping,
     preconditioner, check_acceptance)."""
     lib = _abi.load()
     cfg = _abi.Config()
@@ -303,19 +303,31 @@ class Handle:
     def apply_precond(self, r, z, stream=None):
         self._check(self.lib.hsc_apply_precond(self.h, C.c_void_p(_ptr(r)), C.c_void_p(_ptr(z)), _stream(stream)))
 
+    def apply_pass(self, x, r, y, w: float, stream=None):
+        """y = x + w Dinv (r - D x) (hsc_apply_pass): x, y float64 with the FP64
+        values, or float32 with the FP32 copies; r float64."""
+        import torch
+        if x.dtype != y.dtype or x.dtype not in (torch.float64, torch.float32) or r.dtype != torch.float64:
+            raise TypeError("x and y: both float64 or both float32; r: float64")
+        prec = _abi.PRECISION["fp64"] if x.dtype == torch.float64 else _abi.PRECISION["mixed"]
+        self._check(self.lib.hsc_apply_pass(self.h, prec, float(w), C.c_void_p(_ptr(x)), C.c_void_p(_ptr(r)),
+                                            C.c_void_p(_ptr(y)), _stream(stream)))
+
     def setup_info(self) -> dict:
         si = _abi.SetupInfo()
         self._check(self.lib.hsc_get_setup_info(self.h, C.byref(si)))
         return si.as_dict()
 
-    TIMED_KERNELS = {"spmv64": 0, "pass64": 1, "pass32": 2}
+    TIMED_KERNELS = {"spmv64_rows": 0, "pass64_rows": 1, "pass32_rows": 2, "spmv64": 3, "pass64": 4, "pass32": 5}
 
-    def time_kernel(self, kernel: str, lanes_per_row: int = 8, repeats: int = 50, stream=None) -> float:
+    def time_kernel(self, kernel: str, shape: int = 0, repeats: int = 50, stream=None) -> float:
         """ms per launch of one standalone operator kernel on this handle's D
         (hsc_time_kernel): "spmv64" (y = D x), "pass64" / "pass32" (one
-        block-Jacobi pass in FP64 / FP32)."""
+        block-Jacobi pass in FP64 / FP32) with the streaming kernels, `shape`
+        64-block chunks per wavefront (0: the library's choice); the "_rows"
+        forms map `shape` (4, 8, 16) lanes to each block row."""
         ms = C.c_double()
-        self._check(self.lib.hsc_time_kernel(self.h, self.TIMED_KERNELS[kernel], lanes_per_row, repeats,
+        self._check(self.lib.hsc_time_kernel(self.h, self.TIMED_KERNELS[kernel], shape, repeats,
                                              _stream(stream), C.byref(ms)))
         return ms.value
 

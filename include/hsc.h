@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define HSC_ABI_VERSION 2
+#define HSC_ABI_VERSION 3
 
 /* Status codes: one per failure mode of the reference plugin. */
 enum hsc_status {
@@ -122,6 +122,12 @@ typedef struct hsc_config {
                                     row, Chebyshev/learned or Neumann passes of order >= 2, recursive-residual
                                     stopping): one grid barrier per iteration; q = D p kept by recurrence in
                                     FP64 (no counterpart; default 0 = the reference's loop) */
+  int32_t operator_mapping;      /* kernels of the standalone operators (hsc_apply_*) and of the solve's
+                                    acceptance D x: 0 (default) = by working set: the streaming kernels
+                                    (64-block chunks per wavefront) when the operation reads 256 MiB or more
+                                    (beyond the Infinity Cache), else the row mapping; 1 = row mapping
+                                    (16 lanes per row; 8 for FP32 passes); 2 = streaming */
+  int32_t reserved0;
 } hsc_config;
 
 /* Device view of the assembled Hessian: the stored triangle of 3x3 blocks
@@ -236,6 +242,14 @@ int32_t hsc_solve_batched(hsc_handle* const* handles, int32_t count,
 int32_t hsc_apply_D(hsc_handle* h, const double* x, double* y, hsc_stream stream);
 int32_t hsc_apply_schur(hsc_handle* h, const double* x, double* y, hsc_stream stream);
 int32_t hsc_apply_precond(hsc_handle* h, const double* r, double* z, hsc_stream stream);
+/* One block-Jacobi pass y = x + w Dinv (r - D x) over the FEM part, with the
+ * streaming pass kernel (no reference counterpart: it is one Horner step of
+ * the series that linear_hsc.cu:329-342 sums term by term).  precision
+ * HSC_PRECISION_FP64: x and y are double, with the FP64 values and inverse
+ * blocks; HSC_PRECISION_MIXED: x and y are float, with the FP32 copies (r is
+ * double in both).  x and y must not overlap. */
+int32_t hsc_apply_pass(hsc_handle* h, int32_t precision, double w, const void* x, const double* r, void* y,
+                       hsc_stream stream);
 
 /* Setup summary and host copies of the small factors (any pointer may be NULL).
  * U: qm x rank column-major; V: active x rank column-major; C: rank x rank;
@@ -256,6 +270,9 @@ typedef struct hsc_setup_info {
   int32_t pipelined;             /* 1 when the last solve's launch ran the pipelined PCG loop */
   int32_t pass_pairs;            /* 1 when the last solve ran its preconditioner passes in pairs (pass_sync 2,
                                     and every row tile's halo fit: otherwise one wait per pass) */
+  int32_t streaming_operators;   /* 1 when operator_mapping selects the streaming kernels for D x on this
+                                    system */
+  int32_t reserved0;
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 
@@ -326,10 +343,15 @@ int32_t hsc_assemble_bcoo_noalloc(const int32_t* rows, const int32_t* cols, cons
  * Times `repeats` back-to-back launches of one standalone operator kernel on
  * the set-up handle's D, between HIP events on `stream`: kernel 0 = FP64
  * y = D x, 1 = one FP64 block-Jacobi pass y = x + w Dinv (r - D x), 2 = the
- * same pass with FP32 values, inverse blocks and vectors (FP64 r), with
- * `lanes_per_row` (4, 8, 16) lanes per block row.  The operands are the
- * handle's own scratch vectors (zeroed first; their contents are lost).
- * *ms_per_launch = elapsed time / repeats. */
+ * same pass with FP32 values, inverse blocks and vectors (FP64 r), each with
+ * `lanes_per_row` (4, 8, 16) lanes per block row (the persistent solve's
+ * row mapping); kernels 3, 4, 5 = the same three operations with the
+ * streaming kernels that hsc_apply_D, hsc_apply_schur, hsc_apply_precond,
+ * hsc_apply_pass and the solve's acceptance D x run, where `lanes_per_row`
+ * is instead the number of 64-block chunks per wavefront (1..64; 0 = the
+ * library's choice).  The operands are the handle's own scratch vectors
+ * (zeroed first; their contents are lost).  *ms_per_launch = elapsed time /
+ * repeats. */
 int32_t hsc_time_kernel(hsc_handle* h, int32_t kernel, int32_t lanes_per_row, int32_t repeats, hsc_stream stream,
                         double* ms_per_launch);
 int32_t hsc_get_factors(hsc_handle* h, double* U, double* V, double* C, double* P,

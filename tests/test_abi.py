@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported(lib):
 
 
 def test_abi_version_and_defaults(lib):
-    assert lib.hsc_abi_version() == 2
+    assert lib.hsc_abi_version() == 3
     c = _abi.Config()
     lib.hsc_config_default(C.byref(c))
     # integration.patch:34-55 and scene_default_config.cpp:146-151 of the reference
@@ -46,6 +46,7 @@ def test_abi_version_and_defaults(lib):
     assert c.preconditioner == _abi.PRECONDITIONER["neumann"]
     # extensions: autotuned schedule, Chebyshev interval ratio, neighbour-tile pass sync
     assert (c.schedule, c.poly_interval_ratio, c.pass_sync) == (0, 100.0, 2)
+    assert (c.pipelined, c.operator_mapping) == (0, 0)
 
 
 def test_invalid_config_rejected(lib):
@@ -57,7 +58,8 @@ def test_invalid_config_rejected(lib):
     lib.hsc_config_default(C.byref(c))
     c.neumann_order = 0
     assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
-    for field, bad in (("schedule", 3), ("pass_sync", 3), ("pass_sync", -1), ("lanes_per_row", 6)):
+    for field, bad in (("schedule", 3), ("pass_sync", 3), ("pass_sync", -1), ("lanes_per_row", 6),
+                       ("operator_mapping", 3), ("operator_mapping", -1)):
         lib.hsc_config_default(C.byref(c))
         setattr(c, field, bad)
         assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT, field
@@ -68,6 +70,7 @@ def test_invalid_config_rejected(lib):
     assert lib.hsc_solve(None, None, None, None, None) == _abi.HSC_ERR_INVALID_ARGUMENT
     ms = C.c_double()
     assert lib.hsc_time_kernel(None, 0, 8, 1, None, C.byref(ms)) == _abi.HSC_ERR_INVALID_ARGUMENT
+    assert lib.hsc_apply_pass(None, 0, 0.5, None, None, None, None) == _abi.HSC_ERR_INVALID_ARGUMENT
 
 
 def test_stateless_entry_points_validate_before_any_device_call(lib):

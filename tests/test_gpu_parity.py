@@ -79,13 +79,16 @@ def test_operators_match_oracle(c1, c1_oracle):
 
 def test_time_kernel_measures_and_leaves_the_handle_usable(c1):
     """hsc_time_kernel: a positive time per launch for each standalone kernel
-    and lanes-per-row; the handle's next solve still matches the oracle
-    (the timed kernels overwrite only scratch vectors): within 1e-7 of the
-    FP64 direct solve."""
+    and lanes-per-row or chunks-per-wavefront; the handle's next solve still
+    matches the oracle (the timed kernels overwrite only scratch vectors):
+    within 1e-7 of the FP64 direct solve."""
     h, A = _setup(c1, C1_SCENE, aca_rtol=1e-6)
-    for k in ("spmv64", "pass64", "pass32"):
+    for k in ("spmv64_rows", "pass64_rows", "pass32_rows"):
         for G in (4, 8, 16):
             assert h.time_kernel(k, G, repeats=3) > 0.0
+    for k in ("spmv64", "pass64", "pass32"):
+        for chunks in (0, 1, 3, 64):
+            assert h.time_kernel(k, chunks, repeats=3) > 0.0
     x, st = _solve(h, c1)
     assert st["status"] == 0
     xd = osp.direct_solve(c1)
