@@ -2687,31 +2687,46 @@ hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, i
 // Sorted, distinct node lists through a bitmap over [lo, lo + 32 words): the
 // exclusive prefix of the words' popcounts (contiguous word runs per thread),
 // so a member's rank is pre[w] + popc(bits below it).  Returns the count.
-__device__ int bitmap_prefix(const unsigned* bits, int* pre, int words, int* tsum) {
+// Exclusive prefix sum of v over the workgroup (thread order); *total gets
+// the sum.  wsum holds kBlock / 64 ints.  Contains __syncthreads.
+__device__ int block_scan_excl(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    x += lane >= d ? y : 0;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    const int t = wsum[w];
+    off += w < wv ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + x - v;
+}
+
+// pre[w] = set bits of bits[0 .. w); returns the total.  Each thread counts
+// a run of consecutive words, a workgroup scan places the runs.
+__device__ int bitmap_prefix(const unsigned* bits, int* pre, int words, int* wsum) {
   const int tid = threadIdx.x;
   const int per = (words + kBlock - 1) / kBlock;
   const int w0 = min(words, tid * per), w1 = min(words, w0 + per);
   int mine = 0;
   for (int w = w0; w < w1; ++w) mine += __popc(bits[w]);
-  tsum[tid] = mine;
-  __syncthreads();
-  if (tid == 0) {
-    int acc = 0;
-    for (int i = 0; i < kBlock; ++i) {
-      const int v = tsum[i];
-      tsum[i] = acc;
-      acc += v;
-    }
-    tsum[kBlock] = acc;
-  }
-  __syncthreads();
-  int acc = tsum[tid];
+  int total = 0;
+  int acc = block_scan_excl(mine, wsum, &total);
   for (int w = w0; w < w1; ++w) {
     pre[w] = acc;
     acc += __popc(bits[w]);
   }
   __syncthreads();
-  return tsum[kBlock];
+  return total;
 }
 __device__ __forceinline__ bool bitmap_has(const unsigned* bits, int lo, int words, int c) {
   const int o = c - lo;
@@ -2754,15 +2769,21 @@ __device__ void block_minmaxor(int& lo, int& hi, int& f, int* red) {
 // 4 kHaloU blocks, a column range past a bitmap) sets the system's fail
 // word, and the solve keeps one synchronisation per pass.
 constexpr int kHaloBitmapWords = 4096;  // column range of 131,072 nodes
+constexpr int kHaloBlkMax = 4 * kHaloU * kHaloRows;  // the largest pack capacity (hbl)
+// Every phase is spread over the workgroup: the halo rows' blocks are
+// numbered 0 .. nblk-1 in halo-row order (block offsets by a workgroup scan)
+// and their global block indices and columns staged in LDS, so the column
+// loads, the H2 bitmap, the slots and the value copies run one block (or one
+// block component) per thread.
 __global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ rowptr, const int* __restrict__ colidx,
                                                       const float* __restrict__ val32,
                                                       const float* __restrict__ invD32, int nb, int hbl,
                                                       unsigned char* __restrict__ base, long long stride) {
   __shared__ unsigned s_b1[kHaloBitmapWords], s_b2[kHaloBitmapWords];
   __shared__ int s_p1[kHaloBitmapWords], s_p2[kHaloBitmapWords];
-  __shared__ int s_tsum[kBlock + 1];
-  __shared__ int s_h1[kHaloRows], s_len[kHaloRows], s_lp0[kHaloRows];
-  __shared__ int s_red[3 * (kBlock / 64)];
+  __shared__ int s_qf[kHaloBlkMax], s_col[kHaloBlkMax];
+  __shared__ int s_h1[kHaloRows], s_len[kHaloRows], s_lp0[kHaloRows], s_q0[kHaloRows];
+  __shared__ int s_red[3 * (kBlock / 64)], s_wsum[kBlock / 64];
   constexpr int R = kHaloTileRows;
   const int t = blockIdx.x, tid = threadIdx.x;
   const int r0 = t * R, r1 = min(nb, r0 + R);
@@ -2793,7 +2814,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ ro
     if (c < r0 || c >= r1) atomicOr(&s_b1[(c - lo1) >> 5], 1u << ((c - lo1) & 31));
   }
   __syncthreads();
-  const int nh = bitmap_prefix(s_b1, s_p1, words1, s_tsum);
+  const int nh = bitmap_prefix(s_b1, s_p1, words1, s_wsum);
   if (nh > kHaloRows) return give_up();
   for (int w = tid; w < words1; w += kBlock) {
     unsigned bits = s_b1[w];
@@ -2805,47 +2826,46 @@ __global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ ro
     }
   }
   __syncthreads();
-  // halo rows' lengths, their blocks' offsets, and H2's range
-  int lo2 = 0x7fffffff, hi2 = -1;
-  bad = 0;
+  // halo rows' lengths and first blocks; their blocks' offsets in the pack
+  int len = 0;
   if (tid < nh) {
     const int h = s_h1[tid];
-    const int q0 = rowptr[h], q1 = rowptr[h + 1];
-    bad = q1 - q0 > 4 * kHaloU;
-    s_len[tid] = q1 - q0;
-    for (int p = q0; p < q1; ++p) {
-      const int c = colidx[p];
-      if ((c < r0 || c >= r1) && !bitmap_has(s_b1, lo1, words1, c)) {
-        lo2 = min(lo2, c);
-        hi2 = max(hi2, c);
-      }
+    const int q0 = rowptr[h];
+    len = rowptr[h + 1] - q0;
+    s_q0[tid] = q0;
+    s_len[tid] = len;
+  }
+  int nblk = 0;
+  const int lp0 = block_scan_excl(len, s_wsum, &nblk);
+  if (nblk > hbl) return give_up();
+  if (tid < nh) {
+    s_lp0[tid] = lp0;
+    for (int b = 0; b < len; ++b) s_qf[lp0 + b] = s_q0[tid] + b;
+  }
+  __syncthreads();
+  // the halo blocks' columns, and H2's range
+  int lo2 = 0x7fffffff, hi2 = -1;
+  bad = len > 4 * kHaloU ? 1 : 0;
+  for (int f = tid; f < nblk; f += kBlock) {
+    const int c = colidx[s_qf[f]];
+    s_col[f] = c;
+    if ((c < r0 || c >= r1) && !bitmap_has(s_b1, lo1, words1, c)) {
+      lo2 = min(lo2, c);
+      hi2 = max(hi2, c);
     }
   }
   block_minmaxor(lo2, hi2, bad, s_red);
   const int words2 = hi2 >= lo2 ? (hi2 - lo2 + 1 + 31) / 32 : 0;
   if (bad || words2 > kHaloBitmapWords) return give_up();
-  if (tid == 0) {
-    int acc = 0;
-    for (int i = 0; i < nh; ++i) {
-      s_lp0[i] = acc;
-      acc += s_len[i];
-    }
-    s_tsum[0] = acc;
-  }
   for (int w = tid; w < words2; w += kBlock) s_b2[w] = 0u;
   __syncthreads();
-  const int nblk = s_tsum[0];
-  if (nblk > hbl) return give_up();
-  if (tid < nh) {
-    const int h = s_h1[tid];
-    for (int p = rowptr[h]; p < rowptr[h + 1]; ++p) {
-      const int c = colidx[p];
-      if ((c < r0 || c >= r1) && !bitmap_has(s_b1, lo1, words1, c))
-        atomicOr(&s_b2[(c - lo2) >> 5], 1u << ((c - lo2) & 31));
-    }
+  for (int f = tid; f < nblk; f += kBlock) {
+    const int c = s_col[f];
+    if ((c < r0 || c >= r1) && !bitmap_has(s_b1, lo1, words1, c))
+      atomicOr(&s_b2[(c - lo2) >> 5], 1u << ((c - lo2) & 31));
   }
   __syncthreads();
-  const int n2 = bitmap_prefix(s_b2, s_p2, words2, s_tsum);
+  const int n2 = bitmap_prefix(s_b2, s_p2, words2, s_wsum);
   const int nset = R + nh + n2;
   if (nset > kHaloSet) return give_up();
   auto slot_of = [&](int c) -> int {
@@ -2875,33 +2895,35 @@ __global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ ro
     const int h = s_h1[tid];
     setn[tid] = h;
     reinterpret_cast<int2*>(pk + kHaloOffRows)[tid] = make_int2(s_len[tid], s_lp0[tid]);
-    float* inv = reinterpret_cast<float*>(pk + kHaloOffInv) + 9 * tid;
-    for (int e = 0; e < 9; ++e) inv[e] = invD32[9 * (size_t)h + e];
-    unsigned short* bslot = reinterpret_cast<unsigned short*>(pk + kHaloOffBlk);
-    float* val = reinterpret_cast<float*>(pk + halo_val_off(hbl));
-    const int lp0 = s_lp0[tid], q0 = rowptr[h];
-    for (int b = 0; b < s_len[tid]; ++b) {
-      bslot[lp0 + b] = (unsigned short)slot_of(colidx[q0 + b]);
-      for (int e = 0; e < 9; ++e) val[9 * (size_t)(lp0 + b) + e] = val32[val_index(q0 + b, e)];
-    }
   }
+  float* inv = reinterpret_cast<float*>(pk + kHaloOffInv);
+  for (int j = tid; j < 9 * nh; j += kBlock) inv[j] = invD32[9 * (size_t)s_h1[j / 9] + j % 9];
+  unsigned short* bslot = reinterpret_cast<unsigned short*>(pk + kHaloOffBlk);
+  for (int f = tid; f < nblk; f += kBlock) bslot[f] = (unsigned short)slot_of(s_col[f]);
+  float* val = reinterpret_cast<float*>(pk + halo_val_off(hbl));
+#pragma unroll 4
+  for (int j = tid; j < 9 * nblk; j += kBlock) val[j] = val32[val_index(s_qf[j / 9], j % 9)];
   // set slot of the column of block u of each own-row lane (4 lanes per row);
   // a lane past its row's end gathers its own row (value 0 in the sum)
   {
     const int g = tid >> 2, l = tid & 3, row = r0 + g;
     unsigned short* oslot = reinterpret_cast<unsigned short*>(pk + kHaloOffSlot);
     const int q0 = row < r1 ? rowptr[row] : 0, q1 = row < r1 ? rowptr[row + 1] : 0;
+    int cu[kHaloU];
+#pragma unroll
     for (int u = 0; u < kHaloU; ++u) {
       const int p = q0 + l + 4 * u;
-      oslot[u * kBlock + tid] = (unsigned short)(p < q1 ? slot_of(colidx[p]) : g);
+      cu[u] = p < q1 ? colidx[p] : -1;
     }
+#pragma unroll
+    for (int u = 0; u < kHaloU; ++u) oslot[u * kBlock + tid] = (unsigned short)(cu[u] >= 0 ? slot_of(cu[u]) : g);
   }
 }
 
 hipError_t launch_tile_pairs(const int* rowptr, const int* colidx, const float* val32, const float* invD32, int nb,
                              int hbl, unsigned char* halo, hipStream_t st) {
   const int rtl = (nb + kHaloTileRows - 1) / kHaloTileRows;
-  if (rtl <= 0 || rtl > kNbrMaxTiles || hbl < 0) return hipErrorInvalidValue;
+  if (rtl <= 0 || rtl > kNbrMaxTiles || hbl < 0 || hbl > kHaloBlkMax) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned*>(halo),
                      (long long)(kHaloBaseBytes / 4));
   hipLaunchKernelGGL(k_tile_halo, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, val32, invD32, nb, hbl, halo,

@@ -270,12 +270,16 @@ struct hsc_handle {
   // learned: a third stream runs the network on D beside the power iteration
   hipStream_t aux2 = nullptr;
   hipEvent_t d_ev = nullptr;
+  // the end of each D-side chain, waited for by the main stream
+  hipEvent_t aux_done = nullptr, aux2_done = nullptr;
   HostBuf stage;
   ~hsc_handle() {
     if (aux) (void)hipStreamSynchronize(aux);
     if (aux2) (void)hipStreamSynchronize(aux2);
     if (aux_ev) (void)hipEventDestroy(aux_ev);
     if (d_ev) (void)hipEventDestroy(d_ev);
+    if (aux_done) (void)hipEventDestroy(aux_done);
+    if (aux2_done) (void)hipEventDestroy(aux2_done);
     if (aux) (void)hipStreamDestroy(aux);
     if (aux2) (void)hipStreamDestroy(aux2);
   }
@@ -1231,11 +1235,13 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   if (!h->aux) {
     HIPCHK(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&h->aux_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->aux_done, hipEventDisableTiming));
   }
   const bool learned = h->cfg.preconditioner == HSC_PRECOND_LEARNED;
   if (learned && !h->aux2) {
     HIPCHK(hipStreamCreateWithFlags(&h->aux2, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&h->d_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->aux2_done, hipEventDisableTiming));
   }
   HIPCHK(h->barrier.ensure(sizeof(unsigned) * hsc::kBarrierWords));
   HIPCHK(h->errflag.ensure(64));
@@ -1370,17 +1376,27 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                                  h->hrow.as<int>(), h->aux));
   h->strip_nch = nch;
   AuxJoin join2{nullptr};
+  // the network needs D only: it runs on a third stream beside the power
+  // iteration, enqueued after it (the power iteration's GPU chain is the
+  // shorter one, so it starts first and the two end together)
   if (learned) {
-    // the network needs D only: it runs on a third stream beside the power
-    // iteration
     HIPCHK(hipEventRecord(h->d_ev, h->aux));
     HIPCHK(hipStreamWaitEvent(h->aux2, h->d_ev, 0));
     join2.s = h->aux2;
-    enqueue_prediction(h, h->aux2, ss->pred);
   }
   ss->lam2 = 0.0;
   if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
+  if (learned) enqueue_prediction(h, h->aux2, ss->pred);
+  // the main stream waits for both D-side chains, so that one
+  // synchronisation returns every result of the three streams
+  HIPCHK(hipEventRecord(h->aux_done, h->aux));
+  HIPCHK(hipStreamWaitEvent(st, h->aux_done, 0));
+  if (learned) {
+    HIPCHK(hipEventRecord(h->aux2_done, h->aux2));
+    HIPCHK(hipStreamWaitEvent(st, h->aux2_done, 0));
+  }
   HIPCHK(hipStreamSynchronize(st));
+  join.s = join2.s = nullptr;  // both chains have ended
   const int* ints2 = ss->ints2;
   const double* dm = ss->dm;
   const int aca_err = ss->aca_err;
@@ -1399,8 +1415,6 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   h->aca_err = (limit > 0) ? ss->hist[rank] : 0.0;
   h->k = rank;
   if (rank > 0 && ss->fst != 0.0) throw HscError{HSC_ERR_NONFINITE_FACTORS, "HSC: nonfinite or non-SPD coupling core C"};
-  HIPCHK(hipStreamSynchronize(h->aux));
-  if (learned) HIPCHK(hipStreamSynchronize(h->aux2));
   set_poly_weights(h, ss->lam2, ss->pred);
   prepare_launch(h, st);
   h->setup_ok = true;
