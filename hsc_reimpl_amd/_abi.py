@@ -75,7 +75,7 @@ class SetupInfo(C.Structure):
         ("schedule", C.c_int32), ("pass_sync", C.c_int32), ("lambda_max", C.c_double),
         ("poly_ratio", C.c_double), ("learned_out", C.c_double * 3), ("learned_used", C.c_int32),
         ("fallback_reason", C.c_int32), ("pipelined", C.c_int32), ("pass_pairs", C.c_int32),
-        ("streaming_operators", C.c_int32), ("reserved0", C.c_int32),
+        ("streaming_operators", C.c_int32), ("row_tiles", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -101,6 +101,8 @@ struct SysDesc {
   void* z[2];
   void* q;
   double* q64;            // FP64 D x for the true residual (== q in FP64 mode)
+  const int* tstart;      // latency schedule at 4 lanes per row: row tile t covers rows [tstart[t], tstart[t+1])
+                          // (at most 128, cut along D's components); null: rows [R t, R t + R)
   const int* nbr;         // [row tiles * kNbrStride] neighbour lists (local tile ids), or null
   const unsigned char* halo;  // pass pairs: kHaloBaseBytes header, then one halo pack per row tile, or null
   unsigned long long* zt;     // pass pairs: two buffers of 3 nb tagged pass values (FP32 bits | round << 32)
@@ -108,6 +110,21 @@ struct SysDesc {
   int max_iter;
   int pad_;
 };
+
+// Row of thread group g in local row tile tl of R rows: the tile's first row
+// plus g, or a row past the system (nb or more) when g is past the tile's
+// end.  With a tile table (S.tstart, R = 128) the tiles follow it.
+__device__ __forceinline__ int tile_row(const SysDesc& S, int tl, int R, int g) {
+  if (S.tstart) {
+    const int a = S.tstart[tl];
+    return a + g < S.tstart[tl + 1] ? a + g : S.nb;
+  }
+  return tl * R + g;
+}
+__device__ __forceinline__ int tile_first(const SysDesc& S, int tl, int R) { return S.tstart ? S.tstart[tl] : tl * R; }
+__device__ __forceinline__ int tile_end(const SysDesc& S, int tl, int R) {
+  return S.tstart ? S.tstart[tl + 1] : (tl * R + R < S.nb ? tl * R + R : S.nb);
+}
 
 // Chunk tables of the streaming (strip) operator kernels: for chunk k of 64
 // blocks, bit j of hbits[k] is set when block 64k + j is the first block of

@@ -177,14 +177,22 @@ hipError_t launch_strip_op(int kernel, const SysDesc& S, const StripDesc& T, con
 hipError_t launch_op_precond64(const StripDesc* T, int G, const SysDesc& S, int order, const double* omegas,
                                const double* r, double* z, double* tmp, hipStream_t st);
 hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st);
-// row-tile neighbour lists for R rows per tile (bits: ceil(rtl/32)*rtl words,
-// list: rtl*kNbrStride ints, rtl = ceil(nb/R) <= kNbrMaxTiles)
-hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, int R, unsigned* bits, int* list,
-                                 hipStream_t st);
-// pass pairs of a system (128-row tiles): the halo packs (kHaloBaseBytes
-// header with the fail word, then halo_pack_bytes(hbl) per tile) into halo
+// body-aligned row tiles (one workgroup): tstart (row_tiles_capacity() ints)
+// gets the tile starts and nb, out[0] the tile count (0: none, the fixed
+// tiles apply), out[1] the most coupled nodes in one tile
+hipError_t launch_row_tiles(const int* rowptr, const int* colidx, const int* act_node, int nb, int* tstart, int* out,
+                            hipStream_t st);
+int row_tiles_capacity();
+// row-tile neighbour lists for R rows per tile, or for the ntiles tiles of a
+// tile table (bits: ceil(rtl/32)*rtl words, list: rtl*kNbrStride ints,
+// rtl = ceil(nb/R) or ntiles, at most kNbrMaxTiles)
+hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, int R, const int* tstart, int ntiles,
+                                 unsigned* bits, int* list, hipStream_t st);
+// pass pairs of a system (128-row tiles, or the ntiles tiles of a tile
+// table): the halo packs (kHaloBaseBytes header with the fail word, then
+// halo_pack_bytes(hbl) per tile) into halo
 hipError_t launch_tile_pairs(const int* rowptr, const int* colidx, const float* val32, const float* invD32, int nb,
-                             int hbl, unsigned char* halo, hipStream_t st);
+                             const int* tstart, int ntiles, int hbl, unsigned char* halo, hipStream_t st);
 // static LDS bytes of the persistent PCG kernel variant
 hipError_t pcg_static_lds(int G, bool mixed, bool res, bool pipe, size_t* bytes);
 // power iteration: z = Dinv D x / |x| with |x|^2 the sum of part_in[0:np_in];

@@ -150,7 +150,7 @@ __device__ __forceinline__ void tile_A(const SysDesc& S, int tl, const VT* __res
                                        const VT* __restrict__ pold, VT* __restrict__ pnew,
                                        VT* __restrict__ q, VT beta, double& dot) {
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
-  const int row = tl * (kBlock / G) + g;
+  const int row = tile_row(S, tl, kBlock / G, g);
   VT a0, a1, a2;
   row_spmv<G, VT, U>(S, mat_vals<VT>(S), row, lane,
                   [&](int c, VT& x0, VT& x1, VT& x2) {
@@ -174,7 +174,7 @@ template <int G, typename VT, int U = 1>
 __device__ __forceinline__ void tile_N(const SysDesc& S, int tl, const VT* __restrict__ zin,
                                        VT* __restrict__ zout, VT omega, bool last, double& rz) {
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
-  const int row = tl * (kBlock / G) + g;
+  const int row = tile_row(S, tl, kBlock / G, g);
   VT a0, a1, a2;
   row_spmv<G, VT, U>(S, mat_vals<VT>(S), row, lane,
                   [&](int c, VT& x0, VT& x1, VT& x2) {
@@ -373,7 +373,7 @@ template <int G, int U = 1>
 __device__ __forceinline__ void tile_Dx64(const SysDesc& S, int tl, const double* __restrict__ x,
                                           double* __restrict__ y) {
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
-  const int row = tl * (kBlock / G) + g;
+  const int row = tile_row(S, tl, kBlock / G, g);
   double a0, a1, a2;
   row_spmv<G, double, U>(S, S.val64, row, lane,
                       [&](int c, double& x0, double& x1, double& x2) {
@@ -787,7 +787,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   RowRes<G, VT> rrF;
   if (resF) {
     sF = sys_of_tile_begin(sys, nsys, trF.t0, 0);
-    rowF = (trF.t0 - sys[sF].rt_begin) * (kBlock / G) + tid / G;
+    rowF = tile_row(sys[sF], trF.t0 - sys[sF].rt_begin, kBlock / G, tid / G);
     row_preload<G, VT>(sys[sF], mat_vals<VT>(sys[sF]), inv_vals<VT>(sys[sF]), rowF, tid % G, rrF);
   }
   // stage A on the same resident tile (aligned split)
@@ -799,7 +799,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
   int sE = 0, iE = 0, anE = -1;
   if (resE) {
     sE = sys_of_tile_begin(sys, nsys, trE.t0, 1);
-    iE = (trE.t0 - sys[sE].et_begin) * P.node_tile + tid;
+    iE = tile_row(sys[sE], trE.t0 - sys[sE].et_begin, P.node_tile, tid);
     if (tid < P.node_tile && iE < sys[sE].nb && sys[sE].k > 0) anE = sys[sE].act_node[iE];
   }
 
@@ -899,7 +899,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                 const int st = s_st[s];
                 if (st != ST_RUN) return;
                 if (kind == 2 && s_chk[s] == CHK_NONE) return;
-                const int i = tl * P.node_tile + tid;
+                const int i = tile_row(S, tl, P.node_tile, tid);
                 const bool valid = tid < P.node_tile && i < S.nb;
                 double g[3] = {0.0, 0.0, 0.0};
                 if (kind != 0 && S.k > 0) {
@@ -1084,7 +1084,8 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       const int lane = tid & 3, g = tid >> 2;
       const bool own = rowF < S.nb && lane < 3;
       const int ro = own ? rowF : 0, lo = own ? lane : 0;
-      const int r0t = rowF - g;  // first row of the tile
+      const int tlF = trF.t0 - S.rt_begin;
+      const int r0t = tile_first(S, tlF, R), r1t = tile_end(S, tlF, R);  // the tile's rows
       const int* hdr = reinterpret_cast<const int*>(s_halo);
       const int nh = hdr[0], nset = hdr[1];
       const int2* hrows = reinterpret_cast<const int2*>(s_halo + kHaloOffRows);
@@ -1101,7 +1102,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
       // iteration, r of the own row and of the halo row at slot tid)
       {
         const int k0 = tid, k1 = tid + kBlock;
-        const bool v0 = k0 < nset && (k0 >= R || r0t + k0 < S.nb);
+        const bool v0 = k0 < nset && (k0 >= R || r0t + k0 < r1t);
         const bool v1 = k1 < nset;
         const int n0 = v0 ? (k0 < R ? r0t + k0 : setn[k0 - R]) : 0;
         const int n1 = v1 ? setn[k1 - R] : 0;
@@ -1178,7 +1179,65 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
               pass_out(lane == 0 ? zo.x : (lane == 1 ? zo.y : zo.z), w1, rrF.d0, rrF.d1, rrF.d2, r0, r1, r2, a0, a1,
                        a2);
       }
-      // pass j on the halo rows (4 lanes per row, rows g and 128 + g)
+      // pass j on the halo rows.  Up to 128 of them: 4 lanes per row, one
+      // round.  More (the tiles where two bodies meet): 2 lanes per row in
+      // one round instead of a second round, lane m forming the partial
+      // sums of lanes m and m + 2 of the 4-lane split in turn and adding
+      // them, then the pair's sum: a_m + a_(m+2), then the two -- the
+      // operands and order of the 4-lane butterfly, so every row is summed
+      // bit for bit as its owner sums it.
+      if (nh > R) {
+        const int i = tid >> 1, m = tid & 1;
+        const bool hv = i < nh;
+        const int2 rw = hv ? hrows[i] : make_int2(0, 0);
+        VT s0 = VT(0), s1 = VT(0), s2 = VT(0);
+#pragma unroll 1
+        for (int h2 = 0; h2 < 2; ++h2) {
+          const int vl = m + 2 * h2;
+          int pb[kHaloU], sl[kHaloU];
+#pragma unroll
+          for (int u = 0; u < kHaloU; ++u) {
+            const int bi = vl + 4 * u;
+            pb[u] = bi < rw.x ? rw.y + bi : 0;
+          }
+#pragma unroll
+          for (int u = 0; u < kHaloU; ++u) sl[u] = bslot[pb[u]];
+          float4 xb[kHaloU];
+#pragma unroll
+          for (int u = 0; u < kHaloU; ++u) xb[u] = zs[sl[u]];
+          VT b0 = VT(0), b1 = VT(0), b2 = VT(0);
+#pragma unroll
+          for (int u = 0; u < kHaloU; ++u) {
+            const float* vp = hval + 9 * (size_t)pb[u];
+            VT v[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) v[e] = vp[e];
+            blk_step(vl + 4 * u < rw.x, v, xb[u].x, xb[u].y, xb[u].z, b0, b1, b2);
+          }
+          if (h2 == 0) {
+            s0 = b0;
+            s1 = b1;
+            s2 = b2;
+          } else {
+            s0 = s0 + b0;
+            s1 = s1 + b1;
+            s2 = s2 + b2;
+          }
+        }
+        s0 = group_sum<2>(s0);
+        s1 = group_sum<2>(s1);
+        s2 = group_sum<2>(s2);
+        if (hv) {
+          const float* d = hinv + 9 * i;
+          const float4 rr = rs[i];
+          const float4 zo = zs[R + i];
+          float* zi = reinterpret_cast<float*>(zl + R + i);
+          zi[m] = pass_out(m == 0 ? zo.x : zo.y, w1, d[3 * m], d[3 * m + 1], d[3 * m + 2], (double)rr.x, (double)rr.y,
+                           (double)rr.z, s0, s1, s2);
+          if (m == 0)
+            zi[2] = pass_out(zo.z, w1, d[6], d[7], d[8], (double)rr.x, (double)rr.y, (double)rr.z, s0, s1, s2);
+        }
+      } else
 #pragma unroll
       for (int q = 0; q < kHaloRows / R; ++q) {
         if (q * R >= nh) break;  // (the same for the whole workgroup)
@@ -1799,7 +1858,7 @@ __global__ __launch_bounds__(kBlock) void pcg_persistent(PcgParams P) {
                   [&](int s, int tl) {
                     const SysDesc& S = sys[s];
                     if (s_st[s] != ST_RUN || s_redo[s]) return;
-                    const int i = tl * P.node_tile + tid;
+                    const int i = tile_row(S, tl, P.node_tile, tid);
                     const bool valid = tid < P.node_tile && i < S.nb;
                     double g[3] = {0.0, 0.0, 0.0};
                     if (S.k > 0)
@@ -2626,16 +2685,30 @@ hipError_t launch_op_precond64(const StripDesc* T, int G, const SysDesc& S, int 
 // bits[t][u] = 1 when a row of tile t (R rows per tile) has a block in a row
 // of tile u != t: one workgroup per tile, the tile's blocks are contiguous
 __global__ __launch_bounds__(kBlock) void k_tile_nbr_bits(const int* __restrict__ rowptr,
-                                                          const int* __restrict__ colidx, int nb, int R, int words,
+                                                          const int* __restrict__ colidx, int nb, int R,
+                                                          const int* __restrict__ tstart, int rtl, int words,
                                                           unsigned* __restrict__ bits) {
   __shared__ unsigned s_bits[kNbrMaxTiles / 32];
+  __shared__ int s_ts[kNbrMaxTiles + 1];
   const int t = blockIdx.x;
   for (int w = threadIdx.x; w < words; w += kBlock) s_bits[w] = 0u;
+  if (tstart)
+    for (int i = threadIdx.x; i <= rtl; i += kBlock) s_ts[i] = tstart[i];
   __syncthreads();
-  const int r0 = t * R, r1 = min(nb, r0 + R);
+  const int r0 = tstart ? s_ts[t] : t * R, r1 = tstart ? s_ts[t + 1] : min(nb, r0 + R);
   const int p0 = rowptr[r0], p1 = rowptr[r1];
   for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
-    const int u = colidx[p] / R;
+    const int c = colidx[p];
+    int u = c / R;
+    if (tstart) {  // last tile starting at or before c
+      int lo = 0, hi = rtl - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_ts[mid] <= c) lo = mid;
+        else hi = mid - 1;
+      }
+      u = lo;
+    }
     if (u != t) atomicOr(&s_bits[u >> 5], 1u << (u & 31));
   }
   __syncthreads();
@@ -2673,20 +2746,194 @@ __global__ __launch_bounds__(kBlock) void k_tile_nbr_list(const unsigned* __rest
   if (threadIdx.x == 0) list[(size_t)t * kNbrStride] = base <= kNbrStride - 1 ? base : -1;
 }
 
-hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, int R, unsigned* bits, int* list,
-                                 hipStream_t st) {
-  const int rtl = (nb + R - 1) / R;
+hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, int R, const int* tstart, int ntiles,
+                                 unsigned* bits, int* list, hipStream_t st) {
+  const int rtl = tstart ? ntiles : (nb + R - 1) / R;
   if (rtl <= 0 || rtl > kNbrMaxTiles) return hipErrorInvalidValue;
   const int words = (rtl + 31) / 32;
-  hipLaunchKernelGGL(k_tile_nbr_bits, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, nb, R, words, bits);
+  hipLaunchKernelGGL(k_tile_nbr_bits, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, nb, R, tstart, rtl, words, bits);
   hipLaunchKernelGGL(k_tile_nbr_list, dim3(rtl), dim3(kBlock), 0, st, bits, rtl, words, list);
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- body-aligned row tiles
+// Row tiles of the latency schedule at 4 lanes per row, cut along D's
+// connected components: a component of at least kTileRows rows (a body's
+// mesh) starts a new tile, and tiles are cut every kTileRows rows from the
+// last such start; smaller components share tiles.  A tile that straddles
+// two bodies pays for both bodies' halos in every preconditioner pass, and
+// with fixed 128-row tiles the five tiles where the 78k-tet system's tori
+// meet set the pace of every pass pair.  Row r ends a component when no row
+// r' <= r references a column beyond r: the running maximum of each row's
+// largest column (rows keep their diagonal first, then columns ascending)
+// equals r.  One workgroup: rows are staged kBlock x 16 at a time into LDS
+// with coalesced loads, each thread scans its 16 consecutive rows, and
+// workgroup scans carry the running maximum and the last component end.
+constexpr int kTileRows = 128;
+constexpr int kTileStage = 16;                   // rows per thread per pass
+constexpr int kTiledRowsMax = kTileRows * kNbrMaxTiles;
+
+__device__ int block_scan_excl(int v, int* wsum, int* total);
+
+// Exclusive maximum of v over the workgroup (thread order, -1 for thread
+// 0); *total gets the maximum over all threads.  Contains __syncthreads.
+__device__ int block_scan_max_excl(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    x = lane >= d ? max(x, y) : x;
+  }
+  int ex = __shfl_up(x, 1, 64);
+  ex = lane >= 1 ? ex : -1;
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  int off = -1, tot = -1;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    const int t = wsum[w];
+    off = w < wv ? max(off, t) : off;
+    tot = max(tot, t);
+  }
+  __syncthreads();
+  *total = tot;
+  return max(off, ex);
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                      const int* __restrict__ act_node, int nb,
+                                                      int* __restrict__ tstart, int* __restrict__ out) {
+  __shared__ int s_reach[kBlock * kTileStage];
+  __shared__ int s_big[kNbrMaxTiles + 1];
+  __shared__ int s_tile[kNbrMaxTiles + 1];
+  __shared__ int s_w[kBlock / 64];
+  const int tid = threadIdx.x;
+  if (nb <= 0 || nb > kTiledRowsMax) {
+    if (tid == 0) out[0] = out[1] = 0;
+    return;
+  }
+  int carry_max = -1, carry_end = -1, nbig = 0;
+  for (int base = 0; base < nb; base += kBlock * kTileStage) {
+#pragma unroll
+    for (int j = 0; j < kTileStage; ++j) {
+      const int r = base + j * kBlock + tid;
+      s_reach[j * kBlock + tid] = r < nb ? max(r, colidx[rowptr[r + 1] - 1]) : 0x7fffffff;
+    }
+    __syncthreads();
+    const int r0 = base + tid * kTileStage;
+    const int* rr = s_reach + tid * kTileStage;
+    int lm = -1;
+#pragma unroll
+    for (int j = 0; j < kTileStage; ++j) lm = r0 + j < nb ? max(lm, rr[j]) : lm;
+    int tmax = 0;
+    const int before = max(carry_max, block_scan_max_excl(lm, s_w, &tmax));
+    int run = before, last_end = -1;
+#pragma unroll
+    for (int j = 0; j < kTileStage; ++j) {
+      run = max(run, rr[j]);
+      if (r0 + j < nb && run == r0 + j) last_end = r0 + j;
+    }
+    int tend = 0;
+    const int prev_end = max(carry_end, block_scan_max_excl(last_end, s_w, &tend));
+    // starts of the components of at least kTileRows rows that end in this
+    // run, in order
+    int prev = prev_end, cnt = 0;
+    run = before;
+#pragma unroll
+    for (int j = 0; j < kTileStage; ++j) {
+      run = max(run, rr[j]);
+      if (r0 + j < nb && run == r0 + j) {
+        cnt += (r0 + j - prev >= kTileRows) ? 1 : 0;
+        prev = r0 + j;
+      }
+    }
+    int total = 0;
+    int pos = nbig + block_scan_excl(cnt, s_w, &total);
+    prev = prev_end;
+    run = before;
+#pragma unroll
+    for (int j = 0; j < kTileStage; ++j) {
+      run = max(run, rr[j]);
+      if (r0 + j < nb && run == r0 + j) {
+        if (r0 + j - prev >= kTileRows && pos < kNbrMaxTiles) s_big[pos++] = prev + 1;
+        prev = r0 + j;
+      }
+    }
+    nbig += total;
+    carry_max = max(carry_max, tmax);
+    carry_end = max(carry_end, tend);
+    __syncthreads();
+  }
+  if (nbig > kNbrMaxTiles) {
+    if (tid == 0) out[0] = out[1] = 0;
+    return;
+  }
+  // segments: [0 or a big start, the next big start or nb); tiles every
+  // kTileRows rows from a segment's start
+  const bool lead = nbig == 0 || s_big[0] != 0;
+  const int nseg = nbig + (lead ? 1 : 0);
+  auto seg_start = [&](int j) { return lead ? (j == 0 ? 0 : s_big[j - 1]) : s_big[j]; };
+  auto seg_end = [&](int j) { return j + 1 < nseg ? seg_start(j + 1) : nb; };
+  const int per = (nseg + kBlock - 1) / kBlock;
+  const int j0 = min(nseg, tid * per), j1 = min(nseg, j0 + per);
+  int mine = 0;
+  for (int j = j0; j < j1; ++j) mine += (seg_end(j) - seg_start(j) + kTileRows - 1) / kTileRows;
+  int ntiles = 0;
+  int off = block_scan_excl(mine, s_w, &ntiles);
+  if (ntiles > kNbrMaxTiles) {
+    if (tid == 0) out[0] = out[1] = 0;
+    return;
+  }
+  for (int j = j0; j < j1; ++j)
+    for (int a = seg_start(j), e = seg_end(j); a < e; a += kTileRows) s_tile[off++] = a;
+  if (tid == 0) s_tile[ntiles] = nb;
+  __syncthreads();
+  for (int t = tid; t <= ntiles; t += kBlock) tstart[t] = s_tile[t];
+  // most coupled nodes in one tile (sizes the LDS cache of their W rows):
+  // coupled rows counted into their tile (binary search of the starts)
+  int* s_cnt = s_big;
+  for (int t = tid; t < ntiles; t += kBlock) s_cnt[t] = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += kBlock * kTileStage) {
+    int an[kTileStage];
+#pragma unroll
+    for (int j = 0; j < kTileStage; ++j) {
+      const int r = base + j * kBlock + tid;
+      an[j] = r < nb ? act_node[r] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kTileStage; ++j) {
+      if (an[j] < 0) continue;
+      const int r = base + j * kBlock + tid;
+      int lo = 0, hi = ntiles - 1;  // last tile starting at or before r
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_tile[mid] <= r) lo = mid;
+        else hi = mid - 1;
+      }
+      atomicAdd(&s_cnt[lo], 1);
+    }
+  }
+  __syncthreads();
+  int mc = 0;
+  for (int t = tid; t < ntiles; t += kBlock) mc = max(mc, s_cnt[t]);
+  int mct = 0;
+  (void)block_scan_max_excl(mc, s_w, &mct);
+  if (tid == 0) {
+    out[0] = ntiles;
+    out[1] = max(mct, 0);
+  }
+}
+
+hipError_t launch_row_tiles(const int* rowptr, const int* colidx, const int* act_node, int nb, int* tstart, int* out,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_row_tiles, dim3(1), dim3(kBlock), 0, st, rowptr, colidx, act_node, nb, tstart, out);
+  return hipGetLastError();
+}
+int row_tiles_capacity() { return kNbrMaxTiles + 1; }
+
 // ---------------------------------------------------------------- pass-pair tables
-// Sorted, distinct node lists through a bitmap over [lo, lo + 32 words): the
-// exclusive prefix of the words' popcounts (contiguous word runs per thread),
-// so a member's rank is pre[w] + popc(bits below it).  Returns the count.
 // Exclusive prefix sum of v over the workgroup (thread order); *total gets
 // the sum.  wsum holds kBlock / 64 ints.  Contains __syncthreads.
 __device__ int block_scan_excl(int v, int* wsum, int* total) {
@@ -2711,8 +2958,11 @@ __device__ int block_scan_excl(int v, int* wsum, int* total) {
   return off + x - v;
 }
 
-// pre[w] = set bits of bits[0 .. w); returns the total.  Each thread counts
-// a run of consecutive words, a workgroup scan places the runs.
+// Sorted, distinct node lists through a bitmap over [lo, lo + 32 words): the
+// exclusive prefix of the words' popcounts, so a member's rank is pre[w] +
+// popc(bits below it).  pre[w] = set bits of bits[0 .. w); returns the
+// total.  Each thread counts a run of consecutive words, a workgroup scan
+// places the runs.
 __device__ int bitmap_prefix(const unsigned* bits, int* pre, int words, int* wsum) {
   const int tid = threadIdx.x;
   const int per = (words + kBlock - 1) / kBlock;
@@ -2777,7 +3027,8 @@ constexpr int kHaloBlkMax = 4 * kHaloU * kHaloRows;  // the largest pack capacit
 // block component) per thread.
 __global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ rowptr, const int* __restrict__ colidx,
                                                       const float* __restrict__ val32,
-                                                      const float* __restrict__ invD32, int nb, int hbl,
+                                                      const float* __restrict__ invD32, int nb,
+                                                      const int* __restrict__ tstart, int hbl,
                                                       unsigned char* __restrict__ base, long long stride) {
   __shared__ unsigned s_b1[kHaloBitmapWords], s_b2[kHaloBitmapWords];
   __shared__ int s_p1[kHaloBitmapWords], s_p2[kHaloBitmapWords];
@@ -2786,7 +3037,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ ro
   __shared__ int s_red[3 * (kBlock / 64)], s_wsum[kBlock / 64];
   constexpr int R = kHaloTileRows;
   const int t = blockIdx.x, tid = threadIdx.x;
-  const int r0 = t * R, r1 = min(nb, r0 + R);
+  const int r0 = tstart ? tstart[t] : t * R, r1 = tstart ? tstart[t + 1] : min(nb, r0 + R);
   const int p0 = rowptr[r0], p1 = rowptr[r1];
   int* fail = reinterpret_cast<int*>(base);
   unsigned char* pk = base + kHaloBaseBytes + (size_t)t * stride;
@@ -2921,13 +3172,13 @@ __global__ __launch_bounds__(kBlock) void k_tile_halo(const int* __restrict__ ro
 }
 
 hipError_t launch_tile_pairs(const int* rowptr, const int* colidx, const float* val32, const float* invD32, int nb,
-                             int hbl, unsigned char* halo, hipStream_t st) {
-  const int rtl = (nb + kHaloTileRows - 1) / kHaloTileRows;
+                             const int* tstart, int ntiles, int hbl, unsigned char* halo, hipStream_t st) {
+  const int rtl = tstart ? ntiles : (nb + kHaloTileRows - 1) / kHaloTileRows;
   if (rtl <= 0 || rtl > kNbrMaxTiles || hbl < 0 || hbl > kHaloBlkMax) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned*>(halo),
                      (long long)(kHaloBaseBytes / 4));
-  hipLaunchKernelGGL(k_tile_halo, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, val32, invD32, nb, hbl, halo,
-                     halo_pack_bytes(hbl));
+  hipLaunchKernelGGL(k_tile_halo, dim3(rtl), dim3(kBlock), 0, st, rowptr, colidx, val32, invD32, nb, tstart, hbl,
+                     halo, halo_pack_bytes(hbl));
   return hipGetLastError();
 }
 

@@ -230,12 +230,18 @@ struct hsc_handle {
   bool pairs_used = false;      // the last PCG launch ran pass pairs (set up for them; after a solve, as run)
   int max_coupled128 = 0;  // most coupled nodes in one 128-node tile
   // row-tile neighbour lists for 128, 64 and 32 rows per tile (lanes per row 4, 8, 16)
-  Buf tnbr[3], tnbr_bits;
-  bool tnbr_ok[3] = {false, false, false};
+  Buf tnbr[4], tnbr_bits;  // [3]: 4 lanes per row over the body-aligned tiles
+  bool tnbr_ok[4] = {false, false, false, false};
+  // body-aligned row tiles of the latency schedule at 4 lanes per row
+  // (ntiles 0: none, the fixed 128-row tiles apply) and the most coupled
+  // nodes in one of them
+  Buf tstart;
+  int ntiles = 0, max_coupled_tiles = 0;
   // pass pairs (128-row tiles): halo packs, built for pack capacity
-  // pairs_hbl (-1: not built since the last setup)
+  // pairs_hbl (-1: not built since the last setup), over the tile table or not
   Buf halo;
   int pairs_hbl = -1;
+  bool pairs_tab = false;
   // chunk tables of the streaming operator kernels (strip_nch -1: not built
   // since the last setup)
   Buf hbits, hrow;
@@ -270,8 +276,9 @@ struct hsc_handle {
   // learned: a third stream runs the network on D beside the power iteration
   hipStream_t aux2 = nullptr;
   hipEvent_t d_ev = nullptr;
-  // the end of each D-side chain, waited for by the main stream
-  hipEvent_t aux_done = nullptr, aux2_done = nullptr;
+  // the end of each D-side chain, waited for by the main stream, and the
+  // end of D's column sort there
+  hipEvent_t aux_done = nullptr, aux2_done = nullptr, sort_ev = nullptr;
   HostBuf stage;
   ~hsc_handle() {
     if (aux) (void)hipStreamSynchronize(aux);
@@ -280,6 +287,7 @@ struct hsc_handle {
     if (d_ev) (void)hipEventDestroy(d_ev);
     if (aux_done) (void)hipEventDestroy(aux_done);
     if (aux2_done) (void)hipEventDestroy(aux2_done);
+    if (sort_ev) (void)hipEventDestroy(sort_ev);
     if (aux) (void)hipStreamDestroy(aux);
     if (aux2) (void)hipStreamDestroy(aux2);
   }
@@ -340,8 +348,8 @@ int validate_config(const hsc_config& c, std::string& why) {
     why = "operator_mapping must be 0, 1 or 2";
     return HSC_ERR_INVALID_ARGUMENT;
   }
-  if (c.schedule < 0 || c.schedule > 2) {
-    why = "schedule must be 0, 1 or 2";
+  if (c.schedule < 0 || c.schedule > 3) {
+    why = "schedule must be 0, 1, 2 or 3";
     return HSC_ERR_INVALID_ARGUMENT;
   }
   return HSC_OK;
@@ -478,6 +486,9 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   plan.res = lc.sched != 2;
   std::vector<hsc::SysDesc>& S = plan.S;
   S.resize(nsys);
+  // body-aligned row tiles: schedule 3 (the latency schedule) at 4 lanes per
+  // row (row tiles and 128-node tiles then coincide), standard loop
+  const bool tab_ok = lc.sched == 3 && G == 4 && !h0->cfg.pipelined;
   int rt = 0, et = 0, lt = 0, t_total = 0;
   double lr_bytes = 0, row_bytes = 0;
   const double eltM = mixed ? 4.0 : 8.0;
@@ -485,11 +496,12 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
     S[s] = make_desc(hs[s]);
     if (!S[s].x) throw HscError{HSC_ERR_INVALID_ARGUMENT, "solution vector not bound"};
     if (max_iter_override > 0) S[s].max_iter = max_iter_override;
+    S[s].tstart = tab_ok && hs[s]->ntiles > 0 ? hs[s]->tstart.as<int>() : nullptr;
     S[s].rt_begin = rt;
-    rt += (S[s].nb + R - 1) / R;
+    rt += S[s].tstart ? hs[s]->ntiles : (S[s].nb + R - 1) / R;
     S[s].rt_end = rt;
     S[s].et_begin = et;
-    et += (S[s].nb + node_tile - 1) / node_tile;
+    et += S[s].tstart ? hs[s]->ntiles : (S[s].nb + node_tile - 1) / node_tile;
     S[s].et_end = et;
     S[s].lt_begin = lt;
     if (S[s].k > 0) lt += (S[s].a + hsc::kLrTile - 1) / hsc::kLrTile;
@@ -509,7 +521,7 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   if (lc.sched != 2) {
     int kmax = 0;
     for (int s = 0; s < nsys; ++s) {
-      wc_rows = std::max(wc_rows, hs[s]->max_coupled128);
+      wc_rows = std::max(wc_rows, S[s].tstart ? hs[s]->max_coupled_tiles : hs[s]->max_coupled128);
       kmax = std::max(kmax, hs[s]->k);
     }
     wc_bytes = ((size_t)wc_rows * 3 * kmax * (mixed ? sizeof(float) : sizeof(double)) + 15) / 16 * 16;
@@ -598,16 +610,16 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   // point-to-point sync between preconditioner passes (aligned latency
   // split only: every row workgroup owns one resident row tile)
   bool psync = h0->cfg.pass_sync && aligned && !thr && h0->cfg.neumann_order > 2;
-  for (int s = 0; s < nsys && psync; ++s) psync = (hs[s]->nb + R - 1) / R <= hsc::kNbrMaxTiles;
+  for (int s = 0; s < nsys && psync; ++s) psync = S[s].rt_end - S[s].rt_begin <= hsc::kNbrMaxTiles;
   if (psync) {
-    const int li = G == 4 ? 0 : (G == 8 ? 1 : 2);
     for (int s = 0; s < nsys; ++s) {
       hsc_handle* h = hs[s];
+      const int li = S[s].tstart ? 3 : (G == 4 ? 0 : (G == 8 ? 1 : 2));
       if (!h->tnbr_ok[li]) {
-        const int rtl = (h->nb + R - 1) / R, words = (rtl + 31) / 32;
+        const int rtl = S[s].rt_end - S[s].rt_begin, words = (rtl + 31) / 32;
         HIPCHK(h->tnbr[li].ensure(sizeof(int) * (size_t)rtl * hsc::kNbrStride));
         HIPCHK(h->tnbr_bits.ensure(sizeof(unsigned) * (size_t)rtl * words));
-        HIPCHK(hsc::launch_tile_neighbors(h->rowptr.as<int>(), h->colidx.as<int>(), h->nb, R,
+        HIPCHK(hsc::launch_tile_neighbors(h->rowptr.as<int>(), h->colidx.as<int>(), h->nb, R, S[s].tstart, rtl,
                                           h->tnbr_bits.as<unsigned>(), h->tnbr[li].as<int>(), st));
         h->tnbr_ok[li] = true;
       }
@@ -649,12 +661,15 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   }
   if (pairs) {
     hsc_handle* h = h0;
-    if (h->pairs_hbl != hbl) {
-      const int rtl = (h->nb + R - 1) / R;
+    const bool tab = S[0].tstart != nullptr;
+    if (h->pairs_hbl != hbl || h->pairs_tab != tab) {
+      const int rtl = S[0].rt_end - S[0].rt_begin;
       HIPCHK(h->halo.ensure((size_t)hsc::kHaloBaseBytes + (size_t)rtl * (size_t)hsc::halo_pack_bytes(hbl)));
       HIPCHK(hsc::launch_tile_pairs(h->rowptr.as<int>(), h->colidx.as<int>(), h->val32.as<float>(),
-                                    h->invD32.as<float>(), h->nb, hbl, h->halo.as<unsigned char>(), st));
+                                    h->invD32.as<float>(), h->nb, S[0].tstart, rtl, hbl, h->halo.as<unsigned char>(),
+                                    st));
       h->pairs_hbl = hbl;
+      h->pairs_tab = tab;
     }
     // two buffers of tagged pass values, zeroed before each launch
     sc.ztag_bytes = sizeof(unsigned long long) * 2 * 3 * (size_t)h->nb;
@@ -935,8 +950,17 @@ LaunchCfg tune_now(hsc_handle* const* hs, int nsys, hipStream_t st) {
     if (c.lanes_per_row && Gs[gi] != c.lanes_per_row) continue;
     for (int w = 1; w <= 2; ++w) {
       if (c.wg_per_cu && w != c.wg_per_cu) continue;
-      for (int sc = 1; sc <= 2; ++sc) {
+      for (int sc = 1; sc <= 3; ++sc) {
         if (c.schedule && sc != c.schedule) continue;
+        // schedule 3 (the latency schedule on body-aligned row tiles) is a
+        // candidate of its own at 4 lanes per row when the tiles differ from
+        // the fixed ones
+        if (sc == 3 && !c.schedule) {
+          bool differs = Gs[gi] == 4;
+          for (int s = 0; s < nsys && differs; ++s)
+            differs = hs[s]->ntiles > 0 && hs[s]->ntiles != (hs[s]->nb + 127) / 128;
+          if (!differs) continue;
+        }
         LaunchCfg cand{Gs[gi], w, sc};
         try {
           float ms = 0.f;
@@ -1114,6 +1138,7 @@ struct SetupStage {
   int nnzb, nact;
   int ints2[2];
   int aca_err, max_coupled;
+  int tiles[2];  // body-aligned row tiles: count, most coupled nodes in one
   double dm[4];
   double lam2, fst;
   double pred[4];  // learned: the network's node-mean outputs
@@ -1137,6 +1162,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   for (bool& ok : h->tnbr_ok) ok = false;
   h->pairs_hbl = -1;
   h->strip_nch = -1;
+  h->ntiles = 0;
   if (!Av) throw HscError{HSC_ERR_INVALID_ARGUMENT, "null matrix view"};
   const long long N = 3 * Av->block_dim;
   // linear_hsc.cu:197-200
@@ -1206,6 +1232,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   const int nch = (nnzb + 63) / 64;
   HIPCHK(h->hbits.ensure(sizeof(unsigned long long) * (size_t)std::max(nch, 1)));
   HIPCHK(h->hrow.ensure(sizeof(int) * (size_t)(nch + 1)));
+  HIPCHK(h->tstart.ensure(sizeof(int) * (size_t)hsc::row_tiles_capacity()));
   HIPCHK(h->E.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
   HIPCHK(h->R0.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
   HIPCHK(h->R1.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
@@ -1236,6 +1263,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     HIPCHK(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&h->aux_ev, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&h->aux_done, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->sort_ev, hipEventDisableTiming));
   }
   const bool learned = h->cfg.preconditioner == HSC_PRECOND_LEARNED;
   if (learned && !h->aux2) {
@@ -1375,6 +1403,13 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(hsc::launch_chunk_heads(h->rowptr.as<int>(), nb, nnzb, h->hbits.as<unsigned long long>(),
                                  h->hrow.as<int>(), h->aux));
   h->strip_nch = nch;
+  // body-aligned row tiles from D's columns, on the main stream once the
+  // column sort is done (it has slack after the coupling core there)
+  HIPCHK(hipEventRecord(h->sort_ev, h->aux));
+  HIPCHK(hipStreamWaitEvent(st, h->sort_ev, 0));
+  HIPCHK(hsc::launch_row_tiles(h->rowptr.as<int>(), h->colidx.as<int>(), h->act_node.as<int>(), nb,
+                               h->tstart.as<int>(), err + 16, st));
+  HIPCHK(hipMemcpyAsync(ss->tiles, err + 16, sizeof(ss->tiles), hipMemcpyDeviceToHost, st));
   AuxJoin join2{nullptr};
   // the network needs D only: it runs on a third stream beside the power
   // iteration, enqueued after it (the power iteration's GPU chain is the
@@ -1401,6 +1436,8 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   const double* dm = ss->dm;
   const int aca_err = ss->aca_err;
   h->max_coupled128 = ss->max_coupled;
+  h->ntiles = ss->tiles[0];
+  h->max_coupled_tiles = ss->tiles[1];
   if (aca_err) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "ACA: bounded grid wait expired"};
   const int rank = limit > 0 ? ints2[0] : 0;
   if (ints2[1]) throw HscError{HSC_ERR_ABD_NOT_SPD, "HSC: non-SPD ABD block"};
@@ -2327,6 +2364,7 @@ int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* o) {
   o->pipelined = h->pipe_used ? 1 : 0;
   o->pass_pairs = h->pairs_used ? 1 : 0;
   o->streaming_operators = streaming_ops(h, 0) ? 1 : 0;
+  o->row_tiles = h->ntiles;
   o->lambda_max = h->lambda_max;
   o->neumann_doublings = h->doublings;
   o->abd_block_diagonal = h->blockdiag ? 1 : 0;

@@ -109,7 +109,9 @@ typedef struct hsc_config {
   int32_t check_acceptance;      /* 1 = fail on non-descent / full residual (reference); 0 = report only */
   int32_t schedule;              /* persistent-PCG work split: 0 = autotuned; 1 = latency (grid sized by the
                                     SpMV row tiles, 128-node elementwise tiles); 2 = throughput (grid covers
-                                    row and low-rank tiles, split by tile count, 512-node tiles) */
+                                    row and low-rank tiles, split by tile count, 512-node tiles); 3 = latency
+                                    on row tiles cut along D's connected components (at 4 lanes per row;
+                                    otherwise as 1), so that no tile straddles two bodies */
   double poly_interval_ratio;    /* HSC_PRECOND_CHEBYSHEV: b / a of the fitted interval (100) */
   int32_t pass_sync;             /* latency schedule, order > 2: 1 = each row tile waits only for its
                                     neighbour tiles between preconditioner passes; 0 = grid barriers;
@@ -272,7 +274,9 @@ typedef struct hsc_setup_info {
                                     and every row tile's halo fit: otherwise one wait per pass) */
   int32_t streaming_operators;   /* 1 when operator_mapping selects the streaming kernels for D x on this
                                     system */
-  int32_t reserved0;
+  int32_t row_tiles;             /* row tiles of schedule 3, cut along D's connected components (a component
+                                    of 128 rows or more starts a tile; tiles of at most 128 rows); 0 = none
+                                    (too many rows or components: schedule 3 runs as 1) */
 } hsc_setup_info;
 int32_t hsc_get_setup_info(const hsc_handle* h, hsc_setup_info* out);
 

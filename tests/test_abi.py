@@ -58,7 +58,7 @@ def test_invalid_config_rejected(lib):
     lib.hsc_config_default(C.byref(c))
     c.neumann_order = 0
     assert lib.hsc_create(C.byref(c), 0, C.byref(h)) == _abi.HSC_ERR_INVALID_ARGUMENT
-    for field, bad in (("schedule", 3), ("pass_sync", 3), ("pass_sync", -1), ("lanes_per_row", 6),
+    for field, bad in (("schedule", 4), ("pass_sync", 3), ("pass_sync", -1), ("lanes_per_row", 6),
                        ("operator_mapping", 3), ("operator_mapping", -1)):
         lib.hsc_config_default(C.byref(c))
         setattr(c, field, bad)

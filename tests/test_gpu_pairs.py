@@ -17,7 +17,14 @@ operator to the last bit:
   of the round that wrote it);
 * a system whose tile halo does not fit (a row of more than 20 blocks) falls
   back to one wait per pass on the device, bit for bit the per-pass result;
-* the 2,178-DOF system runs pairs too, and converges to the direct solution.
+* the 2,178-DOF system runs pairs too, and converges to the direct solution;
+* schedule 3's row tiles follow D's connected components (each torus of
+  the 78k-tet system starts a tile), as a NumPy restatement of the rule
+  gives them; the three pass modes stay bitwise equal on them, and the
+  solve agrees with the fixed tiles' to rounding;
+* with fixed tiles, a tile where two bodies meet has more than 128 halo
+  rows and holds 2 lanes per halo row, so its halo takes one round (the
+  bitwise tests above run on it).
 """
 import numpy as np
 import pytest
@@ -33,11 +40,12 @@ DEV = "cuda:0"
 LATENCY = {"lanes_per_row": 4, "wg_per_cu": 2, "schedule": 1}
 
 
-def _setup(sysm, order, ps, ratio=1000.0):
+def _setup(sysm, order, ps, ratio=1000.0, schedule=1):
     scene = {"linear_system/tol_rate": 1e-8, "linear_system/hsc/neumann_order": order,
              "linear_system/hsc/max_iter": 20000}
     cfg = plugin.make_config(scene, precision="mixed", residual_replace_every=25, preconditioner="chebyshev",
-                             poly_interval_ratio=ratio, abd_inverse="exact", pass_sync=ps, **LATENCY)
+                             poly_interval_ratio=ratio, abd_inverse="exact", pass_sync=ps,
+                             **dict(LATENCY, schedule=schedule))
     h = plugin.Handle(cfg)
     A = plugin.BCOO3.from_system(sysm, DEV)
     h.setup(A, sysm.qo, sysm.qm, sysm.fo, sysm.fn)
@@ -61,11 +69,11 @@ def c2():
     return systems.torus_system(seed=1)
 
 
-@pytest.mark.parametrize("order", [3, 5, 16])
-def test_pairs_bitwise_equal_to_per_pass_and_grid_barriers(c2, order):
+@pytest.mark.parametrize("order,schedule", [(3, 1), (5, 1), (16, 1), (16, 3)])
+def test_pairs_bitwise_equal_to_per_pass_and_grid_barriers(c2, order, schedule):
     res = {}
     for ps in (0, 1, 2):
-        h = _setup(c2, order, ps)
+        h = _setup(c2, order, ps, schedule=schedule)
         (x, st), = _solve(h, c2)
         info = h.setup_info()
         assert st["status"] == 0 and st["schur_relative_residual"] <= 1e-8, st
@@ -129,4 +137,50 @@ def test_pairs_on_the_small_system_reach_the_direct_solution():
     assert st["status"] == 0 and h.setup_info()["pass_pairs"] == 1, (st, h.setup_info())
     xd = osp.direct_solve(s)
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) <= 1e-7
+    h.close()
+
+
+def _body_tiles(sysm, rows=128):
+    """The row-tile starts the latency schedule uses at 4 lanes per row: D's
+    connected components by the running maximum of each row's largest
+    column; a component of `rows` rows or more starts a tile, and tiles are
+    cut every `rows` rows from the last such start."""
+    nb = sysm.fn // 3
+    r, c = sysm.rows.astype(np.int64), sysm.cols.astype(np.int64)
+    fem = (r < nb) & (c < nb)
+    reach = np.arange(nb)
+    np.maximum.at(reach, r[fem], c[fem])
+    np.maximum.at(reach, c[fem], r[fem])
+    ends = np.flatnonzero(np.maximum.accumulate(reach) == np.arange(nb))
+    starts = np.concatenate([[0], ends[:-1] + 1])
+    big = [int(a) for a, e in zip(starts, ends) if e - a + 1 >= rows]
+    segs = big if big and big[0] == 0 else [0] + big
+    tiles = []
+    for j, a in enumerate(segs):
+        tiles += list(range(a, segs[j + 1] if j + 1 < len(segs) else nb, rows))
+    return tiles
+
+
+def test_row_tiles_follow_the_bodies(c2):
+    """The 78k-tet system's 7 tori of 3,360 nodes start on tile boundaries:
+    27 tiles each (26 of 128 rows and one of 32), 189 in all where fixed
+    128-row tiles give 184; the cube (one component) keeps the fixed tiles'
+    count; both match the restated rule, and the solve on them converges."""
+    expect = _body_tiles(c2)
+    assert len(expect) == 189 and all(a % 3360 == 0 for a in expect[::27])
+    res = {}
+    for sched in (1, 3):
+        h = _setup(c2, 16, 2, schedule=sched)
+        info = h.setup_info()
+        assert info["row_tiles"] == len(expect) and info["schedule"] == sched
+        (x, st), = _solve(h, c2)
+        assert st["status"] == 0 and st["schur_relative_residual"] <= 1e-8 and h.setup_info()["pass_pairs"] == 1
+        res[sched] = (x, st["pcg_iterations"])
+        h.close()
+    # other tiles, other partial sums: the same solve to rounding
+    assert abs(res[1][1] - res[3][1]) <= 1
+    assert np.linalg.norm(res[1][0] - res[3][0]) <= 1e-6 * np.linalg.norm(res[1][0])
+    cube = systems.cube_system(seed=0)
+    h = _setup(cube, 16, 2, ratio=300.0)
+    assert h.setup_info()["row_tiles"] == len(_body_tiles(cube)) == (cube.fn // 3 + 127) // 128
     h.close()
