@@ -520,7 +520,7 @@ def main():
                          "in one captured persistent launch vs one at a time")
     ap.add_argument("--lanes-per-row", type=int, default=4)
     ap.add_argument("--wg-per-cu", type=int, default=1)
-    ap.add_argument("--schedule", type=int, default=1)
+    ap.add_argument("--schedule", type=int, default=3)
     ap.add_argument("--preconditioner", default="learned", choices=["neumann", "chebyshev", "learned"],
                     help="preconditioner of the --traffic-probe solve")
     ap.add_argument("--cpu-only", action="store_true",

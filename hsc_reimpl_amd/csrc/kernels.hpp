@@ -177,11 +177,11 @@ hipError_t launch_strip_op(int kernel, const SysDesc& S, const StripDesc& T, con
 hipError_t launch_op_precond64(const StripDesc* T, int G, const SysDesc& S, int order, const double* omegas,
                                const double* r, double* z, double* tmp, hipStream_t st);
 hipError_t launch_hash_fill(double* x, int n, unsigned seed, hipStream_t st);
-// body-aligned row tiles (one workgroup): tstart (row_tiles_capacity() ints)
-// gets the tile starts and nb, out[0] the tile count (0: none, the fixed
-// tiles apply), out[1] the most coupled nodes in one tile
-hipError_t launch_row_tiles(const int* rowptr, const int* colidx, const int* act_node, int nb, int* tstart, int* out,
-                            hipStream_t st);
+// body-aligned row tiles: tstart (row_tiles_capacity() ints) gets the tile
+// starts and nb, out[0] the tile count (0: none, the fixed tiles apply),
+// out[1] the most coupled nodes in one tile (reach: nb ints of scratch)
+hipError_t launch_row_tiles(const int* rowptr, const int* colidx, const int* act_node, int nb, int* reach,
+                            int* tstart, int* out, hipStream_t st);
 int row_tiles_capacity();
 // row-tile neighbour lists for R rows per tile, or for the ntiles tiles of a
 // tile table (bits: ceil(rtl/32)*rtl words, list: rtl*kNbrStride ints,

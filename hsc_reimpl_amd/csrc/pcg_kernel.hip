@@ -2766,8 +2766,8 @@ hipError_t launch_tile_neighbors(const int* rowptr, const int* colidx, int nb, i
 // meet set the pace of every pass pair.  Row r ends a component when no row
 // r' <= r references a column beyond r: the running maximum of each row's
 // largest column (rows keep their diagonal first, then columns ascending)
-// equals r.  One workgroup: rows are staged kBlock x 16 at a time into LDS
-// with coalesced loads, each thread scans its 16 consecutive rows, and
+// equals r.  Each row's largest column is read by one thread per row first;
+// then one workgroup: each thread takes 16 consecutive rows at a time, and
 // workgroup scans carry the running maximum and the last component end.
 constexpr int kTileRows = 128;
 constexpr int kTileStage = 16;                   // rows per thread per pass
@@ -2801,10 +2801,16 @@ __device__ int block_scan_max_excl(int v, int* wsum, int* total) {
   return max(off, ex);
 }
 
-__global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ rowptr, const int* __restrict__ colidx,
+// reach[r] = max(r, the largest column of row r), one thread per row
+__global__ __launch_bounds__(kBlock) void k_row_reach(const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                      int nb, int* __restrict__ reach) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < nb) reach[r] = max(r, colidx[rowptr[r + 1] - 1]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ reach_g,
                                                       const int* __restrict__ act_node, int nb,
                                                       int* __restrict__ tstart, int* __restrict__ out) {
-  __shared__ int s_reach[kBlock * kTileStage];
   __shared__ int s_big[kNbrMaxTiles + 1];
   __shared__ int s_tile[kNbrMaxTiles + 1];
   __shared__ int s_w[kBlock / 64];
@@ -2815,24 +2821,26 @@ __global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ ro
   }
   int carry_max = -1, carry_end = -1, nbig = 0;
   for (int base = 0; base < nb; base += kBlock * kTileStage) {
-#pragma unroll
-    for (int j = 0; j < kTileStage; ++j) {
-      const int r = base + j * kBlock + tid;
-      s_reach[j * kBlock + tid] = r < nb ? max(r, colidx[rowptr[r + 1] - 1]) : 0x7fffffff;
-    }
-    __syncthreads();
+    // this thread's kTileStage consecutive rows: their ends, then their last
+    // columns, all in flight together (indices clamped into the system;
+    // rows past it are masked)
     const int r0 = base + tid * kTileStage;
-    const int* rr = s_reach + tid * kTileStage;
+    int reach[kTileStage];
+#pragma unroll
+    for (int j = 0; j < kTileStage; ++j) reach[j] = reach_g[min(r0 + j, nb - 1)];
     int lm = -1;
 #pragma unroll
-    for (int j = 0; j < kTileStage; ++j) lm = r0 + j < nb ? max(lm, rr[j]) : lm;
+    for (int j = 0; j < kTileStage; ++j) {
+      reach[j] = r0 + j < nb ? reach[j] : 0x7fffffff;
+      lm = r0 + j < nb ? max(lm, reach[j]) : lm;
+    }
     int tmax = 0;
     const int before = max(carry_max, block_scan_max_excl(lm, s_w, &tmax));
     int run = before, last_end = -1;
 #pragma unroll
     for (int j = 0; j < kTileStage; ++j) {
-      run = max(run, rr[j]);
-      if (r0 + j < nb && run == r0 + j) last_end = r0 + j;
+      run = max(run, reach[j]);
+      last_end = run == r0 + j ? r0 + j : last_end;  // (rows past nb never match)
     }
     int tend = 0;
     const int prev_end = max(carry_end, block_scan_max_excl(last_end, s_w, &tend));
@@ -2842,11 +2850,10 @@ __global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ ro
     run = before;
 #pragma unroll
     for (int j = 0; j < kTileStage; ++j) {
-      run = max(run, rr[j]);
-      if (r0 + j < nb && run == r0 + j) {
-        cnt += (r0 + j - prev >= kTileRows) ? 1 : 0;
-        prev = r0 + j;
-      }
+      run = max(run, reach[j]);
+      const bool end = run == r0 + j;
+      cnt += end && r0 + j - prev >= kTileRows ? 1 : 0;
+      prev = end ? r0 + j : prev;
     }
     int total = 0;
     int pos = nbig + block_scan_excl(cnt, s_w, &total);
@@ -2854,8 +2861,8 @@ __global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ ro
     run = before;
 #pragma unroll
     for (int j = 0; j < kTileStage; ++j) {
-      run = max(run, rr[j]);
-      if (r0 + j < nb && run == r0 + j) {
+      run = max(run, reach[j]);
+      if (run == r0 + j) {
         if (r0 + j - prev >= kTileRows && pos < kNbrMaxTiles) s_big[pos++] = prev + 1;
         prev = r0 + j;
       }
@@ -2863,8 +2870,8 @@ __global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ ro
     nbig += total;
     carry_max = max(carry_max, tmax);
     carry_end = max(carry_end, tend);
-    __syncthreads();
   }
+  __syncthreads();
   if (nbig > kNbrMaxTiles) {
     if (tid == 0) out[0] = out[1] = 0;
     return;
@@ -2891,29 +2898,37 @@ __global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ ro
   __syncthreads();
   for (int t = tid; t <= ntiles; t += kBlock) tstart[t] = s_tile[t];
   // most coupled nodes in one tile (sizes the LDS cache of their W rows):
-  // coupled rows counted into their tile (binary search of the starts)
+  // each thread counts a run of consecutive rows into their tiles
   int* s_cnt = s_big;
   for (int t = tid; t < ntiles; t += kBlock) s_cnt[t] = 0;
   __syncthreads();
-  for (int base = 0; base < nb; base += kBlock * kTileStage) {
-    int an[kTileStage];
-#pragma unroll
-    for (int j = 0; j < kTileStage; ++j) {
-      const int r = base + j * kBlock + tid;
-      an[j] = r < nb ? act_node[r] : -1;
+  const int per2 = (nb + kBlock - 1) / kBlock;
+  const int a0 = min(nb, tid * per2), a1 = min(nb, a0 + per2);
+  if (a0 < a1) {
+    int lo = 0, hi = ntiles - 1;  // the tile of row a0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_tile[mid] <= a0) lo = mid;
+      else hi = mid - 1;
     }
+    int t = lo, tend2 = s_tile[lo + 1], c = 0;
+    for (int r = a0; r < a1; r += kTileStage) {
+      int an[kTileStage];
 #pragma unroll
-    for (int j = 0; j < kTileStage; ++j) {
-      if (an[j] < 0) continue;
-      const int r = base + j * kBlock + tid;
-      int lo = 0, hi = ntiles - 1;  // last tile starting at or before r
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_tile[mid] <= r) lo = mid;
-        else hi = mid - 1;
+      for (int j = 0; j < kTileStage; ++j) an[j] = act_node[min(r + j, a1 - 1)];
+#pragma unroll
+      for (int j = 0; j < kTileStage; ++j) {
+        if (r + j >= a1) break;
+        if (r + j >= tend2) {
+          if (c) atomicAdd(&s_cnt[t], c);
+          c = 0;
+          while (s_tile[t + 1] <= r + j) ++t;
+          tend2 = s_tile[t + 1];
+        }
+        c += an[j] >= 0 ? 1 : 0;
       }
-      atomicAdd(&s_cnt[lo], 1);
     }
+    if (c) atomicAdd(&s_cnt[t], c);
   }
   __syncthreads();
   int mc = 0;
@@ -2926,9 +2941,11 @@ __global__ __launch_bounds__(kBlock) void k_row_tiles(const int* __restrict__ ro
   }
 }
 
-hipError_t launch_row_tiles(const int* rowptr, const int* colidx, const int* act_node, int nb, int* tstart, int* out,
-                            hipStream_t st) {
-  hipLaunchKernelGGL(k_row_tiles, dim3(1), dim3(kBlock), 0, st, rowptr, colidx, act_node, nb, tstart, out);
+hipError_t launch_row_tiles(const int* rowptr, const int* colidx, const int* act_node, int nb, int* reach,
+                            int* tstart, int* out, hipStream_t st) {
+  if (nb > 0 && nb <= kTiledRowsMax)
+    hipLaunchKernelGGL(k_row_reach, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, st, rowptr, colidx, nb, reach);
+  hipLaunchKernelGGL(k_row_tiles, dim3(1), dim3(kBlock), 0, st, reach, act_node, nb, tstart, out);
   return hipGetLastError();
 }
 int row_tiles_capacity() { return kNbrMaxTiles + 1; }

@@ -235,7 +235,7 @@ struct hsc_handle {
   // body-aligned row tiles of the latency schedule at 4 lanes per row
   // (ntiles 0: none, the fixed 128-row tiles apply) and the most coupled
   // nodes in one of them
-  Buf tstart;
+  Buf tstart, reach;
   int ntiles = 0, max_coupled_tiles = 0;
   // pass pairs (128-row tiles): halo packs, built for pack capacity
   // pairs_hbl (-1: not built since the last setup), over the tile table or not
@@ -1233,6 +1233,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->hbits.ensure(sizeof(unsigned long long) * (size_t)std::max(nch, 1)));
   HIPCHK(h->hrow.ensure(sizeof(int) * (size_t)(nch + 1)));
   HIPCHK(h->tstart.ensure(sizeof(int) * (size_t)hsc::row_tiles_capacity()));
+  HIPCHK(h->reach.ensure(sizeof(int) * (size_t)nb));
   HIPCHK(h->E.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
   HIPCHK(h->R0.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
   HIPCHK(h->R1.ensure(sizeof(double) * (size_t)qm * std::max(a, 1)));
@@ -1403,13 +1404,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(hsc::launch_chunk_heads(h->rowptr.as<int>(), nb, nnzb, h->hbits.as<unsigned long long>(),
                                  h->hrow.as<int>(), h->aux));
   h->strip_nch = nch;
-  // body-aligned row tiles from D's columns, on the main stream once the
-  // column sort is done (it has slack after the coupling core there)
   HIPCHK(hipEventRecord(h->sort_ev, h->aux));
-  HIPCHK(hipStreamWaitEvent(st, h->sort_ev, 0));
-  HIPCHK(hsc::launch_row_tiles(h->rowptr.as<int>(), h->colidx.as<int>(), h->act_node.as<int>(), nb,
-                               h->tstart.as<int>(), err + 16, st));
-  HIPCHK(hipMemcpyAsync(ss->tiles, err + 16, sizeof(ss->tiles), hipMemcpyDeviceToHost, st));
   AuxJoin join2{nullptr};
   // the network needs D only: it runs on a third stream beside the power
   // iteration, enqueued after it (the power iteration's GPU chain is the
@@ -1422,6 +1417,13 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   ss->lam2 = 0.0;
   if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
   if (learned) enqueue_prediction(h, h->aux2, ss->pred);
+  // body-aligned row tiles from D's columns, on the main stream once the
+  // column sort is done (it has slack after the coupling core there),
+  // enqueued after the two longer D-side chains
+  HIPCHK(hipStreamWaitEvent(st, h->sort_ev, 0));
+  HIPCHK(hsc::launch_row_tiles(h->rowptr.as<int>(), h->colidx.as<int>(), h->act_node.as<int>(), nb,
+                               h->reach.as<int>(), h->tstart.as<int>(), err + 16, st));
+  HIPCHK(hipMemcpyAsync(ss->tiles, err + 16, sizeof(ss->tiles), hipMemcpyDeviceToHost, st));
   // the main stream waits for both D-side chains, so that one
   // synchronisation returns every result of the three streams
   HIPCHK(hipEventRecord(h->aux_done, h->aux));
