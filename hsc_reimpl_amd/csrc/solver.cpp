@@ -160,16 +160,16 @@ std::map<TuneKey, TuneVal> g_tune;
 // the file the table is loaded from and appended to ("" = memory only)
 std::string g_tune_path;
 bool g_tune_path_set = false, g_tune_loaded = false;
-constexpr const char* kTuneMagic = "hsc-tune-v3";
+constexpr const char* kTuneMagic = "hsc-tune-v4";
 
 std::string default_tune_path() {
   if (const char* e = std::getenv("HSC_TUNE_FILE")) return e;
-  if (const char* home = std::getenv("HOME")) return std::string(home) + "/.cache/hsc_mi355x/tune_v3.txt";
+  if (const char* home = std::getenv("HOME")) return std::string(home) + "/.cache/hsc_mi355x/tune_v4.txt";
   return "";
 }
 
 // Load the table file once (caller holds g_tune_mu).  Lines:
-// "hsc-tune-v3 cu nb_log2 deg kbucket mixed batch poly order psync G wgpcu sched";
+// "hsc-tune-v4 cu nb_log2 deg kbucket mixed batch poly order psync G wgpcu sched";
 // anything else is skipped.
 void load_tune_locked() {
   if (g_tune_loaded) return;

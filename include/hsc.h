@@ -219,7 +219,7 @@ int32_t hsc_append_jsonl(const hsc_stats* s, const char* path, int32_t frame, in
  * The tune table maps a size class (nodes, blocks per row, rank, precision,
  * batch, preconditioner, order, pass sync, device) to the fastest persistent
  * PCG launch shape.  It is loaded from and appended to the file named here
- * (default: $HSC_TUNE_FILE, else $HOME/.cache/hsc_mi355x/tune_v3.txt; NULL or
+ * (default: $HSC_TUNE_FILE, else $HOME/.cache/hsc_mi355x/tune_v4.txt; NULL or
  * "" = keep it in memory only).  hsc_tune runs the timing for h's current
  * system now (hsc_setup does it too when the class is new).  hsc_alloc_count
  * is the number of device and pinned-host allocations the library has made

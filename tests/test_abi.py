@@ -148,7 +148,7 @@ def test_alloc_count_and_tune_file_without_a_device(lib, tmp_path):
     # of another format are skipped) or an absent one is tolerated
     assert lib.hsc_alloc_count() >= 0
     f = tmp_path / "tune.txt"
-    f.write_text("garbage line\nhsc-tune-v3 256 14 11 3 1 0 1 16 1 4 2 1\n")
+    f.write_text("garbage line\nhsc-tune-v4 256 14 11 3 1 0 1 16 1 4 2 1\n")
     assert lib.hsc_tune_file(str(f).encode()) == 0
     assert lib.hsc_tune_file(str(tmp_path / "absent.txt").encode()) == 0
     assert lib.hsc_tune_file(None) == 0

@@ -229,7 +229,7 @@ def test_tune_table_persists_to_its_file(tmp_path):
         scene = {"linear_system/tol_rate": 1e-8, "linear_system/hsc/neumann_order": 3}
         h = _setup(s, scene)
         assert lib.hsc_tune(h.h, None) == 0
-        lines = [ln.split() for ln in f1.read_text().splitlines() if ln.startswith("hsc-tune-v3")]
+        lines = [ln.split() for ln in f1.read_text().splitlines() if ln.startswith("hsc-tune-v4")]
         assert lines, f1.read_text()
         entry = lines[-1]
         info = h.setup_info()
