@@ -166,6 +166,16 @@ struct ZeroList {
   int n;
 };
 hipError_t launch_zero_multi(const void* const* ptrs, const size_t* bytes, int n, hipStream_t st);
+// up to kGatherMax small copies device -> device by one launch (n words of
+// 4 bytes each; a null src writes zeros)
+constexpr int kGatherMax = 12;
+struct GatherList {
+  const unsigned* src[kGatherMax];
+  unsigned* dst[kGatherMax];
+  int n[kGatherMax];
+  int count;
+};
+hipError_t launch_gather_words(const GatherList& L, hipStream_t st);
 // streaming (strip) operator kernels: chunk tables, and one launch of
 // 0 FP64 D x, 1 FP64 pass, 2 FP32 pass (y = x + w Dinv (r - D x))
 hipError_t launch_chunk_heads(const int* rowptr, int nb, int nnzb, unsigned long long* hbits, int* hrow,

@@ -2531,6 +2531,25 @@ hipError_t launch_zero_multi(const void* const* ptrs, const size_t* bytes, int n
   return hipGetLastError();
 }
 
+// small device results gathered into one staging area by one launch, so
+// that a single copy brings them to the host: entry e copies n[e] 32-bit
+// words from src[e] (zeros when src[e] is null) to dst[e]
+__global__ __launch_bounds__(256) void k_gather_words(GatherList L) {
+  for (int e = 0; e < L.count; ++e) {
+    const unsigned* src = L.src[e];
+    unsigned* dst = L.dst[e];
+    for (int i = threadIdx.x; i < L.n[e]; i += blockDim.x) dst[i] = src ? src[i] : 0u;
+  }
+}
+
+hipError_t launch_gather_words(const GatherList& L, hipStream_t st) {
+  if (L.count < 0 || L.count > kGatherMax) return hipErrorInvalidValue;
+  for (int e = 0; e < L.count; ++e)
+    if (L.n[e] < 0 || (L.n[e] > 0 && !L.dst[e])) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gather_words, dim3(1), dim3(256), 0, st, L);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
   if (bytes % 4 != 0) return hipErrorInvalidValue;
   const long long n = (long long)(bytes / 4);

@@ -996,30 +996,40 @@ __global__ __launch_bounds__(64) void k_body_inverse(const double* A, int qm, do
 }
 
 // In-place Cholesky of an n x n row-major SPD matrix by one workgroup
-// (lower triangle = L).  Returns false (uniformly) when not SPD / not finite.
+// (lower triangle = L; blockDim a multiple of 32).  Returns false
+// (uniformly) when not SPD / not finite.  Right-looking, one barrier per
+// column: step j reads the pivot and column j as step j - 1 left them, every
+// thread forms l_ij = a_ij / d_j itself for the trailing update
+// a_ic -= l_ij l_cj (j < c <= i, threads on a 32-wide grid of (i, c)), and
+// column j is scaled in place during step j + 1, when nothing reads it.
+// The values and operations are those of scaling the column first, so L is
+// the same bit for bit.
 __device__ bool wg_cholesky(double* M, int n) {
-  __shared__ int s_ok;
   const int tid = threadIdx.x, nt = blockDim.x;
-  if (tid == 0) s_ok = 1;
-  __syncthreads();
+  const int tx = tid & 31, ty = tid >> 5, ny = nt >> 5;
+  double dprev = 0.0;
   for (int j = 0; j < n; ++j) {
-    if (tid == 0) {
-      const double d = M[(size_t)j * n + j];
-      if (!(d > 0.0) || !isfinite(d)) s_ok = 0;
-      else M[(size_t)j * n + j] = sqrt(d);
+    const double a = M[(size_t)j * n + j];  // every thread reads the same value
+    if (!(a > 0.0) || !isfinite(a)) return false;
+    const double d = sqrt(a);
+    if (j > 0) {  // column j - 1 is final
+      for (int i = j + tid; i < n; i += nt) M[(size_t)i * n + j - 1] /= dprev;
+      if (tid == 0) M[(size_t)(j - 1) * n + j - 1] = dprev;
     }
-    __syncthreads();
-    if (!s_ok) return false;
-    const double djj = M[(size_t)j * n + j];
-    for (int i = j + 1 + tid; i < n; i += nt) M[(size_t)i * n + j] /= djj;
-    __syncthreads();
     const int m = n - j - 1;
-    for (int e = tid; e < m * m; e += nt) {
-      const int i = j + 1 + e / m, c = j + 1 + e % m;
-      if (c <= i) M[(size_t)i * n + c] -= M[(size_t)i * n + j] * M[(size_t)c * n + j];
+    for (int r = ty; r < m; r += ny) {
+      const int i = j + 1 + r;
+      const double li = M[(size_t)i * n + j] / d;
+      for (int s = tx; s <= r; s += 32) {
+        const int c = j + 1 + s;
+        M[(size_t)i * n + c] -= li * (M[(size_t)c * n + j] / d);
+      }
     }
+    dprev = d;
     __syncthreads();
   }
+  if (n > 0 && tid == 0) M[(size_t)(n - 1) * n + n - 1] = dprev;
+  __syncthreads();
   return true;
 }
 
@@ -1374,10 +1384,11 @@ __global__ __launch_bounds__(1024) void k_coupling_core(FactorParams S) {
 
 // W = V L (a x k row-major, FP64 and FP32): V C V^T = W W^T.  Each entry
 // W[j][l] = sum over m = l .. k-1, in increasing order, of V[j][m] L[m][l].
-// k <= 64: a workgroup takes 64 rows of V (column-major, loaded coalesced)
-// and L into LDS and writes its 64 x k block of W row-major (coalesced);
+// k <= 64: a workgroup takes 16 rows of V (column-major, loaded coalesced)
+// and L into LDS and writes its 16 x k block of W row-major (coalesced), so
+// that a few thousand coupled DOFs spread over a few hundred workgroups;
 // larger ranks: one thread per entry.
-constexpr int kMakeWRows = 64, kMakeWMaxK = 64;
+constexpr int kMakeWRows = 16, kMakeWMaxK = 64;
 __global__ __launch_bounds__(256) void k_make_W(const double* V, const double* Lm, int a, const int* kdev, int kmax,
                                                 double* W64, float* W32) {
   __shared__ double s_V[kMakeWMaxK][kMakeWRows + 1];

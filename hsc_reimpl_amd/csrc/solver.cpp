@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -96,12 +97,19 @@ struct PcgScratch {
   int nflags = 0;  // row-tile pass flags zeroed before each launch (0: point-to-point pass sync off)
   size_t ztag_bytes = 0;  // pass pairs' tagged values zeroed before each launch (0: no pairs)
   HostBuf host;    // pinned copies of the outputs and the error flag
-  // last uploaded descriptor and chunk tables (an identical plan is not uploaded again)
-  std::vector<unsigned char> up_sys, up_ch;
+  // last uploaded descriptor and chunk tables, pinned: the uploads read
+  // them asynchronously, so they are rewritten only once up_ev has passed
+  // (an identical plan is not uploaded again)
+  HostBuf up_sys, up_ch;
+  size_t up_sys_n = 0, up_ch_n = 0;
   void* up_sys_ptr = nullptr;
   void* up_ch_ptr = nullptr;
+  hipEvent_t up_ev = nullptr;
+  bool up_pending = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~PcgScratch() {
+    if (up_pending) (void)hipEventSynchronize(up_ev);
+    if (up_ev) (void)hipEventDestroy(up_ev);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
   }
@@ -280,6 +288,7 @@ struct hsc_handle {
   // end of D's column sort there
   hipEvent_t aux_done = nullptr, aux2_done = nullptr, sort_ev = nullptr;
   HostBuf stage;
+  Buf stage_dev;  // the setup's small results gathered on the device, in the pinned copy's layout
   ~hsc_handle() {
     if (aux) (void)hipStreamSynchronize(aux);
     if (aux2) (void)hipStreamSynchronize(aux2);
@@ -699,15 +708,29 @@ PcgPlan plan_pcg(hsc_handle* const* hs, int nsys, LaunchCfg lc, int max_iter_ove
   std::copy(chL.begin(), chL.end(), ch.begin() + 2 * nwg);
   // upload the tables unless the same ones are already in place (a solve
   // of the same handles with the same launch shape reuses them)
-  auto upload = [&](void* dst, const void* src, size_t bytes, std::vector<unsigned char>& last, void*& last_ptr) {
-    const unsigned char* b = static_cast<const unsigned char*>(src);
-    if (last_ptr == dst && last.size() == bytes && std::memcmp(last.data(), b, bytes) == 0) return;
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
-    last.assign(b, b + bytes);
+  // from pinned staging copies, so the caller need not wait for them
+  bool uploaded = false;
+  auto upload = [&](void* dst, const void* src, size_t bytes, HostBuf& last, size_t& last_n, void*& last_ptr) {
+    if (last_ptr == dst && last_n == bytes && std::memcmp(last.p, src, bytes) == 0) return;
+    if (sc.up_pending) {  // an earlier upload may still read the staging copies
+      HIPCHK(hipEventSynchronize(sc.up_ev));
+      sc.up_pending = false;
+    }
+    last_ptr = nullptr;
+    HIPCHK(last.ensure(bytes));
+    std::memcpy(last.p, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, last.p, bytes, hipMemcpyHostToDevice, st));
+    last_n = bytes;
     last_ptr = dst;
+    uploaded = true;
   };
-  upload(sc.sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, sc.up_sys, sc.up_sys_ptr);
-  upload(sc.chunks.p, ch.data(), sizeof(int) * 3 * nwg, sc.up_ch, sc.up_ch_ptr);
+  upload(sc.sysdesc.p, S.data(), sizeof(hsc::SysDesc) * nsys, sc.up_sys, sc.up_sys_n, sc.up_sys_ptr);
+  upload(sc.chunks.p, ch.data(), sizeof(int) * 3 * nwg, sc.up_ch, sc.up_ch_n, sc.up_ch_ptr);
+  if (uploaded) {
+    if (!sc.up_ev) HIPCHK(hipEventCreateWithFlags(&sc.up_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(sc.up_ev, st));
+    sc.up_pending = true;
+  }
   hsc::PcgParams& P = plan.P;
   P.pass_sync = psync ? 1 : 0;
   P.pass_flags = psync ? sc.flags.as<unsigned>() : nullptr;
@@ -1012,7 +1035,6 @@ void prepare_launch(hsc_handle* h, hipStream_t st, bool retune = false) {
     ensure_host_scratch(h->pcg, 1);
     static const bool stage_timing = std::getenv("HSC_STAGE_TIMING") != nullptr;
     if (stage_timing) HIPCHK(h->profbuf.ensure(sizeof(long long) * 32 * plan.P.nwg));
-    HIPCHK(hipStreamSynchronize(st));  // the plan's table uploads read its host copies
   }
   h->xsol = nullptr;
   h->G = lc.G;
@@ -1045,7 +1067,7 @@ size_t power_buffer_doubles(int n, int nb) {
 // pseudo-random start, each normalising its input by the fixed-order norm of
 // the previous step's partials; *lam2_host (pinned) receives |Dinv D x_11|^2
 // with |x_11| = 1 once st has drained.
-void enqueue_power_iteration(hsc_handle* h, hipStream_t st, double* lam2_host) {
+const double* enqueue_power_iteration(hsc_handle* h, hipStream_t st) {
   const int n = h->fn, nb = h->nb;
   const size_t cap = (size_t)std::max(hsc::power_step_tiles(8, nb), 256);
   double* x = h->pwr.as<double>();
@@ -1066,11 +1088,11 @@ void enqueue_power_iteration(hsc_handle* h, hipStream_t st, double* lam2_host) {
     np_in = tiles;
   }
   HIPCHK(hsc::launch_sum_partials(parts[kPowerSteps & 1], tiles, lam2, st));
-  HIPCHK(hipMemcpyAsync(lam2_host, lam2, sizeof(double), hipMemcpyDeviceToHost, st));
+  return lam2;
 }
 
 int interval_from_prediction(hsc_handle* h, const double pred[3], double* ratio);
-void enqueue_prediction(hsc_handle* h, hipStream_t st, double* pred_host);
+const double* enqueue_prediction(hsc_handle* h, hipStream_t st);
 
 // Per-pass weights: omega for the Neumann series (and for the learned
 // preconditioner when its network's output is rejected); for the Chebyshev
@@ -1191,7 +1213,25 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(h->A.ensure(sizeof(double) * qm * qm));
   HIPCHK(h->ints.ensure(sizeof(int) * 32));
   HIPCHK(h->stage.ensure(sizeof(SetupStage)));
+  HIPCHK(h->stage_dev.ensure(sizeof(SetupStage)));
   SetupStage* ss = h->stage.as<SetupStage>();
+  SetupStage* sd = h->stage_dev.as<SetupStage>();
+  // gather device words into the device copy of the stage (then one copy
+  // of a field range brings them to the pinned one)
+  hsc::GatherList gl{};
+  auto gather = [&](void* dst_field, const void* src, size_t bytes) {
+    if (gl.count >= hsc::kGatherMax) throw HscError{HSC_ERR_HIP, "setup: gather list full"};
+    gl.dst[gl.count] = static_cast<unsigned*>(dst_field);
+    gl.src[gl.count] = static_cast<const unsigned*>(src);
+    gl.n[gl.count] = (int)(bytes / 4);
+    ++gl.count;
+  };
+  auto fetch = [&](const void* first_field, size_t bytes) {
+    const size_t off = (size_t)(static_cast<const char*>(first_field) - reinterpret_cast<const char*>(sd));
+    HIPCHK(hsc::launch_gather_words(gl, st));
+    HIPCHK(hipMemcpyAsync(reinterpret_cast<char*>(ss) + off, first_field, bytes, hipMemcpyDeviceToHost, st));
+    gl.count = 0;
+  };
   {
     const void* zp[] = {h->row_count.p, h->cursor.p, h->active_flag.p, h->invD64.p, h->invD32.p, h->A.p, h->ints.p};
     const size_t zb[] = {sizeof(int) * nb, sizeof(int) * nb, sizeof(int) * nb, sizeof(double) * 9 * nb,
@@ -1205,9 +1245,11 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(hsc::launch_split(sp, st));
   HIPCHK(hsc::launch_scan2(h->row_count.as<int>(), h->rowptr.as<int>(), h->active_flag.as<int>(),
                            h->act_prefix.as<int>(), nb, st));
-  HIPCHK(hipMemcpyAsync(ss->herr, err, sizeof(ss->herr), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&ss->nnzb, h->rowptr.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&ss->nact, h->act_prefix.as<int>() + nb, sizeof(int), hipMemcpyDeviceToHost, st));
+  gather(sd->herr, err, sizeof(sd->herr));
+  gather(&sd->nnzb, h->rowptr.as<int>() + nb, sizeof(int));
+  gather(&sd->nact, h->act_prefix.as<int>() + nb, sizeof(int));
+  static_assert(offsetof(SetupStage, nact) == offsetof(SetupStage, herr) + sizeof(int) * 9, "stage layout");
+  fetch(sd->herr, offsetof(SetupStage, nact) + sizeof(int) - offsetof(SetupStage, herr));
   HIPCHK(hipStreamSynchronize(st));
   const int* herr = ss->herr;
   const int nnzb = ss->nnzb, nact = ss->nact;
@@ -1376,7 +1418,6 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   // ---- C = U^T P U, its Cholesky C = L L^T and W = V L, for the rank ACA
   // found: the kernels read it on the device, so the chain follows ACA
   // without a round trip to the host (grids sized for the largest rank)
-  ss->fst = 0.0;
   if (limit > 0) {
     hsc::FactorParams fp{h->P.as<double>(), h->U.as<double>(), h->V.as<double>(), qm, a, limit, rank_dev,
                          h->W1.as<double>(), h->W1.as<double>() + (size_t)qm * limit, h->C.as<double>(),
@@ -1384,14 +1425,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     HIPCHK(hsc::launch_coupling_core(fp, st));
     HIPCHK(hsc::launch_make_W(h->V.as<double>(), h->Lm.as<double>(), a, rank_dev, limit, h->W64.as<double>(),
                               h->W32.as<float>(), st));
-    HIPCHK(hipMemcpyAsync(&ss->fst, dmisc + 4, sizeof(double), hipMemcpyDeviceToHost, st));
   }
-  ss->aca_err = 0;
-  HIPCHK(hipMemcpyAsync(ss->ints2, err + 8, sizeof(ss->ints2), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&ss->max_coupled, err + 12, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(ss->dm, dmisc, sizeof(ss->dm), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(ss->hist, h->hist.as<double>(), sizeof(double) * (lim1 + 1), hipMemcpyDeviceToHost, st));
-  if (limit > 0) HIPCHK(hipMemcpyAsync(&ss->aca_err, h->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
   // The D side (row sort and value gather, then the network and the power
   // iteration) depends on nothing after k_place.  It is submitted once the
   // main stream's short ACA / ABD-inverse chain and its read-backs are
@@ -1407,23 +1441,21 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   HIPCHK(hipEventRecord(h->sort_ev, h->aux));
   AuxJoin join2{nullptr};
   // the network needs D only: it runs on a third stream beside the power
-  // iteration, enqueued after it (the power iteration's GPU chain is the
-  // shorter one, so it starts first and the two end together)
+  // iteration, enqueued before it (its GPU chain is the longer one: started
+  // first, the two end together)
   if (learned) {
     HIPCHK(hipEventRecord(h->d_ev, h->aux));
     HIPCHK(hipStreamWaitEvent(h->aux2, h->d_ev, 0));
     join2.s = h->aux2;
   }
-  ss->lam2 = 0.0;
-  if (cheb) enqueue_power_iteration(h, h->aux, &ss->lam2);
-  if (learned) enqueue_prediction(h, h->aux2, ss->pred);
+  const double* pred_dev = learned ? enqueue_prediction(h, h->aux2) : nullptr;
+  const double* lam2_dev = cheb ? enqueue_power_iteration(h, h->aux) : nullptr;
   // body-aligned row tiles from D's columns, on the main stream once the
   // column sort is done (it has slack after the coupling core there),
   // enqueued after the two longer D-side chains
   HIPCHK(hipStreamWaitEvent(st, h->sort_ev, 0));
   HIPCHK(hsc::launch_row_tiles(h->rowptr.as<int>(), h->colidx.as<int>(), h->act_node.as<int>(), nb,
                                h->reach.as<int>(), h->tstart.as<int>(), err + 16, st));
-  HIPCHK(hipMemcpyAsync(ss->tiles, err + 16, sizeof(ss->tiles), hipMemcpyDeviceToHost, st));
   // the main stream waits for both D-side chains, so that one
   // synchronisation returns every result of the three streams
   HIPCHK(hipEventRecord(h->aux_done, h->aux));
@@ -1432,6 +1464,22 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
     HIPCHK(hipEventRecord(h->aux2_done, h->aux2));
     HIPCHK(hipStreamWaitEvent(st, h->aux2_done, 0));
   }
+  // every small result of the three streams by one gather and one copy
+  // (zeros where a stage did not run)
+  gather(sd->ints2, err + 8, sizeof(sd->ints2));
+  gather(&sd->aca_err, limit > 0 ? h->errflag.p : nullptr, sizeof(int));
+  gather(&sd->max_coupled, err + 12, sizeof(int));
+  gather(sd->tiles, err + 16, sizeof(sd->tiles));
+  gather(sd->dm, dmisc, sizeof(sd->dm));
+  gather(&sd->lam2, lam2_dev, sizeof(double));
+  gather(&sd->fst, limit > 0 ? dmisc + 4 : nullptr, sizeof(double));
+  gather(sd->pred, pred_dev, sizeof(double) * 3);
+  gather(sd->pred + 3, nullptr, sizeof(double));
+  gather(sd->hist, h->hist.as<double>(), sizeof(double) * (lim1 + 1));
+  static_assert(offsetof(SetupStage, hist) - offsetof(SetupStage, ints2) ==
+                    sizeof(int) * 6 + sizeof(double) * 10,
+                "stage layout: the gathered fields are contiguous");
+  fetch(sd->ints2, offsetof(SetupStage, hist) - offsetof(SetupStage, ints2) + sizeof(double) * (lim1 + 1));
   HIPCHK(hipStreamSynchronize(st));
   join.s = join2.s = nullptr;  // both chains have ended
   const int* ints2 = ss->ints2;
@@ -1518,13 +1566,13 @@ void enqueue_learned_features(hsc_handle* h, hipStream_t st) {
 // Enqueue the network on h's system on st (behind D's rows there):
 // features, forward, node mean of the 3 outputs, copied to pred_host
 // (pinned) once st drains.
-void enqueue_prediction(hsc_handle* h, hipStream_t st, double* pred_host) {
+const double* enqueue_prediction(hsc_handle* h, hipStream_t st) {
   ensure_mlp(h, st);
   enqueue_learned_features(h, st);
   HIPCHK(h->mlp->forward(h->xfeat.as<hsc::bf16_t>(), 1, h->omlp.as<float>(), st));
   double* lp = h->lpred.as<double>();
   HIPCHK(hsc::launch_mean_rows3(h->omlp.as<float>(), h->nb, lp + 4, lp, st));
-  HIPCHK(hipMemcpyAsync(pred_host, lp, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+  return lp;
 }
 
 // The interval ratio for h's degree from the network's 3 node means; returns
