@@ -1065,8 +1065,8 @@ size_t power_buffer_doubles(int n, int nb) {
 // Enqueue the power iteration for lambda_max(Dinv D) on st (D's rows and
 // inverse diagonal blocks ready there): 12 fused steps from a fixed
 // pseudo-random start, each normalising its input by the fixed-order norm of
-// the previous step's partials; *lam2_host (pinned) receives |Dinv D x_11|^2
-// with |x_11| = 1 once st has drained.
+// the previous step's partials; returns the device address that holds
+// |Dinv D x_11|^2 with |x_11| = 1 once st has drained.
 const double* enqueue_power_iteration(hsc_handle* h, hipStream_t st) {
   const int n = h->fn, nb = h->nb;
   const size_t cap = (size_t)std::max(hsc::power_step_tiles(8, nb), 256);
@@ -1564,8 +1564,8 @@ void enqueue_learned_features(hsc_handle* h, hipStream_t st) {
 }
 
 // Enqueue the network on h's system on st (behind D's rows there):
-// features, forward, node mean of the 3 outputs, copied to pred_host
-// (pinned) once st drains.
+// features, forward, node mean of the 3 outputs; returns the device address
+// of the 3 means (set once st drains).
 const double* enqueue_prediction(hsc_handle* h, hipStream_t st) {
   ensure_mlp(h, st);
   enqueue_learned_features(h, st);
