@@ -137,8 +137,22 @@ __global__ void k_mlp_aggregate(const bf16_t* H, const int* ptr, const int* nbr,
   if (row >= M) return;
   const int b = row / nb, i = row - b * nb;
   const int p0 = ptr[i], p1 = ptr[i + 1];
+  const bf16_t* Hb = H + (size_t)b * nb * 64 + c;
   float s = 0.f;
-  for (int p = p0; p < p1; ++p) s += bf2f(H[((size_t)b * nb + nbr[p]) * 64 + c]);
+  int p = p0;
+  // 8 neighbours at a time: their indices, then their features, are loaded
+  // together (the sum keeps the neighbour order)
+  for (; p + 8 <= p1; p += 8) {
+    int q[8];
+    float h[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q[u] = nbr[p + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) h[u] = bf2f(Hb[(size_t)q[u] * 64]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += h[u];
+  }
+  for (; p < p1; ++p) s += bf2f(Hb[(size_t)nbr[p] * 64]);
   Y[(size_t)row * 64 + c] = f2bf(p1 > p0 ? s / (float)(p1 - p0) : 0.f);
 }
 
