@@ -270,6 +270,7 @@ struct hsc_handle {
   double lambda_max = 0.0;  // power-iteration estimate of lambda_max(Dinv D) (Chebyshev weights)
   double omegas[hsc::kMaxPolySteps] = {};  // per-pass preconditioner weights
   double* xsol = nullptr;  // FEM part of the caller's x during a solve
+  double* last_xsol = nullptr;  // that of the last solve (only its address is kept)
   // buffers
   Buf row_count, rowptr, cursor, active_flag, act_prefix, act_node, act_dof;
   Buf key_col, key_src, colidx, val64, val32, invD64, invD32;
@@ -1043,6 +1044,10 @@ void prepare_launch(hsc_handle* h, hipStream_t st, bool retune = false) {
     HIPCHK(hipMemcpyAsync(h->r.p, h->rhs.p, sizeof(double) * h->fn, hipMemcpyDeviceToDevice, st));
     lc = tune_now(hs, 1, st);
   }
+  // plan with the last solve's x: a solve into the same buffer then finds
+  // its descriptor already uploaded (nothing runs with it before a solve,
+  // which plans with its own x)
+  if (h->last_xsol) h->xsol = h->last_xsol;
   {
     PcgPlan plan = plan_pcg(hs, 1, lc, 0, st, h->pcg);
     ensure_host_scratch(h->pcg, 1);
@@ -1727,6 +1732,7 @@ int solve_impl(hsc_handle* h, const double* b, double* x, hsc_stats* s, hipStrea
   // a reconfiguration after setup that changed the launch re-plans it here
   if (!h->launch_ready) prepare_launch(h, st);
   h->xsol = x + h->fo;
+  h->last_xsol = h->xsol;
   const LaunchCfg lc{h->G, h->wgpcu, h->sched};
   enqueue_rhs(h, b, x, st);
   float pms = 0.f;
