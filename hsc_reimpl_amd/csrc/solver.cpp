@@ -45,19 +45,6 @@ double ms_since(Clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
 
-// Wait for st by polling it: the host sees the end of the stream's work
-// within a few microseconds, where a blocking wait sleeps on an interrupt
-// and wakes ~20 us later.  After kPollMs of polling it blocks instead.
-constexpr double kPollMs = 20.0;
-hipError_t wait_stream(hipStream_t st) {
-  const auto t0 = Clock::now();
-  for (;;) {
-    const hipError_t e = hipStreamQuery(st);
-    if (e != hipErrorNotReady) return e;
-    if (ms_since(t0) > kPollMs) return hipStreamSynchronize(st);
-  }
-}
-
 struct Buf {
   void* p = nullptr;
   size_t cap = 0;
@@ -849,7 +836,7 @@ std::vector<hsc::SysOut> collect_pcg(PcgScratch& sc, int nsys, hipStream_t st, h
     for (int i = 0; i < nsys && ce == hipSuccess; ++i)
       ce = hipMemcpyAsync(hfin + 3 * i, fin_hs[i]->misc.as<double>() + 20, 3 * sizeof(double),
                           hipMemcpyDeviceToHost, st);
-  if (ce == hipSuccess) ce = wait_stream(st);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(st);
   if (ce != hipSuccess) throw HscError{HSC_ERR_HIP, std::string("PCG: ") + hipGetErrorString(ce)};
   if (*derr) throw HscError{HSC_ERR_DEVICE_TIMEOUT, "persistent PCG: bounded grid wait expired"};
   std::vector<hsc::SysOut> out(nsys);
@@ -1268,7 +1255,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
   gather(&sd->nact, h->act_prefix.as<int>() + nb, sizeof(int));
   static_assert(offsetof(SetupStage, nact) == offsetof(SetupStage, herr) + sizeof(int) * 9, "stage layout");
   fetch(sd->herr, offsetof(SetupStage, nact) + sizeof(int) - offsetof(SetupStage, herr));
-  HIPCHK(wait_stream(st));
+  HIPCHK(hipStreamSynchronize(st));
   const int* herr = ss->herr;
   const int nnzb = ss->nnzb, nact = ss->nact;
   if (herr[1]) throw HscError{HSC_ERR_INVALID_ARGUMENT, "HSC requires only ABD (12/body) and FEM (3/node) DOFs"};
@@ -1498,7 +1485,7 @@ void do_setup(hsc_handle* h, const hsc_bcoo3_view* Av, int qo, int qm, int fo, i
                     sizeof(int) * 6 + sizeof(double) * 10,
                 "stage layout: the gathered fields are contiguous");
   fetch(sd->ints2, offsetof(SetupStage, hist) - offsetof(SetupStage, ints2) + sizeof(double) * (lim1 + 1));
-  HIPCHK(wait_stream(st));
+  HIPCHK(hipStreamSynchronize(st));
   join.s = join2.s = nullptr;  // both chains have ended
   const int* ints2 = ss->ints2;
   const double* dm = ss->dm;

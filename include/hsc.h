@@ -15,9 +15,7 @@
  * only when a bigger system arrives), picks the persistent-PCG launch shape
  * (timed once per size class and kept in the tune table, see hsc_tune_file)
  * and sizes that launch's scratch; hsc_solve then allocates nothing and runs
- * no tuning (hsc_alloc_count lets a caller check this).  Both return once
- * their results are on the host; the calling thread polls the stream for
- * that (up to 20 ms, then a blocking wait).
+ * no tuning (hsc_alloc_count lets a caller check this).
  */
 #ifndef HSC_MI355X_H
 #define HSC_MI355X_H
